@@ -1,0 +1,122 @@
+/* pmx: MI355X-native particle-mesh engine behind a plain C ABI.
+ *
+ * Every entry point takes raw device pointers and a hipStream_t (passed as void*), is asynchronous
+ * on that stream unless stated otherwise, returns 0 on success or a nonzero code (1 invalid
+ * argument, 2 unsupported size or mode, 3 HIP runtime error, 4 out of device memory), and leaves
+ * a message for pmx_last_error() on failure.
+ *
+ * Layouts follow the reference's arrays, row-major float32:
+ *   particle field  (npx, npy, npz, 3)  displacements from the lattice (mode 0, GRID_RELATIVE) or
+ *                                       absolute grid coordinates (mode 1, GRID_ABSOLUTE)
+ *   density mesh    (nx, ny, nz)
+ *   force mesh      (nx, ny, nz, 3)
+ *   HEALPix map     (12 nside^2,)  RING ordering
+ * In mode 0 the particle grid must divide the mesh (mesh/np integer per axis).
+ * Supported mesh sizes per axis: 8, 16, 32, 64, 96, 128, 192, 256, 512, 1024.
+ * A plan owns its work buffers: calls sharing one plan must be issued on one stream.
+ */
+#ifndef PMX_H
+#define PMX_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pmx_plan_t* pmx_plan;
+
+#define PMX_DECONV 1u      /* divide the force spectrum by the CIC window of paint and readout */
+#define PMX_DRIFT_PRE 2u   /* pmx_step: x += v * c_drift_pre before painting */
+#define PMX_DRIFT_POST 4u  /* pmx_step: x += v * c_drift_post after the kick */
+
+int pmx_abi_version(void);
+const char* pmx_last_error(void);
+int pmx_fft_supported(int n);
+
+/* Plan for one mesh shape; allocates its workspace (synchronous). */
+int pmx_plan_create(pmx_plan* plan, int nx, int ny, int nz, unsigned flags);
+int pmx_plan_destroy(pmx_plan plan);
+size_t pmx_plan_workspace_bytes(pmx_plan plan);
+/* Physical box (Mpc/h) for the k tables of pmx_power and pmx_color_field (synchronous upload). */
+int pmx_plan_set_box(pmx_plan plan, const double box[3]);
+
+/* CIC paint. Replaces cic_paint_dx / cic_paint as called by _single_paint
+ * (src/jax_fli/_src/fields/_painting.py:66-82; ParticleField.paint, fields/particles.py:120-178).
+ * weights: per-particle device array or NULL for the scalar `weight`. accumulate=0 zeroes mesh. */
+int pmx_paint_cic(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, const float* weights,
+                  float weight, float* mesh, int accumulate, void* stream);
+
+/* CIC readout of an ncomp (1 or 3) component mesh (ncomp interleaved last). Replaces
+ * cic_read_dx / cic_read in ParticleField.read_out (fields/particles.py:183-229). out: (np, ncomp). */
+int pmx_readout_cic(pmx_plan plan, const float* mesh, int ncomp, const float* pos, int npx, int npy, int npz,
+                    int mode, float* out, void* stream);
+
+/* Unnormalised 3D real FFT pair (fft3d / ifft3d of the reference, initial.py:175,182 and inside
+ * pm_forces). Spectrum layout: complex64 [nz/2+1][ny][nx] (kz slowest, kx fastest). */
+int pmx_fft_r2c(pmx_plan plan, const float* in, void* spec, void* stream);
+int pmx_fft_c2r(pmx_plan plan, const void* spec, float* out, void* stream);
+
+/* Force mesh (nx,ny,nz,3) of a density mesh: FFT, Green's function -1/k^2, FastPM 4-point gradient,
+ * optional CIC compensation (PMX_DECONV), inverse FFT; times `scale`. The k-space part of
+ * jaxpm.pm.pm_forces as called by _forces (src/jax_fli/pm/solvers.py:19-32). fmesh may be NULL
+ * (result kept in the plan). */
+int pmx_force_mesh(pmx_plan plan, const float* density, float scale, unsigned flags, float* fmesh, void* stream);
+
+/* Particle forces pm_forces(pos) * scale: paint, force mesh, readout. Replaces _forces
+ * (src/jax_fli/pm/solvers.py:19-32) with scale = 1.5 * Omega_m. forces: (npx,npy,npz,3). */
+int pmx_forces(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, float scale, unsigned flags,
+               float* forces, void* stream);
+
+/* Fused PM step on (x, v) in place:
+ *   [PMX_DRIFT_PRE]  x += v * c_drift_pre
+ *   v += (pm_forces(x) * force_scale) * c_kick
+ *   [PMX_DRIFT_POST] x += v * c_drift_post
+ * The drift/kick/drift bodies of EfficientDriftDoubleKick.step (solvers.py:151-202, with the
+ * drift of the next step fused after the kick), ReversibleDoubleKickDrift.step (solvers.py:258-325)
+ * and its exact inverse (in place of ReversibleDoubleKickDrift.reverse, solvers.py:339-411).
+ * Coefficients are the host-computed scalar factors of those methods. */
+int pmx_step(pmx_plan plan, float* x, float* v, int npx, int npy, int npz, int mode, float force_scale,
+             float c_drift_pre, float c_kick, float c_drift_post, unsigned flags, void* stream);
+
+/* Binned power spectrum sums of a real mesh (the estimator of _power,
+ * src/jax_fli/_src/power/_compute.py:68-119): for bin b = #(kedges <= |k|), b in [0, nedges],
+ * pk_sum[b] = sum |FFT(mesh)|^2, k_sum[b] = sum |k|, k_cnt[b] = mode count, over the full c2c grid.
+ * Outputs are float64 device arrays of nedges+1 entries (overwritten), 1 <= nedges <= 512.
+ * Needs pmx_plan_set_box. */
+int pmx_power(pmx_plan plan, const float* mesh, const double* kedges, int nedges, double* pk_sum, double* k_sum,
+              double* k_cnt, void* stream);
+
+/* Counter-based white noise N(0,1) (Philox4x32-10 + Box-Muller) filling (nx,ny,nz) -- stands in for
+ * normal_field(key) (initial.py:93). pmx_philox4x32 writes raw 4x32 blocks for known-answer tests. */
+int pmx_white_noise(pmx_plan plan, uint64_t seed, float* out, void* stream);
+int pmx_philox4x32(uint64_t seed, size_t nblocks, uint32_t* out, void* stream);
+
+/* Colour a real field by sqrt(P(|k|) * ncell / volume): interpolate_initial_conditions
+ * (initial.py:124-199). ktab/ptab: device float64 tables (linear interpolation, linear
+ * extrapolation). Needs pmx_plan_set_box. */
+int pmx_color_field(pmx_plan plan, const float* field, const double* ktab, const double* ptab, int ntab, float* out,
+                    void* stream);
+
+/* First-order LPT at the lattice: F = pm_forces(lattice, delta=ic) (unit scale),
+ * dx = growth * F, p = vel_factor * dx. jaxpm.pm.lpt(order=1) as called by lpt (pm/lpt.py:119-127). */
+int pmx_lpt1(pmx_plan plan, const float* ic, float growth, float vel_factor, float* dx, float* p, void* stream);
+
+/* HEALPix shell paint: particles with rmin < |x - observer| <= rmax (Mpc/h, periodic box) painted
+ * to RING pixels, scheme 0 = ngp, 1 = bilinear (HEALPix ring interpolation weights), map =
+ * weighted count / (pixel solid angle * (rmax^3 - rmin^3) / 3). paint_particles_spherical as called
+ * by _single_paint_spherical (src/jax_fli/_src/fields/_painting.py:249-348). */
+int pmx_paint_healpix(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, const float* weights,
+                      float weight, const double box[3], const double observer[3], double rmin, double rmax,
+                      int nside, int scheme, float* map, int accumulate, void* stream);
+
+/* Flat-sky slab paint: absolute z in (zlo, zhi] (grid units), (x, y) rescaled to (fnx, fny) and
+ * wrapped, 2D CIC, times `scale` (1 / pixel volume). _single_paint_2d (_painting.py:117-202). */
+int pmx_paint_flat(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, const float* weights,
+                   float weight, int fnx, int fny, float zlo, float zhi, double scale, float* map, int accumulate,
+                   void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,159 @@
+"""ctypes binding of libpmx.so (the C ABI declared in include/pmx.h).
+
+The library is loaded from this package directory. There is no fallback: a missing or
+unloadable library raises immediately, and every failing call raises ``PmxError`` with the
+library's message.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpmx.so")
+
+# every symbol include/pmx.h declares, with its argument types
+_c = ctypes
+_vp = _c.c_void_p
+_i = _c.c_int
+_f = _c.c_float
+_d = _c.c_double
+_u = _c.c_uint
+_dp = _c.POINTER(_c.c_double)
+SIGNATURES = {
+    "pmx_abi_version": (_i, []),
+    "pmx_last_error": (_c.c_char_p, []),
+    "pmx_fft_supported": (_i, [_i]),
+    "pmx_plan_create": (_i, [_c.POINTER(_vp), _i, _i, _i, _u]),
+    "pmx_plan_destroy": (_i, [_vp]),
+    "pmx_plan_workspace_bytes": (_c.c_size_t, [_vp]),
+    "pmx_plan_set_box": (_i, [_vp, _dp]),
+    "pmx_paint_cic": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _vp, _i, _vp]),
+    "pmx_readout_cic": (_i, [_vp, _vp, _i, _vp, _i, _i, _i, _i, _vp, _vp]),
+    "pmx_fft_r2c": (_i, [_vp, _vp, _vp, _vp]),
+    "pmx_fft_c2r": (_i, [_vp, _vp, _vp, _vp]),
+    "pmx_force_mesh": (_i, [_vp, _vp, _f, _u, _vp, _vp]),
+    "pmx_forces": (_i, [_vp, _vp, _i, _i, _i, _i, _f, _u, _vp, _vp]),
+    "pmx_step": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _f, _u, _vp]),
+    "pmx_power": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "pmx_white_noise": (_i, [_vp, _c.c_uint64, _vp, _vp]),
+    "pmx_philox4x32": (_i, [_c.c_uint64, _c.c_size_t, _vp, _vp]),
+    "pmx_color_field": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp]),
+    "pmx_lpt1": (_i, [_vp, _vp, _f, _f, _vp, _vp, _vp]),
+    "pmx_paint_healpix": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _dp, _dp, _d, _d, _i, _i, _vp, _i, _vp]),
+    "pmx_paint_flat": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _i, _i, _f, _f, _d, _vp, _i, _vp]),
+}
+
+DECONV = 1
+DRIFT_PRE = 2
+DRIFT_POST = 4
+
+
+class PmxError(RuntimeError):
+    """Error reported by libpmx."""
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load() -> ctypes.CDLL:
+    """Load libpmx.so (once). Raises if it is missing: there is no non-native path."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise PmxError(f"{LIB_PATH} is missing; build it with __graft_entry__.build() (make -C csrc)")
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().pmx_last_error().decode(errors="replace")
+        raise PmxError(f"{what} failed (code {rc}): {msg}")
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise PmxError("pmx kernels need device tensors")
+    if not t.is_contiguous():
+        raise PmxError("pmx kernels need contiguous tensors")
+    return t.data_ptr()
+
+
+def stream_ptr(stream: torch.cuda.Stream | None = None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def dbl3(v) -> ctypes.Array:
+    return (ctypes.c_double * 3)(*[float(x) for x in v])
+
+
+class Plan:
+    """Owns one pmx plan (FFT tables and work buffers) for a mesh shape on one device."""
+
+    def __init__(self, mesh_shape, device: torch.device | None = None):
+        lib = load()
+        self.mesh_shape = tuple(int(n) for n in mesh_shape)
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(lib.pmx_plan_create(ctypes.byref(handle), *self.mesh_shape, 0), "pmx_plan_create")
+        self.handle = handle
+        self._box = None
+
+    @property
+    def workspace_bytes(self) -> int:
+        return int(load().pmx_plan_workspace_bytes(self.handle))
+
+    def set_box(self, box) -> None:
+        box = tuple(float(b) for b in box)
+        if box != self._box:
+            with torch.cuda.device(self.device):
+                check(load().pmx_plan_set_box(self.handle, dbl3(box)), "pmx_plan_set_box")
+            self._box = box
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib is not None:
+            try:
+                with torch.cuda.device(self.device):
+                    torch.cuda.synchronize(self.device)
+                    _lib.pmx_plan_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+
+_plans: dict = {}
+
+
+def get_plan(mesh_shape, device: torch.device | None = None) -> Plan:
+    """Cached plan per (device, mesh shape)."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    if dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    key = (dev.index, tuple(int(n) for n in mesh_shape))
+    p = _plans.get(key)
+    if p is None:
+        p = Plan(mesh_shape, dev)
+        _plans[key] = p
+    return p
+
+
+def clear_plans() -> None:
+    torch.cuda.synchronize()
+    _plans.clear()

@@ -1,0 +1,281 @@
+"""Background cosmology on the host (float64): expansion rate, linear growth, FastPM growth
+functions, comoving distances and the Eisenstein & Hu (1998) linear power spectrum.
+
+These replace the scalar functions the reference takes from ``jaxpm.growth`` (``E``, ``Gf``, ``gp``,
+``dGfa``, ``growth_factor``; imported at src/jax_fli/pm/solvers.py:8-9) and ``jax_cosmo``
+(``background.radial_comoving_distance``, ``a_of_chi``, ``power.linear_matter_power``; called at
+src/jax_fli/pm/_resolve_geometry.py:78-110, src/jax_fli/initial.py:171-173). They feed the
+per-step kick/drift coefficients and the initial-condition power table; none of this runs on the
+device. The growth identities follow the reference's own test mock (tests/conftest.py:64-83):
+Gf = f D a^2 E, gp = f D / a, and dGfa = dGf/da, which the growth equation turns into
+1.5 Omega_m D / (a^2 E).
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import functools
+import math
+
+import numpy as np
+
+C_OVER_H0 = 2997.92458  # Mpc/h
+T_CMB = 2.726
+
+
+@dataclasses.dataclass(frozen=True)
+class Cosmology:
+    """Flat or curved w0waCDM background (no radiation), mirroring jax_cosmo's parameters."""
+
+    Omega_c: float = 0.2607
+    Omega_b: float = 0.0490
+    h: float = 0.6766
+    n_s: float = 0.9665
+    sigma8: float = 0.8102
+    Omega_k: float = 0.0
+    w0: float = -1.0
+    wa: float = 0.0
+
+    @property
+    def Omega_m(self) -> float:
+        return self.Omega_c + self.Omega_b
+
+    @property
+    def Omega_de(self) -> float:
+        return 1.0 - self.Omega_k - self.Omega_m
+
+    def replace(self, **kw) -> "Cosmology":
+        return dataclasses.replace(self, **kw)
+
+
+def Planck18(**kw) -> Cosmology:
+    return Cosmology(**kw)
+
+
+# ---------------------------------------------------------------------- expansion
+def Esqr(cosmo: Cosmology, a):
+    a = np.asarray(a, dtype=np.float64)
+    fde = a ** (-3.0 * (1.0 + cosmo.w0 + cosmo.wa)) * np.exp(-3.0 * cosmo.wa * (1.0 - a))
+    return cosmo.Omega_m * a**-3 + cosmo.Omega_k * a**-2 + cosmo.Omega_de * fde
+
+
+def E(cosmo: Cosmology, a):
+    return np.sqrt(Esqr(cosmo, a))
+
+
+def _dlnE_dlna(cosmo: Cosmology, a):
+    a = np.asarray(a, dtype=np.float64)
+    fde = a ** (-3.0 * (1.0 + cosmo.w0 + cosmo.wa)) * np.exp(-3.0 * cosmo.wa * (1.0 - a))
+    dfde = fde * (-3.0 * (1.0 + cosmo.w0 + cosmo.wa) + 3.0 * cosmo.wa * a)
+    num = -3.0 * cosmo.Omega_m * a**-3 - 2.0 * cosmo.Omega_k * a**-2 + cosmo.Omega_de * dfde
+    return 0.5 * num / Esqr(cosmo, a)
+
+
+# ---------------------------------------------------------------------- linear growth
+_A_INIT = 1e-5
+_NSTEP = 2048
+
+
+def _growth_rhs(cosmo: Cosmology, x, D, G):
+    # y = (D, G = dD/dln a);  dG/dln a = -(2 + dlnE/dlna) G + 1.5 Om D / (a^3 E^2)
+    a = math.exp(x)
+    p = -3.0 * (1.0 + cosmo.w0 + cosmo.wa)
+    fde = a**p * math.exp(-3.0 * cosmo.wa * (1.0 - a))
+    om = cosmo.Omega_m * a**-3
+    ok = cosmo.Omega_k * a**-2
+    e2 = om + ok + cosmo.Omega_de * fde
+    dlne = 0.5 * (-3.0 * om - 2.0 * ok + cosmo.Omega_de * fde * (p + 3.0 * cosmo.wa * a)) / e2
+    return G, -(2.0 + dlne) * G + 1.5 * om * D / e2
+
+
+@functools.lru_cache(maxsize=1024)
+def _growth_table(cosmo: Cosmology):
+    """RK4 in ln a from the matter-dominated growing mode D = a, normalised to D(1) = 1."""
+    x0, x1 = math.log(_A_INIT), 0.0
+    h = (x1 - x0) / _NSTEP
+    xs = x0 + h * np.arange(_NSTEP + 1)
+    Ds = [0.0] * (_NSTEP + 1)
+    Gs = [0.0] * (_NSTEP + 1)
+    D = G = _A_INIT
+    Ds[0], Gs[0] = D, G
+    for i in range(_NSTEP):
+        x = x0 + h * i
+        k1d, k1g = _growth_rhs(cosmo, x, D, G)
+        k2d, k2g = _growth_rhs(cosmo, x + 0.5 * h, D + 0.5 * h * k1d, G + 0.5 * h * k1g)
+        k3d, k3g = _growth_rhs(cosmo, x + 0.5 * h, D + 0.5 * h * k2d, G + 0.5 * h * k2g)
+        k4d, k4g = _growth_rhs(cosmo, x + h, D + h * k3d, G + h * k3g)
+        D += (h / 6.0) * (k1d + 2 * k2d + 2 * k3d + k4d)
+        G += (h / 6.0) * (k1g + 2 * k2g + 2 * k3g + k4g)
+        Ds[i + 1], Gs[i + 1] = D, G
+    Ds = np.array(Ds) / D
+    Gs = np.array(Gs) / D
+    dG = np.array([_growth_rhs(cosmo, xs[i], Ds[i], Gs[i])[1] for i in range(_NSTEP + 1)])
+    return xs, Ds, Gs, dG
+
+
+def _hermite(xs, f, df, x):
+    """Cubic Hermite interpolation on a uniform grid."""
+    x = np.asarray(x, dtype=np.float64)
+    h = xs[1] - xs[0]
+    i = np.clip(((x - xs[0]) / h).astype(np.int64), 0, len(xs) - 2)
+    t = (x - xs[i]) / h
+    h00 = (1 + 2 * t) * (1 - t) ** 2
+    h10 = t * (1 - t) ** 2
+    h01 = t * t * (3 - 2 * t)
+    h11 = t * t * (t - 1)
+    return h00 * f[i] + h10 * h * df[i] + h01 * f[i + 1] + h11 * h * df[i + 1]
+
+
+def _growth_DG(cosmo: Cosmology, a):
+    xs, D, G, dG = _growth_table(cosmo)
+    x = np.log(np.asarray(a, dtype=np.float64))
+    return _hermite(xs, D, G, x), _hermite(xs, G, dG, x)
+
+
+def growth_factor(cosmo: Cosmology, a):
+    """Linear growth D(a), D(1) = 1."""
+    return _growth_DG(cosmo, a)[0]
+
+
+def growth_rate(cosmo: Cosmology, a):
+    """f = dln D / dln a."""
+    D, G = _growth_DG(cosmo, a)
+    return G / D
+
+
+def Gp(cosmo: Cosmology, a):
+    return growth_factor(cosmo, a)
+
+
+def gp(cosmo: Cosmology, a):
+    """dD/da = f D / a."""
+    a = np.asarray(a, dtype=np.float64)
+    return _growth_DG(cosmo, a)[1] / a
+
+
+def Gf(cosmo: Cosmology, a):
+    """FastPM momentum growth a^3 E dD/da = f D a^2 E."""
+    a = np.asarray(a, dtype=np.float64)
+    return _growth_DG(cosmo, a)[1] * a**2 * E(cosmo, a)
+
+
+def dGfa(cosmo: Cosmology, a):
+    """dGf/da = 1.5 Omega_m D / (a^2 E) (growth equation)."""
+    a = np.asarray(a, dtype=np.float64)
+    return 1.5 * cosmo.Omega_m * growth_factor(cosmo, a) / (a**2 * E(cosmo, a))
+
+
+# ---------------------------------------------------------------------- distances
+_NCHI = 8192
+_A_MIN_CHI = 1e-3
+
+
+@functools.lru_cache(maxsize=1024)
+def _chi_table(cosmo: Cosmology):
+    # chi(a) = c/H0 int_a^1 da' / (a'^2 E(a')), tabulated on a uniform ln a grid with Simpson steps
+    xs = np.linspace(math.log(_A_MIN_CHI), 0.0, _NCHI + 1)
+    a = np.exp(xs)
+    integrand = C_OVER_H0 / (a * E(cosmo, a))  # d chi / d(-ln a)
+    h = xs[1] - xs[0]
+    amid = np.exp(0.5 * (xs[1:] + xs[:-1]))
+    fmid = C_OVER_H0 / (amid * E(cosmo, amid))
+    seg = (h / 6.0) * (integrand[:-1] + 4.0 * fmid + integrand[1:])
+    chi = np.concatenate([np.cumsum(seg[::-1])[::-1], [0.0]])
+    return xs, chi, -integrand
+
+
+def radial_comoving_distance(cosmo: Cosmology, a):
+    xs, chi, dchi = _chi_table(cosmo)
+    return _hermite(xs, chi, dchi, np.log(np.asarray(a, dtype=np.float64)))
+
+
+def a_of_chi(cosmo: Cosmology, chi):
+    """Inverse of radial_comoving_distance (Newton on the interpolant)."""
+    chi = np.asarray(chi, dtype=np.float64)
+    xs, tab, _ = _chi_table(cosmo)
+    x = np.interp(-chi, -tab, xs)
+    for _ in range(8):
+        a = np.exp(x)
+        f = radial_comoving_distance(cosmo, a) - chi
+        dfdx = -C_OVER_H0 / (a * E(cosmo, a))
+        x = np.minimum(x - f / dfdx, 0.0)
+    return np.exp(x)
+
+
+def a2z(a):
+    return 1.0 / np.asarray(a, dtype=np.float64) - 1.0
+
+
+def z2a(z):
+    return 1.0 / (1.0 + np.asarray(z, dtype=np.float64))
+
+
+# ---------------------------------------------------------------------- linear power spectrum
+def eisenstein_hu(cosmo: Cosmology, k):
+    """EH98 transfer function with baryon oscillations, k in h/Mpc (Eisenstein & Hu 1998,
+    eqs. 2-7, 10-12, 14-24)."""
+    k = np.atleast_1d(np.asarray(k, dtype=np.float64))
+    th2 = (T_CMB / 2.7) ** 2
+    h = cosmo.h
+    w_m = cosmo.Omega_m * h * h
+    w_b = cosmo.Omega_b * h * h
+    fb = cosmo.Omega_b / cosmo.Omega_m
+    fc = cosmo.Omega_c / cosmo.Omega_m
+    k_eq = 7.46e-2 * w_m / th2 / h
+    b1 = 0.313 * w_m**-0.419 * (1.0 + 0.607 * w_m**0.674)
+    b2 = 0.238 * w_m**0.223
+    z_d = 1291.0 * w_m**0.251 / (1.0 + 0.659 * w_m**0.828) * (1.0 + b1 * w_m**b2)
+    z_eq = 2.50e4 * w_m / (th2 * th2)
+    R_eq = 31.5 * w_b / (th2 * th2) * (1.0e3 / z_eq)
+    R_d = 31.5 * w_b / (th2 * th2) * (1.0e3 / z_d)
+    sh_d = (
+        2.0 / (3.0 * k_eq) * math.sqrt(6.0 / R_eq)
+        * math.log((math.sqrt(1.0 + R_d) + math.sqrt(R_eq + R_d)) / (1.0 + math.sqrt(R_eq)))
+    )
+    k_silk = 1.6 * w_b**0.52 * w_m**0.73 * (1.0 + (10.4 * w_m) ** -0.95) / h
+    a1 = (46.9 * w_m) ** 0.670 * (1.0 + (32.1 * w_m) ** -0.532)
+    a2 = (12.0 * w_m) ** 0.424 * (1.0 + (45.0 * w_m) ** -0.582)
+    alpha_c = a1**-fb * a2 ** (-(fb**3))
+    bb1 = 0.944 / (1.0 + (458.0 * w_m) ** -0.708)
+    bb2 = (0.395 * w_m) ** -0.0266
+    beta_c = 1.0 / (1.0 + bb1 * (fc**bb2 - 1.0))
+
+    def t_tilde(alpha, beta):
+        q = k / (13.41 * k_eq)
+        L = np.log(math.e + 1.8 * beta * q)
+        C = 14.2 / alpha + 386.0 / (1.0 + 69.9 * q**1.08)
+        return L / (L + C * q * q)
+
+    f = 1.0 / (1.0 + (k * sh_d / 5.4) ** 4)
+    Tc = f * t_tilde(1.0, beta_c) + (1.0 - f) * t_tilde(alpha_c, beta_c)
+    y = (1.0 + z_eq) / (1.0 + z_d)
+    x = math.sqrt(1.0 + y)
+    G = y * (-6.0 * x + (2.0 + 3.0 * y) * math.log((x + 1.0) / (x - 1.0)))
+    alpha_b = 2.07 * k_eq * sh_d * (1.0 + R_d) ** -0.75 * G
+    beta_node = 8.41 * w_m**0.435
+    tilde_s = sh_d / (1.0 + (beta_node / (k * sh_d)) ** 3) ** (1.0 / 3.0)
+    beta_b = 0.5 + fb + (3.0 - 2.0 * fb) * math.sqrt((17.2 * w_m) ** 2 + 1.0)
+    Tb = (
+        t_tilde(1.0, 1.0) / (1.0 + (k * sh_d / 5.2) ** 2)
+        + alpha_b / (1.0 + (beta_b / (k * sh_d)) ** 3) * np.exp(-((k / k_silk) ** 1.4))
+    ) * np.sinc(k * tilde_s / np.pi)
+    return fb * Tb + fc * Tc
+
+
+@functools.lru_cache(maxsize=1024)
+def _sigma8_norm(cosmo: Cosmology) -> float:
+    # sigma^2(R=8) = 1/(2 pi^2) int dlnk k^3 T^2 k^ns W(kR)^2, Simpson on a fine ln k grid
+    lnk = np.linspace(math.log(1e-5), math.log(1e3), 2**15 + 1)
+    k = np.exp(lnk)
+    x = k * 8.0
+    w = 3.0 * (np.sin(x) - x * np.cos(x)) / x**3
+    f = k**3 * eisenstein_hu(cosmo, k) ** 2 * k**cosmo.n_s * w * w / (2.0 * math.pi**2)
+    h = lnk[1] - lnk[0]
+    s = (h / 3.0) * (f[0] + f[-1] + 4.0 * f[1:-1:2].sum() + 2.0 * f[2:-1:2].sum())
+    return cosmo.sigma8**2 / s
+
+
+def linear_matter_power(cosmo: Cosmology, k, a: float = 1.0):
+    """P(k) in (Mpc/h)^3: sigma8-normalised k^ns T_EH(k)^2 D(a)^2."""
+    k = np.asarray(k, dtype=np.float64)
+    return _sigma8_norm(cosmo) * k**cosmo.n_s * eisenstein_hu(cosmo, k) ** 2 * growth_factor(cosmo, a) ** 2

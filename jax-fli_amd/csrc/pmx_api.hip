@@ -1,0 +1,414 @@
+// extern "C" entry points of libpmx: plans, argument checks, and the composition of the pass and
+// particle kernels into the PM force and step pipelines.
+#include "pmx_internal.h"
+#include "../../include/pmx.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+// launchers from pmx_particles.hip
+int pmx_launch_fill(float* a, size_t n, float v, hipStream_t s);
+int pmx_launch_fill_d(double* a, size_t n, hipStream_t s);
+int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, const double box[3],
+                       const double obs[3], double rmin, double rmax, int nside, int scheme, double* acc,
+                       hipStream_t s);
+int pmx_launch_finalize_map(const double* acc, float* map, long n, double scale, int accumulate, hipStream_t s);
+int pmx_launch_flat(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, int fnx, int fny, float zlo, float zhi,
+                    double* acc, hipStream_t s);
+int pmx_launch_philox_raw(uint32_t* out, size_t nblocks, uint64_t seed, hipStream_t s);
+int pmx_launch_white_noise(float* out, size_t n, uint64_t seed, hipStream_t s);
+int pmx_launch_lpt1(const float* F, float* dx, float* pv, size_t n, float D, float c, hipStream_t s);
+
+namespace {
+thread_local std::string g_err;
+
+// k-space modes of the x pass (must match the enum in pmx_fft.hip)
+enum { XP_FWD = 0, XP_INV = 1, XP_FORCES = 2, XP_COLOR = 3, XP_POWER = 4 };
+
+#define PMX_HIP(call)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return pmx_set_error(e_ == hipErrorOutOfMemory ? 4 : 3, std::string(#call) + ": " +    \
+                                                                  hipGetErrorString(e_));    \
+  } while (0)
+#define PMX_TRY(expr)       \
+  do {                      \
+    int rc_ = (expr);       \
+    if (rc_) return rc_;    \
+  } while (0)
+
+template <class T> int dev_alloc(T** ptr, size_t count, size_t& total) {
+  size_t bytes = count * sizeof(T);
+  PMX_HIP(hipMalloc((void**)ptr, bytes < 256 ? 256 : bytes));
+  total += bytes;
+  return 0;
+}
+
+std::vector<float2> twiddles(int n) {
+  std::vector<float2> t(n);
+  for (int k = 0; k < n; ++k) {
+    const double a = -2.0 * M_PI * (double)k / (double)n;
+    t[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  return t;
+}
+
+// numpy fftfreq(n)[i] * 2 pi (cell units); the Nyquist bin i = n/2 maps to -pi
+double wfreq(int i, int n) {
+  const int f = (i < (n + 1) / 2) ? i : i - n;
+  return 2.0 * M_PI * (double)f / (double)n;
+}
+
+void axis_tables(int n, int count, std::vector<float>& g, std::vector<float>& w2, std::vector<float>& dec) {
+  g.resize(count);
+  w2.resize(count);
+  dec.resize(count);
+  for (int i = 0; i < count; ++i) {
+    const double w = wfreq(i, n);
+    const bool nyq = (2 * i == n);
+    g[i] = nyq ? 0.f : (float)((8.0 * std::sin(w) - std::sin(2.0 * w)) / 6.0);
+    w2[i] = (float)(w * w);
+    const double h = 0.5 * w;
+    const double s = (h == 0.0) ? 1.0 : std::sin(h) / h;
+    dec[i] = (float)(1.0 / (s * s * s * s));
+  }
+}
+
+template <class T> int upload(T* dst, const std::vector<T>& src) {
+  PMX_HIP(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int check_plan(pmx_plan_t* p) {
+  if (!p) return pmx_set_error(1, "null plan");
+  int dev = -1;
+  PMX_HIP(hipGetDevice(&dev));
+  if (dev != p->device) return pmx_set_error(1, "plan was created on another device than the current one");
+  return 0;
+}
+
+int check_particles(pmx_plan_t* p, const void* pos, int npx, int npy, int npz, int mode) {
+  if (!pos) return pmx_set_error(1, "null particle array");
+  if (npx <= 0 || npy <= 0 || npz <= 0) return pmx_set_error(1, "particle grid must be positive");
+  if (mode != 0 && mode != 1) return pmx_set_error(1, "mode must be 0 (relative) or 1 (absolute)");
+  if (mode == 0 && (p->nx % npx || p->ny % npy || p->nz % npz))
+    return pmx_set_error(1, "relative mode needs the particle grid to divide the mesh");
+  return 0;
+}
+
+PartArgs part_args(float* x, float* v, int npx, int npy, int npz, int mode, pmx_plan_t* p) {
+  PartArgs a;
+  a.x = x;
+  a.v = v;
+  a.npx = npx;
+  a.npy = npy;
+  a.npz = npz;
+  a.mode = mode;
+  a.sx = (float)p->nx / npx;
+  a.sy = (float)p->ny / npy;
+  a.sz = (float)p->nz / npz;
+  return a;
+}
+
+int ensure_acc(pmx_plan_t* p, size_t count) {
+  const size_t bytes = count * sizeof(double);
+  if (p->acc_bytes >= bytes) return 0;
+  if (p->acc) PMX_HIP(hipFree(p->acc));
+  p->acc = nullptr;
+  p->acc_bytes = 0;
+  PMX_HIP(hipMalloc((void**)&p->acc, bytes));
+  p->acc_bytes = bytes;
+  return 0;
+}
+
+int force_mesh_from(pmx_plan_t* p, const float* src, float scale, unsigned flags, float* out, hipStream_t s) {
+  PMX_TRY(pmx_fft_forward_to_B(p, src, s));
+  KOp op{};
+  op.mode = XP_FORCES;
+  op.scale = (float)((double)scale / (double)p->ncell);
+  op.flags = flags & PMX_DECONV;
+  PMX_TRY(pmx_xpass(p, op, p->specB, p->specC, s));
+  return pmx_inverse_from_C(p, 3, p->specC, out, s);
+}
+
+}  // namespace
+
+int pmx_set_error(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+int pmx_check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return pmx_set_error(3, std::string("kernel launch ") + what + ": " + hipGetErrorString(e));
+  return 0;
+}
+
+extern "C" {
+
+int pmx_abi_version(void) { return 1; }
+const char* pmx_last_error(void) { return g_err.c_str(); }
+int pmx_fft_supported(int n) { return pmx_fft_len_ok(n) ? 1 : 0; }
+
+int pmx_plan_create(pmx_plan* out, int nx, int ny, int nz, unsigned flags) {
+  if (!out) return pmx_set_error(1, "null output handle");
+  *out = nullptr;
+  if (!pmx_fft_len_ok(nx) || !pmx_fft_len_ok(ny) || !pmx_fft_len_ok(nz))
+    return pmx_set_error(2, "unsupported mesh size (each axis must be one of 8,16,32,64,96,128,192,256,512,1024)");
+  pmx_plan_t* p = new pmx_plan_t();
+  std::memset(p, 0, sizeof(*p));
+  p->nx = nx;
+  p->ny = ny;
+  p->nz = nz;
+  p->nzh = nz / 2 + 1;
+  p->flags = flags;
+  p->ncell = (size_t)nx * ny * nz;
+  p->nspec = (size_t)p->nzh * nx * ny;
+  auto fail = [&](int rc) {
+    pmx_plan_destroy(p);
+    return rc;
+  };
+  hipError_t e = hipGetDevice(&p->device);
+  if (e != hipSuccess) {
+    delete p;
+    return pmx_set_error(3, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  }
+  size_t tot = 0;
+  int rc = 0;
+  if ((rc = dev_alloc(&p->tw_x, nx, tot)) || (rc = dev_alloc(&p->tw_y, ny, tot)) || (rc = dev_alloc(&p->tw_z, nz, tot)) ||
+      (rc = dev_alloc(&p->tw_zh, nz / 2, tot)) || (rc = dev_alloc(&p->grad_x, nx, tot)) ||
+      (rc = dev_alloc(&p->grad_y, ny, tot)) || (rc = dev_alloc(&p->grad_z, p->nzh, tot)) ||
+      (rc = dev_alloc(&p->w2_x, nx, tot)) || (rc = dev_alloc(&p->w2_y, ny, tot)) || (rc = dev_alloc(&p->w2_z, p->nzh, tot)) ||
+      (rc = dev_alloc(&p->dec_x, nx, tot)) || (rc = dev_alloc(&p->dec_y, ny, tot)) ||
+      (rc = dev_alloc(&p->dec_z, p->nzh, tot)) || (rc = dev_alloc(&p->rho, p->ncell, tot)) ||
+      (rc = dev_alloc(&p->specA, p->nspec, tot)) || (rc = dev_alloc(&p->specB, p->nspec, tot)) ||
+      (rc = dev_alloc(&p->specC, 3 * p->nspec, tot)) || (rc = dev_alloc(&p->specD, 3 * p->nspec, tot)) ||
+      (rc = dev_alloc(&p->fmesh, 3 * p->ncell, tot)))
+    return fail(rc);
+  if ((rc = dev_alloc(&p->kphys_x, nx, tot)) || (rc = dev_alloc(&p->kphys_y, ny, tot)) ||
+      (rc = dev_alloc(&p->kphys_z, p->nzh, tot)))
+    return fail(rc);
+  p->workspace_bytes = tot;
+  std::vector<float> g, w2, dec;
+  if ((rc = upload(p->tw_x, twiddles(nx))) || (rc = upload(p->tw_y, twiddles(ny))) || (rc = upload(p->tw_z, twiddles(nz))) ||
+      (rc = upload(p->tw_zh, twiddles(nz / 2))))
+    return fail(rc);
+  axis_tables(nx, nx, g, w2, dec);
+  if ((rc = upload(p->grad_x, g)) || (rc = upload(p->w2_x, w2)) || (rc = upload(p->dec_x, dec))) return fail(rc);
+  axis_tables(ny, ny, g, w2, dec);
+  if ((rc = upload(p->grad_y, g)) || (rc = upload(p->w2_y, w2)) || (rc = upload(p->dec_y, dec))) return fail(rc);
+  axis_tables(nz, p->nzh, g, w2, dec);
+  if ((rc = upload(p->grad_z, g)) || (rc = upload(p->w2_z, w2)) || (rc = upload(p->dec_z, dec))) return fail(rc);
+  *out = p;
+  return 0;
+}
+
+int pmx_plan_destroy(pmx_plan p) {
+  if (!p) return 0;
+  void* bufs[] = {p->tw_x, p->tw_y, p->tw_z, p->tw_zh, p->grad_x, p->grad_y, p->grad_z, p->w2_x, p->w2_y, p->w2_z,
+                  p->dec_x, p->dec_y, p->dec_z, p->rho, p->specA, p->specB, p->specC, p->specD, p->fmesh, p->acc,
+                  p->kphys_x, p->kphys_y, p->kphys_z};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  delete p;
+  return 0;
+}
+
+size_t pmx_plan_workspace_bytes(pmx_plan p) { return p ? p->workspace_bytes : 0; }
+
+int pmx_plan_set_box(pmx_plan p, const double box[3]) {
+  PMX_TRY(check_plan(p));
+  if (!box || !(box[0] > 0) || !(box[1] > 0) || !(box[2] > 0)) return pmx_set_error(1, "box must be positive");
+  // the estimator's wavevectors: (2 pi n / L) * fftfreq(n), fftfreq(n)[i] = f(i) * (1/n)
+  auto tab = [&](int n, int count, double L) {
+    std::vector<double> t(count);
+    const double val = 1.0 / (double)n;
+    for (int i = 0; i < count; ++i) {
+      const int f = (i < (n + 1) / 2) ? i : i - n;
+      t[i] = (2.0 * M_PI * (double)n / L) * ((double)f * val);
+    }
+    return t;
+  };
+  PMX_TRY(upload(p->kphys_x, tab(p->nx, p->nx, box[0])));
+  PMX_TRY(upload(p->kphys_y, tab(p->ny, p->ny, box[1])));
+  PMX_TRY(upload(p->kphys_z, tab(p->nz, p->nzh, box[2])));
+  p->box_set = true;
+  return 0;
+}
+
+int pmx_paint_cic(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, const float* weights, float weight,
+                  float* mesh, int accumulate, void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
+  if (!mesh) return pmx_set_error(1, "null mesh");
+  hipStream_t s = (hipStream_t)stream;
+  if (!accumulate) PMX_TRY(pmx_launch_fill(mesh, p->ncell, 0.f, s));
+  PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
+  return pmx_launch_paint(p, a, weights, weight, 0.f, mesh, s);
+}
+
+int pmx_readout_cic(pmx_plan p, const float* mesh, int ncomp, const float* pos, int npx, int npy, int npz, int mode,
+                    float* out, void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
+  if (!mesh || !out) return pmx_set_error(1, "null mesh or output");
+  if (ncomp != 1 && ncomp != 3) return pmx_set_error(1, "ncomp must be 1 or 3");
+  PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
+  return pmx_launch_readout(p, a, mesh, ncomp, out, 0.f, 0.f, 1, (hipStream_t)stream);
+}
+
+int pmx_fft_r2c(pmx_plan p, const float* in, void* spec, void* stream) {
+  PMX_TRY(check_plan(p));
+  if (!in || !spec) return pmx_set_error(1, "null buffer");
+  hipStream_t s = (hipStream_t)stream;
+  PMX_TRY(pmx_fft_forward_to_B(p, in, s));
+  KOp op{};
+  op.mode = XP_FWD;
+  return pmx_xpass(p, op, p->specB, (float2*)spec, s);
+}
+
+int pmx_fft_c2r(pmx_plan p, const void* spec, float* out, void* stream) {
+  PMX_TRY(check_plan(p));
+  if (!spec || !out) return pmx_set_error(1, "null buffer");
+  hipStream_t s = (hipStream_t)stream;
+  KOp op{};
+  op.mode = XP_INV;
+  PMX_TRY(pmx_xpass(p, op, (const float2*)spec, p->specC, s));
+  return pmx_inverse_from_C(p, 1, p->specC, out, s);
+}
+
+int pmx_force_mesh(pmx_plan p, const float* density, float scale, unsigned flags, float* fmesh, void* stream) {
+  PMX_TRY(check_plan(p));
+  if (!density) return pmx_set_error(1, "null density");
+  return force_mesh_from(p, density, scale, flags, fmesh ? fmesh : p->fmesh, (hipStream_t)stream);
+}
+
+int pmx_forces(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, float scale, unsigned flags,
+               float* forces, void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
+  if (!forces) return pmx_set_error(1, "null forces");
+  hipStream_t s = (hipStream_t)stream;
+  PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
+  PMX_TRY(pmx_launch_fill(p->rho, p->ncell, 0.f, s));
+  PMX_TRY(pmx_launch_paint(p, a, nullptr, 1.f, 0.f, p->rho, s));
+  PMX_TRY(force_mesh_from(p, p->rho, scale, flags, p->fmesh, s));
+  return pmx_launch_readout(p, a, p->fmesh, 3, forces, 0.f, 0.f, 1, s);
+}
+
+int pmx_step(pmx_plan p, float* x, float* v, int npx, int npy, int npz, int mode, float force_scale, float c_drift_pre,
+             float c_kick, float c_drift_post, unsigned flags, void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, x, npx, npy, npz, mode));
+  if (!v) return pmx_set_error(1, "null velocities");
+  hipStream_t s = (hipStream_t)stream;
+  PartArgs a = part_args(x, (flags & PMX_DRIFT_PRE) ? v : nullptr, npx, npy, npz, mode, p);
+  PMX_TRY(pmx_launch_fill(p->rho, p->ncell, 0.f, s));
+  PMX_TRY(pmx_launch_paint(p, a, nullptr, 1.f, c_drift_pre, p->rho, s));
+  PMX_TRY(force_mesh_from(p, p->rho, force_scale, flags, p->fmesh, s));
+  a.v = v;
+  const int what = 2 | ((flags & PMX_DRIFT_POST) ? 4 : 0);
+  return pmx_launch_readout(p, a, p->fmesh, 3, nullptr, c_kick, c_drift_post, what, s);
+}
+
+int pmx_power(pmx_plan p, const float* mesh, const double* kedges, int nedges, double* pk_sum, double* k_sum,
+              double* k_cnt, void* stream) {
+  PMX_TRY(check_plan(p));
+  if (!p->box_set) return pmx_set_error(1, "pmx_plan_set_box must be called before pmx_power");
+  if (!mesh || !kedges || !pk_sum || !k_sum || !k_cnt || nedges < 1 || nedges > 512)
+    return pmx_set_error(1, "bad power-spectrum arguments");
+  hipStream_t s = (hipStream_t)stream;
+  PMX_TRY(pmx_launch_fill_d(pk_sum, nedges + 1, s));
+  PMX_TRY(pmx_launch_fill_d(k_sum, nedges + 1, s));
+  PMX_TRY(pmx_launch_fill_d(k_cnt, nedges + 1, s));
+  PMX_TRY(pmx_fft_forward_to_B(p, mesh, s));
+  KOp op{};
+  op.mode = XP_POWER;
+  op.kphys_x = p->kphys_x;
+  op.kphys_y = p->kphys_y;
+  op.kphys_z = p->kphys_z;
+  op.kedges = kedges;
+  op.nedges = nedges;
+  op.pk_sum = pk_sum;
+  op.k_sum = k_sum;
+  op.k_cnt = k_cnt;
+  return pmx_xpass(p, op, p->specB, nullptr, s);
+}
+
+int pmx_white_noise(pmx_plan p, uint64_t seed, float* out, void* stream) {
+  PMX_TRY(check_plan(p));
+  if (!out) return pmx_set_error(1, "null output");
+  return pmx_launch_white_noise(out, p->ncell, seed, (hipStream_t)stream);
+}
+
+int pmx_philox4x32(uint64_t seed, size_t nblocks, uint32_t* out, void* stream) {
+  if (!out) return pmx_set_error(1, "null output");
+  return pmx_launch_philox_raw(out, nblocks, seed, (hipStream_t)stream);
+}
+
+int pmx_color_field(pmx_plan p, const float* field, const double* ktab, const double* ptab, int ntab, float* out,
+                    void* stream) {
+  PMX_TRY(check_plan(p));
+  if (!p->box_set) return pmx_set_error(1, "pmx_plan_set_box must be called before pmx_color_field");
+  if (!field || !ktab || !ptab || ntab < 2 || !out) return pmx_set_error(1, "bad colouring arguments");
+  hipStream_t s = (hipStream_t)stream;
+  PMX_TRY(pmx_fft_forward_to_B(p, field, s));
+  KOp op{};
+  op.mode = XP_COLOR;
+  op.scale = (float)(1.0 / (double)p->ncell);
+  op.kphys_x = p->kphys_x;
+  op.kphys_y = p->kphys_y;
+  op.kphys_z = p->kphys_z;
+  op.ktab = ktab;
+  op.ptab = ptab;
+  op.ntab = ntab;
+  op.cnorm = 1.0;  // the caller folds ncell / volume into ptab
+  PMX_TRY(pmx_xpass(p, op, p->specB, p->specC, s));
+  return pmx_inverse_from_C(p, 1, p->specC, out, s);
+}
+
+int pmx_lpt1(pmx_plan p, const float* ic, float growth, float vel_factor, float* dx, float* pv, void* stream) {
+  PMX_TRY(check_plan(p));
+  if (!ic || !dx || !pv) return pmx_set_error(1, "null buffer");
+  hipStream_t s = (hipStream_t)stream;
+  PMX_TRY(force_mesh_from(p, ic, 1.0f, 0u, p->fmesh, s));
+  return pmx_launch_lpt1(p->fmesh, dx, pv, 3 * p->ncell, growth, vel_factor, s);
+}
+
+int pmx_paint_healpix(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, const float* weights,
+                      float weight, const double box[3], const double observer[3], double rmin, double rmax, int nside,
+                      int scheme, float* map, int accumulate, void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
+  if (!box || !observer || !map) return pmx_set_error(1, "null argument");
+  if (nside < 1 || nside > 8192) return pmx_set_error(1, "nside out of range");
+  if (scheme != 0 && scheme != 1) return pmx_set_error(2, "scheme must be 0 (ngp) or 1 (bilinear)");
+  if (!(rmax > rmin) || rmin < 0) return pmx_set_error(1, "need 0 <= rmin < rmax");
+  const long npix = 12L * nside * nside;
+  PMX_TRY(ensure_acc(p, (size_t)npix));
+  hipStream_t s = (hipStream_t)stream;
+  PMX_TRY(pmx_launch_fill_d(p->acc, npix, s));
+  PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
+  PMX_TRY(pmx_launch_healpix(p, a, weights, weight, box, observer, rmin, rmax, nside, scheme, p->acc, s));
+  const double omega = 4.0 * M_PI / (double)npix;
+  const double vol = omega * (rmax * rmax * rmax - rmin * rmin * rmin) / 3.0;
+  return pmx_launch_finalize_map(p->acc, map, npix, 1.0 / vol, accumulate, s);
+}
+
+int pmx_paint_flat(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, const float* weights,
+                   float weight, int fnx, int fny, float zlo, float zhi, double scale, float* map, int accumulate,
+                   void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
+  if (!map || fnx < 1 || fny < 1) return pmx_set_error(1, "bad flat-sky arguments");
+  const size_t n = (size_t)fnx * fny;
+  PMX_TRY(ensure_acc(p, n));
+  hipStream_t s = (hipStream_t)stream;
+  PMX_TRY(pmx_launch_fill_d(p->acc, n, s));
+  PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
+  PMX_TRY(pmx_launch_flat(p, a, weights, weight, fnx, fny, zlo, zhi, p->acc, s));
+  return pmx_launch_finalize_map(p->acc, map, (long)n, scale, accumulate, s);
+}
+
+}  // extern "C"

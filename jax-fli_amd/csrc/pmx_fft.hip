@@ -1,0 +1,519 @@
+// Hand-written 3D real FFT for gfx950: LDS-resident Stockham radix-8/4/3/2 line transforms,
+// arranged as five bandwidth-bound passes whose stores transpose through LDS so every load and
+// store of every pass is a contiguous run.
+//
+// Forward (r2c) chain, per mesh R[x][y][z] (nx, ny, nz), nzh = nz/2 + 1:
+//   P1  r2c along z (nz/2-point complex FFT + split)   R[x][y][z]   -> A[kz][x][y]
+//   P2  c2c along y                                    A[kz][x][y]  -> B[kz][ky][x]
+//   P3  c2c along x  + k-space operator + inverse x     B[kz][ky][x] -> C[c][kz][x][ky]
+// Inverse chain:
+//   P4  inverse c2c along y                            C[c][kz][x][ky] -> D[c][x][y][kz]
+//   P5  c2r along z (pre-merge + nz/2-point FFT)       D[c][x][y][kz]  -> out[x][y][z][c]
+// P3 holds the whole spectral line along x for one (ky, kz): the Green's function, the gradient and
+// the optional CIC compensation are applied there and the three force spectra are transformed
+// back along x before they ever leave LDS. Transforms are unnormalised; the 1/N of the inverse is
+// folded into the P3 operator.
+#include "pmx_internal.h"
+
+namespace {
+
+// ---------------------------------------------------------------- radix plans
+template <int N> struct FP;
+#define PMX_FP(NN, SS, R0, R1, R2, R3)                        \
+  template <> struct FP<NN> {                                 \
+    static constexpr int S = SS;                              \
+    static constexpr int R[4] = {R0, R1, R2, R3};             \
+  };
+PMX_FP(4, 1, 4, 0, 0, 0)
+PMX_FP(8, 1, 8, 0, 0, 0)
+PMX_FP(16, 2, 4, 4, 0, 0)
+PMX_FP(32, 2, 8, 4, 0, 0)
+PMX_FP(48, 3, 4, 4, 3, 0)
+PMX_FP(64, 2, 8, 8, 0, 0)
+PMX_FP(96, 3, 8, 4, 3, 0)
+PMX_FP(128, 3, 8, 4, 4, 0)
+PMX_FP(192, 3, 8, 8, 3, 0)
+PMX_FP(256, 3, 8, 8, 4, 0)
+PMX_FP(512, 3, 8, 8, 8, 0)
+PMX_FP(1024, 4, 8, 8, 4, 4)
+#undef PMX_FP
+
+// threads per line and lines per workgroup
+template <int N> constexpr int TPL() { return N / FP<N>::R[0]; }
+constexpr int pow2_floor(int v) { int p = 1; while (p * 2 <= v) p *= 2; return p; }
+template <int N> constexpr int LPW() {
+  int l = pow2_floor(256 / TPL<N>());
+  return l < 1 ? 1 : (l > 64 ? 64 : l);
+}
+
+// ---------------------------------------------------------------- complex helpers
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV> __device__ __forceinline__ float2 mul_mi(float2 a) {
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------- butterflies
+template <int R, bool INV> struct Bfly;
+template <bool INV> struct Bfly<2, INV> {
+  __device__ static __forceinline__ void run(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+template <bool INV> struct Bfly<3, INV> {
+  __device__ static __forceinline__ void run(float2* v) {
+    const float s = 0.86602540378443864676f;
+    float2 t = cadd(v[1], v[2]);
+    float2 d = mul_mi<INV>(csub(v[1], v[2]));
+    float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+    v[0] = cadd(v[0], t);
+    v[1] = make_float2(m.x + s * d.x, m.y + s * d.y);
+    v[2] = make_float2(m.x - s * d.x, m.y - s * d.y);
+  }
+};
+template <bool INV> struct Bfly<4, INV> {
+  __device__ static __forceinline__ void run(float2* v) {
+    float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    float2 t2 = cadd(v[1], v[3]), t3 = mul_mi<INV>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+template <bool INV> struct Bfly<8, INV> {
+  __device__ static __forceinline__ void run(float2* v) {
+    const float h = 0.70710678118654752440f;
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    Bfly<4, INV>::run(e);
+    Bfly<4, INV>::run(o);
+    // o[k] *= w^k, w = exp(-+2 pi i / 8)
+    float2 o1 = INV ? make_float2((o[1].x - o[1].y) * h, (o[1].x + o[1].y) * h)
+                    : make_float2((o[1].x + o[1].y) * h, (o[1].y - o[1].x) * h);
+    float2 o2 = mul_mi<INV>(o[2]);
+    float2 o3 = INV ? make_float2((-o[3].x - o[3].y) * h, (o[3].x - o[3].y) * h)
+                    : make_float2((o[3].y - o[3].x) * h, (-o[3].x - o[3].y) * h);
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+  }
+};
+
+// ---------------------------------------------------------------- Stockham stage
+// One radix-R stage of an N-point transform held in LDS, NS = product of the radices already
+// applied. Butterfly j reads src[j + r N/R], twiddles by w^(r (j mod NS) N/(NS R)), writes
+// dst[(j/NS) NS R + j mod NS + r NS] (self-sorting: natural order out after the last stage).
+template <int N, int R, int NS, bool INV>
+__device__ __forceinline__ void stage(const float2* __restrict__ src, float2* __restrict__ dst, int t,
+                                      const float2* __restrict__ tw) {
+  constexpr int T = TPL<N>();
+  constexpr int NB = N / R;
+#pragma unroll
+  for (int jj = 0; jj < (NB + T - 1) / T; ++jj) {
+    const int j = t + jj * T;
+    if ((NB % T == 0) || j < NB) {
+      const int k = j % NS;
+      float2 v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = src[j + r * NB];
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const float2 w = tw[r * k * (N / (NS * R))];
+          v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
+        }
+      }
+      Bfly<R, INV>::run(v);
+      const int d = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[d + r * NS] = v[r];
+    }
+  }
+}
+
+// All stages; every thread of the workgroup must call (barriers inside). The result ends in `a`
+// when the stage count is even, in `b` when odd (see res_in_b).
+template <int N, bool INV, int S = 0, int NS = 1>
+__device__ __forceinline__ void fft_run(float2* a, float2* b, int t, const float2* __restrict__ tw) {
+  if constexpr (S < FP<N>::S) {
+    constexpr int R = FP<N>::R[S];
+    stage<N, R, NS, INV>(a, b, t, tw);
+    __syncthreads();
+    fft_run<N, INV, S + 1, NS * R>(b, a, t, tw);
+  }
+}
+template <int N> constexpr bool res_in_b() { return (FP<N>::S & 1) != 0; }
+
+// ---------------------------------------------------------------- P1: r2c along z
+template <int NZ>
+__global__ void __launch_bounds__(256) k_r2c_z(const float* __restrict__ in, float2* __restrict__ outA,
+                                               int NX, int NY, const float2* __restrict__ twM,
+                                               const float2* __restrict__ twN) {
+  constexpr int M = NZ / 2, T = TPL<M>(), L = LPW<M>(), LS = M + 1, NT = L * T;
+  __shared__ float2 s0[L * LS], s1[L * LS];
+  const int x = blockIdx.y, y0 = blockIdx.x * L, tid = threadIdx.x;
+  const int nl = min(L, NY - y0);
+  const float2* src = reinterpret_cast<const float2*>(in + ((size_t)x * NY + y0) * NZ);
+  for (int i = tid; i < nl * M; i += NT) {
+    const int l = i / M, k = i - l * M;
+    s0[l * LS + k] = src[i];
+  }
+  __syncthreads();
+  const int l = tid / T, t = tid - l * T;
+  fft_run<M, false>(s0 + l * LS, s1 + l * LS, t, twM);
+  const float2* res = res_in_b<M>() ? s1 : s0;
+  float2* oth = res_in_b<M>() ? s0 : s1;
+  // split: X_k = E_k + W^k O_k,  E = (Z_k + conj Z_{M-k})/2,  O = -i (Z_k - conj Z_{M-k})/2
+  for (int i = tid; i < L * (M + 1); i += NT) {
+    const int ll = i / (M + 1), k = i - ll * (M + 1);
+    const float2 zk = res[ll * LS + (k == M ? 0 : k)];
+    const float2 zc = cconj(res[ll * LS + (k == 0 ? 0 : M - k)]);
+    const float2 e = cscale(cadd(zk, zc), 0.5f);
+    const float2 o = cscale(mul_mi<false>(csub(zk, zc)), 0.5f);
+    oth[ll * LS + k] = cadd(e, cmul(twN[k], o));
+  }
+  __syncthreads();
+  float2* dst = outA + (size_t)x * NY + y0;
+  const size_t osk = (size_t)NX * NY;
+  for (int i = tid; i < (M + 1) * L; i += NT) {
+    const int k = i / L, ll = i - k * L;
+    if (ll < nl) dst[k * osk + ll] = oth[ll * LS + k];
+  }
+}
+
+// ---------------------------------------------------------------- P2 / P4: c2c lines
+// line (by, bz, l) starts at in + by*isy + bz*isz + l*isl (elements contiguous); element k of line l
+// is stored at out + by*osy + bz*osz + k*osk + l.
+template <int N, bool INV>
+__global__ void __launch_bounds__(256) k_c2c_lines(const float2* __restrict__ in, float2* __restrict__ out,
+                                                   long isy, long isl, long isz, long osy, long osk, long osz,
+                                                   int nlines, const float2* __restrict__ tw) {
+  constexpr int T = TPL<N>(), L = LPW<N>(), LS = N + 1, NT = L * T;
+  __shared__ float2 s0[L * LS], s1[L * LS];
+  const int by = blockIdx.y, bz = blockIdx.z, l0 = blockIdx.x * L, tid = threadIdx.x;
+  const int nl = min(L, nlines - l0);
+  const float2* src = in + by * isy + bz * isz + (long)l0 * isl;
+  for (int i = tid; i < nl * N; i += NT) {
+    const int l = i / N, k = i - l * N;
+    s0[l * LS + k] = src[l * isl + k];
+  }
+  __syncthreads();
+  const int l = tid / T, t = tid - l * T;
+  fft_run<N, INV>(s0 + l * LS, s1 + l * LS, t, tw);
+  const float2* res = res_in_b<N>() ? s1 : s0;
+  float2* dst = out + by * osy + bz * osz + l0;
+  for (int i = tid; i < N * L; i += NT) {
+    const int k = i / L, ll = i - k * L;
+    if (ll < nl) dst[k * osk + ll] = res[ll * LS + k];
+  }
+}
+
+// ---------------------------------------------------------------- P3: x lines + k-space operator
+enum { XP_FWD = 0, XP_INV = 1, XP_FORCES = 2, XP_COLOR = 3, XP_POWER = 4 };
+
+struct XTabs {
+  const float *gx, *gy, *gz, *w2x, *w2y, *w2z, *dx, *dy, *dz;
+};
+
+__device__ __forceinline__ double interp_lin(double x, const double* __restrict__ xp, const double* __restrict__ fp,
+                                             int n) {
+  // ind = clip(searchsorted_left(xp, x) - 1, 0, n - 2); linear, extrapolating at both ends
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (xp[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  int ind = lo - 1;
+  ind = ind < 0 ? 0 : (ind > n - 2 ? n - 2 : ind);
+  const double xi = xp[ind];
+  return fp[ind] + (x - xi) * (fp[ind + 1] - fp[ind]) / (xp[ind + 1] - xi);
+}
+
+template <int N, int MODE>
+__global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, float2* __restrict__ out, int NY,
+                                               int NZ, long ocomp, KOp op, XTabs tb,
+                                               const float2* __restrict__ tw) {
+  constexpr int T = TPL<N>(), L = LPW<N>(), LS = N + 1, NT = L * T;
+  constexpr int NBUF = (MODE == XP_FORCES) ? 3 : 2;
+  __shared__ float2 sm[NBUF * L * LS];
+  float2* s0 = sm;
+  float2* s1 = sm + L * LS;
+  const int kz = blockIdx.y, ky0 = blockIdx.x * L, tid = threadIdx.x;
+  const int nl = min(L, NY - ky0);
+  const float2* src = in + ((size_t)kz * NY + ky0) * N;
+  for (int i = tid; i < nl * N; i += NT) {
+    const int l = i / N, k = i - l * N;
+    s0[l * LS + k] = src[i];
+  }
+  __syncthreads();
+  const int l = tid / T, t = tid - l * T;
+  constexpr bool FWD = (MODE != XP_INV);
+  fft_run<N, !FWD>(s0 + l * LS, s1 + l * LS, t, tw);
+  float2* res = res_in_b<N>() ? s1 : s0;
+  float2* oth = res_in_b<N>() ? s0 : s1;
+
+  if constexpr (MODE == XP_FWD) {
+    float2* dst = out + ((size_t)kz * NY + ky0) * N;
+    for (int i = tid; i < nl * N; i += NT) {
+      const int ll = i / N, k = i - ll * N;
+      dst[i] = res[ll * LS + k];
+    }
+  } else if constexpr (MODE == XP_INV) {
+    float2* dst = out + (size_t)kz * N * NY + ky0;
+    for (int i = tid; i < N * L; i += NT) {
+      const int k = i / L, ll = i - k * L;
+      if (ll < nl) dst[(size_t)k * NY + ll] = res[ll * LS + k];
+    }
+  } else if constexpr (MODE == XP_FORCES) {
+    float2* s2 = sm + 2 * L * LS;
+    for (int c = 0; c < 3; ++c) {
+      for (int i = tid; i < L * N; i += NT) {
+        const int ll = i / N, kx = i - ll * N;
+        const int ky = min(ky0 + ll, NY - 1);
+        const float k2 = tb.w2x[kx] + tb.w2y[ky] + tb.w2z[kz];
+        const float g = (c == 0) ? tb.gx[kx] : (c == 1 ? tb.gy[ky] : tb.gz[kz]);
+        float coef = (k2 > 0.f) ? g * op.scale / k2 : 0.f;
+        if (op.flags & 1u) coef *= tb.dx[kx] * tb.dy[ky] * tb.dz[kz];
+        const float2 d = res[ll * LS + kx];
+        s2[ll * LS + kx] = make_float2(-coef * d.y, coef * d.x);  // i * coef * d
+      }
+      __syncthreads();
+      fft_run<N, true>(s2 + l * LS, oth + l * LS, t, tw);
+      const float2* r2 = res_in_b<N>() ? oth : s2;
+      float2* dst = out + c * ocomp + (size_t)kz * N * NY + ky0;
+      for (int i = tid; i < N * L; i += NT) {
+        const int k = i / L, ll = i - k * L;
+        if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + k];
+      }
+      __syncthreads();
+    }
+  } else if constexpr (MODE == XP_COLOR) {
+    for (int i = tid; i < L * N; i += NT) {
+      const int ll = i / N, kx = i - ll * N;
+      const int ky = min(ky0 + ll, NY - 1);
+      const double kk = sqrt(op.kphys_x[kx] * op.kphys_x[kx] + op.kphys_y[ky] * op.kphys_y[ky] +
+                             op.kphys_z[kz] * op.kphys_z[kz]);
+      double pk = interp_lin(kk, op.ktab, op.ptab, op.ntab) * op.cnorm;
+      const float amp = (float)(sqrt(pk > 0.0 ? pk : 0.0)) * op.scale;
+      res[ll * LS + kx] = cscale(res[ll * LS + kx], amp);
+    }
+    __syncthreads();
+    fft_run<N, true>(res + l * LS, oth + l * LS, t, tw);
+    const float2* r2 = res_in_b<N>() ? oth : res;
+    float2* dst = out + (size_t)kz * N * NY + ky0;
+    for (int i = tid; i < N * L; i += NT) {
+      const int k = i / L, ll = i - k * L;
+      if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + k];
+    }
+  } else if constexpr (MODE == XP_POWER) {
+    // bins: 0 .. nedges (nedges + 1 entries), bin = number of edges <= |k|
+    extern __shared__ double hist[];  // 3 * (nedges + 1)
+    const int nb = op.nedges + 1;
+    for (int i = tid; i < 3 * nb; i += NT) hist[i] = 0.0;
+    __syncthreads();
+    const double mult = (kz == 0 || 2 * kz == NZ) ? 1.0 : 2.0;
+    for (int i = tid; i < nl * N; i += NT) {
+      const int ll = i / N, kx = i - ll * N;
+      const int ky = ky0 + ll;
+      double s = 0.0;
+      s = __dadd_rn(s, __dmul_rn(op.kphys_x[kx], op.kphys_x[kx]));
+      s = __dadd_rn(s, __dmul_rn(op.kphys_y[ky], op.kphys_y[ky]));
+      s = __dadd_rn(s, __dmul_rn(op.kphys_z[kz], op.kphys_z[kz]));
+      const double kk = __dsqrt_rn(s);
+      int lo = 0, hi = op.nedges;  // upper_bound
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (op.kedges[mid] <= kk) lo = mid + 1; else hi = mid;
+      }
+      const float2 d = res[ll * LS + kx];
+      const double p = ((double)d.x * d.x + (double)d.y * d.y) * mult;
+      atomicAdd(&hist[lo], p);
+      atomicAdd(&hist[nb + lo], kk * mult);
+      atomicAdd(&hist[2 * nb + lo], mult);
+    }
+    __syncthreads();
+    for (int i = tid; i < nb; i += NT) {
+      if (hist[i] != 0.0) atomicAdd(&op.pk_sum[i], hist[i]);
+      if (hist[nb + i] != 0.0) atomicAdd(&op.k_sum[i], hist[nb + i]);
+      if (hist[2 * nb + i] != 0.0) atomicAdd(&op.k_cnt[i], hist[2 * nb + i]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- P5: c2r along z
+// lines (c, l): D[c][x][y0+l][0..nzh) -> out[x][y0+l][z][c] (NC components interleaved)
+template <int NZ, int NC>
+__global__ void __launch_bounds__(384) k_c2r_z(const float2* __restrict__ inD, float* __restrict__ out, int NY,
+                                               long icomp, const float2* __restrict__ twM,
+                                               const float2* __restrict__ twN) {
+  constexpr int M = NZ / 2, T = TPL<M>();
+  constexpr int L = (NC == 1) ? LPW<M>() : (LPW<M>() / 2 > 0 ? LPW<M>() / 2 : 1);
+  constexpr int LS = M + 1, NL = NC * L, NT = NL * T, NZH = M + 1;
+  __shared__ float2 s0[NL * LS], s1[NL * LS];
+  const int x = blockIdx.y, y0 = blockIdx.x * L, tid = threadIdx.x;
+  const int nl = min(L, NY - y0);
+  // load X[k], k = 0..M
+  for (int i = tid; i < NC * nl * NZH; i += NT) {
+    const int c = i / (nl * NZH), rem = i - c * (nl * NZH);
+    const int ll = rem / NZH, k = rem - ll * NZH;
+    s1[(c * L + ll) * LS + k] = inD[c * icomp + ((size_t)x * NY + y0) * NZH + rem];
+  }
+  __syncthreads();
+  // merge: Z'_k = (X_k + conj X_{M-k}) + i (X_k - conj X_{M-k}) conj(W^k)
+  for (int i = tid; i < NL * M; i += NT) {
+    const int ln = i / M, k = i - ln * M;
+    const float2 xk = s1[ln * LS + k];
+    const float2 xc = cconj(s1[ln * LS + (M - k)]);
+    const float2 a = cadd(xk, xc);
+    const float2 b = cmulc(csub(xk, xc), twN[k]);
+    s0[ln * LS + k] = make_float2(a.x - b.y, a.y + b.x);
+  }
+  __syncthreads();
+  const int ln = tid / T, t = tid - ln * T;
+  fft_run<M, true>(s0 + ln * LS, s1 + ln * LS, t, twM);
+  const float2* res = res_in_b<M>() ? s1 : s0;
+  float* dst = out + ((size_t)x * NY + y0) * NZ * NC;
+  for (int i = tid; i < nl * NZ * NC; i += NT) {
+    const int ll = i / (NZ * NC), rem = i - ll * (NZ * NC);
+    const int z = rem / NC, c = rem - z * NC;
+    const float2 v = res[(c * L + ll) * LS + (z >> 1)];
+    dst[i] = (z & 1) ? v.y : v.x;
+  }
+}
+
+// ---------------------------------------------------------------- host dispatch
+#define PMX_FOR_AXIS(M_) \
+  M_(8) M_(16) M_(32) M_(64) M_(96) M_(128) M_(192) M_(256) M_(512) M_(1024)
+
+template <int NZ> int launch_r2c_z(pmx_plan_t* p, const float* in, float2* out, hipStream_t s) {
+  constexpr int M = NZ / 2, L = LPW<M>(), T = TPL<M>();
+  dim3 grid((p->ny + L - 1) / L, p->nx), block(L * T);
+  hipLaunchKernelGGL(k_r2c_z<NZ>, grid, block, 0, s, in, out, p->nx, p->ny, p->tw_zh, p->tw_z);
+  return pmx_check_launch("r2c_z");
+}
+
+template <int N, bool INV>
+int launch_c2c(const float2* in, float2* out, long isy, long isl, long isz, long osy, long osk, long osz,
+               int nlines, int ny_blocks, int nz_blocks, const float2* tw, hipStream_t s) {
+  constexpr int L = LPW<N>(), T = TPL<N>();
+  dim3 grid((nlines + L - 1) / L, ny_blocks, nz_blocks), block(L * T);
+  hipLaunchKernelGGL((k_c2c_lines<N, INV>), grid, block, 0, s, in, out, isy, isl, isz, osy, osk, osz, nlines, tw);
+  return pmx_check_launch("c2c_lines");
+}
+
+template <int N, int MODE> int launch_xpass(pmx_plan_t* p, const KOp& op, const float2* in, float2* out, hipStream_t s) {
+  constexpr int L = LPW<N>(), T = TPL<N>();
+  dim3 grid((p->ny + L - 1) / L, p->nzh), block(L * T);
+  XTabs tb{p->grad_x, p->grad_y, p->grad_z, p->w2_x, p->w2_y, p->w2_z, p->dec_x, p->dec_y, p->dec_z};
+  size_t shm = (MODE == XP_POWER) ? 3 * (size_t)(op.nedges + 1) * sizeof(double) : 0;
+  hipLaunchKernelGGL((k_xpass<N, MODE>), grid, block, shm, s, in, out, p->ny, p->nz, (long)p->nspec, op, tb, p->tw_x);
+  return pmx_check_launch("xpass");
+}
+
+template <int NZ, int NC> int launch_c2r_z(pmx_plan_t* p, const float2* in, float* out, hipStream_t s) {
+  constexpr int M = NZ / 2, T = TPL<M>();
+  constexpr int L = (NC == 1) ? LPW<M>() : (LPW<M>() / 2 > 0 ? LPW<M>() / 2 : 1);
+  dim3 grid((p->ny + L - 1) / L, p->nx), block(NC * L * T);
+  hipLaunchKernelGGL((k_c2r_z<NZ, NC>), grid, block, 0, s, in, out, p->ny, (long)p->nspec, p->tw_zh, p->tw_z);
+  return pmx_check_launch("c2r_z");
+}
+
+}  // namespace
+
+bool pmx_fft_len_ok(int n) {
+  switch (n) {
+#define M_(v) case v:
+    PMX_FOR_AXIS(M_)
+#undef M_
+    return true;
+    default:
+      return false;
+  }
+}
+
+int pmx_pass_r2c_z(pmx_plan_t* p, const float* in, float2* outA, hipStream_t s) {
+  switch (p->nz) {
+#define M_(v) case v: return launch_r2c_z<v>(p, in, outA, s);
+    PMX_FOR_AXIS(M_)
+#undef M_
+  }
+  return pmx_set_error(2, "unsupported nz");
+}
+
+int pmx_pass_c2c_y(pmx_plan_t* p, const float2* inA, float2* outB, hipStream_t s) {
+  // lines: A[kz][x][*] (kz = block y, x = line), out B[kz][ky][x]
+  const long NX = p->nx, NY = p->ny;
+  switch (p->ny) {
+#define M_(v) case v: return launch_c2c<v, false>(inA, outB, NX * NY, NY, 0, NY * NX, NX, 0, p->nx, p->nzh, 1, p->tw_y, s);
+    PMX_FOR_AXIS(M_)
+#undef M_
+  }
+  return pmx_set_error(2, "unsupported ny");
+}
+
+int pmx_fft_forward_to_B(pmx_plan_t* p, const float* in, hipStream_t s) {
+  int rc = pmx_pass_r2c_z(p, in, p->specA, s);
+  if (rc) return rc;
+  return pmx_pass_c2c_y(p, p->specA, p->specB, s);
+}
+
+int pmx_xpass(pmx_plan_t* p, const KOp& op, const float2* inB, float2* out, hipStream_t s) {
+  switch (p->nx) {
+#define M_(v)                                                               \
+  case v:                                                                   \
+    switch (op.mode) {                                                      \
+      case XP_FWD: return launch_xpass<v, XP_FWD>(p, op, inB, out, s);      \
+      case XP_INV: return launch_xpass<v, XP_INV>(p, op, inB, out, s);      \
+      case XP_FORCES: return launch_xpass<v, XP_FORCES>(p, op, inB, out, s); \
+      case XP_COLOR: return launch_xpass<v, XP_COLOR>(p, op, inB, out, s);  \
+      case XP_POWER: return launch_xpass<v, XP_POWER>(p, op, inB, out, s);  \
+    }                                                                       \
+    return pmx_set_error(2, "bad k-space mode");
+    PMX_FOR_AXIS(M_)
+#undef M_
+  }
+  return pmx_set_error(2, "unsupported nx");
+}
+
+int pmx_inverse_from_C(pmx_plan_t* p, int ncomp, const float2* inC, float* out, hipStream_t s) {
+  // P4: lines C[c][kz][x][*] for consecutive kz at fixed x -> D[c][x][ky][kz]
+  const long NX = p->nx, NY = p->ny, NZH = p->nzh;
+  int rc = 0;
+  switch (p->ny) {
+#define M_(v)                                                                                     \
+  case v:                                                                                         \
+    rc = launch_c2c<v, true>(inC, p->specD, NY, NX * NY, (long)p->nspec, NY * NZH, NZH, (long)p->nspec, \
+                             p->nzh, p->nx, ncomp, p->tw_y, s);                                   \
+    break;
+    PMX_FOR_AXIS(M_)
+#undef M_
+    default:
+      return pmx_set_error(2, "unsupported ny");
+  }
+  if (rc) return rc;
+  if (ncomp != 1 && ncomp != 3) return pmx_set_error(2, "ncomp must be 1 or 3");
+  switch (p->nz) {
+#define M_(v) \
+  case v: return ncomp == 3 ? launch_c2r_z<v, 3>(p, p->specD, out, s) : launch_c2r_z<v, 1>(p, p->specD, out, s);
+    PMX_FOR_AXIS(M_)
+#undef M_
+  }
+  return pmx_set_error(2, "unsupported nz");
+}

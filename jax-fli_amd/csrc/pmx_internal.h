@@ -1,0 +1,86 @@
+// Internal declarations shared by the pmx translation units.
+//
+// Layout conventions (all row-major, FP32 unless noted):
+//   real mesh      R[x][y][z]                (nx, ny, nz)
+//   particles      P[i][j][k][3]             (npx, npy, npz, 3)  -- the reference's (X, Y, Z, 3)
+//   force mesh     F[x][y][z][3]             interleaved so one readout corner is 12 contiguous bytes
+//   spectra        half spectrum along z, nzh = nz/2 + 1 complex bins, stored in pass-specific
+//                  transposed orders (see the pass kernels in pmx_fft.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+struct pmx_plan_t {
+  int nx, ny, nz, nzh;
+  int device;
+  unsigned flags;
+  size_t ncell;   // nx*ny*nz
+  size_t nspec;   // nzh*nx*ny (complex)
+  // twiddle tables exp(-2 pi i k / n), one per line length that the passes use
+  float2 *tw_x, *tw_y, *tw_z, *tw_zh;
+  // per-axis spectral tables, index = FFT bin along that axis (z: 0..nz/2)
+  float *grad_x, *grad_y, *grad_z;  // FastPM 4-point gradient (8 sin w - sin 2w)/6, w = 2 pi fftfreq
+  float *w2_x, *w2_y, *w2_z;        // w^2
+  float *dec_x, *dec_y, *dec_z;     // CIC window compensation sinc(w/2)^-4
+  // work buffers owned by the plan
+  float* rho;       // density mesh
+  float2* specA;    // [kz][x][y]
+  float2* specB;    // [kz][ky][x]
+  float2* specC;    // 3 x [kz][x][ky]
+  float2* specD;    // 3 x [x][y][kz]
+  float* fmesh;     // force mesh [x][y][z][3]
+  double *kphys_x, *kphys_y, *kphys_z;  // estimator wavevectors (h/Mpc) per bin, set by pmx_plan_set_box
+  bool box_set;
+  double* acc;      // double accumulators (healpix maps, flat maps); grown on demand
+  size_t acc_bytes;
+  size_t workspace_bytes;
+};
+
+// error reporting (pmx_api.hip)
+int pmx_set_error(int code, const std::string& msg);
+int pmx_check_launch(const char* what);
+
+// supported line lengths for the in-LDS Stockham FFT
+bool pmx_fft_len_ok(int n);
+
+// FFT passes (pmx_fft.hip). All asynchronous on `stream`. Return 0 or an error code.
+struct KOp {
+  int mode;               // see enum in pmx_fft.hip
+  float scale;            // multiplies the spectrum (already divided by the cell count where relevant)
+  unsigned flags;         // bit0: CIC deconvolution
+  // colouring: amplitude = sqrt(P(|k|) * norm), P linearly interpolated in (ktab, ptab)
+  const double* kphys_x;  // |k| components in h/Mpc per bin index, as float64
+  const double* kphys_y;
+  const double* kphys_z;
+  const double* ktab;
+  const double* ptab;
+  int ntab;
+  double cnorm;
+  // power spectrum binning
+  const double* kedges;
+  int nedges;
+  double* pk_sum;         // nedges+1 bins
+  double* k_sum;
+  double* k_cnt;
+};
+
+int pmx_fft_forward_to_B(pmx_plan_t* p, const float* in, hipStream_t s);                 // R -> specB
+int pmx_xpass(pmx_plan_t* p, const KOp& op, const float2* inB, float2* out, hipStream_t s);  // specB -> out
+int pmx_inverse_from_C(pmx_plan_t* p, int ncomp, const float2* inC, float* out, hipStream_t s);  // specC -> real
+int pmx_pass_r2c_z(pmx_plan_t* p, const float* in, float2* outA, hipStream_t s);
+int pmx_pass_c2c_y(pmx_plan_t* p, const float2* inA, float2* outB, hipStream_t s);
+
+// particle kernels (pmx_particles.hip)
+struct PartArgs {
+  float* x;            // positions/displacements (npx,npy,npz,3)
+  float* v;            // velocities (may be null)
+  int npx, npy, npz;
+  int mode;            // 0 relative (lattice + dx), 1 absolute
+  float sx, sy, sz;    // lattice spacing in cells (mesh / particle grid)
+};
+int pmx_launch_paint(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, float cd_pre,
+                     float* mesh, hipStream_t s);
+int pmx_launch_readout(pmx_plan_t* p, const PartArgs& a, const float* mesh, int ncomp, float* out,
+                       float ck, float cd_post, int what, hipStream_t s);

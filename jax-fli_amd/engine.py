@@ -1,0 +1,212 @@
+"""Tensor-level wrappers of the pmx C ABI (device tensors in, device tensors out).
+
+These are the building blocks the reference-shaped API (fields, solvers, lpt, nbody) calls.
+All work is enqueued on torch's current stream; nothing synchronises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import check, dbl3, get_plan, load, ptr, stream_ptr
+
+MODES = {"relative": 0, "absolute": 1}
+
+
+def _f32(t: torch.Tensor, name: str) -> torch.Tensor:
+    if t.dtype != torch.float32:
+        raise TypeError(f"{name} must be float32, got {t.dtype}")
+    if not t.is_cuda:
+        raise _lib.PmxError(f"{name} must live on the GPU")
+    return t.contiguous()
+
+
+def _pgrid(pos: torch.Tensor):
+    if pos.ndim != 4 or pos.shape[-1] != 3:
+        raise ValueError(f"particle array must have shape (X, Y, Z, 3), got {tuple(pos.shape)}")
+    return int(pos.shape[0]), int(pos.shape[1]), int(pos.shape[2])
+
+
+def _weights(weights, npart: int):
+    if weights is None:
+        return None, 1.0
+    if isinstance(weights, (int, float)):
+        return None, float(weights)
+    w = _f32(torch.as_tensor(weights), "weights")
+    if w.numel() == 1:
+        return None, float(w.item())
+    if w.numel() != npart:
+        raise ValueError("weights must be a scalar or one value per particle")
+    return w, 1.0
+
+
+def paint(pos: torch.Tensor, mesh_shape, mode="relative", weights=1.0, out: torch.Tensor | None = None,
+          accumulate=False) -> torch.Tensor:
+    pos = _f32(pos, "pos")
+    npx, npy, npz = _pgrid(pos)
+    plan = get_plan(mesh_shape, pos.device)
+    if out is None:
+        out = torch.empty(tuple(mesh_shape), dtype=torch.float32, device=pos.device)
+    w, w0 = _weights(weights, npx * npy * npz)
+    check(load().pmx_paint_cic(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], ptr(w), w0, ptr(out),
+                               int(accumulate), stream_ptr()), "pmx_paint_cic")
+    return out
+
+
+def readout(mesh: torch.Tensor, pos: torch.Tensor, mode="relative") -> torch.Tensor:
+    mesh = _f32(mesh, "mesh")
+    pos = _f32(pos, "pos")
+    npx, npy, npz = _pgrid(pos)
+    ncomp = 1 if mesh.ndim == 3 else int(mesh.shape[3])
+    plan = get_plan(tuple(mesh.shape[:3]), pos.device)
+    out = torch.empty((npx, npy, npz) + ((ncomp,) if mesh.ndim == 4 else ()), dtype=torch.float32, device=pos.device)
+    check(load().pmx_readout_cic(plan.handle, ptr(mesh), ncomp, ptr(pos), npx, npy, npz, MODES[mode], ptr(out),
+                                 stream_ptr()), "pmx_readout_cic")
+    return out
+
+
+def rfftn(mesh: torch.Tensor) -> torch.Tensor:
+    """Unnormalised r2c 3D FFT. Returns complex64 laid out [kz][ky][kx] (nz/2+1, ny, nx)."""
+    mesh = _f32(mesh, "mesh")
+    nx, ny, nz = mesh.shape
+    plan = get_plan(mesh.shape, mesh.device)
+    spec = torch.empty((nz // 2 + 1, ny, nx), dtype=torch.complex64, device=mesh.device)
+    check(load().pmx_fft_r2c(plan.handle, ptr(mesh), ptr(spec), stream_ptr()), "pmx_fft_r2c")
+    return spec
+
+
+def irfftn(spec: torch.Tensor, mesh_shape) -> torch.Tensor:
+    """Unnormalised c2r inverse of rfftn (result = N * x)."""
+    if spec.dtype != torch.complex64:
+        raise TypeError("spectrum must be complex64")
+    spec = spec.contiguous()
+    plan = get_plan(mesh_shape, spec.device)
+    out = torch.empty(tuple(mesh_shape), dtype=torch.float32, device=spec.device)
+    check(load().pmx_fft_c2r(plan.handle, ptr(spec), ptr(out), stream_ptr()), "pmx_fft_c2r")
+    return out
+
+
+def force_mesh(delta: torch.Tensor, scale=1.0, deconv=False) -> torch.Tensor:
+    delta = _f32(delta, "density")
+    plan = get_plan(delta.shape, delta.device)
+    out = torch.empty(tuple(delta.shape) + (3,), dtype=torch.float32, device=delta.device)
+    check(load().pmx_force_mesh(plan.handle, ptr(delta), float(scale), _lib.DECONV if deconv else 0, ptr(out),
+                                stream_ptr()), "pmx_force_mesh")
+    return out
+
+
+def forces(pos: torch.Tensor, mesh_shape, mode="relative", scale=1.0, deconv=False) -> torch.Tensor:
+    pos = _f32(pos, "pos")
+    npx, npy, npz = _pgrid(pos)
+    plan = get_plan(mesh_shape, pos.device)
+    out = torch.empty_like(pos)
+    check(load().pmx_forces(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], float(scale),
+                            _lib.DECONV if deconv else 0, ptr(out), stream_ptr()), "pmx_forces")
+    return out
+
+
+def step(x: torch.Tensor, v: torch.Tensor, mesh_shape, mode="relative", force_scale=1.0, c_kick=0.0,
+         c_drift_pre=None, c_drift_post=None, deconv=False) -> None:
+    """In place: [x += v*c_drift_pre]; v += F(x)*force_scale*c_kick; [x += v*c_drift_post]."""
+    if not (x.is_contiguous() and v.is_contiguous()):
+        raise _lib.PmxError("state tensors must be contiguous")
+    _f32(x, "x")
+    _f32(v, "v")
+    npx, npy, npz = _pgrid(x)
+    if v.shape != x.shape:
+        raise ValueError("x and v must have the same shape")
+    plan = get_plan(mesh_shape, x.device)
+    flags = (_lib.DECONV if deconv else 0)
+    flags |= _lib.DRIFT_PRE if c_drift_pre is not None else 0
+    flags |= _lib.DRIFT_POST if c_drift_post is not None else 0
+    check(load().pmx_step(plan.handle, ptr(x), ptr(v), npx, npy, npz, MODES[mode], float(force_scale),
+                          float(c_drift_pre or 0.0), float(c_kick), float(c_drift_post or 0.0), flags,
+                          stream_ptr()), "pmx_step")
+
+
+def power_sums(mesh: torch.Tensor, box, kedges: torch.Tensor):
+    """Device sums (pk_sum, k_sum, k_cnt) over nedges+1 bins, float64."""
+    mesh = _f32(mesh, "mesh")
+    plan = get_plan(mesh.shape, mesh.device)
+    plan.set_box(box)
+    kedges = kedges.to(device=mesh.device, dtype=torch.float64).contiguous()
+    nb = kedges.numel() + 1
+    pk = torch.empty(nb, dtype=torch.float64, device=mesh.device)
+    ks = torch.empty_like(pk)
+    kc = torch.empty_like(pk)
+    check(load().pmx_power(plan.handle, ptr(mesh), ptr(kedges), kedges.numel(), ptr(pk), ptr(ks), ptr(kc),
+                           stream_ptr()), "pmx_power")
+    return pk, ks, kc
+
+
+def white_noise(mesh_shape, seed: int, device=None) -> torch.Tensor:
+    plan = get_plan(mesh_shape, device)
+    out = torch.empty(tuple(mesh_shape), dtype=torch.float32, device=plan.device)
+    check(load().pmx_white_noise(plan.handle, ctypes.c_uint64(int(seed) & (2**64 - 1)), ptr(out), stream_ptr()),
+          "pmx_white_noise")
+    return out
+
+
+def philox_raw(seed: int, nblocks: int, device=None) -> torch.Tensor:
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    out = torch.empty((nblocks, 4), dtype=torch.int32, device=dev)
+    check(load().pmx_philox4x32(ctypes.c_uint64(int(seed) & (2**64 - 1)), nblocks, ptr(out), stream_ptr()),
+          "pmx_philox4x32")
+    return out
+
+
+def color_field(field: torch.Tensor, box, ktab: torch.Tensor, ptab: torch.Tensor) -> torch.Tensor:
+    field = _f32(field, "field")
+    plan = get_plan(field.shape, field.device)
+    plan.set_box(box)
+    ktab = ktab.to(device=field.device, dtype=torch.float64).contiguous()
+    ptab = ptab.to(device=field.device, dtype=torch.float64).contiguous()
+    out = torch.empty_like(field)
+    check(load().pmx_color_field(plan.handle, ptr(field), ptr(ktab), ptr(ptab), ktab.numel(), ptr(out),
+                                 stream_ptr()), "pmx_color_field")
+    return out
+
+
+def lpt1(ic: torch.Tensor, growth: float, vel_factor: float):
+    ic = _f32(ic, "ic")
+    plan = get_plan(ic.shape, ic.device)
+    dx = torch.empty(tuple(ic.shape) + (3,), dtype=torch.float32, device=ic.device)
+    pv = torch.empty_like(dx)
+    check(load().pmx_lpt1(plan.handle, ptr(ic), float(growth), float(vel_factor), ptr(dx), ptr(pv), stream_ptr()),
+          "pmx_lpt1")
+    return dx, pv
+
+
+def paint_healpix(pos: torch.Tensor, mesh_shape, box, observer, rmin, rmax, nside, scheme="bilinear",
+                  mode="relative", weights=1.0, out=None, accumulate=False) -> torch.Tensor:
+    pos = _f32(pos, "pos")
+    npx, npy, npz = _pgrid(pos)
+    plan = get_plan(mesh_shape, pos.device)
+    if out is None:
+        out = torch.empty(12 * nside * nside, dtype=torch.float32, device=pos.device)
+    w, w0 = _weights(weights, npx * npy * npz)
+    sch = {"ngp": 0, "bilinear": 1}.get(scheme)
+    if sch is None:
+        raise NotImplementedError(f"spherical painting scheme {scheme!r} is not supported (ngp, bilinear)")
+    check(load().pmx_paint_healpix(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], ptr(w), w0, dbl3(box),
+                                   dbl3(observer), float(rmin), float(rmax), int(nside), sch, ptr(out),
+                                   int(accumulate), stream_ptr()), "pmx_paint_healpix")
+    return out
+
+
+def paint_flat(pos: torch.Tensor, mesh_shape, flatsky_npix, zlo, zhi, scale, mode="absolute", weights=1.0,
+               out=None, accumulate=False) -> torch.Tensor:
+    pos = _f32(pos, "pos")
+    npx, npy, npz = _pgrid(pos)
+    plan = get_plan(mesh_shape, pos.device)
+    fnx, fny = int(flatsky_npix[0]), int(flatsky_npix[1])
+    if out is None:
+        out = torch.empty((fnx, fny), dtype=torch.float32, device=pos.device)
+    w, w0 = _weights(weights, npx * npy * npz)
+    check(load().pmx_paint_flat(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], ptr(w), w0, fnx, fny,
+                                float(zlo), float(zhi), float(scale), ptr(out), int(accumulate), stream_ptr()),
+          "pmx_paint_flat")
+    return out

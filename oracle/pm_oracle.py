@@ -1,0 +1,424 @@
+"""CPU oracle for the PM force + integration path.
+
+TEST INFRASTRUCTURE ONLY. This module is a NumPy restatement of the reference algorithm, used
+by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker (and the
+timed CPU baseline). The product (jax-fli_amd/) never imports it.
+
+Each function cites the reference file:line it restates. The PM arithmetic of the reference lives
+in the un-vendored ``jaxpm`` fork (pyproject.toml:32, branch ALL-MODIF, no commit hash); its
+published algorithm (FastPM, Feng et al. 2016; pmwd's multilinear enmesh for ``cic_paint_dx``)
+is restated here and anchored on the reference's call sites. How each function is pinned (closed
+forms, invariants, brute force, known-answer vectors) is listed in tests/test_oracle.py.
+
+Parity status per function:
+  cic_paint_rel / cic_paint_abs / cic_read_*   pinned (mass, zero displacement, roll, adjointness,
+                                               single-particle closed form, brute force)
+  fftk / gradient / invlaplace / pm_forces     pinned (plane wave, momentum conservation,
+                                               two-particle antisymmetry, brute-force DFT)
+  power                                        pinned (plane wave, exact restatement of _power)
+  philox4x32                                   pinned (Random123 known-answer vectors)
+  healpix_ang2pix / healpix_interp_weights     pinned (pixel centres round-trip, nside=1 base
+                                               pixels, equal-area counts, interpolation identities)
+  paint_healpix normalisation / shell edges    parity unpinned (jaxpm.spherical is not in the
+                                               reference tree and no reference test covers it)
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+# ---------------------------------------------------------------------------- CIC
+# jaxpm.painting.cic_paint_dx (called at src/jax_fli/_src/fields/_painting.py:67-73): particle
+# (i,j,k) at lattice (i,j,k)*r + dx, corners floor(dx) + {0,1}, weights 1 - |frac - {0,1}|,
+# periodic wrap. jaxpm.painting.cic_paint (_painting.py:76-82): weights 1 - |x - (floor(x)+{0,1})|.
+
+
+def _lattice(np_shape, mesh_shape):
+    r = [m // n for m, n in zip(mesh_shape, np_shape)]
+    grids = np.meshgrid(*[np.arange(n) * ri for n, ri in zip(np_shape, r)], indexing="ij")
+    return np.stack(grids, axis=-1).reshape(-1, 3).astype(np.int64)
+
+
+def _corners(pos, mesh_shape, mode, np_shape, dtype):
+    """Per-dimension base indices and the two weights, in `dtype` arithmetic."""
+    x = np.asarray(pos, dtype=dtype).reshape(-1, 3)
+    fl = np.floor(x)
+    one = dtype(1.0)
+    if mode == "relative":
+        fr = x - fl
+        w0 = one - np.abs(fr)
+        w1 = one - np.abs(fr - one)
+        base = _lattice(np_shape, mesh_shape) + fl.astype(np.int64)
+    else:
+        w0 = one - np.abs(x - fl)
+        w1 = one - np.abs(x - (fl + one))
+        base = fl.astype(np.int64)
+    return base, w0, w1
+
+
+def _offsets():
+    # x fastest: (0,0,0),(1,0,0),(0,1,0),(1,1,0),(0,0,1),...
+    return [((o >> 0) & 1, (o >> 1) & 1, (o >> 2) & 1) for o in range(8)]
+
+
+def cic_paint(pos, mesh_shape, mode="relative", weight=1.0, dtype=np.float32):
+    """CIC scatter; accumulation in float64, result cast to `dtype`."""
+    pos = np.asarray(pos)
+    np_shape = pos.shape[:3]
+    base, w0, w1 = _corners(pos, mesh_shape, mode, np_shape, dtype)
+    wt = np.broadcast_to(np.asarray(weight, dtype=dtype).reshape(-1), (base.shape[0],))
+    mesh = np.zeros(int(np.prod(mesh_shape)), dtype=np.float64)
+    nx, ny, nz = mesh_shape
+    for ox, oy, oz in _offsets():
+        wx = w1[:, 0] if ox else w0[:, 0]
+        wy = w1[:, 1] if oy else w0[:, 1]
+        wz = w1[:, 2] if oz else w0[:, 2]
+        w = ((wx * wy) * wz) * wt
+        ix = np.mod(base[:, 0] + ox, nx)
+        iy = np.mod(base[:, 1] + oy, ny)
+        iz = np.mod(base[:, 2] + oz, nz)
+        flat = (ix * ny + iy) * nz + iz
+        mesh += np.bincount(flat, weights=w.astype(np.float64), minlength=mesh.size)
+    return mesh.reshape(mesh_shape).astype(dtype)
+
+
+def cic_read(mesh, pos, mode="relative", dtype=np.float32):
+    """CIC gather of a (X,Y,Z) or (X,Y,Z,C) mesh; corners summed in x-fastest order."""
+    mesh = np.asarray(mesh, dtype=dtype)
+    mesh_shape = mesh.shape[:3]
+    ncomp = 1 if mesh.ndim == 3 else mesh.shape[3]
+    m = mesh.reshape(-1, ncomp)
+    pos = np.asarray(pos)
+    np_shape = pos.shape[:3]
+    base, w0, w1 = _corners(pos, mesh_shape, mode, np_shape, dtype)
+    nx, ny, nz = mesh_shape
+    out = np.zeros((base.shape[0], ncomp), dtype=dtype)
+    for ox, oy, oz in _offsets():
+        wx = w1[:, 0] if ox else w0[:, 0]
+        wy = w1[:, 1] if oy else w0[:, 1]
+        wz = w1[:, 2] if oz else w0[:, 2]
+        w = (wx * wy) * wz
+        ix = np.mod(base[:, 0] + ox, nx)
+        iy = np.mod(base[:, 1] + oy, ny)
+        iz = np.mod(base[:, 2] + oz, nz)
+        out = out + m[(ix * ny + iy) * nz + iz] * w[:, None]
+    out = out.reshape(tuple(np_shape) + (ncomp,))
+    return out[..., 0] if mesh.ndim == 3 else out
+
+
+# ---------------------------------------------------------------------------- k-space kernels
+# jaxpm.kernels.fftk / gradient_kernel / invlaplace_kernel as used by pm_forces (called from
+# _forces, src/jax_fli/pm/solvers.py:19-32): k = 2 pi fftfreq(n) in cell units, FastPM 4-point
+# gradient i (8 sin k - sin 2k)/6, Green's function -1/k^2 with 0 at k = 0, no CIC compensation.
+
+
+def fftk(mesh_shape, dtype=np.float64):
+    ks = []
+    for d, n in enumerate(mesh_shape):
+        k = (2.0 * np.pi * np.fft.fftfreq(n)).astype(dtype)
+        shape = [1, 1, 1]
+        shape[d] = n
+        ks.append(k.reshape(shape))
+    return ks
+
+
+def gradient_kernel(kvec, direction):
+    w = kvec[direction]
+    return 1j * (8.0 * np.sin(w) - np.sin(2.0 * w)) / 6.0
+
+
+def invlaplace_kernel(kvec):
+    kk = sum(k**2 for k in kvec)
+    kk_nz = np.where(kk == 0, 1.0, kk)
+    return -np.where(kk == 0, 0.0, 1.0 / kk_nz)
+
+
+def cic_window_comp(kvec):
+    """1 / prod_d sinc(k_d/2)^4 (compensation of paint + readout; off in the reference)."""
+    out = 1.0
+    for k in kvec:
+        s = np.sinc(k / (2.0 * np.pi))
+        out = out / s**4
+    return out
+
+
+def force_mesh(delta, deconv=False, dtype=np.float64):
+    """(X,Y,Z,3) force mesh of a density mesh: ifft(-grad_i * invlaplace * fft(delta)).real.
+    The Nyquist gradient is exactly 0 (sin(pi) = 0)."""
+    mesh_shape = delta.shape
+    cdt = np.complex128 if dtype == np.float64 else np.complex64
+    dk = np.fft.fftn(np.asarray(delta, dtype=dtype)).astype(cdt)
+    kvec = fftk(mesh_shape, np.float64)
+    pot = dk * invlaplace_kernel(kvec)
+    if deconv:
+        pot = pot * cic_window_comp(kvec)
+    out = np.empty(tuple(mesh_shape) + (3,), dtype=dtype)
+    for i in range(3):
+        g = gradient_kernel(kvec, i)
+        n = mesh_shape[i]
+        if n % 2 == 0:
+            sl = [slice(None)] * 3
+            sl[i] = n // 2
+            g = np.array(np.broadcast_to(g, kvec[i].shape))
+            g[tuple(sl)] = 0.0
+        out[..., i] = np.fft.ifftn(-g * pot).real.astype(dtype)
+    return out
+
+
+def pm_forces(pos, mesh_shape, mode="relative", scale=1.0, deconv=False, dtype=np.float64):
+    """jaxpm.pm.pm_forces(pos) * scale (scale = 1.5 Omega_m in _forces, solvers.py:29-30)."""
+    rho = cic_paint(pos, mesh_shape, mode, dtype=dtype)
+    fm = force_mesh(rho, deconv=deconv, dtype=dtype) * dtype(scale)
+    return cic_read(fm.astype(dtype), pos, mode, dtype=dtype)
+
+
+# ---------------------------------------------------------------------------- P(k)
+# _power / _initialize_pk (src/jax_fli/_src/power/_compute.py:23-119), auto spectrum, monopole.
+
+
+def power(mesh, box_shape, kedges=None):
+    mesh = np.asarray(mesh, dtype=np.float64)
+    mesh_shape = mesh.shape
+    box = np.asarray(box_shape, dtype=np.float64)
+    ms = np.asarray(mesh_shape)
+    if kedges is None:
+        kmax = np.pi * np.min(ms / box)
+        dk = 2 * np.pi / np.min(box) * 2
+        kedges = np.arange(dk, kmax, dk) + dk / 2
+        if kedges.size < 2:
+            kedges = np.linspace(kmax / 4, kmax * 0.9, 2)
+    kedges = np.asarray(kedges, dtype=np.float64)
+    kvec = []
+    for i, (m, b) in enumerate(zip(mesh_shape, box)):
+        shape = [1, 1, 1]
+        shape[i] = m
+        kvec.append(((2 * np.pi * m / b) * np.fft.fftfreq(m)).reshape(shape))
+    kmesh = np.sqrt(sum(k**2 for k in kvec))
+    dig = np.digitize(np.broadcast_to(kmesh, mesh_shape).reshape(-1), kedges)
+    nb = kedges.size + 1
+    kcount = np.bincount(dig, minlength=nb)
+    kavg = np.bincount(dig, weights=np.broadcast_to(kmesh, mesh_shape).reshape(-1), minlength=nb)
+    kavg = (kavg / np.where(kcount == 0, 1, kcount))[1:-1]
+    meshk = np.fft.fftn(mesh, norm="ortho")
+    mmk = meshk.real**2 + meshk.imag**2
+    psum = np.bincount(dig, weights=mmk.reshape(-1), minlength=nb)
+    pk = (psum / np.where(kcount > 0, kcount, 1))[1:-1] * np.prod(box / ms)
+    return kavg, pk
+
+
+# ---------------------------------------------------------------------------- Philox4x32-10
+# Salmon, Moraes, Dror, Shaw (SC 2011), Random123. Used for the white-noise input.
+
+_M0, _M1, _W0, _W1 = 0xD2511F53, 0xCD9E8D57, 0x9E3779B9, 0xBB67AE85
+
+
+def philox4x32(counters, key):
+    """counters: (n, 4) uint32; key: (k0, k1). Returns (n, 4) uint32."""
+    c = np.asarray(counters, dtype=np.uint64).copy()
+    k0, k1 = np.uint64(key[0]), np.uint64(key[1])
+    mask = np.uint64(0xFFFFFFFF)
+    for _ in range(10):
+        p0 = np.uint64(_M0) * c[:, 0]
+        p1 = np.uint64(_M1) * c[:, 2]
+        n0 = (p1 >> np.uint64(32)) ^ c[:, 1] ^ k0
+        n1 = p1 & mask
+        n2 = (p0 >> np.uint64(32)) ^ c[:, 3] ^ k1
+        n3 = p0 & mask
+        c = np.stack([n0, n1, n2, n3], axis=1) & mask
+        k0 = (k0 + np.uint64(_W0)) & mask
+        k1 = (k1 + np.uint64(_W1)) & mask
+    return c.astype(np.uint32)
+
+
+def white_noise(shape, seed):
+    n = int(np.prod(shape))
+    nb = (n + 3) // 4
+    b = np.arange(nb, dtype=np.uint64)
+    ctr = np.stack([b & np.uint64(0xFFFFFFFF), b >> np.uint64(32), np.zeros_like(b), np.zeros_like(b)], axis=1)
+    r = philox4x32(ctr, (seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF)).astype(np.float64)
+    u = (np.floor(r / 256.0) + 1.0) / 16777216.0
+    r0 = np.sqrt(-2.0 * np.log(u[:, 0]))
+    r1 = np.sqrt(-2.0 * np.log(u[:, 2]))
+    g = np.stack(
+        [r0 * np.cos(2 * np.pi * u[:, 1]), r0 * np.sin(2 * np.pi * u[:, 1]),
+         r1 * np.cos(2 * np.pi * u[:, 3]), r1 * np.sin(2 * np.pi * u[:, 3])], axis=1
+    ).reshape(-1)[:n]
+    return g.astype(np.float32).reshape(shape)
+
+
+# ---------------------------------------------------------------------------- HEALPix (RING)
+# Textbook HEALPix pixelisation (Gorski et al. 2005), scalar restatement of loc2pix, ring_above,
+# get_ring_info2 and get_interpol of the HEALPix C++ library (the algorithm jax_healpy follows).
+
+
+def healpix_ang2pix(nside, z, phi):
+    za = abs(z)
+    tt = np.fmod(phi * (2.0 / np.pi), 4.0)
+    if tt < 0:
+        tt += 4.0
+    ncap = 2 * nside * (nside - 1)
+    npix = 12 * nside * nside
+    if za <= 2.0 / 3.0:
+        nl4 = 4 * nside
+        t1 = nside * (0.5 + tt)
+        t2 = nside * z * 0.75
+        jp = int(t1 - t2)
+        jm = int(t1 + t2)
+        ir = nside + 1 + jp - jm
+        kshift = 1 - (ir & 1)
+        t = jp + jm - nside + kshift + 1 + nl4 + nl4
+        ip = (t >> 1) % nl4
+        return ncap + (ir - 1) * nl4 + ip
+    tp = tt - int(tt)
+    tmp = nside * np.sqrt(3.0 * (1.0 - za))
+    jp = int(tp * tmp)
+    jm = int((1.0 - tp) * tmp)
+    ir = jp + jm + 1
+    ip = int(tt * ir) % (4 * ir)
+    return 2 * ir * (ir - 1) + ip if z > 0 else npix - 2 * ir * (ir + 1) + ip
+
+
+def healpix_pix2ang(nside, pix):
+    """Pixel centre (theta, phi), RING scheme."""
+    ncap = 2 * nside * (nside - 1)
+    npix = 12 * nside * nside
+    if pix < ncap:
+        iring = int((1 + np.sqrt(1 + 2 * pix)) // 2)
+        iphi = pix - 2 * iring * (iring - 1)
+        z = 1.0 - iring * iring * 4.0 / npix
+        phi = (iphi + 0.5) * np.pi / (2.0 * iring)
+    elif pix < npix - ncap:
+        ip = pix - ncap
+        iring = ip // (4 * nside) + nside
+        iphi = ip % (4 * nside)
+        fodd = 0.5 if ((iring + nside) & 1) else 1.0
+        z = (2 * nside - iring) * 2.0 / (3.0 * nside)
+        phi = (iphi + fodd) * np.pi / (2.0 * nside)
+    else:
+        ip = npix - pix
+        iring = int((1 + np.sqrt(2 * ip - 1)) // 2)
+        iphi = 4 * iring + 1 - (ip - 2 * iring * (iring - 1))
+        z = -1.0 + iring * iring * 4.0 / npix
+        phi = (iphi - 0.5) * np.pi / (2.0 * iring)
+    return np.arccos(z), phi
+
+
+def _ring_above(nside, z):
+    az = abs(z)
+    if az <= 2.0 / 3.0:
+        return int(nside * (2.0 - 1.5 * z))
+    iring = int(nside * np.sqrt(3.0 * (1.0 - az)))
+    return iring if z > 0 else 4 * nside - iring - 1
+
+
+def _ring_info(nside, ring):
+    npix = 12 * nside * nside
+    ncap = 2 * nside * (nside - 1)
+    fact2 = 4.0 / npix
+    fact1 = (2 * nside) * fact2
+    northring = 4 * nside - ring if ring > 2 * nside else ring
+    if northring < nside:
+        tmp = northring * northring * fact2
+        theta = np.arctan2(np.sqrt(tmp * (2.0 - tmp)), 1.0 - tmp)
+        ringpix = 4 * northring
+        shifted = True
+        startpix = 2 * northring * (northring - 1)
+    else:
+        theta = np.arccos((2 * nside - northring) * fact1)
+        ringpix = 4 * nside
+        shifted = ((northring - nside) & 1) == 0
+        startpix = ncap + (northring - nside) * ringpix
+    if northring != ring:
+        theta = np.pi - theta
+        startpix = npix - startpix - ringpix
+    return startpix, ringpix, theta, shifted
+
+
+def healpix_interp_weights(nside, theta, phi):
+    """Four RING pixels and bilinear weights (HEALPix get_interpol)."""
+    npix = 12 * nside * nside
+    z = np.cos(theta)
+    ir1 = _ring_above(nside, z)
+    ir2 = ir1 + 1
+    pix = [0, 0, 0, 0]
+    wgt = [0.0, 0.0, 0.0, 0.0]
+    theta1 = theta2 = 0.0
+    for slot, ir in ((0, ir1), (2, ir2)):
+        if (slot == 0 and ir > 0) or (slot == 2 and ir < 4 * nside):
+            sp, nr, th, shift = _ring_info(nside, ir)
+            if slot == 0:
+                theta1 = th
+            else:
+                theta2 = th
+            dphi = 2.0 * np.pi / nr
+            tmp = phi / dphi - 0.5 * shift
+            i1 = int(tmp) - 1 if tmp < 0 else int(tmp)
+            w1 = (phi - (i1 + 0.5 * shift) * dphi) / dphi
+            i2 = i1 + 1
+            if i1 < 0:
+                i1 += nr
+            if i2 >= nr:
+                i2 -= nr
+            pix[slot], pix[slot + 1] = sp + i1, sp + i2
+            wgt[slot], wgt[slot + 1] = 1 - w1, w1
+    if ir1 == 0:
+        wtheta = theta / theta2
+        wgt[2] *= wtheta
+        wgt[3] *= wtheta
+        fac = (1 - wtheta) * 0.25
+        wgt[0] = fac
+        wgt[1] = fac
+        wgt[2] += fac
+        wgt[3] += fac
+        pix[0] = (pix[2] + 2) & 3
+        pix[1] = (pix[3] + 2) & 3
+    elif ir2 == 4 * nside:
+        wtheta = (theta - theta1) / (np.pi - theta1)
+        wgt[0] *= 1 - wtheta
+        wgt[1] *= 1 - wtheta
+        fac = wtheta * 0.25
+        wgt[0] += fac
+        wgt[1] += fac
+        wgt[2] = fac
+        wgt[3] = fac
+        pix[2] = ((pix[0] + 2) & 3) + npix - 4
+        pix[3] = ((pix[1] + 2) & 3) + npix - 4
+    else:
+        wtheta = (theta - theta1) / (theta2 - theta1)
+        wgt[0] *= 1 - wtheta
+        wgt[1] *= 1 - wtheta
+        wgt[2] *= wtheta
+        wgt[3] *= wtheta
+    return pix, wgt
+
+
+def paint_healpix(pos, mesh_shape, box, observer, rmin, rmax, nside, scheme="bilinear", mode="relative", weight=1.0):
+    """Shell paint (build-defined normalisation: weighted count / (pixel solid angle x shell
+    volume per steradian)); positions wrapped into the periodic box; float64 geometry."""
+    pos = np.asarray(pos, dtype=np.float32)
+    np_shape = pos.shape[:3]
+    g = pos.reshape(-1, 3).astype(np.float64)
+    if mode == "relative":
+        g = g + _lattice(np_shape, mesh_shape).astype(np.float64)
+    ms = np.asarray(mesh_shape, dtype=np.float64)
+    g = g - np.floor(g / ms) * ms
+    rel = g * (np.asarray(box, dtype=np.float64) / ms) - np.asarray(observer, dtype=np.float64)
+    r = np.sqrt((rel**2).sum(axis=1))
+    sel = np.nonzero((r > rmin) & (r <= rmax))[0]
+    wt = np.broadcast_to(np.asarray(weight, dtype=np.float64).reshape(-1), (g.shape[0],))
+    npix = 12 * nside * nside
+    acc = np.zeros(npix)
+    for p in sel:
+        X, Y, Z = rel[p]
+        phi = np.arctan2(Y, X)
+        if phi < 0:
+            phi += 2 * np.pi
+        if scheme == "ngp":
+            acc[healpix_ang2pix(nside, Z / r[p], phi)] += wt[p]
+        else:
+            theta = np.arctan2(np.sqrt(X * X + Y * Y), Z)
+            pix, w = healpix_interp_weights(nside, theta, phi)
+            for q in range(4):
+                acc[pix[q]] += wt[p] * w[q]
+    vol = (4 * np.pi / npix) * (rmax**3 - rmin**3) / 3.0
+    return (acc / vol).astype(np.float32)
