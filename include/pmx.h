@@ -29,6 +29,8 @@ typedef struct pmx_plan_t* pmx_plan;
 #define PMX_DECONV 1u      /* divide the force spectrum by the CIC window of paint and readout */
 #define PMX_DRIFT_PRE 2u   /* pmx_step: x += v * c_drift_pre before painting */
 #define PMX_DRIFT_POST 4u  /* pmx_step: x += v * c_drift_post after the kick */
+#define PMX_PAINT_ATOMIC 8u /* paint relative-mode particles with the one-pass atomic scatter instead of
+                               the owner-computes tiles (plan flag or per-call flag) */
 
 int pmx_abi_version(void);
 const char* pmx_last_error(void);
@@ -38,6 +40,9 @@ int pmx_fft_supported(int n);
 int pmx_plan_create(pmx_plan* plan, int nx, int ny, int nz, unsigned flags);
 int pmx_plan_destroy(pmx_plan plan);
 size_t pmx_plan_workspace_bytes(pmx_plan plan);
+/* Particles the last tiled paint on this plan sent to the atomic escape pass (their displacement
+ * exceeded the tile halo). Synchronises the device. */
+int pmx_plan_escapes(pmx_plan plan, uint64_t* count);
 /* Physical box (Mpc/h) for the k tables of pmx_power and pmx_color_field (synchronous upload). */
 int pmx_plan_set_box(pmx_plan plan, const double box[3]);
 

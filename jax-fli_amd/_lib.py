@@ -32,6 +32,7 @@ SIGNATURES = {
     "pmx_plan_destroy": (_i, [_vp]),
     "pmx_plan_workspace_bytes": (_c.c_size_t, [_vp]),
     "pmx_plan_set_box": (_i, [_vp, _dp]),
+    "pmx_plan_escapes": (_i, [_vp, _c.POINTER(_c.c_uint64)]),
     "pmx_paint_cic": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _vp, _i, _vp]),
     "pmx_readout_cic": (_i, [_vp, _vp, _i, _vp, _i, _i, _i, _i, _vp, _vp]),
     "pmx_fft_r2c": (_i, [_vp, _vp, _vp, _vp]),
@@ -51,6 +52,7 @@ SIGNATURES = {
 DECONV = 1
 DRIFT_PRE = 2
 DRIFT_POST = 4
+PAINT_ATOMIC = 8
 
 
 class PmxError(RuntimeError):
@@ -105,15 +107,22 @@ def dbl3(v) -> ctypes.Array:
 class Plan:
     """Owns one pmx plan (FFT tables and work buffers) for a mesh shape on one device."""
 
-    def __init__(self, mesh_shape, device: torch.device | None = None):
+    def __init__(self, mesh_shape, device: torch.device | None = None, flags: int = 0):
         lib = load()
         self.mesh_shape = tuple(int(n) for n in mesh_shape)
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         handle = ctypes.c_void_p()
         with torch.cuda.device(self.device):
-            check(lib.pmx_plan_create(ctypes.byref(handle), *self.mesh_shape, 0), "pmx_plan_create")
+            check(lib.pmx_plan_create(ctypes.byref(handle), *self.mesh_shape, int(flags)), "pmx_plan_create")
         self.handle = handle
         self._box = None
+
+    def escapes(self) -> int:
+        """Particles the last tiled paint sent to the atomic escape pass (synchronises)."""
+        c = ctypes.c_uint64()
+        with torch.cuda.device(self.device):
+            check(load().pmx_plan_escapes(self.handle, ctypes.byref(c)), "pmx_plan_escapes")
+        return int(c.value)
 
     @property
     def workspace_bytes(self) -> int:

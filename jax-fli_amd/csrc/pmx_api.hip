@@ -208,7 +208,7 @@ int pmx_plan_destroy(pmx_plan p) {
   if (!p) return 0;
   void* bufs[] = {p->tw_x, p->tw_y, p->tw_z, p->tw_zh, p->grad_x, p->grad_y, p->grad_z, p->w2_x, p->w2_y, p->w2_z,
                   p->dec_x, p->dec_y, p->dec_z, p->rho, p->specA, p->specB, p->specC, p->specD, p->fmesh, p->acc,
-                  p->kphys_x, p->kphys_y, p->kphys_z};
+                  p->kphys_x, p->kphys_y, p->kphys_z, p->esc_count, p->esc_list};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete p;
@@ -216,6 +216,18 @@ int pmx_plan_destroy(pmx_plan p) {
 }
 
 size_t pmx_plan_workspace_bytes(pmx_plan p) { return p ? p->workspace_bytes : 0; }
+
+int pmx_plan_escapes(pmx_plan p, uint64_t* count) {
+  PMX_TRY(check_plan(p));
+  if (!count) return pmx_set_error(1, "null output");
+  *count = 0;
+  if (!p->esc_count || !p->esc_ntiles_last) return 0;
+  std::vector<unsigned> c(p->esc_ntiles_last);
+  PMX_HIP(hipDeviceSynchronize());
+  PMX_HIP(hipMemcpy(c.data(), p->esc_count, c.size() * sizeof(unsigned), hipMemcpyDeviceToHost));
+  for (unsigned v : c) *count += v;
+  return 0;
+}
 
 int pmx_plan_set_box(pmx_plan p, const double box[3]) {
   PMX_TRY(check_plan(p));
@@ -243,9 +255,8 @@ int pmx_paint_cic(pmx_plan p, const float* pos, int npx, int npy, int npz, int m
   PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
   if (!mesh) return pmx_set_error(1, "null mesh");
   hipStream_t s = (hipStream_t)stream;
-  if (!accumulate) PMX_TRY(pmx_launch_fill(mesh, p->ncell, 0.f, s));
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
-  return pmx_launch_paint(p, a, weights, weight, 0.f, mesh, s);
+  return pmx_launch_paint(p, a, weights, weight, 0.f, mesh, accumulate, (p->flags & PMX_PAINT_ATOMIC) != 0, s);
 }
 
 int pmx_readout_cic(pmx_plan p, const float* mesh, int ncomp, const float* pos, int npx, int npy, int npz, int mode,
@@ -291,8 +302,7 @@ int pmx_forces(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode
   if (!forces) return pmx_set_error(1, "null forces");
   hipStream_t s = (hipStream_t)stream;
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
-  PMX_TRY(pmx_launch_fill(p->rho, p->ncell, 0.f, s));
-  PMX_TRY(pmx_launch_paint(p, a, nullptr, 1.f, 0.f, p->rho, s));
+  PMX_TRY(pmx_launch_paint(p, a, nullptr, 1.f, 0.f, p->rho, 0, ((p->flags | flags) & PMX_PAINT_ATOMIC) != 0, s));
   PMX_TRY(force_mesh_from(p, p->rho, scale, flags, p->fmesh, s));
   return pmx_launch_readout(p, a, p->fmesh, 3, forces, 0.f, 0.f, 1, s);
 }
@@ -304,8 +314,7 @@ int pmx_step(pmx_plan p, float* x, float* v, int npx, int npy, int npz, int mode
   if (!v) return pmx_set_error(1, "null velocities");
   hipStream_t s = (hipStream_t)stream;
   PartArgs a = part_args(x, (flags & PMX_DRIFT_PRE) ? v : nullptr, npx, npy, npz, mode, p);
-  PMX_TRY(pmx_launch_fill(p->rho, p->ncell, 0.f, s));
-  PMX_TRY(pmx_launch_paint(p, a, nullptr, 1.f, c_drift_pre, p->rho, s));
+  PMX_TRY(pmx_launch_paint(p, a, nullptr, 1.f, c_drift_pre, p->rho, 0, ((p->flags | flags) & PMX_PAINT_ATOMIC) != 0, s));
   PMX_TRY(force_mesh_from(p, p->rho, force_scale, flags, p->fmesh, s));
   a.v = v;
   const int what = 2 | ((flags & PMX_DRIFT_POST) ? 4 : 0);

@@ -33,6 +33,9 @@ struct pmx_plan_t {
   float* fmesh;     // force mesh [x][y][z][3]
   double *kphys_x, *kphys_y, *kphys_z;  // estimator wavevectors (h/Mpc) per bin, set by pmx_plan_set_box
   bool box_set;
+  unsigned* esc_count;  // paint: per tile, particles displaced beyond the tile halo
+  unsigned* esc_list;   // paint: their indices, one segment per tile (esc_cap entries in total)
+  size_t esc_cap, esc_tiles, esc_ntiles_last;
   double* acc;      // double accumulators (healpix maps, flat maps); grown on demand
   size_t acc_bytes;
   size_t workspace_bytes;
@@ -80,7 +83,11 @@ struct PartArgs {
   int mode;            // 0 relative (lattice + dx), 1 absolute
   float sx, sy, sz;    // lattice spacing in cells (mesh / particle grid)
 };
+// tiled paint: halo (cells) within which a particle's floor(displacement) must lie to be painted by
+// the owner-computes tiles; farther particles take the atomic escape pass
+constexpr int PAINT_HALO = 4;
 int pmx_launch_paint(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, float cd_pre,
-                     float* mesh, hipStream_t s);
+                     float* mesh, int accumulate, int atomic, hipStream_t s);
+int pmx_launch_fill(float* a, size_t n, float v, hipStream_t s);
 int pmx_launch_readout(pmx_plan_t* p, const PartArgs& a, const float* mesh, int ncomp, float* out,
                        float ck, float cd_post, int what, hipStream_t s);

@@ -82,6 +82,22 @@ __device__ __forceinline__ float3 axpy3(float3 x, float3 v, float c) {  // x + v
   return make_float3(x.x + v.x * c, x.y + v.y * c, x.z + v.z * c);
 }
 
+// Contiguous particle chunk of this workgroup. Hardware dispatches workgroup b to XCD b % 8; the
+// logical id remaps so each XCD walks one contiguous slab of the Lagrangian grid and the mesh
+// planes it gathers from / scatters to stay in that XCD's L2.
+struct Chunk {
+  size_t begin, end;
+};
+__device__ __forceinline__ unsigned xcd_logical(unsigned b, unsigned G) {
+  return (G % 8u == 0u) ? (b % 8u) * (G / 8u) + b / 8u : b;
+}
+__device__ __forceinline__ Chunk my_chunk(size_t n) {
+  const unsigned G = gridDim.x;
+  const size_t per = (n + G - 1) / G;
+  const size_t beg = min(n, (size_t)xcd_logical(blockIdx.x, G) * per);
+  return Chunk{beg, min(n, beg + per)};
+}
+
 __global__ void k_fill(float* __restrict__ a, size_t n, float v) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
 }
@@ -94,7 +110,8 @@ __global__ void __launch_bounds__(256) k_paint(PartArgs a, const float* __restri
                                                float* __restrict__ mesh, int NX, int NY, int NZ, int rx, int ry,
                                                int rz) {
   const size_t np = (size_t)a.npx * a.npy * a.npz;
-  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < np; p += (size_t)gridDim.x * blockDim.x) {
+  const Chunk ch = my_chunk(np);
+  for (size_t p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x) {
     float3 x = ld3(a.x, p);
     if (a.v != nullptr) {
       x = axpy3(x, ld3(a.v, p), cd);
@@ -120,6 +137,181 @@ __global__ void __launch_bounds__(256) k_paint(PartArgs a, const float* __restri
   }
 }
 
+// Owner-computes CIC paint for relative mode (particles on their Lagrangian lattice): one
+// workgroup per mesh tile (BX, BY, BZ) accumulates in LDS every deposit that lands in its tile from
+// the particles whose lattice lies within the tile grown by H + 1 cells, then writes the tile with
+// plain stores -- every mesh cell is written exactly once, no global atomics and no zero fill.
+// A particle with |floor(dx)| > H along any axis ("escape") is skipped by all tiles; its home tile
+// records it in its own segment of the escape list (LDS counter, no global atomics) and
+// k_paint_escapes deposits those with atomics afterwards.
+constexpr int PT_MAX = 16 * 16 * 32;
+constexpr int PT_UNROLL = 4;
+
+__device__ __forceinline__ int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
+__device__ __forceinline__ int wrap_near(int i, int n) {
+  if (i < 0) i += n;
+  else if (i >= n) i -= n;
+  if ((unsigned)i >= (unsigned)n) {
+    i %= n;
+    i += i < 0 ? n : 0;
+  }
+  return i;
+}
+
+__global__ void __launch_bounds__(256) k_paint_tiles(PartArgs a, const float* __restrict__ w, float w0,
+                                                     float* __restrict__ mesh, int NX, int NY, int NZ, int rx,
+                                                     int ry, int rz, int BX, int BY, int BZ, int H, int accumulate,
+                                                     unsigned* __restrict__ esc_count, unsigned* __restrict__ esc_list,
+                                                     int esc_cap) {
+  __shared__ float tile[PT_MAX];
+  __shared__ unsigned s_nesc;
+  const int ntx = NX / BX, nty = NY / BY, ntz = NZ / BZ;
+  const unsigned L = xcd_logical(blockIdx.x, gridDim.x);
+  const int tz = L % ntz, ty = (L / ntz) % nty, tx = L / (ntz * nty);
+  const int T0x = tx * BX, T0y = ty * BY, T0z = tz * BZ;
+  const int ncell = BX * BY * BZ;
+  for (int i = threadIdx.x; i < ncell; i += blockDim.x) tile[i] = 0.f;
+  if (threadIdx.x == 0) s_nesc = 0u;
+  __syncthreads();
+  // lattice i*r in [T0 - 1 - H, T0 + B - 1 + H] (unwrapped)
+  const int ilo = -floordiv(-(T0x - 1 - H), rx), ihi = floordiv(T0x + BX - 1 + H, rx);
+  const int jlo = -floordiv(-(T0y - 1 - H), ry), jhi = floordiv(T0y + BY - 1 + H, ry);
+  const int klo = -floordiv(-(T0z - 1 - H), rz), khi = floordiv(T0z + BZ - 1 + H, rz);
+  const int ci = ihi - ilo + 1, cj = jhi - jlo + 1, ck = khi - klo + 1;
+  const float fH = (float)H;
+  // division-free walk over (ii, jj, kk), kk fastest, stride blockDim.x
+  const int stride = blockDim.x;
+  const int dk = stride % ck, drow = stride / ck;
+  const int dj = drow % cj, di = drow / cj;
+  int kk = threadIdx.x % ck;
+  int jj = (threadIdx.x / ck) % cj;
+  int ii = threadIdx.x / (ck * cj);
+  unsigned* my_esc = esc_list + (size_t)L * esc_cap;
+  while (ii < ci) {
+    size_t pidx[PT_UNROLL];
+    int li[PT_UNROLL], lj[PT_UNROLL], lk[PT_UNROLL];
+    bool ok[PT_UNROLL];
+#pragma unroll
+    for (int u = 0; u < PT_UNROLL; ++u) {
+      ok[u] = ii < ci;
+      li[u] = ilo + ii;
+      lj[u] = jlo + jj;
+      lk[u] = klo + kk;
+      pidx[u] = ((size_t)wrap_near(li[u], a.npx) * a.npy + wrap_near(lj[u], a.npy)) * a.npz + wrap_near(lk[u], a.npz);
+      kk += dk;
+      int carry = 0;
+      if (kk >= ck) { kk -= ck; carry = 1; }
+      jj += dj + carry;
+      carry = 0;
+      if (jj >= cj) { jj -= cj; carry = 1; }
+      ii += di + carry;
+    }
+    float3 xv[PT_UNROLL];
+#pragma unroll
+    for (int u = 0; u < PT_UNROLL; ++u) xv[u] = ok[u] ? ld3(a.x, pidx[u]) : make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < PT_UNROLL; ++u) {
+      if (!ok[u]) continue;
+      const float3 x = xv[u];
+      const float fx = floorf(x.x), fy = floorf(x.y), fz = floorf(x.z);
+      const bool esc = !(fx >= -fH && fx <= fH && fy >= -fH && fy <= fH && fz >= -fH && fz <= fH);
+      if (esc) {
+        const bool home = (li[u] * rx >= T0x && li[u] * rx < T0x + BX && lj[u] * ry >= T0y && lj[u] * ry < T0y + BY &&
+                           lk[u] * rz >= T0z && lk[u] * rz < T0z + BZ);
+        if (home) my_esc[atomicAdd(&s_nesc, 1u)] = (unsigned)pidx[u];
+        continue;
+      }
+      const float wt = w ? w[pidx[u]] : w0;
+      const float frx = x.x - fx, fry = x.y - fy, frz = x.z - fz;
+      const float wx[2] = {1.0f - fabsf(frx), 1.0f - fabsf(frx - 1.0f)};
+      const float wy[2] = {1.0f - fabsf(fry), 1.0f - fabsf(fry - 1.0f)};
+      const float wz[2] = {1.0f - fabsf(frz), 1.0f - fabsf(frz - 1.0f)};
+      const int bx = li[u] * rx + (int)fx - T0x, by = lj[u] * ry + (int)fy - T0y, bz = lk[u] * rz + (int)fz - T0z;
+#pragma unroll
+      for (int cx = 0; cx < 2; ++cx) {
+        const int ix = bx + cx;
+        if (ix < 0 || ix >= BX) continue;
+#pragma unroll
+        for (int cy = 0; cy < 2; ++cy) {
+          const int iy = by + cy;
+          if (iy < 0 || iy >= BY) continue;
+          const float wxy = wx[cx] * wy[cy];
+#pragma unroll
+          for (int cz = 0; cz < 2; ++cz) {
+            const int iz = bz + cz;
+            if (iz < 0 || iz >= BZ) continue;
+            atomicAdd(&tile[(ix * BY + iy) * BZ + iz], (wxy * wz[cz]) * wt);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) esc_count[L] = s_nesc;
+  if (accumulate) {
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+      const int iz = c % BZ, iy = (c / BZ) % BY, ix = c / (BZ * BY);
+      float* dst = mesh + ((size_t)(T0x + ix) * NY + (T0y + iy)) * NZ + (T0z + iz);
+      *dst = *dst + tile[c];
+    }
+  } else {
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+      const int iz = c % BZ, iy = (c / BZ) % BY, ix = c / (BZ * BY);
+      mesh[((size_t)(T0x + ix) * NY + (T0y + iy)) * NZ + (T0z + iz)] = tile[c];
+    }
+  }
+}
+
+// escapes of tile blockIdx.x: segment [t * esc_cap, t * esc_cap + esc_count[t])
+__global__ void __launch_bounds__(256) k_paint_escapes(PartArgs a, const float* __restrict__ w, float w0,
+                                                       float* __restrict__ mesh, int NX, int NY, int NZ, int rx, int ry,
+                                                       int rz, const unsigned* __restrict__ esc_count,
+                                                       const unsigned* __restrict__ esc_list, int esc_cap) {
+  const unsigned n = esc_count[blockIdx.x];
+  const unsigned* seg = esc_list + (size_t)blockIdx.x * esc_cap;
+  for (unsigned e = threadIdx.x; e < n; e += blockDim.x) {
+    const size_t p = seg[e];
+    const float3 x = ld3(a.x, p);
+    const float wt = w ? w[p] : w0;
+    const Cic q = make_cic(a, p, x, NX, NY, NZ, rx, ry, rz);
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int cx = o & 1, cy = (o >> 1) & 1, cz = (o >> 2) & 1;
+      const float wv = (((cx ? q.c[0].w1 : q.c[0].w0) * (cy ? q.c[1].w1 : q.c[1].w0)) * (cz ? q.c[2].w1 : q.c[2].w0)) * wt;
+      const size_t idx =
+          (((size_t)(cx ? q.c[0].i1 : q.c[0].i0) * NY + (cy ? q.c[1].i1 : q.c[1].i0)) * NZ + (cz ? q.c[2].i1 : q.c[2].i0));
+      unsafeAtomicAdd(mesh + idx, wv);
+    }
+  }
+}
+
+// x <- x + v*c
+__global__ void __launch_bounds__(256) k_drift(float* __restrict__ x, const float* __restrict__ v, size_t n, float c) {
+  const Chunk ch = my_chunk(n);
+  for (size_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) x[i] = x[i] + v[i] * c;
+}
+
+__global__ void __launch_bounds__(256) k_paint_escapes(PartArgs a, const float* __restrict__ w, float w0,
+                                                       float* __restrict__ mesh, int NX, int NY, int NZ, int rx, int ry,
+                                                       int rz, const unsigned* __restrict__ esc_count,
+                                                       const unsigned* __restrict__ esc_list) {
+  const unsigned n = *esc_count;
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const size_t p = esc_list[e];
+    const float3 x = ld3(a.x, p);
+    const float wt = w ? w[p] : w0;
+    const Cic q = make_cic(a, p, x, NX, NY, NZ, rx, ry, rz);
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int cx = o & 1, cy = (o >> 1) & 1, cz = (o >> 2) & 1;
+      const float wv = (((cx ? q.c[0].w1 : q.c[0].w0) * (cy ? q.c[1].w1 : q.c[1].w0)) * (cz ? q.c[2].w1 : q.c[2].w0)) * wt;
+      const size_t idx =
+          (((size_t)(cx ? q.c[0].i1 : q.c[0].i0) * NY + (cy ? q.c[1].i1 : q.c[1].i0)) * NZ + (cz ? q.c[2].i1 : q.c[2].i0));
+      unsafeAtomicAdd(mesh + idx, wv);
+    }
+  }
+}
+
 // CIC readout of an NC-component interleaved mesh. what: bit0 store F in out, bit1 kick
 // v <- v + F*ck (NC == 3), bit2 drift x <- x + v*cd after the kick. Corners are summed
 // sequentially in the order x-fastest (0,0,0),(1,0,0),(0,1,0),(1,1,0),(0,0,1),...
@@ -128,7 +320,8 @@ __global__ void __launch_bounds__(256) k_readout(PartArgs a, const float* __rest
                                                  float ck, float cd, int what, int NX, int NY, int NZ, int rx,
                                                  int ry, int rz) {
   const size_t np = (size_t)a.npx * a.npy * a.npz;
-  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < np; p += (size_t)gridDim.x * blockDim.x) {
+  const Chunk ch = my_chunk(np);
+  for (size_t p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x) {
     float3 x = ld3(a.x, p);
     const Cic q = make_cic(a, p, x, NX, NY, NZ, rx, ry, rz);
     float f[NC];
@@ -318,7 +511,8 @@ __global__ void __launch_bounds__(256) k_paint_healpix(PartArgs a, const float* 
                                                        HpGeom g, double* __restrict__ acc, int NX, int NY, int NZ,
                                                        int rx, int ry, int rz) {
   const size_t np = (size_t)a.npx * a.npy * a.npz;
-  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < np; p += (size_t)gridDim.x * blockDim.x) {
+  const Chunk ch = my_chunk(np);
+  for (size_t p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x) {
     const float3 x = ld3(a.x, p);
     const double gx = grid_pos(a, p, 0, x.x, NX, rx), gy = grid_pos(a, p, 1, x.y, NY, ry),
                  gz = grid_pos(a, p, 2, x.z, NZ, rz);
@@ -362,7 +556,8 @@ struct FlatArgs {
 __global__ void __launch_bounds__(256) k_paint_flat(PartArgs a, const float* __restrict__ w, float w0, FlatArgs fa,
                                                     double* __restrict__ acc, int rx, int ry, int rz) {
   const size_t np = (size_t)a.npx * a.npy * a.npz;
-  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < np; p += (size_t)gridDim.x * blockDim.x) {
+  const Chunk ch = my_chunk(np);
+  for (size_t p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x) {
     float3 x = ld3(a.x, p);
     if (a.mode == 0) {
       const int k = (int)(p % a.npz);
@@ -447,6 +642,13 @@ inline int grid_for(size_t n, int block) {
   const size_t gmax = 1u << 20;
   return (int)(g < 1 ? 1 : (g > gmax ? gmax : g));
 }
+// particle-chunk grids: a multiple of 8 workgroups (one slab per XCD), ~1-4 particles per thread
+inline int grid_particles(size_t n) {
+  size_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  g = (g + 7) / 8 * 8;
+  return (int)(g < 8 ? 8 : g);
+}
 
 }  // namespace
 
@@ -466,14 +668,55 @@ static void ratios(pmx_plan_t* p, const PartArgs& a, int& rx, int& ry, int& rz) 
   rz = p->nz / a.npz;
 }
 
-int pmx_launch_paint(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, float cd_pre, float* mesh,
-                     hipStream_t s) {
+// Paint into `mesh`. Relative mode uses the owner-computes tiled kernel (+ escape pass) unless
+// `atomic` is set; absolute mode always scatters with atomics. accumulate=0 overwrites the mesh.
+// A pre-drift (a.v != null) runs as its own pass first (tiles read each particle several times).
+int pmx_launch_paint(pmx_plan_t* p, const PartArgs& a_in, const float* w, float w0, float cd_pre, float* mesh,
+                     int accumulate, int atomic, hipStream_t s) {
   int rx, ry, rz;
+  PartArgs a = a_in;
   ratios(p, a, rx, ry, rz);
   const size_t np = (size_t)a.npx * a.npy * a.npz;
-  hipLaunchKernelGGL(k_paint, dim3(grid_for(np, 256)), dim3(256), 0, s, a, w, w0, cd_pre, mesh, p->nx, p->ny, p->nz, rx,
-                     ry, rz);
-  return pmx_check_launch("paint");
+  if (a.v != nullptr) {
+    hipLaunchKernelGGL(k_drift, dim3(grid_particles(3 * np)), dim3(256), 0, s, a.x, a.v, 3 * np, cd_pre);
+    int rc = pmx_check_launch("drift");
+    if (rc) return rc;
+    a.v = nullptr;
+  }
+  if (a.mode != 0 || atomic) {
+    if (!accumulate) {
+      int rc = pmx_launch_fill(mesh, p->ncell, 0.f, s);
+      if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_paint, dim3(grid_particles(np)), dim3(256), 0, s, a, w, w0, 0.f, mesh, p->nx, p->ny, p->nz, rx,
+                       ry, rz);
+    return pmx_check_launch("paint");
+  }
+  const int BX = p->nx < 16 ? p->nx : 16, BY = p->ny < 16 ? p->ny : 16, BZ = p->nz < 32 ? p->nz : 32;
+  const int ntiles = (p->nx / BX) * (p->ny / BY) * (p->nz / BZ);
+  const int cap = ((BX + rx - 1) / rx) * ((BY + ry - 1) / ry) * ((BZ + rz - 1) / rz);
+  const size_t need = (size_t)ntiles * cap;
+  if (p->esc_cap < need || p->esc_tiles < (size_t)ntiles) {
+    if (p->esc_list) (void)hipFree(p->esc_list);
+    if (p->esc_count) (void)hipFree(p->esc_count);
+    p->esc_list = nullptr;
+    p->esc_count = nullptr;
+    p->esc_cap = 0;
+    p->esc_tiles = 0;
+    hipError_t e = hipMalloc((void**)&p->esc_list, need * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->esc_count, (size_t)ntiles * sizeof(unsigned));
+    if (e != hipSuccess) return pmx_set_error(4, std::string("escape list: ") + hipGetErrorString(e));
+    p->esc_cap = need;
+    p->esc_tiles = ntiles;
+  }
+  p->esc_ntiles_last = ntiles;
+  hipLaunchKernelGGL(k_paint_tiles, dim3(ntiles), dim3(256), 0, s, a, w, w0, mesh, p->nx, p->ny, p->nz, rx, ry, rz, BX,
+                     BY, BZ, PAINT_HALO, accumulate, p->esc_count, p->esc_list, cap);
+  int rc = pmx_check_launch("paint_tiles");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_paint_escapes, dim3(ntiles), dim3(256), 0, s, a, w, w0, mesh, p->nx, p->ny, p->nz, rx, ry, rz,
+                     p->esc_count, p->esc_list, cap);
+  return pmx_check_launch("paint_escapes");
 }
 
 int pmx_launch_readout(pmx_plan_t* p, const PartArgs& a, const float* mesh, int ncomp, float* out, float ck,
@@ -482,10 +725,10 @@ int pmx_launch_readout(pmx_plan_t* p, const PartArgs& a, const float* mesh, int 
   ratios(p, a, rx, ry, rz);
   const size_t np = (size_t)a.npx * a.npy * a.npz;
   if (ncomp == 3)
-    hipLaunchKernelGGL(k_readout<3>, dim3(grid_for(np, 256)), dim3(256), 0, s, a, mesh, out, ck, cd_post, what, p->nx,
+    hipLaunchKernelGGL(k_readout<3>, dim3(grid_particles(np)), dim3(256), 0, s, a, mesh, out, ck, cd_post, what, p->nx,
                        p->ny, p->nz, rx, ry, rz);
   else
-    hipLaunchKernelGGL(k_readout<1>, dim3(grid_for(np, 256)), dim3(256), 0, s, a, mesh, out, ck, cd_post, what & 1,
+    hipLaunchKernelGGL(k_readout<1>, dim3(grid_particles(np)), dim3(256), 0, s, a, mesh, out, ck, cd_post, what & 1,
                        p->nx, p->ny, p->nz, rx, ry, rz);
   return pmx_check_launch("readout");
 }
@@ -510,7 +753,7 @@ int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w
   g.fact2 = 4.0 / (double)g.npix;
   g.fact1 = (double)(2 * nside) * g.fact2;
   const size_t np = (size_t)a.npx * a.npy * a.npz;
-  hipLaunchKernelGGL(k_paint_healpix, dim3(grid_for(np, 256)), dim3(256), 0, s, a, w, w0, sa, g, acc, p->nx, p->ny,
+  hipLaunchKernelGGL(k_paint_healpix, dim3(grid_particles(np)), dim3(256), 0, s, a, w, w0, sa, g, acc, p->nx, p->ny,
                      p->nz, rx, ry, rz);
   return pmx_check_launch("paint_healpix");
 }
@@ -532,7 +775,7 @@ int pmx_launch_flat(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, 
   fa.zlo = zlo;
   fa.zhi = zhi;
   const size_t np = (size_t)a.npx * a.npy * a.npz;
-  hipLaunchKernelGGL(k_paint_flat, dim3(grid_for(np, 256)), dim3(256), 0, s, a, w, w0, fa, acc, rx, ry, rz);
+  hipLaunchKernelGGL(k_paint_flat, dim3(grid_particles(np)), dim3(256), 0, s, a, w, w0, fa, acc, rx, ry, rz);
   return pmx_check_launch("paint_flat");
 }
 

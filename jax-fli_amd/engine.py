@@ -98,18 +98,19 @@ def force_mesh(delta: torch.Tensor, scale=1.0, deconv=False) -> torch.Tensor:
     return out
 
 
-def forces(pos: torch.Tensor, mesh_shape, mode="relative", scale=1.0, deconv=False) -> torch.Tensor:
+def forces(pos: torch.Tensor, mesh_shape, mode="relative", scale=1.0, deconv=False, atomic_paint=False) -> torch.Tensor:
     pos = _f32(pos, "pos")
     npx, npy, npz = _pgrid(pos)
     plan = get_plan(mesh_shape, pos.device)
     out = torch.empty_like(pos)
-    check(load().pmx_forces(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], float(scale),
-                            _lib.DECONV if deconv else 0, ptr(out), stream_ptr()), "pmx_forces")
+    flags = (_lib.DECONV if deconv else 0) | (_lib.PAINT_ATOMIC if atomic_paint else 0)
+    check(load().pmx_forces(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], float(scale), flags, ptr(out),
+                            stream_ptr()), "pmx_forces")
     return out
 
 
 def step(x: torch.Tensor, v: torch.Tensor, mesh_shape, mode="relative", force_scale=1.0, c_kick=0.0,
-         c_drift_pre=None, c_drift_post=None, deconv=False) -> None:
+         c_drift_pre=None, c_drift_post=None, deconv=False, atomic_paint=False) -> None:
     """In place: [x += v*c_drift_pre]; v += F(x)*force_scale*c_kick; [x += v*c_drift_post]."""
     if not (x.is_contiguous() and v.is_contiguous()):
         raise _lib.PmxError("state tensors must be contiguous")
@@ -119,7 +120,7 @@ def step(x: torch.Tensor, v: torch.Tensor, mesh_shape, mode="relative", force_sc
     if v.shape != x.shape:
         raise ValueError("x and v must have the same shape")
     plan = get_plan(mesh_shape, x.device)
-    flags = (_lib.DECONV if deconv else 0)
+    flags = (_lib.DECONV if deconv else 0) | (_lib.PAINT_ATOMIC if atomic_paint else 0)
     flags |= _lib.DRIFT_PRE if c_drift_pre is not None else 0
     flags |= _lib.DRIFT_POST if c_drift_post is not None else 0
     check(load().pmx_step(plan.handle, ptr(x), ptr(v), npx, npy, npz, MODES[mode], float(force_scale),
