@@ -1,10 +1,20 @@
 """jax_fli_amd: an MI355X-native particle-mesh engine with the public API of jax-fli's PM path.
 
-Imported as ``jax_fli_amd`` (the directory is ``jax-fli_amd/``; see ``__graft_entry__.load_package``).
-The compute path is libpmx.so (hand-written HIP for gfx950, C ABI in include/pmx.h); PyTorch only
-provides device memory and streams.
+Imported as ``jax_fli_amd`` (the directory is ``jax-fli_amd/``; ``__graft_entry__.load_package``
+registers it). The compute path is libpmx.so (hand-written HIP for gfx950, C ABI in include/pmx.h);
+PyTorch only provides device memory and streams. Names follow ``import jax_fli as jfli``.
 """
 
-from . import _lib, cosmology, engine
+from . import _lib, cosmology, engine, fields, initial, pm, power, utils
+from .cosmology import Cosmology, Planck18
+from .fields import (AbstractField, ConvergenceUnit, DensityField, DensityUnit, FieldStatus, FlatDensity,
+                     ParticleField, PhysicalUnit, PositionUnit, SphericalDensity)
+from .initial import gaussian_initial_conditions, interpolate_initial_conditions
+from .painting import PaintingOptions
+from .pm import (AbstractCorrection, AbstractInterp, AbstractNBodySolver, DriftInterp, EfficientDriftDoubleKick,
+                 InterpTilerState, NBodyState, NoCorrection, NoInterp, OnionTiler, PGDKernel,
+                 ReversibleDoubleKickDrift, SharpeningKernel, TelephotoInterp, lpt, nbody)
+from .power import PowerSpectrum, power
+from .utils import centers, compute_lightcone_shells, distances, edges
 
 __version__ = "0.1.0"

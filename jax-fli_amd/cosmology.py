@@ -73,7 +73,8 @@ def _dlnE_dlna(cosmo: Cosmology, a):
 
 # ---------------------------------------------------------------------- linear growth
 _A_INIT = 1e-5
-_NSTEP = 2048
+_A_MAX = 2.0
+_NSTEP = 2048  # RK4 steps from _A_INIT to a = 1
 
 
 def _growth_rhs(cosmo: Cosmology, x, D, G):
@@ -90,15 +91,18 @@ def _growth_rhs(cosmo: Cosmology, x, D, G):
 
 @functools.lru_cache(maxsize=1024)
 def _growth_table(cosmo: Cosmology):
-    """RK4 in ln a from the matter-dominated growing mode D = a, normalised to D(1) = 1."""
-    x0, x1 = math.log(_A_INIT), 0.0
-    h = (x1 - x0) / _NSTEP
-    xs = x0 + h * np.arange(_NSTEP + 1)
-    Ds = [0.0] * (_NSTEP + 1)
-    Gs = [0.0] * (_NSTEP + 1)
+    """RK4 in ln a from the matter-dominated growing mode D = a, normalised to D(1) = 1. The grid
+    continues past a = 1 to a = 2: the KDK double kick evaluates Gf at the nominal next half step
+    sqrt(t1 (t1 + dt)), which exceeds 1 on the last step."""
+    x0 = math.log(_A_INIT)
+    h = -x0 / _NSTEP
+    n_tot = _NSTEP + int(math.ceil(math.log(_A_MAX) / h))
+    xs = x0 + h * np.arange(n_tot + 1)
+    Ds = [0.0] * (n_tot + 1)
+    Gs = [0.0] * (n_tot + 1)
     D = G = _A_INIT
     Ds[0], Gs[0] = D, G
-    for i in range(_NSTEP):
+    for i in range(n_tot):
         x = x0 + h * i
         k1d, k1g = _growth_rhs(cosmo, x, D, G)
         k2d, k2g = _growth_rhs(cosmo, x + 0.5 * h, D + 0.5 * h * k1d, G + 0.5 * h * k1g)
@@ -107,9 +111,10 @@ def _growth_table(cosmo: Cosmology):
         D += (h / 6.0) * (k1d + 2 * k2d + 2 * k3d + k4d)
         G += (h / 6.0) * (k1g + 2 * k2g + 2 * k3g + k4g)
         Ds[i + 1], Gs[i + 1] = D, G
-    Ds = np.array(Ds) / D
-    Gs = np.array(Gs) / D
-    dG = np.array([_growth_rhs(cosmo, xs[i], Ds[i], Gs[i])[1] for i in range(_NSTEP + 1)])
+    norm = Ds[_NSTEP]  # node _NSTEP is a = 1
+    Ds = np.array(Ds) / norm
+    Gs = np.array(Gs) / norm
+    dG = np.array([_growth_rhs(cosmo, xs[i], Ds[i], Gs[i])[1] for i in range(n_tot + 1)])
     return xs, Ds, Gs, dG
 
 

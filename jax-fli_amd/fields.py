@@ -1,0 +1,352 @@
+"""Field containers mirroring the reference's PyTrees (``AbstractField``, ``ParticleField``,
+``DensityField``, ``FlatDensity``, ``SphericalDensity``; src/jax_fli/_src/base/_core.py:175-246,
+src/jax_fli/fields/particles.py:33-442, fields/density.py:24-47, fields/lightcone.py:446-471).
+
+Same field names, enums, static metadata and methods; the ``array`` is a torch tensor on the GPU.
+Fields are immutable: every method returns a new field (``replace``), as in the reference.
+Painting and readout dispatch to the pmx kernels.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+from enum import Enum, auto
+from typing import Any
+
+import numpy as np
+import torch
+
+from . import engine
+
+
+class FieldStatus(str, Enum):
+    UNKNOWN = auto()
+    INITIAL_FIELD = auto()
+    LPT1 = auto()
+    LPT2 = auto()
+    DENSITY_FIELD = auto()
+    PARTICLES = auto()
+    PROJECTED_DENSITY = auto()
+    LIGHTCONE = auto()
+    KAPPA = auto()
+    GAMMA = auto()
+
+
+class PhysicalUnit(str, Enum):
+    INVALID_UNIT = auto()
+
+
+class PositionUnit(str, Enum):
+    GRID_RELATIVE = auto()
+    GRID_ABSOLUTE = auto()
+    MPC_H = auto()
+
+
+class DensityUnit(str, Enum):
+    OVERDENSITY = auto()
+    DENSITY = auto()
+    COUNTS = auto()
+    MSUN_H_PER_MPC3 = auto()
+
+
+class ConvergenceUnit(str, Enum):
+    DIMENSIONLESS = auto()
+    EFFECTIVE_DENSITY = auto()
+
+
+def uniform_particles(mesh_size, device=None) -> torch.Tensor:
+    """Lattice positions (X, Y, Z, 3) in grid units (jaxpm.distributed.uniform_particles)."""
+    axes = [torch.arange(n, dtype=torch.float32, device=device) for n in mesh_size]
+    return torch.stack(torch.meshgrid(*axes, indexing="ij"), dim=-1).contiguous()
+
+
+@dataclasses.dataclass(frozen=True, eq=False)
+class AbstractField:
+    array: Any
+    mesh_size: tuple
+    box_size: tuple
+    z_sources: Any = None
+    scale_factors: Any = None
+    comoving_centers: Any = None
+    density_width: Any = None
+    observer_position: tuple = (0.5, 0.5, 0.5)
+    sharding: Any = None
+    halo_size: tuple = (0, 0)
+    nside: int | None = None
+    flatsky_npix: tuple | None = None
+    field_size: float | None = None
+    status: FieldStatus = FieldStatus.UNKNOWN
+    unit: Any = PhysicalUnit.INVALID_UNIT
+
+    def __post_init__(self):
+        # validation of _core.py:201-229
+        ms, bs = self.mesh_size, self.box_size
+        if not (isinstance(ms, tuple) and len(ms) == 3):
+            raise ValueError(f"mesh_size must be a tuple of 3, got {ms}")
+        if not (isinstance(bs, tuple) and len(bs) == 3):
+            raise ValueError(f"box_size must be a tuple of 3, got {bs}")
+        op = self.observer_position
+        if not (isinstance(op, tuple) and len(op) == 3):
+            raise ValueError(f"observer_position must be a tuple of 3, got {op}")
+        if any(not 0.0 <= f <= 1.0 for f in op):
+            raise ValueError("observer_position entries must lie within [0, 1]")
+        if not (isinstance(self.halo_size, tuple) and len(self.halo_size) == 2) or any(h < 0 for h in self.halo_size):
+            raise ValueError("halo_size must be a tuple of 2 non-negative ints")
+        if self.nside is not None and self.nside <= 0:
+            raise ValueError(f"nside must be positive, got {self.nside}")
+        if self.flatsky_npix is not None and not (len(self.flatsky_npix) == 2 and all(x > 0 for x in self.flatsky_npix)):
+            raise ValueError(f"flatsky_npix must be 2 positive ints, got {self.flatsky_npix}")
+        if self.sharding is not None:
+            raise NotImplementedError("sharded fields are not supported on this single-GPU engine")
+
+    # ------------------------------------------------------------------ pytree-like helpers
+    def replace(self, **kw):
+        return dataclasses.replace(self, **kw)
+
+    @classmethod
+    def FromDensityMetadata(cls, array, field: "AbstractField", **kw):
+        meta = {f.name: getattr(field, f.name) for f in dataclasses.fields(AbstractField) if f.name != "array"}
+        for k in ("status", "unit"):
+            meta.pop(k)
+        meta.update(kw)
+        return cls(array=array, **meta)
+
+    @property
+    def observer_position_mpc(self) -> tuple:
+        return tuple(f * l for f, l in zip(self.observer_position, self.box_size))
+
+    @property
+    def max_comoving_radius(self) -> float:
+        box = np.asarray(self.box_size, dtype=np.float64)
+        f = np.clip(np.asarray(self.observer_position, dtype=np.float64), 0.0, 1.0)
+        return float(np.min(box / (1.0 + 2.0 * np.minimum(f, 1.0 - f))))
+
+    @property
+    def shape(self):
+        return tuple(self.array.shape)
+
+    def _base_ndim(self) -> int:
+        return 3
+
+    def is_batched(self) -> bool:
+        return self.array.ndim > self._base_ndim()
+
+    def __getitem__(self, key):
+        if not self.is_batched():
+            raise ValueError("indexing needs a batched field")
+        def pick(v):
+            if isinstance(v, (torch.Tensor, np.ndarray)) and np.ndim(v) >= 1 and v.shape[0] == self.array.shape[0]:
+                return v[key]
+            return v
+        return self.replace(array=self.array[key], scale_factors=pick(self.scale_factors),
+                            z_sources=pick(self.z_sources), comoving_centers=pick(self.comoving_centers),
+                            density_width=pick(self.density_width))
+
+    def apply_fn(self, fn):
+        return self.replace(array=fn(self.array))
+
+    # arithmetic on the array (_core.py:84-115)
+    def _binop(self, other, op):
+        o = other.array if isinstance(other, AbstractField) else other
+        return self.replace(array=op(self.array, o))
+
+    def __add__(self, o):
+        return self._binop(o, torch.add)
+
+    def __sub__(self, o):
+        return self._binop(o, torch.sub)
+
+    def __mul__(self, o):
+        return self._binop(o, torch.mul)
+
+    def __truediv__(self, o):
+        return self._binop(o, torch.div)
+
+    __radd__ = __add__
+    __rmul__ = __mul__
+
+    def __neg__(self):
+        return self.replace(array=-self.array)
+
+
+def _paint_mode(unit: PositionUnit) -> str:
+    if unit == PositionUnit.MPC_H:
+        raise ValueError("Cannot paint ParticleField with unit MPC_H; convert to GRID_RELATIVE or GRID_ABSOLUTE first.")
+    return "relative" if unit == PositionUnit.GRID_RELATIVE else "absolute"
+
+
+@dataclasses.dataclass(frozen=True, eq=False)
+class ParticleField(AbstractField):
+    """Particle displacements (GRID_RELATIVE), grid positions (GRID_ABSOLUTE) or Mpc/h positions."""
+
+    def __post_init__(self):
+        super().__post_init__()
+        s = tuple(self.array.shape)
+        if not (len(s) in (4, 5, 6) and s[-1] == 3):
+            raise ValueError(f"ParticleField array must have shape (X, Y, Z, 3), (S, X, Y, Z, 3), or (N, S, X, Y, Z, 3); got shape {s}")
+        if not isinstance(self.unit, PositionUnit):
+            raise TypeError(f"ParticleField.unit must be a PositionUnit, got {self.unit!r}")
+
+    def _base_ndim(self) -> int:
+        return 4
+
+    # -------------------------------------------------------------- unit conversion (units.py:78-129)
+    def to(self, unit: PositionUnit) -> "ParticleField":
+        if unit == self.unit:
+            return self
+        ms = torch.tensor(self.mesh_size, dtype=torch.float32, device=self.array.device)
+        bs = torch.tensor(self.box_size, dtype=torch.float32, device=self.array.device)
+        arr = self.array
+        if self.unit == PositionUnit.GRID_RELATIVE:
+            grid = arr + uniform_particles(arr.shape[-4:-1], arr.device)
+        elif self.unit == PositionUnit.MPC_H:
+            grid = arr / bs * ms
+        else:
+            grid = arr
+        if unit == PositionUnit.GRID_ABSOLUTE:
+            out = grid
+        elif unit == PositionUnit.GRID_RELATIVE:
+            out = grid - uniform_particles(arr.shape[-4:-1], arr.device)
+        else:
+            out = grid / ms * bs
+        return self.replace(array=out.contiguous(), unit=unit)
+
+    # -------------------------------------------------------------- 3D paint / readout
+    def paint(self, *, mesh=None, weights=1.0, chunk_size: int = 2**24, batch_size=None) -> "DensityField":
+        """CIC paint (particles.py:120-178); relative -> cic_paint_dx, absolute -> cic_paint."""
+        mode = _paint_mode(self.unit)
+        data = self.array
+        batched = data.ndim == 5
+        if data.ndim == 4:
+            data = data[None]
+        elif data.ndim != 5:
+            raise ValueError(f"paint() expects 4D or 5D array, got shape {tuple(data.shape)}")
+        outs = []
+        for b in range(data.shape[0]):
+            out = None if mesh is None else (mesh.array if isinstance(mesh, AbstractField) else mesh).clone()
+            outs.append(engine.paint(data[b].contiguous(), self.mesh_size, mode=mode, weights=weights, out=out,
+                                     accumulate=mesh is not None))
+        painted = torch.stack(outs) if batched else outs[0]
+        return DensityField(array=painted, mesh_size=self.mesh_size, box_size=self.box_size,
+                            observer_position=self.observer_position, nside=self.nside,
+                            flatsky_npix=self.flatsky_npix, halo_size=self.halo_size,
+                            status=FieldStatus.DENSITY_FIELD, scale_factors=self.scale_factors,
+                            unit=DensityUnit.DENSITY)
+
+    def read_out(self, density_mesh: "DensityField") -> "DensityField":
+        """CIC readout of a mesh at the particles (particles.py:183-229)."""
+        mode = _paint_mode(self.unit)
+        mesh = density_mesh.array if isinstance(density_mesh, AbstractField) else density_mesh
+        vals = engine.readout(mesh.contiguous(), self.array.contiguous(), mode=mode)
+        return DensityField(array=vals, mesh_size=self.mesh_size, box_size=self.box_size,
+                            observer_position=self.observer_position, nside=self.nside,
+                            flatsky_npix=self.flatsky_npix, halo_size=self.halo_size, status=self.status,
+                            scale_factors=self.scale_factors, unit=DensityUnit.DENSITY)
+
+    # -------------------------------------------------------------- flat-sky (particles.py:234-322)
+    def paint_2d(self, center, density_plane_width, *, weights=None, batch_size=None) -> "FlatDensity":
+        if self.flatsky_npix is None:
+            raise ValueError("ParticleField requires `flatsky_npix` for paint_2d.")
+        mode = _paint_mode(self.unit)
+        data = self.array
+        centers = np.atleast_1d(np.asarray(center, dtype=np.float64))
+        widths = np.atleast_1d(np.asarray(density_plane_width, dtype=np.float64))
+        if centers.shape != widths.shape:
+            raise ValueError("center and density_plane_width must have the same shape")
+        if data.ndim == 4:
+            if centers.size != 1:
+                raise ValueError("Painting with mutiple centers/widths requires batched input data ")
+            data = data[None]
+        elif data.ndim != 5 or data.shape[0] != centers.size:
+            raise ValueError("batched input needs one center/width per snapshot")
+        nx, ny, nz = self.mesh_size
+        Lx, Ly, Lz = self.box_size
+        fnx, fny = self.flatsky_npix
+        maps = []
+        for b in range(data.shape[0]):
+            c, wdt = centers[b], widths[b]
+            wg, cg = wdt * nz / Lz, c * nz / Lz
+            pixvol = (Lx / fnx) * (Ly / fny) * wdt
+            maps.append(engine.paint_flat(data[b].contiguous(), self.mesh_size, self.flatsky_npix,
+                                          float(np.float32(cg - wg / 2)), float(np.float32(cg + wg / 2)),
+                                          1.0 / pixvol, mode=mode, weights=1.0 if weights is None else weights))
+        arr = torch.stack(maps).squeeze(0) if len(maps) > 1 else maps[0]
+        return FlatDensity.FromDensityMetadata(array=arr, field=self, status=FieldStatus.LIGHTCONE,
+                                               comoving_centers=centers, density_width=widths,
+                                               unit=DensityUnit.DENSITY)
+
+    # -------------------------------------------------------------- HEALPix (particles.py:326-442)
+    def paint_spherical(self, center, density_plane_width, *, scheme="bilinear", weights=None,
+                        kernel_width_arcmin=None, smoothing_interpretation="fwhm", paint_nside=None,
+                        ud_grade_power=0.0, ud_grade_order_in="RING", ud_grade_order_out="RING",
+                        ud_grade_pess=False, batch_size=None) -> "SphericalDensity":
+        if self.nside is None:
+            raise ValueError("Spherical painting requires `nside`.")
+        if kernel_width_arcmin is not None or paint_nside not in (None, self.nside):
+            raise NotImplementedError("smoothing kernels and paint_nside resampling are not supported")
+        mode = _paint_mode(self.unit)
+        data = self.array
+        centers = np.atleast_1d(np.asarray(center, dtype=np.float64))
+        widths = np.atleast_1d(np.asarray(density_plane_width, dtype=np.float64))
+        if centers.shape != widths.shape:
+            raise ValueError("center and density_plane_width must have the same shape")
+        if data.ndim == 4:
+            if centers.size != 1:
+                raise ValueError("Painting with mutiple centers/widths requires batched input data ")
+            data = data[None]
+        elif data.ndim != 5 or data.shape[0] != centers.size:
+            raise ValueError("batched input needs one center/width per shell")
+        maps = []
+        for b in range(data.shape[0]):
+            c, wdt = centers[b], widths[b]
+            maps.append(engine.paint_healpix(data[b].contiguous(), self.mesh_size, self.box_size,
+                                             self.observer_position_mpc, c - wdt / 2, c + wdt / 2, self.nside,
+                                             scheme=scheme, mode=mode, weights=1.0 if weights is None else weights))
+        arr = torch.stack(maps) if len(maps) > 1 else maps[0]
+        return SphericalDensity.FromDensityMetadata(array=arr, field=self, status=FieldStatus.LIGHTCONE,
+                                                    comoving_centers=centers, density_width=widths,
+                                                    unit=DensityUnit.DENSITY)
+
+
+@dataclasses.dataclass(frozen=True, eq=False)
+class DensityField(AbstractField):
+    def __post_init__(self):
+        super().__post_init__()
+        s = tuple(self.array.shape)
+        if len(s) >= 3 and tuple(s[-3:]) != tuple(self.mesh_size) and self.status not in (
+                FieldStatus.LPT1, FieldStatus.LPT2, FieldStatus.PARTICLES):
+            raise ValueError(f"DensityField array shape {s} does not match mesh_size {self.mesh_size}")
+
+    def power(self, mesh2=None, *, kedges=None, multipoles=0, los=(0.0, 0.0, 1.0), batch_size=None):
+        from .power import PowerSpectrum, power
+
+        data = self.array
+        arrs = [data] if data.ndim == 3 else list(data)
+        ks, pks = [], []
+        for a in arrs:
+            k, pk = power(a, None if mesh2 is None else mesh2.array, box_shape=self.box_size, kedges=kedges,
+                          multipoles=multipoles, los=los)
+            ks.append(k)
+            pks.append(pk)
+        pk = pks[0] if data.ndim == 3 else np.stack(pks)
+        return PowerSpectrum(wavenumber=ks[0], spectra=pk, name="pk", scale_factors=self.scale_factors)
+
+
+@dataclasses.dataclass(frozen=True, eq=False)
+class FlatDensity(AbstractField):
+    def _base_ndim(self) -> int:
+        return 2
+
+
+@dataclasses.dataclass(frozen=True, eq=False)
+class SphericalDensity(AbstractField):
+    def __post_init__(self):
+        super().__post_init__()
+        if self.nside is None:
+            raise ValueError("SphericalDensity requires `nside`.")
+        s = tuple(self.array.shape)
+        if s != () and (len(s) not in (1, 2, 3) or s[-1] != 12 * self.nside**2):
+            raise ValueError(f"SphericalDensity array shape {s} does not match npix {12 * self.nside**2}")
+
+    def _base_ndim(self) -> int:
+        return 1

@@ -290,10 +290,10 @@ def healpix_pix2ang(nside, pix):
     elif pix < npix - ncap:
         ip = pix - ncap
         iring = ip // (4 * nside) + nside
-        iphi = ip % (4 * nside)
-        fodd = 0.5 if ((iring + nside) & 1) else 1.0
+        iphi = ip % (4 * nside) + 1
+        fodd = 1.0 if ((iring + nside) & 1) else 0.5
         z = (2 * nside - iring) * 2.0 / (3.0 * nside)
-        phi = (iphi + fodd) * np.pi / (2.0 * nside)
+        phi = (iphi - fodd) * np.pi / (2.0 * nside)
     else:
         ip = npix - pix
         iring = int((1 + np.sqrt(2 * ip - 1)) // 2)

@@ -1,0 +1,131 @@
+"""Host-side logic against the oracle's independent restatement (no GPU): background cosmology,
+the float32 time grid, the solvers' kick/drift factors, lightcone geometry (mirrors the
+reference's tests/test_equal_vol_shells.py invariants), fields validation."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import nbody_oracle as nbo
+
+PARAMS = [dict(Omega_c=0.25, Omega_b=0.0486, h=0.6774, n_s=0.9667, sigma8=0.8),
+          dict(Omega_c=0.30, Omega_b=0.05, h=0.7, w0=-0.9, wa=0.1),
+          dict(Omega_c=0.95, Omega_b=0.05)]
+
+
+@pytest.fixture(scope="module")
+def cosmo(pkg):
+    return pkg.cosmology
+
+
+@pytest.mark.parametrize("p", PARAMS)
+def test_growth_matches_oracle(cosmo, p):
+    c = cosmo.Cosmology(**p)
+    o = nbo.Cosmo(**p)
+    a = np.array([0.05, 0.1, 0.2317, 0.5, 0.77, 1.0])
+    assert np.allclose(cosmo.growth_factor(c, a), o.growth_factor(a), rtol=2e-9)
+    assert np.allclose(cosmo.growth_rate(c, a), o.growth_rate(a), rtol=2e-9)
+    assert np.allclose(cosmo.Gf(c, a), o.Gf(a), rtol=2e-9)
+    assert np.allclose(cosmo.gp(c, a), o.gp(a), rtol=2e-9)
+    assert np.allclose(cosmo.dGfa(c, a), o.dGfa(a), rtol=1e-6)
+    assert np.allclose(cosmo.E(c, a), o.E(a), rtol=1e-14)
+
+
+@pytest.mark.parametrize("p", PARAMS[:2])
+def test_distances_match_oracle(cosmo, p):
+    c = cosmo.Cosmology(**p)
+    o = nbo.Cosmo(**p)
+    a = np.array([0.3, 0.5, 0.8, 0.95, 1.0])
+    chi = cosmo.radial_comoving_distance(c, a)
+    assert np.allclose(chi, o.chi(a), rtol=1e-9, atol=1e-9)
+    assert np.allclose(cosmo.a_of_chi(c, chi[:-1]), a[:-1], rtol=1e-10)
+
+
+def test_power_spectrum_matches_oracle(cosmo):
+    p = PARAMS[0]
+    c = cosmo.Cosmology(**p)
+    o = nbo.Cosmo(**p)
+    k = np.logspace(-4, 1, 128)
+    assert np.allclose(cosmo.linear_matter_power(c, k), nbo.linear_matter_power(o, k), rtol=1e-6)
+
+
+@pytest.mark.parametrize("t0,ts,dt0,t1", [(0.1, [1.0], 0.09, 1.0), (0.1, [0.4, 0.55, 0.7, 0.9, 1.0], 0.05, 1.0),
+                                          (0.2, [1.0], 0.8 / 79, 1.0), (0.1, [0.95, 0.3], 0.05, 1.0)])
+def test_time_grid_matches_oracle(pkg, t0, ts, dt0, t1):
+    a = pkg.pm.time_grid(t0, ts, dt0, t1)
+    b = nbo.time_grid(t0, ts, dt0, t1)
+    assert len(a) == len(b)
+    for ea, eb in zip(a, b):
+        assert ea[0] == eb[0]
+        assert np.float32(ea[1]) == np.float32(eb[1]) and np.float32(ea[2]) == np.float32(eb[2])
+
+
+@pytest.mark.parametrize("p", PARAMS[:2])
+def test_solver_factors_match_oracle(pkg, cosmo, p):
+    c = cosmo.Cosmology(**p)
+    o = nbo.Cosmo(**p)
+    interp = pkg.NoInterp(painting=pkg.PaintingOptions(target="particles"))
+    kdk = pkg.EfficientDriftDoubleKick(interp_kernel=interp)
+    kkd = pkg.ReversibleDoubleKickDrift(interp_kernel=interp)
+    t0, dt0 = 0.1, 0.05
+    assert kdk.init_op(t0, t0 + dt0, dt0, c).kick == pytest.approx(nbo.kdk_init_coeff(o, t0, t0 + dt0), rel=1e-6)
+    for _, ta, tb in [e for e in nbo.time_grid(t0, [0.43, 1.0], dt0, 1.0) if e[0] == "step"]:
+        op = kdk.step_op(ta, tb, dt0, t0, c)
+        cd, ck = nbo.kdk_coeffs(o, ta, tb, dt0)
+        assert op.pre == pytest.approx(cd, rel=1e-6) and op.kick == pytest.approx(ck, rel=1e-6) and op.post is None
+        op = kkd.step_op(ta, tb, dt0, t0, c)
+        ck, cd = nbo.kkd_coeffs(o, ta, tb, dt0, t0)
+        assert op.kick == pytest.approx(ck, rel=1e-6) and op.post == pytest.approx(cd, rel=1e-6) and op.pre is None
+        r = kkd.reverse_op(ta, tb, dt0, t0, c)
+        assert r.pre == -op.post and r.kick == -op.kick
+
+
+# ---------------------------------------------------------------- lightcone geometry
+# mirrors tests/test_equal_vol_shells.py:37-150 of the reference
+@pytest.mark.parametrize("R,N,wmin", [(500.0, 10, 20.0), (1000.0, 8, 50.0), (2000.0, 20, 50.0)])
+def test_equal_vol_shell_invariants(pkg, R, N, wmin):
+    e = pkg.utils._compute_equal_vol_shells(R, N, wmin)
+    assert e.shape == (N + 1,) and e[0] == 0.0 and e[-1] == pytest.approx(R)
+    w = np.diff(e)
+    assert np.all(w > 0) and np.all(w >= wmin - 1e-9)
+    inner = w[: np.argmax(np.isclose(w, wmin))] if np.any(np.isclose(w, wmin)) else w
+    vols = np.diff(e[: len(inner) + 1] ** 3)
+    if len(vols) > 1:
+        assert np.allclose(vols, vols[0], rtol=1e-9)
+    assert np.sum(np.diff(e**3)) == pytest.approx(R**3)
+
+
+def test_equal_vol_rejects_impossible(pkg):
+    with pytest.raises(ValueError):
+        pkg.utils._compute_equal_vol_shells(100.0, 10, 20.0)
+
+
+def test_lightcone_shells_far_to_near(pkg, cosmo):
+    import torch
+
+    c = cosmo.Cosmology(Omega_c=0.25, Omega_b=0.0486)
+    f = pkg.DensityField(array=torch.zeros(8, 8, 8), mesh_size=(8, 8, 8), box_size=(1000.0,) * 3)
+    r, a = pkg.compute_lightcone_shells(c, f, nb_shells=5)
+    assert np.all(np.diff(r) < 0) and np.all(np.diff(a) > 0)
+    assert r[0] + (r[0] - r[1]) / 2 == pytest.approx(500.0)
+    w = pkg.distances(r, 0.0)
+    assert np.allclose(w, 100.0)
+    assert np.allclose(pkg.centers(w[::-1], 0.0), r[::-1])
+
+
+def test_field_validation(pkg):
+    import torch
+
+    with pytest.raises(ValueError):
+        pkg.ParticleField(array=torch.zeros(4, 4, 4), mesh_size=(4, 4, 4), box_size=(1.0,) * 3,
+                          unit=pkg.PositionUnit.GRID_RELATIVE)
+    with pytest.raises(ValueError):
+        pkg.DensityField(array=torch.zeros(4, 4, 4), mesh_size=(4, 4, 4), box_size=(1.0,) * 3,
+                         observer_position=(1.5, 0.5, 0.5))
+    with pytest.raises(TypeError):
+        pkg.ParticleField(array=torch.zeros(4, 4, 4, 3), mesh_size=(4, 4, 4), box_size=(1.0,) * 3)
+    f = pkg.DensityField(array=torch.zeros(4, 4, 4), mesh_size=(4, 4, 4), box_size=(100.0, 200.0, 300.0),
+                         observer_position=(0.5, 0.5, 0.0))
+    assert f.max_comoving_radius == pytest.approx(50.0)
+    assert f.observer_position_mpc == (50.0, 100.0, 0.0)
