@@ -1,0 +1,258 @@
+#!/usr/bin/env python3
+"""PM step throughput of the fused MI355X engine on the single-GPU forward workload.
+
+Workload (the BASELINE single-GPU forward configuration): 256^3 particles on a 256^3 mesh, box
+1000 Mpc/h, float32, seeded Gaussian ICs (Philox white noise x EH98 P(k), Omega_c=0.25,
+sigma8=0.8, Omega_b=0.0486, h=0.6774, n_s=0.9667), 1LPT to a=0.1, EfficientDriftDoubleKick with
+dt0=0.05 to a=1, snapshots at 4 HEALPix shells (nside 256, bilinear, equal comoving width out to
+z=0.5) plus a final a=1 snapshot whose 3D P(k) is measured.
+
+A "step" is one PM step of that schedule (paint, force mesh, readout + kick + drift), with the
+shell paints that fall between steps inside the timed region. --steps K times K consecutive steps
+of the schedule (wrapping around to the initial state if K exceeds its length); --warmup W runs W
+untimed steps first. Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PM steps/sec @256^3 fused vs plain GPU vs CPU; fwd+VJP cost; served p99 latency"
+N, BOX, A0, DT0, NSIDE, NSHELL, ZMAX = 256, 1000.0, 0.1, 0.05, 256, 4, 0.5
+PEAK_HBM = 8.0e12
+
+
+def build_schedule(pkg, cosmo):
+    """(ops, shells): ops is a list of ('op', StepOp) / ('shell', i) / ('final',) in run order, with
+    drifts fused across steps exactly as pm.integrate does."""
+    from importlib import import_module
+
+    cosmology = pkg.cosmology
+    solvers = import_module("jax_fli_amd.pm.solvers")
+    rmax = float(cosmology.radial_comoving_distance(cosmo, cosmology.z2a(ZMAX)))
+    edges = np.linspace(0.0, rmax, NSHELL + 1)
+    centers = 0.5 * (edges[1:] + edges[:-1])[::-1]  # far -> near
+    widths = np.diff(edges)[::-1]
+    a_c = cosmology.a_of_chi(cosmo, centers).astype(np.float32)
+    ts = np.concatenate([a_c, np.array([1.0], np.float32)])
+    solver = pkg.EfficientDriftDoubleKick(interp_kernel=pkg.NoInterp(painting=pkg.PaintingOptions(target="particles")))
+    raw = [("op", solver.init_op(A0, A0 + DT0, DT0, cosmo))]
+    for ev in pkg.pm.time_grid(A0, ts, DT0, 1.0):
+        if ev[0] == "step":
+            raw.append(("op", solver.step_op(ev[1], ev[2], DT0, A0, cosmo)))
+        else:
+            raw.append(("shell", ev[1]) if ev[1] < NSHELL else ("final",))
+    ops, pending = [], None
+    for item in raw:
+        if item[0] == "op":
+            op = solvers.StepOp(item[1].pre, item[1].kick, item[1].post)
+            if pending is not None and pending.post is None and op.pre is not None:
+                pending.post, op.pre = op.pre, None
+            if pending is not None:
+                ops.append(("op", pending))
+            pending = op
+        else:
+            if pending is not None:
+                ops.append(("op", pending))
+                pending = None
+            ops.append(item)
+    return ops, list(zip(centers, widths))
+
+
+def phase_bytes(n_p, n_c, nspec):
+    """Algorithmic HBM bytes per launch of each pmx_step phase (float32 / complex64)."""
+    return {
+        "paint": 12 * n_p + 4 * n_c,
+        "r2c_z": 4 * n_c + 8 * nspec,
+        "c2c_y": 16 * nspec,
+        "kspace_x": 8 * nspec + 24 * nspec,
+        "c2c_y_inv": 48 * nspec,
+        "c2r_z": 24 * nspec + 12 * n_c,
+        "readout_kick_drift": 48 * n_p + 12 * n_c,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=0, help="timed PM steps (0: one full schedule)")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=1, help="CPU-baseline steps at full size (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (this engine's multi-GPU "
+              "decomposition is a design, not a runnable path)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import __graft_entry__
+
+    pkg = __graft_entry__.load_package()
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    cosmo = pkg.Cosmology(Omega_c=0.25, Omega_b=0.0486, h=0.6774, n_s=0.9667, sigma8=0.8)
+    mesh = (N, N, N)
+
+    t_fwd0 = time.perf_counter()
+    ic = pkg.gaussian_initial_conditions(0, mesh, (BOX,) * 3, cosmo=cosmo, nside=NSIDE)
+    dx, p = pkg.lpt(cosmo, ic, ts=A0)
+    torch.cuda.synchronize()
+    x0, v0 = dx.array.contiguous(), p.array.contiguous()
+    x, v = x0.clone(), v0.clone()
+    ops, shells = build_schedule(pkg, cosmo)
+    nsteps_sched = sum(1 for o in ops if o[0] == "op")
+    obs = tuple(b * 0.5 for b in (BOX,) * 3)
+    maps = torch.empty((NSHELL, 12 * NSIDE**2), device=dev)
+    kedges = np.geomspace(2 * np.pi / BOX, np.pi * N / BOX, 33)
+    force_scale = 1.5 * cosmo.Omega_m
+    plan = pkg._lib.get_plan(mesh)
+
+    def run_item(item):
+        kind = item[0]
+        if kind == "op":
+            o = item[1]
+            pkg.engine.step(x, v, mesh, force_scale=force_scale, c_kick=o.kick, c_drift_pre=o.pre, c_drift_post=o.post)
+        elif kind == "shell":
+            c, w = shells[item[1]]
+            pkg.engine.paint_healpix(x, mesh, (BOX,) * 3, obs, c - w / 2, c + w / 2, NSIDE, scheme="bilinear",
+                                     out=maps[item[1]])
+
+    # full forward (LPT already done above) + final P(k): one untimed pass that also warms up
+    for item in ops:
+        run_item(item)
+    rho = pkg.engine.paint(x, mesh)
+    k_fin, pk_fin = pkg.power(rho, box_shape=(BOX,) * 3, kedges=kedges)
+    torch.cuda.synchronize()
+    forward_ms = (time.perf_counter() - t_fwd0) * 1e3
+
+    def steps_iter(k, start=0):
+        """Yield the schedule items covering k PM steps from position `start` (wrapping)."""
+        done, i = 0, start
+        while done < k:
+            if i == len(ops):
+                i = 0
+                yield ("reset",)
+            item = ops[i]
+            i += 1
+            yield item
+            if item[0] == "op":
+                done += 1
+
+    def run(items):
+        for item in items:
+            if item[0] == "reset":
+                x.copy_(x0)
+                v.copy_(v0)
+            elif item[0] != "final":
+                run_item(item)
+
+    x.copy_(x0)
+    v.copy_(v0)
+    run(steps_iter(args.warmup))
+    x.copy_(x0)
+    v.copy_(v0)
+    K = args.steps if args.steps > 0 else nsteps_sched
+    items = list(steps_iter(K))
+    torch.cuda.synchronize()
+    plan.timing(True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    run(items)
+    e1.record()
+    torch.cuda.synchronize()
+    elapsed_ms = e0.elapsed_time(e1)
+    phases, nrec = plan.timing_read()
+    plan.timing(False)
+    ms_per_step = elapsed_ms / K
+    value = 1000.0 / ms_per_step
+
+    # roofline of the dominant kernel (largest summed time over the timed region)
+    nspec = (N // 2 + 1) * N * N
+    pb = phase_bytes(N**3, N**3, nspec)
+    dom = max(phases, key=lambda k: phases[k])
+    avg_ms = phases[dom] / max(nrec, 1)
+    achieved = pb[dom] / (avg_ms * 1e-3)
+    step_bytes = 60 * N**3 + 64 * N**3
+    per_phase = {k: {"avg_us": round(phases[k] / max(nrec, 1) * 1e3, 2),
+                     "GB_per_s": round(pb[k] / (phases[k] / max(nrec, 1) * 1e-3) / 1e9, 1)} for k in phases}
+
+    # rocFFT beside the hand-written FFT (r2c + c2r round trip of one 256^3 mesh)
+    def t_gpu(fn, reps=10):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    spec = pkg.engine.rfftn(rho)
+    fft_ours = t_gpu(lambda: pkg.engine.irfftn(pkg.engine.rfftn(rho), mesh))
+    fft_roc = t_gpu(lambda: torch.fft.irfftn(torch.fft.rfftn(rho), s=mesh))
+    del spec
+
+    # CPU baseline: the NumPy/SciPy restatement of the reference step on the host cores
+    cpu = None
+    if args.cpu_steps > 0:
+        from oracle import pm_oracle as orc
+
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        xc, vc = x0.cpu().numpy(), v0.cpu().numpy()
+        o = ops[1][1]
+        t = time.perf_counter()
+        for _ in range(args.cpu_steps):
+            xc, vc = orc.cpu_pm_step(xc, vc, force_scale, o.kick, o.post or 0.0, workers=cores)
+        cpu_s = (time.perf_counter() - t) / args.cpu_steps
+        cpu = {"value": round(1.0 / cpu_s, 5), "unit": "PM steps/s", "cores": cores, "kind": "port",
+               "sample": f"{args.cpu_steps} full 256^3 PM step(s) of the NumPy/SciPy restatement of the reference "
+                         f"(c2c scipy.fft on {cores} threads, single-threaded CIC scatter/gather), "
+                         f"{cpu_s:.1f} s/step"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "PM steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Philox white noise x EH98 P(k), 1LPT at a=0.1)",
+        "config": {"workload": "single-GPU forward: 256^3 particles, 256^3 mesh, box 1000 Mpc/h, KDK dt0=0.05 "
+                               "a=0.1->1, 4 HEALPix shells nside 256 to z=0.5",
+                   "mesh": N, "particles": N**3, "box_mpc_h": BOX, "dt0": DT0, "schedule_steps": nsteps_sched,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": pb[dom], "avg_launch_us": round(avg_ms * 1e3, 2)},
+        "step_roofline": {"algorithmic_bytes_per_step": step_bytes,
+                          "achieved_GB_per_s": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+                          "frac": round(step_bytes / (ms_per_step * 1e-3) / PEAK_HBM, 4)},
+        "phases": per_phase,
+        "cpu_baseline": cpu,
+        "forward_total_ms": round(forward_ms, 1),
+        "fft_roundtrip_ms": {"pmx": round(fft_ours, 4), "rocfft": round(fft_roc, 4)},
+        "final_pk": {"k": [round(float(a), 5) for a in k_fin[:8]], "pk": [round(float(a), 3) for a in pk_fin[:8]]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -43,6 +43,12 @@ size_t pmx_plan_workspace_bytes(pmx_plan plan);
 /* Particles the last tiled paint on this plan sent to the atomic escape pass (their displacement
  * exceeded the tile halo). Synchronises the device. */
 int pmx_plan_escapes(pmx_plan plan, uint64_t* count);
+/* Per-kernel timing of pmx_step on this plan. enable=1 clears previous records and brackets every
+ * following pmx_step with HIP events on its stream; pmx_plan_timing_read synchronises on the last
+ * event and returns the summed milliseconds of the 7 phases (paint, r2c_z, c2c_y, k-space x pass,
+ * inverse c2c_y, c2r_z, readout+kick+drift) and the number of steps recorded. */
+int pmx_plan_timing(pmx_plan plan, int enable);
+int pmx_plan_timing_read(pmx_plan plan, double* ms7, int* nsteps);
 /* Physical box (Mpc/h) for the k tables of pmx_power and pmx_color_field (synchronous upload). */
 int pmx_plan_set_box(pmx_plan plan, const double box[3]);
 

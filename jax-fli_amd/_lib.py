@@ -33,6 +33,8 @@ SIGNATURES = {
     "pmx_plan_workspace_bytes": (_c.c_size_t, [_vp]),
     "pmx_plan_set_box": (_i, [_vp, _dp]),
     "pmx_plan_escapes": (_i, [_vp, _c.POINTER(_c.c_uint64)]),
+    "pmx_plan_timing": (_i, [_vp, _i]),
+    "pmx_plan_timing_read": (_i, [_vp, _dp, _c.POINTER(_i)]),
     "pmx_paint_cic": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _vp, _i, _vp]),
     "pmx_readout_cic": (_i, [_vp, _vp, _i, _vp, _i, _i, _i, _i, _vp, _vp]),
     "pmx_fft_r2c": (_i, [_vp, _vp, _vp, _vp]),
@@ -123,6 +125,20 @@ class Plan:
         with torch.cuda.device(self.device):
             check(load().pmx_plan_escapes(self.handle, ctypes.byref(c)), "pmx_plan_escapes")
         return int(c.value)
+
+    PHASES = ("paint", "r2c_z", "c2c_y", "kspace_x", "c2c_y_inv", "c2r_z", "readout_kick_drift")
+
+    def timing(self, enable: bool) -> None:
+        with torch.cuda.device(self.device):
+            check(load().pmx_plan_timing(self.handle, int(bool(enable))), "pmx_plan_timing")
+
+    def timing_read(self):
+        """(phase -> summed ms, number of steps) of the pmx_step calls since timing(True)."""
+        ms = (ctypes.c_double * 7)()
+        n = ctypes.c_int()
+        with torch.cuda.device(self.device):
+            check(load().pmx_plan_timing_read(self.handle, ms, ctypes.byref(n)), "pmx_plan_timing_read")
+        return dict(zip(self.PHASES, list(ms))), int(n.value)
 
     @property
     def workspace_bytes(self) -> int:

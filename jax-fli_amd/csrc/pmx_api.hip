@@ -123,14 +123,43 @@ int ensure_acc(pmx_plan_t* p, size_t count) {
   return 0;
 }
 
-int force_mesh_from(pmx_plan_t* p, const float* src, float scale, unsigned flags, float* out, hipStream_t s) {
-  PMX_TRY(pmx_fft_forward_to_B(p, src, s));
+// Per-kernel timing of pmx_step: when enabled, 8 events per step bracket paint, the five FFT passes
+// and the readout on the caller's stream.
+void mark(StepMarks* m, int i, hipStream_t s) {
+  if (m) (void)hipEventRecord(m->e[i], s);
+}
+
+int force_mesh_from(pmx_plan_t* p, const float* src, float scale, unsigned flags, float* out, hipStream_t s,
+                    StepMarks* m = nullptr) {
+  PMX_TRY(pmx_pass_r2c_z(p, src, p->specA, s));
+  mark(m, 2, s);
+  PMX_TRY(pmx_pass_c2c_y(p, p->specA, p->specB, s));
+  mark(m, 3, s);
   KOp op{};
   op.mode = XP_FORCES;
   op.scale = (float)((double)scale / (double)p->ncell);
   op.flags = flags & PMX_DECONV;
   PMX_TRY(pmx_xpass(p, op, p->specB, p->specC, s));
-  return pmx_inverse_from_C(p, 3, p->specC, out, s);
+  mark(m, 4, s);
+  PMX_TRY(pmx_pass_c2c_y_inv(p, 3, p->specC, p->specD, s));
+  mark(m, 5, s);
+  return pmx_pass_c2r_z(p, 3, p->specD, out, s);
+}
+
+StepMarks* new_marks(pmx_plan_t* p) {
+  if (!p->timing || !p->marks) return nullptr;
+  StepMarks m;
+  for (int i = 0; i < kStepMarks; ++i)
+    if (hipEventCreate(&m.e[i]) != hipSuccess) return nullptr;
+  p->marks->push_back(m);
+  return &p->marks->back();
+}
+
+void clear_marks(pmx_plan_t* p) {
+  if (!p->marks) return;
+  for (auto& m : *p->marks)
+    for (int i = 0; i < kStepMarks; ++i) (void)hipEventDestroy(m.e[i]);
+  p->marks->clear();
 }
 
 }  // namespace
@@ -211,6 +240,8 @@ int pmx_plan_destroy(pmx_plan p) {
                   p->kphys_x, p->kphys_y, p->kphys_z, p->esc_count, p->esc_list};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  clear_marks(p);
+  delete p->marks;
   delete p;
   return 0;
 }
@@ -313,12 +344,44 @@ int pmx_step(pmx_plan p, float* x, float* v, int npx, int npy, int npz, int mode
   PMX_TRY(check_particles(p, x, npx, npy, npz, mode));
   if (!v) return pmx_set_error(1, "null velocities");
   hipStream_t s = (hipStream_t)stream;
+  StepMarks* m = new_marks(p);
+  mark(m, 0, s);
   PartArgs a = part_args(x, (flags & PMX_DRIFT_PRE) ? v : nullptr, npx, npy, npz, mode, p);
   PMX_TRY(pmx_launch_paint(p, a, nullptr, 1.f, c_drift_pre, p->rho, 0, ((p->flags | flags) & PMX_PAINT_ATOMIC) != 0, s));
-  PMX_TRY(force_mesh_from(p, p->rho, force_scale, flags, p->fmesh, s));
+  mark(m, 1, s);
+  PMX_TRY(force_mesh_from(p, p->rho, force_scale, flags, p->fmesh, s, m));
+  mark(m, 6, s);
   a.v = v;
   const int what = 2 | ((flags & PMX_DRIFT_POST) ? 4 : 0);
-  return pmx_launch_readout(p, a, p->fmesh, 3, nullptr, c_kick, c_drift_post, what, s);
+  PMX_TRY(pmx_launch_readout(p, a, p->fmesh, 3, nullptr, c_kick, c_drift_post, what, s));
+  mark(m, 7, s);
+  return 0;
+}
+
+int pmx_plan_timing(pmx_plan p, int enable) {
+  PMX_TRY(check_plan(p));
+  if (!p->marks) p->marks = new std::vector<StepMarks>();
+  if (enable) clear_marks(p);
+  p->timing = enable ? 1 : 0;
+  return 0;
+}
+
+int pmx_plan_timing_read(pmx_plan p, double* ms, int* nsteps) {
+  PMX_TRY(check_plan(p));
+  if (!ms || !nsteps) return pmx_set_error(1, "null output");
+  for (int i = 0; i < kStepMarks - 1; ++i) ms[i] = 0.0;
+  *nsteps = 0;
+  if (!p->marks || p->marks->empty()) return 0;
+  PMX_HIP(hipEventSynchronize(p->marks->back().e[kStepMarks - 1]));
+  for (auto& mk : *p->marks) {
+    for (int i = 0; i < kStepMarks - 1; ++i) {
+      float t = 0.f;
+      PMX_HIP(hipEventElapsedTime(&t, mk.e[i], mk.e[i + 1]));
+      ms[i] += t;
+    }
+  }
+  *nsteps = (int)p->marks->size();
+  return 0;
 }
 
 int pmx_power(pmx_plan p, const float* mesh, const double* kedges, int nedges, double* pk_sum, double* k_sum,

@@ -492,28 +492,34 @@ int pmx_xpass(pmx_plan_t* p, const KOp& op, const float2* inB, float2* out, hipS
   return pmx_set_error(2, "unsupported nx");
 }
 
-int pmx_inverse_from_C(pmx_plan_t* p, int ncomp, const float2* inC, float* out, hipStream_t s) {
+int pmx_pass_c2c_y_inv(pmx_plan_t* p, int ncomp, const float2* inC, float2* outD, hipStream_t s) {
   // P4: lines C[c][kz][x][*] for consecutive kz at fixed x -> D[c][x][ky][kz]
   const long NX = p->nx, NY = p->ny, NZH = p->nzh;
-  int rc = 0;
+  if (ncomp != 1 && ncomp != 3) return pmx_set_error(2, "ncomp must be 1 or 3");
   switch (p->ny) {
-#define M_(v)                                                                                     \
-  case v:                                                                                         \
-    rc = launch_c2c<v, true>(inC, p->specD, NY, NX * NY, (long)p->nspec, NY * NZH, NZH, (long)p->nspec, \
-                             p->nzh, p->nx, ncomp, p->tw_y, s);                                   \
-    break;
+#define M_(v)                                                                                         \
+  case v:                                                                                             \
+    return launch_c2c<v, true>(inC, outD, NY, NX * NY, (long)p->nspec, NY * NZH, NZH, (long)p->nspec, \
+                               p->nzh, p->nx, ncomp, p->tw_y, s);
     PMX_FOR_AXIS(M_)
 #undef M_
-    default:
-      return pmx_set_error(2, "unsupported ny");
   }
-  if (rc) return rc;
+  return pmx_set_error(2, "unsupported ny");
+}
+
+int pmx_pass_c2r_z(pmx_plan_t* p, int ncomp, const float2* inD, float* out, hipStream_t s) {
   if (ncomp != 1 && ncomp != 3) return pmx_set_error(2, "ncomp must be 1 or 3");
   switch (p->nz) {
 #define M_(v) \
-  case v: return ncomp == 3 ? launch_c2r_z<v, 3>(p, p->specD, out, s) : launch_c2r_z<v, 1>(p, p->specD, out, s);
+  case v: return ncomp == 3 ? launch_c2r_z<v, 3>(p, inD, out, s) : launch_c2r_z<v, 1>(p, inD, out, s);
     PMX_FOR_AXIS(M_)
 #undef M_
   }
   return pmx_set_error(2, "unsupported nz");
+}
+
+int pmx_inverse_from_C(pmx_plan_t* p, int ncomp, const float2* inC, float* out, hipStream_t s) {
+  int rc = pmx_pass_c2c_y_inv(p, ncomp, inC, p->specD, s);
+  if (rc) return rc;
+  return pmx_pass_c2r_z(p, ncomp, p->specD, out, s);
 }

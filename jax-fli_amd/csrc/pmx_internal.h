@@ -11,6 +11,13 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
+
+// per-step timing marks of pmx_step (paint | r2c_z | c2c_y | xpass | c2c_y_inv | c2r_z | readout)
+constexpr int kStepMarks = 8;
+struct StepMarks {
+  hipEvent_t e[kStepMarks];
+};
 
 struct pmx_plan_t {
   int nx, ny, nz, nzh;
@@ -39,6 +46,8 @@ struct pmx_plan_t {
   double* acc;      // double accumulators (healpix maps, flat maps); grown on demand
   size_t acc_bytes;
   size_t workspace_bytes;
+  int timing;
+  std::vector<StepMarks>* marks;
 };
 
 // error reporting (pmx_api.hip)
@@ -74,6 +83,8 @@ int pmx_xpass(pmx_plan_t* p, const KOp& op, const float2* inB, float2* out, hipS
 int pmx_inverse_from_C(pmx_plan_t* p, int ncomp, const float2* inC, float* out, hipStream_t s);  // specC -> real
 int pmx_pass_r2c_z(pmx_plan_t* p, const float* in, float2* outA, hipStream_t s);
 int pmx_pass_c2c_y(pmx_plan_t* p, const float2* inA, float2* outB, hipStream_t s);
+int pmx_pass_c2c_y_inv(pmx_plan_t* p, int ncomp, const float2* inC, float2* outD, hipStream_t s);
+int pmx_pass_c2r_z(pmx_plan_t* p, int ncomp, const float2* inD, float* out, hipStream_t s);
 
 // particle kernels (pmx_particles.hip)
 struct PartArgs {

@@ -422,3 +422,28 @@ def paint_healpix(pos, mesh_shape, box, observer, rmin, rmax, nside, scheme="bil
                 acc[pix[q]] += wt[p] * w[q]
     vol = (4 * np.pi / npix) * (rmax**3 - rmin**3) / 3.0
     return (acc / vol).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------- CPU baseline step
+def cpu_pm_step(x, v, force_scale, c_kick, c_drift, workers=1):
+    """One PM step of the reference's op structure on the host, float32, for the CPU baseline:
+    CIC paint (8 scatter passes), complex-to-complex 3D FFT, Green x gradient per component,
+    3 inverse c2c FFTs, 3 separate CIC readouts, kick, drift (pm_forces as called by _forces,
+    solvers.py:19-32, then the kick/drift of solvers.py:296-318). FFTs use scipy.fft with
+    `workers` threads; the scatter/gather passes are single-threaded NumPy."""
+    import scipy.fft as sfft
+
+    mesh_shape = x.shape[:3]
+    rho = cic_paint(x, mesh_shape, "relative", dtype=np.float32)
+    dk = sfft.fftn(rho.astype(np.complex64), workers=workers)
+    kvec = fftk(mesh_shape, np.float32)
+    pot = dk * invlaplace_kernel(kvec).astype(np.float32)
+    f = np.empty(x.shape, dtype=np.float32)
+    for i in range(3):
+        g = gradient_kernel(kvec, i).astype(np.complex64)
+        fm = sfft.ifftn(-g * pot, workers=workers).real.astype(np.float32)
+        f[..., i] = cic_read(fm, x, "relative", dtype=np.float32)
+    f *= np.float32(force_scale)
+    v = v + f * np.float32(c_kick)
+    x = x + v * np.float32(c_drift)
+    return x, v
