@@ -11,11 +11,11 @@
 int pmx_launch_fill(float* a, size_t n, float v, hipStream_t s);
 int pmx_launch_fill_d(double* a, size_t n, hipStream_t s);
 int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, const double box[3],
-                       const double obs[3], double rmin, double rmax, int nside, int scheme, double* acc,
+                       const double obs[3], double rmin, double rmax, int nside, int scheme, float* acc,
                        hipStream_t s);
-int pmx_launch_finalize_map(const double* acc, float* map, long n, double scale, int accumulate, hipStream_t s);
+int pmx_launch_finalize_map(const float* acc, float* map, long n, double scale, int accumulate, hipStream_t s);
 int pmx_launch_flat(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, int fnx, int fny, float zlo, float zhi,
-                    double* acc, hipStream_t s);
+                    float* acc, hipStream_t s);
 int pmx_launch_philox_raw(uint32_t* out, size_t nblocks, uint64_t seed, hipStream_t s);
 int pmx_launch_white_noise(float* out, size_t n, uint64_t seed, hipStream_t s);
 int pmx_launch_lpt1(const float* F, float* dx, float* pv, size_t n, float D, float c, hipStream_t s);
@@ -460,12 +460,13 @@ int pmx_paint_healpix(pmx_plan p, const float* pos, int npx, int npy, int npz, i
   const long npix = 12L * nside * nside;
   PMX_TRY(ensure_acc(p, (size_t)npix));
   hipStream_t s = (hipStream_t)stream;
-  PMX_TRY(pmx_launch_fill_d(p->acc, npix, s));
+  float* acc = reinterpret_cast<float*>(p->acc);
+  PMX_TRY(pmx_launch_fill(acc, npix, 0.f, s));
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
-  PMX_TRY(pmx_launch_healpix(p, a, weights, weight, box, observer, rmin, rmax, nside, scheme, p->acc, s));
+  PMX_TRY(pmx_launch_healpix(p, a, weights, weight, box, observer, rmin, rmax, nside, scheme, acc, s));
   const double omega = 4.0 * M_PI / (double)npix;
   const double vol = omega * (rmax * rmax * rmax - rmin * rmin * rmin) / 3.0;
-  return pmx_launch_finalize_map(p->acc, map, npix, 1.0 / vol, accumulate, s);
+  return pmx_launch_finalize_map(acc, map, npix, 1.0 / vol, accumulate, s);
 }
 
 int pmx_paint_flat(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, const float* weights,
@@ -477,10 +478,11 @@ int pmx_paint_flat(pmx_plan p, const float* pos, int npx, int npy, int npz, int 
   const size_t n = (size_t)fnx * fny;
   PMX_TRY(ensure_acc(p, n));
   hipStream_t s = (hipStream_t)stream;
-  PMX_TRY(pmx_launch_fill_d(p->acc, n, s));
+  float* acc = reinterpret_cast<float*>(p->acc);
+  PMX_TRY(pmx_launch_fill(acc, n, 0.f, s));
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
-  PMX_TRY(pmx_launch_flat(p, a, weights, weight, fnx, fny, zlo, zhi, p->acc, s));
-  return pmx_launch_finalize_map(p->acc, map, (long)n, scale, accumulate, s);
+  PMX_TRY(pmx_launch_flat(p, a, weights, weight, fnx, fny, zlo, zhi, acc, s));
+  return pmx_launch_finalize_map(acc, map, (long)n, scale, accumulate, s);
 }
 
 }  // extern "C"

@@ -1,8 +1,11 @@
-// Particle-side kernels: CIC paint (with an optional drift fused in front), CIC readout fused with
-// the kick and the next drift, HEALPix and flat-sky shell painting, counter-based white noise.
+// Particle-side kernels: CIC paint (owner-computes tiles + escape pass, or one-pass atomic scatter),
+// CIC readout fused with the kick and the next drift, HEALPix and flat-sky shell painting,
+// counter-based white noise.
 //
 // Every particle kernel walks the particle grid in its Lagrangian (i, j, k) order, so a wavefront
-// touches a compact patch of the mesh and the 8-corner gathers/scatters stay in L2.
+// touches a compact patch of the mesh and the 8-corner gathers/scatters stay in L2. Index math is
+// 32-bit (meshes up to 1024^3 cells, x3 components, fit) and the lattice coordinates of a thread's
+// particles are advanced incrementally instead of being divided out of the flat index.
 // FP contraction is disabled here so x + v*c and the CIC weights round exactly like the
 // unfused float32 expressions of the reference (two roundings, no FMA).
 #include "pmx_internal.h"
@@ -47,36 +50,68 @@ __device__ __forceinline__ Corner1D corner_abs(float x, int n) {
   return c;
 }
 
+// The 8 CIC corners of one particle: mesh cell offsets and weights, x-fastest order
+// (0,0,0),(1,0,0),(0,1,0),(1,1,0),(0,0,1),...
 struct Cic {
-  Corner1D c[3];
+  unsigned off[8];
+  float w[8];
 };
 
-__device__ __forceinline__ Cic make_cic(const PartArgs& a, size_t p, float3 x, int NX, int NY, int NZ, int rx, int ry,
-                                        int rz) {
+__device__ __forceinline__ Cic cic_from(Corner1D cx, Corner1D cy, Corner1D cz, int NY, int NZ) {
+  const unsigned nyz = (unsigned)NY * NZ;
+  const unsigned x0 = cx.i0 * nyz, x1 = cx.i1 * nyz;
+  const unsigned y0 = (unsigned)cy.i0 * NZ, y1 = (unsigned)cy.i1 * NZ;
   Cic q;
-  if (a.mode == 0) {
-    const int k = (int)(p % a.npz);
-    const size_t q2 = p / a.npz;
-    const int j = (int)(q2 % a.npy);
-    const int i = (int)(q2 / a.npy);
-    q.c[0] = corner_rel(i * rx, x.x, NX);
-    q.c[1] = corner_rel(j * ry, x.y, NY);
-    q.c[2] = corner_rel(k * rz, x.z, NZ);
-  } else {
-    q.c[0] = corner_abs(x.x, NX);
-    q.c[1] = corner_abs(x.y, NY);
-    q.c[2] = corner_abs(x.z, NZ);
+#pragma unroll
+  for (int o = 0; o < 8; ++o) {
+    const int ox = o & 1, oy = (o >> 1) & 1, oz = (o >> 2) & 1;
+    q.off[o] = (ox ? x1 : x0) + (oy ? y1 : y0) + (unsigned)(oz ? cz.i1 : cz.i0);
+    q.w[o] = ((ox ? cx.w1 : cx.w0) * (oy ? cy.w1 : cy.w0)) * (oz ? cz.w1 : cz.w0);
   }
   return q;
 }
 
-__device__ __forceinline__ float3 ld3(const float* a, size_t p) {
-  return make_float3(a[3 * p], a[3 * p + 1], a[3 * p + 2]);
+// Lattice coordinates (i, j, k) of a thread's current particle, advanced by a fixed stride.
+struct LatticeWalk {
+  int i, j, k, di, dj, dk, npy, npz;
+  __device__ __forceinline__ void init(unsigned p, unsigned stride, int ny, int nz) {
+    npy = ny;
+    npz = nz;
+    k = (int)(p % (unsigned)nz);
+    const unsigned q = p / (unsigned)nz;
+    j = (int)(q % (unsigned)ny);
+    i = (int)(q / (unsigned)ny);
+    dk = (int)(stride % (unsigned)nz);
+    const unsigned s = stride / (unsigned)nz;
+    dj = (int)(s % (unsigned)ny);
+    di = (int)(s / (unsigned)ny);
+  }
+  __device__ __forceinline__ void step() {
+    k += dk;
+    int c = 0;
+    if (k >= npz) { k -= npz; c = 1; }
+    j += dj + c;
+    c = 0;
+    if (j >= npy) { j -= npy; c = 1; }
+    i += di + c;
+  }
+};
+
+__device__ __forceinline__ Cic particle_cic(const PartArgs& a, const LatticeWalk& lw, float3 x, int NX, int NY, int NZ,
+                                            int rx, int ry, int rz) {
+  if (a.mode == 0)
+    return cic_from(corner_rel(lw.i * rx, x.x, NX), corner_rel(lw.j * ry, x.y, NY), corner_rel(lw.k * rz, x.z, NZ), NY,
+                    NZ);
+  return cic_from(corner_abs(x.x, NX), corner_abs(x.y, NY), corner_abs(x.z, NZ), NY, NZ);
 }
-__device__ __forceinline__ void st3(float* a, size_t p, float3 v) {
-  a[3 * p] = v.x;
-  a[3 * p + 1] = v.y;
-  a[3 * p + 2] = v.z;
+
+__device__ __forceinline__ float3 ld3(const float* a, unsigned p) {
+  return make_float3(a[3 * (size_t)p], a[3 * (size_t)p + 1], a[3 * (size_t)p + 2]);
+}
+__device__ __forceinline__ void st3(float* a, unsigned p, float3 v) {
+  a[3 * (size_t)p] = v.x;
+  a[3 * (size_t)p + 1] = v.y;
+  a[3 * (size_t)p + 2] = v.z;
 }
 __device__ __forceinline__ float3 axpy3(float3 x, float3 v, float c) {  // x + v*c, two roundings
   return make_float3(x.x + v.x * c, x.y + v.y * c, x.z + v.z * c);
@@ -86,15 +121,15 @@ __device__ __forceinline__ float3 axpy3(float3 x, float3 v, float c) {  // x + v
 // logical id remaps so each XCD walks one contiguous slab of the Lagrangian grid and the mesh
 // planes it gathers from / scatters to stay in that XCD's L2.
 struct Chunk {
-  size_t begin, end;
+  unsigned begin, end;
 };
 __device__ __forceinline__ unsigned xcd_logical(unsigned b, unsigned G) {
   return (G % 8u == 0u) ? (b % 8u) * (G / 8u) + b / 8u : b;
 }
-__device__ __forceinline__ Chunk my_chunk(size_t n) {
+__device__ __forceinline__ Chunk my_chunk(unsigned n) {
   const unsigned G = gridDim.x;
-  const size_t per = (n + G - 1) / G;
-  const size_t beg = min(n, (size_t)xcd_logical(blockIdx.x, G) * per);
+  const unsigned per = (n + G - 1) / G;
+  const unsigned beg = min(n, xcd_logical(blockIdx.x, G) * per);
   return Chunk{beg, min(n, beg + per)};
 }
 
@@ -105,35 +140,20 @@ __global__ void k_fill_d(double* __restrict__ a, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = 0.0;
 }
 
-// CIC paint. Optional fused drift: x <- x + v*cd (written back) before painting.
-__global__ void __launch_bounds__(256) k_paint(PartArgs a, const float* __restrict__ w, float w0, float cd,
+// One-pass CIC scatter with global atomics (absolute mode, or relative mode on request).
+__global__ void __launch_bounds__(256) k_paint(PartArgs a, const float* __restrict__ w, float w0,
                                                float* __restrict__ mesh, int NX, int NY, int NZ, int rx, int ry,
                                                int rz) {
-  const size_t np = (size_t)a.npx * a.npy * a.npz;
+  const unsigned np = (unsigned)a.npx * a.npy * a.npz;
   const Chunk ch = my_chunk(np);
-  for (size_t p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x) {
-    float3 x = ld3(a.x, p);
-    if (a.v != nullptr) {
-      x = axpy3(x, ld3(a.v, p), cd);
-      st3(a.x, p, x);
-    }
+  LatticeWalk lw;
+  lw.init(ch.begin + threadIdx.x, blockDim.x, a.npy, a.npz);
+  for (unsigned p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x, lw.step()) {
+    const float3 x = ld3(a.x, p);
     const float wt = w ? w[p] : w0;
-    const Cic q = make_cic(a, p, x, NX, NY, NZ, rx, ry, rz);
+    const Cic q = particle_cic(a, lw, x, NX, NY, NZ, rx, ry, rz);
 #pragma unroll
-    for (int cx = 0; cx < 2; ++cx) {
-      const float wx = cx ? q.c[0].w1 : q.c[0].w0;
-      const size_t bx = (size_t)(cx ? q.c[0].i1 : q.c[0].i0) * NY;
-#pragma unroll
-      for (int cy = 0; cy < 2; ++cy) {
-        const float wxy = wx * (cy ? q.c[1].w1 : q.c[1].w0);
-        const size_t bxy = (bx + (cy ? q.c[1].i1 : q.c[1].i0)) * NZ;
-#pragma unroll
-        for (int cz = 0; cz < 2; ++cz) {
-          const float wv = (wxy * (cz ? q.c[2].w1 : q.c[2].w0)) * wt;
-          unsafeAtomicAdd(mesh + bxy + (cz ? q.c[2].i1 : q.c[2].i0), wv);
-        }
-      }
-    }
+    for (int o = 0; o < 8; ++o) unsafeAtomicAdd(mesh + q.off[o], q.w[o] * wt);
   }
 }
 
@@ -143,7 +163,9 @@ __global__ void __launch_bounds__(256) k_paint(PartArgs a, const float* __restri
 // plain stores -- every mesh cell is written exactly once, no global atomics and no zero fill.
 // A particle with |floor(dx)| > H along any axis ("escape") is skipped by all tiles; its home tile
 // records it in its own segment of the escape list (LDS counter, no global atomics) and
-// k_paint_escapes deposits those with atomics afterwards.
+// k_paint_escapes deposits those with atomics afterwards. Halo particles whose two corners per axis
+// miss the tile leave after one bounds test; particles whose 8 corners are all inside the tile
+// deposit without per-corner checks.
 constexpr int PT_MAX = 16 * 16 * 32;
 constexpr int PT_UNROLL = 4;
 
@@ -158,20 +180,29 @@ __device__ __forceinline__ int wrap_near(int i, int n) {
   return i;
 }
 
-__global__ void __launch_bounds__(256) k_paint_tiles(PartArgs a, const float* __restrict__ w, float w0,
-                                                     float* __restrict__ mesh, int NX, int NY, int NZ, int rx,
-                                                     int ry, int rz, int BX, int BY, int BZ, int H, int accumulate,
+// Fixed-point LDS accumulation: deposits are added as int64 multiples of 2^-32 (exact for CIC
+// weights, order independent, so the painted mesh is bitwise deterministic) -- integer LDS atomics
+// also run several times faster than float ones. Before issuing, each particle's upper-z corner
+// values are handed to the next lane when that lane is its z-neighbour on the lattice with the
+// same base (x, y) and base z + 1, halving the LDS atomics on smooth displacement fields.
+constexpr float PT_FIX = 4294967296.0f;  // 2^32
+__device__ __forceinline__ unsigned long long fix64(float v) { return (unsigned long long)__float2ll_rn(v * PT_FIX); }
+
+template <bool R1>
+__global__ void __launch_bounds__(512) k_paint_tiles(PartArgs a, const float* __restrict__ w, float w0,
+                                                     float* __restrict__ mesh, int NX, int NY, int NZ, int rx_,
+                                                     int ry_, int rz_, int BX, int BY, int BZ, int H, int accumulate,
                                                      unsigned* __restrict__ esc_count, unsigned* __restrict__ esc_list,
                                                      int esc_cap) {
-  __shared__ float tile[PT_MAX];
-  __shared__ unsigned s_nesc;
-  const int ntx = NX / BX, nty = NY / BY, ntz = NZ / BZ;
+  __shared__ unsigned long long tile[PT_MAX];
+  const int rx = R1 ? 1 : rx_, ry = R1 ? 1 : ry_, rz = R1 ? 1 : rz_;
+  const int nty = NY / BY, ntz = NZ / BZ;
   const unsigned L = xcd_logical(blockIdx.x, gridDim.x);
   const int tz = L % ntz, ty = (L / ntz) % nty, tx = L / (ntz * nty);
   const int T0x = tx * BX, T0y = ty * BY, T0z = tz * BZ;
   const int ncell = BX * BY * BZ;
-  for (int i = threadIdx.x; i < ncell; i += blockDim.x) tile[i] = 0.f;
-  if (threadIdx.x == 0) s_nesc = 0u;
+  for (int i = threadIdx.x; i < ncell; i += blockDim.x) tile[i] = 0ull;
+  if (threadIdx.x == 0) esc_count[L] = 0u;
   __syncthreads();
   // lattice i*r in [T0 - 1 - H, T0 + B - 1 + H] (unwrapped)
   const int ilo = -floordiv(-(T0x - 1 - H), rx), ihi = floordiv(T0x + BX - 1 + H, rx);
@@ -179,6 +210,8 @@ __global__ void __launch_bounds__(256) k_paint_tiles(PartArgs a, const float* __
   const int klo = -floordiv(-(T0z - 1 - H), rz), khi = floordiv(T0z + BZ - 1 + H, rz);
   const int ci = ihi - ilo + 1, cj = jhi - jlo + 1, ck = khi - klo + 1;
   const float fH = (float)H;
+  const int BYZ = BY * BZ;
+  const int lane = __lane_id();
   // division-free walk over (ii, jj, kk), kk fastest, stride blockDim.x
   const int stride = blockDim.x;
   const int dk = stride % ck, drow = stride / ck;
@@ -187,8 +220,12 @@ __global__ void __launch_bounds__(256) k_paint_tiles(PartArgs a, const float* __
   int jj = (threadIdx.x / ck) % cj;
   int ii = threadIdx.x / (ck * cj);
   unsigned* my_esc = esc_list + (size_t)L * esc_cap;
-  while (ii < ci) {
-    size_t pidx[PT_UNROLL];
+  // every lane of a wave runs the same number of iterations (shuffles below need full waves)
+  const int rows_total = ci * cj * ck;
+  const int wave0 = threadIdx.x - lane;
+  const int iters = (rows_total - wave0 + PT_UNROLL * stride - 1) / (PT_UNROLL * stride);
+  for (int it = 0; it < iters; ++it) {
+    unsigned pidx[PT_UNROLL];
     int li[PT_UNROLL], lj[PT_UNROLL], lk[PT_UNROLL];
     bool ok[PT_UNROLL];
 #pragma unroll
@@ -197,7 +234,9 @@ __global__ void __launch_bounds__(256) k_paint_tiles(PartArgs a, const float* __
       li[u] = ilo + ii;
       lj[u] = jlo + jj;
       lk[u] = klo + kk;
-      pidx[u] = ((size_t)wrap_near(li[u], a.npx) * a.npy + wrap_near(lj[u], a.npy)) * a.npz + wrap_near(lk[u], a.npz);
+      pidx[u] = ok[u] ? ((unsigned)wrap_near(li[u], a.npx) * a.npy + wrap_near(lj[u], a.npy)) * a.npz +
+                            wrap_near(lk[u], a.npz)
+                      : 0u;
       kk += dk;
       int carry = 0;
       if (kk >= ck) { kk -= ck; carry = 1; }
@@ -211,54 +250,63 @@ __global__ void __launch_bounds__(256) k_paint_tiles(PartArgs a, const float* __
     for (int u = 0; u < PT_UNROLL; ++u) xv[u] = ok[u] ? ld3(a.x, pidx[u]) : make_float3(0.f, 0.f, 0.f);
 #pragma unroll
     for (int u = 0; u < PT_UNROLL; ++u) {
-      if (!ok[u]) continue;
       const float3 x = xv[u];
       const float fx = floorf(x.x), fy = floorf(x.y), fz = floorf(x.z);
-      const bool esc = !(fx >= -fH && fx <= fH && fy >= -fH && fy <= fH && fz >= -fH && fz <= fH);
-      if (esc) {
-        const bool home = (li[u] * rx >= T0x && li[u] * rx < T0x + BX && lj[u] * ry >= T0y && lj[u] * ry < T0y + BY &&
-                           lk[u] * rz >= T0z && lk[u] * rz < T0z + BZ);
-        if (home) my_esc[atomicAdd(&s_nesc, 1u)] = (unsigned)pidx[u];
-        continue;
-      }
-      const float wt = w ? w[pidx[u]] : w0;
+      const bool esc = ok[u] && !(fx >= -fH && fx <= fH && fy >= -fH && fy <= fH && fz >= -fH && fz <= fH);
+      const int Lx = li[u] * rx - T0x, Ly = lj[u] * ry - T0y, Lz = lk[u] * rz - T0z;
+      if (esc && Lx >= 0 && Lx < BX && Ly >= 0 && Ly < BY && Lz >= 0 && Lz < BZ)
+        my_esc[atomicAdd(esc_count + L, 1u)] = pidx[u];
+      const bool use = ok[u] && !esc;
+      const int bx = Lx + (use ? (int)fx : 0), by = Ly + (use ? (int)fy : 0), bz = Lz + (use ? (int)fz : 0);
+      const bool touch = use && bx >= -1 && bx < BX && by >= -1 && by < BY && bz >= -1 && bz < BZ;
+      const bool inside = touch && bx >= 0 && bx < BX - 1 && by >= 0 && by < BY - 1 && bz >= 0 && bz < BZ - 1;
+      const float wt = (touch && w) ? w[pidx[u]] : w0;
       const float frx = x.x - fx, fry = x.y - fy, frz = x.z - fz;
-      const float wx[2] = {1.0f - fabsf(frx), 1.0f - fabsf(frx - 1.0f)};
-      const float wy[2] = {1.0f - fabsf(fry), 1.0f - fabsf(fry - 1.0f)};
-      const float wz[2] = {1.0f - fabsf(frz), 1.0f - fabsf(frz - 1.0f)};
-      const int bx = li[u] * rx + (int)fx - T0x, by = lj[u] * ry + (int)fy - T0y, bz = lk[u] * rz + (int)fz - T0z;
+      const float wx0 = 1.0f - fabsf(frx), wx1 = 1.0f - fabsf(frx - 1.0f);
+      const float wy0 = 1.0f - fabsf(fry), wy1 = 1.0f - fabsf(fry - 1.0f);
+      const float wz0 = 1.0f - fabsf(frz), wz1 = 1.0f - fabsf(frz - 1.0f);
+      const float wxy[4] = {wx0 * wy0, wx1 * wy0, wx0 * wy1, wx1 * wy1};
+      float lo[4], hi[4];
 #pragma unroll
-      for (int cx = 0; cx < 2; ++cx) {
-        const int ix = bx + cx;
-        if (ix < 0 || ix >= BX) continue;
+      for (int c = 0; c < 4; ++c) {
+        lo[c] = (wxy[c] * wz0) * wt;
+        hi[c] = (wxy[c] * wz1) * wt;
+      }
+      // z-neighbour merge among fast-path lanes (full wave participates in the shuffles)
+      const int base = inside ? (bx * BY + by) * BZ + bz : -4;
+      const int prev_base = __shfl_up(base, 1);
+      float prev_hi[4];
 #pragma unroll
-        for (int cy = 0; cy < 2; ++cy) {
-          const int iy = by + cy;
-          if (iy < 0 || iy >= BY) continue;
-          const float wxy = wx[cx] * wy[cy];
+      for (int c = 0; c < 4; ++c) prev_hi[c] = __shfl_up(hi[c], 1);
+      const bool took = inside && lane > 0 && prev_base >= 0 && prev_base + 1 == base;
+      const int took_i = took ? 1 : 0;
+      const bool next_took = (__shfl_down(took_i, 1) != 0) && lane < 63;
+      if (inside) {
+        unsigned long long* t = tile + base;
+        const int offs[4] = {0, BYZ, BZ, BYZ + BZ};
 #pragma unroll
-          for (int cz = 0; cz < 2; ++cz) {
-            const int iz = bz + cz;
-            if (iz < 0 || iz >= BZ) continue;
-            atomicAdd(&tile[(ix * BY + iy) * BZ + iz], (wxy * wz[cz]) * wt);
-          }
+        for (int c = 0; c < 4; ++c) atomicAdd(t + offs[c], fix64(took ? lo[c] + prev_hi[c] : lo[c]));
+        if (!next_took) {
+#pragma unroll
+          for (int c = 0; c < 4; ++c) atomicAdd(t + offs[c] + 1, fix64(hi[c]));
+        }
+      } else if (touch) {
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+          const int ix = bx + (o & 1), iy = by + ((o >> 1) & 1), iz = bz + ((o >> 2) & 1);
+          if (ix >= 0 && ix < BX && iy >= 0 && iy < BY && iz >= 0 && iz < BZ)
+            atomicAdd(&tile[(ix * BY + iy) * BZ + iz], fix64((o >> 2) ? hi[o & 3] : lo[o & 3]));
         }
       }
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) esc_count[L] = s_nesc;
-  if (accumulate) {
-    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-      const int iz = c % BZ, iy = (c / BZ) % BY, ix = c / (BZ * BY);
-      float* dst = mesh + ((size_t)(T0x + ix) * NY + (T0y + iy)) * NZ + (T0z + iz);
-      *dst = *dst + tile[c];
-    }
-  } else {
-    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-      const int iz = c % BZ, iy = (c / BZ) % BY, ix = c / (BZ * BY);
-      mesh[((size_t)(T0x + ix) * NY + (T0y + iy)) * NZ + (T0z + iz)] = tile[c];
-    }
+  // rows of BZ contiguous cells
+  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+    const int iz = c % BZ, iy = (c / BZ) % BY, ix = c / BYZ;
+    float* dst = mesh + ((size_t)(T0x + ix) * NY + (T0y + iy)) * NZ + (T0z + iz);
+    const float v = (float)((double)(long long)tile[c] * 2.3283064365386962890625e-10);
+    *dst = accumulate ? *dst + v : v;
   }
 }
 
@@ -270,75 +318,49 @@ __global__ void __launch_bounds__(256) k_paint_escapes(PartArgs a, const float* 
   const unsigned n = esc_count[blockIdx.x];
   const unsigned* seg = esc_list + (size_t)blockIdx.x * esc_cap;
   for (unsigned e = threadIdx.x; e < n; e += blockDim.x) {
-    const size_t p = seg[e];
+    const unsigned p = seg[e];
     const float3 x = ld3(a.x, p);
     const float wt = w ? w[p] : w0;
-    const Cic q = make_cic(a, p, x, NX, NY, NZ, rx, ry, rz);
+    LatticeWalk lw;
+    lw.init(p, 1u, a.npy, a.npz);
+    const Cic q = particle_cic(a, lw, x, NX, NY, NZ, rx, ry, rz);
 #pragma unroll
-    for (int o = 0; o < 8; ++o) {
-      const int cx = o & 1, cy = (o >> 1) & 1, cz = (o >> 2) & 1;
-      const float wv = (((cx ? q.c[0].w1 : q.c[0].w0) * (cy ? q.c[1].w1 : q.c[1].w0)) * (cz ? q.c[2].w1 : q.c[2].w0)) * wt;
-      const size_t idx =
-          (((size_t)(cx ? q.c[0].i1 : q.c[0].i0) * NY + (cy ? q.c[1].i1 : q.c[1].i0)) * NZ + (cz ? q.c[2].i1 : q.c[2].i0));
-      unsafeAtomicAdd(mesh + idx, wv);
-    }
+    for (int o = 0; o < 8; ++o) unsafeAtomicAdd(mesh + q.off[o], q.w[o] * wt);
   }
 }
 
 // x <- x + v*c
-__global__ void __launch_bounds__(256) k_drift(float* __restrict__ x, const float* __restrict__ v, size_t n, float c) {
+__global__ void __launch_bounds__(256) k_drift(float* __restrict__ x, const float* __restrict__ v, unsigned n, float c) {
   const Chunk ch = my_chunk(n);
-  for (size_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) x[i] = x[i] + v[i] * c;
-}
-
-__global__ void __launch_bounds__(256) k_paint_escapes(PartArgs a, const float* __restrict__ w, float w0,
-                                                       float* __restrict__ mesh, int NX, int NY, int NZ, int rx, int ry,
-                                                       int rz, const unsigned* __restrict__ esc_count,
-                                                       const unsigned* __restrict__ esc_list) {
-  const unsigned n = *esc_count;
-  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-    const size_t p = esc_list[e];
-    const float3 x = ld3(a.x, p);
-    const float wt = w ? w[p] : w0;
-    const Cic q = make_cic(a, p, x, NX, NY, NZ, rx, ry, rz);
-#pragma unroll
-    for (int o = 0; o < 8; ++o) {
-      const int cx = o & 1, cy = (o >> 1) & 1, cz = (o >> 2) & 1;
-      const float wv = (((cx ? q.c[0].w1 : q.c[0].w0) * (cy ? q.c[1].w1 : q.c[1].w0)) * (cz ? q.c[2].w1 : q.c[2].w0)) * wt;
-      const size_t idx =
-          (((size_t)(cx ? q.c[0].i1 : q.c[0].i0) * NY + (cy ? q.c[1].i1 : q.c[1].i0)) * NZ + (cz ? q.c[2].i1 : q.c[2].i0));
-      unsafeAtomicAdd(mesh + idx, wv);
-    }
-  }
+  for (unsigned i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) x[i] = x[i] + v[i] * c;
 }
 
 // CIC readout of an NC-component interleaved mesh. what: bit0 store F in out, bit1 kick
 // v <- v + F*ck (NC == 3), bit2 drift x <- x + v*cd after the kick. Corners are summed
-// sequentially in the order x-fastest (0,0,0),(1,0,0),(0,1,0),(1,1,0),(0,0,1),...
+// sequentially in x-fastest order.
 template <int NC>
 __global__ void __launch_bounds__(256) k_readout(PartArgs a, const float* __restrict__ mesh, float* __restrict__ out,
                                                  float ck, float cd, int what, int NX, int NY, int NZ, int rx,
                                                  int ry, int rz) {
-  const size_t np = (size_t)a.npx * a.npy * a.npz;
+  const unsigned np = (unsigned)a.npx * a.npy * a.npz;
   const Chunk ch = my_chunk(np);
-  for (size_t p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x) {
+  LatticeWalk lw;
+  lw.init(ch.begin + threadIdx.x, blockDim.x, a.npy, a.npz);
+  for (unsigned p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x, lw.step()) {
     float3 x = ld3(a.x, p);
-    const Cic q = make_cic(a, p, x, NX, NY, NZ, rx, ry, rz);
+    const Cic q = particle_cic(a, lw, x, NX, NY, NZ, rx, ry, rz);
     float f[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) f[c] = 0.f;
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
-      const int cx = o & 1, cy = (o >> 1) & 1, cz = (o >> 2) & 1;
-      const float wv = ((cx ? q.c[0].w1 : q.c[0].w0) * (cy ? q.c[1].w1 : q.c[1].w0)) * (cz ? q.c[2].w1 : q.c[2].w0);
-      const size_t idx =
-          (((size_t)(cx ? q.c[0].i1 : q.c[0].i0) * NY + (cy ? q.c[1].i1 : q.c[1].i0)) * NZ + (cz ? q.c[2].i1 : q.c[2].i0));
+      const float* m = mesh + (size_t)q.off[o] * NC;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) f[c] = f[c] + mesh[idx * NC + c] * wv;
+      for (int c = 0; c < NC; ++c) f[c] = f[c] + m[c] * q.w[o];
     }
     if (what & 1) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) out[p * NC + c] = f[c];
+      for (int c = 0; c < NC; ++c) out[(size_t)p * NC + c] = f[c];
     }
     if constexpr (NC == 3) {
       if (what & 2) {
@@ -487,6 +509,7 @@ __device__ __forceinline__ void hp_interpol(const HpGeom& g, double theta, doubl
   }
 }
 
+
 struct ShellArgs {
   double box[3];    // Mpc/h
   double obs[3];    // observer, Mpc/h
@@ -494,31 +517,24 @@ struct ShellArgs {
   int scheme;       // 0 ngp, 1 bilinear
 };
 
-__device__ __forceinline__ double grid_pos(const PartArgs& a, size_t p, int c, float xc, int n, int r) {
-  double g = (double)xc;
-  if (a.mode == 0) {
-    const int k = (int)(p % a.npz);
-    const size_t q2 = p / a.npz;
-    const int j = (int)(q2 % a.npy);
-    const int i = (int)(q2 / a.npy);
-    g += (double)((c == 0 ? i : (c == 1 ? j : k)) * r);
-  }
-  g -= floor(g / n) * n;  // periodic box
-  return g;
+// particle position in grid units, lattice added in relative mode, wrapped into the periodic box
+__device__ __forceinline__ double grid_pos(int mode, int lat, float xc, int n) {
+  double g = (double)xc + (mode == 0 ? (double)lat : 0.0);
+  return g - floor(g / n) * n;
 }
 
 __global__ void __launch_bounds__(256) k_paint_healpix(PartArgs a, const float* __restrict__ w, float w0, ShellArgs sa,
-                                                       HpGeom g, double* __restrict__ acc, int NX, int NY, int NZ,
+                                                       HpGeom g, float* __restrict__ acc, int NX, int NY, int NZ,
                                                        int rx, int ry, int rz) {
-  const size_t np = (size_t)a.npx * a.npy * a.npz;
+  const unsigned np = (unsigned)a.npx * a.npy * a.npz;
   const Chunk ch = my_chunk(np);
-  for (size_t p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x) {
+  LatticeWalk lw;
+  lw.init(ch.begin + threadIdx.x, blockDim.x, a.npy, a.npz);
+  for (unsigned p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x, lw.step()) {
     const float3 x = ld3(a.x, p);
-    const double gx = grid_pos(a, p, 0, x.x, NX, rx), gy = grid_pos(a, p, 1, x.y, NY, ry),
-                 gz = grid_pos(a, p, 2, x.z, NZ, rz);
-    const double X = gx * sa.box[0] / NX - sa.obs[0];
-    const double Y = gy * sa.box[1] / NY - sa.obs[1];
-    const double Z = gz * sa.box[2] / NZ - sa.obs[2];
+    const double X = grid_pos(a.mode, lw.i * rx, x.x, NX) * sa.box[0] / NX - sa.obs[0];
+    const double Y = grid_pos(a.mode, lw.j * ry, x.y, NY) * sa.box[1] / NY - sa.obs[1];
+    const double Z = grid_pos(a.mode, lw.k * rz, x.z, NZ) * sa.box[2] / NZ - sa.obs[2];
     const double r = sqrt(X * X + Y * Y + Z * Z);
     if (!(r > sa.rmin && r <= sa.rmax)) continue;
     const double wt = w ? (double)w[p] : (double)w0;
@@ -527,21 +543,22 @@ __global__ void __launch_bounds__(256) k_paint_healpix(PartArgs a, const float* 
     if (!(phi < 2.0 * M_PI)) phi = 0.0;  // -tiny + 2 pi can round up to 2 pi
     if (sa.scheme == 0) {
       const long pix = hp_ang2pix(g, Z / r, phi);
-      atomicAdd(acc + pix, wt);
+      unsafeAtomicAdd(acc + pix, (float)wt);
     } else {
       const double theta = atan2(sqrt(X * X + Y * Y), Z);
       long pix[4];
       double wg[4];
       hp_interpol(g, theta, phi, pix, wg);
 #pragma unroll
-      for (int m = 0; m < 4; ++m) atomicAdd(acc + pix[m], wt * wg[m]);
+      for (int m = 0; m < 4; ++m) unsafeAtomicAdd(acc + pix[m], (float)(wt * wg[m]));
     }
   }
 }
 
-__global__ void k_finalize_map(const double* __restrict__ acc, float* __restrict__ map, long n, double scale, int accumulate) {
+__global__ void k_finalize_map(const float* __restrict__ acc, float* __restrict__ map, long n, double scale,
+                               int accumulate) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const float v = (float)(acc[i] * scale);
+    const float v = (float)((double)acc[i] * scale);
     map[i] = accumulate ? map[i] + v : v;
   }
 }
@@ -554,18 +571,14 @@ struct FlatArgs {
   float zlo, zhi;          // slab bounds in grid units
 };
 __global__ void __launch_bounds__(256) k_paint_flat(PartArgs a, const float* __restrict__ w, float w0, FlatArgs fa,
-                                                    double* __restrict__ acc, int rx, int ry, int rz) {
-  const size_t np = (size_t)a.npx * a.npy * a.npz;
+                                                    float* __restrict__ acc, int rx, int ry, int rz) {
+  const unsigned np = (unsigned)a.npx * a.npy * a.npz;
   const Chunk ch = my_chunk(np);
-  for (size_t p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x) {
+  LatticeWalk lw;
+  lw.init(ch.begin + threadIdx.x, blockDim.x, a.npy, a.npz);
+  for (unsigned p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x, lw.step()) {
     float3 x = ld3(a.x, p);
-    if (a.mode == 0) {
-      const int k = (int)(p % a.npz);
-      const size_t q2 = p / a.npz;
-      const int j = (int)(q2 % a.npy);
-      const int i = (int)(q2 / a.npy);
-      x = make_float3(x.x + (float)(i * rx), x.y + (float)(j * ry), x.z + (float)(k * rz));
-    }
+    if (a.mode == 0) x = make_float3(x.x + (float)(lw.i * rx), x.y + (float)(lw.j * ry), x.z + (float)(lw.k * rz));
     if (!(x.z > fa.zlo && x.z <= fa.zhi)) continue;
     const float wt = w ? w[p] : w0;
     float px = x.x * fa.sxf, py = x.y * fa.syf;
@@ -576,7 +589,7 @@ __global__ void __launch_bounds__(256) k_paint_flat(PartArgs a, const float* __r
     for (int o = 0; o < 4; ++o) {
       const int ox = o & 1, oy = o >> 1;
       const float wv = ((ox ? cx.w1 : cx.w0) * (oy ? cy.w1 : cy.w0)) * wt;
-      atomicAdd(acc + (size_t)(ox ? cx.i1 : cx.i0) * fa.fny + (oy ? cy.i1 : cy.i0), (double)wv);
+      unsafeAtomicAdd(acc + (size_t)(ox ? cx.i1 : cx.i0) * fa.fny + (oy ? cy.i1 : cy.i0), wv);
     }
   }
 }
@@ -678,7 +691,7 @@ int pmx_launch_paint(pmx_plan_t* p, const PartArgs& a_in, const float* w, float 
   ratios(p, a, rx, ry, rz);
   const size_t np = (size_t)a.npx * a.npy * a.npz;
   if (a.v != nullptr) {
-    hipLaunchKernelGGL(k_drift, dim3(grid_particles(3 * np)), dim3(256), 0, s, a.x, a.v, 3 * np, cd_pre);
+    hipLaunchKernelGGL(k_drift, dim3(grid_particles(3 * np)), dim3(256), 0, s, a.x, a.v, (unsigned)(3 * np), cd_pre);
     int rc = pmx_check_launch("drift");
     if (rc) return rc;
     a.v = nullptr;
@@ -688,8 +701,8 @@ int pmx_launch_paint(pmx_plan_t* p, const PartArgs& a_in, const float* w, float 
       int rc = pmx_launch_fill(mesh, p->ncell, 0.f, s);
       if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_paint, dim3(grid_particles(np)), dim3(256), 0, s, a, w, w0, 0.f, mesh, p->nx, p->ny, p->nz, rx,
-                       ry, rz);
+    hipLaunchKernelGGL(k_paint, dim3(grid_particles(np)), dim3(256), 0, s, a, w, w0, mesh, p->nx, p->ny, p->nz, rx, ry,
+                       rz);
     return pmx_check_launch("paint");
   }
   const int BX = p->nx < 16 ? p->nx : 16, BY = p->ny < 16 ? p->ny : 16, BZ = p->nz < 32 ? p->nz : 32;
@@ -710,8 +723,12 @@ int pmx_launch_paint(pmx_plan_t* p, const PartArgs& a_in, const float* w, float 
     p->esc_tiles = ntiles;
   }
   p->esc_ntiles_last = ntiles;
-  hipLaunchKernelGGL(k_paint_tiles, dim3(ntiles), dim3(256), 0, s, a, w, w0, mesh, p->nx, p->ny, p->nz, rx, ry, rz, BX,
-                     BY, BZ, PAINT_HALO, accumulate, p->esc_count, p->esc_list, cap);
+  if (rx == 1 && ry == 1 && rz == 1)
+    hipLaunchKernelGGL(k_paint_tiles<true>, dim3(ntiles), dim3(512), 0, s, a, w, w0, mesh, p->nx, p->ny, p->nz, rx, ry,
+                       rz, BX, BY, BZ, PAINT_HALO, accumulate, p->esc_count, p->esc_list, cap);
+  else
+    hipLaunchKernelGGL(k_paint_tiles<false>, dim3(ntiles), dim3(512), 0, s, a, w, w0, mesh, p->nx, p->ny, p->nz, rx, ry,
+                       rz, BX, BY, BZ, PAINT_HALO, accumulate, p->esc_count, p->esc_list, cap);
   int rc = pmx_check_launch("paint_tiles");
   if (rc) return rc;
   hipLaunchKernelGGL(k_paint_escapes, dim3(ntiles), dim3(256), 0, s, a, w, w0, mesh, p->nx, p->ny, p->nz, rx, ry, rz,
@@ -734,7 +751,7 @@ int pmx_launch_readout(pmx_plan_t* p, const PartArgs& a, const float* mesh, int 
 }
 
 int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, const double box[3],
-                       const double obs[3], double rmin, double rmax, int nside, int scheme, double* acc,
+                       const double obs[3], double rmin, double rmax, int nside, int scheme, float* acc,
                        hipStream_t s) {
   int rx, ry, rz;
   ratios(p, a, rx, ry, rz);
@@ -758,13 +775,13 @@ int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w
   return pmx_check_launch("paint_healpix");
 }
 
-int pmx_launch_finalize_map(const double* acc, float* map, long n, double scale, int accumulate, hipStream_t s) {
+int pmx_launch_finalize_map(const float* acc, float* map, long n, double scale, int accumulate, hipStream_t s) {
   hipLaunchKernelGGL(k_finalize_map, dim3(grid_for(n, 256)), dim3(256), 0, s, acc, map, n, scale, accumulate);
   return pmx_check_launch("finalize_map");
 }
 
 int pmx_launch_flat(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, int fnx, int fny, float zlo, float zhi,
-                    double* acc, hipStream_t s) {
+                    float* acc, hipStream_t s) {
   int rx, ry, rz;
   ratios(p, a, rx, ry, rz);
   FlatArgs fa;
