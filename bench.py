@@ -75,9 +75,10 @@ def phase_bytes(n_p, n_c, nspec):
         "paint": 12 * n_p + 4 * n_c,
         "r2c_z": 4 * n_c + 8 * nspec,
         "c2c_y": 16 * nspec,
-        "kspace_x": 8 * nspec + 24 * nspec,
-        "c2c_y_inv": 48 * nspec,
-        "c2r_z": 24 * nspec + 12 * n_c,
+        "kspace_x": 16 * nspec,
+        "c2c_y_inv": 16 * nspec,
+        "c2r_z": 8 * nspec + 4 * n_c,
+        "grad4": 4 * n_c + 12 * n_c,
         "readout_kick_drift": 48 * n_p + 12 * n_c,
     }
 

@@ -45,10 +45,10 @@ size_t pmx_plan_workspace_bytes(pmx_plan plan);
 int pmx_plan_escapes(pmx_plan plan, uint64_t* count);
 /* Per-kernel timing of pmx_step on this plan. enable=1 clears previous records and brackets every
  * following pmx_step with HIP events on its stream; pmx_plan_timing_read synchronises on the last
- * event and returns the summed milliseconds of the 7 phases (paint, r2c_z, c2c_y, k-space x pass,
- * inverse c2c_y, c2r_z, readout+kick+drift) and the number of steps recorded. */
+ * event and returns the summed milliseconds of the 8 phases (paint, r2c_z, c2c_y, k-space x pass,
+ * inverse c2c_y, c2r_z, gradient stencil, readout+kick+drift) and the number of steps recorded. */
 int pmx_plan_timing(pmx_plan plan, int enable);
-int pmx_plan_timing_read(pmx_plan plan, double* ms7, int* nsteps);
+int pmx_plan_timing_read(pmx_plan plan, double* ms8, int* nsteps);
 /* Physical box (Mpc/h) for the k tables of pmx_power and pmx_color_field (synchronous upload). */
 int pmx_plan_set_box(pmx_plan plan, const double box[3]);
 
@@ -74,12 +74,13 @@ int pmx_fft_c2r(pmx_plan plan, const void* spec, float* out, void* stream);
  * (result kept in the plan). */
 int pmx_force_mesh(pmx_plan plan, const float* density, float scale, unsigned flags, float* fmesh, void* stream);
 
-/* Particle forces pm_forces(pos) * scale: paint, force mesh, readout. Replaces _forces
+/* Particle forces pm_forces(pos) * scale: paint, potential (one inverse FFT), 4-point gradient
+ * stencil, readout. Replaces _forces
  * (src/jax_fli/pm/solvers.py:19-32) with scale = 1.5 * Omega_m. forces: (npx,npy,npz,3). */
 int pmx_forces(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, float scale, unsigned flags,
                float* forces, void* stream);
 
-/* Fused PM step on (x, v) in place:
+/* Fused PM step on (x, v) in place (force as in pmx_forces):
  *   [PMX_DRIFT_PRE]  x += v * c_drift_pre
  *   v += (pm_forces(x) * force_scale) * c_kick
  *   [PMX_DRIFT_POST] x += v * c_drift_post

@@ -126,7 +126,7 @@ class Plan:
             check(load().pmx_plan_escapes(self.handle, ctypes.byref(c)), "pmx_plan_escapes")
         return int(c.value)
 
-    PHASES = ("paint", "r2c_z", "c2c_y", "kspace_x", "c2c_y_inv", "c2r_z", "readout_kick_drift")
+    PHASES = ("paint", "r2c_z", "c2c_y", "kspace_x", "c2c_y_inv", "c2r_z", "grad4", "readout_kick_drift")
 
     def timing(self, enable: bool) -> None:
         with torch.cuda.device(self.device):
@@ -134,7 +134,7 @@ class Plan:
 
     def timing_read(self):
         """(phase -> summed ms, number of steps) of the pmx_step calls since timing(True)."""
-        ms = (ctypes.c_double * 7)()
+        ms = (ctypes.c_double * len(self.PHASES))()
         n = ctypes.c_int()
         with torch.cuda.device(self.device):
             check(load().pmx_plan_timing_read(self.handle, ms, ctypes.byref(n)), "pmx_plan_timing_read")

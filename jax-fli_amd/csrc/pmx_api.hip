@@ -23,8 +23,6 @@ int pmx_launch_lpt1(const float* F, float* dx, float* pv, size_t n, float D, flo
 namespace {
 thread_local std::string g_err;
 
-// k-space modes of the x pass (must match the enum in pmx_fft.hip)
-enum { XP_FWD = 0, XP_INV = 1, XP_FORCES = 2, XP_COLOR = 3, XP_POWER = 4 };
 
 #define PMX_HIP(call)                                                                        \
   do {                                                                                       \
@@ -144,6 +142,28 @@ int force_mesh_from(pmx_plan_t* p, const float* src, float scale, unsigned flags
   PMX_TRY(pmx_pass_c2c_y_inv(p, 3, p->specC, p->specD, s));
   mark(m, 5, s);
   return pmx_pass_c2r_z(p, 3, p->specD, out, s);
+}
+
+// Force mesh through the potential: phi = ifft(delta_hat * force_scale / (12 k^2)) (one inverse
+// FFT instead of three), then F = 8 (phi[+1] - phi[-1]) - (phi[+2] - phi[-2]) per axis, which is the
+// reference's k-space gradient i (8 sin k - sin 2k) / 6 exactly. `src` may be p->rho (phi reuses it).
+int pot_force_mesh_from(pmx_plan_t* p, const float* src, float force_scale, unsigned flags, float* out, hipStream_t s,
+                        StepMarks* m = nullptr) {
+  PMX_TRY(pmx_pass_r2c_z(p, src, p->specA, s));
+  mark(m, 2, s);
+  PMX_TRY(pmx_pass_c2c_y(p, p->specA, p->specB, s));
+  mark(m, 3, s);
+  KOp op{};
+  op.mode = XP_POT;
+  op.scale = (float)((double)force_scale / (12.0 * (double)p->ncell));
+  op.flags = flags & PMX_DECONV;
+  PMX_TRY(pmx_xpass(p, op, p->specB, p->specC, s));
+  mark(m, 4, s);
+  PMX_TRY(pmx_pass_c2c_y_inv(p, 1, p->specC, p->specD, s));
+  mark(m, 5, s);
+  PMX_TRY(pmx_pass_c2r_z(p, 1, p->specD, p->rho, s));
+  mark(m, 6, s);
+  return pmx_grad4(p, p->rho, out, s);
 }
 
 StepMarks* new_marks(pmx_plan_t* p) {
@@ -334,7 +354,7 @@ int pmx_forces(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode
   hipStream_t s = (hipStream_t)stream;
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
   PMX_TRY(pmx_launch_paint(p, a, nullptr, 1.f, 0.f, p->rho, 0, ((p->flags | flags) & PMX_PAINT_ATOMIC) != 0, s));
-  PMX_TRY(force_mesh_from(p, p->rho, scale, flags, p->fmesh, s));
+  PMX_TRY(pot_force_mesh_from(p, p->rho, scale, flags, p->fmesh, s));
   return pmx_launch_readout(p, a, p->fmesh, 3, forces, 0.f, 0.f, 1, s);
 }
 
@@ -349,12 +369,12 @@ int pmx_step(pmx_plan p, float* x, float* v, int npx, int npy, int npz, int mode
   PartArgs a = part_args(x, (flags & PMX_DRIFT_PRE) ? v : nullptr, npx, npy, npz, mode, p);
   PMX_TRY(pmx_launch_paint(p, a, nullptr, 1.f, c_drift_pre, p->rho, 0, ((p->flags | flags) & PMX_PAINT_ATOMIC) != 0, s));
   mark(m, 1, s);
-  PMX_TRY(force_mesh_from(p, p->rho, force_scale, flags, p->fmesh, s, m));
-  mark(m, 6, s);
+  PMX_TRY(pot_force_mesh_from(p, p->rho, force_scale, flags, p->fmesh, s, m));
+  mark(m, 7, s);
   a.v = v;
   const int what = 2 | ((flags & PMX_DRIFT_POST) ? 4 : 0);
   PMX_TRY(pmx_launch_readout(p, a, p->fmesh, 3, nullptr, c_kick, c_drift_post, what, s));
-  mark(m, 7, s);
+  mark(m, 8, s);
   return 0;
 }
 
@@ -369,7 +389,7 @@ int pmx_plan_timing(pmx_plan p, int enable) {
 int pmx_plan_timing_read(pmx_plan p, double* ms, int* nsteps) {
   PMX_TRY(check_plan(p));
   if (!ms || !nsteps) return pmx_set_error(1, "null output");
-  for (int i = 0; i < kStepMarks - 1; ++i) ms[i] = 0.0;
+  for (int i = 0; i < kStepMarks - 1; ++i) ms[i] = 0.0;  // kStepMarks - 1 phases
   *nsteps = 0;
   if (!p->marks || p->marks->empty()) return 0;
   PMX_HIP(hipEventSynchronize(p->marks->back().e[kStepMarks - 1]));

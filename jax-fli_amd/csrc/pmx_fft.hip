@@ -226,7 +226,6 @@ __global__ void __launch_bounds__(256) k_c2c_lines(const float2* __restrict__ in
 }
 
 // ---------------------------------------------------------------- P3: x lines + k-space operator
-enum { XP_FWD = 0, XP_INV = 1, XP_FORCES = 2, XP_COLOR = 3, XP_POWER = 4 };
 
 struct XTabs {
   const float *gx, *gy, *gz, *w2x, *w2y, *w2z, *dx, *dy, *dz;
@@ -303,6 +302,24 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
         if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + k];
       }
       __syncthreads();
+    }
+  } else if constexpr (MODE == XP_POT) {
+    // potential for the real-space FastPM gradient: phi_hat = delta_hat * scale / k^2 (0 at k = 0)
+    for (int i = tid; i < L * N; i += NT) {
+      const int ll = i / N, kx = i - ll * N;
+      const int ky = min(ky0 + ll, NY - 1);
+      const float k2 = tb.w2x[kx] + tb.w2y[ky] + tb.w2z[kz];
+      float coef = (k2 > 0.f) ? op.scale / k2 : 0.f;
+      if (op.flags & 1u) coef *= tb.dx[kx] * tb.dy[ky] * tb.dz[kz];
+      res[ll * LS + kx] = cscale(res[ll * LS + kx], coef);
+    }
+    __syncthreads();
+    fft_run<N, true>(res + l * LS, oth + l * LS, t, tw);
+    const float2* r2 = res_in_b<N>() ? oth : res;
+    float2* dst = out + (size_t)kz * N * NY + ky0;
+    for (int i = tid; i < N * L; i += NT) {
+      const int k = i / L, ll = i - k * L;
+      if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + k];
     }
   } else if constexpr (MODE == XP_COLOR) {
     for (int i = tid; i < L * N; i += NT) {
@@ -398,6 +415,34 @@ __global__ void __launch_bounds__(384) k_c2r_z(const float2* __restrict__ inD, f
   }
 }
 
+// ---------------------------------------------------------------- force mesh from the potential
+// F_d = 8 (phi[+e_d] - phi[-e_d]) - (phi[+2 e_d] - phi[-2 e_d]) per cell, written interleaved (x,y,z,3):
+// the real-space form of the FastPM gradient i (8 sin k - sin 2k) / 6 (the 1/12 is folded into the
+// potential). One workgroup per (x, y) row; rows are remapped so each XCD streams one x slab and
+// the x +- 2 / y +- 2 neighbour rows come from its L2.
+__global__ void __launch_bounds__(256) k_grad4(const float* __restrict__ phi, float* __restrict__ F, int NX, int NY,
+                                               int NZ) {
+  const unsigned G = gridDim.x, b = blockIdx.x;
+  const unsigned row = (G % 8u == 0u) ? (b % 8u) * (G / 8u) + b / 8u : b;
+  const int x = (int)(row / (unsigned)NY), y = (int)(row % (unsigned)NY);
+  auto wr = [](int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); };
+  const size_t r0 = (size_t)row * NZ;
+  const size_t xp1 = ((size_t)wr(x + 1, NX) * NY + y) * NZ, xm1 = ((size_t)wr(x - 1, NX) * NY + y) * NZ;
+  const size_t xp2 = ((size_t)wr(x + 2, NX) * NY + y) * NZ, xm2 = ((size_t)wr(x - 2, NX) * NY + y) * NZ;
+  const size_t yp1 = ((size_t)x * NY + wr(y + 1, NY)) * NZ, ym1 = ((size_t)x * NY + wr(y - 1, NY)) * NZ;
+  const size_t yp2 = ((size_t)x * NY + wr(y + 2, NY)) * NZ, ym2 = ((size_t)x * NY + wr(y - 2, NY)) * NZ;
+  float* out = F + r0 * 3;
+  for (int z = threadIdx.x; z < NZ; z += blockDim.x) {
+    const float gx = 8.0f * (phi[xp1 + z] - phi[xm1 + z]) - (phi[xp2 + z] - phi[xm2 + z]);
+    const float gy = 8.0f * (phi[yp1 + z] - phi[ym1 + z]) - (phi[yp2 + z] - phi[ym2 + z]);
+    const float gz = 8.0f * (phi[r0 + wr(z + 1, NZ)] - phi[r0 + wr(z - 1, NZ)]) -
+                     (phi[r0 + wr(z + 2, NZ)] - phi[r0 + wr(z - 2, NZ)]);
+    out[3 * z] = gx;
+    out[3 * z + 1] = gy;
+    out[3 * z + 2] = gz;
+  }
+}
+
 // ---------------------------------------------------------------- host dispatch
 #define PMX_FOR_AXIS(M_) \
   M_(8) M_(16) M_(32) M_(64) M_(96) M_(128) M_(192) M_(256) M_(512) M_(1024)
@@ -484,6 +529,7 @@ int pmx_xpass(pmx_plan_t* p, const KOp& op, const float2* inB, float2* out, hipS
       case XP_FORCES: return launch_xpass<v, XP_FORCES>(p, op, inB, out, s); \
       case XP_COLOR: return launch_xpass<v, XP_COLOR>(p, op, inB, out, s);  \
       case XP_POWER: return launch_xpass<v, XP_POWER>(p, op, inB, out, s);  \
+      case XP_POT: return launch_xpass<v, XP_POT>(p, op, inB, out, s);      \
     }                                                                       \
     return pmx_set_error(2, "bad k-space mode");
     PMX_FOR_AXIS(M_)
@@ -516,6 +562,13 @@ int pmx_pass_c2r_z(pmx_plan_t* p, int ncomp, const float2* inD, float* out, hipS
 #undef M_
   }
   return pmx_set_error(2, "unsupported nz");
+}
+
+int pmx_grad4(pmx_plan_t* p, const float* phi, float* F, hipStream_t s) {
+  const unsigned rows = (unsigned)p->nx * p->ny;
+  const int threads = p->nz < 256 ? p->nz : 256;
+  hipLaunchKernelGGL(k_grad4, dim3(rows), dim3(threads), 0, s, phi, F, p->nx, p->ny, p->nz);
+  return pmx_check_launch("grad4");
 }
 
 int pmx_inverse_from_C(pmx_plan_t* p, int ncomp, const float2* inC, float* out, hipStream_t s) {

@@ -13,8 +13,8 @@
 #include <string>
 #include <vector>
 
-// per-step timing marks of pmx_step (paint | r2c_z | c2c_y | xpass | c2c_y_inv | c2r_z | readout)
-constexpr int kStepMarks = 8;
+// per-step timing marks of pmx_step (paint | r2c_z | c2c_y | xpass | c2c_y_inv | c2r_z | grad4 | readout)
+constexpr int kStepMarks = 9;
 struct StepMarks {
   hipEvent_t e[kStepMarks];
 };
@@ -57,6 +57,10 @@ int pmx_check_launch(const char* what);
 // supported line lengths for the in-LDS Stockham FFT
 bool pmx_fft_len_ok(int n);
 
+// k-space operators of the x pass: forward only, inverse only, 3 force components (Green x FastPM
+// gradient), colouring by sqrt(P(k)), power-spectrum binning, potential delta/k^2
+enum { XP_FWD = 0, XP_INV = 1, XP_FORCES = 2, XP_COLOR = 3, XP_POWER = 4, XP_POT = 5 };
+
 // FFT passes (pmx_fft.hip). All asynchronous on `stream`. Return 0 or an error code.
 struct KOp {
   int mode;               // see enum in pmx_fft.hip
@@ -85,6 +89,8 @@ int pmx_pass_r2c_z(pmx_plan_t* p, const float* in, float2* outA, hipStream_t s);
 int pmx_pass_c2c_y(pmx_plan_t* p, const float2* inA, float2* outB, hipStream_t s);
 int pmx_pass_c2c_y_inv(pmx_plan_t* p, int ncomp, const float2* inC, float2* outD, hipStream_t s);
 int pmx_pass_c2r_z(pmx_plan_t* p, int ncomp, const float2* inD, float* out, hipStream_t s);
+// force mesh (x,y,z,3) from the potential by the FastPM 4-point stencil
+int pmx_grad4(pmx_plan_t* p, const float* phi, float* F, hipStream_t s);
 
 // particle kernels (pmx_particles.hip)
 struct PartArgs {
@@ -102,3 +108,4 @@ int pmx_launch_paint(pmx_plan_t* p, const PartArgs& a, const float* w, float w0,
 int pmx_launch_fill(float* a, size_t n, float v, hipStream_t s);
 int pmx_launch_readout(pmx_plan_t* p, const PartArgs& a, const float* mesh, int ncomp, float* out,
                        float ck, float cd_post, int what, hipStream_t s);
+
