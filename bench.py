@@ -30,6 +30,19 @@ sys.path.insert(0, ROOT)
 METRIC = "PM steps/sec @256^3 fused vs plain GPU vs CPU; fwd+VJP cost; served p99 latency"
 N, BOX, A0, DT0, NSIDE, NSHELL, ZMAX = 256, 1000.0, 0.1, 0.05, 256, 4, 0.5
 PEAK_HBM = 8.0e12
+CAL_FLOATS = 128 << 20  # calibration copy for the HBM-traffic counters: 512 MiB read + 512 MiB written
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pm_step_256cubed_pmc_traffic.txt")
+# kernels of each pmx_step phase (substring of the demangled name), for the rocprofv3 counter summary
+PHASE_KERNELS = {
+    "paint": ("k_paint_tiles", "k_paint_escapes"),
+    "r2c_z": ("k_r2c_z",),
+    "c2c_y": ("k_c2c_lines<256, false>",),
+    "kspace_x": ("k_xpass<256, 5>",),
+    "c2c_y_inv": ("k_c2c_lines<256, true>",),
+    "c2r_z": ("k_c2r_z<256, 1>",),
+    "grad4": ("k_grad4",),
+    "readout_kick_drift": ("k_readout<3>",),
+}
 
 
 def build_schedule(pkg, cosmo):
@@ -83,13 +96,66 @@ def phase_bytes(n_p, n_c, nspec):
     }
 
 
+def _pmc_per_kernel(csv_path):
+    """{kernel name: (dispatches, mean counter value)} from a rocprofv3 counter_collection.csv."""
+    import csv
+
+    acc = {}
+    with open(csv_path) as fh:
+        for r in csv.DictReader(fh):
+            n, tot = acc.get(r["Kernel_Name"], (0, 0.0))
+            acc[r["Kernel_Name"]] = (n + 1, tot + float(r["Counter_Value"]))
+    return {k: (n, tot / n) for k, (n, tot) in acc.items()}
+
+
+def pmc_summary(fetch_csv, write_csv):
+    """HBM traffic per launch of each step phase from one FETCH_SIZE run and one WRITE_SIZE run (KiB per
+    dispatch), each scaled by what the same counter reads for the calibration copy (torch.neg over
+    CAL_FLOATS floats: exactly 4*CAL_FLOATS bytes read and written)."""
+    fetch, write = _pmc_per_kernel(fetch_csv), _pmc_per_kernel(write_csv)
+    cal = [k for k in fetch if "neg_kernel" in k]
+    if not cal:
+        raise SystemExit("calibration kernel (neg_kernel) not found in the counter file")
+    cal_bytes = 4.0 * CAL_FLOATS
+    f_scale = cal_bytes / (fetch[cal[0]][1] * 1024.0)
+    w_scale = cal_bytes / (write[cal[0]][1] * 1024.0)
+    phases = {}
+    for ph, names in PHASE_KERNELS.items():
+        fb = wb = 0.0
+        found = False
+        for nm in names:
+            for kf, (n, val) in fetch.items():
+                if nm in kf:
+                    fb += val * 1024.0 * f_scale
+                    wb += write.get(kf, (0, 0.0))[1] * 1024.0 * w_scale
+                    found = True
+        if found:
+            phases[ph] = {"fetch_bytes": round(fb), "write_bytes": round(wb), "traffic_bytes": round(fb + wb)}
+    return {"fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4), "phases": phases}
+
+
+def read_traffic(phase):
+    """HBM bytes per launch of `phase` from the committed counter summary (its last line is JSON)."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            last = [ln for ln in fh if ln.strip()][-1]
+        return json.loads(last)["phases"][phase]["traffic_bytes"]
+    except (OSError, KeyError, ValueError, IndexError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed PM steps (0: one full schedule)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=1, help="CPU-baseline steps at full size (0: skip)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="print the per-phase HBM traffic from two rocprofv3 --pmc runs of this script and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(*args.pmc_summary)))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (this engine's multi-GPU "
               "decomposition is a design, not a runnable path)", file=sys.stderr)
@@ -189,6 +255,45 @@ def main():
     per_phase = {k: {"avg_us": round(phases[k] / max(nrec, 1) * 1e3, 2),
                      "GB_per_s": round(pb[k] / (phases[k] / max(nrec, 1) * 1e-3) / 1e9, 1)} for k in phases}
 
+    # calibration copy for the traffic counters (one dispatch of a known 512 MiB read + 512 MiB write)
+    cal = torch.ones(CAL_FLOATS, device=dev)
+    torch.neg(cal, out=torch.empty_like(cal))
+    del cal
+
+    # unfused per-op GPU form of the same step (rocFFT c2c x4, 3 readouts, separate kick/drift)
+    from importlib import import_module
+
+    unf = import_module("jax_fli_amd.pm.unfused").UnfusedPM(mesh, dev)
+    f_unf = unf.forces(x0, force_scale)
+    f_fused = pkg.engine.forces(x0, mesh, scale=force_scale)
+    force_err = float((f_unf - f_fused).norm() / f_unf.norm())
+    xu, vu = x0.clone(), v0.clone()
+    sched = [it[1] for it in ops if it[0] == "op"]
+
+    def time_schedule(step_fn):
+        """ms per step of the schedule's PM steps (shell paints left out) from the initial state."""
+        xu.copy_(x0)
+        vu.copy_(v0)
+        torch.cuda.synchronize()
+        a_, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a_.record()
+        for o in sched:
+            step_fn(o)
+        b_.record()
+        torch.cuda.synchronize()
+        return a_.elapsed_time(b_) / len(sched)
+
+    def unfused_step(o):
+        unf.step(xu, vu, force_scale, o.kick, o.post or 0.0, c_drift_pre=o.pre)
+
+    def fused_step(o):
+        pkg.engine.step(xu, vu, mesh, force_scale=force_scale, c_kick=o.kick, c_drift_pre=o.pre, c_drift_post=o.post)
+
+    time_schedule(unfused_step)
+    unf_ms = time_schedule(unfused_step)
+    fused_ms = time_schedule(fused_step)
+    del xu, vu, unf, f_unf, f_fused
+
     # rocFFT beside the hand-written FFT (r2c + c2r round trip of one 256^3 mesh)
     def t_gpu(fn, reps=10):
         fn()
@@ -241,13 +346,18 @@ def main():
                    "mesh": N, "particles": N**3, "box_mpc_h": BOX, "dt0": DT0, "schedule_steps": nsteps_sched,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4), "traffic": read_traffic(dom),
                      "algorithmic_bytes_per_launch": pb[dom], "avg_launch_us": round(avg_ms * 1e3, 2)},
         "step_roofline": {"algorithmic_bytes_per_step": step_bytes,
                           "achieved_GB_per_s": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
                           "frac": round(step_bytes / (ms_per_step * 1e-3) / PEAK_HBM, 4)},
         "phases": per_phase,
         "cpu_baseline": cpu,
+        "unfused_gpu": {"ms_per_step": round(unf_ms, 4), "fused_ms_per_step": round(fused_ms, 4),
+                        "speedup": round(unf_ms / fused_ms, 2), "force_rel_l2_vs_fused": force_err,
+                        "form": "atomic scatter paint + rocFFT c2c fwd + 3 x (k-space multiply, rocFFT c2c inv, "
+                                "1-component readout) + separate kick and drift (torch ops); both forms timed over "
+                                "the schedule's PM steps without the shell paints"},
         "forward_total_ms": round(forward_ms, 1),
         "fft_roundtrip_ms": {"pmx": round(fft_ours, 4), "rocfft": round(fft_roc, 4)},
         "final_pk": {"k": [round(float(a), 5) for a in k_fin[:8]], "pk": [round(float(a), 3) for a in pk_fin[:8]]},
