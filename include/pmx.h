@@ -29,8 +29,8 @@ typedef struct pmx_plan_t* pmx_plan;
 #define PMX_DECONV 1u      /* divide the force spectrum by the CIC window of paint and readout */
 #define PMX_DRIFT_PRE 2u   /* pmx_step: x += v * c_drift_pre before painting */
 #define PMX_DRIFT_POST 4u  /* pmx_step: x += v * c_drift_post after the kick */
-#define PMX_PAINT_ATOMIC 8u /* paint relative-mode particles with the one-pass atomic scatter instead of
-                               the owner-computes tiles (plan flag or per-call flag) */
+#define PMX_PAINT_ATOMIC 8u /* paint with the one-pass global-atomic scatter instead of the sorted
+                               owner-computes tiles (plan flag or per-call flag) */
 
 int pmx_abi_version(void);
 const char* pmx_last_error(void);
@@ -40,8 +40,9 @@ int pmx_fft_supported(int n);
 int pmx_plan_create(pmx_plan* plan, int nx, int ny, int nz, unsigned flags);
 int pmx_plan_destroy(pmx_plan plan);
 size_t pmx_plan_workspace_bytes(pmx_plan plan);
-/* Particles the last tiled paint on this plan sent to the atomic escape pass (their displacement
- * exceeded the tile halo). Synchronises the device. */
+/* Entries of the last sorted paint on this plan that fell outside their particle sub-block's
+ * 5x5x5-tile window and were counted and placed one by one with global atomics (particles displaced
+ * by more than about two tiles). Synchronises the device. */
 int pmx_plan_escapes(pmx_plan plan, uint64_t* count);
 /* Per-kernel timing of pmx_step on this plan. enable=1 clears previous records and brackets every
  * following pmx_step with HIP events on its stream; pmx_plan_timing_read synchronises on the last

@@ -238,6 +238,17 @@ int pmx_plan_create(pmx_plan* out, int nx, int ny, int nz, unsigned flags) {
   if ((rc = dev_alloc(&p->kphys_x, nx, tot)) || (rc = dev_alloc(&p->kphys_y, ny, tot)) ||
       (rc = dev_alloc(&p->kphys_z, p->nzh, tot)))
     return fail(rc);
+  int sx, sy, sz, ntiles;
+  pmx_paint_tiles(p, sx, sy, sz, ntiles);
+  if ((rc = dev_alloc(&p->ps_count, ntiles, tot)) || (rc = dev_alloc(&p->ps_offset, ntiles, tot)) ||
+      (rc = dev_alloc(&p->ps_cursor, ntiles, tot)) || (rc = dev_alloc(&p->ps_misc, 4, tot)) ||
+      (rc = dev_alloc(&p->ps_scale, 2, tot)) ||
+      (rc = dev_alloc(&p->ps_faces, (size_t)ntiles * pmx_paint_face_cells(p), tot)))
+    return fail(rc);
+  // the paint's count pass accumulates into zeroed counters; its deposit pass re-zeroes them
+  if (hipMemset(p->ps_count, 0, ntiles * sizeof(unsigned)) != hipSuccess ||
+      hipMemset(p->ps_misc, 0, 4 * sizeof(unsigned)) != hipSuccess)
+    return fail(pmx_set_error(3, "plan: clearing paint counters failed"));
   p->workspace_bytes = tot;
   std::vector<float> g, w2, dec;
   if ((rc = upload(p->tw_x, twiddles(nx))) || (rc = upload(p->tw_y, twiddles(ny))) || (rc = upload(p->tw_z, twiddles(nz))) ||
@@ -257,7 +268,8 @@ int pmx_plan_destroy(pmx_plan p) {
   if (!p) return 0;
   void* bufs[] = {p->tw_x, p->tw_y, p->tw_z, p->tw_zh, p->grad_x, p->grad_y, p->grad_z, p->w2_x, p->w2_y, p->w2_z,
                   p->dec_x, p->dec_y, p->dec_z, p->rho, p->specA, p->specB, p->specC, p->specD, p->fmesh, p->acc,
-                  p->kphys_x, p->kphys_y, p->kphys_z, p->esc_count, p->esc_list};
+                  p->kphys_x, p->kphys_y, p->kphys_z, p->ps_count, p->ps_offset, p->ps_cursor, p->ps_wgcnt,
+                  p->ps_seg, p->ps_misc, p->ps_scale, p->ps_faces};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   clear_marks(p);
@@ -271,12 +283,10 @@ size_t pmx_plan_workspace_bytes(pmx_plan p) { return p ? p->workspace_bytes : 0;
 int pmx_plan_escapes(pmx_plan p, uint64_t* count) {
   PMX_TRY(check_plan(p));
   if (!count) return pmx_set_error(1, "null output");
-  *count = 0;
-  if (!p->esc_count || !p->esc_ntiles_last) return 0;
-  std::vector<unsigned> c(p->esc_ntiles_last);
+  unsigned c = 0;
   PMX_HIP(hipDeviceSynchronize());
-  PMX_HIP(hipMemcpy(c.data(), p->esc_count, c.size() * sizeof(unsigned), hipMemcpyDeviceToHost));
-  for (unsigned v : c) *count += v;
+  PMX_HIP(hipMemcpy(&c, p->ps_misc + 2, sizeof(unsigned), hipMemcpyDeviceToHost));
+  *count = c;
   return 0;
 }
 

@@ -40,9 +40,14 @@ struct pmx_plan_t {
   float* fmesh;     // force mesh [x][y][z][3]
   double *kphys_x, *kphys_y, *kphys_z;  // estimator wavevectors (h/Mpc) per bin, set by pmx_plan_set_box
   bool box_set;
-  unsigned* esc_count;  // paint: per tile, particles displaced beyond the tile halo
-  unsigned* esc_list;   // paint: their indices, one segment per tile (esc_cap entries in total)
-  size_t esc_cap, esc_tiles, esc_ntiles_last;
+  // sorted paint (pmx_particles.hip): per paint tile particle counts (zero between paints), segment
+  // offsets, reservation cursors; per particle sub-block window counts; segments of particle
+  // indices; misc = {max |weight| bits, far entries, far entries of the last paint}; fixed-point
+  // scale {2^k, 2^-k}
+  unsigned *ps_count, *ps_offset, *ps_cursor, *ps_wgcnt, *ps_seg, *ps_misc;
+  float* ps_scale;
+  float* ps_faces;  // per tile: its upper face cells (pmx_paint_face_cells floats)
+  size_t ps_wg_cap, ps_seg_cap;
   double* acc;      // double accumulators (healpix maps, flat maps); grown on demand
   size_t acc_bytes;
   size_t workspace_bytes;
@@ -100,9 +105,9 @@ struct PartArgs {
   int mode;            // 0 relative (lattice + dx), 1 absolute
   float sx, sy, sz;    // lattice spacing in cells (mesh / particle grid)
 };
-// tiled paint: halo (cells) within which a particle's floor(displacement) must lie to be painted by
-// the owner-computes tiles; farther particles take the atomic escape pass
-constexpr int PAINT_HALO = 4;
+// paint tiles of a plan (log2 of the tile dims, tile count)
+void pmx_paint_tiles(const pmx_plan_t* p, int& sx, int& sy, int& sz, int& ntiles);
+int pmx_paint_face_cells(const pmx_plan_t* p);
 int pmx_launch_paint(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, float cd_pre,
                      float* mesh, int accumulate, int atomic, hipStream_t s);
 int pmx_launch_fill(float* a, size_t n, float v, hipStream_t s);

@@ -157,176 +157,456 @@ __global__ void __launch_bounds__(256) k_paint(PartArgs a, const float* __restri
   }
 }
 
-// Owner-computes CIC paint for relative mode (particles on their Lagrangian lattice): one
-// workgroup per mesh tile (BX, BY, BZ) accumulates in LDS every deposit that lands in its tile from
-// the particles whose lattice lies within the tile grown by H + 1 cells, then writes the tile with
-// plain stores -- every mesh cell is written exactly once, no global atomics and no zero fill.
-// A particle with |floor(dx)| > H along any axis ("escape") is skipped by all tiles; its home tile
-// records it in its own segment of the escape list (LDS counter, no global atomics) and
-// k_paint_escapes deposits those with atomics afterwards. Halo particles whose two corners per axis
-// miss the tile leave after one bounds test; particles whose 8 corners are all inside the tile
-// deposit without per-corner checks.
-constexpr int PT_MAX = 16 * 16 * 32;
-constexpr int PT_UNROLL = 4;
+// ------------------------------------------------------------------ sorted CIC paint
+// Counting sort of the particles by the mesh tile that holds their base cell (floor of the
+// position), then one workgroup per tile accumulates the 8 corners of its particles in an LDS copy
+// of the tile grown by one cell on each upper face, writes the tile's own cells with plain stores
+// and its upper face cells to a face buffer; a last pass adds every tile's face cells to the cells
+// of its upper neighbours. Every mesh cell is written by its owner, in a fixed order: no global
+// float atomics, no zero fill, bitwise deterministic.
+//   k_sort_count    [x += v c]; per-tile particle counts (LDS window histogram -> global counts)
+//   k_sort_scan     exclusive prefix of the counts -> segment offsets and reservation cursors
+//   k_sort_scatter  particle indices -> their tile's segment (per-workgroup bulk reservation)
+//   k_paint_deposit per tile: LDS int64 fixed-point accumulation, tile cells + face cells out
+//   k_paint_faces   tile cells on lower faces += the neighbours' face cells
+// Deposits are added as int64 multiples of 2^-k, k chosen so the largest possible deposit is
+// 2^30 (exact, order-independent sums); each deposit is truncated to that grid.
+//
+// Particles are read in Lagrangian sub-blocks of 8 x 8 x 64 lattice sites; a sub-block's tiles lie
+// within +-2 tiles of the tile of its first site, so they are histogrammed in a 5x5x5 LDS window
+// and reserved with one global atomic per window slot. Particles outside the window (displaced by
+// more than about two tiles) count and reserve one by one with global atomics.
+constexpr int SB_X = 8, SB_Y = 8, SB_Z = 64;  // particle sub-block (4096 sites, 256 threads x 16)
+constexpr int SORT_WIN = 5, SORT_SLOTS = SORT_WIN * SORT_WIN * SORT_WIN;
+constexpr int TILE_PAD_MAX = 17 * 17 * 33;    // padded tile cells (int64 LDS: 76 KiB)
 
-__device__ __forceinline__ int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
-__device__ __forceinline__ int wrap_near(int i, int n) {
-  if (i < 0) i += n;
-  else if (i >= n) i -= n;
-  if ((unsigned)i >= (unsigned)n) {
-    i %= n;
-    i += i < 0 ? n : 0;
-  }
-  return i;
+// Division by a runtime constant: q = floor(n / d) for all 32-bit n (round-up multiplier).
+struct FDiv {
+  unsigned m, d;
+  int s1, s2;
+};
+__device__ __forceinline__ unsigned fdiv(unsigned n, const FDiv& f) {
+  const unsigned t = __umulhi(f.m, n);
+  return (t + ((n - t) >> f.s1)) >> f.s2;
 }
 
-// Fixed-point LDS accumulation: deposits are added as int64 multiples of 2^-32 (exact for CIC
-// weights, order independent, so the painted mesh is bitwise deterministic) -- integer LDS atomics
-// also run several times faster than float ones. Before issuing, each particle's upper-z corner
-// values are handed to the next lane when that lane is its z-neighbour on the lattice with the
-// same base (x, y) and base z + 1, halving the LDS atomics on smooth displacement fields.
-constexpr float PT_FIX = 4294967296.0f;  // 2^32
-__device__ __forceinline__ unsigned long long fix64(float v) { return (unsigned long long)__float2ll_rn(v * PT_FIX); }
+struct SortGeom {
+  int NX, NY, NZ;        // mesh
+  int sx, sy, sz;        // log2 tile dims
+  int ntx, nty, ntz;     // tiles per axis
+  int rx, ry, rz;        // mesh / particle grid (relative mode)
+  int nbx, nby, nbz;     // particle sub-blocks per axis
+  int lpz, lpy;          // log2 of npz, npy when both are powers of two, else -1
+  FDiv div_npz, div_npy;
+};
 
-template <bool R1>
-__global__ void __launch_bounds__(512) k_paint_tiles(PartArgs a, const float* __restrict__ w, float w0,
-                                                     float* __restrict__ mesh, int NX, int NY, int NZ, int rx_,
-                                                     int ry_, int rz_, int BX, int BY, int BZ, int H, int accumulate,
-                                                     unsigned* __restrict__ esc_count, unsigned* __restrict__ esc_list,
-                                                     int esc_cap) {
-  __shared__ unsigned long long tile[PT_MAX];
-  const int rx = R1 ? 1 : rx_, ry = R1 ? 1 : ry_, rz = R1 ? 1 : rz_;
-  const int nty = NY / BY, ntz = NZ / BZ;
-  const unsigned L = xcd_logical(blockIdx.x, gridDim.x);
-  const int tz = L % ntz, ty = (L / ntz) % nty, tx = L / (ntz * nty);
-  const int T0x = tx * BX, T0y = ty * BY, T0z = tz * BZ;
-  const int ncell = BX * BY * BZ;
-  for (int i = threadIdx.x; i < ncell; i += blockDim.x) tile[i] = 0ull;
-  if (threadIdx.x == 0) esc_count[L] = 0u;
-  __syncthreads();
-  // lattice i*r in [T0 - 1 - H, T0 + B - 1 + H] (unwrapped)
-  const int ilo = -floordiv(-(T0x - 1 - H), rx), ihi = floordiv(T0x + BX - 1 + H, rx);
-  const int jlo = -floordiv(-(T0y - 1 - H), ry), jhi = floordiv(T0y + BY - 1 + H, ry);
-  const int klo = -floordiv(-(T0z - 1 - H), rz), khi = floordiv(T0z + BZ - 1 + H, rz);
-  const int ci = ihi - ilo + 1, cj = jhi - jlo + 1, ck = khi - klo + 1;
-  const float fH = (float)H;
-  const int BYZ = BY * BZ;
+// periodic wrap, one compare per side for the usual |b| < N, exact modulo otherwise
+__device__ __forceinline__ int wrap_cell(int b, int n) {
+  int c = b + (b < 0 ? n : 0);
+  c -= (c >= n ? n : 0);
+  if ((unsigned)c >= (unsigned)n) {
+    c = b % n;
+    c += c < 0 ? n : 0;
+  }
+  return c;
+}
+
+// lattice coordinates of particle p
+__device__ __forceinline__ void lattice_of(const PartArgs& a, const SortGeom& g, unsigned p, int& i, int& j, int& k) {
+  if (g.lpz >= 0) {
+    k = (int)(p & (unsigned)(a.npz - 1));
+    const unsigned q = p >> g.lpz;
+    j = (int)(q & (unsigned)(a.npy - 1));
+    i = (int)(q >> g.lpy);
+  } else {
+    const unsigned q = fdiv(p, g.div_npz);
+    k = (int)(p - q * (unsigned)a.npz);
+    const unsigned ii = fdiv(q, g.div_npy);
+    j = (int)(q - ii * (unsigned)a.npy);
+    i = (int)ii;
+  }
+}
+
+// Wrapped base cell of a particle at lattice (i, j, k) with displacement / position x, and the
+// fractional offsets. Relative mode: lattice * r + floor(dx), weights from dx - floor(dx) (the
+// displacement form of the reference's cic_paint_dx); absolute mode: floor(x).
+struct Base {
+  int cx, cy, cz;
+  float fx, fy, fz;
+};
+__device__ __forceinline__ Base base_of(const PartArgs& a, const SortGeom& g, int i, int j, int k, float3 x) {
+  const float flx = floorf(x.x), fly = floorf(x.y), flz = floorf(x.z);
+  int bx = (int)flx, by = (int)fly, bz = (int)flz;
+  if (a.mode == 0) {
+    bx += __mul24(i, g.rx);
+    by += __mul24(j, g.ry);
+    bz += __mul24(k, g.rz);
+  }
+  Base b;
+  b.cx = wrap_cell(bx, g.NX);
+  b.cy = wrap_cell(by, g.NY);
+  b.cz = wrap_cell(bz, g.NZ);
+  b.fx = x.x - flx;
+  b.fy = x.y - fly;
+  b.fz = x.z - flz;
+  return b;
+}
+
+// wrapped tile difference d in [-(n/2), n - n/2): a unique representative for every tile
+__device__ __forceinline__ int tdiff(int t, int c, int n) {
+  int d = t - c;
+  const int lo = -(n >> 1);
+  if (d < lo) d += n;
+  else if (d >= n + lo) d -= n;
+  return d;
+}
+
+// Sub-block b (logical) -> its origin on the particle grid.
+__device__ __forceinline__ void subblock_origin(const SortGeom& g, unsigned b, int& i0, int& j0, int& k0) {
+  const int bk = (int)(b % (unsigned)g.nbz);
+  const unsigned q = b / (unsigned)g.nbz;
+  const int bj = (int)(q % (unsigned)g.nby);
+  const int bi = (int)(q / (unsigned)g.nby);
+  i0 = bi * SB_X;
+  j0 = bj * SB_Y;
+  k0 = bk * SB_Z;
+}
+
+// Window centre: tile of the sub-block's first site (relative mode: its lattice cell; absolute
+// mode: that particle's base cell). Identical in k_sort_count and k_sort_scatter.
+__device__ __forceinline__ int3 window_centre(const PartArgs& a, const SortGeom& g, int i0, int j0, int k0) {
+  int bx, by, bz;
+  if (a.mode == 0) {
+    bx = i0 * g.rx;
+    by = j0 * g.ry;
+    bz = k0 * g.rz;
+  } else {
+    const unsigned p = ((unsigned)i0 * a.npy + j0) * a.npz + k0;
+    const Base b = base_of(a, g, 0, 0, 0, ld3(a.x, p));
+    bx = b.cx;
+    by = b.cy;
+    bz = b.cz;
+  }
+  return make_int3(bx >> g.sx, by >> g.sy, bz >> g.sz);
+}
+
+__device__ __forceinline__ unsigned slot_tile(const SortGeom& g, int3 c, int s) {
+  const int h = SORT_WIN / 2;
+  const int dz = s % SORT_WIN - h, dy = (s / SORT_WIN) % SORT_WIN - h, dx = s / (SORT_WIN * SORT_WIN) - h;
+  const int tx = wrap_idx(c.x + dx, g.ntx), ty = wrap_idx(c.y + dy, g.nty), tz = wrap_idx(c.z + dz, g.ntz);
+  return ((unsigned)tx * g.nty + ty) * g.ntz + tz;
+}
+
+// Wave-aggregated counter add: lanes with `on` add 1 to cnt[key]; each gets the counter's old value
+// plus its rank among the lanes sharing its key (one LDS atomic per distinct key in the wave).
+__device__ __forceinline__ unsigned wave_add(bool on, int key, unsigned* cnt) {
   const int lane = __lane_id();
-  // division-free walk over (ii, jj, kk), kk fastest, stride blockDim.x
-  const int stride = blockDim.x;
-  const int dk = stride % ck, drow = stride / ck;
-  const int dj = drow % cj, di = drow / cj;
-  int kk = threadIdx.x % ck;
-  int jj = (threadIdx.x / ck) % cj;
-  int ii = threadIdx.x / (ck * cj);
-  unsigned* my_esc = esc_list + (size_t)L * esc_cap;
-  // every lane of a wave runs the same number of iterations (shuffles below need full waves)
-  const int rows_total = ci * cj * ck;
-  const int wave0 = threadIdx.x - lane;
-  const int iters = (rows_total - wave0 + PT_UNROLL * stride - 1) / (PT_UNROLL * stride);
-  for (int it = 0; it < iters; ++it) {
-    unsigned pidx[PT_UNROLL];
-    int li[PT_UNROLL], lj[PT_UNROLL], lk[PT_UNROLL];
-    bool ok[PT_UNROLL];
+  unsigned long long pending = __ballot(on);
+  unsigned res = 0;
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const int lk = __builtin_amdgcn_readlane(key, leader);
+    const unsigned long long mask = __ballot(on && key == lk) & pending;
+    unsigned old = 0;
+    if (lane == leader) old = atomicAdd(&cnt[lk], (unsigned)__popcll(mask));
+    old = __builtin_amdgcn_readlane(old, leader);
+    if ((mask >> lane) & 1ull) res = old + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+    pending &= ~mask;
+  }
+  return res;
+}
+
+// Visit the particles of this thread in sub-block (i0, j0, k0) (4 loads in flight):
+// fn(valid, p, tile, slot) with slot = window slot of the tile or -1; called by every lane, so fn
+// may use wave-wide operations.
+template <class Fn>
+__device__ __forceinline__ void for_particles(const PartArgs& a, const SortGeom& g, int i0, int j0, int k0, int3 c,
+                                              Fn&& fn) {
+  constexpr int PER = (SB_X * SB_Y * SB_Z) / 256, U = 4, h = SORT_WIN / 2;
+  for (int r0 = 0; r0 < PER; r0 += U) {
+    unsigned ps[U];
+    bool vs[U];
+    int is[U], js[U], ks[U];
+    float3 xs[U];
 #pragma unroll
-    for (int u = 0; u < PT_UNROLL; ++u) {
-      ok[u] = ii < ci;
-      li[u] = ilo + ii;
-      lj[u] = jlo + jj;
-      lk[u] = klo + kk;
-      pidx[u] = ok[u] ? ((unsigned)wrap_near(li[u], a.npx) * a.npy + wrap_near(lj[u], a.npy)) * a.npz +
-                            wrap_near(lk[u], a.npz)
-                      : 0u;
-      kk += dk;
-      int carry = 0;
-      if (kk >= ck) { kk -= ck; carry = 1; }
-      jj += dj + carry;
-      carry = 0;
-      if (jj >= cj) { jj -= cj; carry = 1; }
-      ii += di + carry;
+    for (int u = 0; u < U; ++u) {
+      const int l = (r0 + u) * 256 + (int)threadIdx.x;
+      is[u] = i0 + (l >> 9);
+      js[u] = j0 + ((l >> 6) & 7);
+      ks[u] = k0 + (l & 63);
+      vs[u] = is[u] < a.npx && js[u] < a.npy && ks[u] < a.npz;
+      ps[u] = vs[u] ? ((unsigned)is[u] * a.npy + js[u]) * a.npz + ks[u] : 0u;
+      xs[u] = vs[u] ? ld3(a.x, ps[u]) : make_float3(0.f, 0.f, 0.f);
     }
-    float3 xv[PT_UNROLL];
 #pragma unroll
-    for (int u = 0; u < PT_UNROLL; ++u) xv[u] = ok[u] ? ld3(a.x, pidx[u]) : make_float3(0.f, 0.f, 0.f);
+    for (int u = 0; u < U; ++u) {
+      const Base b = base_of(a, g, is[u], js[u], ks[u], xs[u]);
+      const int tx = b.cx >> g.sx, ty = b.cy >> g.sy, tz = b.cz >> g.sz;
+      const unsigned tile = ((unsigned)tx * g.nty + ty) * g.ntz + tz;
+      const int dx = tdiff(tx, c.x, g.ntx), dy = tdiff(ty, c.y, g.nty), dz = tdiff(tz, c.z, g.ntz);
+      const bool inw = dx >= -h && dx <= h && dy >= -h && dy <= h && dz >= -h && dz <= h;
+      fn(vs[u], ps[u], tile, inw ? ((dx + h) * SORT_WIN + (dy + h)) * SORT_WIN + (dz + h) : -1);
+    }
+  }
+}
+
+// Pass 1: optional pre-drift x += v * cd (written back), then per-tile particle counts. The window
+// histogram of each sub-block is kept in wgcnt for the scatter pass. far: particles outside windows.
+__global__ void __launch_bounds__(256) k_sort_count(PartArgs a, SortGeom g, float cd, unsigned* __restrict__ count,
+                                                    unsigned* __restrict__ wgcnt, unsigned* __restrict__ far,
+                                                    const float* __restrict__ w, unsigned* __restrict__ wmax_bits) {
+  __shared__ unsigned lcnt[SORT_SLOTS];
+  __shared__ int3 cen;
+  const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
+  int i0, j0, k0;
+  subblock_origin(g, b, i0, j0, k0);
+  for (int s = threadIdx.x; s < SORT_SLOTS; s += blockDim.x) lcnt[s] = 0u;
+  if (a.v) {  // drift this sub-block (each thread its own particles) before anything reads them
+    for (int r = 0; r < (SB_X * SB_Y * SB_Z) / 256; ++r) {
+      const int l = r * 256 + (int)threadIdx.x;
+      const int i = i0 + (l >> 9), j = j0 + ((l >> 6) & 7), k = k0 + (l & 63);
+      if (i >= a.npx || j >= a.npy || k >= a.npz) continue;
+      const unsigned p = ((unsigned)i * a.npy + j) * a.npz + k;
+      st3(a.x, p, axpy3(ld3(a.x, p), ld3(a.v, p), cd));
+    }
+  }
+  if (threadIdx.x == 0) cen = window_centre(a, g, i0, j0, k0);
+  __syncthreads();
+  const int3 c = cen;
+  unsigned nfar = 0;
+  for_particles(a, g, i0, j0, k0, c, [&](bool on, unsigned, unsigned tile, int slot) {
+    (void)wave_add(on && slot >= 0, slot, lcnt);
+    if (on && slot < 0) {
+      atomicAdd(count + tile, 1u);
+      ++nfar;
+    }
+  });
+  if (w) {  // largest |weight| for the fixed-point scale (non-negative floats order like their bits)
+    float m = 0.f;
+    for (int r = 0; r < (SB_X * SB_Y * SB_Z) / 256; ++r) {
+      const int l = r * 256 + (int)threadIdx.x;
+      const int i = i0 + (l >> 9), j = j0 + ((l >> 6) & 7), k = k0 + (l & 63);
+      if (i >= a.npx || j >= a.npy || k >= a.npz) continue;
+      m = fmaxf(m, fabsf(w[((unsigned)i * a.npy + j) * a.npz + k]));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if (__lane_id() == 0) atomicMax(wmax_bits, __float_as_uint(m));
+  }
+  if (nfar) atomicAdd(far, nfar);
+  __syncthreads();
+  for (int s = threadIdx.x; s < SORT_SLOTS; s += blockDim.x) {
+    const unsigned n = lcnt[s];
+    wgcnt[(size_t)b * SORT_SLOTS + s] = n;
+    if (n) atomicAdd(count + slot_tile(g, c, s), n);
+  }
+}
+
+// Pass 2 (one workgroup): offsets = exclusive prefix of count, cursor = offsets; the fixed-point
+// scale 2^k (largest deposit 2^30) from the weight bound; far counter moved to far_last.
+__global__ void __launch_bounds__(1024) k_sort_scan(const unsigned* __restrict__ count, unsigned* __restrict__ offset,
+                                                    unsigned* __restrict__ cursor, int ntiles, float wmax_host,
+                                                    unsigned* __restrict__ wmax_bits, float* __restrict__ scale,
+                                                    unsigned* __restrict__ far, unsigned* __restrict__ far_last) {
+  __shared__ unsigned part[1024];
+  const int T = blockDim.x, t = threadIdx.x;
+  const int per = (ntiles + T - 1) / T;
+  const int beg = min(ntiles, t * per), end = min(ntiles, beg + per);
+  unsigned sum = 0;
+  for (int i = beg; i < end; ++i) sum += count[i];
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < T; o <<= 1) {  // Hillis-Steele inclusive scan of the per-thread sums
+    const unsigned v = t >= o ? part[t - o] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  unsigned run = part[t] - sum;
+  for (int i = beg; i < end; ++i) {
+    offset[i] = run;
+    cursor[i] = run;
+    run += count[i];
+  }
+  if (t == 0) {
+    const float m = wmax_host >= 0.f ? wmax_host : __uint_as_float(*wmax_bits);
+    int e = 0;
+    if (m > 0.f) (void)frexpf(m, &e);  // m < 2^e
+    e = max(-90, min(e, 90));
+    scale[0] = ldexpf(1.0f, 30 - e);
+    scale[1] = ldexpf(1.0f, e - 30);
+    *wmax_bits = 0u;
+    *far_last = *far;
+    *far = 0u;
+  }
+}
+
+// Pass 3: each particle's index into its tile's segment.
+__global__ void __launch_bounds__(256) k_sort_scatter(PartArgs a, SortGeom g, unsigned* __restrict__ cursor,
+                                                      const unsigned* __restrict__ wgcnt, unsigned* __restrict__ seg) {
+  __shared__ unsigned lbase[SORT_SLOTS];
+  __shared__ unsigned lrank[SORT_SLOTS];
+  __shared__ int3 cen;
+  const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
+  int i0, j0, k0;
+  subblock_origin(g, b, i0, j0, k0);
+  if (threadIdx.x == 0) cen = window_centre(a, g, i0, j0, k0);
+  __syncthreads();
+  const int3 c = cen;
+  for (int s = threadIdx.x; s < SORT_SLOTS; s += blockDim.x) {
+    const unsigned n = wgcnt[(size_t)b * SORT_SLOTS + s];
+    lbase[s] = n ? atomicAdd(cursor + slot_tile(g, c, s), n) : 0u;
+    lrank[s] = 0u;
+  }
+  __syncthreads();
+  for_particles(a, g, i0, j0, k0, c, [&](bool on, unsigned p, unsigned tile, int slot) {
+    const unsigned r = wave_add(on && slot >= 0, slot, lrank);
+    if (on) seg[slot >= 0 ? lbase[slot] + r : atomicAdd(cursor + tile, 1u)] = p;
+  });
+}
+
+// Face cells of a tile grown by one cell on its upper faces (padded local coordinates with at
+// least one coordinate equal to the tile length), enumerated as: x face (ly, lz in [0, B]),
+// y face (lx in [0, BX), lz in [0, BZ]), z face (lx in [0, BX), ly in [0, BY)).
+__device__ __forceinline__ int face_index(int lx, int ly, int lz, int BX, int BY, int BZ) {
+  if (lx == BX) return ly * (BZ + 1) + lz;
+  if (ly == BY) return (BY + 1) * (BZ + 1) + lx * (BZ + 1) + lz;
+  return (BY + 1) * (BZ + 1) + BX * (BZ + 1) + lx * BY + ly;
+}
+
+// Pass 4: one workgroup per tile: deposit the 8 corners of the tile's particles into the padded LDS
+// tile, write the tile's cells to the mesh (accumulate: add) and its face cells to faces.
+template <bool SIGNED>
+__global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, const float* __restrict__ w, float w0,
+                                                       unsigned* __restrict__ count,
+                                                       const unsigned* __restrict__ offset,
+                                                       const unsigned* __restrict__ seg,
+                                                       const float* __restrict__ scale, float* __restrict__ mesh,
+                                                       float* __restrict__ faces, int face_stride, int accumulate) {
+  __shared__ unsigned long long tile[TILE_PAD_MAX];
+  const int BX = 1 << g.sx, BY = 1 << g.sy, BZ = 1 << g.sz;
+  const int PY = BY + 1, PZ = BZ + 1;
+  const int npad = (BX + 1) * PY * PZ;
+  const unsigned L = xcd_logical(blockIdx.x, gridDim.x);
+  const int tz = (int)(L % (unsigned)g.ntz), ty = (int)((L / (unsigned)g.ntz) % (unsigned)g.nty),
+            tx = (int)(L / ((unsigned)g.ntz * g.nty));
+  const int T0x = tx << g.sx, T0y = ty << g.sy, T0z = tz << g.sz;
+  for (int i = threadIdx.x; i < npad; i += blockDim.x) tile[i] = 0ull;
+  const unsigned n = count[L], beg = offset[L];
+  const float S = scale[0];
+  const float wS = w0 * S;
+  // byte offsets of the +1 neighbours in the padded tile
+  const int oz = 8, oy = PZ * 8, ox = PY * PZ * 8;
+  char* lds = reinterpret_cast<char*>(tile);
+  __syncthreads();
+  constexpr int U = 4;
+  for (unsigned e0 = threadIdx.x; e0 < n; e0 += U * blockDim.x) {
+    unsigned ps[U];
+    float3 xs[U];
 #pragma unroll
-    for (int u = 0; u < PT_UNROLL; ++u) {
-      const float3 x = xv[u];
-      const float fx = floorf(x.x), fy = floorf(x.y), fz = floorf(x.z);
-      const bool esc = ok[u] && !(fx >= -fH && fx <= fH && fy >= -fH && fy <= fH && fz >= -fH && fz <= fH);
-      const int Lx = li[u] * rx - T0x, Ly = lj[u] * ry - T0y, Lz = lk[u] * rz - T0z;
-      if (esc && Lx >= 0 && Lx < BX && Ly >= 0 && Ly < BY && Lz >= 0 && Lz < BZ)
-        my_esc[atomicAdd(esc_count + L, 1u)] = pidx[u];
-      const bool use = ok[u] && !esc;
-      const int bx = Lx + (use ? (int)fx : 0), by = Ly + (use ? (int)fy : 0), bz = Lz + (use ? (int)fz : 0);
-      const bool touch = use && bx >= -1 && bx < BX && by >= -1 && by < BY && bz >= -1 && bz < BZ;
-      const bool inside = touch && bx >= 0 && bx < BX - 1 && by >= 0 && by < BY - 1 && bz >= 0 && bz < BZ - 1;
-      const float wt = (touch && w) ? w[pidx[u]] : w0;
-      const float frx = x.x - fx, fry = x.y - fy, frz = x.z - fz;
-      const float wx0 = 1.0f - fabsf(frx), wx1 = 1.0f - fabsf(frx - 1.0f);
-      const float wy0 = 1.0f - fabsf(fry), wy1 = 1.0f - fabsf(fry - 1.0f);
-      const float wz0 = 1.0f - fabsf(frz), wz1 = 1.0f - fabsf(frz - 1.0f);
-      const float wxy[4] = {wx0 * wy0, wx1 * wy0, wx0 * wy1, wx1 * wy1};
-      float lo[4], hi[4];
+    for (int u = 0; u < U; ++u) {
+      const unsigned e = e0 + u * blockDim.x;
+      ps[u] = e < n ? seg[beg + e] : 0u;
+    }
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        lo[c] = (wxy[c] * wz0) * wt;
-        hi[c] = (wxy[c] * wz1) * wt;
-      }
-      // z-neighbour merge among fast-path lanes (full wave participates in the shuffles)
-      const int base = inside ? (bx * BY + by) * BZ + bz : -4;
-      const int prev_base = __shfl_up(base, 1);
-      float prev_hi[4];
+    for (int u = 0; u < U; ++u) xs[u] = (e0 + u * blockDim.x < n) ? ld3(a.x, ps[u]) : make_float3(0.f, 0.f, 0.f);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) prev_hi[c] = __shfl_up(hi[c], 1);
-      const bool took = inside && lane > 0 && prev_base >= 0 && prev_base + 1 == base;
-      const int took_i = took ? 1 : 0;
-      const bool next_took = (__shfl_down(took_i, 1) != 0) && lane < 63;
-      if (inside) {
-        unsigned long long* t = tile + base;
-        const int offs[4] = {0, BYZ, BZ, BYZ + BZ};
+    for (int u = 0; u < U; ++u) {
+      if (e0 + u * blockDim.x >= n) continue;
+      const unsigned p = ps[u];
+      int i = 0, j = 0, k = 0;
+      if (a.mode == 0) lattice_of(a, g, p, i, j, k);
+      const Base b = base_of(a, g, i, j, k, xs[u]);
+      const int base = (((b.cx - T0x) * PY + (b.cy - T0y)) * PZ + (b.cz - T0z)) * 8;
+      const float c = w ? w[p] * S : wS;
+      const float wx[2] = {1.0f - fabsf(b.fx), 1.0f - fabsf(b.fx - 1.0f)};
+      const float wy[2] = {1.0f - fabsf(b.fy), 1.0f - fabsf(b.fy - 1.0f)};
+      // c is folded into the z weights: exact when c is a power of two (scalar weight 1)
+      const float wz[2] = {(1.0f - fabsf(b.fz)) * c, (1.0f - fabsf(b.fz - 1.0f)) * c};
 #pragma unroll
-        for (int c = 0; c < 4; ++c) atomicAdd(t + offs[c], fix64(took ? lo[c] + prev_hi[c] : lo[c]));
-        if (!next_took) {
-#pragma unroll
-          for (int c = 0; c < 4; ++c) atomicAdd(t + offs[c] + 1, fix64(hi[c]));
-        }
-      } else if (touch) {
-#pragma unroll
-        for (int o = 0; o < 8; ++o) {
-          const int ix = bx + (o & 1), iy = by + ((o >> 1) & 1), iz = bz + ((o >> 2) & 1);
-          if (ix >= 0 && ix < BX && iy >= 0 && iy < BY && iz >= 0 && iz < BZ)
-            atomicAdd(&tile[(ix * BY + iy) * BZ + iz], fix64((o >> 2) ? hi[o & 3] : lo[o & 3]));
-        }
+      for (int o = 0; o < 8; ++o) {
+        const int cx = o & 1, cy = (o >> 1) & 1, cz = o >> 2;
+        const float v = (wx[cx] * wy[cy]) * wz[cz];
+        const unsigned long long d =
+            SIGNED ? (unsigned long long)(long long)__float2int_rz(v) : (unsigned long long)__float2uint_rz(v);
+        atomicAdd(reinterpret_cast<unsigned long long*>(lds + base + cx * ox + cy * oy + cz * oz), d);
       }
     }
   }
   __syncthreads();
-  // rows of BZ contiguous cells
-  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-    const int iz = c % BZ, iy = (c / BZ) % BY, ix = c / BYZ;
-    float* dst = mesh + ((size_t)(T0x + ix) * NY + (T0y + iy)) * NZ + (T0z + iz);
-    const float v = (float)((double)(long long)tile[c] * 2.3283064365386962890625e-10);
+  if (threadIdx.x == 0) count[L] = 0u;
+  const double inv = (double)scale[1];
+  const int ncell = BX * BY * BZ;
+  for (int q = threadIdx.x; q < ncell; q += blockDim.x) {
+    const int iz = q & (BZ - 1), iy = (q >> g.sz) & (BY - 1), ix = q >> (g.sz + g.sy);
+    float* dst = mesh + ((size_t)(T0x + ix) * g.NY + (T0y + iy)) * g.NZ + (T0z + iz);
+    const float v = (float)((double)(long long)tile[(ix * PY + iy) * PZ + iz] * inv);
     *dst = accumulate ? *dst + v : v;
   }
+  float* f = faces + (size_t)L * face_stride;
+  const int nface = PY * PZ + BX * PZ + BX * BY;
+  for (int q = threadIdx.x; q < nface; q += blockDim.x) {
+    int lx, ly, lz;
+    if (q < PY * PZ) {
+      lx = BX;
+      ly = q / PZ;
+      lz = q % PZ;
+    } else if (q < PY * PZ + BX * PZ) {
+      const int r = q - PY * PZ;
+      lx = r / PZ;
+      ly = BY;
+      lz = r % PZ;
+    } else {
+      const int r = q - PY * PZ - BX * PZ;
+      lx = r / BY;
+      ly = r % BY;
+      lz = BZ;
+    }
+    f[q] = (float)((double)(long long)tile[(lx * PY + ly) * PZ + lz] * inv);
+  }
 }
 
-// escapes of tile blockIdx.x: segment [t * esc_cap, t * esc_cap + esc_count[t])
-__global__ void __launch_bounds__(256) k_paint_escapes(PartArgs a, const float* __restrict__ w, float w0,
-                                                       float* __restrict__ mesh, int NX, int NY, int NZ, int rx, int ry,
-                                                       int rz, const unsigned* __restrict__ esc_count,
-                                                       const unsigned* __restrict__ esc_list, int esc_cap) {
-  const unsigned n = esc_count[blockIdx.x];
-  const unsigned* seg = esc_list + (size_t)blockIdx.x * esc_cap;
-  for (unsigned e = threadIdx.x; e < n; e += blockDim.x) {
-    const unsigned p = seg[e];
-    const float3 x = ld3(a.x, p);
-    const float wt = w ? w[p] : w0;
-    LatticeWalk lw;
-    lw.init(p, 1u, a.npy, a.npz);
-    const Cic q = particle_cic(a, lw, x, NX, NY, NZ, rx, ry, rz);
-#pragma unroll
-    for (int o = 0; o < 8; ++o) unsafeAtomicAdd(mesh + q.off[o], q.w[o] * wt);
+// Pass 5: cells on a tile's lower faces receive the face cells of its lower neighbours (x, y, z,
+// xy, xz, yz, xyz, in that order). One thread per (tile, lower-shell cell).
+__global__ void __launch_bounds__(256) k_paint_faces(SortGeom g, const float* __restrict__ faces, int face_stride,
+                                                     float* __restrict__ mesh, int shell) {
+  const int BX = 1 << g.sx, BY = 1 << g.sy, BZ = 1 << g.sz;
+  const size_t gid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t ntiles = (size_t)g.ntx * g.nty * g.ntz;
+  if (gid >= ntiles * shell) return;
+  const unsigned L = (unsigned)(gid / shell);
+  const int q = (int)(gid % shell);
+  // lower shell cell q: lx == 0 plane (BY x BZ), then ly == 0 (lx >= 1), then lz == 0 (lx, ly >= 1)
+  int lx, ly, lz;
+  if (q < BY * BZ) {
+    lx = 0;
+    ly = q / BZ;
+    lz = q % BZ;
+  } else if (q < BY * BZ + (BX - 1) * BZ) {
+    const int r = q - BY * BZ;
+    lx = 1 + r / BZ;
+    ly = 0;
+    lz = r % BZ;
+  } else {
+    const int r = q - BY * BZ - (BX - 1) * BZ;
+    lx = 1 + r / (BY - 1);
+    ly = 1 + r % (BY - 1);
+    lz = 0;
   }
+  const int tz = (int)(L % (unsigned)g.ntz), ty = (int)((L / (unsigned)g.ntz) % (unsigned)g.nty),
+            tx = (int)(L / ((unsigned)g.ntz * g.nty));
+  float sum = 0.f;
+#pragma unroll
+  for (int m = 1; m < 8; ++m) {
+    const int ex = m & 1, ey = (m >> 1) & 1, ez = m >> 2;
+    if ((ex && lx != 0) || (ey && ly != 0) || (ez && lz != 0)) continue;
+    const int nx = ex ? (tx == 0 ? g.ntx - 1 : tx - 1) : tx;
+    const int ny = ey ? (ty == 0 ? g.nty - 1 : ty - 1) : ty;
+    const int nz = ez ? (tz == 0 ? g.ntz - 1 : tz - 1) : tz;
+    const unsigned nb = ((unsigned)nx * g.nty + ny) * g.ntz + nz;
+    sum += faces[(size_t)nb * face_stride + face_index(ex ? BX : lx, ey ? BY : ly, ez ? BZ : lz, BX, BY, BZ)];
+  }
+  float* dst = mesh + ((size_t)((tx << g.sx) + lx) * g.NY + ((ty << g.sy) + ly)) * g.NZ + ((tz << g.sz) + lz);
+  *dst = *dst + sum;
 }
 
 // x <- x + v*c
@@ -681,22 +961,55 @@ static void ratios(pmx_plan_t* p, const PartArgs& a, int& rx, int& ry, int& rz) 
   rz = p->nz / a.npz;
 }
 
-// Paint into `mesh`. Relative mode uses the owner-computes tiled kernel (+ escape pass) unless
-// `atomic` is set; absolute mode always scatters with atomics. accumulate=0 overwrites the mesh.
-// A pre-drift (a.v != null) runs as its own pass first (tiles read each particle several times).
+static FDiv make_fdiv(unsigned d) {
+  FDiv f;
+  f.d = d;
+  int l = 0;
+  while ((1ull << l) < d) ++l;  // ceil(log2 d)
+  f.m = (unsigned)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+  f.s1 = l > 0 ? 1 : 0;
+  f.s2 = l > 0 ? l - 1 : 0;
+  return f;
+}
+
+static int ilog2(int n) {
+  int l = 0;
+  while ((1 << (l + 1)) <= n) ++l;
+  return l;
+}
+
+// Paint tiles of a plan: (min(nx,16), min(ny,16), min(nz,32)) cells; every supported axis length
+// is a multiple of its tile length.
+void pmx_paint_tiles(const pmx_plan_t* p, int& sx, int& sy, int& sz, int& ntiles) {
+  sx = ilog2(p->nx < 16 ? p->nx : 16);
+  sy = ilog2(p->ny < 16 ? p->ny : 16);
+  sz = ilog2(p->nz < 32 ? p->nz : 32);
+  ntiles = (p->nx >> sx) * (p->ny >> sy) * (p->nz >> sz);
+}
+int pmx_paint_face_cells(const pmx_plan_t* p) {
+  int sx, sy, sz, ntiles;
+  pmx_paint_tiles(p, sx, sy, sz, ntiles);
+  const int BX = 1 << sx, BY = 1 << sy, BZ = 1 << sz;
+  return (BY + 1) * (BZ + 1) + BX * (BZ + 1) + BX * BY;
+}
+
+// Paint into `mesh` (accumulate=0 overwrites it). Default: the sorted owner-computes paint, both
+// modes; `atomic`: one-pass scatter with global atomics (after a separate pre-drift pass).
+// a.v != null applies the pre-drift x += v * cd_pre first (written back to x).
 int pmx_launch_paint(pmx_plan_t* p, const PartArgs& a_in, const float* w, float w0, float cd_pre, float* mesh,
                      int accumulate, int atomic, hipStream_t s) {
   int rx, ry, rz;
   PartArgs a = a_in;
   ratios(p, a, rx, ry, rz);
   const size_t np = (size_t)a.npx * a.npy * a.npz;
-  if (a.v != nullptr) {
-    hipLaunchKernelGGL(k_drift, dim3(grid_particles(3 * np)), dim3(256), 0, s, a.x, a.v, (unsigned)(3 * np), cd_pre);
-    int rc = pmx_check_launch("drift");
-    if (rc) return rc;
-    a.v = nullptr;
-  }
-  if (a.mode != 0 || atomic) {
+  if (atomic) {
+    if (a.v != nullptr) {
+      hipLaunchKernelGGL(k_drift, dim3(grid_particles(3 * np)), dim3(256), 0, s, a.x, a.v, (unsigned)(3 * np),
+                         cd_pre);
+      int rc = pmx_check_launch("drift");
+      if (rc) return rc;
+      a.v = nullptr;
+    }
     if (!accumulate) {
       int rc = pmx_launch_fill(mesh, p->ncell, 0.f, s);
       if (rc) return rc;
@@ -705,35 +1018,70 @@ int pmx_launch_paint(pmx_plan_t* p, const PartArgs& a_in, const float* w, float 
                        rz);
     return pmx_check_launch("paint");
   }
-  const int BX = p->nx < 16 ? p->nx : 16, BY = p->ny < 16 ? p->ny : 16, BZ = p->nz < 32 ? p->nz : 32;
-  const int ntiles = (p->nx / BX) * (p->ny / BY) * (p->nz / BZ);
-  const int cap = ((BX + rx - 1) / rx) * ((BY + ry - 1) / ry) * ((BZ + rz - 1) / rz);
-  const size_t need = (size_t)ntiles * cap;
-  if (p->esc_cap < need || p->esc_tiles < (size_t)ntiles) {
-    if (p->esc_list) (void)hipFree(p->esc_list);
-    if (p->esc_count) (void)hipFree(p->esc_count);
-    p->esc_list = nullptr;
-    p->esc_count = nullptr;
-    p->esc_cap = 0;
-    p->esc_tiles = 0;
-    hipError_t e = hipMalloc((void**)&p->esc_list, need * sizeof(unsigned));
-    if (e == hipSuccess) e = hipMalloc((void**)&p->esc_count, (size_t)ntiles * sizeof(unsigned));
-    if (e != hipSuccess) return pmx_set_error(4, std::string("escape list: ") + hipGetErrorString(e));
-    p->esc_cap = need;
-    p->esc_tiles = ntiles;
+  SortGeom g;
+  g.NX = p->nx;
+  g.NY = p->ny;
+  g.NZ = p->nz;
+  int ntiles;
+  pmx_paint_tiles(p, g.sx, g.sy, g.sz, ntiles);
+  g.ntx = p->nx >> g.sx;
+  g.nty = p->ny >> g.sy;
+  g.ntz = p->nz >> g.sz;
+  g.rx = a.mode == 0 ? rx : 1;
+  g.ry = a.mode == 0 ? ry : 1;
+  g.rz = a.mode == 0 ? rz : 1;
+  g.nbx = (a.npx + SB_X - 1) / SB_X;
+  g.nby = (a.npy + SB_Y - 1) / SB_Y;
+  g.nbz = (a.npz + SB_Z - 1) / SB_Z;
+  g.div_npz = make_fdiv((unsigned)a.npz);
+  g.div_npy = make_fdiv((unsigned)a.npy);
+  const bool pow2 = (a.npz & (a.npz - 1)) == 0 && (a.npy & (a.npy - 1)) == 0;
+  g.lpz = pow2 ? ilog2(a.npz) : -1;
+  g.lpy = pow2 ? ilog2(a.npy) : -1;
+  const size_t nsb = (size_t)g.nbx * g.nby * g.nbz;
+  if (p->ps_wg_cap < nsb * SORT_SLOTS) {
+    if (p->ps_wgcnt) (void)hipFree(p->ps_wgcnt);
+    p->ps_wgcnt = nullptr;
+    p->ps_wg_cap = 0;
+    hipError_t e = hipMalloc((void**)&p->ps_wgcnt, nsb * SORT_SLOTS * sizeof(unsigned));
+    if (e != hipSuccess) return pmx_set_error(4, std::string("paint sort counts: ") + hipGetErrorString(e));
+    p->ps_wg_cap = nsb * SORT_SLOTS;
   }
-  p->esc_ntiles_last = ntiles;
-  if (rx == 1 && ry == 1 && rz == 1)
-    hipLaunchKernelGGL(k_paint_tiles<true>, dim3(ntiles), dim3(512), 0, s, a, w, w0, mesh, p->nx, p->ny, p->nz, rx, ry,
-                       rz, BX, BY, BZ, PAINT_HALO, accumulate, p->esc_count, p->esc_list, cap);
-  else
-    hipLaunchKernelGGL(k_paint_tiles<false>, dim3(ntiles), dim3(512), 0, s, a, w, w0, mesh, p->nx, p->ny, p->nz, rx, ry,
-                       rz, BX, BY, BZ, PAINT_HALO, accumulate, p->esc_count, p->esc_list, cap);
-  int rc = pmx_check_launch("paint_tiles");
+  if (p->ps_seg_cap < np) {
+    if (p->ps_seg) (void)hipFree(p->ps_seg);
+    p->ps_seg = nullptr;
+    p->ps_seg_cap = 0;
+    hipError_t e = hipMalloc((void**)&p->ps_seg, np * sizeof(unsigned));
+    if (e != hipSuccess) return pmx_set_error(4, std::string("paint sort segments: ") + hipGetErrorString(e));
+    p->ps_seg_cap = np;
+  }
+  unsigned* wmax_bits = p->ps_misc;
+  unsigned* far = p->ps_misc + 1;
+  unsigned* far_last = p->ps_misc + 2;
+  hipLaunchKernelGGL(k_sort_count, dim3((unsigned)nsb), dim3(256), 0, s, a, g, cd_pre, p->ps_count, p->ps_wgcnt, far, w,
+                     wmax_bits);
+  int rc = pmx_check_launch("sort_count");
   if (rc) return rc;
-  hipLaunchKernelGGL(k_paint_escapes, dim3(ntiles), dim3(256), 0, s, a, w, w0, mesh, p->nx, p->ny, p->nz, rx, ry, rz,
-                     p->esc_count, p->esc_list, cap);
-  return pmx_check_launch("paint_escapes");
+  a.v = nullptr;
+  hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, p->ps_count, p->ps_offset, p->ps_cursor, ntiles,
+                     w ? -1.0f : fabsf(w0), wmax_bits, p->ps_scale, far, far_last);
+  if ((rc = pmx_check_launch("sort_scan"))) return rc;
+  hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)nsb), dim3(256), 0, s, a, g, p->ps_cursor, p->ps_wgcnt, p->ps_seg);
+  if ((rc = pmx_check_launch("sort_scatter"))) return rc;
+  const int BX = 1 << g.sx, BY = 1 << g.sy, BZ = 1 << g.sz;
+  const int face_stride = pmx_paint_face_cells(p);
+  if (w != nullptr || w0 < 0.f)
+    hipLaunchKernelGGL(k_paint_deposit<true>, dim3(ntiles), dim3(512), 0, s, a, g, w, w0, p->ps_count, p->ps_offset,
+                       p->ps_seg, p->ps_scale, mesh, p->ps_faces, face_stride, accumulate);
+  else
+    hipLaunchKernelGGL(k_paint_deposit<false>, dim3(ntiles), dim3(512), 0, s, a, g, w, w0, p->ps_count, p->ps_offset,
+                       p->ps_seg, p->ps_scale, mesh, p->ps_faces, face_stride, accumulate);
+  if ((rc = pmx_check_launch("paint_deposit"))) return rc;
+  const int shell = BX * BY * BZ - (BX - 1) * (BY - 1) * (BZ - 1);
+  const size_t nthreads = (size_t)ntiles * shell;
+  hipLaunchKernelGGL(k_paint_faces, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s, g, p->ps_faces,
+                     face_stride, mesh, shell);
+  return pmx_check_launch("paint_faces");
 }
 
 int pmx_launch_readout(pmx_plan_t* p, const PartArgs& a, const float* mesh, int ncomp, float* out, float ck,

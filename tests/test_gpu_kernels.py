@@ -168,10 +168,10 @@ def test_healpix_paint(pkg, cuda, scheme):
     assert rel(got.cpu().numpy(), ref) < 1e-6
 
 
-@pytest.mark.parametrize("n,scale", [(64, 1.5), (128, 2.5)])
+@pytest.mark.parametrize("n,scale", [(64, 1.5), (128, 2.5), (128, 24.0)])
 def test_tiled_paint_matches_atomic_and_oracle(pkg, cuda, n, scale):
-    """Owner-computes tiles (+ escape pass) against the one-pass atomic scatter and the oracle, with a
-    displacement spread that sends a few percent of the particles through the escape pass."""
+    """Sorted owner-computes paint against the one-pass atomic scatter and the oracle; the widest
+    spread sends particles outside their sub-block's tile window (the per-entry atomic path)."""
     dx = _disp((n, n, n), scale, 13)
     xd = torch.from_numpy(dx).to(cuda)
     tiled = pkg.engine.paint(xd, (n, n, n)).cpu().numpy()
@@ -182,9 +182,10 @@ def test_tiled_paint_matches_atomic_and_oracle(pkg, cuda, n, scale):
     pkg._lib.check(lib.pmx_paint_cic(atomic_plan.handle, xd.data_ptr(), n, n, n, 0, None, 1.0, out.data_ptr(), 0,
                                      pkg._lib.stream_ptr()), "paint")
     ref = orc.cic_paint(dx, (n, n, n), "relative")
-    assert 0 < esc < n**3 // 3
     assert rel(tiled, ref) < 1e-6
     assert rel(out.cpu().numpy(), ref) < 1e-6
+    if scale > 20:  # 128^3 mesh: 8 x 8 x 4 tiles, so +-2 tile windows do not cover the grid
+        assert esc > 0
 
 
 def test_tiled_paint_accumulates(pkg, cuda):
