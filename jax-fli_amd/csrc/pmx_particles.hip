@@ -803,35 +803,80 @@ __device__ __forceinline__ double grid_pos(int mode, int lat, float xc, int n) {
   return g - floor(g / n) * n;
 }
 
+// Shell paint with per-workgroup aggregation: a workgroup takes a 16^3 Lagrangian sub-block of
+// particles (a compact patch of the sky), accumulates its (pixel, weight) deposits in an LDS hash
+// table, then adds each distinct pixel to the map with one global atomic. Device-wide float
+// atomics are the bottleneck of the one-atomic-per-deposit form; particles on the same line of
+// sight in the patch share pixels, which the table merges. A deposit that finds no free slot
+// within HP_PROBES probes goes straight to the map.
+constexpr int HP_HASH = 4096, HP_PROBES = 32, HP_SB = 16;
+constexpr unsigned HP_EMPTY = 0xFFFFFFFFu;
+
+__device__ __forceinline__ void hp_table_add(volatile unsigned* keys, float* vals, unsigned pix, float v,
+                                             float* __restrict__ acc) {
+  unsigned h = (pix * 2654435761u) >> 20;  // 12-bit multiplicative hash
+#pragma unroll 1
+  for (int t = 0; t < HP_PROBES; ++t) {
+    const unsigned k = keys[h];
+    if (k == pix) {
+      atomicAdd(&vals[h], v);
+      return;
+    }
+    if (k == HP_EMPTY) {
+      const unsigned old = atomicCAS(const_cast<unsigned*>(keys + h), HP_EMPTY, pix);
+      if (old == HP_EMPTY || old == pix) {
+        atomicAdd(&vals[h], v);
+        return;
+      }
+    }
+    h = (h + 1) & (HP_HASH - 1);
+  }
+  unsafeAtomicAdd(acc + pix, v);
+}
+
 __global__ void __launch_bounds__(256) k_paint_healpix(PartArgs a, const float* __restrict__ w, float w0, ShellArgs sa,
                                                        HpGeom g, float* __restrict__ acc, int NX, int NY, int NZ,
-                                                       int rx, int ry, int rz) {
-  const unsigned np = (unsigned)a.npx * a.npy * a.npz;
-  const Chunk ch = my_chunk(np);
-  LatticeWalk lw;
-  lw.init(ch.begin + threadIdx.x, blockDim.x, a.npy, a.npz);
-  for (unsigned p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x, lw.step()) {
+                                                       int rx, int ry, int rz, int nbx, int nby, int nbz) {
+  __shared__ unsigned keys[HP_HASH];
+  __shared__ float vals[HP_HASH];
+  for (int i = threadIdx.x; i < HP_HASH; i += blockDim.x) {
+    keys[i] = HP_EMPTY;
+    vals[i] = 0.f;
+  }
+  const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
+  const int bk = (int)(b % (unsigned)nbz), bj = (int)((b / (unsigned)nbz) % (unsigned)nby),
+            bi = (int)(b / ((unsigned)nbz * nby));
+  __syncthreads();
+  for (int r = 0; r < HP_SB * HP_SB * HP_SB / 256; ++r) {
+    const int l = r * 256 + (int)threadIdx.x;
+    const int i = bi * HP_SB + (l >> 8), j = bj * HP_SB + ((l >> 4) & 15), k = bk * HP_SB + (l & 15);
+    if (i >= a.npx || j >= a.npy || k >= a.npz) continue;
+    const unsigned p = ((unsigned)i * a.npy + j) * a.npz + k;
     const float3 x = ld3(a.x, p);
-    const double X = grid_pos(a.mode, lw.i * rx, x.x, NX) * sa.box[0] / NX - sa.obs[0];
-    const double Y = grid_pos(a.mode, lw.j * ry, x.y, NY) * sa.box[1] / NY - sa.obs[1];
-    const double Z = grid_pos(a.mode, lw.k * rz, x.z, NZ) * sa.box[2] / NZ - sa.obs[2];
-    const double r = sqrt(X * X + Y * Y + Z * Z);
-    if (!(r > sa.rmin && r <= sa.rmax)) continue;
+    const double X = grid_pos(a.mode, i * rx, x.x, NX) * sa.box[0] / NX - sa.obs[0];
+    const double Y = grid_pos(a.mode, j * ry, x.y, NY) * sa.box[1] / NY - sa.obs[1];
+    const double Z = grid_pos(a.mode, k * rz, x.z, NZ) * sa.box[2] / NZ - sa.obs[2];
+    const double rr = sqrt(X * X + Y * Y + Z * Z);
+    if (!(rr > sa.rmin && rr <= sa.rmax)) continue;
     const double wt = w ? (double)w[p] : (double)w0;
     const double phi0 = atan2(Y, X);
     double phi = phi0 < 0 ? phi0 + 2.0 * M_PI : phi0;
     if (!(phi < 2.0 * M_PI)) phi = 0.0;  // -tiny + 2 pi can round up to 2 pi
     if (sa.scheme == 0) {
-      const long pix = hp_ang2pix(g, Z / r, phi);
-      unsafeAtomicAdd(acc + pix, (float)wt);
+      hp_table_add(keys, vals, (unsigned)hp_ang2pix(g, Z / rr, phi), (float)wt, acc);
     } else {
       const double theta = atan2(sqrt(X * X + Y * Y), Z);
       long pix[4];
       double wg[4];
       hp_interpol(g, theta, phi, pix, wg);
 #pragma unroll
-      for (int m = 0; m < 4; ++m) unsafeAtomicAdd(acc + pix[m], (float)(wt * wg[m]));
+      for (int m = 0; m < 4; ++m) hp_table_add(keys, vals, (unsigned)pix[m], (float)(wt * wg[m]), acc);
     }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < HP_HASH; i += blockDim.x) {
+    const unsigned k = keys[i];
+    if (k != HP_EMPTY) unsafeAtomicAdd(acc + k, vals[i]);
   }
 }
 
@@ -1117,9 +1162,9 @@ int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w
   g.ncap = 2L * nside * (nside - 1);
   g.fact2 = 4.0 / (double)g.npix;
   g.fact1 = (double)(2 * nside) * g.fact2;
-  const size_t np = (size_t)a.npx * a.npy * a.npz;
-  hipLaunchKernelGGL(k_paint_healpix, dim3(grid_particles(np)), dim3(256), 0, s, a, w, w0, sa, g, acc, p->nx, p->ny,
-                     p->nz, rx, ry, rz);
+  const int nbx = (a.npx + HP_SB - 1) / HP_SB, nby = (a.npy + HP_SB - 1) / HP_SB, nbz = (a.npz + HP_SB - 1) / HP_SB;
+  hipLaunchKernelGGL(k_paint_healpix, dim3((unsigned)nbx * nby * nbz), dim3(256), 0, s, a, w, w0, sa, g, acc, p->nx,
+                     p->ny, p->nz, rx, ry, rz, nbx, nby, nbz);
   return pmx_check_launch("paint_healpix");
 }
 
