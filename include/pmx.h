@@ -129,6 +129,51 @@ int pmx_paint_flat(pmx_plan plan, const float* pos, int npx, int npy, int npz, i
                    float weight, int fnx, int fny, float zlo, float zhi, double scale, float* map, int accumulate,
                    void* stream);
 
+/* ---- adjoints (vector-Jacobian products) --------------------------------------------------------
+ * Hand-written transposes of the entry points above, for reverse-mode gradients of the reference's
+ * `adjoint="reverse"` / `"checkpointed"` integration (src/jax_fli/pm/integrate.py:147-193, 288-538).
+ * "accumulate" adds into the cotangent output instead of overwriting it. Dot outputs are float64
+ * device scalars, accumulated with atomics (zero them first). */
+
+/* Position (and weight) cotangent of pmx_paint_cic: pos_bar = d<mesh_bar, paint(pos)>/d pos,
+ * weights_bar (optional, per particle) = d<mesh_bar, paint>/d w. The CIC weight derivatives follow
+ * the reference's 1 - |f - {0,1}| (derivative -sign). Replaces jax.vjp of cic_paint_dx / cic_paint. */
+int pmx_paint_cic_vjp(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, const float* weights,
+                      float weight, const float* mesh_bar, float* pos_bar, float* weights_bar, int accumulate,
+                      void* stream);
+
+/* Cotangents of pmx_readout_cic: mesh_bar (nx,ny,nz,ncomp; optional) = scatter of out_bar by the
+ * CIC weights (the readout's transpose, sorted deposit), pos_bar (optional) = d<out_bar, read>/d pos.
+ * Replaces jax.vjp of cic_read_dx / cic_read (fields/particles.py:183-229). */
+int pmx_readout_cic_vjp(pmx_plan plan, const float* mesh, int ncomp, const float* pos, int npx, int npy, int npz,
+                        int mode, const float* out_bar, float* mesh_bar, float* pos_bar, int accumulate,
+                        void* stream);
+
+/* Density cotangent of pmx_force_mesh: density_bar = G^T S^T fmesh_bar (S the 4-point gradient,
+ * antisymmetric; G the Green's operator, symmetric). Replaces jax.vjp of the k-space part of
+ * pm_forces (used by the LPT adjoint, pm/lpt.py:119). */
+int pmx_force_mesh_vjp(pmx_plan plan, const float* fmesh_bar, float scale, unsigned flags, float* density_bar,
+                       void* stream);
+
+/* Position cotangent of pmx_forces: pos_bar (+)= J(pos)^T forces_bar, J the Jacobian of
+ * pm_forces(pos) * scale (readout-weight and paint-weight terms); dot (optional) += forces_bar . F.
+ * Recomputes the force mesh at pos. Replaces jax.vjp of _forces (src/jax_fli/pm/solvers.py:19-32). */
+int pmx_forces_vjp(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, float scale, unsigned flags,
+                   const float* forces_bar, float* pos_bar, int accumulate, double* dot, void* stream);
+
+/* One reverse-adjoint step of the reversible KKD solver, in place. The forward step was
+ *   v <- v + F(x) * c_kick ;  x <- x + v * c_drift      (F = pm_forces(x) * force_scale)
+ * (ReversibleDoubleKickDrift.step, solvers.py:258-325). Given the state after it and the cotangents
+ * (x_bar, v_bar) of that state, this restores the state before it (the exact inverse, in place of
+ * ReversibleDoubleKickDrift.reverse, solvers.py:339-411) and replaces the cotangents by those of
+ * the state before it, reusing the reverse pass's force mesh for the Jacobian:
+ *   v_bar <- v_bar + c_drift x_bar ;  x_bar <- x_bar + J(x)^T (c_kick v_bar)
+ * coef_bar (2 doubles, overwritten) = {dL/dc_drift, dL/dc_kick} for parameter gradients.
+ * Replaces the per-step reverse + jax.vjp(step) of _integrate_bwd (integrate.py:304-358). */
+int pmx_step_reverse_vjp(pmx_plan plan, float* x, float* v, int npx, int npy, int npz, int mode, float force_scale,
+                         float c_kick, float c_drift, unsigned flags, float* x_bar, float* v_bar, double* coef_bar,
+                         void* stream);
+
 #ifdef __cplusplus
 }
 #endif

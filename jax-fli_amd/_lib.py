@@ -49,6 +49,11 @@ SIGNATURES = {
     "pmx_lpt1": (_i, [_vp, _vp, _f, _f, _vp, _vp, _vp]),
     "pmx_paint_healpix": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _dp, _dp, _d, _d, _i, _i, _vp, _i, _vp]),
     "pmx_paint_flat": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _i, _i, _f, _f, _d, _vp, _i, _vp]),
+    "pmx_paint_cic_vjp": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _vp, _vp, _vp, _i, _vp]),
+    "pmx_readout_cic_vjp": (_i, [_vp, _vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp]),
+    "pmx_force_mesh_vjp": (_i, [_vp, _vp, _f, _u, _vp, _vp]),
+    "pmx_forces_vjp": (_i, [_vp, _vp, _i, _i, _i, _i, _f, _u, _vp, _vp, _i, _vp, _vp]),
+    "pmx_step_reverse_vjp": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _u, _vp, _vp, _vp, _vp]),
 }
 
 DECONV = 1

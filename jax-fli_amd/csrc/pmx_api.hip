@@ -166,6 +166,35 @@ int pot_force_mesh_from(pmx_plan_t* p, const float* src, float force_scale, unsi
   return pmx_grad4(p, p->rho, out, s);
 }
 
+// The potential half of pot_force_mesh_from alone: p->rho = G src, G = ifft((force_scale / (12 N k^2)) fft)
+// (optionally CIC-compensated), a real symmetric operator, so it is also its own adjoint.
+int pot_only(pmx_plan_t* p, const float* src, float force_scale, unsigned flags, hipStream_t s) {
+  PMX_TRY(pmx_pass_r2c_z(p, src, p->specA, s));
+  PMX_TRY(pmx_pass_c2c_y(p, p->specA, p->specB, s));
+  KOp op{};
+  op.mode = XP_POT;
+  op.scale = (float)((double)force_scale / (12.0 * (double)p->ncell));
+  op.flags = flags & PMX_DECONV;
+  PMX_TRY(pmx_xpass(p, op, p->specB, p->specC, s));
+  PMX_TRY(pmx_pass_c2c_y_inv(p, 1, p->specC, p->specD, s));
+  return pmx_pass_c2r_z(p, 1, p->specD, p->rho, s);
+}
+
+// Force adjoint at the last sorted particle set `a` (whose force mesh is in p->fmesh):
+//   xbar (+)= J^T (umul * u),  J = d forces / d x,  forces = R(x) S G P(x)
+// = sum_c grad w_c(x) (umul u . M[c] + rhobar[c]),  rhobar = G S^T R(x)^T (umul u).
+// R^T is three sorted deposits (scale from max |u| in p->ps_misc[3]); S^T = -sum_d S_d; dot (+)= u . F.
+int force_adjoint(pmx_plan_t* p, const PartArgs& a, const float* u, float umul, float force_scale, unsigned flags,
+                  float* xbar, int accumulate, double* dot, hipStream_t s) {
+  float* U = reinterpret_cast<float*>(p->specC);  // 3 scalar meshes; consumed by div4 before the FFT reuses it
+  PMX_TRY(pmx_launch_set_scale(p->ps_misc + 3, umul, p->ps_scale, s));
+  for (int d = 0; d < 3; ++d)
+    PMX_TRY(pmx_paint_deposit(p, a, u + d, 3, umul, 0.f, 1, U + d * p->ncell, 1, 0, s));
+  PMX_TRY(pmx_launch_div4(p, U, U + p->ncell, U + 2 * p->ncell, 1, p->rho, s));
+  PMX_TRY(pot_only(p, p->rho, force_scale, flags, s));
+  return pmx_launch_vjp_gather(p, a, 3, u, umul, p->fmesh, p->rho, nullptr, 1.f, xbar, accumulate, nullptr, dot, s);
+}
+
 StepMarks* new_marks(pmx_plan_t* p) {
   if (!p->timing || !p->marks) return nullptr;
   StepMarks m;
@@ -240,7 +269,8 @@ int pmx_plan_create(pmx_plan* out, int nx, int ny, int nz, unsigned flags) {
     return fail(rc);
   int sx, sy, sz, ntiles;
   pmx_paint_tiles(p, sx, sy, sz, ntiles);
-  if ((rc = dev_alloc(&p->ps_count, ntiles, tot)) || (rc = dev_alloc(&p->ps_offset, ntiles, tot)) ||
+  if ((rc = dev_alloc(&p->ps_count, ntiles, tot)) || (rc = dev_alloc(&p->ps_tcount, ntiles, tot)) ||
+      (rc = dev_alloc(&p->ps_offset, ntiles, tot)) ||
       (rc = dev_alloc(&p->ps_cursor, ntiles, tot)) || (rc = dev_alloc(&p->ps_misc, 4, tot)) ||
       (rc = dev_alloc(&p->ps_scale, 2, tot)) ||
       (rc = dev_alloc(&p->ps_faces, (size_t)ntiles * pmx_paint_face_cells(p), tot)))
@@ -268,7 +298,7 @@ int pmx_plan_destroy(pmx_plan p) {
   if (!p) return 0;
   void* bufs[] = {p->tw_x, p->tw_y, p->tw_z, p->tw_zh, p->grad_x, p->grad_y, p->grad_z, p->w2_x, p->w2_y, p->w2_z,
                   p->dec_x, p->dec_y, p->dec_z, p->rho, p->specA, p->specB, p->specC, p->specD, p->fmesh, p->acc,
-                  p->kphys_x, p->kphys_y, p->kphys_z, p->ps_count, p->ps_offset, p->ps_cursor, p->ps_wgcnt,
+                  p->kphys_x, p->kphys_y, p->kphys_z, p->ps_count, p->ps_tcount, p->ps_offset, p->ps_cursor, p->ps_wgcnt,
                   p->ps_seg, p->ps_misc, p->ps_scale, p->ps_faces};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -386,6 +416,86 @@ int pmx_step(pmx_plan p, float* x, float* v, int npx, int npy, int npz, int mode
   PMX_TRY(pmx_launch_readout(p, a, p->fmesh, 3, nullptr, c_kick, c_drift_post, what, s));
   mark(m, 8, s);
   return 0;
+}
+
+int pmx_forces_vjp(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, float scale, unsigned flags,
+                   const float* forces_bar, float* pos_bar, int accumulate, double* dot, void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
+  if (!forces_bar || !pos_bar) return pmx_set_error(1, "null cotangent");
+  hipStream_t s = (hipStream_t)stream;
+  PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
+  const size_t np = (size_t)npx * npy * npz;
+  PMX_TRY(pmx_paint_sort(p, a, 0.f, nullptr, 1, 1.f, s));
+  PMX_TRY(pmx_paint_deposit(p, a, nullptr, 1, 1.f, 1.f, 0, p->rho, 1, 0, s));
+  PMX_TRY(pot_force_mesh_from(p, p->rho, scale, flags, p->fmesh, s));
+  PMX_TRY(pmx_launch_absmax(forces_bar, 3 * np, p->ps_misc + 3, s));
+  return force_adjoint(p, a, forces_bar, 1.f, scale, flags, pos_bar, accumulate, dot, s);
+}
+
+int pmx_force_mesh_vjp(pmx_plan p, const float* fmesh_bar, float scale, unsigned flags, float* density_bar,
+                       void* stream) {
+  PMX_TRY(check_plan(p));
+  if (!fmesh_bar || !density_bar) return pmx_set_error(1, "null buffer");
+  hipStream_t s = (hipStream_t)stream;
+  PMX_TRY(pmx_launch_div4(p, fmesh_bar, fmesh_bar + 1, fmesh_bar + 2, 3, p->rho, s));
+  PMX_TRY(pot_only(p, p->rho, scale, flags, s));
+  PMX_HIP(hipMemcpyAsync(density_bar, p->rho, p->ncell * sizeof(float), hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+int pmx_paint_cic_vjp(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, const float* weights,
+                      float weight, const float* mesh_bar, float* pos_bar, float* weights_bar, int accumulate,
+                      void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
+  if (!mesh_bar || !pos_bar) return pmx_set_error(1, "null cotangent");
+  PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
+  return pmx_launch_vjp_gather(p, a, 1, nullptr, 0.f, nullptr, mesh_bar, weights, weight, pos_bar, accumulate,
+                               weights_bar, nullptr, (hipStream_t)stream);
+}
+
+int pmx_readout_cic_vjp(pmx_plan p, const float* mesh, int ncomp, const float* pos, int npx, int npy, int npz, int mode,
+                        const float* out_bar, float* mesh_bar, float* pos_bar, int accumulate, void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
+  if (ncomp != 1 && ncomp != 3) return pmx_set_error(1, "ncomp must be 1 or 3");
+  if (!out_bar || (!mesh_bar && !pos_bar) || (pos_bar && !mesh)) return pmx_set_error(1, "null buffer");
+  hipStream_t s = (hipStream_t)stream;
+  PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
+  if (pos_bar)
+    PMX_TRY(pmx_launch_vjp_gather(p, a, ncomp, out_bar, 1.f, mesh, nullptr, nullptr, 0.f, pos_bar, accumulate,
+                                  nullptr, nullptr, s));
+  if (mesh_bar) {
+    PMX_TRY(pmx_paint_sort(p, a, 0.f, out_bar, ncomp, 0.f, s));
+    for (int d = 0; d < ncomp; ++d)
+      PMX_TRY(pmx_paint_deposit(p, a, out_bar + d, ncomp, 1.f, 0.f, 1, mesh_bar + d, ncomp, accumulate, s));
+  }
+  return 0;
+}
+
+int pmx_step_reverse_vjp(pmx_plan p, float* x, float* v, int npx, int npy, int npz, int mode, float force_scale,
+                         float c_kick, float c_drift, unsigned flags, float* x_bar, float* v_bar, double* coef_bar,
+                         void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, x, npx, npy, npz, mode));
+  if (!v || !x_bar || !v_bar || !coef_bar) return pmx_set_error(1, "null buffer");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t n = (size_t)npx * npy * npz * 3;
+  PMX_TRY(pmx_launch_fill_d(coef_bar, 2, s));
+  // cotangents through the drift x' = x + v' cd (v' the kicked velocity, still in v)
+  PMX_TRY(pmx_launch_adj_prep(x_bar, v_bar, v, n, c_drift, coef_bar, p->ps_misc + 3, s));
+  // reverse the state: x -= v cd, then v -= F(x) ck (sorted paint, force mesh kept in p->fmesh)
+  PartArgs a = part_args(x, v, npx, npy, npz, mode, p);
+  PMX_TRY(pmx_paint_sort(p, a, -c_drift, nullptr, 1, 1.f, s));
+  a.v = nullptr;
+  PMX_TRY(pmx_paint_deposit(p, a, nullptr, 1, 1.f, 1.f, 0, p->rho, 1, 0, s));
+  PMX_TRY(pot_force_mesh_from(p, p->rho, force_scale, flags, p->fmesh, s));
+  a.v = v;
+  PMX_TRY(pmx_launch_readout(p, a, p->fmesh, 3, nullptr, -c_kick, 0.f, 2, s));
+  a.v = nullptr;
+  // x_bar += J(x)^T (ck v_bar); coef_bar[1] = v_bar . F(x)
+  return force_adjoint(p, a, v_bar, c_kick, force_scale, flags, x_bar, 1, coef_bar + 1, s);
 }
 
 int pmx_plan_timing(pmx_plan p, int enable) {

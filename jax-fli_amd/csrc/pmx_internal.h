@@ -40,11 +40,11 @@ struct pmx_plan_t {
   float* fmesh;     // force mesh [x][y][z][3]
   double *kphys_x, *kphys_y, *kphys_z;  // estimator wavevectors (h/Mpc) per bin, set by pmx_plan_set_box
   bool box_set;
-  // sorted paint (pmx_particles.hip): per paint tile particle counts (zero between paints), segment
-  // offsets, reservation cursors; per particle sub-block window counts; segments of particle
+  // sorted paint (pmx_particles.hip): per paint tile particle counts (zero between paints), the
+  // last sort's counts, segment offsets, reservation cursors; per particle sub-block window counts; segments of particle
   // indices; misc = {max |weight| bits, far entries, far entries of the last paint}; fixed-point
   // scale {2^k, 2^-k}
-  unsigned *ps_count, *ps_offset, *ps_cursor, *ps_wgcnt, *ps_seg, *ps_misc;
+  unsigned *ps_count, *ps_tcount, *ps_offset, *ps_cursor, *ps_wgcnt, *ps_seg, *ps_misc;
   float* ps_scale;
   float* ps_faces;  // per tile: its upper face cells (pmx_paint_face_cells floats)
   size_t ps_wg_cap, ps_seg_cap;
@@ -111,6 +111,22 @@ int pmx_paint_face_cells(const pmx_plan_t* p);
 int pmx_launch_paint(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, float cd_pre,
                      float* mesh, int accumulate, int atomic, hipStream_t s);
 int pmx_launch_fill(float* a, size_t n, float v, hipStream_t s);
+// the sorted paint in two halves: sort (count, scan, scatter) then deposit(s) of the sorted particles
+int pmx_paint_sort(pmx_plan_t* p, const PartArgs& a, float cd_pre, const float* w, int wstride, float w0,
+                   hipStream_t s);
+int pmx_paint_deposit(pmx_plan_t* p, const PartArgs& a, const float* w, int wstride, float wmul, float w0,
+                      int is_signed, float* mesh, int mstride, int accumulate, hipStream_t s);
+// adjoint building blocks (pmx_particles.hip)
+int pmx_launch_absmax(const float* a, size_t n, unsigned* bits, hipStream_t s);
+int pmx_launch_set_scale(unsigned* bits, float mul, float* scale, hipStream_t s);
+int pmx_launch_div4(pmx_plan_t* p, const float* U0, const float* U1, const float* U2, int ustride, float* out,
+                    hipStream_t s);
+int pmx_launch_vjp_gather(pmx_plan_t* p, const PartArgs& a, int ncomp, const float* u, float umul, const float* M,
+                          const float* rhob, const float* w, float w0, float* xbar, int accumulate, float* wbar,
+                          double* dot, hipStream_t s);
+int pmx_launch_adj_prep(const float* xbar, float* vbar, const float* v, size_t n, float cd, double* dot,
+                        unsigned* bits, hipStream_t s);
+int pmx_launch_fill_d(double* a, size_t n, hipStream_t s);
 int pmx_launch_readout(pmx_plan_t* p, const PartArgs& a, const float* mesh, int ncomp, float* out,
                        float ck, float cd_post, int what, hipStream_t s);
 

@@ -352,9 +352,11 @@ __device__ __forceinline__ void for_particles(const PartArgs& a, const SortGeom&
 
 // Pass 1: optional pre-drift x += v * cd (written back), then per-tile particle counts. The window
 // histogram of each sub-block is kept in wgcnt for the scatter pass. far: particles outside windows.
+// With weights (wstride values per particle) the largest |weight| goes to wmax_bits.
 __global__ void __launch_bounds__(256) k_sort_count(PartArgs a, SortGeom g, float cd, unsigned* __restrict__ count,
                                                     unsigned* __restrict__ wgcnt, unsigned* __restrict__ far,
-                                                    const float* __restrict__ w, unsigned* __restrict__ wmax_bits) {
+                                                    const float* __restrict__ w, int wstride,
+                                                    unsigned* __restrict__ wmax_bits) {
   __shared__ unsigned lcnt[SORT_SLOTS];
   __shared__ int3 cen;
   const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
@@ -387,7 +389,8 @@ __global__ void __launch_bounds__(256) k_sort_count(PartArgs a, SortGeom g, floa
       const int l = r * 256 + (int)threadIdx.x;
       const int i = i0 + (l >> 9), j = j0 + ((l >> 6) & 7), k = k0 + (l & 63);
       if (i >= a.npx || j >= a.npy || k >= a.npz) continue;
-      m = fmaxf(m, fabsf(w[((unsigned)i * a.npy + j) * a.npz + k]));
+      const size_t q = (size_t)(((unsigned)i * a.npy + j) * a.npz + k) * wstride;
+      for (int c = 0; c < wstride; ++c) m = fmaxf(m, fabsf(w[q + c]));
     }
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
     if (__lane_id() == 0) atomicMax(wmax_bits, __float_as_uint(m));
@@ -401,9 +404,11 @@ __global__ void __launch_bounds__(256) k_sort_count(PartArgs a, SortGeom g, floa
   }
 }
 
-// Pass 2 (one workgroup): offsets = exclusive prefix of count, cursor = offsets; the fixed-point
-// scale 2^k (largest deposit 2^30) from the weight bound; far counter moved to far_last.
-__global__ void __launch_bounds__(1024) k_sort_scan(const unsigned* __restrict__ count, unsigned* __restrict__ offset,
+// Pass 2 (one workgroup): offsets = exclusive prefix of count, cursor = offsets, tcount = count
+// (count re-zeroed); the fixed-point scale 2^k (largest deposit 2^30) from the weight bound; far
+// counter moved to far_last.
+__global__ void __launch_bounds__(1024) k_sort_scan(unsigned* __restrict__ count, unsigned* __restrict__ tcount,
+                                                    unsigned* __restrict__ offset,
                                                     unsigned* __restrict__ cursor, int ntiles, float wmax_host,
                                                     unsigned* __restrict__ wmax_bits, float* __restrict__ scale,
                                                     unsigned* __restrict__ far, unsigned* __restrict__ far_last) {
@@ -423,9 +428,12 @@ __global__ void __launch_bounds__(1024) k_sort_scan(const unsigned* __restrict__
   }
   unsigned run = part[t] - sum;
   for (int i = beg; i < end; ++i) {
+    const unsigned c = count[i];
     offset[i] = run;
     cursor[i] = run;
-    run += count[i];
+    tcount[i] = c;
+    count[i] = 0u;  // the next paint's count pass starts from zero
+    run += c;
   }
   if (t == 0) {
     const float m = wmax_host >= 0.f ? wmax_host : __uint_as_float(*wmax_bits);
@@ -474,14 +482,18 @@ __device__ __forceinline__ int face_index(int lx, int ly, int lz, int BX, int BY
 }
 
 // Pass 4: one workgroup per tile: deposit the 8 corners of the tile's particles into the padded LDS
-// tile, write the tile's cells to the mesh (accumulate: add) and its face cells to faces.
+// tile, write the tile's cells to the mesh (accumulate: add) and its face cells to faces. Particle
+// p carries w[p * wstride] * wmul (w != null) or w0. Reads only the sort's output, so several
+// deposits (e.g. the components of a vector) can follow one sort.
 template <bool SIGNED>
-__global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, const float* __restrict__ w, float w0,
-                                                       unsigned* __restrict__ count,
+__global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, const float* __restrict__ w,
+                                                       int wstride, float wmul, float w0,
+                                                       const unsigned* __restrict__ tcount,
                                                        const unsigned* __restrict__ offset,
                                                        const unsigned* __restrict__ seg,
                                                        const float* __restrict__ scale, float* __restrict__ mesh,
-                                                       float* __restrict__ faces, int face_stride, int accumulate) {
+                                                       int mstride, float* __restrict__ faces, int face_stride,
+                                                       int accumulate) {
   __shared__ unsigned long long tile[TILE_PAD_MAX];
   const int BX = 1 << g.sx, BY = 1 << g.sy, BZ = 1 << g.sz;
   const int PY = BY + 1, PZ = BZ + 1;
@@ -491,9 +503,9 @@ __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, c
             tx = (int)(L / ((unsigned)g.ntz * g.nty));
   const int T0x = tx << g.sx, T0y = ty << g.sy, T0z = tz << g.sz;
   for (int i = threadIdx.x; i < npad; i += blockDim.x) tile[i] = 0ull;
-  const unsigned n = count[L], beg = offset[L];
+  const unsigned n = tcount[L], beg = offset[L];
   const float S = scale[0];
-  const float wS = w0 * S;
+  const float wS = w0 * S, mS = wmul * S;
   // byte offsets of the +1 neighbours in the padded tile
   const int oz = 8, oy = PZ * 8, ox = PY * PZ * 8;
   char* lds = reinterpret_cast<char*>(tile);
@@ -517,7 +529,7 @@ __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, c
       if (a.mode == 0) lattice_of(a, g, p, i, j, k);
       const Base b = base_of(a, g, i, j, k, xs[u]);
       const int base = (((b.cx - T0x) * PY + (b.cy - T0y)) * PZ + (b.cz - T0z)) * 8;
-      const float c = w ? w[p] * S : wS;
+      const float c = w ? w[(size_t)p * wstride] * mS : wS;
       const float wx[2] = {1.0f - fabsf(b.fx), 1.0f - fabsf(b.fx - 1.0f)};
       const float wy[2] = {1.0f - fabsf(b.fy), 1.0f - fabsf(b.fy - 1.0f)};
       // c is folded into the z weights: exact when c is a power of two (scalar weight 1)
@@ -533,12 +545,11 @@ __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, c
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) count[L] = 0u;
   const double inv = (double)scale[1];
   const int ncell = BX * BY * BZ;
   for (int q = threadIdx.x; q < ncell; q += blockDim.x) {
     const int iz = q & (BZ - 1), iy = (q >> g.sz) & (BY - 1), ix = q >> (g.sz + g.sy);
-    float* dst = mesh + ((size_t)(T0x + ix) * g.NY + (T0y + iy)) * g.NZ + (T0z + iz);
+    float* dst = mesh + (((size_t)(T0x + ix) * g.NY + (T0y + iy)) * g.NZ + (T0z + iz)) * mstride;
     const float v = (float)((double)(long long)tile[(ix * PY + iy) * PZ + iz] * inv);
     *dst = accumulate ? *dst + v : v;
   }
@@ -568,7 +579,7 @@ __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, c
 // Pass 5: cells on a tile's lower faces receive the face cells of its lower neighbours (x, y, z,
 // xy, xz, yz, xyz, in that order). One thread per (tile, lower-shell cell).
 __global__ void __launch_bounds__(256) k_paint_faces(SortGeom g, const float* __restrict__ faces, int face_stride,
-                                                     float* __restrict__ mesh, int shell) {
+                                                     float* __restrict__ mesh, int mstride, int shell) {
   const int BX = 1 << g.sx, BY = 1 << g.sy, BZ = 1 << g.sz;
   const size_t gid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   const size_t ntiles = (size_t)g.ntx * g.nty * g.ntz;
@@ -605,8 +616,176 @@ __global__ void __launch_bounds__(256) k_paint_faces(SortGeom g, const float* __
     const unsigned nb = ((unsigned)nx * g.nty + ny) * g.ntz + nz;
     sum += faces[(size_t)nb * face_stride + face_index(ex ? BX : lx, ey ? BY : ly, ez ? BZ : lz, BX, BY, BZ)];
   }
-  float* dst = mesh + ((size_t)((tx << g.sx) + lx) * g.NY + ((ty << g.sy) + ly)) * g.NZ + ((tz << g.sz) + lz);
+  float* dst =
+      mesh + (((size_t)((tx << g.sx) + lx) * g.NY + ((ty << g.sy) + ly)) * g.NZ + ((tz << g.sz) + lz)) * mstride;
   *dst = *dst + sum;
+}
+
+// ------------------------------------------------------------------ adjoint (VJP) kernels
+// Sign of the reference's CIC weight derivatives: d(1 - |f|)/df = -sign(f), d(1 - |f - 1|)/df =
+// -sign(f - 1), with sign(0) = 0.
+__device__ __forceinline__ float sgnf(float v) { return v > 0.f ? 1.f : (v < 0.f ? -1.f : 0.f); }
+
+// Largest |a[i]| (non-negative floats order like their bits) into *bits.
+__global__ void __launch_bounds__(256) k_absmax(const float* __restrict__ a, size_t n, unsigned* __restrict__ bits) {
+  float m = 0.f;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(a[i]));
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if (__lane_id() == 0 && m > 0.f) atomicMax(bits, __float_as_uint(m));
+}
+
+// Fixed-point paint scale for deposits of at most |mul| * (*bits as float); *bits re-zeroed.
+__global__ void k_set_scale(unsigned* __restrict__ bits, float mul, float* __restrict__ scale) {
+  const float m = __uint_as_float(*bits) * fabsf(mul);
+  int e = 0;
+  if (m > 0.f) (void)frexpf(m, &e);
+  e = max(-90, min(e, 90));
+  scale[0] = ldexpf(1.0f, 30 - e);
+  scale[1] = ldexpf(1.0f, e - 30);
+  *bits = 0u;
+}
+
+// out = S^T U = -sum_d S_d U_d, S_d the 4-point stencil 8 (u[+1] - u[-1]) - (u[+2] - u[-2]) along d;
+// component d of U at U[d] + cell * ustride. One workgroup per (x, y) row, as k_grad4.
+__global__ void __launch_bounds__(256) k_div4(const float* __restrict__ U0, const float* __restrict__ U1,
+                                              const float* __restrict__ U2, int ustride, float* __restrict__ out,
+                                              int NX, int NY, int NZ) {
+  const unsigned G = gridDim.x, b = blockIdx.x;
+  const unsigned row = (G % 8u == 0u) ? (b % 8u) * (G / 8u) + b / 8u : b;
+  const int x = (int)(row / (unsigned)NY), y = (int)(row % (unsigned)NY);
+  auto wr = [](int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); };
+  const size_t r0 = (size_t)row * NZ;
+  const size_t xp1 = ((size_t)wr(x + 1, NX) * NY + y) * NZ, xm1 = ((size_t)wr(x - 1, NX) * NY + y) * NZ;
+  const size_t xp2 = ((size_t)wr(x + 2, NX) * NY + y) * NZ, xm2 = ((size_t)wr(x - 2, NX) * NY + y) * NZ;
+  const size_t yp1 = ((size_t)x * NY + wr(y + 1, NY)) * NZ, ym1 = ((size_t)x * NY + wr(y - 1, NY)) * NZ;
+  const size_t yp2 = ((size_t)x * NY + wr(y + 2, NY)) * NZ, ym2 = ((size_t)x * NY + wr(y - 2, NY)) * NZ;
+  const size_t st = (size_t)ustride;
+  for (int z = threadIdx.x; z < NZ; z += blockDim.x) {
+    const float gx = 8.0f * (U0[(xp1 + z) * st] - U0[(xm1 + z) * st]) - (U0[(xp2 + z) * st] - U0[(xm2 + z) * st]);
+    const float gy = 8.0f * (U1[(yp1 + z) * st] - U1[(ym1 + z) * st]) - (U1[(yp2 + z) * st] - U1[(ym2 + z) * st]);
+    const float gz = 8.0f * (U2[(r0 + wr(z + 1, NZ)) * st] - U2[(r0 + wr(z - 1, NZ)) * st]) -
+                     (U2[(r0 + wr(z + 2, NZ)) * st] - U2[(r0 + wr(z - 2, NZ)) * st]);
+    out[r0 + z] = -(gx + gy + gz);
+  }
+}
+
+// block sum of a double into *dst (one atomic per workgroup)
+__device__ __forceinline__ void block_sum_add(double v, double* dst) {
+  __shared__ double part[8];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int wv = threadIdx.x >> 6;
+  if (__lane_id() == 0) part[wv] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += part[i];
+    atomicAdd(dst, t);
+  }
+}
+
+// Position cotangent of readout and/or paint at the particles:
+//   xbar_p (+)= sum_c grad w_c(x_p) * ( u_p . M[c]  +  wt_p * rhob[c] )
+// with u_p = umul * u[p] (NC components, HAS_U: readout of the NC-component mesh M), rhob the
+// density cotangent of a paint with particle weights wt (HAS_RHO). WBAR: wbar_p = sum_c w_c rhob[c]
+// (the weight cotangent). dot (HAS_U, non-null): += sum_p u[p] . F_p with F_p = sum_c w_c M[c].
+template <int NC, bool HAS_U, bool HAS_RHO, bool WBAR>
+__global__ void __launch_bounds__(256) k_vjp_gather(PartArgs a, const float* __restrict__ u, float umul,
+                                                    const float* __restrict__ M, const float* __restrict__ rhob,
+                                                    const float* __restrict__ w, float w0, float* __restrict__ xbar,
+                                                    int accumulate, float* __restrict__ wbar, double* __restrict__ dot,
+                                                    int NX, int NY, int NZ, int rx, int ry, int rz) {
+  const unsigned np = (unsigned)a.npx * a.npy * a.npz;
+  const Chunk ch = my_chunk(np);
+  LatticeWalk lw;
+  lw.init(ch.begin + threadIdx.x, blockDim.x, a.npy, a.npz);
+  double local = 0.0;
+  for (unsigned p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x, lw.step()) {
+    const float3 x = ld3(a.x, p);
+    Corner1D cx, cy, cz;
+    float fx, fy, fz;
+    if (a.mode == 0) {
+      cx = corner_rel(lw.i * rx, x.x, NX);
+      cy = corner_rel(lw.j * ry, x.y, NY);
+      cz = corner_rel(lw.k * rz, x.z, NZ);
+      fx = x.x - floorf(x.x);
+      fy = x.y - floorf(x.y);
+      fz = x.z - floorf(x.z);
+    } else {
+      cx = corner_abs(x.x, NX);
+      cy = corner_abs(x.y, NY);
+      cz = corner_abs(x.z, NZ);
+      fx = x.x - floorf(x.x);
+      fy = x.y - floorf(x.y);
+      fz = x.z - floorf(x.z);
+    }
+    const float dwx[2] = {-sgnf(fx), -sgnf(fx - 1.0f)}, dwy[2] = {-sgnf(fy), -sgnf(fy - 1.0f)},
+                dwz[2] = {-sgnf(fz), -sgnf(fz - 1.0f)};
+    const float wxv[2] = {cx.w0, cx.w1}, wyv[2] = {cy.w0, cy.w1}, wzv[2] = {cz.w0, cz.w1};
+    const Cic q = cic_from(cx, cy, cz, NY, NZ);
+    float ur[NC > 0 ? NC : 1] = {}, uu[NC > 0 ? NC : 1] = {};
+    if (HAS_U) {
+#pragma unroll
+      for (int d = 0; d < NC; ++d) {
+        ur[d] = u[(size_t)p * NC + d];
+        uu[d] = ur[d] * umul;
+      }
+    }
+    const float wt = HAS_RHO ? (w ? w[p] : w0) : 0.f;
+    float gx = 0.f, gy = 0.f, gz = 0.f, ws = 0.f, Fd = 0.f;
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+      float sv = 0.f;
+      if (HAS_U) {
+        float um = 0.f, uf = 0.f;
+#pragma unroll
+        for (int d = 0; d < NC; ++d) {
+          const float m = M[(size_t)q.off[o] * NC + d];
+          um += uu[d] * m;
+          uf += ur[d] * m;
+        }
+        sv += um;
+        Fd += q.w[o] * uf;
+      }
+      if (HAS_RHO) {
+        const float r = rhob[q.off[o]];
+        sv += wt * r;
+        if (WBAR) ws += q.w[o] * r;
+      }
+      gx += sv * ((dwx[ox] * wyv[oy]) * wzv[oz]);
+      gy += sv * ((wxv[ox] * dwy[oy]) * wzv[oz]);
+      gz += sv * ((wxv[ox] * wyv[oy]) * dwz[oz]);
+    }
+    float3 g = make_float3(gx, gy, gz);
+    if (accumulate) {
+      const float3 o = ld3(xbar, p);
+      g = make_float3(o.x + g.x, o.y + g.y, o.z + g.z);
+    }
+    st3(xbar, p, g);
+    if (WBAR) wbar[p] = ws;
+    if (HAS_U) local += (double)Fd;
+  }
+  if (HAS_U && dot != nullptr) block_sum_add(local, dot);
+}
+
+// Reverse-adjoint preparation of a KKD step (v += F ck; x += v cd) before the state is reversed:
+// dot += <xbar, v>, vbar += cd * xbar, *bits = max |vbar| (for the deposit scale).
+__global__ void __launch_bounds__(256) k_adj_prep(const float* __restrict__ xbar, float* __restrict__ vbar,
+                                                  const float* __restrict__ v, size_t n, float cd,
+                                                  double* __restrict__ dot, unsigned* __restrict__ bits) {
+  double local = 0.0;
+  float m = 0.f;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float xb = xbar[i];
+    local += (double)xb * (double)v[i];
+    const float vb = vbar[i] + cd * xb;
+    vbar[i] = vb;
+    m = fmaxf(m, fabsf(vb));
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if (__lane_id() == 0 && m > 0.f) atomicMax(bits, __float_as_uint(m));
+  block_sum_add(local, dot);
 }
 
 // x <- x + v*c
@@ -1038,6 +1217,98 @@ int pmx_paint_face_cells(const pmx_plan_t* p) {
   return (BY + 1) * (BZ + 1) + BX * (BZ + 1) + BX * BY;
 }
 
+static SortGeom sort_geom(const pmx_plan_t* p, const PartArgs& a, int& ntiles) {
+  int rx, ry, rz;
+  ratios(const_cast<pmx_plan_t*>(p), a, rx, ry, rz);
+  SortGeom g;
+  g.NX = p->nx;
+  g.NY = p->ny;
+  g.NZ = p->nz;
+  pmx_paint_tiles(p, g.sx, g.sy, g.sz, ntiles);
+  g.ntx = p->nx >> g.sx;
+  g.nty = p->ny >> g.sy;
+  g.ntz = p->nz >> g.sz;
+  g.rx = a.mode == 0 ? rx : 1;
+  g.ry = a.mode == 0 ? ry : 1;
+  g.rz = a.mode == 0 ? rz : 1;
+  g.nbx = (a.npx + SB_X - 1) / SB_X;
+  g.nby = (a.npy + SB_Y - 1) / SB_Y;
+  g.nbz = (a.npz + SB_Z - 1) / SB_Z;
+  g.div_npz = make_fdiv((unsigned)a.npz);
+  g.div_npy = make_fdiv((unsigned)a.npy);
+  const bool pow2 = (a.npz & (a.npz - 1)) == 0 && (a.npy & (a.npy - 1)) == 0;
+  g.lpz = pow2 ? ilog2(a.npz) : -1;
+  g.lpy = pow2 ? ilog2(a.npy) : -1;
+  return g;
+}
+
+// Sort the particles of `a` by paint tile ([a.v] pre-drift x += v * cd_pre first). The fixed-point
+// scale for the following deposits comes from |w0| or, with w (wstride values per particle), from
+// the largest |w|.
+int pmx_paint_sort(pmx_plan_t* p, const PartArgs& a, float cd_pre, const float* w, int wstride, float w0,
+                   hipStream_t s) {
+  int ntiles;
+  const SortGeom g = sort_geom(p, a, ntiles);
+  const size_t np = (size_t)a.npx * a.npy * a.npz;
+  const size_t nsb = (size_t)g.nbx * g.nby * g.nbz;
+  if (p->ps_wg_cap < nsb * SORT_SLOTS) {
+    if (p->ps_wgcnt) (void)hipFree(p->ps_wgcnt);
+    p->ps_wgcnt = nullptr;
+    p->ps_wg_cap = 0;
+    hipError_t e = hipMalloc((void**)&p->ps_wgcnt, nsb * SORT_SLOTS * sizeof(unsigned));
+    if (e != hipSuccess) return pmx_set_error(4, std::string("paint sort counts: ") + hipGetErrorString(e));
+    p->ps_wg_cap = nsb * SORT_SLOTS;
+  }
+  if (p->ps_seg_cap < np) {
+    if (p->ps_seg) (void)hipFree(p->ps_seg);
+    p->ps_seg = nullptr;
+    p->ps_seg_cap = 0;
+    hipError_t e = hipMalloc((void**)&p->ps_seg, np * sizeof(unsigned));
+    if (e != hipSuccess) return pmx_set_error(4, std::string("paint sort segments: ") + hipGetErrorString(e));
+    p->ps_seg_cap = np;
+  }
+  unsigned* wmax_bits = p->ps_misc;
+  unsigned* far = p->ps_misc + 1;
+  unsigned* far_last = p->ps_misc + 2;
+  hipLaunchKernelGGL(k_sort_count, dim3((unsigned)nsb), dim3(256), 0, s, a, g, cd_pre, p->ps_count, p->ps_wgcnt, far, w,
+                     wstride, wmax_bits);
+  int rc = pmx_check_launch("sort_count");
+  if (rc) return rc;
+  PartArgs b = a;
+  b.v = nullptr;
+  hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, p->ps_count, p->ps_tcount, p->ps_offset, p->ps_cursor,
+                     ntiles, w ? -1.0f : fabsf(w0), wmax_bits, p->ps_scale, far, far_last);
+  if ((rc = pmx_check_launch("sort_scan"))) return rc;
+  hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)nsb), dim3(256), 0, s, b, g, p->ps_cursor, p->ps_wgcnt, p->ps_seg);
+  return pmx_check_launch("sort_scatter");
+}
+
+// Deposit the last sorted particles into `mesh`: particle p carries w[p * wstride] * wmul (w != null)
+// or w0; `is_signed` when any carried value may be negative.
+int pmx_paint_deposit(pmx_plan_t* p, const PartArgs& a_in, const float* w, int wstride, float wmul, float w0,
+                      int is_signed, float* mesh, int mstride, int accumulate, hipStream_t s) {
+  PartArgs a = a_in;
+  a.v = nullptr;
+  int ntiles;
+  const SortGeom g = sort_geom(p, a, ntiles);
+  const int BX = 1 << g.sx, BY = 1 << g.sy, BZ = 1 << g.sz;
+  const int face_stride = pmx_paint_face_cells(p);
+  if (is_signed)
+    hipLaunchKernelGGL(k_paint_deposit<true>, dim3(ntiles), dim3(512), 0, s, a, g, w, wstride, wmul, w0, p->ps_tcount,
+                       p->ps_offset, p->ps_seg, p->ps_scale, mesh, mstride, p->ps_faces, face_stride, accumulate);
+  else
+    hipLaunchKernelGGL(k_paint_deposit<false>, dim3(ntiles), dim3(512), 0, s, a, g, w, wstride, wmul, w0,
+                       p->ps_tcount, p->ps_offset, p->ps_seg, p->ps_scale, mesh, mstride, p->ps_faces, face_stride,
+                       accumulate);
+  int rc = pmx_check_launch("paint_deposit");
+  if (rc) return rc;
+  const int shell = BX * BY * BZ - (BX - 1) * (BY - 1) * (BZ - 1);
+  const size_t nthreads = (size_t)ntiles * shell;
+  hipLaunchKernelGGL(k_paint_faces, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s, g, p->ps_faces,
+                     face_stride, mesh, mstride, shell);
+  return pmx_check_launch("paint_faces");
+}
+
 // Paint into `mesh` (accumulate=0 overwrites it). Default: the sorted owner-computes paint, both
 // modes; `atomic`: one-pass scatter with global atomics (after a separate pre-drift pass).
 // a.v != null applies the pre-drift x += v * cd_pre first (written back to x).
@@ -1063,70 +1334,9 @@ int pmx_launch_paint(pmx_plan_t* p, const PartArgs& a_in, const float* w, float 
                        rz);
     return pmx_check_launch("paint");
   }
-  SortGeom g;
-  g.NX = p->nx;
-  g.NY = p->ny;
-  g.NZ = p->nz;
-  int ntiles;
-  pmx_paint_tiles(p, g.sx, g.sy, g.sz, ntiles);
-  g.ntx = p->nx >> g.sx;
-  g.nty = p->ny >> g.sy;
-  g.ntz = p->nz >> g.sz;
-  g.rx = a.mode == 0 ? rx : 1;
-  g.ry = a.mode == 0 ? ry : 1;
-  g.rz = a.mode == 0 ? rz : 1;
-  g.nbx = (a.npx + SB_X - 1) / SB_X;
-  g.nby = (a.npy + SB_Y - 1) / SB_Y;
-  g.nbz = (a.npz + SB_Z - 1) / SB_Z;
-  g.div_npz = make_fdiv((unsigned)a.npz);
-  g.div_npy = make_fdiv((unsigned)a.npy);
-  const bool pow2 = (a.npz & (a.npz - 1)) == 0 && (a.npy & (a.npy - 1)) == 0;
-  g.lpz = pow2 ? ilog2(a.npz) : -1;
-  g.lpy = pow2 ? ilog2(a.npy) : -1;
-  const size_t nsb = (size_t)g.nbx * g.nby * g.nbz;
-  if (p->ps_wg_cap < nsb * SORT_SLOTS) {
-    if (p->ps_wgcnt) (void)hipFree(p->ps_wgcnt);
-    p->ps_wgcnt = nullptr;
-    p->ps_wg_cap = 0;
-    hipError_t e = hipMalloc((void**)&p->ps_wgcnt, nsb * SORT_SLOTS * sizeof(unsigned));
-    if (e != hipSuccess) return pmx_set_error(4, std::string("paint sort counts: ") + hipGetErrorString(e));
-    p->ps_wg_cap = nsb * SORT_SLOTS;
-  }
-  if (p->ps_seg_cap < np) {
-    if (p->ps_seg) (void)hipFree(p->ps_seg);
-    p->ps_seg = nullptr;
-    p->ps_seg_cap = 0;
-    hipError_t e = hipMalloc((void**)&p->ps_seg, np * sizeof(unsigned));
-    if (e != hipSuccess) return pmx_set_error(4, std::string("paint sort segments: ") + hipGetErrorString(e));
-    p->ps_seg_cap = np;
-  }
-  unsigned* wmax_bits = p->ps_misc;
-  unsigned* far = p->ps_misc + 1;
-  unsigned* far_last = p->ps_misc + 2;
-  hipLaunchKernelGGL(k_sort_count, dim3((unsigned)nsb), dim3(256), 0, s, a, g, cd_pre, p->ps_count, p->ps_wgcnt, far, w,
-                     wmax_bits);
-  int rc = pmx_check_launch("sort_count");
+  int rc = pmx_paint_sort(p, a, cd_pre, w, 1, w0, s);
   if (rc) return rc;
-  a.v = nullptr;
-  hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, p->ps_count, p->ps_offset, p->ps_cursor, ntiles,
-                     w ? -1.0f : fabsf(w0), wmax_bits, p->ps_scale, far, far_last);
-  if ((rc = pmx_check_launch("sort_scan"))) return rc;
-  hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)nsb), dim3(256), 0, s, a, g, p->ps_cursor, p->ps_wgcnt, p->ps_seg);
-  if ((rc = pmx_check_launch("sort_scatter"))) return rc;
-  const int BX = 1 << g.sx, BY = 1 << g.sy, BZ = 1 << g.sz;
-  const int face_stride = pmx_paint_face_cells(p);
-  if (w != nullptr || w0 < 0.f)
-    hipLaunchKernelGGL(k_paint_deposit<true>, dim3(ntiles), dim3(512), 0, s, a, g, w, w0, p->ps_count, p->ps_offset,
-                       p->ps_seg, p->ps_scale, mesh, p->ps_faces, face_stride, accumulate);
-  else
-    hipLaunchKernelGGL(k_paint_deposit<false>, dim3(ntiles), dim3(512), 0, s, a, g, w, w0, p->ps_count, p->ps_offset,
-                       p->ps_seg, p->ps_scale, mesh, p->ps_faces, face_stride, accumulate);
-  if ((rc = pmx_check_launch("paint_deposit"))) return rc;
-  const int shell = BX * BY * BZ - (BX - 1) * (BY - 1) * (BZ - 1);
-  const size_t nthreads = (size_t)ntiles * shell;
-  hipLaunchKernelGGL(k_paint_faces, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s, g, p->ps_faces,
-                     face_stride, mesh, shell);
-  return pmx_check_launch("paint_faces");
+  return pmx_paint_deposit(p, a, w, 1, 1.0f, w0, w != nullptr || w0 < 0.f, mesh, 1, accumulate, s);
 }
 
 int pmx_launch_readout(pmx_plan_t* p, const PartArgs& a, const float* mesh, int ncomp, float* out, float ck,
@@ -1204,4 +1414,51 @@ int pmx_launch_white_noise(float* out, size_t n, uint64_t seed, hipStream_t s) {
 int pmx_launch_lpt1(const float* F, float* dx, float* pv, size_t n, float D, float c, hipStream_t s) {
   hipLaunchKernelGGL(k_lpt1, dim3(grid_for(n, 256)), dim3(256), 0, s, F, dx, pv, n, D, c);
   return pmx_check_launch("lpt1");
+}
+
+// ------------------------------------------------------------------ adjoint launchers
+int pmx_launch_absmax(const float* a, size_t n, unsigned* bits, hipStream_t s) {
+  hipLaunchKernelGGL(k_absmax, dim3(grid_for(n, 256) < 4096 ? grid_for(n, 256) : 4096), dim3(256), 0, s, a, n, bits);
+  return pmx_check_launch("absmax");
+}
+int pmx_launch_set_scale(unsigned* bits, float mul, float* scale, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_scale, dim3(1), dim3(1), 0, s, bits, mul, scale);
+  return pmx_check_launch("set_scale");
+}
+int pmx_launch_div4(pmx_plan_t* p, const float* U0, const float* U1, const float* U2, int ustride, float* out,
+                    hipStream_t s) {
+  const unsigned rows = (unsigned)p->nx * p->ny;
+  const int threads = p->nz < 256 ? p->nz : 256;
+  hipLaunchKernelGGL(k_div4, dim3(rows), dim3(threads), 0, s, U0, U1, U2, ustride, out, p->nx, p->ny, p->nz);
+  return pmx_check_launch("div4");
+}
+int pmx_launch_vjp_gather(pmx_plan_t* p, const PartArgs& a, int ncomp, const float* u, float umul, const float* M,
+                          const float* rhob, const float* w, float w0, float* xbar, int accumulate, float* wbar,
+                          double* dot, hipStream_t s) {
+  int rx, ry, rz;
+  ratios(p, a, rx, ry, rz);
+  const size_t np = (size_t)a.npx * a.npy * a.npz;
+  const dim3 G(grid_particles(np)), B(256);
+#define PMX_G(NC, HU, HR, WB)                                                                                         \
+  hipLaunchKernelGGL((k_vjp_gather<NC, HU, HR, WB>), G, B, 0, s, a, u, umul, M, rhob, w, w0, xbar, accumulate, wbar, \
+                     dot, p->nx, p->ny, p->nz, rx, ry, rz)
+  if (u && rhob) {
+    if (ncomp == 3) PMX_G(3, true, true, false);
+    else PMX_G(1, true, true, false);
+  } else if (u) {
+    if (ncomp == 3) PMX_G(3, true, false, false);
+    else PMX_G(1, true, false, false);
+  } else if (wbar) {
+    PMX_G(1, false, true, true);
+  } else {
+    PMX_G(1, false, true, false);
+  }
+#undef PMX_G
+  return pmx_check_launch("vjp_gather");
+}
+int pmx_launch_adj_prep(const float* xbar, float* vbar, const float* v, size_t n, float cd, double* dot,
+                        unsigned* bits, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_prep, dim3(grid_for(n, 256) < 4096 ? grid_for(n, 256) : 4096), dim3(256), 0, s, xbar, vbar,
+                     v, n, cd, dot, bits);
+  return pmx_check_launch("adj_prep");
 }

@@ -211,3 +211,83 @@ def paint_flat(pos: torch.Tensor, mesh_shape, flatsky_npix, zlo, zhi, scale, mod
                                 float(zlo), float(zhi), float(scale), ptr(out), int(accumulate), stream_ptr()),
           "pmx_paint_flat")
     return out
+
+
+# ---------------------------------------------------------------------------- adjoints
+def _zeros_like_or(t, like):
+    return torch.zeros_like(like) if t is None else t
+
+
+def paint_vjp(pos: torch.Tensor, mesh_bar: torch.Tensor, mode="relative", weights=1.0, want_weights=False):
+    """(pos_bar, weights_bar | None): cotangents of paint(pos, weights) for the mesh cotangent mesh_bar."""
+    pos = _f32(pos, "pos")
+    mesh_bar = _f32(mesh_bar, "mesh_bar")
+    npx, npy, npz = _pgrid(pos)
+    plan = get_plan(tuple(mesh_bar.shape), pos.device)
+    w, w0 = _weights(weights, npx * npy * npz)
+    pos_bar = torch.empty_like(pos)
+    wbar = torch.empty((npx, npy, npz), dtype=torch.float32, device=pos.device) if want_weights else None
+    check(load().pmx_paint_cic_vjp(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], ptr(w), w0, ptr(mesh_bar),
+                                   ptr(pos_bar), ptr(wbar), 0, stream_ptr()), "pmx_paint_cic_vjp")
+    return pos_bar, wbar
+
+
+def readout_vjp(mesh: torch.Tensor, pos: torch.Tensor, out_bar: torch.Tensor, mode="relative", want_mesh=True,
+                want_pos=True):
+    """(mesh_bar | None, pos_bar | None): cotangents of readout(mesh, pos) for out_bar."""
+    mesh = _f32(mesh, "mesh")
+    pos = _f32(pos, "pos")
+    out_bar = _f32(out_bar, "out_bar")
+    npx, npy, npz = _pgrid(pos)
+    ncomp = 1 if mesh.ndim == 3 else int(mesh.shape[3])
+    plan = get_plan(tuple(mesh.shape[:3]), pos.device)
+    mesh_bar = torch.empty_like(mesh) if want_mesh else None
+    pos_bar = torch.empty_like(pos) if want_pos else None
+    check(load().pmx_readout_cic_vjp(plan.handle, ptr(mesh), ncomp, ptr(pos), npx, npy, npz, MODES[mode],
+                                     ptr(out_bar), ptr(mesh_bar), ptr(pos_bar), 0, stream_ptr()),
+          "pmx_readout_cic_vjp")
+    return mesh_bar, pos_bar
+
+
+def force_mesh_vjp(fmesh_bar: torch.Tensor, scale=1.0, deconv=False) -> torch.Tensor:
+    """Density cotangent of force_mesh(delta, scale) for the force-mesh cotangent (nx, ny, nz, 3)."""
+    fmesh_bar = _f32(fmesh_bar, "fmesh_bar")
+    plan = get_plan(tuple(fmesh_bar.shape[:3]), fmesh_bar.device)
+    out = torch.empty(tuple(fmesh_bar.shape[:3]), dtype=torch.float32, device=fmesh_bar.device)
+    check(load().pmx_force_mesh_vjp(plan.handle, ptr(fmesh_bar), float(scale), _lib.DECONV if deconv else 0, ptr(out),
+                                    stream_ptr()), "pmx_force_mesh_vjp")
+    return out
+
+
+def forces_vjp(pos: torch.Tensor, mesh_shape, forces_bar: torch.Tensor, mode="relative", scale=1.0, deconv=False,
+               pos_bar: torch.Tensor | None = None, accumulate=False, dot: torch.Tensor | None = None) -> torch.Tensor:
+    """pos_bar (+)= J(pos)^T forces_bar for forces(pos, scale); dot (float64 device scalar) += forces_bar . F."""
+    pos = _f32(pos, "pos")
+    forces_bar = _f32(forces_bar, "forces_bar")
+    npx, npy, npz = _pgrid(pos)
+    plan = get_plan(mesh_shape, pos.device)
+    if pos_bar is None:
+        pos_bar = torch.empty_like(pos)
+        accumulate = False
+    check(load().pmx_forces_vjp(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], float(scale),
+                                _lib.DECONV if deconv else 0, ptr(forces_bar), ptr(pos_bar), int(accumulate),
+                                ptr(dot), stream_ptr()), "pmx_forces_vjp")
+    return pos_bar
+
+
+def step_reverse_vjp(x, v, x_bar, v_bar, mesh_shape, mode="relative", force_scale=1.0, c_kick=0.0, c_drift=0.0,
+                     deconv=False, coef_bar: torch.Tensor | None = None) -> torch.Tensor:
+    """In place: undo one reversible step (v += F ck; x += v cd) on (x, v) and pull (x_bar, v_bar) back
+    through it. Returns coef_bar (float64 device [dL/dc_drift, dL/dc_kick])."""
+    for t, nm in ((x, "x"), (v, "v"), (x_bar, "x_bar"), (v_bar, "v_bar")):
+        _f32(t, nm)
+        if not t.is_contiguous():
+            raise _lib.PmxError(f"{nm} must be contiguous")
+    npx, npy, npz = _pgrid(x)
+    plan = get_plan(mesh_shape, x.device)
+    if coef_bar is None:
+        coef_bar = torch.empty(2, dtype=torch.float64, device=x.device)
+    check(load().pmx_step_reverse_vjp(plan.handle, ptr(x), ptr(v), npx, npy, npz, MODES[mode], float(force_scale),
+                                      float(c_kick), float(c_drift), _lib.DECONV if deconv else 0, ptr(x_bar),
+                                      ptr(v_bar), ptr(coef_bar), stream_ptr()), "pmx_step_reverse_vjp")
+    return coef_bar

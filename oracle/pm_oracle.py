@@ -21,6 +21,8 @@ Parity status per function:
                                                pixels, equal-area counts, interpolation identities)
   paint_healpix normalisation / shell edges    parity unpinned (jaxpm.spherical is not in the
                                                reference tree and no reference test covers it)
+  cic_paint_vjp / cic_read_vjp /               pinned (float64 central differences along random
+  force_mesh_vjp / pm_forces_vjp                 directions; exact transpose identities)
 """
 
 from __future__ import annotations
@@ -422,6 +424,89 @@ def paint_healpix(pos, mesh_shape, box, observer, rmin, rmax, nside, scheme="bil
                 acc[pix[q]] += wt[p] * w[q]
     vol = (4 * np.pi / npix) * (rmax**3 - rmin**3) / 3.0
     return (acc / vol).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------- adjoints
+# Reverse-mode derivatives of the functions above, as jax.vjp produces them for the reference's
+# cic_paint_dx / cic_read_dx / pm_forces (differentiated inside _integrate_bwd,
+# src/jax_fli/pm/integrate.py:304-358): d(1 - |f|)/df = -sign(f), d(1 - |f - 1|)/df = -sign(f - 1).
+
+
+def _corner_grads(pos, mesh_shape, mode, np_shape):
+    """Base indices, weights w0/w1 and their derivatives dw0/dw1 per axis (float64)."""
+    x = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+    fl = np.floor(x)
+    f = x - fl
+    w0, w1 = 1.0 - np.abs(f), 1.0 - np.abs(f - 1.0)
+    d0, d1 = -np.sign(f), -np.sign(f - 1.0)
+    if mode == "relative":
+        base = _lattice(np_shape, mesh_shape) + fl.astype(np.int64)
+    else:
+        base = fl.astype(np.int64)
+    return base, (w0, w1), (d0, d1)
+
+
+def _corner_terms(pos, mesh_shape, mode):
+    """Yields (flat cell index, weight, gradient of the weight (N,3)) for the 8 corners."""
+    pos = np.asarray(pos)
+    base, (w0, w1), (d0, d1) = _corner_grads(pos, mesh_shape, mode, pos.shape[:3])
+    nx, ny, nz = mesh_shape
+    for ox, oy, oz in _offsets():
+        wx, dx = (w1[:, 0], d1[:, 0]) if ox else (w0[:, 0], d0[:, 0])
+        wy, dy = (w1[:, 1], d1[:, 1]) if oy else (w0[:, 1], d0[:, 1])
+        wz, dz = (w1[:, 2], d1[:, 2]) if oz else (w0[:, 2], d0[:, 2])
+        flat = (np.mod(base[:, 0] + ox, nx) * ny + np.mod(base[:, 1] + oy, ny)) * nz + np.mod(base[:, 2] + oz, nz)
+        grad = np.stack([dx * wy * wz, wx * dy * wz, wx * wy * dz], axis=-1)
+        yield flat, (wx * wy) * wz, grad
+
+
+def cic_paint_vjp(pos, mesh_bar, mode="relative", weight=1.0):
+    """(pos_bar, weight_bar) of <mesh_bar, cic_paint(pos, weight)> (float64)."""
+    pos = np.asarray(pos)
+    mb = np.asarray(mesh_bar, dtype=np.float64).reshape(-1)
+    n = int(np.prod(pos.shape[:3]))
+    wt = np.broadcast_to(np.asarray(weight, dtype=np.float64).reshape(-1), (n,))
+    pb = np.zeros((n, 3))
+    wb = np.zeros(n)
+    for flat, w, g in _corner_terms(pos, np.asarray(mesh_bar).shape, mode):
+        pb += g * (wt * mb[flat])[:, None]
+        wb += w * mb[flat]
+    return pb.reshape(pos.shape), wb.reshape(pos.shape[:3])
+
+
+def cic_read_vjp(mesh, pos, out_bar, mode="relative"):
+    """(mesh_bar, pos_bar) of <out_bar, cic_read(mesh, pos)> (float64); mesh (X,Y,Z[,C])."""
+    mesh = np.asarray(mesh, dtype=np.float64)
+    shape = mesh.shape[:3]
+    nc = 1 if mesh.ndim == 3 else mesh.shape[3]
+    m = mesh.reshape(-1, nc)
+    pos = np.asarray(pos)
+    ob = np.asarray(out_bar, dtype=np.float64).reshape(-1, nc)
+    mb = np.zeros_like(m)
+    pb = np.zeros((ob.shape[0], 3))
+    for flat, w, g in _corner_terms(pos, shape, mode):
+        for c in range(nc):
+            mb[:, c] += np.bincount(flat, weights=w * ob[:, c], minlength=m.shape[0])
+        pb += g * (ob * m[flat]).sum(axis=1)[:, None]
+    return mb.reshape(mesh.shape), pb.reshape(pos.shape)
+
+
+def force_mesh_vjp(fmesh_bar, deconv=False):
+    """Density cotangent of force_mesh: each component is ifft(K_i fft(delta)) with K_i imaginary
+    and odd, whose transpose is its negative, so delta_bar = -sum_i force_mesh(fbar_i)[..., i]."""
+    fb = np.asarray(fmesh_bar, dtype=np.float64)
+    return -sum(force_mesh(fb[..., i], deconv=deconv, dtype=np.float64)[..., i] for i in range(3))
+
+
+def pm_forces_vjp(pos, mesh_shape, forces_bar, mode="relative", scale=1.0, deconv=False):
+    """pos_bar of <forces_bar, pm_forces(pos) * scale> (float64): readout-weight term plus the
+    paint-weight term through the transposed force operator."""
+    rho = cic_paint(pos, mesh_shape, mode, dtype=np.float64)
+    fm = force_mesh(rho, deconv=deconv, dtype=np.float64) * scale
+    mesh_bar, pb_read = cic_read_vjp(fm, pos, forces_bar, mode)
+    rho_bar = force_mesh_vjp(mesh_bar, deconv=deconv) * scale
+    pb_paint, _ = cic_paint_vjp(pos, rho_bar, mode)
+    return pb_read + pb_paint
 
 
 # ---------------------------------------------------------------------------- CPU baseline step

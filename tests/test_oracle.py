@@ -242,3 +242,40 @@ def test_kkd_reverse_inverts_step_fp64():
     x0 = x1 - v1 * cd
     v0 = v1 - ff(x0) * ck
     assert np.abs(x0 - x).max() < 1e-12 and np.abs(v0 - v).max() < 1e-12
+
+
+# ---------------------------------------------------------------------------- adjoints
+def _dd64(fn, x, d, eps=1e-7):
+    x = np.asarray(x, dtype=np.float64)
+    return (fn(x + eps * d) - fn(x - eps * d)) / (2 * eps)
+
+
+@pytest.mark.parametrize("mode", ["relative", "absolute"])
+def test_adjoints_match_finite_differences(mode):
+    """Each oracle VJP against float64 central differences along random directions (exact for the
+    piecewise-polynomial CIC functions away from cell boundaries), plus the transposes."""
+    rng = np.random.default_rng(50)
+    n = 8
+    mesh = (n, n, n)
+    pos = rng.standard_normal((n, n, n, 3)) * 1.3 if mode == "relative" else rng.random((4, 4, 4, 3)) * n
+    d = rng.standard_normal(pos.shape)
+    mbar = rng.standard_normal(mesh)
+    pb, wb = orc.cic_paint_vjp(pos, mbar, mode, weight=0.6)
+    ref = _dd64(lambda x: (orc.cic_paint(x, mesh, mode, weight=0.6, dtype=np.float64) * mbar).sum(), pos, d)
+    assert abs((pb * d).sum() - ref) < 1e-7 * np.abs(pb).sum()
+    assert abs(wb.sum() - (orc.cic_paint(pos, mesh, mode, dtype=np.float64) * mbar).sum()) < 1e-9 * np.abs(wb).sum()
+    m3 = rng.standard_normal(mesh + (3,))
+    ob = rng.standard_normal(pos.shape)
+    mb, pb = orc.cic_read_vjp(m3, pos, ob, mode)
+    assert abs((mb * m3).sum() - (orc.cic_read(m3, pos, mode, dtype=np.float64) * ob).sum()) < 1e-9 * np.abs(mb).sum()
+    ref = _dd64(lambda x: (orc.cic_read(m3, x, mode, dtype=np.float64) * ob).sum(), pos, d)
+    assert abs((pb * d).sum() - ref) < 1e-7 * np.abs(pb).sum()
+    delta = rng.standard_normal(mesh)
+    for deconv in (False, True):
+        fm = orc.force_mesh(delta, deconv=deconv)
+        db = orc.force_mesh_vjp(m3, deconv=deconv)
+        assert abs((fm * m3).sum() - (delta * db).sum()) < 1e-9 * np.abs(fm * m3).sum()
+    fb = rng.standard_normal(pos.shape)
+    pb = orc.pm_forces_vjp(pos, mesh, fb, mode, scale=0.7)
+    ref = _dd64(lambda x: (orc.pm_forces(x, mesh, mode, scale=0.7) * fb).sum(), pos, d)
+    assert abs((pb * d).sum() - ref) < 1e-7 * np.abs(pb).sum()
