@@ -8,6 +8,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+from . import autodiff
 from . import cosmology as _c
 from . import engine
 from .fields import DensityField, DensityUnit, FieldStatus
@@ -33,10 +34,19 @@ def interpolate_initial_conditions(initial_field, mesh_size, box_size, *, cosmo=
     field = initial_field.array if isinstance(initial_field, DensityField) else initial_field
     if not isinstance(field, torch.Tensor):
         field = torch.as_tensor(np.asarray(field, dtype=np.float32), device="cuda")
-    k, pk = _pk_table(cosmo, pk_fn, mesh_size, box_size)
     factor = np.prod(mesh_size) / np.prod(box_size)
-    out = engine.color_field(field.to(torch.float32).contiguous(), box_size, torch.from_numpy(k),
-                             torch.from_numpy(pk * factor))
+
+    def table(c):
+        k, pk = _pk_table(c, pk_fn, mesh_size, box_size)
+        return k, pk * factor
+
+    def colour_fn(noise, c):
+        k, pt = table(c)
+        return engine.color_field(noise.to(torch.float32).contiguous(), box_size, torch.from_numpy(k),
+                                  torch.from_numpy(pt))
+
+    # differentiable in the noise and in tensor-valued cosmological parameters (white noise -> IC)
+    out = autodiff.colour(field, cosmo, mesh_size, box_size, colour_fn, table)
     return DensityField(array=out, mesh_size=mesh_size, box_size=box_size, observer_position=observer_position,
                         sharding=sharding, halo_size=tuple(halo_size) if not np.isscalar(halo_size) else (halo_size,) * 2,
                         nside=nside, field_size=field_size, flatsky_npix=flatsky_npix, scale_factors=0.0,

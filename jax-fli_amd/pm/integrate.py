@@ -16,7 +16,8 @@ from typing import Any
 import numpy as np
 import torch
 
-from .solvers import AbstractNBodySolver, NBodyState, StepOp, _mode, run_op
+from .. import autodiff, engine
+from .solvers import AbstractNBodySolver, NBodyState, ReversibleDoubleKickDrift, StepOp, _mode, force_scale, run_op
 
 f32 = np.float32
 
@@ -56,12 +57,71 @@ def _stack(snaps):
     return first.replace(array=arr, scale_factors=sf, comoving_centers=rc, density_width=dw)
 
 
+def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0):
+    """Differentiable integration: the reversible solver's fused steps forward, the reverse adjoint
+    (pmx_step_reverse_vjp per step, snapshot paint VJPs) backward (integrate.py:147-193, 288-538)."""
+    if not isinstance(solver, ReversibleDoubleKickDrift):
+        raise NotImplementedError("gradients need the reversible solver (ReversibleDoubleKickDrift) with "
+                                  "adjoint='reverse'")
+    target = solver.interp_kernel.painting.target if solver.interp_kernel.painting is not None else "particles"
+    if target not in ("density", "particles"):
+        raise NotImplementedError(f"gradients through '{target}' snapshots are not implemented")
+    cosmo, _, _ = autodiff.split_cosmo(cosmo_in)
+    mesh_size = displacements.mesh_size
+    mode = _mode(displacements)
+    events, grid = [], []
+    for ev in time_grid(t0, ts, dt0, t1):
+        if ev[0] == "step":
+            op = solver.step_op(ev[1], ev[2], dt0, float(t0), cosmo)
+            events.append(("step", len(grid), op.kick, op.post))
+            grid.append((ev[1], ev[2]))
+        else:
+            events.append(("save", ev[1], ev[2]))
+    saved_fields = []
+
+    def run(x0, v0, cosmo_):
+        x = x0.detach().contiguous().clone()
+        v = v0.detach().contiguous().clone()
+        fs = force_scale(cosmo_)
+        snaps = []
+        saved_fields.clear()
+        for ev in events:
+            if ev[0] == "step":
+                engine.step(x, v, mesh_size, mode=mode, force_scale=fs, c_kick=ev[2], c_drift_post=ev[3])
+            else:
+                _, idx, tt = ev
+                st = NBodyState(interp_state=solver.interp_kernel.init().__class__(rotation_idx=-1, shell_idx=idx),
+                                t_initial=float(t0))
+                fld = solver.save_at(displacements.replace(array=x, scale_factors=tt),
+                                     velocities.replace(array=v, scale_factors=tt), tt, dt0, st, cosmo_)
+                saved_fields.append(fld)
+                snaps.append(fld.array)
+        return x, v, torch.stack(snaps), len(grid)
+
+    run.info = {"mesh": mesh_size, "mode": mode, "target": target, "events": events, "force_scale": force_scale}
+
+    def coef_fn(c):
+        return np.asarray([[(o := solver.step_op(a, b, dt0, float(t0), c, raw=True)).kick, o.post] for a, b in grid],
+                          dtype=np.float64).reshape(-1, 2)
+
+    arr = autodiff.nbody_reverse(displacements.array, velocities.array, cosmo_in, run, coef_fn)
+    out = _stack(saved_fields)
+    return out.replace(array=arr)
+
+
 def integrate(displacements, velocities, cosmo, ts, solver: AbstractNBodySolver, t0: float, t1: float,
               dt0: float, adjoint: str = "checkpointed", checkpoints: int | None = None) -> Any:
     if adjoint not in ("checkpointed", "reverse", "lax"):
         raise ValueError(f"Unknown adjoint type: {adjoint}")
     if adjoint == "reverse" and not solver.pgd_kernel.reversible:
         raise ValueError(f"Cannot use adjoint='reverse' with {type(solver.pgd_kernel).__name__}.")
+    _, _, grad_params = autodiff.split_cosmo(cosmo)
+    if displacements.array.requires_grad or velocities.array.requires_grad or grad_params:
+        if adjoint != "reverse":
+            raise NotImplementedError("gradients are implemented for adjoint='reverse' (the reversible solver's "
+                                      "recomputation); use ReversibleDoubleKickDrift with adjoint='reverse'")
+        return _integrate_grad(displacements, velocities, cosmo, ts, solver, t0, t1, dt0)
+    cosmo, _, _ = autodiff.split_cosmo(cosmo)
     mesh_size = displacements.mesh_size
     mode = _mode(displacements)
     x = displacements.array.contiguous().clone()

@@ -6,8 +6,8 @@ from __future__ import annotations
 
 import numpy as np
 
+from .. import autodiff
 from .. import cosmology as _c
-from .. import engine
 from ..fields import DensityField, FieldStatus, ParticleField, PositionUnit
 from ..painting import PaintingOptions
 from ._resolve_geometry import resolve_ts_geometry
@@ -26,6 +26,8 @@ def lpt(cosmo, initial_field: DensityField, *, ts=None, nb_shells=None, density_
         raise NotImplementedError("second-order LPT is not implemented on the pmx engine")
     if initial_particles is not None:
         raise NotImplementedError("user-defined initial particles are not supported")
+    cosmo_in = cosmo
+    cosmo, _, _ = autodiff.split_cosmo(cosmo_in)
     ts_resolved, r_centers, widths, is_lightcone = resolve_ts_geometry(
         cosmo, initial_field, painting=painting, ts=ts, nb_shells=nb_shells, density_widths=density_widths,
         equal_vol=equal_vol, min_width=min_width)
@@ -34,9 +36,8 @@ def lpt(cosmo, initial_field: DensityField, *, ts=None, nb_shells=None, density_
     if painting.target not in ("particles", "density"):
         raise ValueError(f"Painting target {painting.target} is incompatible with snapshot mode. Use 'particles' or 'density'.")
     a = float(np.asarray(ts_resolved, dtype=np.float32).reshape(-1)[0])
-    D = float(_c.growth_factor(cosmo, a))
-    vf = float(a**2 * _c.growth_rate(cosmo, a) * _c.E(cosmo, a))
-    dx, p = engine.lpt1(initial_field.array.contiguous(), D, vf)
+    # dx = D(a) F(delta), p = a^2 f E dx; differentiable in the field and in tensor-valued parameters
+    dx, p = autodiff.lpt1(initial_field.array, cosmo_in, a)
     meta = dict(status=FieldStatus.LPT1, scale_factors=a, comoving_centers=float(_c.radial_comoving_distance(cosmo, a)),
                 z_sources=1.0 / a - 1.0, unit=PositionUnit.GRID_RELATIVE)
     dx_field = ParticleField.FromDensityMetadata(array=dx, field=initial_field, **meta)

@@ -7,6 +7,7 @@ import dataclasses
 
 import numpy as np
 
+from .. import autodiff
 from .. import cosmology as _c
 from ..fields import FieldStatus, ParticleField
 from ..painting import PaintingOptions
@@ -35,7 +36,8 @@ def nbody(cosmo, dx_field: ParticleField, p_field: ParticleField, *, t0: float =
         raise ValueError(f"Starting scale factor t0={t0} does not match LPT fields' scale factor {lpt_t0}.")
     if ts is None and nb_shells is None:
         ts = np.array([t1])
-    ts_resolved, r_centers, widths, _ = resolve_ts_geometry(cosmo, dx_field, painting=solver.interp_kernel.painting,
+    cosmo_f, _, _ = autodiff.split_cosmo(cosmo)
+    ts_resolved, r_centers, widths, _ = resolve_ts_geometry(cosmo_f, dx_field, painting=solver.interp_kernel.painting,
                                                             ts=ts, nb_shells=nb_shells, density_widths=density_widths,
                                                             equal_vol=equal_vol, min_width=min_width)
     interp = solver.interp_kernel.update_geometry(ts=ts_resolved, r_centers=np.asarray(r_centers),

@@ -147,7 +147,9 @@ class ReversibleDoubleKickDrift(AbstractNBodySolver):
     def init_op(self, t0, t1, dt, cosmo):
         return None
 
-    def step_op(self, t0, t1, dt, t_initial, cosmo) -> StepOp:
+    def step_op(self, t0, t1, dt, t_initial, cosmo, raw: bool = False) -> StepOp:
+        """raw: float64 coefficients (for derivatives w.r.t. the cosmology), else rounded to float32
+        as the reference's traced float32 arithmetic produces them."""
         t0, t1 = f32(t0), f32(t1)
         t_prev = max(f32(t0 - f32(dt)), f32(t_initial))  # solvers.py:276-277
         ai_1 = _sqrt32(t_prev * t0)
@@ -157,6 +159,8 @@ class ReversibleDoubleKickDrift(AbstractNBodySolver):
         ck = (k1 + k2) / (float(t0) ** 2 * _c.E(cosmo, t0))
         drift = (_c.Gp(cosmo, t1) - _c.Gp(cosmo, t0)) / _c.gp(cosmo, af_2)
         cd = drift / (float(af_2) ** 3 * _c.E(cosmo, af_2))
+        if raw:
+            return StepOp(kick=float(ck), post=float(cd))
         return StepOp(kick=float(f32(ck)), post=float(f32(cd)))
 
     def reverse_op(self, t0, t1, dt, t_initial, cosmo) -> StepOp:
