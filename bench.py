@@ -144,12 +144,96 @@ def read_traffic(phase):
         return None
 
 
+def bench_vjp(pkg, dev):
+    """Differentiable configuration: 128^3 particles on a 128^3 mesh, box 500 Mpc/h, ReversibleDoubleKickDrift
+    (dt0 = 0.09, a = 0.1 -> 1, reverse adjoint), loss sum((delta - delta_target)^2) with the target from a
+    second seed; gradient w.r.t. the white-noise field and (Omega_c, sigma8). Returns forward and
+    forward+VJP wall times and the agreement with central finite differences (eps = 1e-3, 16 random
+    directions of the noise field, float32 forward)."""
+    import torch
+
+    n, box, a0, dt0 = 128, 500.0, 0.1, 0.09
+    P = dict(Omega_b=0.0486, h=0.6774, n_s=0.9667)
+    solver = pkg.ReversibleDoubleKickDrift(interp_kernel=pkg.NoInterp(painting=pkg.PaintingOptions(target="density")))
+
+    def loss(noise, Oc, s8, target):
+        c = pkg.Cosmology(Omega_c=Oc, sigma8=s8, **P)
+        ic = pkg.interpolate_initial_conditions(noise, (n,) * 3, (box,) * 3, cosmo=c)
+        dx, p = pkg.lpt(c, ic, ts=a0)
+        out = pkg.nbody(c, dx, p, t0=a0, t1=1.0, dt0=dt0, ts=[1.0], solver=solver, adjoint="reverse")
+        rho = out.array.double()
+        delta = rho / rho.mean() - 1.0
+        return delta if target is None else ((delta - target) ** 2).sum()
+
+    with torch.no_grad():
+        target = loss(pkg.engine.white_noise((n,) * 3, 2), 0.25, 0.8, None)
+    noise0 = pkg.engine.white_noise((n,) * 3, 1)
+
+    def timed(fn, reps=5):
+        ts_ = []
+        for _ in range(reps + 1):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts_.append(time.perf_counter() - t)
+        return float(np.median(ts_[1:])) * 1e3
+
+    def fwd():
+        with torch.no_grad():
+            return float(loss(noise0, 0.25, 0.8, target))
+
+    grads = {}
+
+    def fwd_bwd():
+        nz = noise0.clone().requires_grad_(True)
+        Oc = torch.tensor(0.25, dtype=torch.float64, device=dev, requires_grad=True)
+        s8 = torch.tensor(0.8, dtype=torch.float64, device=dev, requires_grad=True)
+        L = loss(nz, Oc, s8, target)
+        L.backward()
+        grads.update(noise=nz.grad, Oc=Oc.grad, s8=s8.grad)
+
+    def fwd_bwd_noise():
+        nz = noise0.clone().requires_grad_(True)
+        loss(nz, 0.25, 0.8, target).backward()
+
+    fwd_ms = timed(fwd)
+    fwd_bwd_noise_ms = timed(fwd_bwd_noise)
+    fwd_bwd_ms = timed(fwd_bwd)
+    g = grads["noise"].double()
+    errs = []
+    eps = 1e-3
+    for j in range(16):
+        d = torch.randn(noise0.shape, generator=torch.Generator(device=dev).manual_seed(100 + j), device=dev)
+        with torch.no_grad():
+            fd = (float(loss(noise0 + eps * d, 0.25, 0.8, target)) - float(loss(noise0 - eps * d, 0.25, 0.8, target))) / (
+                2 * eps)
+        errs.append(abs(float((g * d.double()).sum()) - fd) / abs(fd))
+    with torch.no_grad():
+        h = 1e-3
+        fd_oc = (float(loss(noise0, 0.25 + h, 0.8, target)) - float(loss(noise0, 0.25 - h, 0.8, target))) / (2 * h)
+        fd_s8 = (float(loss(noise0, 0.25, 0.8 + h, target)) - float(loss(noise0, 0.25, 0.8 - h, target))) / (2 * h)
+    return {"workload": "128^3 particles, 128^3 mesh, box 500 Mpc/h, ReversibleDoubleKickDrift dt0=0.09 a=0.1->1 "
+                        "(10 steps), reverse adjoint, L2 loss on the final density vs a second seed",
+            "forward_ms": round(fwd_ms, 2), "forward_plus_vjp_ms": round(fwd_bwd_ms, 2),
+            "vjp_over_forward": round(fwd_bwd_ms / fwd_ms, 2),
+            "noise_only": {"forward_plus_vjp_ms": round(fwd_bwd_noise_ms, 2),
+                           "vjp_over_forward": round(fwd_bwd_noise_ms / fwd_ms, 2)},
+            "note": "wall clock; the (Omega_c, sigma8) gradient adds float64 finite differences of the host "
+                    "growth/coefficient/P(k) functions (see vjp_over_forward vs noise_only)",
+            "fd_check": {"eps": eps, "directions": 16, "median_rel_err": float(np.median(errs)),
+                         "max_rel_err": float(np.max(errs)),
+                         "Omega_c": {"adjoint": float(grads["Oc"]), "fd": fd_oc},
+                         "sigma8": {"adjoint": float(grads["s8"]), "fd": fd_s8}}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed PM steps (0: one full schedule)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=1, help="CPU-baseline steps at full size (0: skip)")
+    ap.add_argument("--no-vjp", action="store_true", help="skip the forward+VJP measurement")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the per-phase HBM traffic from two rocprofv3 --pmc runs of this script and exit")
     args = ap.parse_args()
@@ -311,6 +395,8 @@ def main():
     fft_roc = t_gpu(lambda: torch.fft.irfftn(torch.fft.rfftn(rho), s=mesh))
     del spec
 
+    vjp = None if args.no_vjp else bench_vjp(pkg, dev)
+
     # CPU baseline: the NumPy/SciPy restatement of the reference step on the host cores
     cpu = None
     if args.cpu_steps > 0:
@@ -359,6 +445,7 @@ def main():
                                 "1-component readout) + separate kick and drift (torch ops); both forms timed over "
                                 "the schedule's PM steps without the shell paints"},
         "forward_total_ms": round(forward_ms, 1),
+        "vjp": vjp,
         "fft_roundtrip_ms": {"pmx": round(fft_ours, 4), "rocfft": round(fft_roc, 4)},
         "final_pk": {"k": [round(float(a), 5) for a in k_fin[:8]], "pk": [round(float(a), 3) for a in pk_fin[:8]]},
     }

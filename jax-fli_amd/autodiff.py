@@ -56,7 +56,11 @@ def fd(fn, cosmo, name, rel=1e-4):
     return (hi - lo) / (2.0 * h)
 
 
-def _param_grad(t: torch.Tensor, value: float):
+def _param_grad(t: torch.Tensor, value):
+    """Gradient for parameter tensor t from a float or a device scalar (kept asynchronous when t
+    lives on the same device)."""
+    if isinstance(value, torch.Tensor):
+        return value.to(dtype=t.dtype, device=t.device).reshape(t.shape)
     return torch.as_tensor(value, dtype=t.dtype, device=t.device).reshape(t.shape)
 
 
@@ -70,11 +74,18 @@ def _kgrid(mesh_size, box_size, device):
     return torch.sqrt(kx[:, None, None] ** 2 + ky[None, :, None] ** 2 + kz[None, None, :] ** 2)
 
 
-def _interp_lin(x, xp, fp):
-    """Linear interpolation with linear extrapolation at both ends (the colouring's table lookup)."""
+def _interp_setup(x, xp):
+    """Bracketing indices and fractions of a linear interpolation with linear extrapolation at both
+    ends (the colouring's table lookup); apply to a table fp with _interp_apply."""
     ind = torch.clamp(torch.searchsorted(xp, x, right=False) - 1, 0, xp.numel() - 2)
-    x0, x1, f0, f1 = xp[ind], xp[ind + 1], fp[ind], fp[ind + 1]
-    return f0 + (x - x0) * (f1 - f0) / (x1 - x0)
+    x0, x1 = xp[ind], xp[ind + 1]
+    return ind, (x - x0) / (x1 - x0)
+
+
+def _interp_apply(setup, fp):
+    ind, t = setup
+    f0, f1 = fp[ind], fp[ind + 1]
+    return f0 + t * (f1 - f0)
 
 
 class _Colour(torch.autograd.Function):
@@ -83,7 +94,10 @@ class _Colour(torch.autograd.Function):
         colour_fn, table_fn, cosmo, names, mesh_size, box_size = spec
         ctx.save_for_backward(noise)
         ctx.spec, ctx.params = spec, params
-        return colour_fn(noise, cosmo)
+        out = colour_fn(noise, cosmo)
+        # host derivatives of the P(k) table, computed while the device colours the field
+        ctx.dpts = [fd(lambda c: table_fn(c)[1], cosmo, name) for name in names]
+        return out
 
     @staticmethod
     def backward(ctx, g):
@@ -93,17 +107,24 @@ class _Colour(torch.autograd.Function):
         gn = colour_fn(g, cosmo) if ctx.needs_input_grad[0] else None
         grads = []
         if names:
-            # d ic / d theta = irfft(rfft(noise) dA/dtheta), A = sqrt(interp(P N / V)) per mode
-            kk = _kgrid(mesh_size, box_size, noise.device)
+            # <g, irfft(rfft(noise) dA/dtheta)> = (1/N) sum_k w_k Re(conj(G_k) E_k) dA/dtheta(k) over the
+            # half spectrum (w_k = 1 on the kz = 0 and Nyquist planes, 2 elsewhere), A = sqrt(interp(P N/V))
+            dev = noise.device
+            cross = (torch.conj(torch.fft.rfftn(g)) * torch.fft.rfftn(noise.to(torch.float32))).real.double()
             k, pt = table_fn(cosmo)
-            kt = torch.as_tensor(k, dtype=torch.float64, device=noise.device)
-            amp = torch.sqrt(torch.clamp(_interp_lin(kk, kt, torch.as_tensor(pt, device=noise.device)), min=0.0))
-            nf = torch.fft.rfftn(noise.double())
-            for name, t in zip(names, ctx.params):
-                dpt = torch.as_tensor(fd(lambda c: table_fn(c)[1], cosmo, name), device=noise.device)
-                damp = torch.where(amp > 0, _interp_lin(kk, kt, dpt) / (2.0 * amp), torch.zeros_like(amp))
-                dic = torch.fft.irfftn(nf * damp, s=mesh_size)
-                grads.append(_param_grad(t, float((g.double() * dic).sum())))
+            dpts = ctx.dpts
+            kt = torch.as_tensor(k, dtype=torch.float64, device=dev)
+            setup = _interp_setup(_kgrid(mesh_size, box_size, dev), kt)
+            amp = torch.sqrt(torch.clamp(_interp_apply(setup, torch.as_tensor(pt, device=dev)), min=0.0))
+            nz = mesh_size[2]
+            wz = torch.full((nz // 2 + 1,), 2.0, dtype=torch.float64, device=dev)
+            wz[0] = 1.0
+            if nz % 2 == 0:
+                wz[-1] = 1.0
+            cross = cross * wz / float(np.prod(mesh_size))
+            inv2a = torch.where(amp > 0, 0.5 / torch.where(amp > 0, amp, torch.ones_like(amp)), torch.zeros_like(amp))
+            sums = [(cross * _interp_apply(setup, torch.as_tensor(d, device=dev)) * inv2a).sum() for d in dpts]
+            grads = [_param_grad(t, v) for t, v in zip(ctx.params, sums)]
         return (gn, None, *grads)
 
 
@@ -130,6 +151,7 @@ class _LPT(torch.autograd.Function):
         dx, p = engine.lpt1(ic.contiguous(), D, vf)
         ctx.save_for_backward(dx)
         ctx.spec, ctx.params, ctx.DV = spec, params, (D, vf)
+        ctx.ders = [fd(lambda c: _lpt_factors(c, a), cosmo, name) for name in names]  # overlaps the device
         return dx, p
 
     @staticmethod
@@ -144,10 +166,9 @@ class _LPT(torch.autograd.Function):
         grads = []
         if names:
             # dx = D F, p = vf D F:  dL/dD = <dx_bar + vf p_bar, dx> / D,  dL/dvf = <p_bar, dx>
-            gD = float(((dx_bar.double() + vf * p_bar.double()) * dx.double()).sum()) / D
-            gv = float((p_bar.double() * dx.double()).sum())
-            for name, t in zip(names, ctx.params):
-                dD, dv = fd(lambda c: _lpt_factors(c, a), cosmo, name)
+            gD = ((dx_bar + vf * p_bar) * dx).sum(dtype=torch.float64) / D
+            gv = (p_bar * dx).sum(dtype=torch.float64)
+            for (dD, dv), t in zip(ctx.ders, ctx.params):
                 grads.append(_param_grad(t, gD * dD + gv * dv))
         return (ic_bar, None, *grads)
 
@@ -171,6 +192,9 @@ class _NBodyReverse(torch.autograd.Function):
         x, v, snaps, steps = run(x0, v0, cosmo)
         ctx.save_for_backward(x.clone(), v.clone())
         ctx.spec, ctx.params, ctx.steps = spec, params, steps
+        # host derivatives of the step coefficients, computed while the device runs the steps
+        ctx.co = np.asarray(coef_fn(cosmo)) if names else None
+        ctx.ders = [(fd(coef_fn, cosmo, name), float(fd(run.info["force_scale"], cosmo, name))) for name in names]
         return snaps
 
     @staticmethod
@@ -194,20 +218,17 @@ class _NBodyReverse(torch.autograd.Function):
             else:
                 _, i, ck, cd = ev
                 cb = engine.step_reverse_vjp(x, v, xb, vb, mesh, mode=mode, force_scale=fs, c_kick=ck, c_drift=cd)
-                coef_bars.append((i, cb.clone()))
+                coef_bars.append((i, cb))
         grads = []
         if names:
-            cbs = {i: cb.cpu().numpy() for i, cb in coef_bars}
-            for name, t in zip(names, ctx.params):
-                dco = fd(coef_fn, cosmo, name)            # (nsteps, 2): d(ck, cd)/d name
-                dfs = float(fd(info["force_scale"], cosmo, name))
-                co = np.asarray(coef_fn(cosmo))
-                tot = 0.0
-                for i, cb in cbs.items():
-                    tot += cb[1] * dco[i, 0] + cb[0] * dco[i, 1]
-                    # F = fs F1 enters every kick: dL/dfs = sum_i ck_i <v_bar_i, F_i> / fs
-                    tot += co[i, 0] * cb[1] / fs * dfs
-                grads.append(_param_grad(t, tot))
+            co, ders = ctx.co, ctx.ders
+            idx = [i for i, _ in coef_bars]
+            allcb = torch.stack([cb for _, cb in coef_bars])  # (nsteps, 2) = (dL/dcd, dL/dck), on device
+            for (dco, dfs), t in zip(ders, ctx.params):
+                # v' = v + F ck, x' = x + v' cd; F = fs F1 enters every kick
+                wck = torch.as_tensor(dco[idx, 0] + co[idx, 0] / fs * dfs, device=allcb.device)
+                wcd = torch.as_tensor(dco[idx, 1], device=allcb.device)
+                grads.append(_param_grad(t, (allcb[:, 1] * wck + allcb[:, 0] * wcd).sum()))
         return (xb, vb, None, *grads)
 
 

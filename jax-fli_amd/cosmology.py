@@ -77,7 +77,22 @@ _A_MAX = 2.0
 _NSTEP = 2048  # RK4 steps from _A_INIT to a = 1
 
 
-def _growth_rhs(cosmo: Cosmology, x, D, G):
+@dataclasses.dataclass(frozen=True)
+class _Background:
+    """The parameters the expansion history and the growth ODE depend on (cache key)."""
+
+    Omega_m: float
+    Omega_k: float
+    Omega_de: float
+    w0: float
+    wa: float
+
+
+def _background(cosmo: Cosmology) -> _Background:
+    return _Background(cosmo.Omega_m, cosmo.Omega_k, cosmo.Omega_de, cosmo.w0, cosmo.wa)
+
+
+def _growth_rhs(cosmo, x, D, G):
     # y = (D, G = dD/dln a);  dG/dln a = -(2 + dlnE/dlna) G + 1.5 Om D / (a^3 E^2)
     a = math.exp(x)
     p = -3.0 * (1.0 + cosmo.w0 + cosmo.wa)
@@ -89,8 +104,12 @@ def _growth_rhs(cosmo: Cosmology, x, D, G):
     return G, -(2.0 + dlne) * G + 1.5 * om * D / e2
 
 
-@functools.lru_cache(maxsize=1024)
 def _growth_table(cosmo: Cosmology):
+    return _growth_table_bg(_background(cosmo))
+
+
+@functools.lru_cache(maxsize=1024)
+def _growth_table_bg(cosmo: _Background):
     """RK4 in ln a from the matter-dominated growing mode D = a, normalised to D(1) = 1. The grid
     continues past a = 1 to a = 2: the KDK double kick evaluates Gf at the nominal next half step
     sqrt(t1 (t1 + dt)), which exceeds 1 on the last step."""
@@ -270,7 +289,7 @@ def eisenstein_hu(cosmo: Cosmology, k):
 @functools.lru_cache(maxsize=1024)
 def _sigma8_norm(cosmo: Cosmology) -> float:
     # sigma^2(R=8) = 1/(2 pi^2) int dlnk k^3 T^2 k^ns W(kR)^2, Simpson on a fine ln k grid
-    lnk = np.linspace(math.log(1e-5), math.log(1e3), 2**15 + 1)
+    lnk = np.linspace(math.log(1e-5), math.log(1e3), 2**12 + 1)
     k = np.exp(lnk)
     x = k * 8.0
     w = 3.0 * (np.sin(x) - x * np.cos(x)) / x**3

@@ -183,16 +183,18 @@ int pot_only(pmx_plan_t* p, const float* src, float force_scale, unsigned flags,
 // Force adjoint at the last sorted particle set `a` (whose force mesh is in p->fmesh):
 //   xbar (+)= J^T (umul * u),  J = d forces / d x,  forces = R(x) S G P(x)
 // = sum_c grad w_c(x) (umul u . M[c] + rhobar[c]),  rhobar = G S^T R(x)^T (umul u).
-// R^T is three sorted deposits (scale from max |u| in p->ps_misc[3]); S^T = -sum_d S_d; dot (+)= u . F.
+// R^T is the sorted int64 deposit of the three components (scale from max |u| in p->ps_misc[3]);
+// S^T = -sum_d S_d; dot (+)= u . F. With vkick the same gather also applies v += F ckick (the
+// reverse step's un-kick).
 int force_adjoint(pmx_plan_t* p, const PartArgs& a, const float* u, float umul, float force_scale, unsigned flags,
-                  float* xbar, int accumulate, double* dot, hipStream_t s) {
+                  float* xbar, int accumulate, double* dot, float* vkick, float ckick, hipStream_t s) {
   float* U = reinterpret_cast<float*>(p->specC);  // 3 scalar meshes; consumed by div4 before the FFT reuses it
   PMX_TRY(pmx_launch_set_scale(p->ps_misc + 3, umul, p->ps_scale, s));
-  for (int d = 0; d < 3; ++d)
-    PMX_TRY(pmx_paint_deposit(p, a, u + d, 3, umul, 0.f, 1, U + d * p->ncell, 1, 0, s));
+  PMX_TRY(pmx_paint_deposit(p, a, u, 3, umul, 0.f, 1, U, 1, 0, s, 3, p->ncell));
   PMX_TRY(pmx_launch_div4(p, U, U + p->ncell, U + 2 * p->ncell, 1, p->rho, s));
   PMX_TRY(pot_only(p, p->rho, force_scale, flags, s));
-  return pmx_launch_vjp_gather(p, a, 3, u, umul, p->fmesh, p->rho, nullptr, 1.f, xbar, accumulate, nullptr, dot, s);
+  return pmx_launch_vjp_gather(p, a, 3, u, umul, p->fmesh, p->rho, nullptr, 1.f, xbar, accumulate, nullptr, dot, vkick,
+                               ckick, s);
 }
 
 StepMarks* new_marks(pmx_plan_t* p) {
@@ -273,7 +275,8 @@ int pmx_plan_create(pmx_plan* out, int nx, int ny, int nz, unsigned flags) {
       (rc = dev_alloc(&p->ps_offset, ntiles, tot)) ||
       (rc = dev_alloc(&p->ps_cursor, ntiles, tot)) || (rc = dev_alloc(&p->ps_misc, 4, tot)) ||
       (rc = dev_alloc(&p->ps_scale, 2, tot)) ||
-      (rc = dev_alloc(&p->ps_faces, (size_t)ntiles * pmx_paint_face_cells(p), tot)))
+      (rc = dev_alloc(&p->ps_faces, (size_t)ntiles * pmx_paint_face_cells(p) * 3, tot)) ||
+      (rc = dev_alloc(&p->red, (size_t)kRedMax, tot)))
     return fail(rc);
   // the paint's count pass accumulates into zeroed counters; its deposit pass re-zeroes them
   if (hipMemset(p->ps_count, 0, ntiles * sizeof(unsigned)) != hipSuccess ||
@@ -299,7 +302,7 @@ int pmx_plan_destroy(pmx_plan p) {
   void* bufs[] = {p->tw_x, p->tw_y, p->tw_z, p->tw_zh, p->grad_x, p->grad_y, p->grad_z, p->w2_x, p->w2_y, p->w2_z,
                   p->dec_x, p->dec_y, p->dec_z, p->rho, p->specA, p->specB, p->specC, p->specD, p->fmesh, p->acc,
                   p->kphys_x, p->kphys_y, p->kphys_z, p->ps_count, p->ps_tcount, p->ps_offset, p->ps_cursor, p->ps_wgcnt,
-                  p->ps_seg, p->ps_misc, p->ps_scale, p->ps_faces};
+                  p->ps_seg, p->ps_misc, p->ps_scale, p->ps_faces, p->red};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   clear_marks(p);
@@ -430,7 +433,7 @@ int pmx_forces_vjp(pmx_plan p, const float* pos, int npx, int npy, int npz, int 
   PMX_TRY(pmx_paint_deposit(p, a, nullptr, 1, 1.f, 1.f, 0, p->rho, 1, 0, s));
   PMX_TRY(pot_force_mesh_from(p, p->rho, scale, flags, p->fmesh, s));
   PMX_TRY(pmx_launch_absmax(forces_bar, 3 * np, p->ps_misc + 3, s));
-  return force_adjoint(p, a, forces_bar, 1.f, scale, flags, pos_bar, accumulate, dot, s);
+  return force_adjoint(p, a, forces_bar, 1.f, scale, flags, pos_bar, accumulate, dot, nullptr, 0.f, s);
 }
 
 int pmx_force_mesh_vjp(pmx_plan p, const float* fmesh_bar, float scale, unsigned flags, float* density_bar,
@@ -452,7 +455,7 @@ int pmx_paint_cic_vjp(pmx_plan p, const float* pos, int npx, int npy, int npz, i
   if (!mesh_bar || !pos_bar) return pmx_set_error(1, "null cotangent");
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
   return pmx_launch_vjp_gather(p, a, 1, nullptr, 0.f, nullptr, mesh_bar, weights, weight, pos_bar, accumulate,
-                               weights_bar, nullptr, (hipStream_t)stream);
+                               weights_bar, nullptr, nullptr, 0.f, (hipStream_t)stream);
 }
 
 int pmx_readout_cic_vjp(pmx_plan p, const float* mesh, int ncomp, const float* pos, int npx, int npy, int npz, int mode,
@@ -465,11 +468,10 @@ int pmx_readout_cic_vjp(pmx_plan p, const float* mesh, int ncomp, const float* p
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
   if (pos_bar)
     PMX_TRY(pmx_launch_vjp_gather(p, a, ncomp, out_bar, 1.f, mesh, nullptr, nullptr, 0.f, pos_bar, accumulate,
-                                  nullptr, nullptr, s));
+                                  nullptr, nullptr, nullptr, 0.f, s));
   if (mesh_bar) {
     PMX_TRY(pmx_paint_sort(p, a, 0.f, out_bar, ncomp, 0.f, s));
-    for (int d = 0; d < ncomp; ++d)
-      PMX_TRY(pmx_paint_deposit(p, a, out_bar + d, ncomp, 1.f, 0.f, 1, mesh_bar + d, ncomp, accumulate, s));
+    PMX_TRY(pmx_paint_deposit(p, a, out_bar, ncomp, 1.f, 0.f, 1, mesh_bar, ncomp, accumulate, s, ncomp, 1));
   }
   return 0;
 }
@@ -484,18 +486,15 @@ int pmx_step_reverse_vjp(pmx_plan p, float* x, float* v, int npx, int npy, int n
   const size_t n = (size_t)npx * npy * npz * 3;
   PMX_TRY(pmx_launch_fill_d(coef_bar, 2, s));
   // cotangents through the drift x' = x + v' cd (v' the kicked velocity, still in v)
-  PMX_TRY(pmx_launch_adj_prep(x_bar, v_bar, v, n, c_drift, coef_bar, p->ps_misc + 3, s));
-  // reverse the state: x -= v cd, then v -= F(x) ck (sorted paint, force mesh kept in p->fmesh)
+  PMX_TRY(pmx_launch_adj_prep(p, x_bar, v_bar, v, n, c_drift, coef_bar, p->ps_misc + 3, s));
+  // reverse the state: x -= v cd, then (in the adjoint gather) v -= F(x) ck
   PartArgs a = part_args(x, v, npx, npy, npz, mode, p);
   PMX_TRY(pmx_paint_sort(p, a, -c_drift, nullptr, 1, 1.f, s));
   a.v = nullptr;
   PMX_TRY(pmx_paint_deposit(p, a, nullptr, 1, 1.f, 1.f, 0, p->rho, 1, 0, s));
   PMX_TRY(pot_force_mesh_from(p, p->rho, force_scale, flags, p->fmesh, s));
-  a.v = v;
-  PMX_TRY(pmx_launch_readout(p, a, p->fmesh, 3, nullptr, -c_kick, 0.f, 2, s));
-  a.v = nullptr;
-  // x_bar += J(x)^T (ck v_bar); coef_bar[1] = v_bar . F(x)
-  return force_adjoint(p, a, v_bar, c_kick, force_scale, flags, x_bar, 1, coef_bar + 1, s);
+  // x_bar += J(x)^T (ck v_bar); coef_bar[1] = v_bar . F(x); the same gather un-kicks v -= F(x) ck
+  return force_adjoint(p, a, v_bar, c_kick, force_scale, flags, x_bar, 1, coef_bar + 1, v, -c_kick, s);
 }
 
 int pmx_plan_timing(pmx_plan p, int enable) {

@@ -46,7 +46,8 @@ struct pmx_plan_t {
   // scale {2^k, 2^-k}
   unsigned *ps_count, *ps_tcount, *ps_offset, *ps_cursor, *ps_wgcnt, *ps_seg, *ps_misc;
   float* ps_scale;
-  float* ps_faces;  // per tile: its upper face cells (pmx_paint_face_cells floats)
+  float* ps_faces;  // per tile: its upper face cells (pmx_paint_face_cells floats), 3 blocks for vectors
+  double* red;      // per-workgroup partial sums of the adjoint dot products (kRedMax doubles)
   size_t ps_wg_cap, ps_seg_cap;
   double* acc;      // double accumulators (healpix maps, flat maps); grown on demand
   size_t acc_bytes;
@@ -54,6 +55,8 @@ struct pmx_plan_t {
   int timing;
   std::vector<StepMarks>* marks;
 };
+
+constexpr int kRedMax = 65536;  // partial-sum slots: the largest particle-kernel grid
 
 // error reporting (pmx_api.hip)
 int pmx_set_error(int code, const std::string& msg);
@@ -114,8 +117,10 @@ int pmx_launch_fill(float* a, size_t n, float v, hipStream_t s);
 // the sorted paint in two halves: sort (count, scan, scatter) then deposit(s) of the sorted particles
 int pmx_paint_sort(pmx_plan_t* p, const PartArgs& a, float cd_pre, const float* w, int wstride, float w0,
                    hipStream_t s);
+// ncomp > 1: component c carries w[p * wstride + c] and goes to mesh + c * mcomp
 int pmx_paint_deposit(pmx_plan_t* p, const PartArgs& a, const float* w, int wstride, float wmul, float w0,
-                      int is_signed, float* mesh, int mstride, int accumulate, hipStream_t s);
+                      int is_signed, float* mesh, int mstride, int accumulate, hipStream_t s, int ncomp = 1,
+                      size_t mcomp = 0);
 // adjoint building blocks (pmx_particles.hip)
 int pmx_launch_absmax(const float* a, size_t n, unsigned* bits, hipStream_t s);
 int pmx_launch_set_scale(unsigned* bits, float mul, float* scale, hipStream_t s);
@@ -123,9 +128,10 @@ int pmx_launch_div4(pmx_plan_t* p, const float* U0, const float* U1, const float
                     hipStream_t s);
 int pmx_launch_vjp_gather(pmx_plan_t* p, const PartArgs& a, int ncomp, const float* u, float umul, const float* M,
                           const float* rhob, const float* w, float w0, float* xbar, int accumulate, float* wbar,
-                          double* dot, hipStream_t s);
-int pmx_launch_adj_prep(const float* xbar, float* vbar, const float* v, size_t n, float cd, double* dot,
-                        unsigned* bits, hipStream_t s);
+                          double* dot, float* vkick, float ckick, hipStream_t s);
+
+int pmx_launch_adj_prep(pmx_plan_t* p, const float* xbar, float* vbar, const float* v, size_t n, float cd,
+                        double* dot, unsigned* bits, hipStream_t s);
 int pmx_launch_fill_d(double* a, size_t n, hipStream_t s);
 int pmx_launch_readout(pmx_plan_t* p, const PartArgs& a, const float* mesh, int ncomp, float* out,
                        float ck, float cd_post, int what, hipStream_t s);

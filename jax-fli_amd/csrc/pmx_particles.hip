@@ -133,6 +133,20 @@ __device__ __forceinline__ Chunk my_chunk(unsigned n) {
   return Chunk{beg, min(n, beg + per)};
 }
 
+// One atomicMax per workgroup of the workgroup's largest m (non-negative floats order like their
+// bits): same-address atomics from every wave of a large grid serialise.
+__device__ __forceinline__ void block_max_to(float m, unsigned* bits) {
+  __shared__ float wm[8];
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if (__lane_id() == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0.f;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t = fmaxf(t, wm[i]);
+    if (t > 0.f) atomicMax(bits, __float_as_uint(t));
+  }
+}
+
 __global__ void k_fill(float* __restrict__ a, size_t n, float v) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
 }
@@ -392,8 +406,7 @@ __global__ void __launch_bounds__(256) k_sort_count(PartArgs a, SortGeom g, floa
       const size_t q = (size_t)(((unsigned)i * a.npy + j) * a.npz + k) * wstride;
       for (int c = 0; c < wstride; ++c) m = fmaxf(m, fabsf(w[q + c]));
     }
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    if (__lane_id() == 0) atomicMax(wmax_bits, __float_as_uint(m));
+    block_max_to(m, wmax_bits);
   }
   if (nfar) atomicAdd(far, nfar);
   __syncthreads();
@@ -487,13 +500,13 @@ __device__ __forceinline__ int face_index(int lx, int ly, int lz, int BX, int BY
 // deposits (e.g. the components of a vector) can follow one sort.
 template <bool SIGNED>
 __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, const float* __restrict__ w,
-                                                       int wstride, float wmul, float w0,
+                                                       int wstride, float wmul, float w0, int ncomp,
                                                        const unsigned* __restrict__ tcount,
                                                        const unsigned* __restrict__ offset,
                                                        const unsigned* __restrict__ seg,
                                                        const float* __restrict__ scale, float* __restrict__ mesh,
-                                                       int mstride, float* __restrict__ faces, int face_stride,
-                                                       int accumulate) {
+                                                       int mstride, size_t mcomp, float* __restrict__ faces,
+                                                       int face_stride, size_t fcomp, int accumulate) {
   __shared__ unsigned long long tile[TILE_PAD_MAX];
   const int BX = 1 << g.sx, BY = 1 << g.sy, BZ = 1 << g.sz;
   const int PY = BY + 1, PZ = BZ + 1;
@@ -502,84 +515,90 @@ __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, c
   const int tz = (int)(L % (unsigned)g.ntz), ty = (int)((L / (unsigned)g.ntz) % (unsigned)g.nty),
             tx = (int)(L / ((unsigned)g.ntz * g.nty));
   const int T0x = tx << g.sx, T0y = ty << g.sy, T0z = tz << g.sz;
-  for (int i = threadIdx.x; i < npad; i += blockDim.x) tile[i] = 0ull;
   const unsigned n = tcount[L], beg = offset[L];
   const float S = scale[0];
   const float wS = w0 * S, mS = wmul * S;
+  const double inv = (double)scale[1];
   // byte offsets of the +1 neighbours in the padded tile
   const int oz = 8, oy = PZ * 8, ox = PY * PZ * 8;
   char* lds = reinterpret_cast<char*>(tile);
-  __syncthreads();
-  constexpr int U = 4;
-  for (unsigned e0 = threadIdx.x; e0 < n; e0 += U * blockDim.x) {
-    unsigned ps[U];
-    float3 xs[U];
+  const int ncell = BX * BY * BZ;
+  const int nface = PY * PZ + BX * PZ + BX * BY;
+  // one component after the other through the same LDS tile (the entries stay in L1/L2)
+  for (int comp = 0; comp < ncomp; ++comp) {
+    for (int i = threadIdx.x; i < npad; i += blockDim.x) tile[i] = 0ull;
+    __syncthreads();
+    constexpr int U = 4;
+    for (unsigned e0 = threadIdx.x; e0 < n; e0 += U * blockDim.x) {
+      unsigned ps[U];
+      float3 xs[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const unsigned e = e0 + u * blockDim.x;
-      ps[u] = e < n ? seg[beg + e] : 0u;
-    }
+      for (int u = 0; u < U; ++u) {
+        const unsigned e = e0 + u * blockDim.x;
+        ps[u] = e < n ? seg[beg + e] : 0u;
+      }
 #pragma unroll
-    for (int u = 0; u < U; ++u) xs[u] = (e0 + u * blockDim.x < n) ? ld3(a.x, ps[u]) : make_float3(0.f, 0.f, 0.f);
+      for (int u = 0; u < U; ++u) xs[u] = (e0 + u * blockDim.x < n) ? ld3(a.x, ps[u]) : make_float3(0.f, 0.f, 0.f);
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (e0 + u * blockDim.x >= n) continue;
-      const unsigned p = ps[u];
-      int i = 0, j = 0, k = 0;
-      if (a.mode == 0) lattice_of(a, g, p, i, j, k);
-      const Base b = base_of(a, g, i, j, k, xs[u]);
-      const int base = (((b.cx - T0x) * PY + (b.cy - T0y)) * PZ + (b.cz - T0z)) * 8;
-      const float c = w ? w[(size_t)p * wstride] * mS : wS;
-      const float wx[2] = {1.0f - fabsf(b.fx), 1.0f - fabsf(b.fx - 1.0f)};
-      const float wy[2] = {1.0f - fabsf(b.fy), 1.0f - fabsf(b.fy - 1.0f)};
-      // c is folded into the z weights: exact when c is a power of two (scalar weight 1)
-      const float wz[2] = {(1.0f - fabsf(b.fz)) * c, (1.0f - fabsf(b.fz - 1.0f)) * c};
+      for (int u = 0; u < U; ++u) {
+        if (e0 + u * blockDim.x >= n) continue;
+        const unsigned p = ps[u];
+        int i = 0, j = 0, k = 0;
+        if (a.mode == 0) lattice_of(a, g, p, i, j, k);
+        const Base b = base_of(a, g, i, j, k, xs[u]);
+        const int base = (((b.cx - T0x) * PY + (b.cy - T0y)) * PZ + (b.cz - T0z)) * 8;
+        const float c = w ? w[(size_t)p * wstride + comp] * mS : wS;
+        const float wx[2] = {1.0f - fabsf(b.fx), 1.0f - fabsf(b.fx - 1.0f)};
+        const float wy[2] = {1.0f - fabsf(b.fy), 1.0f - fabsf(b.fy - 1.0f)};
+        // c is folded into the z weights: exact when c is a power of two (scalar weight 1)
+        const float wz[2] = {(1.0f - fabsf(b.fz)) * c, (1.0f - fabsf(b.fz - 1.0f)) * c};
 #pragma unroll
-      for (int o = 0; o < 8; ++o) {
-        const int cx = o & 1, cy = (o >> 1) & 1, cz = o >> 2;
-        const float v = (wx[cx] * wy[cy]) * wz[cz];
-        const unsigned long long d =
-            SIGNED ? (unsigned long long)(long long)__float2int_rz(v) : (unsigned long long)__float2uint_rz(v);
-        atomicAdd(reinterpret_cast<unsigned long long*>(lds + base + cx * ox + cy * oy + cz * oz), d);
+        for (int o = 0; o < 8; ++o) {
+          const int cx = o & 1, cy = (o >> 1) & 1, cz = o >> 2;
+          const float v = (wx[cx] * wy[cy]) * wz[cz];
+          const unsigned long long d =
+              SIGNED ? (unsigned long long)(long long)__float2int_rz(v) : (unsigned long long)__float2uint_rz(v);
+          atomicAdd(reinterpret_cast<unsigned long long*>(lds + base + cx * ox + cy * oy + cz * oz), d);
+        }
       }
     }
-  }
-  __syncthreads();
-  const double inv = (double)scale[1];
-  const int ncell = BX * BY * BZ;
-  for (int q = threadIdx.x; q < ncell; q += blockDim.x) {
-    const int iz = q & (BZ - 1), iy = (q >> g.sz) & (BY - 1), ix = q >> (g.sz + g.sy);
-    float* dst = mesh + (((size_t)(T0x + ix) * g.NY + (T0y + iy)) * g.NZ + (T0z + iz)) * mstride;
-    const float v = (float)((double)(long long)tile[(ix * PY + iy) * PZ + iz] * inv);
-    *dst = accumulate ? *dst + v : v;
-  }
-  float* f = faces + (size_t)L * face_stride;
-  const int nface = PY * PZ + BX * PZ + BX * BY;
-  for (int q = threadIdx.x; q < nface; q += blockDim.x) {
-    int lx, ly, lz;
-    if (q < PY * PZ) {
-      lx = BX;
-      ly = q / PZ;
-      lz = q % PZ;
-    } else if (q < PY * PZ + BX * PZ) {
-      const int r = q - PY * PZ;
-      lx = r / PZ;
-      ly = BY;
-      lz = r % PZ;
-    } else {
-      const int r = q - PY * PZ - BX * PZ;
-      lx = r / BY;
-      ly = r % BY;
-      lz = BZ;
+    __syncthreads();
+    float* mc = mesh + comp * mcomp;
+    for (int q = threadIdx.x; q < ncell; q += blockDim.x) {
+      const int iz = q & (BZ - 1), iy = (q >> g.sz) & (BY - 1), ix = q >> (g.sz + g.sy);
+      float* dst = mc + (((size_t)(T0x + ix) * g.NY + (T0y + iy)) * g.NZ + (T0z + iz)) * mstride;
+      const float v = (float)((double)(long long)tile[(ix * PY + iy) * PZ + iz] * inv);
+      *dst = accumulate ? *dst + v : v;
     }
-    f[q] = (float)((double)(long long)tile[(lx * PY + ly) * PZ + lz] * inv);
+    float* f = faces + comp * fcomp + (size_t)L * face_stride;
+    for (int q = threadIdx.x; q < nface; q += blockDim.x) {
+      int lx, ly, lz;
+      if (q < PY * PZ) {
+        lx = BX;
+        ly = q / PZ;
+        lz = q % PZ;
+      } else if (q < PY * PZ + BX * PZ) {
+        const int r = q - PY * PZ;
+        lx = r / PZ;
+        ly = BY;
+        lz = r % PZ;
+      } else {
+        const int r = q - PY * PZ - BX * PZ;
+        lx = r / BY;
+        ly = r % BY;
+        lz = BZ;
+      }
+      f[q] = (float)((double)(long long)tile[(lx * PY + ly) * PZ + lz] * inv);
+    }
+    __syncthreads();
   }
 }
 
 // Pass 5: cells on a tile's lower faces receive the face cells of its lower neighbours (x, y, z,
 // xy, xz, yz, xyz, in that order). One thread per (tile, lower-shell cell).
 __global__ void __launch_bounds__(256) k_paint_faces(SortGeom g, const float* __restrict__ faces, int face_stride,
-                                                     float* __restrict__ mesh, int mstride, int shell) {
+                                                     size_t fcomp, float* __restrict__ mesh, int mstride, size_t mcomp,
+                                                     int ncomp, int shell) {
   const int BX = 1 << g.sx, BY = 1 << g.sy, BZ = 1 << g.sz;
   const size_t gid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   const size_t ntiles = (size_t)g.ntx * g.nty * g.ntz;
@@ -605,7 +624,8 @@ __global__ void __launch_bounds__(256) k_paint_faces(SortGeom g, const float* __
   }
   const int tz = (int)(L % (unsigned)g.ntz), ty = (int)((L / (unsigned)g.ntz) % (unsigned)g.nty),
             tx = (int)(L / ((unsigned)g.ntz * g.nty));
-  float sum = 0.f;
+  size_t fi[7];
+  int nsrc = 0;
 #pragma unroll
   for (int m = 1; m < 8; ++m) {
     const int ex = m & 1, ey = (m >> 1) & 1, ez = m >> 2;
@@ -614,11 +634,15 @@ __global__ void __launch_bounds__(256) k_paint_faces(SortGeom g, const float* __
     const int ny = ey ? (ty == 0 ? g.nty - 1 : ty - 1) : ty;
     const int nz = ez ? (tz == 0 ? g.ntz - 1 : tz - 1) : tz;
     const unsigned nb = ((unsigned)nx * g.nty + ny) * g.ntz + nz;
-    sum += faces[(size_t)nb * face_stride + face_index(ex ? BX : lx, ey ? BY : ly, ez ? BZ : lz, BX, BY, BZ)];
+    fi[nsrc++] = (size_t)nb * face_stride + face_index(ex ? BX : lx, ey ? BY : ly, ez ? BZ : lz, BX, BY, BZ);
   }
-  float* dst =
-      mesh + (((size_t)((tx << g.sx) + lx) * g.NY + ((ty << g.sy) + ly)) * g.NZ + ((tz << g.sz) + lz)) * mstride;
-  *dst = *dst + sum;
+  const size_t cell = ((size_t)((tx << g.sx) + lx) * g.NY + ((ty << g.sy) + ly)) * g.NZ + ((tz << g.sz) + lz);
+  for (int c = 0; c < ncomp; ++c) {
+    float sum = 0.f;
+    for (int m = 0; m < nsrc; ++m) sum += faces[c * fcomp + fi[m]];
+    float* dst = mesh + c * mcomp + cell * mstride;
+    *dst = *dst + sum;
+  }
 }
 
 // ------------------------------------------------------------------ adjoint (VJP) kernels
@@ -626,13 +650,12 @@ __global__ void __launch_bounds__(256) k_paint_faces(SortGeom g, const float* __
 // -sign(f - 1), with sign(0) = 0.
 __device__ __forceinline__ float sgnf(float v) { return v > 0.f ? 1.f : (v < 0.f ? -1.f : 0.f); }
 
-// Largest |a[i]| (non-negative floats order like their bits) into *bits.
+// Largest |a[i]| into *bits.
 __global__ void __launch_bounds__(256) k_absmax(const float* __restrict__ a, size_t n, unsigned* __restrict__ bits) {
   float m = 0.f;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     m = fmaxf(m, fabsf(a[i]));
-  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-  if (__lane_id() == 0 && m > 0.f) atomicMax(bits, __float_as_uint(m));
+  block_max_to(m, bits);
 }
 
 // Fixed-point paint scale for deposits of at most |mul| * (*bits as float); *bits re-zeroed.
@@ -670,8 +693,9 @@ __global__ void __launch_bounds__(256) k_div4(const float* __restrict__ U0, cons
   }
 }
 
-// block sum of a double into *dst (one atomic per workgroup)
-__device__ __forceinline__ void block_sum_add(double v, double* dst) {
+// block sum of a double into part[blockIdx.x] (k_finish_sum adds the partials in a fixed order:
+// deterministic, and no same-address atomics from thousands of workgroups)
+__device__ __forceinline__ void block_sum_store(double v, double* part_out) {
   __shared__ double part[8];
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   const int wv = threadIdx.x >> 6;
@@ -680,8 +704,22 @@ __device__ __forceinline__ void block_sum_add(double v, double* dst) {
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += part[i];
-    atomicAdd(dst, t);
+    part_out[blockIdx.x] = t;
   }
+}
+
+// *dst += sum of part[0..n) (one workgroup of 256)
+__global__ void __launch_bounds__(256) k_finish_sum(const double* __restrict__ part, int n, double* __restrict__ dst) {
+  __shared__ double s[256];
+  double t = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) t += part[i];
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *dst += s[0];
 }
 
 // Position cotangent of readout and/or paint at the particles:
@@ -693,7 +731,8 @@ template <int NC, bool HAS_U, bool HAS_RHO, bool WBAR>
 __global__ void __launch_bounds__(256) k_vjp_gather(PartArgs a, const float* __restrict__ u, float umul,
                                                     const float* __restrict__ M, const float* __restrict__ rhob,
                                                     const float* __restrict__ w, float w0, float* __restrict__ xbar,
-                                                    int accumulate, float* __restrict__ wbar, double* __restrict__ dot,
+                                                    int accumulate, float* __restrict__ wbar, double* __restrict__ part,
+                                                    float* __restrict__ vkick, float ckick,
                                                     int NX, int NY, int NZ, int rx, int ry, int rz) {
   const unsigned np = (unsigned)a.npx * a.npy * a.npz;
   const Chunk ch = my_chunk(np);
@@ -733,6 +772,7 @@ __global__ void __launch_bounds__(256) k_vjp_gather(PartArgs a, const float* __r
     }
     const float wt = HAS_RHO ? (w ? w[p] : w0) : 0.f;
     float gx = 0.f, gy = 0.f, gz = 0.f, ws = 0.f, Fd = 0.f;
+    float Fv[NC > 0 ? NC : 1] = {};
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
       const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
@@ -744,6 +784,7 @@ __global__ void __launch_bounds__(256) k_vjp_gather(PartArgs a, const float* __r
           const float m = M[(size_t)q.off[o] * NC + d];
           um += uu[d] * m;
           uf += ur[d] * m;
+          Fv[d] = Fv[d] + m * q.w[o];
         }
         sv += um;
         Fd += q.w[o] * uf;
@@ -763,17 +804,19 @@ __global__ void __launch_bounds__(256) k_vjp_gather(PartArgs a, const float* __r
       g = make_float3(o.x + g.x, o.y + g.y, o.z + g.z);
     }
     st3(xbar, p, g);
+    if (HAS_U && NC == 3 && vkick != nullptr)  // v += F ck, F summed in the readout's corner order
+      st3(vkick, p, axpy3(ld3(vkick, p), make_float3(Fv[0], Fv[1], Fv[2 % NC]), ckick));
     if (WBAR) wbar[p] = ws;
     if (HAS_U) local += (double)Fd;
   }
-  if (HAS_U && dot != nullptr) block_sum_add(local, dot);
+  if (HAS_U && part != nullptr) block_sum_store(local, part);
 }
 
 // Reverse-adjoint preparation of a KKD step (v += F ck; x += v cd) before the state is reversed:
 // dot += <xbar, v>, vbar += cd * xbar, *bits = max |vbar| (for the deposit scale).
 __global__ void __launch_bounds__(256) k_adj_prep(const float* __restrict__ xbar, float* __restrict__ vbar,
                                                   const float* __restrict__ v, size_t n, float cd,
-                                                  double* __restrict__ dot, unsigned* __restrict__ bits) {
+                                                  double* __restrict__ part, unsigned* __restrict__ bits) {
   double local = 0.0;
   float m = 0.f;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -783,9 +826,8 @@ __global__ void __launch_bounds__(256) k_adj_prep(const float* __restrict__ xbar
     vbar[i] = vb;
     m = fmaxf(m, fabsf(vb));
   }
-  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-  if (__lane_id() == 0 && m > 0.f) atomicMax(bits, __float_as_uint(m));
-  block_sum_add(local, dot);
+  block_max_to(m, bits);
+  block_sum_store(local, part);
 }
 
 // x <- x + v*c
@@ -1286,26 +1328,29 @@ int pmx_paint_sort(pmx_plan_t* p, const PartArgs& a, float cd_pre, const float* 
 // Deposit the last sorted particles into `mesh`: particle p carries w[p * wstride] * wmul (w != null)
 // or w0; `is_signed` when any carried value may be negative.
 int pmx_paint_deposit(pmx_plan_t* p, const PartArgs& a_in, const float* w, int wstride, float wmul, float w0,
-                      int is_signed, float* mesh, int mstride, int accumulate, hipStream_t s) {
+                      int is_signed, float* mesh, int mstride, int accumulate, hipStream_t s, int ncomp,
+                      size_t mcomp) {
   PartArgs a = a_in;
   a.v = nullptr;
   int ntiles;
   const SortGeom g = sort_geom(p, a, ntiles);
   const int BX = 1 << g.sx, BY = 1 << g.sy, BZ = 1 << g.sz;
   const int face_stride = pmx_paint_face_cells(p);
+  const size_t fcomp = (size_t)ntiles * face_stride;
   if (is_signed)
-    hipLaunchKernelGGL(k_paint_deposit<true>, dim3(ntiles), dim3(512), 0, s, a, g, w, wstride, wmul, w0, p->ps_tcount,
-                       p->ps_offset, p->ps_seg, p->ps_scale, mesh, mstride, p->ps_faces, face_stride, accumulate);
+    hipLaunchKernelGGL(k_paint_deposit<true>, dim3(ntiles), dim3(512), 0, s, a, g, w, wstride, wmul, w0, ncomp,
+                       p->ps_tcount, p->ps_offset, p->ps_seg, p->ps_scale, mesh, mstride, mcomp, p->ps_faces,
+                       face_stride, fcomp, accumulate);
   else
-    hipLaunchKernelGGL(k_paint_deposit<false>, dim3(ntiles), dim3(512), 0, s, a, g, w, wstride, wmul, w0,
-                       p->ps_tcount, p->ps_offset, p->ps_seg, p->ps_scale, mesh, mstride, p->ps_faces, face_stride,
-                       accumulate);
+    hipLaunchKernelGGL(k_paint_deposit<false>, dim3(ntiles), dim3(512), 0, s, a, g, w, wstride, wmul, w0, ncomp,
+                       p->ps_tcount, p->ps_offset, p->ps_seg, p->ps_scale, mesh, mstride, mcomp, p->ps_faces,
+                       face_stride, fcomp, accumulate);
   int rc = pmx_check_launch("paint_deposit");
   if (rc) return rc;
   const int shell = BX * BY * BZ - (BX - 1) * (BY - 1) * (BZ - 1);
   const size_t nthreads = (size_t)ntiles * shell;
   hipLaunchKernelGGL(k_paint_faces, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s, g, p->ps_faces,
-                     face_stride, mesh, mstride, shell);
+                     face_stride, fcomp, mesh, mstride, mcomp, ncomp, shell);
   return pmx_check_launch("paint_faces");
 }
 
@@ -1418,7 +1463,7 @@ int pmx_launch_lpt1(const float* F, float* dx, float* pv, size_t n, float D, flo
 
 // ------------------------------------------------------------------ adjoint launchers
 int pmx_launch_absmax(const float* a, size_t n, unsigned* bits, hipStream_t s) {
-  hipLaunchKernelGGL(k_absmax, dim3(grid_for(n, 256) < 4096 ? grid_for(n, 256) : 4096), dim3(256), 0, s, a, n, bits);
+  hipLaunchKernelGGL(k_absmax, dim3(grid_for(n, 256) < 1024 ? grid_for(n, 256) : 1024), dim3(256), 0, s, a, n, bits);
   return pmx_check_launch("absmax");
 }
 int pmx_launch_set_scale(unsigned* bits, float mul, float* scale, hipStream_t s) {
@@ -1434,14 +1479,15 @@ int pmx_launch_div4(pmx_plan_t* p, const float* U0, const float* U1, const float
 }
 int pmx_launch_vjp_gather(pmx_plan_t* p, const PartArgs& a, int ncomp, const float* u, float umul, const float* M,
                           const float* rhob, const float* w, float w0, float* xbar, int accumulate, float* wbar,
-                          double* dot, hipStream_t s) {
+                          double* dot, float* vkick, float ckick, hipStream_t s) {
   int rx, ry, rz;
   ratios(p, a, rx, ry, rz);
   const size_t np = (size_t)a.npx * a.npy * a.npz;
   const dim3 G(grid_particles(np)), B(256);
+  double* part = dot ? p->red : nullptr;
 #define PMX_G(NC, HU, HR, WB)                                                                                         \
   hipLaunchKernelGGL((k_vjp_gather<NC, HU, HR, WB>), G, B, 0, s, a, u, umul, M, rhob, w, w0, xbar, accumulate, wbar, \
-                     dot, p->nx, p->ny, p->nz, rx, ry, rz)
+                     part, vkick, ckick, p->nx, p->ny, p->nz, rx, ry, rz)
   if (u && rhob) {
     if (ncomp == 3) PMX_G(3, true, true, false);
     else PMX_G(1, true, true, false);
@@ -1454,11 +1500,17 @@ int pmx_launch_vjp_gather(pmx_plan_t* p, const PartArgs& a, int ncomp, const flo
     PMX_G(1, false, true, false);
   }
 #undef PMX_G
-  return pmx_check_launch("vjp_gather");
+  int rc = pmx_check_launch("vjp_gather");
+  if (rc || !dot || !u) return rc;
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, s, p->red, (int)G.x, dot);
+  return pmx_check_launch("finish_sum");
 }
-int pmx_launch_adj_prep(const float* xbar, float* vbar, const float* v, size_t n, float cd, double* dot,
+int pmx_launch_adj_prep(pmx_plan_t* p, const float* xbar, float* vbar, const float* v, size_t n, float cd, double* dot,
                         unsigned* bits, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_prep, dim3(grid_for(n, 256) < 4096 ? grid_for(n, 256) : 4096), dim3(256), 0, s, xbar, vbar,
-                     v, n, cd, dot, bits);
-  return pmx_check_launch("adj_prep");
+  const int G = grid_for(n, 256) < 1024 ? grid_for(n, 256) : 1024;
+  hipLaunchKernelGGL(k_adj_prep, dim3(G), dim3(256), 0, s, xbar, vbar, v, n, cd, p->red, bits);
+  int rc = pmx_check_launch("adj_prep");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, s, p->red, G, dot);
+  return pmx_check_launch("finish_sum");
 }

@@ -69,12 +69,14 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
     cosmo, _, _ = autodiff.split_cosmo(cosmo_in)
     mesh_size = displacements.mesh_size
     mode = _mode(displacements)
-    events, grid = [], []
-    for ev in time_grid(t0, ts, dt0, t1):
+    tg = time_grid(t0, ts, dt0, t1)
+    grid = [(ev[1], ev[2]) for ev in tg if ev[0] == "step"]
+    coef = solver.step_coeffs(grid, dt0, float(t0), cosmo)
+    events, i = [], 0
+    for ev in tg:
         if ev[0] == "step":
-            op = solver.step_op(ev[1], ev[2], dt0, float(t0), cosmo)
-            events.append(("step", len(grid), op.kick, op.post))
-            grid.append((ev[1], ev[2]))
+            events.append(("step", i, float(coef[i, 0]), float(coef[i, 1])))
+            i += 1
         else:
             events.append(("save", ev[1], ev[2]))
     saved_fields = []
@@ -101,8 +103,7 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
     run.info = {"mesh": mesh_size, "mode": mode, "target": target, "events": events, "force_scale": force_scale}
 
     def coef_fn(c):
-        return np.asarray([[(o := solver.step_op(a, b, dt0, float(t0), c, raw=True)).kick, o.post] for a, b in grid],
-                          dtype=np.float64).reshape(-1, 2)
+        return solver.step_coeffs(grid, dt0, float(t0), c, raw=True)
 
     arr = autodiff.nbody_reverse(displacements.array, velocities.array, cosmo_in, run, coef_fn)
     out = _stack(saved_fields)
@@ -132,11 +133,10 @@ def integrate(displacements, velocities, cosmo, ts, solver: AbstractNBodySolver,
     op0 = solver.init_op(t0, t_init1, dt0, cosmo)
     if op0 is not None:
         ops.append(("op", op0))
-    for ev in time_grid(t0, ts, dt0, t1):
-        if ev[0] == "step":
-            ops.append(("op", solver.step_op(ev[1], ev[2], dt0, state.t_initial, cosmo)))
-        else:
-            ops.append(ev)
+    tg = time_grid(t0, ts, dt0, t1)
+    step_ops = iter(solver.step_ops([(ev[1], ev[2]) for ev in tg if ev[0] == "step"], dt0, state.t_initial, cosmo))
+    for ev in tg:
+        ops.append(("op", next(step_ops)) if ev[0] == "step" else ev)
     pending: StepOp | None = None
     snaps = []
     for item in ops:

@@ -83,6 +83,10 @@ class AbstractNBodySolver:
     def step_op(self, t0, t1, dt, t_initial, cosmo) -> StepOp:
         raise NotImplementedError
 
+    def step_ops(self, grid, dt, t_initial, cosmo) -> list:
+        """step_op for every (t0, t1) of grid."""
+        return [self.step_op(a, b, dt, t_initial, cosmo) for a, b in grid]
+
     # reference interface --------------------------------------------------------
     def _apply(self, op, displacement, velocities, cosmo):
         x = displacement.array.clone()
@@ -123,6 +127,24 @@ class EfficientDriftDoubleKick(AbstractNBodySolver):
         ck = (k1 + k2) / (float(t1) ** 2 * _c.E(cosmo, t1))
         return StepOp(pre=float(f32(cd)), kick=float(f32(ck)))
 
+    def step_ops(self, grid, dt, t_initial, cosmo) -> list:
+        """step_op vectorised over the steps (same float32 time arithmetic and float64 factors)."""
+        if len(grid) == 0:
+            return []
+        t0 = np.asarray([g[0] for g in grid], dtype=np.float32)
+        t1 = np.asarray([g[1] for g in grid], dtype=np.float32)
+        t0t1 = np.sqrt((t0 * t1).astype(np.float32)).astype(np.float32)
+        drift = (_c.Gp(cosmo, t1) - _c.Gp(cosmo, t0)) / _c.gp(cosmo, t0t1)
+        cd = drift / (t0t1.astype(np.float64) ** 3 * _c.E(cosmo, t0t1))
+        t2 = (t1 + f32(dt)).astype(np.float32)
+        t1t2 = np.sqrt((t1 * t2).astype(np.float32)).astype(np.float32)
+        dgf = _c.dGfa(cosmo, t1)
+        gf1 = _c.Gf(cosmo, t1)
+        k1 = (gf1 - _c.Gf(cosmo, t0t1)) / dgf
+        k2 = (_c.Gf(cosmo, t1t2) - gf1) / dgf
+        ck = (k1 + k2) / (t1.astype(np.float64) ** 2 * _c.E(cosmo, t1))
+        return [StepOp(pre=float(f32(a)), kick=float(f32(b))) for a, b in zip(cd, ck)]
+
     def init(self, displacement, velocities, t0, t1, dt, cosmo):
         op = self.init_op(t0, t1, dt, cosmo)
         x, v = self._apply(op, displacement, velocities, cosmo)
@@ -162,6 +184,29 @@ class ReversibleDoubleKickDrift(AbstractNBodySolver):
         if raw:
             return StepOp(kick=float(ck), post=float(cd))
         return StepOp(kick=float(f32(ck)), post=float(f32(cd)))
+
+    def step_coeffs(self, grid, dt, t_initial, cosmo, raw: bool = False) -> np.ndarray:
+        """(nsteps, 2) array of (kick, drift) for the steps [(t0, t1), ...]: step_op vectorised over the
+        steps (the same float32 time arithmetic and float64 factors, element by element)."""
+        if len(grid) == 0:
+            return np.zeros((0, 2))
+        t0 = np.asarray([g[0] for g in grid], dtype=np.float32)
+        t1 = np.asarray([g[1] for g in grid], dtype=np.float32)
+        t_prev = np.maximum((t0 - f32(dt)).astype(np.float32), f32(t_initial))
+        ai_1 = np.sqrt((t_prev * t0).astype(np.float32)).astype(np.float32)
+        af_2 = np.sqrt((t0 * t1).astype(np.float32)).astype(np.float32)
+        dgf = _c.dGfa(cosmo, t0)
+        gf_t0 = _c.Gf(cosmo, t0)
+        k1 = (gf_t0 - _c.Gf(cosmo, ai_1)) / dgf
+        k2 = (_c.Gf(cosmo, af_2) - gf_t0) / dgf
+        ck = (k1 + k2) / (t0.astype(np.float64) ** 2 * _c.E(cosmo, t0))
+        drift = (_c.Gp(cosmo, t1) - _c.Gp(cosmo, t0)) / _c.gp(cosmo, af_2)
+        cd = drift / (af_2.astype(np.float64) ** 3 * _c.E(cosmo, af_2))
+        out = np.stack([ck, cd], axis=-1).astype(np.float64)
+        return out if raw else out.astype(np.float32).astype(np.float64)
+
+    def step_ops(self, grid, dt, t_initial, cosmo) -> list:
+        return [StepOp(kick=float(a), post=float(b)) for a, b in self.step_coeffs(grid, dt, t_initial, cosmo)]
 
     def reverse_op(self, t0, t1, dt, t_initial, cosmo) -> StepOp:
         """Exact inverse of step_op(t0, t1): x -= v cd, then v -= F(x) ck."""

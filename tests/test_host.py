@@ -129,3 +129,27 @@ def test_field_validation(pkg):
                          observer_position=(0.5, 0.5, 0.0))
     assert f.max_comoving_radius == pytest.approx(50.0)
     assert f.observer_position_mpc == (50.0, 100.0, 0.0)
+
+
+def test_vectorised_kkd_coefficients_equal_step_op(pkg):
+    """step_coeffs (all steps at once, used by the differentiable path) reproduces step_op exactly."""
+    cosmo = pkg.Cosmology(Omega_c=0.25, Omega_b=0.0486, h=0.6774, n_s=0.9667, sigma8=0.8)
+    solver = pkg.ReversibleDoubleKickDrift(interp_kernel=pkg.NoInterp(painting=pkg.PaintingOptions(target="density")))
+    grid = [(ev[1], ev[2]) for ev in pkg.pm.time_grid(0.1, [0.37, 1.0], 0.09, 1.0) if ev[0] == "step"]
+    vec = solver.step_coeffs(grid, 0.09, 0.1, cosmo)
+    raw = solver.step_coeffs(grid, 0.09, 0.1, cosmo, raw=True)
+    for i, (a, b) in enumerate(grid):
+        op = solver.step_op(a, b, 0.09, 0.1, cosmo)
+        opr = solver.step_op(a, b, 0.09, 0.1, cosmo, raw=True)
+        assert vec[i, 0] == op.kick and vec[i, 1] == op.post
+        # float64 values (used only for parameter derivatives) agree to the last bits
+        assert abs(raw[i, 0] - opr.kick) <= 4e-16 * abs(opr.kick) and abs(raw[i, 1] - opr.post) <= 4e-16 * abs(opr.post)
+
+
+def test_vectorised_kdk_ops_equal_step_op(pkg):
+    cosmo = pkg.Cosmology(Omega_c=0.25, Omega_b=0.0486, h=0.6774, n_s=0.9667, sigma8=0.8)
+    solver = pkg.EfficientDriftDoubleKick(interp_kernel=pkg.NoInterp(painting=pkg.PaintingOptions(target="density")))
+    grid = [(ev[1], ev[2]) for ev in pkg.pm.time_grid(0.1, [0.41, 0.73, 1.0], 0.05, 1.0) if ev[0] == "step"]
+    for (a, b), op in zip(grid, solver.step_ops(grid, 0.05, 0.1, cosmo)):
+        ref = solver.step_op(a, b, 0.05, 0.1, cosmo)
+        assert op.pre == ref.pre and op.kick == ref.kick and op.post == ref.post
