@@ -125,7 +125,8 @@ class Plan:
         self._box = None
 
     def escapes(self) -> int:
-        """Particles the last tiled paint sent to the atomic escape pass (synchronises)."""
+        """Particles the last sorted paint placed one by one (outside their sub-block's tile window;
+        synchronises)."""
         c = ctypes.c_uint64()
         with torch.cuda.device(self.device):
             check(load().pmx_plan_escapes(self.handle, ctypes.byref(c)), "pmx_plan_escapes")

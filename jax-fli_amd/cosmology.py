@@ -13,9 +13,11 @@ Gf = f D a^2 E, gp = f D / a, and dGfa = dGf/da, which the growth equation turns
 
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 import functools
 import math
+import os
 
 import numpy as np
 
@@ -108,8 +110,29 @@ def _growth_table(cosmo: Cosmology):
     return _growth_table_bg(_background(cosmo))
 
 
+@functools.lru_cache(maxsize=1)
+def _host_lib():
+    """libpmx_host.so (the growth-table loop in C++), or None when it has not been built."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpmx_host.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    d, i, p = ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+    lib.pmx_host_growth_table.argtypes = [d, d, d, d, d, d, d, d, i, i, p, p, p]
+    lib.pmx_host_growth_table.restype = i
+    return lib
+
+
+def _growth_table_native(lib, cosmo: _Background, x0, h, n_tot):
+    D, G, dG = (np.empty(n_tot + 1) for _ in range(3))
+    p = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    lib.pmx_host_growth_table(cosmo.Omega_m, cosmo.Omega_k, cosmo.Omega_de, cosmo.w0, cosmo.wa, x0, _A_INIT, h,
+                              _NSTEP, n_tot, p(D), p(G), p(dG))
+    return D, G, dG
+
+
 @functools.lru_cache(maxsize=1024)
-def _growth_table_bg(cosmo: _Background):
+def _growth_table_bg(cosmo: _Background, native: bool = True):
     """RK4 in ln a from the matter-dominated growing mode D = a, normalised to D(1) = 1. The grid
     continues past a = 1 to a = 2: the KDK double kick evaluates Gf at the nominal next half step
     sqrt(t1 (t1 + dt)), which exceeds 1 on the last step."""
@@ -117,6 +140,10 @@ def _growth_table_bg(cosmo: _Background):
     h = -x0 / _NSTEP
     n_tot = _NSTEP + int(math.ceil(math.log(_A_MAX) / h))
     xs = x0 + h * np.arange(n_tot + 1)
+    lib = _host_lib() if native else None
+    if lib is not None:  # the same loop, operation for operation, in C++
+        Ds, Gs, dG = _growth_table_native(lib, cosmo, x0, h, n_tot)
+        return xs, Ds, Gs, dG
     Ds = [0.0] * (n_tot + 1)
     Gs = [0.0] * (n_tot + 1)
     D = G = _A_INIT

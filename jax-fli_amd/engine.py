@@ -44,10 +44,10 @@ def _weights(weights, npart: int):
 
 
 def paint(pos: torch.Tensor, mesh_shape, mode="relative", weights=1.0, out: torch.Tensor | None = None,
-          accumulate=False) -> torch.Tensor:
+          accumulate=False, plan=None) -> torch.Tensor:
     pos = _f32(pos, "pos")
     npx, npy, npz = _pgrid(pos)
-    plan = get_plan(mesh_shape, pos.device)
+    plan = plan or get_plan(mesh_shape, pos.device)
     if out is None:
         out = torch.empty(tuple(mesh_shape), dtype=torch.float32, device=pos.device)
     w, w0 = _weights(weights, npx * npy * npz)
@@ -110,7 +110,7 @@ def forces(pos: torch.Tensor, mesh_shape, mode="relative", scale=1.0, deconv=Fal
 
 
 def step(x: torch.Tensor, v: torch.Tensor, mesh_shape, mode="relative", force_scale=1.0, c_kick=0.0,
-         c_drift_pre=None, c_drift_post=None, deconv=False, atomic_paint=False) -> None:
+         c_drift_pre=None, c_drift_post=None, deconv=False, atomic_paint=False, plan=None) -> None:
     """In place: [x += v*c_drift_pre]; v += F(x)*force_scale*c_kick; [x += v*c_drift_post]."""
     if not (x.is_contiguous() and v.is_contiguous()):
         raise _lib.PmxError("state tensors must be contiguous")
@@ -119,7 +119,7 @@ def step(x: torch.Tensor, v: torch.Tensor, mesh_shape, mode="relative", force_sc
     npx, npy, npz = _pgrid(x)
     if v.shape != x.shape:
         raise ValueError("x and v must have the same shape")
-    plan = get_plan(mesh_shape, x.device)
+    plan = plan or get_plan(mesh_shape, x.device)
     flags = (_lib.DECONV if deconv else 0) | (_lib.PAINT_ATOMIC if atomic_paint else 0)
     flags |= _lib.DRIFT_PRE if c_drift_pre is not None else 0
     flags |= _lib.DRIFT_POST if c_drift_post is not None else 0
@@ -143,9 +143,10 @@ def power_sums(mesh: torch.Tensor, box, kedges: torch.Tensor):
     return pk, ks, kc
 
 
-def white_noise(mesh_shape, seed: int, device=None) -> torch.Tensor:
-    plan = get_plan(mesh_shape, device)
-    out = torch.empty(tuple(mesh_shape), dtype=torch.float32, device=plan.device)
+def white_noise(mesh_shape, seed: int, device=None, out: torch.Tensor | None = None, plan=None) -> torch.Tensor:
+    plan = plan or get_plan(mesh_shape, device)
+    if out is None:
+        out = torch.empty(tuple(mesh_shape), dtype=torch.float32, device=plan.device)
     check(load().pmx_white_noise(plan.handle, ctypes.c_uint64(int(seed) & (2**64 - 1)), ptr(out), stream_ptr()),
           "pmx_white_noise")
     return out
@@ -159,23 +160,28 @@ def philox_raw(seed: int, nblocks: int, device=None) -> torch.Tensor:
     return out
 
 
-def color_field(field: torch.Tensor, box, ktab: torch.Tensor, ptab: torch.Tensor) -> torch.Tensor:
+def color_field(field: torch.Tensor, box, ktab: torch.Tensor, ptab: torch.Tensor, out: torch.Tensor | None = None,
+                plan=None) -> torch.Tensor:
     field = _f32(field, "field")
-    plan = get_plan(field.shape, field.device)
+    plan = plan or get_plan(field.shape, field.device)
     plan.set_box(box)
     ktab = ktab.to(device=field.device, dtype=torch.float64).contiguous()
     ptab = ptab.to(device=field.device, dtype=torch.float64).contiguous()
-    out = torch.empty_like(field)
+    if out is None:
+        out = torch.empty_like(field)
     check(load().pmx_color_field(plan.handle, ptr(field), ptr(ktab), ptr(ptab), ktab.numel(), ptr(out),
                                  stream_ptr()), "pmx_color_field")
     return out
 
 
-def lpt1(ic: torch.Tensor, growth: float, vel_factor: float):
+def lpt1(ic: torch.Tensor, growth: float, vel_factor: float, out=None, plan=None):
     ic = _f32(ic, "ic")
-    plan = get_plan(ic.shape, ic.device)
-    dx = torch.empty(tuple(ic.shape) + (3,), dtype=torch.float32, device=ic.device)
-    pv = torch.empty_like(dx)
+    plan = plan or get_plan(ic.shape, ic.device)
+    if out is None:
+        dx = torch.empty(tuple(ic.shape) + (3,), dtype=torch.float32, device=ic.device)
+        pv = torch.empty_like(dx)
+    else:
+        dx, pv = out
     check(load().pmx_lpt1(plan.handle, ptr(ic), float(growth), float(vel_factor), ptr(dx), ptr(pv), stream_ptr()),
           "pmx_lpt1")
     return dx, pv

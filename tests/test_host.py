@@ -153,3 +153,15 @@ def test_vectorised_kdk_ops_equal_step_op(pkg):
     for (a, b), op in zip(grid, solver.step_ops(grid, 0.05, 0.1, cosmo)):
         ref = solver.step_op(a, b, 0.05, 0.1, cosmo)
         assert op.pre == ref.pre and op.kick == ref.kick and op.post == ref.post
+
+
+def test_native_growth_table_equals_python_loop(pkg):
+    """The C++ growth-table loop (libpmx_host.so) reproduces the Python RK4 loop bit for bit."""
+    c = pkg.cosmology
+    if c._host_lib() is None:
+        pytest.skip("libpmx_host.so not built")
+    for p in ({}, {"Omega_k": 0.02, "w0": -0.9, "wa": 0.1}):
+        bg = c._background(c.Cosmology(Omega_c=0.27, **p))
+        xs_n, D_n, G_n, dG_n = c._growth_table_bg(bg, True)
+        xs_p, D_p, G_p, dG_p = c._growth_table_bg(bg, False)
+        assert np.array_equal(D_n, D_p) and np.array_equal(G_n, G_p) and np.array_equal(dG_n, dG_p)
