@@ -227,6 +227,29 @@ def bench_vjp(pkg, dev):
                          "sigma8": {"adjoint": float(grads["s8"]), "fd": fd_s8}}}
 
 
+def bench_serving(pkg, dev, count=512):
+    """Serving workload: `count` forward-simulation requests, mesh uniform over {64, 96, 128, 192}^3
+    (box 3.9 N Mpc/h), own seed, Omega_c ~ U(0.2, 0.35), sigma8 ~ U(0.7, 0.9), 1LPT + KDK dt0 = 0.09 to
+    a = 1 (10 steps), float32. Reports the saturated throughput (all requests at t = 0), the same
+    requests one at a time (no batching), and p50/p99 latency of a Poisson stream offered at 70% of the
+    saturated throughput."""
+    srv = pkg.serving.Server(device=dev, max_active=48)
+    srv.prewarm({64: 12, 96: 12, 128: 12, 192: 12})
+    burst = pkg.serving.poisson_requests(count, float("inf"), seed=11)
+    srv.serve(burst[:16])  # warm-up
+    _, sat = srv.serve(burst)
+    one = pkg.serving.Server(device=dev, max_active=1)
+    one._pool, one._slot_bytes = srv._pool, srv._slot_bytes
+    _, serial = one.serve(burst)
+    rate = 0.7 * sat["throughput_req_s"]
+    _, pois = srv.serve(pkg.serving.poisson_requests(count, rate, seed=12))
+    r3 = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
+    return {"workload": f"{count} requests, mesh uniform over 64/96/128/192^3, box 3.9 N Mpc/h, 1LPT + KDK "
+                        "dt0=0.09 a=0.1->1, per-request seed and (Omega_c, sigma8)",
+            "saturated": r3(sat), "one_at_a_time": r3(serial),
+            "poisson": {"offered_req_s": round(rate, 1), **r3(pois)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,6 +257,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=1, help="CPU-baseline steps at full size (0: skip)")
     ap.add_argument("--no-vjp", action="store_true", help="skip the forward+VJP measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the per-phase HBM traffic from two rocprofv3 --pmc runs of this script and exit")
     args = ap.parse_args()
@@ -396,6 +420,7 @@ def main():
     del spec
 
     vjp = None if args.no_vjp else bench_vjp(pkg, dev)
+    serving = None if args.no_serving else bench_serving(pkg, dev)
 
     # CPU baseline: the NumPy/SciPy restatement of the reference step on the host cores
     cpu = None
@@ -446,6 +471,7 @@ def main():
                                 "the schedule's PM steps without the shell paints"},
         "forward_total_ms": round(forward_ms, 1),
         "vjp": vjp,
+        "serving": serving,
         "fft_roundtrip_ms": {"pmx": round(fft_ours, 4), "rocfft": round(fft_roc, 4)},
         "final_pk": {"k": [round(float(a), 5) for a in k_fin[:8]], "pk": [round(float(a), 3) for a in pk_fin[:8]]},
     }

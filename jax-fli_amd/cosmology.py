@@ -177,10 +177,37 @@ def _hermite(xs, f, df, x):
     return h00 * f[i] + h10 * h * df[i] + h01 * f[i + 1] + h11 * h * df[i + 1]
 
 
+def _hermite2(xs, f, df, g, dg, x):
+    """_hermite of (f, df) and (g, dg) at the same points (shared index and basis)."""
+    x = np.asarray(x, dtype=np.float64)
+    h = xs[1] - xs[0]
+    i = np.clip(((x - xs[0]) / h).astype(np.int64), 0, len(xs) - 2)
+    t = (x - xs[i]) / h
+    h00 = (1 + 2 * t) * (1 - t) ** 2
+    h10 = t * (1 - t) ** 2
+    h01 = t * t * (3 - 2 * t)
+    h11 = t * t * (t - 1)
+    return (h00 * f[i] + h10 * h * df[i] + h01 * f[i + 1] + h11 * h * df[i + 1],
+            h00 * g[i] + h10 * h * dg[i] + h01 * g[i + 1] + h11 * h * dg[i + 1])
+
+
+_DG_CACHE: dict = {}
+
+
 def _growth_DG(cosmo: Cosmology, a):
+    """(D, dD/dln a) at a; memoised per (background, a) since the coefficient formulas ask for the
+    same scale factors several times."""
+    a = np.asarray(a, dtype=np.float64)
+    key = (_background(cosmo), a.shape, a.tobytes())
+    hit = _DG_CACHE.get(key)
+    if hit is not None:
+        return hit
     xs, D, G, dG = _growth_table(cosmo)
-    x = np.log(np.asarray(a, dtype=np.float64))
-    return _hermite(xs, D, G, x), _hermite(xs, G, dG, x)
+    out = _hermite2(xs, D, G, G, dG, np.log(a))
+    if len(_DG_CACHE) > 4096:
+        _DG_CACHE.clear()
+    _DG_CACHE[key] = out
+    return out
 
 
 def growth_factor(cosmo: Cosmology, a):

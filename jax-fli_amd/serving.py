@@ -11,7 +11,8 @@ displacements and the painted density. The reference runs such simulations one a
   arrive while others run wait in arrival order and are admitted as memory frees up;
 * every admitted request owns a stream and a plan (pooled per mesh size), and the scheduler issues
   the requests' steps round-robin, at most `lookahead` steps ahead of the device per request, so the
-  steps of different requests overlap on the GPU (small meshes fill the GPU only together);
+  steps of different requests overlap on the GPU (small meshes fill the GPU only together) and the
+  queued steps cover the host's preparation of newly admitted requests;
 * all host work of a request (cosmology tables, time grid, kick/drift coefficients) is done once at
   admission; the device work of a step is one fused pmx_step call.
 
@@ -133,7 +134,8 @@ class _Slot:
 class Server:
     """Continuous-batching forward-simulation server (one GPU, one host thread)."""
 
-    def __init__(self, device=None, hbm_budget: int | None = None, max_active: int = 64, lookahead: int = 2):
+    def __init__(self, device=None, hbm_budget: int | None = None, max_active: int = 64, lookahead: int = 4,
+                 admit_per_pass: int = 4):
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info(self.device)
@@ -141,6 +143,7 @@ class Server:
         self.hbm_budget = int(hbm_budget)
         self.max_active = int(max_active)
         self.lookahead = int(lookahead)
+        self.admit_per_pass = int(admit_per_pass)
         self._pool: dict[int, list[_Slot]] = {}
         self._slot_bytes: dict[int, int] = {}
 
@@ -213,8 +216,11 @@ class Server:
         t0 = clock()
         while pending or active:
             now = clock() - t0
-            # admission: arrival order, by free slot and HBM budget
-            while pending and pending[0].arrival <= now and len(active) < self.max_active:
+            # admission: arrival order, by free slot and HBM budget; a few per pass so that the host
+            # preparation of new requests interleaves with issuing the queued ones' steps
+            admitted = 0
+            while (pending and pending[0].arrival <= now and len(active) < self.max_active
+                   and admitted < self.admit_per_pass):
                 req = pending[0]
                 need = self._slot_size(req.mesh)
                 if used + need > self.hbm_budget:
@@ -225,6 +231,7 @@ class Server:
                 s.stages = self._stages(s, req)
                 used += need
                 active.append(s)
+                admitted += 1
             peak_active = max(peak_active, len(active))
             issued = False
             for s in active:  # round-robin: one stage per request per pass
