@@ -16,6 +16,7 @@ untimed steps first. Prints one JSON line.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import math
 import os
@@ -325,11 +326,19 @@ def main():
             if item[0] == "op":
                 done += 1
 
-    def run(items):
+    shell_events = []
+
+    def run(items, time_shells=False):
         for item in items:
             if item[0] == "reset":
                 x.copy_(x0)
                 v.copy_(v0)
+            elif item[0] == "shell" and time_shells:
+                a_, b_ = time_shells[len(shell_events)]
+                a_.record()
+                run_item(item)
+                b_.record()
+                shell_events.append((a_, b_))
             elif item[0] != "final":
                 run_item(item)
 
@@ -341,13 +350,19 @@ def main():
     K = args.steps if args.steps > 0 else nsteps_sched
     items = list(steps_iter(K))
     torch.cuda.synchronize()
-    plan.timing(True)
+    plan.timing(True, steps=K)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    shell_pool = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                  for _ in range(sum(1 for it in items if it[0] == "shell"))]
+    gc.collect()
+    gc.disable()  # no collector pauses while the host issues the timed work
     e0.record()
-    run(items)
+    run(items, time_shells=shell_pool)
     e1.record()
     torch.cuda.synchronize()
+    gc.enable()
     elapsed_ms = e0.elapsed_time(e1)
+    shell_ms = [round(a_.elapsed_time(b_), 3) for a_, b_ in shell_events]
     phases, nrec = plan.timing_read()
     plan.timing(False)
     ms_per_step = elapsed_ms / K
@@ -463,6 +478,7 @@ def main():
                           "achieved_GB_per_s": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
                           "frac": round(step_bytes / (ms_per_step * 1e-3) / PEAK_HBM, 4)},
         "phases": per_phase,
+        "shell_paint_ms": shell_ms,
         "cpu_baseline": cpu,
         "unfused_gpu": {"ms_per_step": round(unf_ms, 4), "fused_ms_per_step": round(fused_ms, 4),
                         "speedup": round(unf_ms / fused_ms, 2), "force_rel_l2_vs_fused": force_err,

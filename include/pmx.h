@@ -44,8 +44,9 @@ size_t pmx_plan_workspace_bytes(pmx_plan plan);
  * 5x5x5-tile window and were counted and placed one by one with global atomics (particles displaced
  * by more than about two tiles). Synchronises the device. */
 int pmx_plan_escapes(pmx_plan plan, uint64_t* count);
-/* Per-kernel timing of pmx_step on this plan. enable=1 clears previous records and brackets every
- * following pmx_step with HIP events on its stream; pmx_plan_timing_read synchronises on the last
+/* Per-kernel timing of pmx_step on this plan. enable > 0 clears previous records, creates the HIP
+ * events for max(enable, 64) steps up front (timed steps create none) and brackets every following
+ * pmx_step with them on its stream; enable = 0 stops; pmx_plan_timing_read synchronises on the last
  * event and returns the summed milliseconds of the 8 phases (paint, r2c_z, c2c_y, k-space x pass,
  * inverse c2c_y, c2r_z, gradient stencil, readout+kick+drift) and the number of steps recorded. */
 int pmx_plan_timing(pmx_plan plan, int enable);

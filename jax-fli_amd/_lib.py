@@ -134,9 +134,10 @@ class Plan:
 
     PHASES = ("paint", "r2c_z", "c2c_y", "kspace_x", "c2c_y_inv", "c2r_z", "grad4", "readout_kick_drift")
 
-    def timing(self, enable: bool) -> None:
+    def timing(self, enable: bool, steps: int = 64) -> None:
+        """Start (events for `steps` steps created now) or stop per-phase timing of pmx_step."""
         with torch.cuda.device(self.device):
-            check(load().pmx_plan_timing(self.handle, int(bool(enable))), "pmx_plan_timing")
+            check(load().pmx_plan_timing(self.handle, int(steps) if enable else 0), "pmx_plan_timing")
 
     def timing_read(self):
         """(phase -> summed ms, number of steps) of the pmx_step calls since timing(True)."""

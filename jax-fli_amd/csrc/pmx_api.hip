@@ -197,13 +197,20 @@ int force_adjoint(pmx_plan_t* p, const PartArgs& a, const float* u, float umul, 
                                ckick, s);
 }
 
-StepMarks* new_marks(pmx_plan_t* p) {
-  if (!p->timing || !p->marks) return nullptr;
+// Events come from a pool created when timing is enabled, so a timed step creates none (event
+// creation can stall the host); the pool grows only when more steps are timed than reserved.
+bool add_marks(pmx_plan_t* p) {
   StepMarks m;
   for (int i = 0; i < kStepMarks; ++i)
-    if (hipEventCreate(&m.e[i]) != hipSuccess) return nullptr;
+    if (hipEventCreate(&m.e[i]) != hipSuccess) return false;
   p->marks->push_back(m);
-  return &p->marks->back();
+  return true;
+}
+
+StepMarks* new_marks(pmx_plan_t* p) {
+  if (!p->timing || !p->marks) return nullptr;
+  if (p->marks_used == p->marks->size() && !add_marks(p)) return nullptr;
+  return &(*p->marks)[p->marks_used++];
 }
 
 void clear_marks(pmx_plan_t* p) {
@@ -211,6 +218,7 @@ void clear_marks(pmx_plan_t* p) {
   for (auto& m : *p->marks)
     for (int i = 0; i < kStepMarks; ++i) (void)hipEventDestroy(m.e[i]);
   p->marks->clear();
+  p->marks_used = 0;
 }
 
 }  // namespace
@@ -500,8 +508,14 @@ int pmx_step_reverse_vjp(pmx_plan p, float* x, float* v, int npx, int npy, int n
 int pmx_plan_timing(pmx_plan p, int enable) {
   PMX_TRY(check_plan(p));
   if (!p->marks) p->marks = new std::vector<StepMarks>();
-  if (enable) clear_marks(p);
-  p->timing = enable ? 1 : 0;
+  if (enable > 0) {
+    clear_marks(p);
+    const int reserve = enable > 64 ? enable : 64;
+    p->marks->reserve(reserve);
+    for (int i = 0; i < reserve; ++i)
+      if (!add_marks(p)) return pmx_set_error(3, "timing: hipEventCreate failed");
+  }
+  p->timing = enable > 0 ? 1 : 0;
   return 0;
 }
 
@@ -510,16 +524,17 @@ int pmx_plan_timing_read(pmx_plan p, double* ms, int* nsteps) {
   if (!ms || !nsteps) return pmx_set_error(1, "null output");
   for (int i = 0; i < kStepMarks - 1; ++i) ms[i] = 0.0;  // kStepMarks - 1 phases
   *nsteps = 0;
-  if (!p->marks || p->marks->empty()) return 0;
-  PMX_HIP(hipEventSynchronize(p->marks->back().e[kStepMarks - 1]));
-  for (auto& mk : *p->marks) {
+  if (!p->marks || p->marks_used == 0) return 0;
+  PMX_HIP(hipEventSynchronize((*p->marks)[p->marks_used - 1].e[kStepMarks - 1]));
+  for (size_t k = 0; k < p->marks_used; ++k) {
+    const StepMarks& mk = (*p->marks)[k];
     for (int i = 0; i < kStepMarks - 1; ++i) {
       float t = 0.f;
       PMX_HIP(hipEventElapsedTime(&t, mk.e[i], mk.e[i + 1]));
       ms[i] += t;
     }
   }
-  *nsteps = (int)p->marks->size();
+  *nsteps = (int)p->marks_used;
   return 0;
 }
 

@@ -54,6 +54,7 @@ struct pmx_plan_t {
   size_t workspace_bytes;
   int timing;
   std::vector<StepMarks>* marks;
+  size_t marks_used;
 };
 
 constexpr int kRedMax = 65536;  // partial-sum slots: the largest particle-kernel grid
