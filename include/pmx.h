@@ -116,6 +116,19 @@ int pmx_color_field(pmx_plan plan, const float* field, const double* ktab, const
  * dx = growth * F, p = vel_factor * dx. jaxpm.pm.lpt(order=1) as called by lpt (pm/lpt.py:119-127). */
 int pmx_lpt1(pmx_plan plan, const float* ic, float growth, float vel_factor, float* dx, float* p, void* stream);
 
+/* pmx_lpt1 for a particle lattice coarser than the mesh (npx, npy, npz dividing it): particle
+ * (i, j, k) sits on cell (i nx/npx, j ny/npy, k nz/npz) and reads the force there. This is
+ * jaxpm.pm.lpt with initial_particles set to that sub-lattice (pm/lpt.py:86-127), the set-up of
+ * 512^3 particles on a 1024^3 mesh. */
+int pmx_lpt1_lattice(pmx_plan plan, const float* ic, int npx, int npy, int npz, float growth, float vel_factor,
+                     float* dx, float* p, void* stream);
+
+/* Cotangent of the initial field for pmx_lpt1 / pmx_lpt1_lattice: ic_bar = the transpose of
+ * F(ic) applied to growth * (dx_bar + vel_factor * p_bar) placed on the lattice cells. dx_bar or
+ * p_bar may be null (zero). Replaces jax.vjp of jaxpm.pm.lpt(order=1). */
+int pmx_lpt1_vjp(pmx_plan plan, const float* dx_bar, const float* p_bar, int npx, int npy, int npz, float growth,
+                 float vel_factor, float* ic_bar, void* stream);
+
 /* HEALPix shell paint: particles with rmin < |x - observer| <= rmax (Mpc/h, periodic box) painted
  * to RING pixels, scheme 0 = ngp, 1 = bilinear (HEALPix ring interpolation weights), map =
  * weighted count / (pixel solid angle * (rmax^3 - rmin^3) / 3). paint_particles_spherical as called
@@ -123,6 +136,18 @@ int pmx_lpt1(pmx_plan plan, const float* ic, float growth, float vel_factor, flo
 int pmx_paint_healpix(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, const float* weights,
                       float weight, const double box[3], const double observer[3], double rmin, double rmax,
                       int nside, int scheme, float* map, int accumulate, void* stream);
+
+/* pmx_paint_healpix over a table of `ntiles` images of the periodic box (at most 4096): image t
+ * places the observer-centred position X at R_t X + tile_shift[3t..3t+2] (Mpc/h), R_t the cube
+ * symmetry tile_rot[t] of the 48-entry table of get_all_47_symmetries (pm/interp.py:31-43; index
+ * 6 * sign_index + permutation_index, 0 = identity). Device arrays tile_rot (int32) and tile_shift
+ * (float64). The images are summed into one map: OnionTiler._paint_tiled_27 (pm/interp.py:425-504)
+ * with 27 images, and plain periodic replication with identity rotations. tile_rot == NULL paints
+ * the box alone (= pmx_paint_healpix). */
+int pmx_paint_healpix_tiled(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode,
+                            const float* weights, float weight, const double box[3], const double observer[3],
+                            double rmin, double rmax, int nside, int scheme, int ntiles, const int* tile_rot,
+                            const double* tile_shift, float* map, int accumulate, void* stream);
 
 /* Flat-sky slab paint: absolute z in (zlo, zhi] (grid units), (x, y) rescaled to (fnx, fny) and
  * wrapped, 2D CIC, times `scale` (1 / pixel volume). _single_paint_2d (_painting.py:117-202). */
@@ -150,6 +175,16 @@ int pmx_readout_cic_vjp(pmx_plan plan, const float* mesh, int ncomp, const float
                         int mode, const float* out_bar, float* mesh_bar, float* pos_bar, int accumulate,
                         void* stream);
 
+/* Position cotangent of pmx_paint_healpix_tiled (tile_rot NULL: of pmx_paint_healpix):
+ * pos_bar (+)= d<map_bar, map(pos)>/d pos through the bilinear HEALPix weights of every image of a
+ * particle inside the shell (the ngp scheme is piecewise constant: zero). Shell membership is a
+ * step and contributes nothing, as under jax.vjp. Replaces jax.vjp of paint_particles_spherical
+ * (called by _single_paint_spherical, _painting.py:249-348). */
+int pmx_paint_healpix_vjp(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, const float* weights,
+                          float weight, const double box[3], const double observer[3], double rmin, double rmax,
+                          int nside, int scheme, int ntiles, const int* tile_rot, const double* tile_shift,
+                          const float* map_bar, float* pos_bar, int accumulate, void* stream);
+
 /* Density cotangent of pmx_force_mesh: density_bar = G^T S^T fmesh_bar (S the 4-point gradient,
  * antisymmetric; G the Green's operator, symmetric). Replaces jax.vjp of the k-space part of
  * pm_forces (used by the LPT adjoint, pm/lpt.py:119). */
@@ -174,6 +209,23 @@ int pmx_forces_vjp(pmx_plan plan, const float* pos, int npx, int npy, int npz, i
 int pmx_step_reverse_vjp(pmx_plan plan, float* x, float* v, int npx, int npy, int npz, int mode, float force_scale,
                          float c_kick, float c_drift, unsigned flags, float* x_bar, float* v_bar, double* coef_bar,
                          void* stream);
+
+/* ---- Born convergence ---------------------------------------------------------------------------
+ * kappa[k][p] = sum_s coef[k][s] * (maps[s][p] / mean_s - 1) over `nshell` HEALPix (or any
+ * same-length) density maps, mean_s the map's pixel mean (a zero mean is replaced by FLT_EPSILON).
+ * coef (device float64, nsrc x nshell, at most 16 x 256) holds
+ * 3/2 Om (H0/c)^2 d_r r_s / a_s times the source-weighted lensing efficiency clip(1 - r_s/chi, 0, 1000):
+ * _born_core_impl and _born_spherical (src/jax_fli/_src/lensing/_born.py:29-127) with the n(z)
+ * quadrature folded into coef on the host. Uses the plan only for scratch. */
+int pmx_born(pmx_plan plan, const float* maps, int nshell, int64_t npix, const double* coef, int nsrc, float* kappa,
+             int accumulate, void* stream);
+
+/* Cotangent of pmx_born: maps_bar (+)= d<kappa_bar, kappa>/d maps, including the dependence of
+ * the shell means. dots (optional, device float64, nsrc*nshell + nsrc + nshell): D[k*nshell+s] =
+ * sum_p kappa_bar[k][p] maps[s][p], then E[k] = sum_p kappa_bar[k][p], then the means m[s] as
+ * used, from which the host forms d<kappa_bar, kappa>/d coef[k][s] = D / m_s - E. */
+int pmx_born_vjp(pmx_plan plan, const float* maps, int nshell, int64_t npix, const double* coef, int nsrc,
+                 const float* kappa_bar, float* maps_bar, int accumulate, double* dots, void* stream);
 
 #ifdef __cplusplus
 }

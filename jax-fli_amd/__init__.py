@@ -5,11 +5,14 @@ registers it). The compute path is libpmx.so (hand-written HIP for gfx950, C ABI
 PyTorch only provides device memory and streams. Names follow ``import jax_fli as jfli``.
 """
 
-from . import _lib, autodiff, cosmology, engine, fields, initial, pm, power, serving, utils
+from . import _lib, autodiff, cosmology, engine, fields, initial, lensing, pm, power, redshift, serving, utils
 from .cosmology import Cosmology, Planck18
 from .fields import (AbstractField, ConvergenceUnit, DensityField, DensityUnit, FieldStatus, FlatDensity,
-                     ParticleField, PhysicalUnit, PositionUnit, SphericalDensity)
+                     FlatKappaField, ParticleField, PhysicalUnit, PositionUnit, SphericalDensity,
+                     SphericalKappaField)
 from .initial import gaussian_initial_conditions, interpolate_initial_conditions
+from .lensing import born
+from .redshift import redshift_distribution, tophat_z
 from .painting import PaintingOptions
 from .pm import (AbstractCorrection, AbstractInterp, AbstractNBodySolver, DriftInterp, EfficientDriftDoubleKick,
                  InterpTilerState, NBodyState, NoCorrection, NoInterp, OnionTiler, PGDKernel,

@@ -54,6 +54,14 @@ SIGNATURES = {
     "pmx_force_mesh_vjp": (_i, [_vp, _vp, _f, _u, _vp, _vp]),
     "pmx_forces_vjp": (_i, [_vp, _vp, _i, _i, _i, _i, _f, _u, _vp, _vp, _i, _vp, _vp]),
     "pmx_step_reverse_vjp": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _u, _vp, _vp, _vp, _vp]),
+    "pmx_lpt1_lattice": (_i, [_vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp]),
+    "pmx_lpt1_vjp": (_i, [_vp, _vp, _vp, _i, _i, _i, _f, _f, _vp, _vp]),
+    "pmx_paint_healpix_tiled": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _dp, _dp, _d, _d, _i, _i, _i, _vp, _vp, _vp,
+                                     _i, _vp]),
+    "pmx_paint_healpix_vjp": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _dp, _dp, _d, _d, _i, _i, _i, _vp, _vp, _vp,
+                                   _vp, _i, _vp]),
+    "pmx_born": (_i, [_vp, _vp, _i, _c.c_int64, _vp, _i, _vp, _i, _vp]),
+    "pmx_born_vjp": (_i, [_vp, _vp, _i, _c.c_int64, _vp, _i, _vp, _vp, _i, _vp, _vp]),
 }
 
 DECONV = 1

@@ -146,39 +146,43 @@ def _lpt_factors(cosmo, a):
 class _LPT(torch.autograd.Function):
     @staticmethod
     def forward(ctx, ic, spec, *params):
-        cosmo, names, a = spec
+        cosmo, names, a, particles = spec
         D, vf = _lpt_factors(cosmo, a)
-        dx, p = engine.lpt1(ic.contiguous(), D, vf)
+        dx, p = engine.lpt1(ic.contiguous(), D, vf, particles=particles)
         ctx.save_for_backward(dx)
         ctx.spec, ctx.params, ctx.DV = spec, params, (D, vf)
+        ctx.mesh = tuple(ic.shape)
         ctx.ders = [fd(lambda c: _lpt_factors(c, a), cosmo, name) for name in names]  # overlaps the device
         return dx, p
 
     @staticmethod
     def backward(ctx, dx_bar, p_bar):
         (dx,) = ctx.saved_tensors
-        cosmo, names, a = ctx.spec
         D, vf = ctx.DV
-        dx_bar = torch.zeros_like(dx) if dx_bar is None else dx_bar.contiguous()
-        p_bar = torch.zeros_like(dx) if p_bar is None else p_bar.contiguous()
-        w = (dx_bar + vf * p_bar) * D
-        ic_bar = engine.force_mesh_vjp(w.contiguous(), scale=1.0) if ctx.needs_input_grad[0] else None
+        dx_bar = None if dx_bar is None else dx_bar.contiguous()
+        p_bar = None if p_bar is None else p_bar.contiguous()
+        ic_bar = None
+        if ctx.needs_input_grad[0] and (dx_bar is not None or p_bar is not None):
+            ic_bar = engine.lpt1_vjp(dx_bar, p_bar, ctx.mesh, D, vf)
         grads = []
-        if names:
+        if ctx.params:
             # dx = D F, p = vf D F:  dL/dD = <dx_bar + vf p_bar, dx> / D,  dL/dvf = <p_bar, dx>
-            gD = ((dx_bar + vf * p_bar) * dx).sum(dtype=torch.float64) / D
-            gv = (p_bar * dx).sum(dtype=torch.float64)
+            zero = torch.zeros((), dtype=torch.float64, device=dx.device)
+            gx = zero if dx_bar is None else (dx_bar * dx).sum(dtype=torch.float64)
+            gv = zero if p_bar is None else (p_bar * dx).sum(dtype=torch.float64)
+            gD = (gx + vf * gv) / D
             for (dD, dv), t in zip(ctx.ders, ctx.params):
                 grads.append(_param_grad(t, gD * dD + gv * dv))
         return (ic_bar, None, *grads)
 
 
-def lpt1(ic, cosmo_in, a):
+def lpt1(ic, cosmo_in, a, particles=None):
+    """(dx, p) of first-order LPT at the particle lattice `particles` (default: one per cell)."""
     cosmo, names, tensors = split_cosmo(cosmo_in)
     if not (ic.requires_grad or tensors):
         D, vf = _lpt_factors(cosmo, a)
-        return engine.lpt1(ic.contiguous(), D, vf)
-    return _LPT.apply(ic, (cosmo, names, a), *tensors)
+        return engine.lpt1(ic.contiguous(), D, vf, particles=particles)
+    return _LPT.apply(ic, (cosmo, names, a, particles), *tensors)
 
 
 # ---------------------------------------------------------------------------- N-body
@@ -202,19 +206,14 @@ class _NBodyReverse(torch.autograd.Function):
         run, coef_fn, cosmo, names = ctx.spec
         x, v = (t.clone() for t in ctx.saved_tensors)
         info = run.info
-        mesh, mode, target = info["mesh"], info["mode"], info["target"]
+        mesh, mode = info["mesh"], info["mode"]
         fs = float(info["force_scale"](cosmo))
         xb = torch.zeros_like(x)
         vb = torch.zeros_like(v)
         coef_bars = []
         for ev in reversed(info["events"]):
             if ev[0] == "save":
-                g = snaps_bar[ev[1]]
-                if target == "density":
-                    pb, _ = engine.paint_vjp(x, g.contiguous(), mode=mode)
-                    xb += pb
-                else:
-                    xb += g
+                xb += info["save_vjp"](ev[1], x, snaps_bar[ev[1]])
             else:
                 _, i, ck, cd = ev
                 cb = engine.step_reverse_vjp(x, v, xb, vb, mesh, mode=mode, force_scale=fs, c_kick=ck, c_drift=cd)

@@ -10,15 +10,30 @@
 // launchers from pmx_particles.hip
 int pmx_launch_fill(float* a, size_t n, float v, hipStream_t s);
 int pmx_launch_fill_d(double* a, size_t n, hipStream_t s);
-int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, const double box[3],
-                       const double obs[3], double rmin, double rmax, int nside, int scheme, float* acc,
-                       hipStream_t s);
 int pmx_launch_finalize_map(const float* acc, float* map, long n, double scale, int accumulate, hipStream_t s);
 int pmx_launch_flat(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, int fnx, int fny, float zlo, float zhi,
                     float* acc, hipStream_t s);
 int pmx_launch_philox_raw(uint32_t* out, size_t nblocks, uint64_t seed, hipStream_t s);
 int pmx_launch_white_noise(float* out, size_t n, uint64_t seed, hipStream_t s);
-int pmx_launch_lpt1(const float* F, float* dx, float* pv, size_t n, float D, float c, hipStream_t s);
+int pmx_launch_lpt1(pmx_plan_t* p, const float* F, float* dx, float* pv, int npx, int npy, int npz, float D, float c,
+                    hipStream_t s);
+int pmx_launch_lpt1_vjp(pmx_plan_t* p, const float* dxb, const float* pvb, int npx, int npy, int npz, float D, float c,
+                        float* G, hipStream_t s);
+// launchers from pmx_sky.hip
+int pmx_sky_max_tiles();
+int pmx_born_max_shells();
+int pmx_born_max_sources();
+int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, const double box[3],
+                       const double obs[3], double rmin, double rmax, int nside, int scheme, int ntiles,
+                       const int* rot, const double* shift, float* map, int accumulate, hipStream_t s);
+int pmx_launch_healpix_vjp(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, const double box[3],
+                           const double obs[3], double rmin, double rmax, int nside, int scheme, int ntiles,
+                           const int* rot, const double* shift, const float* map_bar, float* pos_bar, int accumulate,
+                           hipStream_t s);
+int pmx_launch_born(pmx_plan_t* p, const float* maps, int S, long npix, const double* C, int K, float* kappa,
+                    int accumulate, hipStream_t s);
+int pmx_launch_born_vjp(pmx_plan_t* p, const float* maps, int S, long npix, const double* C, int K,
+                        const float* kappa_bar, float* maps_bar, int accumulate, double* dots, hipStream_t s);
 
 namespace {
 thread_local std::string g_err;
@@ -596,31 +611,104 @@ int pmx_color_field(pmx_plan p, const float* field, const double* ktab, const do
 
 int pmx_lpt1(pmx_plan p, const float* ic, float growth, float vel_factor, float* dx, float* pv, void* stream) {
   PMX_TRY(check_plan(p));
-  if (!ic || !dx || !pv) return pmx_set_error(1, "null buffer");
+  return pmx_lpt1_lattice(p, ic, p->nx, p->ny, p->nz, growth, vel_factor, dx, pv, stream);
+}
+
+int pmx_lpt1_lattice(pmx_plan p, const float* ic, int npx, int npy, int npz, float growth, float vel_factor, float* dx,
+                     float* pv, void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, dx, npx, npy, npz, 0));
+  if (!ic || !pv) return pmx_set_error(1, "null buffer");
   hipStream_t s = (hipStream_t)stream;
   PMX_TRY(force_mesh_from(p, ic, 1.0f, 0u, p->fmesh, s));
-  return pmx_launch_lpt1(p->fmesh, dx, pv, 3 * p->ncell, growth, vel_factor, s);
+  return pmx_launch_lpt1(p, p->fmesh, dx, pv, npx, npy, npz, growth, vel_factor, s);
 }
+
+int pmx_lpt1_vjp(pmx_plan p, const float* dx_bar, const float* p_bar, int npx, int npy, int npz, float growth,
+                 float vel_factor, float* ic_bar, void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, dx_bar ? (const void*)dx_bar : (const void*)p_bar, npx, npy, npz, 0));
+  if (!ic_bar) return pmx_set_error(1, "null output");
+  hipStream_t s = (hipStream_t)stream;
+  PMX_TRY(pmx_launch_lpt1_vjp(p, dx_bar, p_bar, npx, npy, npz, growth, vel_factor, p->fmesh, s));
+  return pmx_force_mesh_vjp(p, p->fmesh, 1.0f, 0u, ic_bar, stream);
+}
+
+namespace {
+int check_shell(pmx_plan_t* p, const float* pos, int npx, int npy, int npz, int mode, const double* box,
+                const double* observer, double rmin, double rmax, int nside, int scheme, int ntiles, const int* rot,
+                const double* shift) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
+  if (!box || !observer) return pmx_set_error(1, "null argument");
+  if (nside < 1 || nside > 8192) return pmx_set_error(1, "nside out of range");
+  if (scheme != 0 && scheme != 1) return pmx_set_error(2, "scheme must be 0 (ngp) or 1 (bilinear)");
+  if (!(rmax > rmin) || rmin < 0) return pmx_set_error(1, "need 0 <= rmin < rmax");
+  if (rot && (ntiles < 1 || ntiles > pmx_sky_max_tiles() || !shift))
+    return pmx_set_error(1, "image table: need 1 <= ntiles <= " + std::to_string(pmx_sky_max_tiles()) +
+                                " and a shift table");
+  return 0;
+}
+}  // namespace
 
 int pmx_paint_healpix(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, const float* weights,
                       float weight, const double box[3], const double observer[3], double rmin, double rmax, int nside,
                       int scheme, float* map, int accumulate, void* stream) {
-  PMX_TRY(check_plan(p));
-  PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
-  if (!box || !observer || !map) return pmx_set_error(1, "null argument");
-  if (nside < 1 || nside > 8192) return pmx_set_error(1, "nside out of range");
-  if (scheme != 0 && scheme != 1) return pmx_set_error(2, "scheme must be 0 (ngp) or 1 (bilinear)");
-  if (!(rmax > rmin) || rmin < 0) return pmx_set_error(1, "need 0 <= rmin < rmax");
-  const long npix = 12L * nside * nside;
-  PMX_TRY(ensure_acc(p, (size_t)npix));
-  hipStream_t s = (hipStream_t)stream;
-  float* acc = reinterpret_cast<float*>(p->acc);
-  PMX_TRY(pmx_launch_fill(acc, npix, 0.f, s));
+  return pmx_paint_healpix_tiled(p, pos, npx, npy, npz, mode, weights, weight, box, observer, rmin, rmax, nside, scheme,
+                                 1, nullptr, nullptr, map, accumulate, stream);
+}
+
+int pmx_paint_healpix_tiled(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, const float* weights,
+                            float weight, const double box[3], const double observer[3], double rmin, double rmax,
+                            int nside, int scheme, int ntiles, const int* tile_rot, const double* tile_shift,
+                            float* map, int accumulate, void* stream) {
+  PMX_TRY(check_shell(p, pos, npx, npy, npz, mode, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
+                      tile_shift));
+  if (!map) return pmx_set_error(1, "null map");
+  PMX_TRY(ensure_acc(p, (size_t)12 * nside * nside));
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
-  PMX_TRY(pmx_launch_healpix(p, a, weights, weight, box, observer, rmin, rmax, nside, scheme, acc, s));
-  const double omega = 4.0 * M_PI / (double)npix;
-  const double vol = omega * (rmax * rmax * rmax - rmin * rmin * rmin) / 3.0;
-  return pmx_launch_finalize_map(acc, map, npix, 1.0 / vol, accumulate, s);
+  return pmx_launch_healpix(p, a, weights, weight, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
+                            tile_shift, map, accumulate, (hipStream_t)stream);
+}
+
+int pmx_paint_healpix_vjp(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, const float* weights,
+                          float weight, const double box[3], const double observer[3], double rmin, double rmax,
+                          int nside, int scheme, int ntiles, const int* tile_rot, const double* tile_shift,
+                          const float* map_bar, float* pos_bar, int accumulate, void* stream) {
+  PMX_TRY(check_shell(p, pos, npx, npy, npz, mode, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
+                      tile_shift));
+  if (!map_bar || !pos_bar) return pmx_set_error(1, "null cotangent");
+  PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
+  return pmx_launch_healpix_vjp(p, a, weights, weight, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
+                                tile_shift, map_bar, pos_bar, accumulate, (hipStream_t)stream);
+}
+
+namespace {
+int check_born(pmx_plan_t* p, const float* maps, int nshell, int64_t npix, const double* coef, int nsrc) {
+  PMX_TRY(check_plan(p));
+  if (!maps || !coef) return pmx_set_error(1, "null buffer");
+  if (nshell < 1 || nshell > pmx_born_max_shells())
+    return pmx_set_error(1, "need 1 <= nshell <= " + std::to_string(pmx_born_max_shells()));
+  if (nsrc < 1 || nsrc > pmx_born_max_sources())
+    return pmx_set_error(1, "need 1 <= nsrc <= " + std::to_string(pmx_born_max_sources()));
+  if (npix < 1) return pmx_set_error(1, "npix must be positive");
+  return 0;
+}
+}  // namespace
+
+int pmx_born(pmx_plan p, const float* maps, int nshell, int64_t npix, const double* coef, int nsrc, float* kappa,
+             int accumulate, void* stream) {
+  PMX_TRY(check_born(p, maps, nshell, npix, coef, nsrc));
+  if (!kappa) return pmx_set_error(1, "null output");
+  return pmx_launch_born(p, maps, nshell, (long)npix, coef, nsrc, kappa, accumulate, (hipStream_t)stream);
+}
+
+int pmx_born_vjp(pmx_plan p, const float* maps, int nshell, int64_t npix, const double* coef, int nsrc,
+                 const float* kappa_bar, float* maps_bar, int accumulate, double* dots, void* stream) {
+  PMX_TRY(check_born(p, maps, nshell, npix, coef, nsrc));
+  if (!kappa_bar || !maps_bar) return pmx_set_error(1, "null cotangent");
+  return pmx_launch_born_vjp(p, maps, nshell, (long)npix, coef, nsrc, kappa_bar, maps_bar, accumulate, dots,
+                             (hipStream_t)stream);
 }
 
 int pmx_paint_flat(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, const float* weights,

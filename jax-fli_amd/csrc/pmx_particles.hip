@@ -1,6 +1,6 @@
 // Particle-side kernels: CIC paint (owner-computes tiles + escape pass, or one-pass atomic scatter),
-// CIC readout fused with the kick and the next drift, HEALPix and flat-sky shell painting,
-// counter-based white noise.
+// CIC readout fused with the kick and the next drift, flat-sky slab painting, the adjoint gathers,
+// counter-based white noise. HEALPix painting and Born lensing are in pmx_sky.hip.
 //
 // Every particle kernel walks the particle grid in its Lagrangian (i, j, k) order, so a wavefront
 // touches a compact patch of the mesh and the 8-corner gathers/scatters stay in L2. Index math is
@@ -9,6 +9,7 @@
 // FP contraction is disabled here so x + v*c and the CIC weights round exactly like the
 // unfused float32 expressions of the reference (two roundings, no FMA).
 #include "pmx_internal.h"
+#include "pmx_device.h"
 #include <cmath>
 
 #pragma clang fp contract(off)
@@ -105,14 +106,6 @@ __device__ __forceinline__ Cic particle_cic(const PartArgs& a, const LatticeWalk
   return cic_from(corner_abs(x.x, NX), corner_abs(x.y, NY), corner_abs(x.z, NZ), NY, NZ);
 }
 
-__device__ __forceinline__ float3 ld3(const float* a, unsigned p) {
-  return make_float3(a[3 * (size_t)p], a[3 * (size_t)p + 1], a[3 * (size_t)p + 2]);
-}
-__device__ __forceinline__ void st3(float* a, unsigned p, float3 v) {
-  a[3 * (size_t)p] = v.x;
-  a[3 * (size_t)p + 1] = v.y;
-  a[3 * (size_t)p + 2] = v.z;
-}
 __device__ __forceinline__ float3 axpy3(float3 x, float3 v, float c) {  // x + v*c, two roundings
   return make_float3(x.x + v.x * c, x.y + v.y * c, x.z + v.z * c);
 }
@@ -123,9 +116,6 @@ __device__ __forceinline__ float3 axpy3(float3 x, float3 v, float c) {  // x + v
 struct Chunk {
   unsigned begin, end;
 };
-__device__ __forceinline__ unsigned xcd_logical(unsigned b, unsigned G) {
-  return (G % 8u == 0u) ? (b % 8u) * (G / 8u) + b / 8u : b;
-}
 __device__ __forceinline__ Chunk my_chunk(unsigned n) {
   const unsigned G = gridDim.x;
   const unsigned per = (n + G - 1) / G;
@@ -696,16 +686,7 @@ __global__ void __launch_bounds__(256) k_div4(const float* __restrict__ U0, cons
 // block sum of a double into part[blockIdx.x] (k_finish_sum adds the partials in a fixed order:
 // deterministic, and no same-address atomics from thousands of workgroups)
 __device__ __forceinline__ void block_sum_store(double v, double* part_out) {
-  __shared__ double part[8];
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  const int wv = threadIdx.x >> 6;
-  if (__lane_id() == 0) part[wv] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += part[i];
-    part_out[blockIdx.x] = t;
-  }
+  block_sum_to(v, part_out + blockIdx.x);
 }
 
 // *dst += sum of part[0..n) (one workgroup of 256)
@@ -877,230 +858,6 @@ __global__ void __launch_bounds__(256) k_readout(PartArgs a, const float* __rest
   }
 }
 
-// ---------------------------------------------------------------- HEALPix (RING scheme)
-// Restates the published HEALPix pixelisation (Gorski et al. 2005) and its bilinear ring
-// interpolation: loc2pix, ring_above, get_ring_info2, get_interpol of the HEALPix C++ library.
-struct HpGeom {
-  int nside;
-  long npix, ncap;
-  double fact1, fact2;
-};
-
-__device__ __forceinline__ long hp_ang2pix(const HpGeom& g, double z, double phi) {
-  const double za = fabs(z);
-  const double inv_halfpi = 0.63661977236758134308;
-  double tt = fmod(phi * inv_halfpi, 4.0);
-  if (tt < 0) tt += 4.0;
-  const long ns = g.nside;
-  if (za <= 2.0 / 3.0) {
-    const long nl4 = 4 * ns;
-    const double t1 = ns * (0.5 + tt), t2 = ns * z * 0.75;
-    const long jp = (long)(t1 - t2), jm = (long)(t1 + t2);
-    const long ir = ns + 1 + jp - jm;
-    const long kshift = 1 - (ir & 1);
-    const long t = jp + jm - ns + kshift + 1 + nl4 + nl4;
-    const long ip = (t >> 1) % nl4;
-    return g.ncap + (ir - 1) * nl4 + ip;
-  }
-  const double tp = tt - (long)tt;
-  const double tmp = ns * sqrt(3.0 * (1.0 - za));
-  const long jp = (long)(tp * tmp), jm = (long)((1.0 - tp) * tmp);
-  const long ir = jp + jm + 1;
-  long ip = (long)(tt * ir);
-  ip = ip % (4 * ir);
-  if (ip < 0) ip += 4 * ir;
-  return (z > 0) ? 2 * ir * (ir - 1) + ip : g.npix - 2 * ir * (ir + 1) + ip;
-}
-
-__device__ __forceinline__ long hp_ring_above(const HpGeom& g, double z) {
-  const double az = fabs(z);
-  if (az <= 2.0 / 3.0) return (long)(g.nside * (2.0 - 1.5 * z));
-  const long iring = (long)(g.nside * sqrt(3.0 * (1.0 - az)));
-  return (z > 0) ? iring : 4 * (long)g.nside - iring - 1;
-}
-
-__device__ __forceinline__ void hp_ring_info(const HpGeom& g, long ring, long& startpix, long& ringpix, double& theta,
-                                             bool& shifted) {
-  const long ns = g.nside;
-  const long northring = (ring > 2 * ns) ? 4 * ns - ring : ring;
-  if (northring < ns) {
-    const double tmp = (double)northring * northring * g.fact2;
-    const double costheta = 1.0 - tmp;
-    const double sintheta = sqrt(tmp * (2.0 - tmp));
-    theta = atan2(sintheta, costheta);
-    ringpix = 4 * northring;
-    shifted = true;
-    startpix = 2 * northring * (northring - 1);
-  } else {
-    theta = acos((2 * ns - northring) * g.fact1);
-    ringpix = 4 * ns;
-    shifted = ((northring - ns) & 1) == 0;
-    startpix = g.ncap + (northring - ns) * ringpix;
-  }
-  if (northring != ring) {
-    theta = M_PI - theta;
-    startpix = g.npix - startpix - ringpix;
-  }
-}
-
-__device__ __forceinline__ void hp_interpol(const HpGeom& g, double theta, double phi, long pix[4], double wgt[4]) {
-  const double z = cos(theta);
-  const long ir1 = hp_ring_above(g, z);
-  const long ir2 = ir1 + 1;
-  const double twopi = 2.0 * M_PI;
-  double theta1 = 0, theta2 = 0;
-  long sp, nr;
-  bool shift;
-  if (ir1 > 0) {
-    hp_ring_info(g, ir1, sp, nr, theta1, shift);
-    const double dphi = twopi / nr;
-    const double tmp = phi / dphi - 0.5 * shift;
-    long i1 = (tmp < 0) ? (long)tmp - 1 : (long)tmp;
-    const double w1 = (phi - (i1 + 0.5 * shift) * dphi) / dphi;
-    long i2 = i1 + 1;
-    if (i1 < 0) i1 += nr;
-    if (i2 >= nr) i2 -= nr;
-    pix[0] = sp + i1;
-    pix[1] = sp + i2;
-    wgt[0] = 1 - w1;
-    wgt[1] = w1;
-  }
-  if (ir2 < 4 * (long)g.nside) {
-    hp_ring_info(g, ir2, sp, nr, theta2, shift);
-    const double dphi = twopi / nr;
-    const double tmp = phi / dphi - 0.5 * shift;
-    long i1 = (tmp < 0) ? (long)tmp - 1 : (long)tmp;
-    const double w1 = (phi - (i1 + 0.5 * shift) * dphi) / dphi;
-    long i2 = i1 + 1;
-    if (i1 < 0) i1 += nr;
-    if (i2 >= nr) i2 -= nr;
-    pix[2] = sp + i1;
-    pix[3] = sp + i2;
-    wgt[2] = 1 - w1;
-    wgt[3] = w1;
-  }
-  if (ir1 == 0) {
-    const double wtheta = theta / theta2;
-    wgt[2] *= wtheta;
-    wgt[3] *= wtheta;
-    const double fac = (1 - wtheta) * 0.25;
-    wgt[0] = fac;
-    wgt[1] = fac;
-    wgt[2] += fac;
-    wgt[3] += fac;
-    pix[0] = (pix[2] + 2) & 3;
-    pix[1] = (pix[3] + 2) & 3;
-  } else if (ir2 == 4 * (long)g.nside) {
-    const double wtheta = (theta - theta1) / (M_PI - theta1);
-    wgt[0] *= (1 - wtheta);
-    wgt[1] *= (1 - wtheta);
-    const double fac = wtheta * 0.25;
-    wgt[0] += fac;
-    wgt[1] += fac;
-    wgt[2] = fac;
-    wgt[3] = fac;
-    pix[2] = ((pix[0] + 2) & 3) + g.npix - 4;
-    pix[3] = ((pix[1] + 2) & 3) + g.npix - 4;
-  } else {
-    const double wtheta = (theta - theta1) / (theta2 - theta1);
-    wgt[0] *= (1 - wtheta);
-    wgt[1] *= (1 - wtheta);
-    wgt[2] *= wtheta;
-    wgt[3] *= wtheta;
-  }
-}
-
-
-struct ShellArgs {
-  double box[3];    // Mpc/h
-  double obs[3];    // observer, Mpc/h
-  double rmin, rmax;
-  int scheme;       // 0 ngp, 1 bilinear
-};
-
-// particle position in grid units, lattice added in relative mode, wrapped into the periodic box
-__device__ __forceinline__ double grid_pos(int mode, int lat, float xc, int n) {
-  double g = (double)xc + (mode == 0 ? (double)lat : 0.0);
-  return g - floor(g / n) * n;
-}
-
-// Shell paint with per-workgroup aggregation: a workgroup takes a 16^3 Lagrangian sub-block of
-// particles (a compact patch of the sky), accumulates its (pixel, weight) deposits in an LDS hash
-// table, then adds each distinct pixel to the map with one global atomic. Device-wide float
-// atomics are the bottleneck of the one-atomic-per-deposit form; particles on the same line of
-// sight in the patch share pixels, which the table merges. A deposit that finds no free slot
-// within HP_PROBES probes goes straight to the map.
-constexpr int HP_HASH = 4096, HP_PROBES = 32, HP_SB = 16;
-constexpr unsigned HP_EMPTY = 0xFFFFFFFFu;
-
-__device__ __forceinline__ void hp_table_add(volatile unsigned* keys, float* vals, unsigned pix, float v,
-                                             float* __restrict__ acc) {
-  unsigned h = (pix * 2654435761u) >> 20;  // 12-bit multiplicative hash
-#pragma unroll 1
-  for (int t = 0; t < HP_PROBES; ++t) {
-    const unsigned k = keys[h];
-    if (k == pix) {
-      atomicAdd(&vals[h], v);
-      return;
-    }
-    if (k == HP_EMPTY) {
-      const unsigned old = atomicCAS(const_cast<unsigned*>(keys + h), HP_EMPTY, pix);
-      if (old == HP_EMPTY || old == pix) {
-        atomicAdd(&vals[h], v);
-        return;
-      }
-    }
-    h = (h + 1) & (HP_HASH - 1);
-  }
-  unsafeAtomicAdd(acc + pix, v);
-}
-
-__global__ void __launch_bounds__(256) k_paint_healpix(PartArgs a, const float* __restrict__ w, float w0, ShellArgs sa,
-                                                       HpGeom g, float* __restrict__ acc, int NX, int NY, int NZ,
-                                                       int rx, int ry, int rz, int nbx, int nby, int nbz) {
-  __shared__ unsigned keys[HP_HASH];
-  __shared__ float vals[HP_HASH];
-  for (int i = threadIdx.x; i < HP_HASH; i += blockDim.x) {
-    keys[i] = HP_EMPTY;
-    vals[i] = 0.f;
-  }
-  const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
-  const int bk = (int)(b % (unsigned)nbz), bj = (int)((b / (unsigned)nbz) % (unsigned)nby),
-            bi = (int)(b / ((unsigned)nbz * nby));
-  __syncthreads();
-  for (int r = 0; r < HP_SB * HP_SB * HP_SB / 256; ++r) {
-    const int l = r * 256 + (int)threadIdx.x;
-    const int i = bi * HP_SB + (l >> 8), j = bj * HP_SB + ((l >> 4) & 15), k = bk * HP_SB + (l & 15);
-    if (i >= a.npx || j >= a.npy || k >= a.npz) continue;
-    const unsigned p = ((unsigned)i * a.npy + j) * a.npz + k;
-    const float3 x = ld3(a.x, p);
-    const double X = grid_pos(a.mode, i * rx, x.x, NX) * sa.box[0] / NX - sa.obs[0];
-    const double Y = grid_pos(a.mode, j * ry, x.y, NY) * sa.box[1] / NY - sa.obs[1];
-    const double Z = grid_pos(a.mode, k * rz, x.z, NZ) * sa.box[2] / NZ - sa.obs[2];
-    const double rr = sqrt(X * X + Y * Y + Z * Z);
-    if (!(rr > sa.rmin && rr <= sa.rmax)) continue;
-    const double wt = w ? (double)w[p] : (double)w0;
-    const double phi0 = atan2(Y, X);
-    double phi = phi0 < 0 ? phi0 + 2.0 * M_PI : phi0;
-    if (!(phi < 2.0 * M_PI)) phi = 0.0;  // -tiny + 2 pi can round up to 2 pi
-    if (sa.scheme == 0) {
-      hp_table_add(keys, vals, (unsigned)hp_ang2pix(g, Z / rr, phi), (float)wt, acc);
-    } else {
-      const double theta = atan2(sqrt(X * X + Y * Y), Z);
-      long pix[4];
-      double wg[4];
-      hp_interpol(g, theta, phi, pix, wg);
-#pragma unroll
-      for (int m = 0; m < 4; ++m) hp_table_add(keys, vals, (unsigned)pix[m], (float)(wt * wg[m]), acc);
-    }
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < HP_HASH; i += blockDim.x) {
-    const unsigned k = keys[i];
-    if (k != HP_EMPTY) unsafeAtomicAdd(acc + k, vals[i]);
-  }
-}
-
 __global__ void k_finalize_map(const float* __restrict__ acc, float* __restrict__ map, long n, double scale,
                                int accumulate) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
@@ -1187,20 +944,42 @@ __global__ void k_white_noise(float* __restrict__ out, size_t n, uint32_t k0, ui
 }
 
 // dx = D * F ; p = c * dx  (first-order LPT at the lattice, F the unit-scale force mesh)
-__global__ void k_lpt1(const float* __restrict__ F, float* __restrict__ dx, float* __restrict__ pv, size_t n, float D,
-                       float c) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const float d = F[i] * D;
-    dx[i] = d;
-    pv[i] = d * c;
+// dx = D F at the particle lattice (cell (i rx, j ry, k rz) of the force mesh), pv = c dx
+__global__ void k_lpt1(const float* __restrict__ F, float* __restrict__ dx, float* __restrict__ pv, int npx, int npy,
+                       int npz, int rx, int ry, int rz, int NY, int NZ, float D, float c) {
+  const size_t n = (size_t)npx * npy * npz;
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+    const unsigned k = (unsigned)(p % (unsigned)npz), j = (unsigned)((p / (unsigned)npz) % (unsigned)npy),
+                   i = (unsigned)(p / ((size_t)npz * npy));
+    const size_t cell = ((size_t)i * rx * NY + (size_t)j * ry) * NZ + (size_t)k * rz;
+    for (int d = 0; d < 3; ++d) {
+      const float v = F[3 * cell + d] * D;
+      dx[3 * p + d] = v;
+      pv[3 * p + d] = v * c;
+    }
   }
 }
 
-inline int grid_for(size_t n, int block) {
-  size_t g = (n + block - 1) / block;
-  const size_t gmax = 1u << 20;
-  return (int)(g < 1 ? 1 : (g > gmax ? gmax : g));
+// transpose of k_lpt1 onto the force mesh: G[cell] = D (dx_bar + c pv_bar) of the particle on that
+// lattice cell, 0 on the cells between lattice sites (every cell is written)
+__global__ void k_lpt1_vjp(const float* __restrict__ dxb, const float* __restrict__ pvb, float* __restrict__ G,
+                           int NX, int NY, int NZ, int rx, int ry, int rz, int npy, int npz, float D, float c) {
+  const size_t n = (size_t)NX * NY * NZ;
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    const unsigned z = (unsigned)(q % (unsigned)NZ), y = (unsigned)((q / (unsigned)NZ) % (unsigned)NY),
+                   x = (unsigned)(q / ((size_t)NZ * NY));
+    float g[3] = {0.f, 0.f, 0.f};
+    if (x % rx == 0 && y % ry == 0 && z % rz == 0) {
+      const size_t p = ((size_t)(x / rx) * npy + y / ry) * npz + z / rz;
+      for (int d = 0; d < 3; ++d) {
+        const float a = dxb ? dxb[3 * p + d] : 0.f, b = pvb ? pvb[3 * p + d] : 0.f;
+        g[d] = D * (a + c * b);
+      }
+    }
+    for (int d = 0; d < 3; ++d) G[3 * q + d] = g[d];
+  }
 }
+
 // particle-chunk grids: a multiple of 8 workgroups (one slab per XCD), ~1-4 particles per thread
 inline int grid_particles(size_t n) {
   size_t g = (n + 255) / 256;
@@ -1398,31 +1177,6 @@ int pmx_launch_readout(pmx_plan_t* p, const PartArgs& a, const float* mesh, int 
   return pmx_check_launch("readout");
 }
 
-int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, const double box[3],
-                       const double obs[3], double rmin, double rmax, int nside, int scheme, float* acc,
-                       hipStream_t s) {
-  int rx, ry, rz;
-  ratios(p, a, rx, ry, rz);
-  ShellArgs sa;
-  for (int c = 0; c < 3; ++c) {
-    sa.box[c] = box[c];
-    sa.obs[c] = obs[c];
-  }
-  sa.rmin = rmin;
-  sa.rmax = rmax;
-  sa.scheme = scheme;
-  HpGeom g;
-  g.nside = nside;
-  g.npix = 12L * nside * nside;
-  g.ncap = 2L * nside * (nside - 1);
-  g.fact2 = 4.0 / (double)g.npix;
-  g.fact1 = (double)(2 * nside) * g.fact2;
-  const int nbx = (a.npx + HP_SB - 1) / HP_SB, nby = (a.npy + HP_SB - 1) / HP_SB, nbz = (a.npz + HP_SB - 1) / HP_SB;
-  hipLaunchKernelGGL(k_paint_healpix, dim3((unsigned)nbx * nby * nbz), dim3(256), 0, s, a, w, w0, sa, g, acc, p->nx,
-                     p->ny, p->nz, rx, ry, rz, nbx, nby, nbz);
-  return pmx_check_launch("paint_healpix");
-}
-
 int pmx_launch_finalize_map(const float* acc, float* map, long n, double scale, int accumulate, hipStream_t s) {
   hipLaunchKernelGGL(k_finalize_map, dim3(grid_for(n, 256)), dim3(256), 0, s, acc, map, n, scale, accumulate);
   return pmx_check_launch("finalize_map");
@@ -1456,9 +1210,18 @@ int pmx_launch_white_noise(float* out, size_t n, uint64_t seed, hipStream_t s) {
   return pmx_check_launch("white_noise");
 }
 
-int pmx_launch_lpt1(const float* F, float* dx, float* pv, size_t n, float D, float c, hipStream_t s) {
-  hipLaunchKernelGGL(k_lpt1, dim3(grid_for(n, 256)), dim3(256), 0, s, F, dx, pv, n, D, c);
+int pmx_launch_lpt1(pmx_plan_t* p, const float* F, float* dx, float* pv, int npx, int npy, int npz, float D, float c,
+                    hipStream_t s) {
+  const size_t n = (size_t)npx * npy * npz;
+  hipLaunchKernelGGL(k_lpt1, dim3(grid_for(n, 256)), dim3(256), 0, s, F, dx, pv, npx, npy, npz, p->nx / npx,
+                     p->ny / npy, p->nz / npz, p->ny, p->nz, D, c);
   return pmx_check_launch("lpt1");
+}
+int pmx_launch_lpt1_vjp(pmx_plan_t* p, const float* dxb, const float* pvb, int npx, int npy, int npz, float D, float c,
+                        float* G, hipStream_t s) {
+  hipLaunchKernelGGL(k_lpt1_vjp, dim3(grid_for(p->ncell, 256)), dim3(256), 0, s, dxb, pvb, G, p->nx, p->ny, p->nz,
+                     p->nx / npx, p->ny / npy, p->nz / npz, npy, npz, D, c);
+  return pmx_check_launch("lpt1_vjp");
 }
 
 // ------------------------------------------------------------------ adjoint launchers

@@ -1,0 +1,745 @@
+// Sky projections: HEALPix shell painting over a table of box images (periodic replicas, optionally
+// rotated by cube symmetries), its position adjoint, and Born convergence with its adjoint.
+//
+// The HEALPix kernels take a 16^3 Lagrangian sub-block of particles per workgroup. With more than
+// one image, the workgroup first bounds its particles (observer-centred, Mpc/h) and keeps only the
+// images whose copy of that box meets the shell, in table order, so a particle is tested against a
+// handful of images instead of all of them. Pixel geometry is float64 throughout.
+#include "pmx_internal.h"
+#include "pmx_device.h"
+#include <cfloat>
+#include <cmath>
+
+#define PMX_CHECK_LAUNCH(what)             \
+  do {                                     \
+    const int rc_ = pmx_check_launch(what); \
+    if (rc_) return rc_;                   \
+  } while (0)
+
+namespace {
+
+// ---------------------------------------------------------------- HEALPix (RING scheme)
+// Restates the published HEALPix pixelisation (Gorski et al. 2005) and its bilinear ring
+// interpolation: loc2pix, ring_above, get_ring_info2 and get_interpol of the HEALPix C++ library.
+struct HpGeom {
+  int nside;
+  long npix, ncap;
+  double fact1, fact2;
+};
+
+__device__ __forceinline__ long hp_ang2pix(const HpGeom& g, double z, double phi) {
+  const double za = fabs(z);
+  const double inv_halfpi = 0.63661977236758134308;
+  double tt = fmod(phi * inv_halfpi, 4.0);
+  if (tt < 0) tt += 4.0;
+  const long ns = g.nside;
+  if (za <= 2.0 / 3.0) {
+    const long nl4 = 4 * ns;
+    const double t1 = ns * (0.5 + tt), t2 = ns * z * 0.75;
+    const long jp = (long)(t1 - t2), jm = (long)(t1 + t2);
+    const long ir = ns + 1 + jp - jm;
+    const long kshift = 1 - (ir & 1);
+    const long t = jp + jm - ns + kshift + 1 + nl4 + nl4;
+    const long ip = (t >> 1) % nl4;
+    return g.ncap + (ir - 1) * nl4 + ip;
+  }
+  const double tp = tt - (long)tt;
+  const double tmp = ns * sqrt(3.0 * (1.0 - za));
+  const long jp = (long)(tp * tmp), jm = (long)((1.0 - tp) * tmp);
+  const long ir = jp + jm + 1;
+  long ip = (long)(tt * ir);
+  ip = ip % (4 * ir);
+  if (ip < 0) ip += 4 * ir;
+  return (z > 0) ? 2 * ir * (ir - 1) + ip : g.npix - 2 * ir * (ir + 1) + ip;
+}
+
+__device__ __forceinline__ long hp_ring_above(const HpGeom& g, double z) {
+  const double az = fabs(z);
+  if (az <= 2.0 / 3.0) return (long)(g.nside * (2.0 - 1.5 * z));
+  const long iring = (long)(g.nside * sqrt(3.0 * (1.0 - az)));
+  return (z > 0) ? iring : 4 * (long)g.nside - iring - 1;
+}
+
+__device__ __forceinline__ void hp_ring_info(const HpGeom& g, long ring, long& startpix, long& ringpix, double& theta,
+                                             bool& shifted) {
+  const long ns = g.nside;
+  const long northring = (ring > 2 * ns) ? 4 * ns - ring : ring;
+  if (northring < ns) {
+    const double tmp = (double)northring * northring * g.fact2;
+    const double costheta = 1.0 - tmp;
+    const double sintheta = sqrt(tmp * (2.0 - tmp));
+    theta = atan2(sintheta, costheta);
+    ringpix = 4 * northring;
+    shifted = true;
+    startpix = 2 * northring * (northring - 1);
+  } else {
+    theta = acos((2 * ns - northring) * g.fact1);
+    ringpix = 4 * ns;
+    shifted = ((northring - ns) & 1) == 0;
+    startpix = g.ncap + (northring - ns) * ringpix;
+  }
+  if (northring != ring) {
+    theta = M_PI - theta;
+    startpix = g.npix - startpix - ringpix;
+  }
+}
+
+// pixels on one ring around phi and the phi weight of the second one; dphi = ring pixel width
+__device__ __forceinline__ void hp_ring_pair(const HpGeom& g, long ring, double phi, long& p1, long& p2, double& w1,
+                                             double& theta, double& dphi) {
+  long sp, nr;
+  bool shift;
+  hp_ring_info(g, ring, sp, nr, theta, shift);
+  dphi = 2.0 * M_PI / nr;
+  const double tmp = phi / dphi - 0.5 * shift;
+  long i1 = (tmp < 0) ? (long)tmp - 1 : (long)tmp;
+  w1 = (phi - (i1 + 0.5 * shift) * dphi) / dphi;
+  long i2 = i1 + 1;
+  if (i1 < 0) i1 += nr;
+  if (i2 >= nr) i2 -= nr;
+  p1 = sp + i1;
+  p2 = sp + i2;
+}
+
+// Bilinear weights of the four RING pixels around (theta, phi); DER also gives d wgt / d theta and
+// d wgt / d phi (the integer ring/pixel choices are locally constant).
+template <bool DER>
+__device__ __forceinline__ void hp_interpol(const HpGeom& g, double theta, double phi, long pix[4], double wgt[4],
+                                            double dth[4], double dph[4]) {
+  const double z = cos(theta);
+  const long ir1 = hp_ring_above(g, z);
+  const long ir2 = ir1 + 1;
+  const long nring = 4 * (long)g.nside;
+  double theta1 = 0, theta2 = 0, dphi;
+  double w1;
+  if (ir1 > 0) {
+    hp_ring_pair(g, ir1, phi, pix[0], pix[1], w1, theta1, dphi);
+    wgt[0] = 1 - w1;
+    wgt[1] = w1;
+    if (DER) {
+      dph[0] = -1.0 / dphi;
+      dph[1] = 1.0 / dphi;
+    }
+  }
+  if (ir2 < nring) {
+    hp_ring_pair(g, ir2, phi, pix[2], pix[3], w1, theta2, dphi);
+    wgt[2] = 1 - w1;
+    wgt[3] = w1;
+    if (DER) {
+      dph[2] = -1.0 / dphi;
+      dph[3] = 1.0 / dphi;
+    }
+  }
+  if (ir1 == 0) {  // north polar cap: the missing ring is the 4 pixels around the pole
+    const double wtheta = theta / theta2, dw = 1.0 / theta2;
+    if (DER) {
+      dth[0] = dth[1] = -0.25 * dw;
+      dth[2] = (wgt[2] - 0.25) * dw;
+      dth[3] = (wgt[3] - 0.25) * dw;
+      dph[0] = dph[1] = 0.0;
+      dph[2] *= wtheta;
+      dph[3] *= wtheta;
+    }
+    wgt[2] *= wtheta;
+    wgt[3] *= wtheta;
+    const double fac = (1 - wtheta) * 0.25;
+    wgt[0] = fac;
+    wgt[1] = fac;
+    wgt[2] += fac;
+    wgt[3] += fac;
+    pix[0] = (pix[2] + 2) & 3;
+    pix[1] = (pix[3] + 2) & 3;
+  } else if (ir2 == nring) {  // south polar cap
+    const double wtheta = (theta - theta1) / (M_PI - theta1), dw = 1.0 / (M_PI - theta1);
+    if (DER) {
+      dth[0] = (0.25 - wgt[0]) * dw;
+      dth[1] = (0.25 - wgt[1]) * dw;
+      dth[2] = dth[3] = 0.25 * dw;
+      dph[0] *= (1 - wtheta);
+      dph[1] *= (1 - wtheta);
+      dph[2] = dph[3] = 0.0;
+    }
+    wgt[0] *= (1 - wtheta);
+    wgt[1] *= (1 - wtheta);
+    const double fac = wtheta * 0.25;
+    wgt[0] += fac;
+    wgt[1] += fac;
+    wgt[2] = fac;
+    wgt[3] = fac;
+    pix[2] = ((pix[0] + 2) & 3) + g.npix - 4;
+    pix[3] = ((pix[1] + 2) & 3) + g.npix - 4;
+  } else {
+    const double wtheta = (theta - theta1) / (theta2 - theta1), dw = 1.0 / (theta2 - theta1);
+    if (DER) {
+      dth[0] = -wgt[0] * dw;
+      dth[1] = -wgt[1] * dw;
+      dth[2] = wgt[2] * dw;
+      dth[3] = wgt[3] * dw;
+      dph[0] *= (1 - wtheta);
+      dph[1] *= (1 - wtheta);
+      dph[2] *= wtheta;
+      dph[3] *= wtheta;
+    }
+    wgt[0] *= (1 - wtheta);
+    wgt[1] *= (1 - wtheta);
+    wgt[2] *= wtheta;
+    wgt[3] *= wtheta;
+  }
+}
+
+// ---------------------------------------------------------------- box images
+// Image t maps an observer-centred position X to R_t X + shift_t. R_t is the cube symmetry with
+// index rot[t] in the enumeration "for signs in product([1, -1], repeat=3): for perm in
+// permutations((0, 1, 2))" (index = 6 * sign_index + perm_index; 0 is the identity):
+// (R X)_i = sign_i * X_{perm_i}. This is the table of get_all_47_symmetries
+// (src/jax_fli/pm/interp.py:31-43), whose entries 1..47 the reference uses.
+struct Sym {
+  int p[3];
+  double s[3];
+};
+__device__ __forceinline__ Sym sym_of(int code) {
+  Sym r;
+  code = ((code % 48) + 48) % 48;  // the table has 48 entries
+  const int pi = code % 6, si = code / 6;
+  const int p0 = pi >> 1;
+  const int a = (p0 == 0) ? 1 : 0, b = (p0 == 2) ? 1 : 2;  // the two remaining axes, a < b
+  r.p[0] = p0;
+  r.p[1] = (pi & 1) ? b : a;
+  r.p[2] = (pi & 1) ? a : b;
+  r.s[0] = (si & 4) ? -1.0 : 1.0;
+  r.s[1] = (si & 2) ? -1.0 : 1.0;
+  r.s[2] = (si & 1) ? -1.0 : 1.0;
+  return r;
+}
+
+struct ShellArgs {
+  double cell[3];   // Mpc/h per grid unit
+  double obs[3];    // observer, Mpc/h
+  double rmin, rmax;
+  int scheme;       // 0 ngp, 1 bilinear
+  int NX, NY, NZ;
+  int rx, ry, rz;   // lattice spacing of the particle grid in cells
+  int nbx, nby, nbz;
+  int ntiles;       // image count; rot == nullptr and ntiles == 1: the box itself
+  const int* rot;
+  const double* shift;  // 3 per image, Mpc/h
+};
+
+constexpr int HP_HASH = 4096, HP_PROBES = 32, HP_SB = 16, HP_PER_THREAD = HP_SB * HP_SB * HP_SB / 256;
+constexpr int HP_MAX_TILES = 4096;
+constexpr unsigned HP_EMPTY = 0xFFFFFFFFu;
+
+// particle position in grid units, lattice added in relative mode, wrapped into the periodic box
+__device__ __forceinline__ double grid_pos(int mode, int lat, float xc, int n) {
+  double g = (double)xc + (mode == 0 ? (double)lat : 0.0);
+  return g - floor(g / n) * n;
+}
+
+// the r-th particle of this thread in the workgroup's sub-block: flat index, or -1 past the grid
+__device__ __forceinline__ long sb_particle(const PartArgs& a, const ShellArgs& sa, unsigned b, int r, int& i, int& j,
+                                            int& k) {
+  const int bk = (int)(b % (unsigned)sa.nbz), bj = (int)((b / (unsigned)sa.nbz) % (unsigned)sa.nby),
+            bi = (int)(b / ((unsigned)sa.nbz * sa.nby));
+  const int l = r * 256 + (int)threadIdx.x;
+  i = bi * HP_SB + (l >> 8);
+  j = bj * HP_SB + ((l >> 4) & 15);
+  k = bk * HP_SB + (l & 15);
+  if (i >= a.npx || j >= a.npy || k >= a.npz) return -1;
+  return ((long)i * a.npy + j) * a.npz + k;
+}
+
+__device__ __forceinline__ void centred(const PartArgs& a, const ShellArgs& sa, long p, int i, int j, int k,
+                                        double X[3]) {
+  const float3 x = ld3(a.x, (unsigned)p);
+  X[0] = grid_pos(a.mode, i * sa.rx, x.x, sa.NX) * sa.cell[0] - sa.obs[0];
+  X[1] = grid_pos(a.mode, j * sa.ry, x.y, sa.NY) * sa.cell[1] - sa.obs[1];
+  X[2] = grid_pos(a.mode, k * sa.rz, x.z, sa.NZ) * sa.cell[2] - sa.obs[2];
+}
+
+__device__ __forceinline__ void image_of(const ShellArgs& sa, int t, const double X[3], double Xt[3]) {
+  if (!sa.rot) {
+    for (int c = 0; c < 3; ++c) Xt[c] = X[c];
+    return;
+  }
+  const Sym R = sym_of(sa.rot[t]);
+  for (int c = 0; c < 3; ++c) Xt[c] = R.s[c] * X[R.p[c]] + sa.shift[3 * t + c];
+}
+
+// Builds the workgroup's list of images (table order) whose copy of the particles' bounding box
+// meets the shell. Returns the count; hits[] holds the image indices.
+__device__ int shell_images(const PartArgs& a, const ShellArgs& sa, unsigned b, unsigned short* hits) {
+  __shared__ double red[6][4];
+  __shared__ int wcount[4];
+  if (!sa.rot) {  // the box alone: every particle is tested once
+    if (threadIdx.x == 0) hits[0] = 0;
+    __syncthreads();
+    return 1;
+  }
+  double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (int r = 0; r < HP_PER_THREAD; ++r) {
+    int i, j, k;
+    const long p = sb_particle(a, sa, b, r, i, j, k);
+    if (p < 0) continue;
+    double X[3];
+    centred(a, sa, p, i, j, k, X);
+    for (int c = 0; c < 3; ++c) {
+      lo[c] = fmin(lo[c], X[c]);
+      hi[c] = fmax(hi[c], X[c]);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int c = 0; c < 3; ++c) {
+      lo[c] = fmin(lo[c], __shfl_xor(lo[c], o));
+      hi[c] = fmax(hi[c], __shfl_xor(hi[c], o));
+    }
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0)
+    for (int c = 0; c < 3; ++c) {
+      red[c][wv] = lo[c];
+      red[3 + c][wv] = hi[c];
+    }
+  __syncthreads();
+  for (int c = 0; c < 3; ++c) {
+    lo[c] = fmin(fmin(red[c][0], red[c][1]), fmin(red[c][2], red[c][3]));
+    hi[c] = fmax(fmax(red[3 + c][0], red[3 + c][1]), fmax(red[3 + c][2], red[3 + c][3]));
+  }
+  const bool empty = !(lo[0] <= hi[0]);
+  // conservative margins; the per-particle test decides
+  const double r2max = sa.rmax * sa.rmax * (1.0 + 1e-9) + 1e-9, r2min = sa.rmin * sa.rmin * (1.0 - 1e-9);
+  int base = 0;
+  for (int t0 = 0; t0 < sa.ntiles; t0 += 256) {
+    const int t = t0 + (int)threadIdx.x;
+    bool hit = false;
+    if (!empty && t < sa.ntiles) {
+      const Sym R = sym_of(sa.rot[t]);
+      double dmin2 = 0.0, dmax2 = 0.0;
+      for (int c = 0; c < 3; ++c) {
+        double l = lo[R.p[c]], h = hi[R.p[c]];
+        if (R.s[c] < 0) {
+          const double tl = -h;
+          h = -l;
+          l = tl;
+        }
+        l += sa.shift[3 * t + c];
+        h += sa.shift[3 * t + c];
+        const double dn = (l > 0) ? l : ((h < 0) ? -h : 0.0);
+        dmin2 += dn * dn;
+        dmax2 += fmax(l * l, h * h);
+      }
+      hit = dmin2 <= r2max && dmax2 >= r2min;
+    }
+    const unsigned long long m = __ballot(hit);
+    if (lane == 0) wcount[wv] = __popcll(m);
+    __syncthreads();
+    int off = base;
+    for (int w = 0; w < wv; ++w) off += wcount[w];
+    if (hit) hits[off + __popcll(m & ((1ull << lane) - 1ull))] = (unsigned short)t;
+    base += wcount[0] + wcount[1] + wcount[2] + wcount[3];
+    __syncthreads();
+  }
+  return base;
+}
+
+// Per-workgroup aggregation: the (pixel, weight) deposits of the sub-block go through an LDS hash
+// table, and each distinct pixel then reaches the map with one float64 atomic. A deposit that
+// finds no free slot within HP_PROBES probes goes straight to the map.
+__device__ __forceinline__ void hp_table_add(volatile unsigned* keys, double* vals, unsigned pix, double v,
+                                             double* __restrict__ acc) {
+  unsigned h = (pix * 2654435761u) >> 20;  // 12-bit multiplicative hash
+#pragma unroll 1
+  for (int t = 0; t < HP_PROBES; ++t) {
+    const unsigned k = keys[h];
+    if (k == pix) {
+      atomicAdd(&vals[h], v);
+      return;
+    }
+    if (k == HP_EMPTY) {
+      const unsigned old = atomicCAS(const_cast<unsigned*>(keys + h), HP_EMPTY, pix);
+      if (old == HP_EMPTY || old == pix) {
+        atomicAdd(&vals[h], v);
+        return;
+      }
+    }
+    h = (h + 1) & (HP_HASH - 1);
+  }
+  unsafeAtomicAdd(acc + pix, v);
+}
+
+__device__ __forceinline__ bool in_shell(const ShellArgs& sa, const double Xt[3], double& rr) {
+  rr = sqrt(Xt[0] * Xt[0] + Xt[1] * Xt[1] + Xt[2] * Xt[2]);
+  return rr > sa.rmin && rr <= sa.rmax;
+}
+
+__device__ __forceinline__ double phi_of(const double Xt[3]) {
+  const double phi0 = atan2(Xt[1], Xt[0]);
+  const double phi = phi0 < 0 ? phi0 + 2.0 * M_PI : phi0;
+  return (phi < 2.0 * M_PI) ? phi : 0.0;  // -tiny + 2 pi can round up to 2 pi
+}
+
+__global__ void __launch_bounds__(256) k_paint_healpix(PartArgs a, const float* __restrict__ w, float w0, ShellArgs sa,
+                                                       HpGeom g, double* __restrict__ acc) {
+  __shared__ unsigned keys[HP_HASH];
+  __shared__ double vals[HP_HASH];
+  __shared__ unsigned short hits[HP_MAX_TILES];
+  for (int i = threadIdx.x; i < HP_HASH; i += blockDim.x) {
+    keys[i] = HP_EMPTY;
+    vals[i] = 0.0;
+  }
+  const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
+  const int nhit = shell_images(a, sa, b, hits);  // synchronises the workgroup
+  for (int r = 0; r < HP_PER_THREAD; ++r) {
+    int i, j, k;
+    const long p = sb_particle(a, sa, b, r, i, j, k);
+    if (p < 0) continue;
+    double X[3];
+    centred(a, sa, p, i, j, k, X);
+    const double wt = w ? (double)w[p] : (double)w0;
+    for (int h = 0; h < nhit; ++h) {
+      double Xt[3], rr;
+      image_of(sa, hits[h], X, Xt);
+      if (!in_shell(sa, Xt, rr)) continue;
+      const double phi = phi_of(Xt);
+      if (sa.scheme == 0) {
+        hp_table_add(keys, vals, (unsigned)hp_ang2pix(g, Xt[2] / rr, phi), wt, acc);
+      } else {
+        const double theta = atan2(sqrt(Xt[0] * Xt[0] + Xt[1] * Xt[1]), Xt[2]);
+        long pix[4];
+        double wg[4], d0[4], d1[4];
+        hp_interpol<false>(g, theta, phi, pix, wg, d0, d1);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) hp_table_add(keys, vals, (unsigned)pix[m], wt * wg[m], acc);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < HP_HASH; i += blockDim.x) {
+    const unsigned k = keys[i];
+    if (k != HP_EMPTY) unsafeAtomicAdd(acc + k, vals[i]);
+  }
+}
+
+// Position cotangent of the bilinear shell paint: for every image of the particle inside the shell,
+// d/dX sum_m map_bar[pix_m] wgt_m(theta, phi), rotated back to the box frame, times
+// scale * weight * (Mpc/h per grid unit). The ngp scheme is piecewise constant: zero.
+__global__ void __launch_bounds__(256) k_healpix_vjp(PartArgs a, const float* __restrict__ w, float w0, ShellArgs sa,
+                                                     HpGeom g, const float* __restrict__ map_bar, double scale,
+                                                     float* __restrict__ pos_bar, int accumulate) {
+  __shared__ unsigned short hits[HP_MAX_TILES];
+  const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
+  const int nhit = shell_images(a, sa, b, hits);
+  for (int r = 0; r < HP_PER_THREAD; ++r) {
+    int i, j, k;
+    const long p = sb_particle(a, sa, b, r, i, j, k);
+    if (p < 0) continue;
+    double X[3];
+    centred(a, sa, p, i, j, k, X);
+    double gX[3] = {0.0, 0.0, 0.0};
+    if (sa.scheme == 1) {
+      for (int h = 0; h < nhit; ++h) {
+        const int t = hits[h];
+        double Xt[3], rr;
+        image_of(sa, t, X, Xt);
+        if (!in_shell(sa, Xt, rr)) continue;
+        const double rho2 = Xt[0] * Xt[0] + Xt[1] * Xt[1];
+        if (!(rho2 > 0.0)) continue;  // on the polar axis phi is undefined
+        const double rho = sqrt(rho2);
+        const double phi = phi_of(Xt);
+        const double theta = atan2(rho, Xt[2]);
+        long pix[4];
+        double wg[4], dth[4], dph[4];
+        hp_interpol<true>(g, theta, phi, pix, wg, dth, dph);
+        double gt = 0.0, gp = 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const double mb = (double)map_bar[pix[m]];
+          gt += mb * dth[m];
+          gp += mb * dph[m];
+        }
+        const double r2 = rr * rr;
+        // theta = atan2(rho, Z), phi = atan2(Y, X)
+        const double ct = gt * Xt[2] / (rho * r2);
+        double G[3];
+        G[0] = ct * Xt[0] - gp * Xt[1] / rho2;
+        G[1] = ct * Xt[1] + gp * Xt[0] / rho2;
+        G[2] = -gt * rho / r2;
+        if (sa.rot) {
+          const Sym R = sym_of(sa.rot[t]);
+          for (int c = 0; c < 3; ++c) gX[R.p[c]] += R.s[c] * G[c];
+        } else {
+          for (int c = 0; c < 3; ++c) gX[c] += G[c];
+        }
+      }
+    }
+    const double wt = (w ? (double)w[p] : (double)w0) * scale;
+    float3 o = make_float3((float)(gX[0] * wt * sa.cell[0]), (float)(gX[1] * wt * sa.cell[1]),
+                           (float)(gX[2] * wt * sa.cell[2]));
+    if (accumulate) {
+      const float3 q = ld3(pos_bar, (unsigned)p);
+      o = make_float3(q.x + o.x, q.y + o.y, q.z + o.z);
+    }
+    st3(pos_bar, (unsigned)p, o);
+  }
+}
+
+__global__ void k_finalize_map_d(const double* __restrict__ acc, float* __restrict__ map, long n, double scale,
+                                 int accumulate) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float v = (float)(acc[i] * scale);
+    map[i] = accumulate ? map[i] + v : v;
+  }
+}
+
+__global__ void k_zero_d(double* __restrict__ a, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) a[i] = 0.0;
+}
+
+// ---------------------------------------------------------------- Born convergence
+// kappa[k][p] = sum_s C[k][s] (maps[s][p] / mean_s - 1), mean_s = the shell's pixel mean (0 is
+// replaced by FLT_EPSILON, as the reference does with the float32 eps). C (float64, host-built)
+// carries 3/2 Om (H0/c)^2 d_r r_s / a_s times the n(z)-weighted lensing efficiency.
+// Scratch in plan->red: [0, kBornCoef) per-workgroup partial sums, then the coefficient area.
+constexpr int kBornCoef = kRedMax - 8192;
+constexpr int kBornMaxShells = 256, kBornMaxSources = 16;
+
+__device__ __forceinline__ void chunk_of(long n, int nblk, int b, long& beg, long& end) {
+  const long per = (n + nblk - 1) / nblk;
+  beg = (long)b * per;
+  end = beg + per < n ? beg + per : n;
+  if (beg > n) beg = n;
+}
+
+// part[s * nblk + b] = sum of maps[s][chunk b]
+__global__ void __launch_bounds__(256) k_map_sums(const float* __restrict__ maps, long npix, int nblk,
+                                                  double* __restrict__ part) {
+  const int s = blockIdx.y, b = blockIdx.x;
+  long beg, end;
+  chunk_of(npix, nblk, b, beg, end);
+  const float* m = maps + (size_t)s * npix;
+  double t = 0.0;
+  for (long i = beg + threadIdx.x; i < end; i += blockDim.x) t += (double)m[i];
+  block_sum_to(t, part + (size_t)s * nblk + b);
+}
+
+// coefficient area: f[k*S+s] = C/mean_s (K*S), mean[s] (raw), cb[k] = sum_s C[k][s]
+__global__ void __launch_bounds__(256) k_born_coef(const double* __restrict__ part, int nblk, int S, int K, long npix,
+                                                   const double* __restrict__ C, double* __restrict__ coef) {
+  double* f = coef;
+  double* mean = coef + K * S;
+  double* cb = mean + S;
+  for (int s = threadIdx.x; s < S; s += blockDim.x) {
+    double t = 0.0;
+    for (int b = 0; b < nblk; ++b) t += part[(size_t)s * nblk + b];
+    const double m = t / (double)npix;
+    mean[s] = m;
+    const double ms = (m == 0.0) ? (double)FLT_EPSILON : m;
+    for (int k = 0; k < K; ++k) f[k * S + s] = C[k * S + s] / ms;
+  }
+  for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    double t = 0.0;
+    for (int s = 0; s < S; ++s) t += C[k * S + s];
+    cb[k] = t;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_born(const float* __restrict__ maps, long npix, int S, int K,
+                                              const double* __restrict__ coef, float* __restrict__ kappa,
+                                              int accumulate) {
+  const double* f = coef;
+  const double* cb = coef + K * S + S;
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < npix; p += (long)gridDim.x * blockDim.x) {
+    double acc[kBornMaxSources];
+#pragma unroll
+    for (int k = 0; k < kBornMaxSources; ++k) acc[k] = 0.0;
+    for (int s = 0; s < S; ++s) {
+      const double rho = (double)maps[(size_t)s * npix + p];
+#pragma unroll
+      for (int k = 0; k < kBornMaxSources; ++k)
+        if (k < K) acc[k] += f[k * S + s] * rho;
+    }
+#pragma unroll
+    for (int k = 0; k < kBornMaxSources; ++k)
+      if (k < K) {
+        const float v = (float)(acc[k] - cb[k]);
+        float* o = kappa + (size_t)k * npix + p;
+        *o = accumulate ? *o + v : v;
+      }
+  }
+}
+
+// partials of D[k][s] = sum_p kbar[k][p] maps[s][p] (blockIdx.y = s < S) and E[k] = sum_p kbar[k][p]
+// (blockIdx.y = S), at part[((y * K) + k) * nblk + b]
+__global__ void __launch_bounds__(256) k_born_dots(const float* __restrict__ maps, const float* __restrict__ kbar,
+                                                   long npix, int S, int K, int nblk, double* __restrict__ part) {
+  const int s = blockIdx.y, b = blockIdx.x;
+  long beg, end;
+  chunk_of(npix, nblk, b, beg, end);
+  double acc[kBornMaxSources];
+#pragma unroll
+  for (int k = 0; k < kBornMaxSources; ++k) acc[k] = 0.0;
+  for (long p = beg + threadIdx.x; p < end; p += blockDim.x) {
+    const double m = (s < S) ? (double)maps[(size_t)s * npix + p] : 1.0;
+#pragma unroll
+    for (int k = 0; k < kBornMaxSources; ++k)
+      if (k < K) acc[k] += (double)kbar[(size_t)k * npix + p] * m;
+  }
+  for (int k = 0; k < K; ++k) block_sum_to(acc[k], part + ((size_t)s * K + k) * nblk + b);
+}
+
+// h[s] = sum_k C[k][s] D[k][s] / (mean_s^2 npix) (0 where the mean was 0: the eps replacement is a
+// constant); dots (optional) = {D (K*S, index k*S+s), E (K), mean (S, as used: 0 -> FLT_EPSILON)}
+__global__ void __launch_bounds__(256) k_born_vjp_coef(const double* __restrict__ part, int nblk, int S, int K,
+                                                       long npix, const double* __restrict__ C,
+                                                       double* __restrict__ coef, double* __restrict__ dots) {
+  const double* mean = coef + K * S;
+  double* h = coef + K * S + S + K;
+  for (int s = threadIdx.x; s < S; s += blockDim.x) {
+    const double m = mean[s];
+    double t = 0.0;
+    for (int k = 0; k < K; ++k) {
+      double d = 0.0;
+      for (int b = 0; b < nblk; ++b) d += part[((size_t)s * K + k) * nblk + b];
+      if (dots) dots[k * S + s] = d;
+      t += C[k * S + s] * d;
+    }
+    h[s] = (m == 0.0) ? 0.0 : t / (m * m * (double)npix);
+    if (dots) dots[K * S + K + s] = (m == 0.0) ? (double)FLT_EPSILON : m;
+  }
+  if (dots)
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+      double e = 0.0;
+      for (int b = 0; b < nblk; ++b) e += part[((size_t)S * K + k) * nblk + b];
+      dots[K * S + k] = e;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_born_vjp(const float* __restrict__ kbar, long npix, int S, int K,
+                                                  const double* __restrict__ coef, float* __restrict__ maps_bar,
+                                                  int accumulate) {
+  const double* f = coef;
+  const double* h = coef + K * S + S + K;
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < npix; p += (long)gridDim.x * blockDim.x) {
+    double kb[kBornMaxSources];
+#pragma unroll
+    for (int k = 0; k < kBornMaxSources; ++k) kb[k] = (k < K) ? (double)kbar[(size_t)k * npix + p] : 0.0;
+    for (int s = 0; s < S; ++s) {
+      double t = -h[s];
+#pragma unroll
+      for (int k = 0; k < kBornMaxSources; ++k)
+        if (k < K) t += f[k * S + s] * kb[k];
+      float* o = maps_bar + (size_t)s * npix + p;
+      *o = accumulate ? *o + (float)t : (float)t;
+    }
+  }
+}
+
+HpGeom hp_geom(int nside) {
+  HpGeom g;
+  g.nside = nside;
+  g.npix = 12L * nside * nside;
+  g.ncap = 2L * nside * (nside - 1);
+  g.fact2 = 4.0 / (double)g.npix;
+  g.fact1 = (double)(2 * nside) * g.fact2;
+  return g;
+}
+
+ShellArgs shell_args(const pmx_plan_t* p, const PartArgs& a, const double box[3], const double obs[3], double rmin,
+                     double rmax, int scheme, int ntiles, const int* rot, const double* shift) {
+  ShellArgs sa;
+  sa.NX = p->nx;
+  sa.NY = p->ny;
+  sa.NZ = p->nz;
+  const int n[3] = {p->nx, p->ny, p->nz};
+  for (int c = 0; c < 3; ++c) {
+    sa.cell[c] = box[c] / n[c];
+    sa.obs[c] = obs[c];
+  }
+  sa.rmin = rmin;
+  sa.rmax = rmax;
+  sa.scheme = scheme;
+  sa.rx = p->nx / a.npx;
+  sa.ry = p->ny / a.npy;
+  sa.rz = p->nz / a.npz;
+  sa.nbx = (a.npx + HP_SB - 1) / HP_SB;
+  sa.nby = (a.npy + HP_SB - 1) / HP_SB;
+  sa.nbz = (a.npz + HP_SB - 1) / HP_SB;
+  sa.ntiles = rot ? ntiles : 1;
+  sa.rot = rot;
+  sa.shift = shift;
+  return sa;
+}
+
+int born_blocks(long npix, long rows) {
+  long nblk = (npix + 1023) / 1024;
+  const long cap = kBornCoef / (rows > 0 ? rows : 1);
+  if (nblk > 256) nblk = 256;
+  if (nblk > cap) nblk = cap;
+  return (int)(nblk < 1 ? 1 : nblk);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers used by pmx_api.hip
+int pmx_sky_max_tiles() { return HP_MAX_TILES; }
+int pmx_born_max_shells() { return kBornMaxShells; }
+int pmx_born_max_sources() { return kBornMaxSources; }
+
+int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, const double box[3],
+                       const double obs[3], double rmin, double rmax, int nside, int scheme, int ntiles,
+                       const int* rot, const double* shift, float* map, int accumulate, hipStream_t s) {
+  const ShellArgs sa = shell_args(p, a, box, obs, rmin, rmax, scheme, ntiles, rot, shift);
+  const HpGeom g = hp_geom(nside);
+  double* acc = p->acc;
+  hipLaunchKernelGGL(k_zero_d, dim3(grid_for((size_t)g.npix, 256)), dim3(256), 0, s, acc, g.npix);
+  PMX_CHECK_LAUNCH("healpix_zero");
+  hipLaunchKernelGGL(k_paint_healpix, dim3((unsigned)sa.nbx * sa.nby * sa.nbz), dim3(256), 0, s, a, w, w0, sa, g,
+                     acc);
+  PMX_CHECK_LAUNCH("paint_healpix");
+  const double vol = (4.0 * M_PI / (double)g.npix) * (rmax * rmax * rmax - rmin * rmin * rmin) / 3.0;
+  hipLaunchKernelGGL(k_finalize_map_d, dim3(grid_for((size_t)g.npix, 256)), dim3(256), 0, s, acc, map, g.npix,
+                     1.0 / vol, accumulate);
+  return pmx_check_launch("healpix_finalize");
+}
+
+int pmx_launch_healpix_vjp(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, const double box[3],
+                           const double obs[3], double rmin, double rmax, int nside, int scheme, int ntiles,
+                           const int* rot, const double* shift, const float* map_bar, float* pos_bar, int accumulate,
+                           hipStream_t s) {
+  const ShellArgs sa = shell_args(p, a, box, obs, rmin, rmax, scheme, ntiles, rot, shift);
+  const HpGeom g = hp_geom(nside);
+  const double vol = (4.0 * M_PI / (double)g.npix) * (rmax * rmax * rmax - rmin * rmin * rmin) / 3.0;
+  hipLaunchKernelGGL(k_healpix_vjp, dim3((unsigned)sa.nbx * sa.nby * sa.nbz), dim3(256), 0, s, a, w, w0, sa, g,
+                     map_bar, 1.0 / vol, pos_bar, accumulate);
+  return pmx_check_launch("healpix_vjp");
+}
+
+int pmx_launch_born(pmx_plan_t* p, const float* maps, int S, long npix, const double* C, int K, float* kappa,
+                    int accumulate, hipStream_t s) {
+  const int nblk = born_blocks(npix, S);
+  double* coef = p->red + kBornCoef;
+  hipLaunchKernelGGL(k_map_sums, dim3(nblk, S), dim3(256), 0, s, maps, npix, nblk, p->red);
+  PMX_CHECK_LAUNCH("born_sums");
+  hipLaunchKernelGGL(k_born_coef, dim3(1), dim3(256), 0, s, p->red, nblk, S, K, npix, C, coef);
+  PMX_CHECK_LAUNCH("born_coef");
+  hipLaunchKernelGGL(k_born, dim3(grid_for((size_t)npix, 256)), dim3(256), 0, s, maps, npix, S, K, coef, kappa,
+                     accumulate);
+  return pmx_check_launch("born");
+}
+
+int pmx_launch_born_vjp(pmx_plan_t* p, const float* maps, int S, long npix, const double* C, int K,
+                        const float* kappa_bar, float* maps_bar, int accumulate, double* dots, hipStream_t s) {
+  double* coef = p->red + kBornCoef;
+  // means and f of the forward
+  int nblk = born_blocks(npix, S);
+  hipLaunchKernelGGL(k_map_sums, dim3(nblk, S), dim3(256), 0, s, maps, npix, nblk, p->red);
+  PMX_CHECK_LAUNCH("born_sums");
+  hipLaunchKernelGGL(k_born_coef, dim3(1), dim3(256), 0, s, p->red, nblk, S, K, npix, C, coef);
+  PMX_CHECK_LAUNCH("born_coef");
+  nblk = born_blocks(npix, (long)(S + 1) * K);
+  hipLaunchKernelGGL(k_born_dots, dim3(nblk, S + 1), dim3(256), 0, s, maps, kappa_bar, npix, S, K, nblk, p->red);
+  PMX_CHECK_LAUNCH("born_dots");
+  hipLaunchKernelGGL(k_born_vjp_coef, dim3(1), dim3(256), 0, s, p->red, nblk, S, K, npix, C, coef, dots);
+  PMX_CHECK_LAUNCH("born_vjp_coef");
+  hipLaunchKernelGGL(k_born_vjp, dim3(grid_for((size_t)npix, 256)), dim3(256), 0, s, kappa_bar, npix, S, K, coef,
+                     maps_bar, accumulate);
+  return pmx_check_launch("born_vjp");
+}

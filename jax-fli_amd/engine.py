@@ -174,34 +174,138 @@ def color_field(field: torch.Tensor, box, ktab: torch.Tensor, ptab: torch.Tensor
     return out
 
 
-def lpt1(ic: torch.Tensor, growth: float, vel_factor: float, out=None, plan=None):
+def lpt1(ic: torch.Tensor, growth: float, vel_factor: float, out=None, plan=None, particles=None):
+    """dx = growth F(ic), p = vel_factor dx at the particle lattice; `particles` = (npx, npy, npz), a grid
+    dividing the mesh (default: one particle per cell)."""
     ic = _f32(ic, "ic")
     plan = plan or get_plan(ic.shape, ic.device)
+    npx, npy, npz = tuple(ic.shape) if particles is None else tuple(int(n) for n in particles)
     if out is None:
-        dx = torch.empty(tuple(ic.shape) + (3,), dtype=torch.float32, device=ic.device)
+        dx = torch.empty((npx, npy, npz, 3), dtype=torch.float32, device=ic.device)
         pv = torch.empty_like(dx)
     else:
         dx, pv = out
-    check(load().pmx_lpt1(plan.handle, ptr(ic), float(growth), float(vel_factor), ptr(dx), ptr(pv), stream_ptr()),
-          "pmx_lpt1")
+    check(load().pmx_lpt1_lattice(plan.handle, ptr(ic), npx, npy, npz, float(growth), float(vel_factor), ptr(dx),
+                                  ptr(pv), stream_ptr()), "pmx_lpt1_lattice")
     return dx, pv
 
 
+def lpt1_vjp(dx_bar, p_bar, mesh_shape, growth: float, vel_factor: float) -> torch.Tensor:
+    """Cotangent of the initial field for lpt1 given (dx_bar, p_bar) (either may be None)."""
+    ref = dx_bar if dx_bar is not None else p_bar
+    ref = _f32(ref, "dx_bar")
+    npx, npy, npz = _pgrid(ref)
+    dx_bar = None if dx_bar is None else _f32(dx_bar, "dx_bar")
+    p_bar = None if p_bar is None else _f32(p_bar, "p_bar")
+    plan = get_plan(mesh_shape, ref.device)
+    out = torch.empty(tuple(mesh_shape), dtype=torch.float32, device=ref.device)
+    check(load().pmx_lpt1_vjp(plan.handle, ptr(dx_bar), ptr(p_bar), npx, npy, npz, float(growth), float(vel_factor),
+                              ptr(out), stream_ptr()), "pmx_lpt1_vjp")
+    return out
+
+
+_SCHEMES = {"ngp": 0, "bilinear": 1}
+
+
+def _scheme(scheme):
+    sch = _SCHEMES.get(scheme)
+    if sch is None:
+        raise NotImplementedError(f"spherical painting scheme {scheme!r} is not supported (ngp, bilinear)")
+    return sch
+
+
+def _tiles(tiles, device):
+    """(ntiles, rot ptr, shift ptr, keep-alive) for an image table (rot codes, shifts in Mpc/h) or None."""
+    if tiles is None:
+        return 1, None, None, None
+    rot, shift = tiles
+    rot = torch.as_tensor(rot, dtype=torch.int32).reshape(-1).to(device).contiguous()
+    shift = torch.as_tensor(shift, dtype=torch.float64).reshape(-1, 3).to(device).contiguous()
+    if rot.numel() != shift.shape[0] or rot.numel() < 1:
+        raise ValueError("image table: one rotation code and one shift per image")
+    return rot.numel(), rot, shift, (rot, shift)
+
+
 def paint_healpix(pos: torch.Tensor, mesh_shape, box, observer, rmin, rmax, nside, scheme="bilinear",
-                  mode="relative", weights=1.0, out=None, accumulate=False) -> torch.Tensor:
+                  mode="relative", weights=1.0, out=None, accumulate=False, tiles=None) -> torch.Tensor:
+    """HEALPix shell map; `tiles` = (rotation codes, shifts (n, 3) Mpc/h) sums the box's images."""
     pos = _f32(pos, "pos")
     npx, npy, npz = _pgrid(pos)
     plan = get_plan(mesh_shape, pos.device)
     if out is None:
         out = torch.empty(12 * nside * nside, dtype=torch.float32, device=pos.device)
     w, w0 = _weights(weights, npx * npy * npz)
-    sch = {"ngp": 0, "bilinear": 1}.get(scheme)
-    if sch is None:
-        raise NotImplementedError(f"spherical painting scheme {scheme!r} is not supported (ngp, bilinear)")
-    check(load().pmx_paint_healpix(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], ptr(w), w0, dbl3(box),
-                                   dbl3(observer), float(rmin), float(rmax), int(nside), sch, ptr(out),
-                                   int(accumulate), stream_ptr()), "pmx_paint_healpix")
+    nt, rot, shift, _keep = _tiles(tiles, pos.device)
+    check(load().pmx_paint_healpix_tiled(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], ptr(w), w0, dbl3(box),
+                                         dbl3(observer), float(rmin), float(rmax), int(nside), _scheme(scheme), nt,
+                                         ptr(rot), ptr(shift), ptr(out), int(accumulate), stream_ptr()),
+          "pmx_paint_healpix_tiled")
     return out
+
+
+def paint_healpix_vjp(pos: torch.Tensor, map_bar: torch.Tensor, mesh_shape, box, observer, rmin, rmax, nside,
+                      scheme="bilinear", mode="relative", weights=1.0, tiles=None, pos_bar=None,
+                      accumulate=False) -> torch.Tensor:
+    """pos_bar (+)= d<map_bar, paint_healpix(pos)>/d pos."""
+    pos = _f32(pos, "pos")
+    map_bar = _f32(map_bar, "map_bar")
+    if map_bar.numel() != 12 * nside * nside:
+        raise ValueError("map_bar must have 12 nside^2 pixels")
+    npx, npy, npz = _pgrid(pos)
+    plan = get_plan(mesh_shape, pos.device)
+    if pos_bar is None:
+        pos_bar = torch.empty_like(pos)
+        accumulate = False
+    w, w0 = _weights(weights, npx * npy * npz)
+    nt, rot, shift, _keep = _tiles(tiles, pos.device)
+    check(load().pmx_paint_healpix_vjp(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], ptr(w), w0, dbl3(box),
+                                       dbl3(observer), float(rmin), float(rmax), int(nside), _scheme(scheme), nt,
+                                       ptr(rot), ptr(shift), ptr(map_bar), ptr(pos_bar), int(accumulate),
+                                       stream_ptr()), "pmx_paint_healpix_vjp")
+    return pos_bar
+
+
+def _born_args(maps, coef):
+    maps = _f32(maps, "maps")
+    if maps.ndim != 2:
+        raise ValueError("maps must be (nshell, npix)")
+    coef = torch.as_tensor(coef, dtype=torch.float64).to(maps.device).contiguous()
+    if coef.ndim != 2 or coef.shape[1] != maps.shape[0]:
+        raise ValueError("coef must be (nsrc, nshell)")
+    return maps, coef
+
+
+def born(maps: torch.Tensor, coef, out=None, accumulate=False) -> torch.Tensor:
+    """kappa (nsrc, npix) = sum_s coef[k, s] (maps[s] / mean(maps[s]) - 1)."""
+    maps, coef = _born_args(maps, coef)
+    S, npix = int(maps.shape[0]), int(maps.shape[1])
+    K = int(coef.shape[0])
+    plan = get_plan((8, 8, 8), maps.device)
+    if out is None:
+        out = torch.empty((K, npix), dtype=torch.float32, device=maps.device)
+    check(load().pmx_born(plan.handle, ptr(maps), S, npix, ptr(coef), K, ptr(out), int(accumulate), stream_ptr()),
+          "pmx_born")
+    return out
+
+
+def born_vjp(maps: torch.Tensor, coef, kappa_bar: torch.Tensor, want_dots=False):
+    """(maps_bar, dots | None): cotangent of born for kappa_bar; dots = (D (nsrc, nshell), E (nsrc,),
+    mean (nshell,)) with D = <kappa_bar[k], maps[s]>, E = sum kappa_bar[k], the shell means as used
+    (float64, device)."""
+    maps, coef = _born_args(maps, coef)
+    kappa_bar = _f32(kappa_bar, "kappa_bar")
+    S, npix = int(maps.shape[0]), int(maps.shape[1])
+    K = int(coef.shape[0])
+    if tuple(kappa_bar.shape) != (K, npix):
+        raise ValueError("kappa_bar must be (nsrc, npix)")
+    plan = get_plan((8, 8, 8), maps.device)
+    maps_bar = torch.empty_like(maps)
+    dots = torch.empty(K * S + K + S, dtype=torch.float64, device=maps.device) if want_dots else None
+    check(load().pmx_born_vjp(plan.handle, ptr(maps), S, npix, ptr(coef), K, ptr(kappa_bar), ptr(maps_bar), 0,
+                              ptr(dots), stream_ptr()), "pmx_born_vjp")
+    if dots is None:
+        return maps_bar, None
+    return maps_bar, (dots[:K * S].view(K, S), dots[K * S:K * S + K], dots[K * S + K:])
 
 
 def paint_flat(pos: torch.Tensor, mesh_shape, flatsky_npix, zlo, zhi, scale, mode="absolute", weights=1.0,

@@ -197,8 +197,15 @@ class ParticleField(AbstractField):
         ms = torch.tensor(self.mesh_size, dtype=torch.float32, device=self.array.device)
         bs = torch.tensor(self.box_size, dtype=torch.float32, device=self.array.device)
         arr = self.array
+        lattice = None
+        if PositionUnit.GRID_RELATIVE in (self.unit, unit):
+            # the particle lattice: (i, j, k) * mesh / np (one particle per cell when np = mesh)
+            npart = tuple(int(n) for n in arr.shape[-4:-1])
+            spacing = torch.tensor([m / n for m, n in zip(self.mesh_size, npart)], dtype=torch.float32,
+                                   device=arr.device)
+            lattice = uniform_particles(npart, arr.device) * spacing
         if self.unit == PositionUnit.GRID_RELATIVE:
-            grid = arr + uniform_particles(arr.shape[-4:-1], arr.device)
+            grid = arr + lattice
         elif self.unit == PositionUnit.MPC_H:
             grid = arr / bs * ms
         else:
@@ -206,7 +213,7 @@ class ParticleField(AbstractField):
         if unit == PositionUnit.GRID_ABSOLUTE:
             out = grid
         elif unit == PositionUnit.GRID_RELATIVE:
-            out = grid - uniform_particles(arr.shape[-4:-1], arr.device)
+            out = grid - lattice
         else:
             out = grid / ms * bs
         return self.replace(array=out.contiguous(), unit=unit)
@@ -279,7 +286,9 @@ class ParticleField(AbstractField):
     def paint_spherical(self, center, density_plane_width, *, scheme="bilinear", weights=None,
                         kernel_width_arcmin=None, smoothing_interpretation="fwhm", paint_nside=None,
                         ud_grade_power=0.0, ud_grade_order_in="RING", ud_grade_order_out="RING",
-                        ud_grade_pess=False, batch_size=None) -> "SphericalDensity":
+                        ud_grade_pess=False, batch_size=None, tiles=None) -> "SphericalDensity":
+        """HEALPix shell maps (particles.py:326-442). `tiles` (an extension): (rotation codes, shifts
+        (n, 3) Mpc/h) sums the shell paint over that table of images of the periodic box."""
         if self.nside is None:
             raise ValueError("Spherical painting requires `nside`.")
         if kernel_width_arcmin is not None or paint_nside not in (None, self.nside):
@@ -301,7 +310,8 @@ class ParticleField(AbstractField):
             c, wdt = centers[b], widths[b]
             maps.append(engine.paint_healpix(data[b].contiguous(), self.mesh_size, self.box_size,
                                              self.observer_position_mpc, c - wdt / 2, c + wdt / 2, self.nside,
-                                             scheme=scheme, mode=mode, weights=1.0 if weights is None else weights))
+                                             scheme=scheme, mode=mode, weights=1.0 if weights is None else weights,
+                                             tiles=tiles))
         arr = torch.stack(maps) if len(maps) > 1 else maps[0]
         return SphericalDensity.FromDensityMetadata(array=arr, field=self, status=FieldStatus.LIGHTCONE,
                                                     comoving_centers=centers, density_width=widths,
@@ -350,3 +360,13 @@ class SphericalDensity(AbstractField):
 
     def _base_ndim(self) -> int:
         return 1
+
+
+@dataclasses.dataclass(frozen=True, eq=False)
+class FlatKappaField(FlatDensity):
+    """Convergence map, flat sky (fields/lensing_maps.py:19): (ny, nx) or (n_sources, ny, nx)."""
+
+
+@dataclasses.dataclass(frozen=True, eq=False)
+class SphericalKappaField(SphericalDensity):
+    """Convergence map, HEALPix RING (fields/lensing_maps.py:72): (npix,) or (n_sources, npix)."""

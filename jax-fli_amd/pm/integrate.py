@@ -10,6 +10,7 @@ separates them, and calls ``save_at`` at each snapshot.
 
 from __future__ import annotations
 
+import dataclasses
 import math
 from typing import Any
 
@@ -63,8 +64,9 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
     if not isinstance(solver, ReversibleDoubleKickDrift):
         raise NotImplementedError("gradients need the reversible solver (ReversibleDoubleKickDrift) with "
                                   "adjoint='reverse'")
-    target = solver.interp_kernel.painting.target if solver.interp_kernel.painting is not None else "particles"
-    if target not in ("density", "particles"):
+    interp = solver.interp_kernel
+    target = interp.painting.target if interp.painting is not None else "particles"
+    if target not in ("density", "particles", "spherical"):
         raise NotImplementedError(f"gradients through '{target}' snapshots are not implemented")
     cosmo, _, _ = autodiff.split_cosmo(cosmo_in)
     mesh_size = displacements.mesh_size
@@ -80,6 +82,25 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
         else:
             events.append(("save", ev[1], ev[2]))
     saved_fields = []
+    base_state = interp.init()
+
+    def _state(idx):
+        return NBodyState(interp_state=dataclasses.replace(base_state, rotation_idx=-1, shell_idx=idx),
+                          t_initial=float(t0))
+
+    def save_vjp(idx, x, g):
+        """position cotangent of snapshot idx (the paint of x) for the snapshot cotangent g"""
+        if target == "particles":
+            return g
+        weights = 1.0 if interp.painting.weights is None else interp.painting.weights
+        if target == "density":
+            return engine.paint_vjp(x, g.contiguous(), mode=mode, weights=weights)[0]
+        st = _state(idx).interp_state
+        rc, wd = interp.shell(st)
+        return engine.paint_healpix_vjp(x, g.contiguous(), mesh_size, displacements.box_size,
+                                        displacements.observer_position_mpc, rc - wd / 2, rc + wd / 2,
+                                        displacements.nside, scheme=interp.painting.scheme, mode=mode,
+                                        weights=weights, tiles=interp.images(st, displacements))
 
     def run(x0, v0, cosmo_):
         x = x0.detach().contiguous().clone()
@@ -92,15 +113,13 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
                 engine.step(x, v, mesh_size, mode=mode, force_scale=fs, c_kick=ev[2], c_drift_post=ev[3])
             else:
                 _, idx, tt = ev
-                st = NBodyState(interp_state=solver.interp_kernel.init().__class__(rotation_idx=-1, shell_idx=idx),
-                                t_initial=float(t0))
                 fld = solver.save_at(displacements.replace(array=x, scale_factors=tt),
-                                     velocities.replace(array=v, scale_factors=tt), tt, dt0, st, cosmo_)
+                                     velocities.replace(array=v, scale_factors=tt), tt, dt0, _state(idx), cosmo_)
                 saved_fields.append(fld)
                 snaps.append(fld.array)
         return x, v, torch.stack(snaps), len(grid)
 
-    run.info = {"mesh": mesh_size, "mode": mode, "target": target, "events": events, "force_scale": force_scale}
+    run.info = {"mesh": mesh_size, "mode": mode, "events": events, "force_scale": force_scale, "save_vjp": save_vjp}
 
     def coef_fn(c):
         return solver.step_coeffs(grid, dt0, float(t0), c, raw=True)
@@ -152,7 +171,7 @@ def integrate(displacements, velocities, cosmo, ts, solver: AbstractNBodySolver,
                 run_op(pending, x, v, mesh_size, mode, cosmo)
                 pending = None
             _, idx, tt = item
-            st = NBodyState(interp_state=state.interp_state.__class__(rotation_idx=-1, shell_idx=idx),
+            st = NBodyState(interp_state=dataclasses.replace(state.interp_state, rotation_idx=-1, shell_idx=idx),
                             t_initial=state.t_initial)
             dx_f = displacements.replace(array=x, scale_factors=tt)
             p_f = velocities.replace(array=v, scale_factors=tt)

@@ -1,10 +1,12 @@
-"""Snapshot interpolation kernels (src/jax_fli/pm/interp.py:46-209): the abstract interface and
-``NoInterp`` (paint the particles as they are at each snapshot). Drift/Onion/Telephoto tilers are
-not implemented on this engine."""
+"""Snapshot interpolation kernels (src/jax_fli/pm/interp.py:46-504): the abstract interface,
+``NoInterp`` (paint the particles as they are at each snapshot) and ``OnionTiler`` (spherical shells
+beyond the box painted from rotated periodic images of it). Drift-to-lightcone corrections and the
+Drift/Telephoto kernels are not implemented on this engine."""
 
 from __future__ import annotations
 
 import dataclasses
+import itertools
 import warnings
 from typing import Any
 
@@ -43,6 +45,16 @@ class AbstractInterp:
 
     def paint(self, state: InterpTilerState, t, y, cosmo):
         raise NotImplementedError
+
+    def shell(self, state: InterpTilerState):
+        """(r_center, width) of the current shell (Mpc/h)."""
+        return (float(np.asarray(self.r_centers).reshape(-1)[state.shell_idx]),
+                float(np.asarray(self.density_widths).reshape(-1)[state.shell_idx]))
+
+    def images(self, state: InterpTilerState, field):
+        """Image table (rotation codes, shifts (n, 3) Mpc/h) the current shell is painted from, or None
+        for the box alone."""
+        return None
 
 
 @dataclasses.dataclass(frozen=True)
@@ -94,9 +106,71 @@ class DriftInterp(AbstractInterp):
         raise NotImplementedError("DriftInterp is not implemented on the pmx engine")
 
 
+@dataclasses.dataclass(frozen=True)
 class OnionTiler(AbstractInterp):
-    def __init__(self, *a, **k):
-        raise NotImplementedError("OnionTiler is not implemented on the pmx engine")
+    """Spherical shells from images of the periodic box (src/jax_fli/pm/interp.py:323-504): a shell
+    inside the box (r_center + width/2 <= max_comoving_distance) is painted from the box alone;
+    a farther one from the 27 images at offsets product([0, 1, -1], repeat=3) x box (shifted +1 / -1
+    on axes where the observer sits on the face at 0 / 1), each rotated by a random cube symmetry,
+    all summed into one map by a single pmx_paint_healpix_tiled launch.
+
+    Differences from the reference, by design: the rotations come from numpy's PCG64 seeded with
+    `seed` (the reference draws jax.random.randint(PRNGKey(42), (27,), 0, 47), whose bits this
+    engine does not reproduce); particles are wrapped into the box before imaging (the reference
+    images unwrapped positions); `layers` > 1 extends the offsets to [-layers, layers]^3 so shells
+    beyond 1.5 boxes are covered, and `rotate=False` gives plain periodic replication. The
+    drift-to-lightcone correction (drift_on_lightcone=True, the reference default) is not
+    implemented and is refused."""
+
+    drift_on_lightcone: bool = True
+    layers: int = 1
+    rotate: bool = True
+    seed: int = 42
+
+    def init(self) -> InterpTilerState:
+        if self.painting.target != "spherical":
+            raise ValueError("OnionTiler only supports spherical painting.")
+        if self.drift_on_lightcone:
+            raise NotImplementedError("drift to the lightcone is not implemented on the pmx engine; "
+                                      "construct OnionTiler(..., drift_on_lightcone=False)")
+        if self.layers < 1:
+            raise ValueError("layers must be >= 1")
+        n = (2 * self.layers + 1) ** 3
+        if self.rotate:  # entries 1..47 of the symmetry table (0 is the identity)
+            rots = np.random.default_rng(self.seed).integers(0, 47, size=n) + 1
+        else:
+            rots = np.zeros(n, dtype=np.int64)
+        return InterpTilerState(rotation_idx=-1, shell_idx=-1, rotations=rots.astype(np.int32))
+
+    def advance(self, state, t):
+        hit = self.ts is not None and bool(np.isin(np.float32(t), np.asarray(self.ts, dtype=np.float32)))
+        return dataclasses.replace(state, shell_idx=state.shell_idx + 1) if hit else state
+
+    def rewind(self, state, t):
+        hit = self.ts is not None and bool(np.isin(np.float32(t), np.asarray(self.ts, dtype=np.float32)))
+        return dataclasses.replace(state, shell_idx=state.shell_idx - 1) if hit else state
+
+    def images(self, state, field):
+        r_center, width = self.shell(state)
+        if self.max_comoving_distance is not None and r_center + width / 2 <= self.max_comoving_distance:
+            return None
+        order = [0] + [s * k for k in range(1, self.layers + 1) for s in (1, -1)]
+        shifts = np.array(list(itertools.product(order, repeat=3)), dtype=np.float64)
+        f = np.asarray(field.observer_position, dtype=np.float64)
+        shifts = np.where(f == 0, shifts + 1, shifts)
+        shifts = np.where(f == 1, shifts - 1, shifts)
+        return np.asarray(state.rotations, dtype=np.int32), shifts * np.asarray(field.box_size, dtype=np.float64)
+
+    def paint(self, state, t, y, cosmo):
+        dx, _ = y
+        r_center, width = self.shell(state)
+        pt = self.painting
+        result = dx.paint_spherical(center=r_center, density_plane_width=width, scheme=pt.scheme, weights=pt.weights,
+                                    kernel_width_arcmin=pt.kernel_width_arcmin,
+                                    smoothing_interpretation=pt.smoothing_interpretation,
+                                    paint_nside=pt.paint_nside, batch_size=pt.batch_size,
+                                    tiles=self.images(state, dx))
+        return result.replace(scale_factors=float(t), comoving_centers=r_center, density_width=width)
 
 
 class TelephotoInterp(AbstractInterp):

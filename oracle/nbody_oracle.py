@@ -99,7 +99,7 @@ class Cosmo:
 
     def chi(self, a):
         a = np.atleast_1d(np.asarray(a, dtype=np.float64))
-        f = lambda x: C_OVER_H0 / (x * x * float(self.E(x)))
+        f = lambda x: C_OVER_H0 / (x * x * float(self.E(x)[0]))
         return np.array([_si.quad(f, ai, 1.0, epsabs=1e-12, epsrel=1e-12)[0] for ai in a])
 
 

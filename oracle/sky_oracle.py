@@ -1,0 +1,266 @@
+"""CPU oracle for the sky projections: HEALPix shell painting over box images, its position
+adjoint, and Born convergence with its adjoint.
+
+TEST INFRASTRUCTURE ONLY (imported by tests/ and __graft_entry__.smoke() as the checker). The
+product (jax-fli_amd/) never imports it. Float64 NumPy; per-particle Python loops for the HEALPix
+parts, so test sizes stay small.
+
+Parity status per function:
+  cube_symmetry                         pinned (the 48 matrices are distinct signed permutations,
+                                        entry 0 is the identity, orthogonality; the enumeration
+                                        order restates get_all_47_symmetries, interp.py:31-43)
+  onion_shifts                          pinned (27 distinct offsets in product([0, 1, -1]) order;
+                                        observer-on-face shifts, interp.py:447-451)
+  paint_healpix_tiled                   pinned (one identity image = pm_oracle.paint_healpix;
+                                        periodic images of a uniform lattice fill a far shell with
+                                        the lattice's mean density); normalisation as pm_oracle
+  healpix_interp_weights_grad           pinned (float64 central differences of the weights)
+  paint_healpix_vjp                     pinned (central differences of paint_healpix_tiled)
+  simps_weights                         pinned (exact for cubics; jax_cosmo simps restated)
+  born_spherical                        pinned (thin-plane sum -> the Born integral
+                                        3/2 Om (H0/c)^2 int chi (chi_s - chi) / (chi_s a) delta dchi
+                                        by quadrature; linearity in delta)
+  redshift-distribution normalisation   parity unpinned (jax_cosmo's redshift_distribution, not
+                                        in the reference tree: restated as Simpson over [0, 10]
+                                        with 256 intervals)
+  born_vjp                              pinned (central differences of born_spherical)
+"""
+
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+
+from . import pm_oracle as orc
+
+H0_OVER_C = 100.0 / 299792.458  # h / Mpc (jax_cosmo.constants H0 = 100 km/s/(Mpc/h), c in km/s)
+FLT_EPS = float(np.finfo(np.float32).eps)
+
+
+# ---------------------------------------------------------------------------- images
+def cube_symmetry(code):
+    """Matrix `code` of the enumeration of get_all_47_symmetries (src/jax_fli/pm/interp.py:31-43)
+    before it drops entry 0 (the identity): signs outer, permutations inner, M[i, p[i]] = s[i]."""
+    signs = list(itertools.product([1, -1], repeat=3))
+    perms = list(itertools.permutations([0, 1, 2]))
+    s = signs[code // 6]
+    p = perms[code % 6]
+    m = np.zeros((3, 3))
+    for i in range(3):
+        m[i, p[i]] = s[i]
+    return m
+
+
+def onion_shifts(observer_fraction, box, layers=1):
+    """Image offsets (Mpc/h) of OnionTiler._paint_tiled_27 (interp.py:447-451) for layers=1:
+    product([0, 1, -1], repeat=3), +1 on axes where the observer sits at 0 and -1 where at 1."""
+    order = [0] + [s * n for n in range(1, layers + 1) for s in (1, -1)]
+    sh = np.array(list(itertools.product(order, repeat=3)), dtype=np.float64)
+    f = np.asarray(observer_fraction, dtype=np.float64)
+    sh = np.where(f == 0, sh + 1, sh)
+    sh = np.where(f == 1, sh - 1, sh)
+    return sh * np.asarray(box, dtype=np.float64)
+
+
+def _centred(pos, mesh_shape, box, observer, mode):
+    pos = np.asarray(pos)
+    if pos.dtype != np.float64:  # float32 inputs as the device sees them; float64 for differencing
+        pos = pos.astype(np.float32)
+    np_shape = pos.shape[:3]
+    g = pos.reshape(-1, 3).astype(np.float64)
+    if mode == "relative":
+        g = g + orc._lattice(np_shape, mesh_shape).astype(np.float64)
+    ms = np.asarray(mesh_shape, dtype=np.float64)
+    g = g - np.floor(g / ms) * ms
+    cell = np.asarray(box, dtype=np.float64) / ms
+    return g * cell - np.asarray(observer, dtype=np.float64), cell
+
+
+def _angles(X):
+    x, y, z = X
+    phi = np.arctan2(y, x)
+    if phi < 0:
+        phi += 2 * np.pi
+    if not phi < 2 * np.pi:
+        phi = 0.0
+    return np.arctan2(np.sqrt(x * x + y * y), z), phi
+
+
+def paint_healpix_tiled(pos, mesh_shape, box, observer, rmin, rmax, nside, scheme="bilinear", mode="relative",
+                        weight=1.0, tiles=None):
+    """Sum over images (code, shift) of the shell paint of R_code X + shift, X the observer-centred,
+    box-wrapped position; tiles=None: the box alone. Normalisation as pm_oracle.paint_healpix."""
+    X, _ = _centred(pos, mesh_shape, box, observer, mode)
+    n = X.shape[0]
+    wt = np.broadcast_to(np.asarray(weight, dtype=np.float64).reshape(-1), (n,))
+    tiles = [(0, np.zeros(3))] if tiles is None else [(int(c), np.asarray(s, dtype=np.float64)) for c, s in tiles]
+    npix = 12 * nside * nside
+    acc = np.zeros(npix)
+    for code, sh in tiles:
+        Xt = X @ cube_symmetry(code).T + sh
+        r = np.sqrt((Xt**2).sum(axis=1))
+        for p in np.nonzero((r > rmin) & (r <= rmax))[0]:
+            theta, phi = _angles(Xt[p])
+            if scheme == "ngp":
+                acc[orc.healpix_ang2pix(nside, Xt[p, 2] / r[p], phi)] += wt[p]
+            else:
+                pix, w = orc.healpix_interp_weights(nside, theta, phi)
+                for q in range(4):
+                    acc[pix[q]] += wt[p] * w[q]
+    vol = (4 * np.pi / npix) * (rmax**3 - rmin**3) / 3.0
+    return acc / vol
+
+
+def healpix_interp_weights_grad(nside, theta, phi):
+    """(pix, wgt, d wgt / d theta, d wgt / d phi) of the bilinear ring interpolation: the derivative
+    of each weight formula of HEALPix get_interpol, the ring and pixel choices held fixed."""
+    z = np.cos(theta)
+    ir1 = orc._ring_above(nside, z)
+    ir2 = ir1 + 1
+    nring = 4 * nside
+    pix, _ = orc.healpix_interp_weights(nside, theta, phi)
+    w = {}
+    th = {}
+    dphi = {}
+    for slot, ir in ((0, ir1), (2, ir2)):
+        if (slot == 0 and ir > 0) or (slot == 2 and ir < nring):
+            sp, nr, t, shift = orc._ring_info(nside, ir)
+            d = 2.0 * np.pi / nr
+            tmp = phi / d - 0.5 * shift
+            i1 = int(tmp) - 1 if tmp < 0 else int(tmp)
+            w[slot] = (phi - (i1 + 0.5 * shift) * d) / d
+            th[slot] = t
+            dphi[slot] = d
+    wgt = np.zeros(4)
+    dth = np.zeros(4)
+    dph = np.zeros(4)
+    if ir1 == 0:
+        wt_, dw = theta / th[2], 1.0 / th[2]
+        a, b = 1 - w[2], w[2]
+        wgt[:] = [(1 - wt_) / 4, (1 - wt_) / 4, a * wt_ + (1 - wt_) / 4, b * wt_ + (1 - wt_) / 4]
+        dth[:] = [-dw / 4, -dw / 4, a * dw - dw / 4, b * dw - dw / 4]
+        dph[:] = [0.0, 0.0, -wt_ / dphi[2], wt_ / dphi[2]]
+    elif ir2 == nring:
+        wt_, dw = (theta - th[0]) / (np.pi - th[0]), 1.0 / (np.pi - th[0])
+        a, b = 1 - w[0], w[0]
+        wgt[:] = [a * (1 - wt_) + wt_ / 4, b * (1 - wt_) + wt_ / 4, wt_ / 4, wt_ / 4]
+        dth[:] = [-a * dw + dw / 4, -b * dw + dw / 4, dw / 4, dw / 4]
+        dph[:] = [-(1 - wt_) / dphi[0], (1 - wt_) / dphi[0], 0.0, 0.0]
+    else:
+        wt_, dw = (theta - th[0]) / (th[2] - th[0]), 1.0 / (th[2] - th[0])
+        a0, b0, a2, b2 = 1 - w[0], w[0], 1 - w[2], w[2]
+        wgt[:] = [a0 * (1 - wt_), b0 * (1 - wt_), a2 * wt_, b2 * wt_]
+        dth[:] = [-a0 * dw, -b0 * dw, a2 * dw, b2 * dw]
+        dph[:] = [-(1 - wt_) / dphi[0], (1 - wt_) / dphi[0], -wt_ / dphi[2], wt_ / dphi[2]]
+    return pix, wgt, dth, dph
+
+
+def paint_healpix_vjp(pos, map_bar, mesh_shape, box, observer, rmin, rmax, nside, scheme="bilinear",
+                      mode="relative", weight=1.0, tiles=None):
+    """pos_bar = d<map_bar, paint_healpix_tiled(pos)>/d pos (float64); zero for ngp."""
+    pos = np.asarray(pos)
+    X, cell = _centred(pos, mesh_shape, box, observer, mode)
+    n = X.shape[0]
+    out = np.zeros((n, 3))
+    if scheme == "ngp":
+        return out.reshape(pos.shape)
+    wt = np.broadcast_to(np.asarray(weight, dtype=np.float64).reshape(-1), (n,))
+    tiles = [(0, np.zeros(3))] if tiles is None else [(int(c), np.asarray(s, dtype=np.float64)) for c, s in tiles]
+    npix = 12 * nside * nside
+    scale = 1.0 / ((4 * np.pi / npix) * (rmax**3 - rmin**3) / 3.0)
+    mb = np.asarray(map_bar, dtype=np.float64)
+    for code, sh in tiles:
+        R = cube_symmetry(code)
+        Xt = X @ R.T + sh
+        r = np.sqrt((Xt**2).sum(axis=1))
+        for p in np.nonzero((r > rmin) & (r <= rmax))[0]:
+            x, y, z = Xt[p]
+            rho2 = x * x + y * y
+            if not rho2 > 0:
+                continue
+            theta, phi = _angles(Xt[p])
+            pix, _, dth, dph = healpix_interp_weights_grad(nside, theta, phi)
+            gt = sum(mb[pix[m]] * dth[m] for m in range(4))
+            gp = sum(mb[pix[m]] * dph[m] for m in range(4))
+            rho, r2 = np.sqrt(rho2), r[p] ** 2
+            G = np.array([gt * z * x / (rho * r2) - gp * y / rho2, gt * z * y / (rho * r2) + gp * x / rho2,
+                          -gt * rho / r2])
+            out[p] += (R.T @ G) * wt[p] * scale
+    return (out * cell).reshape(pos.shape)
+
+
+# ---------------------------------------------------------------------------- Born
+def simps_weights(a, b, N):
+    """_simps_weights (src/jax_fli/_src/lensing/_born.py:16-26): dx/3 [1, 4, 2, ..., 4, 1]."""
+    dx = (b - a) / N
+    w = np.ones(N + 1)
+    w[1:-1:2] = 4.0
+    w[2:-2:2] = 2.0
+    return w * dx / 3.0
+
+
+def simps(f, a, b, N):
+    """jax_cosmo.scipy.integrate.simps: composite Simpson over N (even) intervals."""
+    x = np.linspace(a, b, N + 1)
+    return float((simps_weights(a, b, N) * f(x)).sum())
+
+
+def tophat_pdf(zmin, zmax):
+    """tophat_z (src/jax_fli/power/theory.py:28-47) normalised as a jax_cosmo redshift_distribution:
+    pz(z) / simps(pz, 0, 10, 256) (the normalisation is parity unpinned)."""
+    pz = lambda z: np.where((np.asarray(z) >= zmin) & (np.asarray(z) <= zmax), 1.0, 0.0)
+    norm = simps(pz, 0.0, 10.0, 256)
+    return lambda z: pz(z) / norm
+
+
+def born_core(maps, r, a, z_source, d_r, Om, chi_fn):
+    """_born_core_impl (src/jax_fli/_src/lensing/_born.py:29-97), spherical maps (S, npix), in float64:
+    delta = rho / mean - 1 (mean 0 -> float32 eps), kappa(z) = 3/2 Om (H0/c)^2 sum_s delta_s d_r
+    r_s / a_s clip(1 - r_s / chi(z), 0, 1000). Returns (nz, npix)."""
+    maps = np.asarray(maps, dtype=np.float64)
+    r = np.asarray(r, dtype=np.float64).reshape(-1)
+    a = np.asarray(a, dtype=np.float64).reshape(-1)
+    zs = np.atleast_1d(np.asarray(z_source, dtype=np.float64))
+    const = 1.5 * Om * H0_OVER_C**2
+    chi_s = np.asarray(chi_fn(1.0 / (1.0 + zs)), dtype=np.float64)
+    mean = maps.mean(axis=1, keepdims=True)
+    mean = np.where(mean == 0, FLT_EPS, mean)
+    contrib = (maps / mean - 1.0) * (d_r * r / a)[:, None] * const
+    eff = np.clip(1.0 - r[:, None] / chi_s[None, :], 0.0, 1000.0)  # (S, nz)
+    return eff.T @ contrib
+
+
+def born_spherical(maps, r, a, sources, d_r, Om, chi_fn, min_z=0.01, max_z=1.5, n_integrate=32):
+    """_born_spherical (_born.py:100-127): distributions -> Simpson over z_grid weighted by n(z);
+    scalars -> one map per source redshift."""
+    if callable(sources[0]):
+        z = np.linspace(min_z, max_z, n_integrate + 1)
+        kg = born_core(maps, r, a, z, d_r, Om, chi_fn)
+        w = simps_weights(min_z, max_z, n_integrate)
+        nzw = np.stack([nz(z) for nz in sources])
+        return (nzw * w[None, :]) @ kg
+    return born_core(maps, r, a, np.asarray(sources, dtype=np.float64), d_r, Om, chi_fn)
+
+
+def born_vjp(maps, kappa_bar, r, a, sources, d_r, Om, chi_fn, **kw):
+    """d<kappa_bar, born_spherical(maps)>/d maps by the chain rule through the means (float64)."""
+    maps = np.asarray(maps, dtype=np.float64)
+    S, npix = maps.shape
+    kb = np.asarray(kappa_bar, dtype=np.float64)
+    # born is affine in delta: C[k, s] = d kappa_k / d delta_s, read off two-pixel maps whose shell s
+    # has delta = (+1, -1) and the others delta = 0
+    C = np.zeros((kb.shape[0], S))
+    for s in range(S):
+        m2 = np.ones((S, 2))
+        m2[s] = (2.0, 0.0)
+        C[:, s] = born_spherical(m2, r, a, sources, d_r, Om, chi_fn, **kw)[:, 0]
+    mean = maps.mean(axis=1)
+    out = np.zeros_like(maps)
+    for s in range(S):
+        g = C[:, s] @ kb  # d L / d delta_s, per pixel
+        if mean[s] == 0:
+            out[s] = g / FLT_EPS
+        else:
+            out[s] = g / mean[s] - (g * maps[s]).sum() / (mean[s] ** 2 * npix)
+    return out

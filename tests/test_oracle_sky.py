@@ -1,0 +1,176 @@
+"""Pins of the sky oracle (HEALPix shell painting over box images, its adjoint, Born convergence):
+each restated function is checked against what mathematics fixes, chosen so that a dropped term, a
+wrong sign, index or axis fails one of them. The reference commits no golden vectors for lensing or
+spherical painting."""
+
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+import pytest
+from scipy import integrate as si
+
+from oracle import nbody_oracle as nbo
+from oracle import pm_oracle as orc
+from oracle import sky_oracle as so
+
+
+# ---------------------------------------------------------------- images
+def test_cube_symmetries_are_the_48_signed_permutations():
+    mats = [so.cube_symmetry(c) for c in range(48)]
+    assert np.array_equal(mats[0], np.eye(3))
+    keys = {m.tobytes() for m in mats}
+    assert len(keys) == 48
+    for m in mats:
+        assert np.allclose(m @ m.T, np.eye(3))
+        assert np.array_equal(np.abs(m).sum(axis=0), np.ones(3)) and np.array_equal(np.abs(m).sum(axis=1), np.ones(3))
+    # enumeration order: signs outer (product([1, -1])), permutations inner (lexicographic)
+    assert np.array_equal(mats[1], np.array([[1, 0, 0], [0, 0, 1], [0, 1, 0]]))
+    assert np.array_equal(mats[6], np.diag([1, 1, -1]))
+    assert np.array_equal(mats[6 * 7 + 5], -np.array([[0, 0, 1], [0, 1, 0], [1, 0, 0]]))
+
+
+def test_onion_shifts():
+    box = (100.0, 200.0, 300.0)
+    sh = so.onion_shifts((0.5, 0.5, 0.5), box)
+    assert sh.shape == (27, 3) and len({tuple(s) for s in sh}) == 27
+    assert np.array_equal(sh[0], np.zeros(3)) and np.array_equal(sh[1], [0, 0, 300.0])
+    assert np.array_equal(sh[2], [0, 0, -300.0])
+    sh0 = so.onion_shifts((0.0, 1.0, 0.5), box)
+    assert (sh0[:, 0] >= 0).all() and (sh0[:, 1] <= 0).all()
+    assert len(so.onion_shifts((0.5, 0.5, 0.5), box, layers=2)) == 125
+
+
+# ---------------------------------------------------------------- tiled shell paint
+def test_single_identity_image_is_the_plain_paint():
+    rng = np.random.default_rng(3)
+    dx = (rng.standard_normal((8, 8, 8, 3)) * 0.8).astype(np.float32)
+    args = ((8, 8, 8), (100.0, 100.0, 100.0), (50.0, 50.0, 50.0), 10.0, 45.0, 4)
+    for scheme in ("ngp", "bilinear"):
+        a = so.paint_healpix_tiled(dx, *args, scheme=scheme, tiles=[(0, (0.0, 0.0, 0.0))])
+        b = orc.paint_healpix(dx, *args, scheme=scheme)
+        assert np.allclose(a, b.astype(np.float64), rtol=1e-6, atol=1e-9)
+
+
+def test_periodic_images_fill_a_far_shell_with_the_mean_density():
+    """A uniform lattice replicated over [-2, 2]^3 boxes: a shell between 1.6 and 2.2 box lengths
+    holds the mean density n = N / L^3, i.e. the map's pixel mean (count per pixel volume)."""
+    n, L = 8, 100.0
+    tiles = [(0, np.array(s, dtype=np.float64) * L) for s in itertools.product(range(-2, 3), repeat=3)]
+    m = so.paint_healpix_tiled(np.zeros((n, n, n, 3), np.float32), (n, n, n), (L, L, L), (50.0, 50.0, 50.0),
+                               160.0, 220.0, 2, scheme="ngp", tiles=tiles)
+    assert m.mean() == pytest.approx(n**3 / L**3, rel=0.02)
+
+
+def test_rotated_image_of_a_symmetric_set_is_unchanged():
+    """The lattice about a lattice-point observer is invariant under every cube symmetry."""
+    n, L = 8, 80.0
+    base = so.paint_healpix_tiled(np.zeros((n, n, n, 3), np.float32), (n, n, n), (L, L, L), (40.0, 40.0, 40.0),
+                                  5.0, 35.0, 2, scheme="ngp")
+    for code in (1, 17, 46):
+        rot = so.paint_healpix_tiled(np.zeros((n, n, n, 3), np.float32), (n, n, n), (L, L, L), (40.0, 40.0, 40.0),
+                                     5.0, 35.0, 2, scheme="ngp", tiles=[(code, np.zeros(3))])
+        assert rot.sum() == pytest.approx(base.sum(), rel=1e-12)
+
+
+def test_interp_weight_derivatives_match_differences():
+    rng = np.random.default_rng(4)
+    for nside in (1, 4, 16):
+        for theta in list(rng.uniform(0.01, np.pi - 0.01, 12)) + [0.05, np.pi - 0.05]:
+            phi = rng.uniform(0, 2 * np.pi)
+            pix, w, dth, dph = so.healpix_interp_weights_grad(nside, theta, phi)
+            assert np.allclose(w, orc.healpix_interp_weights(nside, theta, phi)[1], atol=1e-14)
+            h = 1e-7
+            for d, (t1, p1, t0, p0) in ((dth, (theta + h, phi, theta - h, phi)), (dph, (theta, phi + h, theta, phi - h))):
+                p_hi, w_hi = orc.healpix_interp_weights(nside, t1, p1)
+                p_lo, w_lo = orc.healpix_interp_weights(nside, t0, p0)
+                if list(p_hi) != list(pix) or list(p_lo) != list(pix):
+                    continue  # a ring or pixel boundary lies within h
+                assert np.allclose((np.array(w_hi) - np.array(w_lo)) / (2 * h), d, atol=1e-6)
+
+
+@pytest.mark.parametrize("tiles", [None, [(0, (0.0, 0.0, 0.0)), (13, (100.0, 0.0, 0.0)), (40, (0.0, -100.0, 100.0))]])
+def test_healpix_vjp_matches_differences(tiles):
+    rng = np.random.default_rng(5)
+    args = ((8, 8, 8), (100.0, 100.0, 100.0), (50.0, 50.0, 50.0), 20.0, 150.0, 4)
+    dx = rng.standard_normal((8, 8, 8, 3)) * 0.7
+    m = so.paint_healpix_tiled(dx, *args, tiles=tiles)
+    mb = rng.standard_normal(m.shape)
+    g = so.paint_healpix_vjp(dx, mb, *args, tiles=tiles)
+    for j in range(2):
+        d = rng.standard_normal(dx.shape)
+        e = 1e-6
+        fd = ((so.paint_healpix_tiled(dx + e * d, *args, tiles=tiles) -
+               so.paint_healpix_tiled(dx - e * d, *args, tiles=tiles)) * mb).sum() / (2 * e)
+        assert (g * d).sum() == pytest.approx(fd, rel=1e-6)
+    assert not so.paint_healpix_vjp(dx, mb, *args, scheme="ngp", tiles=tiles).any()
+
+
+# ---------------------------------------------------------------- Born
+def test_simpson_weights_exact_for_cubics():
+    w = so.simps_weights(0.3, 2.1, 16)
+    z = np.linspace(0.3, 2.1, 17)
+    f = 1 - 2 * z + 0.5 * z**2 + 0.25 * z**3
+    F = lambda x: x - x**2 + x**3 / 6 + x**4 / 16
+    assert (w * f).sum() == pytest.approx(F(2.1) - F(0.3), rel=1e-13)
+
+
+def test_born_thin_planes_converge_to_the_born_integral():
+    """delta(chi, p) = Y(p) on planes of width d_r out to the source: the plane sum is the midpoint
+    rule of 3/2 Om (H0/c)^2 Y int_0^chi_s chi (chi_s - chi) / (chi_s a(chi)) dchi."""
+    c = nbo.Cosmo(Omega_c=0.25, Omega_b=0.0486, h=0.6774)
+    zs = 0.8
+    chi_s = float(c.chi(1 / (1 + zs))[0])
+    S = 400
+    d_r = chi_s / S
+    r = (np.arange(S) + 0.5) * d_r
+    agrid = np.linspace(0.3, 1.0, 4001)
+    chig = c.chi(agrid)
+    a = np.interp(r, chig[::-1], agrid[::-1])
+    Y = np.array([0.3, -0.3, 0.1, -0.1])
+    maps = 1.0 + np.outer(np.ones(S), Y)
+    k = so.born_spherical(maps, r, a, [zs], d_r, c.Omega_m, c.chi)
+    a_of = lambda x: float(np.interp(x, chig[::-1], agrid[::-1]))
+    integral = si.quad(lambda x: x * (chi_s - x) / (chi_s * a_of(x)), 0, chi_s, limit=200)[0]
+    expect = 1.5 * c.Omega_m * so.H0_OVER_C**2 * integral * Y
+    assert np.allclose(k[0], expect, rtol=2e-5)
+
+
+def test_born_is_linear_in_delta_and_ignores_the_mean():
+    c = nbo.Cosmo()
+    rng = np.random.default_rng(6)
+    Y = rng.standard_normal((3, 40))
+    Y -= Y.mean(axis=1, keepdims=True)
+    r, a = np.array([200.0, 400.0, 600.0]), np.array([0.93, 0.87, 0.81])
+    src = [so.tophat_pdf(0.0, 1.0)]
+    k1 = so.born_spherical(1.0 + 0.1 * Y, r, a, src, 100.0, c.Omega_m, c.chi)
+    k2 = so.born_spherical(5.0 * (1.0 + 0.2 * Y), r, a, src, 100.0, c.Omega_m, c.chi)
+    assert np.allclose(k2, 2 * k1, rtol=1e-12, atol=1e-18)
+    assert so.born_spherical(np.ones((3, 40)), r, a, src, 100.0, c.Omega_m, c.chi) == pytest.approx(0.0, abs=1e-18)
+
+
+def test_born_vjp_matches_differences():
+    c = nbo.Cosmo(Omega_c=0.25, Omega_b=0.0486, h=0.6774)
+    rng = np.random.default_rng(7)
+    maps = rng.random((4, 48)) + 0.5
+    maps[2] = 0.0  # an empty shell: the eps mean is a constant
+    r = np.array([100.0, 200.0, 300.0, 400.0])
+    a = 1 / (1 + np.array([0.03, 0.07, 0.1, 0.14]))
+    for src in ([so.tophat_pdf(0, 1), so.tophat_pdf(0.2, 0.6)], [0.5, 1.0]):
+        k = so.born_spherical(maps, r, a, src, 62.5, c.Omega_m, c.chi)
+        kb = rng.standard_normal(k.shape)
+        g = so.born_vjp(maps, kb, r, a, src, 62.5, c.Omega_m, c.chi)
+        d = rng.standard_normal(maps.shape)
+        d[2] = 0.0
+        e = 1e-6
+        fd = ((so.born_spherical(maps + e * d, r, a, src, 62.5, c.Omega_m, c.chi) -
+               so.born_spherical(maps - e * d, r, a, src, 62.5, c.Omega_m, c.chi)) * kb).sum() / (2 * e)
+        assert (g * d).sum() == pytest.approx(fd, rel=1e-7)
+
+
+def test_tophat_normalisation():
+    nz = so.tophat_pdf(0.0, 1.0)
+    z = np.linspace(0, 10, 257)
+    assert (so.simps_weights(0, 10, 256) * nz(z)).sum() == pytest.approx(1.0, rel=1e-12)
+    assert nz(np.array([0.5]))[0] == pytest.approx(nz(np.array([0.9]))[0]) and nz(np.array([1.01]))[0] == 0
