@@ -228,6 +228,100 @@ def bench_vjp(pkg, dev):
                          "sigma8": {"adjoint": float(grads["s8"]), "fd": fd_s8}}}
 
 
+def bench_large(pkg, dev, fd_dirs=2):
+    """Large single-GPU configuration: 512^3 particles (the regular sub-lattice, 2 cells apart) on a
+    1024^3 force mesh, box 1000 Mpc/h, float32; seeded ICs, 1LPT at a = 0.1, ReversibleDoubleKickDrift
+    with dt0 = 0.03 (reverse adjoint); 8 HEALPix shells (nside 512, bilinear) of equal comoving width out
+    to z = 1, the shells beyond the box painted from its periodic images (OnionTiler with layers=2,
+    rotate=False, no drift to the lightcone); Born convergence for one top-hat n(z) on 0 < z < 1. Times
+    the forward (noise -> kappa) and the forward + VJP of sum(kappa^2) w.r.t. the white-noise field.
+    The gradient is also set beside central differences of the float32 forward along two directions
+    with a large smooth response: the noise field itself (an amplitude rescaling) and the gradient's
+    own direction. Differences along a random white-noise direction are not used here: kappa barely
+    responds to small-scale noise, and the jumps of particle-images crossing shell edges (shell
+    membership is a step, whose derivative the adjoint takes as zero, as autodiff does) then swamp the
+    smooth part. The exact check of the same pipeline is the float64 one at 16^3 in the GPU tests."""
+    import torch
+
+    n, npart, box, a0, dt0, nside, nsh, zmax = 1024, 512, 1000.0, 0.1, 0.03, 512, 8, 1.0
+    c = pkg.Cosmology(Omega_c=0.25, sigma8=0.8, Omega_b=0.0486, h=0.6774, n_s=0.9667)
+    chi_max = float(pkg.cosmology.radial_comoving_distance(c, 1.0 / (1.0 + zmax)))
+    edges = np.linspace(0.0, chi_max, nsh + 1)
+    r_c = (0.5 * (edges[1:] + edges[:-1]))[::-1]  # far -> near: ascending time
+    w = np.diff(edges)[::-1]
+    ts = np.stack([pkg.cosmology.a_of_chi(c, r_c - w / 2), pkg.cosmology.a_of_chi(c, r_c + w / 2)])
+    layers = int(math.ceil((chi_max - box / 2) / box))
+    painting = pkg.PaintingOptions(target="spherical", scheme="bilinear")
+    tiler = pkg.OnionTiler(painting=painting, drift_on_lightcone=False, layers=layers, rotate=False)
+    solver = pkg.ReversibleDoubleKickDrift(interp_kernel=tiler)
+    lat = torch.stack(torch.meshgrid(*[torch.arange(npart, dtype=torch.float32, device=dev) * (n // npart)] * 3,
+                                     indexing="ij"), dim=-1)
+    nz = [pkg.tophat_z(0.0, zmax)]
+    info = {}
+
+    def loss(noise):
+        ic = pkg.interpolate_initial_conditions(noise, (n,) * 3, (box,) * 3, cosmo=c, nside=nside)
+        dx, p = pkg.lpt(c, ic, ts=a0, initial_particles=lat)
+        del ic
+        lc = pkg.nbody(c, dx, p, t0=a0, t1=1.0, dt0=dt0, ts=ts, solver=solver, adjoint="reverse")
+        kap = pkg.born(c, lc, nz)
+        info["kappa"] = kap.array
+        return (kap.array.double() ** 2).sum()
+
+    noise0 = pkg.engine.white_noise((n,) * 3, 5)
+
+    def timed(fn, reps):
+        out = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            out.append((time.perf_counter() - t) * 1e3)
+        return float(np.median(out))
+
+    def fwd():
+        with torch.no_grad():
+            return loss(noise0)
+
+    grad = {}
+
+    def fwd_bwd():
+        nzt = noise0.clone().requires_grad_(True)
+        loss(nzt).backward()
+        grad["g"] = nzt.grad
+
+    fwd()  # warm-up: plans, pools
+    fwd_ms = timed(fwd, 2)
+    fwd_bwd()
+    fwd_bwd_ms = timed(fwd_bwd, 1)
+    kap = info.pop("kappa")
+    krms = float(kap.double().pow(2).mean().sqrt())
+    g = grad.pop("g").double()
+    errs = {}
+    eps = 1e-3
+    gn = float(g.norm())
+    for name in ("noise", "gradient")[:fd_dirs]:
+        d = noise0 if name == "noise" else (g * (math.sqrt(g.numel()) / gn)).float()
+        with torch.no_grad():
+            fd = (float(loss(noise0 + eps * d)) - float(loss(noise0 - eps * d))) / (2 * eps)
+        adj = float((g * d.double()).sum())
+        errs[name] = {"adjoint": adj, "fd": fd, "rel_err": round(abs(adj - fd) / abs(fd), 6)}
+        del d
+    nsteps = pkg.pm.count_steps(a0, np.sort(np.asarray(pkg.cosmology.a_of_chi(c, r_c), np.float32)), dt0, 1.0)
+    del g, noise0, lat
+    pkg._lib.release_plan((n,) * 3, dev)
+    torch.cuda.empty_cache()
+    return {"workload": "512^3 particles (sub-lattice) on a 1024^3 mesh, box 1000 Mpc/h, 1LPT a=0.1, "
+                        f"ReversibleDoubleKickDrift dt0=0.03 ({nsteps} steps to the nearest shell), 8 HEALPix shells "
+                        f"nside 512 to z=1 from {(2 * layers + 1) ** 3} periodic images, Born for tophat n(z) 0<z<1",
+            "forward_ms": round(fwd_ms, 1), "forward_plus_vjp_ms": round(fwd_bwd_ms, 1),
+            "vjp_over_forward": round(fwd_bwd_ms / fwd_ms, 2), "steps": nsteps,
+            "kappa_rms": krms, "fd_check": {"eps": eps, **errs},
+            "note": "wall clock including white noise, IC colouring, LPT, the N-body run with its 8 shell paints and "
+                    "Born; the VJP runs the reverse adjoint with the shell-paint and Born adjoints"}
+
+
 def bench_serving(pkg, dev, count=512):
     """Serving workload: `count` forward-simulation requests, mesh uniform over {64, 96, 128, 192}^3
     (box 3.9 N Mpc/h), own seed, Omega_c ~ U(0.2, 0.35), sigma8 ~ U(0.7, 0.9), 1LPT + KDK dt0 = 0.09 to
@@ -259,6 +353,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1, help="CPU-baseline steps at full size (0: skip)")
     ap.add_argument("--no-vjp", action="store_true", help="skip the forward+VJP measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-large", action="store_true", help="skip the 512^3-on-1024^3 forward + VJP measurement")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the per-phase HBM traffic from two rocprofv3 --pmc runs of this script and exit")
     args = ap.parse_args()
@@ -435,6 +530,7 @@ def main():
     del spec
 
     vjp = None if args.no_vjp else bench_vjp(pkg, dev)
+    large = None if args.no_large else bench_large(pkg, dev)
     serving = None if args.no_serving else bench_serving(pkg, dev)
 
     # CPU baseline: the NumPy/SciPy restatement of the reference step on the host cores
@@ -487,6 +583,7 @@ def main():
                                 "the schedule's PM steps without the shell paints"},
         "forward_total_ms": round(forward_ms, 1),
         "vjp": vjp,
+        "large": large,
         "serving": serving,
         "fft_roundtrip_ms": {"pmx": round(fft_ours, 4), "rocfft": round(fft_roc, 4)},
         "final_pk": {"k": [round(float(a), 5) for a in k_fin[:8]], "pk": [round(float(a), 3) for a in pk_fin[:8]]},

@@ -197,3 +197,11 @@ def get_plan(mesh_shape, device: torch.device | None = None) -> Plan:
 def clear_plans() -> None:
     torch.cuda.synchronize()
     _plans.clear()
+
+
+def release_plan(mesh_shape, device: torch.device | None = None) -> None:
+    """Drop the cached plan of a mesh shape (its device buffers are freed once no caller holds it)."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    if dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    _plans.pop((dev.index, tuple(int(n) for n in mesh_shape)), None)

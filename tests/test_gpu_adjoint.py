@@ -42,11 +42,14 @@ def _dd(fn, x, d):
     return (fn(x + EPS * d) - fn(x - EPS * d)) / (2 * EPS)
 
 
-@pytest.mark.parametrize("mode", ["relative", "absolute"])
+@pytest.mark.parametrize("mode", ["relative", "absolute", "relative-sublattice"])
 def test_paint_vjp_directional(pkg, cuda, mode):
     n = 16
     mesh = (n, n, n)
-    if mode == "relative":
+    if mode == "relative-sublattice":
+        pos = _disp((8, 8, 8), 1.3, 31)
+        mode = "relative"
+    elif mode == "relative":
         pos = _disp(mesh, 1.3, 31)
     else:
         pos = (np.random.default_rng(31).random((n, n, n, 3)) * n).astype(np.float32)
@@ -112,13 +115,15 @@ def test_forces_vjp_directional(pkg, cuda, mode):
     assert _rel(float(dot.cpu()), ref_dot) < 1e-5
 
 
-def test_step_reverse_vjp(pkg, cuda):
+@pytest.mark.parametrize("npart", [32, 16])
+def test_step_reverse_vjp(pkg, cuda, npart):
     """One KKD step (v += F ck; x += v cd): the reverse restores the state and the cotangents match
-    the float64 directional derivative of L = <xb, x'> + <vb, v'>; coefficient cotangents too."""
+    the float64 directional derivative of L = <xb, x'> + <vb, v'>; coefficient cotangents too. npart
+    16: particles on the sub-lattice two cells apart (the 512^3-on-1024^3 set-up)."""
     n = 32
     mesh = (n, n, n)
-    x = _disp(mesh, 1.2, 38)
-    v = _disp(mesh, 0.4, 39)
+    x = _disp((npart,) * 3, 1.2, 38)
+    v = _disp((npart,) * 3, 0.4, 39)
     fs, ck, cd = 0.45, 0.7, 0.3
     rng = np.random.default_rng(40)
     xb = rng.standard_normal(x.shape).astype(np.float32)
