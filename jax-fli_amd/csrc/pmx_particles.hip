@@ -858,6 +858,113 @@ __global__ void __launch_bounds__(256) k_readout(PartArgs a, const float* __rest
   }
 }
 
+// Readout staged through LDS. A workgroup takes an RB_I x RB_J x RB_K Lagrangian block of particles
+// and bounds the cells of their CIC corners (unwrapped cell coordinates). When that box holds at
+// most RO_CAP cells it is loaded into LDS with z-contiguous reads (each mesh cell once per
+// workgroup instead of once per corner that touches it) and the 8 corners are gathered from LDS;
+// larger boxes (shell crossings, wide spreads) gather from global memory as k_readout does. The
+// weights and the summation order are those of k_readout: the results are bitwise identical.
+constexpr int RB_I = 4, RB_J = 8, RB_K = 8, RO_CAP = 1664;
+
+__device__ __forceinline__ int base_cell(int mode, int lattice, float xc) {
+  return (mode == 0 ? lattice : 0) + (int)floorf(xc);
+}
+
+template <int NC>
+__global__ void __launch_bounds__(256) k_readout_lds(PartArgs a, const float* __restrict__ mesh, float* __restrict__ out,
+                                                     float ck, float cd, int what, int NX, int NY, int NZ, int rx,
+                                                     int ry, int rz, int nbj, int nbk) {
+  __shared__ float sm[RO_CAP * NC];
+  __shared__ int red[6][4];
+  const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
+  const int bk = (int)(b % (unsigned)nbk), bj = (int)((b / (unsigned)nbk) % (unsigned)nbj),
+            bi = (int)(b / ((unsigned)nbk * nbj));
+  const int t = threadIdx.x;
+  const int i = bi * RB_I + t / (RB_J * RB_K), j = bj * RB_J + (t / RB_K) % RB_J, k = bk * RB_K + t % RB_K;
+  const unsigned p = ((unsigned)i * a.npy + j) * a.npz + k;
+  float3 x = ld3(a.x, p);
+  Corner1D cx, cy, cz;
+  if (a.mode == 0) {
+    cx = corner_rel(i * rx, x.x, NX);
+    cy = corner_rel(j * ry, x.y, NY);
+    cz = corner_rel(k * rz, x.z, NZ);
+  } else {
+    cx = corner_abs(x.x, NX);
+    cy = corner_abs(x.y, NY);
+    cz = corner_abs(x.z, NZ);
+  }
+  const int b0[3] = {base_cell(a.mode, i * rx, x.x), base_cell(a.mode, j * ry, x.y), base_cell(a.mode, k * rz, x.z)};
+  int mn[3] = {b0[0], b0[1], b0[2]}, mx[3] = {b0[0], b0[1], b0[2]};
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      mn[c] = min(mn[c], __shfl_xor(mn[c], o));
+      mx[c] = max(mx[c], __shfl_xor(mx[c], o));
+    }
+  const int wv = t >> 6;
+  if ((t & 63) == 0)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      red[c][wv] = mn[c];
+      red[3 + c][wv] = mx[c];
+    }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    mn[c] = min(min(red[c][0], red[c][1]), min(red[c][2], red[c][3]));
+    mx[c] = max(max(red[3 + c][0], red[3 + c][1]), max(red[3 + c][2], red[3 + c][3])) + 1;  // upper corner
+  }
+  const long long ex = (long long)mx[0] - mn[0] + 1, ey = (long long)mx[1] - mn[1] + 1,
+                  ez = (long long)mx[2] - mn[2] + 1;
+  float f[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) f[c] = 0.f;
+  if (ex * ey * ez <= RO_CAP) {
+    const int EY = (int)ey, EZ = (int)ez, ncell = (int)(ex * ey * ez);
+    for (int q = t; q < ncell; q += 256) {
+      const int qz = q % EZ, qy = (q / EZ) % EY, qx = q / (EZ * EY);
+      const unsigned g = ((unsigned)wrap_idx(mn[0] + qx, NX) * NY + (unsigned)wrap_idx(mn[1] + qy, NY)) * NZ +
+                         (unsigned)wrap_idx(mn[2] + qz, NZ);
+      const float* m = mesh + (size_t)g * NC;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) sm[q * NC + c] = m[c];
+    }
+    __syncthreads();
+    const int lx = b0[0] - mn[0], ly = b0[1] - mn[1], lz = b0[2] - mn[2];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int ox = o & 1, oy = (o >> 1) & 1, oz = (o >> 2) & 1;
+      const float w = ((ox ? cx.w1 : cx.w0) * (oy ? cy.w1 : cy.w0)) * (oz ? cz.w1 : cz.w0);
+      const float* m = sm + (((lx + ox) * EY + (ly + oy)) * EZ + (lz + oz)) * NC;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) f[c] = f[c] + m[c] * w;
+    }
+  } else {
+    const Cic q = cic_from(cx, cy, cz, NY, NZ);
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const float* m = mesh + (size_t)q.off[o] * NC;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) f[c] = f[c] + m[c] * q.w[o];
+    }
+  }
+  if (what & 1) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) out[(size_t)p * NC + c] = f[c];
+  }
+  if constexpr (NC == 3) {
+    if (what & 2) {
+      float3 v = ld3(a.v, p);
+      v = axpy3(v, make_float3(f[0], f[1], f[2]), ck);
+      st3(a.v, p, v);
+      if (what & 4) {
+        x = axpy3(x, v, cd);
+        st3(a.x, p, x);
+      }
+    }
+  }
+}
+
 __global__ void k_finalize_map(const float* __restrict__ acc, float* __restrict__ map, long n, double scale,
                                int accumulate) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
@@ -1168,6 +1275,21 @@ int pmx_launch_readout(pmx_plan_t* p, const PartArgs& a, const float* mesh, int 
   int rx, ry, rz;
   ratios(p, a, rx, ry, rz);
   const size_t np = (size_t)a.npx * a.npy * a.npz;
+  // LDS staging pays when a block's corner box is small: whole blocks, and a lattice spacing of one
+  // cell (at two cells apart an 4x8x8 block alone spans 8x16x16 cells)
+  const bool staged = a.npx % RB_I == 0 && a.npy % RB_J == 0 && a.npz % RB_K == 0 &&
+                      (RB_I * rx + 2) * (RB_J * ry + 2) * (RB_K * rz + 2) * 2 <= RO_CAP;
+  if (staged) {
+    const int nbj = a.npy / RB_J, nbk = a.npz / RB_K;
+    const unsigned G = (unsigned)(np / (RB_I * RB_J * RB_K));
+    if (ncomp == 3)
+      hipLaunchKernelGGL(k_readout_lds<3>, dim3(G), dim3(256), 0, s, a, mesh, out, ck, cd_post, what, p->nx, p->ny,
+                         p->nz, rx, ry, rz, nbj, nbk);
+    else
+      hipLaunchKernelGGL(k_readout_lds<1>, dim3(G), dim3(256), 0, s, a, mesh, out, ck, cd_post, what & 1, p->nx,
+                         p->ny, p->nz, rx, ry, rz, nbj, nbk);
+    return pmx_check_launch("readout");
+  }
   if (ncomp == 3)
     hipLaunchKernelGGL(k_readout<3>, dim3(grid_particles(np)), dim3(256), 0, s, a, mesh, out, ck, cd_post, what, p->nx,
                        p->ny, p->nz, rx, ry, rz);

@@ -66,6 +66,31 @@ def test_readout(pkg, cuda, ncomp, mode):
     assert rel(got.cpu().numpy(), ref) < 1e-6
 
 
+@pytest.mark.parametrize("ncomp", [1, 3])
+def test_readout_lds_staged_and_global_blocks(pkg, cuda, ncomp):
+    """A smooth displacement field (whose 4x8x8 particle blocks fit the LDS-staged readout) with one
+    slab of wildly scattered particles (whose blocks take the global-gather path), relative and
+    absolute, and the fused kick + drift of the step on the same field."""
+    n = 64
+    rng = np.random.default_rng(41)
+    ii = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).astype(np.float64)
+    dx = 2.5 * np.sin(2 * np.pi * ii[..., ::-1] / n + np.arange(3)) + 0.2 * rng.standard_normal(ii.shape)
+    dx[8:12] = rng.standard_normal((4, n, n, 3)) * 9.0
+    dx = dx.astype(np.float32)
+    mesh = rng.standard_normal((n, n, n) + ((3,) if ncomp == 3 else ())).astype(np.float32)
+    for mode, pos in (("relative", dx), ("absolute", (dx + ii).astype(np.float32))):
+        got = pkg.engine.readout(torch.from_numpy(mesh).to(cuda), torch.from_numpy(pos).to(cuda), mode=mode)
+        assert rel(got.cpu().numpy(), orc.cic_read(mesh, pos, mode)) < 1e-6, mode
+    if ncomp == 3:
+        v = rng.standard_normal(dx.shape).astype(np.float32)
+        xd, vd = torch.from_numpy(dx).to(cuda), torch.from_numpy(v).to(cuda)
+        pkg.engine.step(xd, vd, (n, n, n), force_scale=0.45, c_kick=0.3, c_drift_post=0.25)
+        f = orc.pm_forces(dx, (n, n, n), scale=0.45, dtype=np.float64)
+        v2 = v + f * 0.3
+        assert rel(vd.cpu().numpy(), v2) < 1e-5
+        assert rel(xd.cpu().numpy(), dx + v2 * 0.25) < 1e-6
+
+
 @pytest.mark.parametrize("shape", [(8, 8, 8), (16, 32, 8), (32, 32, 64), (64, 64, 64), (96, 96, 96), (128, 64, 192),
                                    (256, 16, 512), (16, 1024, 16)])
 def test_rfft_matches_numpy(pkg, cuda, shape):
