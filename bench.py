@@ -35,14 +35,14 @@ CAL_FLOATS = 128 << 20  # calibration copy for the HBM-traffic counters: 512 MiB
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pm_step_256cubed_pmc_traffic.txt")
 # kernels of each pmx_step phase (substring of the demangled name), for the rocprofv3 counter summary
 PHASE_KERNELS = {
-    "paint": ("k_sort_count", "k_sort_scan", "k_sort_scatter", "k_paint_deposit"),
+    "paint": ("k_sort_count", "k_sort_scan", "k_sort_scatter", "k_paint_deposit", "k_paint_faces"),
     "r2c_z": ("k_r2c_z",),
     "c2c_y": ("k_c2c_lines<256, false>",),
     "kspace_x": ("k_xpass<256, 5>",),
     "c2c_y_inv": ("k_c2c_lines<256, true>",),
     "c2r_z": ("k_c2r_z<256, 1>",),
     "grad4": ("k_grad4",),
-    "readout_kick_drift": ("k_readout<3>",),
+    "readout_kick_drift": ("k_readout_lds<3>",),
 }
 
 

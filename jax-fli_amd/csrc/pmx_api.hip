@@ -159,11 +159,10 @@ int force_mesh_from(pmx_plan_t* p, const float* src, float scale, unsigned flags
   return pmx_pass_c2r_z(p, 3, p->specD, out, s);
 }
 
-// Force mesh through the potential: phi = ifft(delta_hat * force_scale / (12 k^2)) (one inverse
-// FFT instead of three), then F = 8 (phi[+1] - phi[-1]) - (phi[+2] - phi[-2]) per axis, which is the
-// reference's k-space gradient i (8 sin k - sin 2k) / 6 exactly. `src` may be p->rho (phi reuses it).
-int pot_force_mesh_from(pmx_plan_t* p, const float* src, float force_scale, unsigned flags, float* out, hipStream_t s,
-                        StepMarks* m = nullptr) {
+// The potential: p->rho = G src, G = ifft((force_scale / (12 N k^2)) fft) (optionally
+// CIC-compensated), a real symmetric operator, so it is also its own adjoint. `src` may be p->rho.
+int pot_only(pmx_plan_t* p, const float* src, float force_scale, unsigned flags, hipStream_t s,
+             StepMarks* m = nullptr) {
   PMX_TRY(pmx_pass_r2c_z(p, src, p->specA, s));
   mark(m, 2, s);
   PMX_TRY(pmx_pass_c2c_y(p, p->specA, p->specB, s));
@@ -178,21 +177,16 @@ int pot_force_mesh_from(pmx_plan_t* p, const float* src, float force_scale, unsi
   mark(m, 5, s);
   PMX_TRY(pmx_pass_c2r_z(p, 1, p->specD, p->rho, s));
   mark(m, 6, s);
-  return pmx_grad4(p, p->rho, out, s);
+  return 0;
 }
 
-// The potential half of pot_force_mesh_from alone: p->rho = G src, G = ifft((force_scale / (12 N k^2)) fft)
-// (optionally CIC-compensated), a real symmetric operator, so it is also its own adjoint.
-int pot_only(pmx_plan_t* p, const float* src, float force_scale, unsigned flags, hipStream_t s) {
-  PMX_TRY(pmx_pass_r2c_z(p, src, p->specA, s));
-  PMX_TRY(pmx_pass_c2c_y(p, p->specA, p->specB, s));
-  KOp op{};
-  op.mode = XP_POT;
-  op.scale = (float)((double)force_scale / (12.0 * (double)p->ncell));
-  op.flags = flags & PMX_DECONV;
-  PMX_TRY(pmx_xpass(p, op, p->specB, p->specC, s));
-  PMX_TRY(pmx_pass_c2c_y_inv(p, 1, p->specC, p->specD, s));
-  return pmx_pass_c2r_z(p, 1, p->specD, p->rho, s);
+// Force mesh through the potential: phi = ifft(delta_hat * force_scale / (12 k^2)) (one inverse
+// FFT instead of three), then F = 8 (phi[+1] - phi[-1]) - (phi[+2] - phi[-2]) per axis, which is the
+// reference's k-space gradient i (8 sin k - sin 2k) / 6 exactly.
+int pot_force_mesh_from(pmx_plan_t* p, const float* src, float force_scale, unsigned flags, float* out, hipStream_t s,
+                        StepMarks* m = nullptr) {
+  PMX_TRY(pot_only(p, src, force_scale, flags, s, m));
+  return pmx_grad4(p, p->rho, out, s);
 }
 
 // Force adjoint at the last sorted particle set `a` (whose force mesh is in p->fmesh):

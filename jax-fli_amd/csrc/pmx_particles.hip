@@ -866,6 +866,16 @@ __global__ void __launch_bounds__(256) k_readout(PartArgs a, const float* __rest
 // weights and the summation order are those of k_readout: the results are bitwise identical.
 constexpr int RB_I = 4, RB_J = 8, RB_K = 8, RO_CAP = 1664;
 
+// (q / e, q % e) for the box indices of the staging loops, without an integer division:
+// q = a e + r, 0 <= r < e, puts (q + 1/2) / e at least 1/(2e) away from an integer, and the float
+// product (q + 1/2) * fl(1/e) is within about 2^-23 (q + 1) / e of it, inside that margin for any
+// q < 2^20 (the boxes hold at most a few thousand cells).
+__device__ __forceinline__ int div_small(int q, float inv_e, int e, int& rem) {
+  const int a = (int)(((float)q + 0.5f) * inv_e);
+  rem = q - a * e;
+  return a;
+}
+
 __device__ __forceinline__ int base_cell(int mode, int lattice, float xc) {
   return (mode == 0 ? lattice : 0) + (int)floorf(xc);
 }
@@ -921,8 +931,10 @@ __global__ void __launch_bounds__(256) k_readout_lds(PartArgs a, const float* __
   for (int c = 0; c < NC; ++c) f[c] = 0.f;
   if (ex * ey * ez <= RO_CAP) {
     const int EY = (int)ey, EZ = (int)ez, ncell = (int)(ex * ey * ez);
+    const float iy = 1.0f / (float)EY, iz = 1.0f / (float)EZ;
     for (int q = t; q < ncell; q += 256) {
-      const int qz = q % EZ, qy = (q / EZ) % EY, qx = q / (EZ * EY);
+      int qz, qy;
+      const int qx = div_small(div_small(q, iz, EZ, qz), iy, EY, qy);
       const unsigned g = ((unsigned)wrap_idx(mn[0] + qx, NX) * NY + (unsigned)wrap_idx(mn[1] + qy, NY)) * NZ +
                          (unsigned)wrap_idx(mn[2] + qz, NZ);
       const float* m = mesh + (size_t)g * NC;
