@@ -880,45 +880,71 @@ __device__ __forceinline__ int base_cell(int mode, int lattice, float xc) {
   return (mode == 0 ? lattice : 0) + (int)floorf(xc);
 }
 
+// Wave-wide min / max through DPP lane moves (quad swaps, row mirrors, row broadcasts 15 and 31):
+// lane 63 ends with the result of all 64 lanes, which every lane reads back. All lanes must be active.
+template <bool MAX>
+__device__ __forceinline__ int wave_minmax(int v) {
+  auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));  // row_mirror
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));  // row_bcast:15 into rows 1, 3
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));  // row_bcast:31 into rows 2, 3
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+// base cell (unwrapped) and the two CIC weights along one axis, with the expressions of corner_rel
+// (relative mode) and corner_abs (absolute mode)
+struct Axis1D {
+  int b;
+  float w0, w1;
+};
+__device__ __forceinline__ Axis1D axis_of(int mode, int lattice, float xc) {
+  const float fl = floorf(xc);
+  Axis1D r;
+  if (mode == 0) {
+    const float fr = xc - fl;
+    r.b = lattice + (int)fl;
+    r.w0 = 1.0f - fabsf(fr);
+    r.w1 = 1.0f - fabsf(fr - 1.0f);
+  } else {
+    r.b = (int)fl;
+    r.w0 = 1.0f - fabsf(xc - fl);
+    r.w1 = 1.0f - fabsf(xc - (fl + 1.0f));
+  }
+  return r;
+}
+
 template <int NC>
 __global__ void __launch_bounds__(256) k_readout_lds(PartArgs a, const float* __restrict__ mesh, float* __restrict__ out,
                                                      float ck, float cd, int what, int NX, int NY, int NZ, int rx,
-                                                     int ry, int rz, int nbj, int nbk) {
+                                                     int ry, int rz, int nbj, int nbk, FDiv dj, FDiv dk) {
   __shared__ float sm[RO_CAP * NC];
   __shared__ int red[6][4];
   const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
-  const int bk = (int)(b % (unsigned)nbk), bj = (int)((b / (unsigned)nbk) % (unsigned)nbj),
-            bi = (int)(b / ((unsigned)nbk * nbj));
+  const unsigned bq = fdiv(b, dk), bi_ = fdiv(bq, dj);
+  const int bk = (int)(b - bq * (unsigned)nbk), bj = (int)(bq - bi_ * (unsigned)nbj), bi = (int)bi_;
   const int t = threadIdx.x;
   const int i = bi * RB_I + t / (RB_J * RB_K), j = bj * RB_J + (t / RB_K) % RB_J, k = bk * RB_K + t % RB_K;
   const unsigned p = ((unsigned)i * a.npy + j) * a.npz + k;
   float3 x = ld3(a.x, p);
-  Corner1D cx, cy, cz;
-  if (a.mode == 0) {
-    cx = corner_rel(i * rx, x.x, NX);
-    cy = corner_rel(j * ry, x.y, NY);
-    cz = corner_rel(k * rz, x.z, NZ);
-  } else {
-    cx = corner_abs(x.x, NX);
-    cy = corner_abs(x.y, NY);
-    cz = corner_abs(x.z, NZ);
-  }
-  const int b0[3] = {base_cell(a.mode, i * rx, x.x), base_cell(a.mode, j * ry, x.y), base_cell(a.mode, k * rz, x.z)};
-  int mn[3] = {b0[0], b0[1], b0[2]}, mx[3] = {b0[0], b0[1], b0[2]};
-  for (int o = 32; o > 0; o >>= 1)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      mn[c] = min(mn[c], __shfl_xor(mn[c], o));
-      mx[c] = max(mx[c], __shfl_xor(mx[c], o));
-    }
+  const Axis1D ax = axis_of(a.mode, i * rx, x.x), ay = axis_of(a.mode, j * ry, x.y), az = axis_of(a.mode, k * rz, x.z);
   const int wv = t >> 6;
-  if ((t & 63) == 0)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      red[c][wv] = mn[c];
-      red[3 + c][wv] = mx[c];
+  {
+    const int r0 = wave_minmax<false>(ax.b), r1 = wave_minmax<false>(ay.b), r2 = wave_minmax<false>(az.b);
+    const int r3 = wave_minmax<true>(ax.b), r4 = wave_minmax<true>(ay.b), r5 = wave_minmax<true>(az.b);
+    if ((t & 63) == 0) {
+      red[0][wv] = r0;
+      red[1][wv] = r1;
+      red[2][wv] = r2;
+      red[3][wv] = r3;
+      red[4][wv] = r4;
+      red[5][wv] = r5;
     }
+  }
   __syncthreads();
+  int mn[3], mx[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     mn[c] = min(min(red[c][0], red[c][1]), min(red[c][2], red[c][3]));
@@ -932,32 +958,49 @@ __global__ void __launch_bounds__(256) k_readout_lds(PartArgs a, const float* __
   if (ex * ey * ez <= RO_CAP) {
     const int EY = (int)ey, EZ = (int)ez, ncell = (int)(ex * ey * ez);
     const float iy = 1.0f / (float)EY, iz = 1.0f / (float)EZ;
+    // the box origin wrapped once; inside the box one conditional subtraction wraps each axis when
+    // the box is no longer than the mesh, an exact modulo otherwise (meshes smaller than the box)
+    const int ox = wrap_cell(mn[0], NX), oy = wrap_cell(mn[1], NY), oz = wrap_cell(mn[2], NZ);
+    const bool short_box = ex <= NX && ey <= NY && ez <= NZ;
     for (int q = t; q < ncell; q += 256) {
       int qz, qy;
       const int qx = div_small(div_small(q, iz, EZ, qz), iy, EY, qy);
-      const unsigned g = ((unsigned)wrap_idx(mn[0] + qx, NX) * NY + (unsigned)wrap_idx(mn[1] + qy, NY)) * NZ +
-                         (unsigned)wrap_idx(mn[2] + qz, NZ);
-      const float* m = mesh + (size_t)g * NC;
+      int gx = ox + qx, gy = oy + qy, gz = oz + qz;
+      if (short_box) {
+        gx -= gx >= NX ? NX : 0;
+        gy -= gy >= NY ? NY : 0;
+        gz -= gz >= NZ ? NZ : 0;
+      } else {
+        gx %= NX;
+        gy %= NY;
+        gz %= NZ;
+      }
+      const float* m = mesh + (size_t)(((unsigned)gx * NY + (unsigned)gy) * NZ + (unsigned)gz) * NC;
 #pragma unroll
       for (int c = 0; c < NC; ++c) sm[q * NC + c] = m[c];
     }
     __syncthreads();
-    const int lx = b0[0] - mn[0], ly = b0[1] - mn[1], lz = b0[2] - mn[2];
+    const int lx = ax.b - mn[0], ly = ay.b - mn[1], lz = az.b - mn[2];
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
-      const int ox = o & 1, oy = (o >> 1) & 1, oz = (o >> 2) & 1;
-      const float w = ((ox ? cx.w1 : cx.w0) * (oy ? cy.w1 : cy.w0)) * (oz ? cz.w1 : cz.w0);
-      const float* m = sm + (((lx + ox) * EY + (ly + oy)) * EZ + (lz + oz)) * NC;
+      const int ox_ = o & 1, oy_ = (o >> 1) & 1, oz_ = (o >> 2) & 1;
+      const float w = ((ox_ ? ax.w1 : ax.w0) * (oy_ ? ay.w1 : ay.w0)) * (oz_ ? az.w1 : az.w0);
+      const float* m = sm + (((lx + ox_) * EY + (ly + oy_)) * EZ + (lz + oz_)) * NC;
 #pragma unroll
       for (int c = 0; c < NC; ++c) f[c] = f[c] + m[c] * w;
     }
   } else {
-    const Cic q = cic_from(cx, cy, cz, NY, NZ);
+    const int x0 = wrap_idx(ax.b, NX), x1 = wrap_idx(ax.b + 1, NX), y0 = wrap_idx(ay.b, NY),
+              y1 = wrap_idx(ay.b + 1, NY), z0 = wrap_idx(az.b, NZ), z1 = wrap_idx(az.b + 1, NZ);
+    const unsigned nyz = (unsigned)NY * NZ;
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
-      const float* m = mesh + (size_t)q.off[o] * NC;
+      const int ox_ = o & 1, oy_ = (o >> 1) & 1, oz_ = (o >> 2) & 1;
+      const float w = ((ox_ ? ax.w1 : ax.w0) * (oy_ ? ay.w1 : ay.w0)) * (oz_ ? az.w1 : az.w0);
+      const unsigned off = (unsigned)(ox_ ? x1 : x0) * nyz + (unsigned)(oy_ ? y1 : y0) * NZ + (unsigned)(oz_ ? z1 : z0);
+      const float* m = mesh + (size_t)off * NC;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) f[c] = f[c] + m[c] * q.w[o];
+      for (int c = 0; c < NC; ++c) f[c] = f[c] + m[c] * w;
     }
   }
   if (what & 1) {
@@ -1293,13 +1336,14 @@ int pmx_launch_readout(pmx_plan_t* p, const PartArgs& a, const float* mesh, int 
                       (RB_I * rx + 2) * (RB_J * ry + 2) * (RB_K * rz + 2) * 2 <= RO_CAP;
   if (staged) {
     const int nbj = a.npy / RB_J, nbk = a.npz / RB_K;
+    const FDiv dj = make_fdiv((unsigned)nbj), dk = make_fdiv((unsigned)nbk);
     const unsigned G = (unsigned)(np / (RB_I * RB_J * RB_K));
     if (ncomp == 3)
       hipLaunchKernelGGL(k_readout_lds<3>, dim3(G), dim3(256), 0, s, a, mesh, out, ck, cd_post, what, p->nx, p->ny,
-                         p->nz, rx, ry, rz, nbj, nbk);
+                         p->nz, rx, ry, rz, nbj, nbk, dj, dk);
     else
       hipLaunchKernelGGL(k_readout_lds<1>, dim3(G), dim3(256), 0, s, a, mesh, out, ck, cd_post, what & 1, p->nx,
-                         p->ny, p->nz, rx, ry, rz, nbj, nbk);
+                         p->ny, p->nz, rx, ry, rz, nbj, nbk, dj, dk);
     return pmx_check_launch("readout");
   }
   if (ncomp == 3)
