@@ -319,7 +319,7 @@ int pmx_plan_destroy(pmx_plan p) {
   void* bufs[] = {p->tw_x, p->tw_y, p->tw_z, p->tw_zh, p->grad_x, p->grad_y, p->grad_z, p->w2_x, p->w2_y, p->w2_z,
                   p->dec_x, p->dec_y, p->dec_z, p->rho, p->specA, p->specB, p->specC, p->specD, p->fmesh, p->acc,
                   p->kphys_x, p->kphys_y, p->kphys_z, p->ps_count, p->ps_tcount, p->ps_offset, p->ps_cursor, p->ps_wgcnt,
-                  p->ps_seg, p->ps_misc, p->ps_scale, p->ps_faces, p->red};
+                  p->ps_seg, p->ps_pack, p->ps_misc, p->ps_scale, p->ps_faces, p->red};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   clear_marks(p);

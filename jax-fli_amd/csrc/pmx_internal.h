@@ -45,6 +45,7 @@ struct pmx_plan_t {
   // indices; misc = {max |weight| bits, far entries, far entries of the last paint}; fixed-point
   // scale {2^k, 2^-k}
   unsigned *ps_count, *ps_tcount, *ps_offset, *ps_cursor, *ps_wgcnt, *ps_seg, *ps_misc;
+  unsigned* ps_pack;  // per particle: window slot << 16 | rank within the sub-block's slot (count -> scatter)
   float* ps_scale;
   float* ps_faces;  // per tile: its upper face cells (pmx_paint_face_cells floats), 3 blocks for vectors
   double* red;      // per-workgroup partial sums of the adjoint dot products (kRedMax doubles)

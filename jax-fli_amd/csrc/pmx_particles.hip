@@ -355,12 +355,15 @@ __device__ __forceinline__ void for_particles(const PartArgs& a, const SortGeom&
 }
 
 // Pass 1: optional pre-drift x += v * cd (written back), then per-tile particle counts. The window
-// histogram of each sub-block is kept in wgcnt for the scatter pass. far: particles outside windows.
-// With weights (wstride values per particle) the largest |weight| goes to wmax_bits.
+// histogram of each sub-block is kept in wgcnt and each particle's (window slot, rank within the
+// slot) in pack for the scatter pass. far: particles outside windows. With weights (wstride
+// values per particle) the largest |weight| goes to wmax_bits.
+constexpr unsigned SORT_FAR = 0xFFFFFFFFu;  // pack entry of a particle outside its sub-block's window
+
 __global__ void __launch_bounds__(256) k_sort_count(PartArgs a, SortGeom g, float cd, unsigned* __restrict__ count,
                                                     unsigned* __restrict__ wgcnt, unsigned* __restrict__ far,
                                                     const float* __restrict__ w, int wstride,
-                                                    unsigned* __restrict__ wmax_bits) {
+                                                    unsigned* __restrict__ wmax_bits, unsigned* __restrict__ pack) {
   __shared__ unsigned lcnt[SORT_SLOTS];
   __shared__ int3 cen;
   const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
@@ -380,8 +383,9 @@ __global__ void __launch_bounds__(256) k_sort_count(PartArgs a, SortGeom g, floa
   __syncthreads();
   const int3 c = cen;
   unsigned nfar = 0;
-  for_particles(a, g, i0, j0, k0, c, [&](bool on, unsigned, unsigned tile, int slot) {
-    (void)wave_add(on && slot >= 0, slot, lcnt);
+  for_particles(a, g, i0, j0, k0, c, [&](bool on, unsigned p, unsigned tile, int slot) {
+    const unsigned r = wave_add(on && slot >= 0, slot, lcnt);  // rank within the slot's segment
+    if (on) pack[p] = slot >= 0 ? ((unsigned)slot << 16) | r : SORT_FAR;
     if (on && slot < 0) {
       atomicAdd(count + tile, 1u);
       ++nfar;
@@ -451,11 +455,13 @@ __global__ void __launch_bounds__(1024) k_sort_scan(unsigned* __restrict__ count
   }
 }
 
-// Pass 3: each particle's index into its tile's segment.
+// Pass 3: each particle's index into its tile's segment: the segment base the sub-block reserved for
+// the particle's window slot plus the rank the count pass gave it (no re-binning); particles
+// outside the window take one global cursor increment on their tile.
 __global__ void __launch_bounds__(256) k_sort_scatter(PartArgs a, SortGeom g, unsigned* __restrict__ cursor,
-                                                      const unsigned* __restrict__ wgcnt, unsigned* __restrict__ seg) {
+                                                      const unsigned* __restrict__ wgcnt,
+                                                      const unsigned* __restrict__ pack, unsigned* __restrict__ seg) {
   __shared__ unsigned lbase[SORT_SLOTS];
-  __shared__ unsigned lrank[SORT_SLOTS];
   __shared__ int3 cen;
   const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
   int i0, j0, k0;
@@ -466,13 +472,22 @@ __global__ void __launch_bounds__(256) k_sort_scatter(PartArgs a, SortGeom g, un
   for (int s = threadIdx.x; s < SORT_SLOTS; s += blockDim.x) {
     const unsigned n = wgcnt[(size_t)b * SORT_SLOTS + s];
     lbase[s] = n ? atomicAdd(cursor + slot_tile(g, c, s), n) : 0u;
-    lrank[s] = 0u;
   }
   __syncthreads();
-  for_particles(a, g, i0, j0, k0, c, [&](bool on, unsigned p, unsigned tile, int slot) {
-    const unsigned r = wave_add(on && slot >= 0, slot, lrank);
-    if (on) seg[slot >= 0 ? lbase[slot] + r : atomicAdd(cursor + tile, 1u)] = p;
-  });
+  for (int r = 0; r < (SB_X * SB_Y * SB_Z) / 256; ++r) {
+    const int l = r * 256 + (int)threadIdx.x;
+    const int i = i0 + (l >> 9), j = j0 + ((l >> 6) & 7), k = k0 + (l & 63);
+    if (i >= a.npx || j >= a.npy || k >= a.npz) continue;
+    const unsigned p = ((unsigned)i * a.npy + j) * a.npz + k;
+    const unsigned q = pack[p];
+    if (q != SORT_FAR) {
+      seg[lbase[q >> 16] + (q & 0xFFFFu)] = p;
+    } else {
+      const Base bb = base_of(a, g, i, j, k, ld3(a.x, p));
+      const unsigned tile = ((unsigned)(bb.cx >> g.sx) * g.nty + (bb.cy >> g.sy)) * g.ntz + (bb.cz >> g.sz);
+      seg[atomicAdd(cursor + tile, 1u)] = p;
+    }
+  }
 }
 
 // Face cells of a tile grown by one cell on its upper faces (padded local coordinates with at
@@ -1244,9 +1259,11 @@ int pmx_paint_sort(pmx_plan_t* p, const PartArgs& a, float cd_pre, const float* 
   }
   if (p->ps_seg_cap < np) {
     if (p->ps_seg) (void)hipFree(p->ps_seg);
-    p->ps_seg = nullptr;
+    if (p->ps_pack) (void)hipFree(p->ps_pack);
+    p->ps_seg = p->ps_pack = nullptr;
     p->ps_seg_cap = 0;
     hipError_t e = hipMalloc((void**)&p->ps_seg, np * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->ps_pack, np * sizeof(unsigned));
     if (e != hipSuccess) return pmx_set_error(4, std::string("paint sort segments: ") + hipGetErrorString(e));
     p->ps_seg_cap = np;
   }
@@ -1254,7 +1271,7 @@ int pmx_paint_sort(pmx_plan_t* p, const PartArgs& a, float cd_pre, const float* 
   unsigned* far = p->ps_misc + 1;
   unsigned* far_last = p->ps_misc + 2;
   hipLaunchKernelGGL(k_sort_count, dim3((unsigned)nsb), dim3(256), 0, s, a, g, cd_pre, p->ps_count, p->ps_wgcnt, far, w,
-                     wstride, wmax_bits);
+                     wstride, wmax_bits, p->ps_pack);
   int rc = pmx_check_launch("sort_count");
   if (rc) return rc;
   PartArgs b = a;
@@ -1262,7 +1279,8 @@ int pmx_paint_sort(pmx_plan_t* p, const PartArgs& a, float cd_pre, const float* 
   hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, p->ps_count, p->ps_tcount, p->ps_offset, p->ps_cursor,
                      ntiles, w ? -1.0f : fabsf(w0), wmax_bits, p->ps_scale, far, far_last);
   if ((rc = pmx_check_launch("sort_scan"))) return rc;
-  hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)nsb), dim3(256), 0, s, b, g, p->ps_cursor, p->ps_wgcnt, p->ps_seg);
+  hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)nsb), dim3(256), 0, s, b, g, p->ps_cursor, p->ps_wgcnt, p->ps_pack,
+                     p->ps_seg);
   return pmx_check_launch("sort_scatter");
 }
 
