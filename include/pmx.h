@@ -129,6 +129,15 @@ int pmx_lpt1_lattice(pmx_plan plan, const float* ic, int npx, int npy, int npz, 
 int pmx_lpt1_vjp(pmx_plan plan, const float* dx_bar, const float* p_bar, int npx, int npy, int npz, float growth,
                  float vel_factor, float* ic_bar, void* stream);
 
+/* PGD / sharpening displacement S = alpha * read(x, F_pgd), F_pgd the PM force of paint(x) with the
+ * Green's function multiplied by pgd_range(k) = exp(-kl^2 / k^2) exp(-k^4 / ks^4) (0 at k = 0, k in
+ * cell units, the FastPM 4-point gradient): PGDKernel / SharpeningKernel._compute_boost
+ * (src/jax_fli/pm/correction.py:131-333). target 0: x += S (PGDKernel.apply); target 1:
+ * v += S * v_scale (SharpeningKernel: v_scale = +1/full_drift_factor to boost, -1/full_drift_factor
+ * to rewind). */
+int pmx_pgd(pmx_plan plan, float* x, float* v, int npx, int npy, int npz, int mode, float alpha, float kl, float ks,
+            int target, float v_scale, void* stream);
+
 /* HEALPix shell paint: particles with rmin < |x - observer| <= rmax (Mpc/h, periodic box) painted
  * to RING pixels, scheme 0 = ngp, 1 = bilinear (HEALPix ring interpolation weights), map =
  * weighted count / (pixel solid angle * (rmax^3 - rmin^3) / 3). paint_particles_spherical as called

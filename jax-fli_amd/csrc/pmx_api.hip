@@ -438,6 +438,32 @@ int pmx_step(pmx_plan p, float* x, float* v, int npx, int npy, int npz, int mode
   return 0;
 }
 
+int pmx_pgd(pmx_plan p, float* x, float* v, int npx, int npy, int npz, int mode, float alpha, float kl, float ks,
+            int target, float v_scale, void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, x, npx, npy, npz, mode));
+  if (target != 0 && target != 1) return pmx_set_error(1, "target must be 0 (positions) or 1 (velocities)");
+  if (target == 1 && !v) return pmx_set_error(1, "null velocities");
+  if (!(ks > 0.f) || !(kl >= 0.f)) return pmx_set_error(1, "need kl >= 0 and ks > 0");
+  hipStream_t s = (hipStream_t)stream;
+  PartArgs a = part_args(x, nullptr, npx, npy, npz, mode, p);
+  PMX_TRY(pmx_launch_paint(p, a, nullptr, 1.f, 0.f, p->rho, 0, (p->flags & PMX_PAINT_ATOMIC) != 0, s));
+  KOp op{};
+  op.mode = XP_POT;
+  op.scale = (float)((double)alpha / (12.0 * (double)p->ncell));
+  op.flags = 2u;
+  op.pgd_kl2 = kl * kl;
+  op.pgd_ks4 = (ks * ks) * (ks * ks);
+  PMX_TRY(pmx_pass_r2c_z(p, p->rho, p->specA, s));
+  PMX_TRY(pmx_pass_c2c_y(p, p->specA, p->specB, s));
+  PMX_TRY(pmx_xpass(p, op, p->specB, p->specC, s));
+  PMX_TRY(pmx_pass_c2c_y_inv(p, 1, p->specC, p->specD, s));
+  PMX_TRY(pmx_pass_c2r_z(p, 1, p->specD, p->rho, s));
+  PMX_TRY(pmx_grad4(p, p->rho, p->fmesh, s));
+  a.v = v;
+  return pmx_launch_readout(p, a, p->fmesh, 3, nullptr, target == 0 ? 1.f : v_scale, 0.f, target == 0 ? 8 : 2, s);
+}
+
 int pmx_forces_vjp(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, float scale, unsigned flags,
                    const float* forces_bar, float* pos_bar, int accumulate, double* dot, void* stream) {
   PMX_TRY(check_plan(p));

@@ -868,6 +868,8 @@ __global__ void __launch_bounds__(256) k_readout(PartArgs a, const float* __rest
           x = axpy3(x, v, cd);
           st3(a.x, p, x);
         }
+      } else if (what & 8) {  // x += F * ck (a displacement correction)
+        st3(a.x, p, axpy3(x, make_float3(f[0], f[1], f[2]), ck));
       }
     }
   }
@@ -1031,6 +1033,8 @@ __global__ void __launch_bounds__(256) k_readout_lds(PartArgs a, const float* __
         x = axpy3(x, v, cd);
         st3(a.x, p, x);
       }
+    } else if (what & 8) {  // x += F * ck (a displacement correction)
+      st3(a.x, p, axpy3(x, make_float3(f[0], f[1], f[2]), ck));
     }
   }
 }

@@ -128,6 +128,20 @@ def step(x: torch.Tensor, v: torch.Tensor, mesh_shape, mode="relative", force_sc
                           stream_ptr()), "pmx_step")
 
 
+def pgd(x: torch.Tensor, v, mesh_shape, mode="relative", alpha=0.2, kl=0.3, ks=1.0, target="x", v_scale=1.0,
+        plan=None):
+    """In place: x += S(x) (target "x") or v += S(x) * v_scale (target "v"), S = alpha * PGD force."""
+    x = _f32(x, "x")
+    npx, npy, npz = _pgrid(x)
+    plan = plan or get_plan(mesh_shape, x.device)
+    tv = {"x": 0, "v": 1}[target]
+    if tv == 1:
+        v = _f32(v, "v")
+    check(load().pmx_pgd(plan.handle, ptr(x), ptr(v) if tv == 1 else None, npx, npy, npz, MODES[mode], float(alpha),
+                         float(kl), float(ks), tv, float(v_scale), stream_ptr()), "pmx_pgd")
+    return x, v
+
+
 def power_sums(mesh: torch.Tensor, box, kedges: torch.Tensor):
     """Device sums (pk_sum, k_sum, k_cnt) over nedges+1 bins, float64."""
     mesh = _f32(mesh, "mesh")

@@ -64,6 +64,8 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
     if not isinstance(solver, ReversibleDoubleKickDrift):
         raise NotImplementedError("gradients need the reversible solver (ReversibleDoubleKickDrift) with "
                                   "adjoint='reverse'")
+    if solver.pgd_kernel.op(1.0) is not None:
+        raise NotImplementedError("gradients through PGD / sharpening corrections are not implemented")
     interp = solver.interp_kernel
     target = interp.painting.target if interp.painting is not None else "particles"
     if target not in ("density", "particles", "spherical"):
@@ -160,7 +162,7 @@ def integrate(displacements, velocities, cosmo, ts, solver: AbstractNBodySolver,
     snaps = []
     for item in ops:
         if item[0] == "op":
-            op = StepOp(item[1].pre, item[1].kick, item[1].post)
+            op = StepOp(item[1].pre, item[1].kick, item[1].post, item[1].corr)
             if pending is not None and pending.post is None and op.pre is not None:
                 pending.post, op.pre = op.pre, None
             if pending is not None:

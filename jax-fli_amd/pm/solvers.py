@@ -40,11 +40,13 @@ class NBodyState:
 
 @dataclasses.dataclass
 class StepOp:
-    """v += F(x) * force_scale * kick (if kick is not None), with optional drifts before/after."""
+    """v += F(x) * force_scale * kick (if kick is not None), with optional drifts before/after and an
+    optional correction (PGD displacement or sharpening boost) between the leading drift and the kick."""
 
     pre: float | None = None
     kick: float | None = None
     post: float | None = None
+    corr: Any = None
 
 
 def force_scale(cosmo) -> float:
@@ -67,7 +69,13 @@ def run_op(op: StepOp, x, v, mesh_size, mode, cosmo, deconv=False) -> None:
             if c is not None:
                 x.add_(v * f32(c))
         return
-    engine.step(x, v, mesh_size, mode=mode, force_scale=force_scale(cosmo), c_kick=op.kick, c_drift_pre=op.pre,
+    pre = op.pre
+    if op.corr is not None:  # the correction sees the drifted positions, the kick the corrected ones
+        if pre is not None:
+            x.add_(v * f32(pre))
+            pre = None
+        op.corr.run(x, v, mesh_size, mode)
+    engine.step(x, v, mesh_size, mode=mode, force_scale=force_scale(cosmo), c_kick=op.kick, c_drift_pre=pre,
                 c_drift_post=op.post, deconv=deconv)
 
 
@@ -125,7 +133,8 @@ class EfficientDriftDoubleKick(AbstractNBodySolver):
         k1 = (_c.Gf(cosmo, t1) - _c.Gf(cosmo, t0t1)) / _c.dGfa(cosmo, t1)
         k2 = (_c.Gf(cosmo, t1t2) - _c.Gf(cosmo, t1)) / _c.dGfa(cosmo, t1)
         ck = (k1 + k2) / (float(t1) ** 2 * _c.E(cosmo, t1))
-        return StepOp(pre=float(f32(cd)), kick=float(f32(ck)))
+        # solvers.py:175: the correction gets the drift factor without its 1/(a^3 E) prefactor
+        return StepOp(pre=float(f32(cd)), kick=float(f32(ck)), corr=self.pgd_kernel.op(float(f32(drift))))
 
     def step_ops(self, grid, dt, t_initial, cosmo) -> list:
         """step_op vectorised over the steps (same float32 time arithmetic and float64 factors)."""
@@ -143,7 +152,8 @@ class EfficientDriftDoubleKick(AbstractNBodySolver):
         k1 = (gf1 - _c.Gf(cosmo, t0t1)) / dgf
         k2 = (_c.Gf(cosmo, t1t2) - gf1) / dgf
         ck = (k1 + k2) / (t1.astype(np.float64) ** 2 * _c.E(cosmo, t1))
-        return [StepOp(pre=float(f32(a)), kick=float(f32(b))) for a, b in zip(cd, ck)]
+        return [StepOp(pre=float(f32(a)), kick=float(f32(b)), corr=self.pgd_kernel.op(float(f32(d))))
+                for a, b, d in zip(cd, ck, drift)]
 
     def init(self, displacement, velocities, t0, t1, dt, cosmo):
         op = self.init_op(t0, t1, dt, cosmo)
@@ -183,7 +193,8 @@ class ReversibleDoubleKickDrift(AbstractNBodySolver):
         cd = drift / (float(af_2) ** 3 * _c.E(cosmo, af_2))
         if raw:
             return StepOp(kick=float(ck), post=float(cd))
-        return StepOp(kick=float(f32(ck)), post=float(f32(cd)))
+        # solvers.py:311-316: the boost is S(x_i) / full_drift_factor, applied before the drift
+        return StepOp(kick=float(f32(ck)), post=float(f32(cd)), corr=self.pgd_kernel.op(float(f32(cd))))
 
     def step_coeffs(self, grid, dt, t_initial, cosmo, raw: bool = False) -> np.ndarray:
         """(nsteps, 2) array of (kick, drift) for the steps [(t0, t1), ...]: step_op vectorised over the
@@ -206,12 +217,14 @@ class ReversibleDoubleKickDrift(AbstractNBodySolver):
         return out if raw else out.astype(np.float32).astype(np.float64)
 
     def step_ops(self, grid, dt, t_initial, cosmo) -> list:
-        return [StepOp(kick=float(a), post=float(b)) for a, b in self.step_coeffs(grid, dt, t_initial, cosmo)]
+        return [StepOp(kick=float(a), post=float(b), corr=self.pgd_kernel.op(float(b)))
+                for a, b in self.step_coeffs(grid, dt, t_initial, cosmo)]
 
     def reverse_op(self, t0, t1, dt, t_initial, cosmo) -> StepOp:
-        """Exact inverse of step_op(t0, t1): x -= v cd, then v -= F(x) ck."""
+        """Exact inverse of step_op(t0, t1): x -= v cd, then the boost is rewound at that x
+        (solvers.py:372-376), then v -= F(x) ck."""
         op = self.step_op(t0, t1, dt, t_initial, cosmo)
-        return StepOp(pre=-op.post, kick=-op.kick)
+        return StepOp(pre=-op.post, kick=-op.kick, corr=self.pgd_kernel.rewind_op(op.post))
 
     def init(self, displacement, velocities, t0, t1, dt, cosmo):
         return displacement, velocities, NBodyState(interp_state=self.interp_kernel.init(), t_initial=float(t0))

@@ -247,9 +247,13 @@ def kkd_coeffs(c: Cosmo, t0, t1, dt, t_initial):
     return ck, cd
 
 
-def run_nbody(c: Cosmo, dx, p, t0, t1, dt0, ts, solver="kdk", force_fn=None, dtype=np.float64, on_save=None):
+def run_nbody(c: Cosmo, dx, p, t0, t1, dt0, ts, solver="kdk", force_fn=None, dtype=np.float64, on_save=None,
+              correction=None):
     """Full integration; force_fn(x) defaults to 1.5 Om pm_forces on the particle lattice of dx.
-    Arithmetic on x, v in `dtype` with the reference's operation order (x + v*c, v + F*c)."""
+    Arithmetic on x, v in `dtype` with the reference's operation order (x + v*c, v + F*c).
+    correction = ("pgd" | "sharpen", alpha, kl, ks): KDK applies it after the drift with the drift
+    factor without its prefactor (solvers.py:175), KKD as a boost before the drift with the full
+    drift factor (solvers.py:311-316)."""
     x = np.asarray(dx, dtype=dtype).copy()
     v = np.asarray(p, dtype=dtype).copy()
     mesh = x.shape[:3]
@@ -259,6 +263,10 @@ def run_nbody(c: Cosmo, dx, p, t0, t1, dt0, ts, solver="kdk", force_fn=None, dty
     if solver == "kdk":
         ck = kdk_init_coeff(c, t0, t0 + dt0)
         v = v + force_fn(x).astype(dtype) * cast(ck)
+    corr = None
+    if correction is not None:
+        kind, alpha, kl, ks = correction
+        corr = lambda xx: orc.pgd_displacement(xx, mesh, alpha, kl, ks, "relative", dtype=dtype)
     for ev in time_grid(t0, ts, dt0, t1):
         if ev[0] == "save":
             if on_save is not None:
@@ -268,9 +276,23 @@ def run_nbody(c: Cosmo, dx, p, t0, t1, dt0, ts, solver="kdk", force_fn=None, dty
         if solver == "kdk":
             cd, ck = kdk_coeffs(c, ta, tb, dt0)
             x = x + v * cast(cd)
+            if corr is not None:
+                s = corr(x).astype(dtype)
+                if kind == "pgd":
+                    x = x + s
+                else:
+                    ta32, tb32 = np.float32(ta), np.float32(tb)
+                    tm = np.float32(np.sqrt(np.float32(ta32 * tb32)))
+                    df = float((_g(c.growth_factor, tb32) - _g(c.growth_factor, ta32)) / _g(c.gp, tm))
+                    v = v + s / cast(df)
             v = v + force_fn(x).astype(dtype) * cast(ck)
         else:
             ck, cd = kkd_coeffs(c, ta, tb, dt0, t0)
-            v = v + force_fn(x).astype(dtype) * cast(ck)
+            f = force_fn(x).astype(dtype)
+            v = v + f * cast(ck)
+            if corr is not None:
+                if kind != "sharpen":
+                    raise ValueError("the reversible solver takes the sharpening correction")
+                v = v + corr(x).astype(dtype) / cast(cd)
             x = x + v * cast(cd)
     return x, v

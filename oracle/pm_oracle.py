@@ -21,6 +21,8 @@ Parity status per function:
                                                pixels, equal-area counts, interpolation identities)
   paint_healpix normalisation / shell edges    parity unpinned (jaxpm.spherical is not in the
                                                reference tree and no reference test covers it)
+  pgd_range / pgd_displacement                 pinned (kl = 0, ks -> inf gives the force; a single
+                                               plane wave is scaled by the filter at its k)
   cic_paint_vjp / cic_read_vjp /               pinned (float64 central differences along random
   force_mesh_vjp / pm_forces_vjp                 directions; exact transpose identities)
 """
@@ -144,14 +146,25 @@ def cic_window_comp(kvec):
     return out
 
 
-def force_mesh(delta, deconv=False, dtype=np.float64):
+def pgd_range(kvec, kl, ks):
+    """PGD filter exp(-kl^2/k^2) exp(-k^4/ks^4), 0 at k = 0 (src/jax_fli/pm/correction.py:182-190)."""
+    kk = sum(k**2 for k in kvec)
+    kk_safe = np.where(kk == 0, 1.0, kk)
+    r = np.exp(-(kl**2) / kk_safe) * np.exp(-(kk**2) / ks**4)
+    return np.where(kk == 0, 0.0, r)
+
+
+def force_mesh(delta, deconv=False, dtype=np.float64, pgd=None):
     """(X,Y,Z,3) force mesh of a density mesh: ifft(-grad_i * invlaplace * fft(delta)).real.
-    The Nyquist gradient is exactly 0 (sin(pi) = 0)."""
+    The Nyquist gradient is exactly 0 (sin(pi) = 0). pgd = (kl, ks) multiplies the potential by
+    the PGD filter."""
     mesh_shape = delta.shape
     cdt = np.complex128 if dtype == np.float64 else np.complex64
     dk = np.fft.fftn(np.asarray(delta, dtype=dtype)).astype(cdt)
     kvec = fftk(mesh_shape, np.float64)
     pot = dk * invlaplace_kernel(kvec)
+    if pgd is not None:
+        pot = pot * pgd_range(kvec, *pgd)
     if deconv:
         pot = pot * cic_window_comp(kvec)
     out = np.empty(tuple(mesh_shape) + (3,), dtype=dtype)
@@ -172,6 +185,15 @@ def pm_forces(pos, mesh_shape, mode="relative", scale=1.0, deconv=False, dtype=n
     rho = cic_paint(pos, mesh_shape, mode, dtype=dtype)
     fm = force_mesh(rho, deconv=deconv, dtype=dtype) * dtype(scale)
     return cic_read(fm.astype(dtype), pos, mode, dtype=dtype)
+
+
+def pgd_displacement(pos, mesh_shape, alpha, kl, ks, mode="relative", dtype=np.float64):
+    """S(x) = alpha * read(x, force mesh of paint(x) with the PGD filter): the displacement of
+    PGDKernel.apply and, divided by the drift factor, the velocity boost of SharpeningKernel
+    (src/jax_fli/pm/correction.py:169-200, 301-333)."""
+    rho = cic_paint(pos, mesh_shape, mode, dtype=dtype)
+    fm = force_mesh(rho, dtype=dtype, pgd=(kl, ks))
+    return cic_read(fm.astype(dtype), pos, mode, dtype=dtype) * dtype(alpha)
 
 
 # ---------------------------------------------------------------------------- P(k)

@@ -279,3 +279,44 @@ def test_adjoints_match_finite_differences(mode):
     pb = orc.pm_forces_vjp(pos, mesh, fb, mode, scale=0.7)
     ref = _dd64(lambda x: (orc.pm_forces(x, mesh, mode, scale=0.7) * fb).sum(), pos, d)
     assert abs((pb * d).sum() - ref) < 1e-7 * np.abs(pb).sum()
+
+
+# ---------------------------------------------------------------- PGD / sharpening corrections
+def test_pgd_filter_limits_and_plane_wave():
+    n = 16
+    rng = np.random.default_rng(70)
+    dx = rng.normal(0, 0.8, (n, n, n, 3))
+    # kl = 0 and ks -> infinity: the filter is 1 (0 at k = 0) and S is alpha times the PM force
+    s = orc.pgd_displacement(dx, (n, n, n), 0.7, 0.0, 1e6)
+    f = orc.pm_forces(dx, (n, n, n), scale=0.7)
+    assert np.allclose(s, f, rtol=1e-12, atol=1e-14)
+    # a single plane wave of the density is scaled by the filter at its wavenumber
+    x = np.arange(n)
+    delta = np.cos(2 * np.pi * 3 * x / n)[:, None, None] * np.ones((1, n, n))
+    k0 = 2 * np.pi * 3 / n
+    kl, ks = 0.3, 1.0
+    ratio = orc.force_mesh(delta, pgd=(kl, ks)) / np.where(orc.force_mesh(delta) == 0, 1, orc.force_mesh(delta))
+    expect = np.exp(-kl**2 / k0**2) * np.exp(-k0**4 / ks**4)
+    fx = orc.force_mesh(delta)[..., 0]
+    sel = np.abs(fx) > 1e-3 * np.abs(fx).max()
+    assert np.allclose(ratio[..., 0][sel], expect, rtol=1e-10)
+
+
+def test_sharpened_kkd_is_reversible_in_float64():
+    """KKD with the sharpening boost, undone step by step (un-drift, un-boost at the recovered x,
+    un-kick) returns the initial state to float64 rounding."""
+    c = nbo.Cosmo()
+    n = 8
+    rng = np.random.default_rng(71)
+    x0 = rng.normal(0, 0.5, (n, n, n, 3))
+    v0 = rng.normal(0, 0.1, (n, n, n, 3))
+    corr = lambda xx: orc.pgd_displacement(xx, (n, n, n), 0.2, 0.3, 1.0)
+    force = lambda xx: orc.pm_forces(xx, (n, n, n), scale=1.5 * c.Omega_m)
+    ck, cd = nbo.kkd_coeffs(c, 0.3, 0.4, 0.1, 0.3)
+    v1 = v0 + force(x0) * ck
+    v1 = v1 + corr(x0) / cd
+    x1 = x0 + v1 * cd
+    xr = x1 - v1 * cd
+    vr = v1 - corr(xr) / cd
+    vr = vr - force(xr) * ck
+    assert np.abs(xr - x0).max() < 1e-12 and np.abs(vr - v0).max() < 1e-12
