@@ -158,6 +158,22 @@ int pmx_paint_healpix_tiled(pmx_plan plan, const float* pos, int npx, int npy, i
                             double rmin, double rmax, int nside, int scheme, int ntiles, const int* tile_rot,
                             const double* tile_shift, float* map, int accumulate, void* stream);
 
+/* pmx_paint_healpix_tiled with the drift to the lightcone of DriftInterp / OnionTiler
+ * (pm/interp.py:110-121, 212-504): every image position X_t (observer-centred, Mpc/h) first moves by
+ * f (R_t vel) with f = (Gp(a_t) - gp_now) / dGfa(sqrt(a_now a_t)), a_t = a(chi = |X_t|), and the shell
+ * test and the pixel use the moved position. vel: the velocity array (npx, npy, npz, 3) as given
+ * (added to Mpc/h positions unchanged, as _drift_to_time does). drift_tables (device float64, 3 x ntab):
+ * a(chi) on the uniform grid [0, chi_max], then Gp(a) and dGfa(a) on [a_lo, a_hi], linearly
+ * interpolated and clamped; gp_now = Gp(a_now); drift_bound >= max |f| over the tables (it widens the
+ * per-workgroup image culling). tile_rot == NULL: the box alone. */
+int pmx_paint_healpix_lightcone(pmx_plan plan, const float* pos, const float* vel, int npx, int npy, int npz,
+                                int mode, const float* weights, float weight, const double box[3],
+                                const double observer[3], double rmin, double rmax, int nside, int scheme,
+                                int ntiles, const int* tile_rot, const double* tile_shift,
+                                const double* drift_tables, int ntab, double chi_max, double a_lo, double a_hi,
+                                double a_now, double gp_now, double drift_bound, float* map, int accumulate,
+                                void* stream);
+
 /* Flat-sky slab paint: absolute z in (zlo, zhi] (grid units), (x, y) rescaled to (fnx, fny) and
  * wrapped, 2D CIC, times `scale` (1 / pixel volume). _single_paint_2d (_painting.py:117-202). */
 int pmx_paint_flat(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, const float* weights,

@@ -25,7 +25,8 @@ int pmx_born_max_shells();
 int pmx_born_max_sources();
 int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, const double box[3],
                        const double obs[3], double rmin, double rmax, int nside, int scheme, int ntiles,
-                       const int* rot, const double* shift, float* map, int accumulate, hipStream_t s);
+                       const int* rot, const double* shift, float* map, int accumulate, hipStream_t s,
+                       const PmxDrift* drift = nullptr);
 int pmx_launch_healpix_vjp(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, const double box[3],
                            const double obs[3], double rmin, double rmax, int nside, int scheme, int ntiles,
                            const int* rot, const double* shift, const float* map_bar, float* pos_bar, int accumulate,
@@ -689,6 +690,24 @@ int pmx_paint_healpix_tiled(pmx_plan p, const float* pos, int npx, int npy, int 
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
   return pmx_launch_healpix(p, a, weights, weight, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
                             tile_shift, map, accumulate, (hipStream_t)stream);
+}
+
+int pmx_paint_healpix_lightcone(pmx_plan p, const float* pos, const float* vel, int npx, int npy, int npz, int mode,
+                                const float* weights, float weight, const double box[3], const double observer[3],
+                                double rmin, double rmax, int nside, int scheme, int ntiles, const int* tile_rot,
+                                const double* tile_shift, const double* drift_tables, int ntab, double chi_max,
+                                double a_lo, double a_hi, double a_now, double gp_now, double drift_bound, float* map,
+                                int accumulate, void* stream) {
+  PMX_TRY(check_shell(p, pos, npx, npy, npz, mode, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
+                      tile_shift));
+  if (!map || !vel || !drift_tables) return pmx_set_error(1, "null argument");
+  if (ntab < 2 || !(chi_max > 0.0) || !(a_hi > a_lo) || !(a_now > 0.0) || !(drift_bound >= 0.0))
+    return pmx_set_error(1, "bad drift tables: need ntab >= 2, chi_max > 0, a_hi > a_lo, a_now > 0");
+  PMX_TRY(ensure_acc(p, (size_t)12 * nside * nside));
+  PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
+  const PmxDrift d{vel, drift_tables, ntab, chi_max, a_lo, a_hi, a_now, gp_now, drift_bound};
+  return pmx_launch_healpix(p, a, weights, weight, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
+                            tile_shift, map, accumulate, (hipStream_t)stream, &d);
 }
 
 int pmx_paint_healpix_vjp(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, const float* weights,

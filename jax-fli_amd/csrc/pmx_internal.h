@@ -111,6 +111,13 @@ struct PartArgs {
   int mode;            // 0 relative (lattice + dx), 1 absolute
   float sx, sy, sz;    // lattice spacing in cells (mesh / particle grid)
 };
+// drift to the lightcone of the HEALPix shell paint (pmx_sky.hip)
+struct PmxDrift {
+  const float* v;
+  const double* tables;  // a(chi) on [0, chi_max]; Gp(a), dGfa(a) on [a_lo, a_hi]; ntab points each
+  int ntab;
+  double chi_max, a_lo, a_hi, a_now, gp_now, fbound;
+};
 // paint tiles of a plan (log2 of the tile dims, tile count)
 void pmx_paint_tiles(const pmx_plan_t* p, int& sx, int& sy, int& sz, int& ntiles);
 int pmx_paint_face_cells(const pmx_plan_t* p);

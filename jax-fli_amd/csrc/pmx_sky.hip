@@ -223,6 +223,14 @@ struct ShellArgs {
   int ntiles;       // image count; rot == nullptr and ntiles == 1: the box itself
   const int* rot;
   const double* shift;  // 3 per image, Mpc/h
+  // drift to the lightcone (v != nullptr): each image position moves by f (R v) with
+  // f = (Gp(a_t) - gp_now) / dGfa(sqrt(a_now a_t)), a_t = a(chi = |position|), from the tables
+  // dtab = {a(chi) on [0, chi_max], Gp(a) and dGfa(a) on [a_lo, a_hi]}, ntab points each
+  const float* v;
+  const double* dtab;
+  int ntab;
+  double chi_max, a_lo, a_hi, a_now, gp_now;
+  double fbound;    // bound on |f| over the tables' range (widens the image culling boxes)
 };
 
 constexpr int HP_HASH = 4096, HP_PROBES = 32, HP_SB = 16, HP_PER_THREAD = HP_SB * HP_SB * HP_SB / 256;
@@ -265,8 +273,38 @@ __device__ __forceinline__ void image_of(const ShellArgs& sa, int t, const doubl
   for (int c = 0; c < 3; ++c) Xt[c] = R.s[c] * X[R.p[c]] + sa.shift[3 * t + c];
 }
 
+// linear interpolation in a table of n values on the uniform grid [x0, x1], clamped at the ends
+__device__ __forceinline__ double tab_lin(const double* t, int n, double x0, double x1, double x) {
+  const double u = (x - x0) / (x1 - x0) * (double)(n - 1);
+  if (!(u > 0.0)) return t[0];
+  if (u >= (double)(n - 1)) return t[n - 1];
+  const int i = (int)u;
+  return t[i] + (t[i + 1] - t[i]) * (u - (double)i);
+}
+
+// velocity of image t (rotated like the position, not shifted)
+__device__ __forceinline__ void image_vel(const ShellArgs& sa, int t, const double V[3], double Vt[3]) {
+  if (!sa.rot) {
+    for (int c = 0; c < 3; ++c) Vt[c] = V[c];
+    return;
+  }
+  const Sym R = sym_of(sa.rot[t]);
+  for (int c = 0; c < 3; ++c) Vt[c] = R.s[c] * V[R.p[c]];
+}
+
+// _drift_to_time (src/jax_fli/pm/interp.py:110-121) at the image's own lightcone time
+__device__ __forceinline__ void drift_image(const ShellArgs& sa, double Xt[3], const double Vt[3]) {
+  const double r = sqrt(Xt[0] * Xt[0] + Xt[1] * Xt[1] + Xt[2] * Xt[2]);
+  const double at = tab_lin(sa.dtab, sa.ntab, 0.0, sa.chi_max, r);
+  const double gpt = tab_lin(sa.dtab + sa.ntab, sa.ntab, sa.a_lo, sa.a_hi, at);
+  const double dg = tab_lin(sa.dtab + 2 * sa.ntab, sa.ntab, sa.a_lo, sa.a_hi, sqrt(sa.a_now * at));
+  const double f = (gpt - sa.gp_now) / dg;
+  for (int c = 0; c < 3; ++c) Xt[c] += f * Vt[c];
+}
+
 // Builds the workgroup's list of images (table order) whose copy of the particles' bounding box
-// meets the shell. Returns the count; hits[] holds the image indices.
+// (grown by the largest drift when drifting) meets the shell. Returns the count; hits[] holds the
+// image indices.
 __device__ int shell_images(const PartArgs& a, const ShellArgs& sa, unsigned b, unsigned short* hits) {
   __shared__ double red[6][4];
   __shared__ int wcount[4];
@@ -282,9 +320,14 @@ __device__ int shell_images(const PartArgs& a, const ShellArgs& sa, unsigned b, 
     if (p < 0) continue;
     double X[3];
     centred(a, sa, p, i, j, k, X);
+    double pad = 0.0;
+    if (sa.v) {  // |f (R v)_c| <= fbound max_c |v_c| on every axis
+      const float3 v = ld3(sa.v, (unsigned)p);
+      pad = sa.fbound * (double)fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z)));
+    }
     for (int c = 0; c < 3; ++c) {
-      lo[c] = fmin(lo[c], X[c]);
-      hi[c] = fmax(hi[c], X[c]);
+      lo[c] = fmin(lo[c], X[c] - pad);
+      hi[c] = fmax(hi[c], X[c] + pad);
     }
   }
   for (int o = 32; o > 0; o >>= 1)
@@ -391,12 +434,23 @@ __global__ void __launch_bounds__(256) k_paint_healpix(PartArgs a, const float* 
     int i, j, k;
     const long p = sb_particle(a, sa, b, r, i, j, k);
     if (p < 0) continue;
-    double X[3];
+    double X[3], V[3] = {0.0, 0.0, 0.0};
     centred(a, sa, p, i, j, k, X);
+    if (sa.v) {
+      const float3 v = ld3(sa.v, (unsigned)p);
+      V[0] = v.x;
+      V[1] = v.y;
+      V[2] = v.z;
+    }
     const double wt = w ? (double)w[p] : (double)w0;
     for (int h = 0; h < nhit; ++h) {
       double Xt[3], rr;
       image_of(sa, hits[h], X, Xt);
+      if (sa.v) {
+        double Vt[3];
+        image_vel(sa, hits[h], V, Vt);
+        drift_image(sa, Xt, Vt);
+      }
       if (!in_shell(sa, Xt, rr)) continue;
       const double phi = phi_of(Xt);
       if (sa.scheme == 0) {
@@ -665,6 +719,10 @@ ShellArgs shell_args(const pmx_plan_t* p, const PartArgs& a, const double box[3]
   sa.ntiles = rot ? ntiles : 1;
   sa.rot = rot;
   sa.shift = shift;
+  sa.v = nullptr;
+  sa.dtab = nullptr;
+  sa.ntab = 0;
+  sa.chi_max = sa.a_lo = sa.a_hi = sa.a_now = sa.gp_now = sa.fbound = 0.0;
   return sa;
 }
 
@@ -685,8 +743,20 @@ int pmx_born_max_sources() { return kBornMaxSources; }
 
 int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, const double box[3],
                        const double obs[3], double rmin, double rmax, int nside, int scheme, int ntiles,
-                       const int* rot, const double* shift, float* map, int accumulate, hipStream_t s) {
-  const ShellArgs sa = shell_args(p, a, box, obs, rmin, rmax, scheme, ntiles, rot, shift);
+                       const int* rot, const double* shift, float* map, int accumulate, hipStream_t s,
+                       const PmxDrift* drift) {
+  ShellArgs sa = shell_args(p, a, box, obs, rmin, rmax, scheme, ntiles, rot, shift);
+  if (drift) {
+    sa.v = drift->v;
+    sa.dtab = drift->tables;
+    sa.ntab = drift->ntab;
+    sa.chi_max = drift->chi_max;
+    sa.a_lo = drift->a_lo;
+    sa.a_hi = drift->a_hi;
+    sa.a_now = drift->a_now;
+    sa.gp_now = drift->gp_now;
+    sa.fbound = drift->fbound;
+  }
   const HpGeom g = hp_geom(nside);
   double* acc = p->acc;
   hipLaunchKernelGGL(k_zero_d, dim3(grid_for((size_t)g.npix, 256)), dim3(256), 0, s, acc, g.npix);

@@ -257,6 +257,31 @@ def paint_healpix(pos: torch.Tensor, mesh_shape, box, observer, rmin, rmax, nsid
     return out
 
 
+def paint_healpix_drift(pos: torch.Tensor, vel: torch.Tensor, mesh_shape, box, observer, rmin, rmax, nside, drift,
+                        scheme="bilinear", mode="relative", weights=1.0, out=None, accumulate=False, tiles=None):
+    """paint_healpix with every (image) position drifted to its own lightcone time first. drift: dict with
+    "tables" (device float64 (3, ntab): a(chi), Gp(a), dGfa(a)), "chi_max", "a_lo", "a_hi", "a_now",
+    "gp_now", "bound"."""
+    pos = _f32(pos, "pos")
+    vel = _f32(vel, "vel")
+    npx, npy, npz = _pgrid(pos)
+    if tuple(vel.shape) != tuple(pos.shape):
+        raise ValueError("vel must have the shape of pos")
+    plan = get_plan(mesh_shape, pos.device)
+    if out is None:
+        out = torch.empty(12 * nside * nside, dtype=torch.float32, device=pos.device)
+    w, w0 = _weights(weights, npx * npy * npz)
+    nt, rot, shift, _keep = _tiles(tiles, pos.device)
+    tab = drift["tables"]
+    check(load().pmx_paint_healpix_lightcone(
+        plan.handle, ptr(pos), ptr(vel), npx, npy, npz, MODES[mode], ptr(w), w0, dbl3(box), dbl3(observer),
+        float(rmin), float(rmax), int(nside), _scheme(scheme), nt, ptr(rot), ptr(shift), ptr(tab),
+        int(tab.shape[-1]), float(drift["chi_max"]), float(drift["a_lo"]), float(drift["a_hi"]),
+        float(drift["a_now"]), float(drift["gp_now"]), float(drift["bound"]), ptr(out), int(accumulate),
+        stream_ptr()), "pmx_paint_healpix_lightcone")
+    return out
+
+
 def paint_healpix_vjp(pos: torch.Tensor, map_bar: torch.Tensor, mesh_shape, box, observer, rmin, rmax, nside,
                       scheme="bilinear", mode="relative", weights=1.0, tiles=None, pos_bar=None,
                       accumulate=False) -> torch.Tensor:

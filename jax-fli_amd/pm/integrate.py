@@ -67,6 +67,8 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
     if solver.pgd_kernel.op(1.0) is not None:
         raise NotImplementedError("gradients through PGD / sharpening corrections are not implemented")
     interp = solver.interp_kernel
+    if type(interp).__name__ == "DriftInterp" or getattr(interp, "drift_on_lightcone", False):
+        raise NotImplementedError("gradients through the drift to the lightcone are not implemented")
     target = interp.painting.target if interp.painting is not None else "particles"
     if target not in ("density", "particles", "spherical"):
         raise NotImplementedError(f"gradients through '{target}' snapshots are not implemented")

@@ -1,7 +1,9 @@
 """Snapshot interpolation kernels (src/jax_fli/pm/interp.py:46-504): the abstract interface,
-``NoInterp`` (paint the particles as they are at each snapshot) and ``OnionTiler`` (spherical shells
-beyond the box painted from rotated periodic images of it). Drift-to-lightcone corrections and the
-Drift/Telephoto kernels are not implemented on this engine."""
+``NoInterp`` (paint the particles as they are at each snapshot), ``DriftInterp`` (spherical shells
+with every particle drifted to its own lightcone crossing) and ``OnionTiler`` (spherical shells
+beyond the box painted from rotated periodic images of it, optionally drifted). The drift runs inside
+the HEALPix paint kernel from host-built float64 tables of a(chi), Gp(a) and dGfa(a). The Telephoto
+kernel is not implemented on this engine."""
 
 from __future__ import annotations
 
@@ -11,8 +13,47 @@ import warnings
 from typing import Any
 
 import numpy as np
+import torch
 
+from .. import cosmology as _c
+from .. import engine
+from ..fields import DensityUnit, FieldStatus, SphericalDensity, _paint_mode
 from ..painting import PaintingOptions
+
+DRIFT_TABLE = 4096
+
+
+def drift_tables(cosmo, a_now: float, chi_max: float, device) -> dict:
+    """Tables of _drift_to_time (interp.py:110-121) for particles at time a_now: a(chi) on
+    [0, chi_max], Gp(a) and dGfa(a) on [a(chi_max), 1] (float64, linear interpolation on the device),
+    and a bound on |(Gp(a_t) - Gp(a_now)) / dGfa(sqrt(a_now a_t))| over that range."""
+    chi = np.linspace(0.0, chi_max, DRIFT_TABLE)
+    a_chi = np.asarray(_c.a_of_chi(cosmo, chi), dtype=np.float64)
+    a_lo, a_hi = float(min(a_chi[-1], a_now)), 1.0
+    ag = np.linspace(a_lo, a_hi, DRIFT_TABLE)
+    gp_t = np.asarray(_c.Gp(cosmo, ag), dtype=np.float64)
+    dg_t = np.asarray(_c.dGfa(cosmo, ag), dtype=np.float64)
+    gp_now = float(np.asarray(_c.Gp(cosmo, a_now)).reshape(-1)[0])
+    f = (gp_t - gp_now) / np.asarray(_c.dGfa(cosmo, np.sqrt(a_now * ag)), dtype=np.float64)
+    tab = torch.as_tensor(np.stack([a_chi, gp_t, dg_t]), dtype=torch.float64).to(device)
+    return {"tables": tab.contiguous(), "chi_max": chi_max, "a_lo": a_lo, "a_hi": a_hi, "a_now": float(a_now),
+            "gp_now": gp_now, "bound": float(np.abs(f).max()) * 1.01 + 1e-12}
+
+
+def paint_drifted(dx, p, t, cosmo, r_center, width, painting, tiles=None):
+    """Spherical shell map of the particles each drifted to the lightcone (DriftInterp.paint and
+    OnionTiler with drift_on_lightcone, interp.py:258-319, 375-419, 476-493)."""
+    if dx.nside is None:
+        raise ValueError("Spherical painting requires `nside`.")
+    rmin, rmax = r_center - width / 2, r_center + width / 2
+    drift = drift_tables(cosmo, float(t), 2.0 * rmax + 100.0, dx.array.device)
+    arr = engine.paint_healpix_drift(dx.array.contiguous(), p.array.contiguous(), dx.mesh_size, dx.box_size,
+                                     dx.observer_position_mpc, rmin, rmax, dx.nside, drift, scheme=painting.scheme,
+                                     mode=_paint_mode(dx.unit),
+                                     weights=1.0 if painting.weights is None else painting.weights, tiles=tiles)
+    return SphericalDensity.FromDensityMetadata(array=arr, field=dx, status=FieldStatus.LIGHTCONE,
+                                                comoving_centers=np.atleast_1d(r_center),
+                                                density_width=np.atleast_1d(width), unit=DensityUnit.DENSITY)
 
 
 @dataclasses.dataclass(frozen=True)
@@ -101,9 +142,37 @@ class NoInterp(AbstractInterp):
         return result.replace(scale_factors=float(t), comoving_centers=r_center, density_width=width)
 
 
+@dataclasses.dataclass(frozen=True)
 class DriftInterp(AbstractInterp):
-    def __init__(self, *a, **k):
-        raise NotImplementedError("DriftInterp is not implemented on the pmx engine")
+    """No tiling; each particle is drifted from the snapshot time to the time its distance from the
+    observer corresponds to before painting (interp.py:212-320). On this engine the drift is applied
+    to spherical (HEALPix) targets; other targets raise."""
+
+    def init(self) -> InterpTilerState:
+        if self.r_centers is not None and self.density_widths is not None and self.max_comoving_distance is not None:
+            far = float(np.asarray(self.r_centers).reshape(-1)[0] + np.asarray(self.density_widths).reshape(-1)[0] / 2)
+            if far > self.max_comoving_distance:
+                warnings.warn(
+                    f"DriftInterp does not support tiling. Your furthest shell is at {far:.2f} Mpc/h, but your box "
+                    f"only extends to {self.max_comoving_distance:.2f} Mpc/h. Your simulations will run but will "
+                    f"have artifacts for the far shells.")
+        return InterpTilerState(rotation_idx=-1, shell_idx=-1, rotations=None)
+
+    def advance(self, state, t):
+        hit = self.ts is not None and bool(np.isin(np.float32(t), np.asarray(self.ts, dtype=np.float32)))
+        return dataclasses.replace(state, shell_idx=state.shell_idx + 1) if hit else state
+
+    def rewind(self, state, t):
+        hit = self.ts is not None and bool(np.isin(np.float32(t), np.asarray(self.ts, dtype=np.float32)))
+        return dataclasses.replace(state, shell_idx=state.shell_idx - 1) if hit else state
+
+    def paint(self, state, t, y, cosmo):
+        dx, p = y
+        if self.painting is None or self.painting.target != "spherical":
+            raise NotImplementedError("DriftInterp paints spherical shells on the pmx engine")
+        r_center, width = self.shell(state)
+        out = paint_drifted(dx, p, t, cosmo, r_center, width, self.painting)
+        return out.replace(scale_factors=float(t), comoving_centers=r_center, density_width=width)
 
 
 @dataclasses.dataclass(frozen=True)
@@ -118,9 +187,9 @@ class OnionTiler(AbstractInterp):
     `seed` (the reference draws jax.random.randint(PRNGKey(42), (27,), 0, 47), whose bits this
     engine does not reproduce); particles are wrapped into the box before imaging (the reference
     images unwrapped positions); `layers` > 1 extends the offsets to [-layers, layers]^3 so shells
-    beyond 1.5 boxes are covered, and `rotate=False` gives plain periodic replication. The
-    drift-to-lightcone correction (drift_on_lightcone=True, the reference default) is not
-    implemented and is refused."""
+    beyond 1.5 boxes are covered, and `rotate=False` gives plain periodic replication. With
+    drift_on_lightcone (the reference default) every image is drifted to its own lightcone time
+    inside the same paint kernel."""
 
     drift_on_lightcone: bool = True
     layers: int = 1
@@ -130,9 +199,6 @@ class OnionTiler(AbstractInterp):
     def init(self) -> InterpTilerState:
         if self.painting.target != "spherical":
             raise ValueError("OnionTiler only supports spherical painting.")
-        if self.drift_on_lightcone:
-            raise NotImplementedError("drift to the lightcone is not implemented on the pmx engine; "
-                                      "construct OnionTiler(..., drift_on_lightcone=False)")
         if self.layers < 1:
             raise ValueError("layers must be >= 1")
         n = (2 * self.layers + 1) ** 3
@@ -162,9 +228,12 @@ class OnionTiler(AbstractInterp):
         return np.asarray(state.rotations, dtype=np.int32), shifts * np.asarray(field.box_size, dtype=np.float64)
 
     def paint(self, state, t, y, cosmo):
-        dx, _ = y
+        dx, p = y
         r_center, width = self.shell(state)
         pt = self.painting
+        if self.drift_on_lightcone:
+            out = paint_drifted(dx, p, t, cosmo, r_center, width, pt, tiles=self.images(state, dx))
+            return out.replace(scale_factors=float(t), comoving_centers=r_center, density_width=width)
         result = dx.paint_spherical(center=r_center, density_plane_width=width, scheme=pt.scheme, weights=pt.weights,
                                     kernel_width_arcmin=pt.kernel_width_arcmin,
                                     smoothing_interpretation=pt.smoothing_interpretation,

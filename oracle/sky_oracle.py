@@ -15,6 +15,8 @@ Parity status per function:
                                         periodic images of a uniform lattice fill a far shell with
                                         the lattice's mean density); normalisation as pm_oracle
   healpix_interp_weights_grad           pinned (float64 central differences of the weights)
+  Drift                                 pinned (EdS closed form f = (a_t - a) / (1.5 (a a_t)^(1/4));
+                                        zero drift at the distance of the snapshot time)
   paint_healpix_vjp                     pinned (central differences of paint_healpix_tiled)
   simps_weights                         pinned (exact for cubics; jax_cosmo simps restated)
   born_spherical                        pinned (thin-plane sum -> the Born integral
@@ -87,18 +89,43 @@ def _angles(X):
     return np.arctan2(np.sqrt(x * x + y * y), z), phi
 
 
+class Drift:
+    """_drift_to_time (src/jax_fli/pm/interp.py:110-121) for particles at time a_now:
+    f(r) = (Gp(a_t) - Gp(a_now)) / dGfa(sqrt(a_now a_t)), a_t = a(chi = r), from the oracle's own
+    distance quadrature and growth ODE (cubic splines on a grid of a in [a_min, 1])."""
+
+    def __init__(self, cosmo, a_now, a_min=0.3, n=801):
+        from scipy.interpolate import CubicSpline
+
+        a = np.linspace(a_min, 1.0, n)
+        chi = cosmo.chi(a)
+        self._a_of_chi = CubicSpline(chi[::-1], a[::-1])
+        self._gp = CubicSpline(a, cosmo.growth_factor(a))
+        self._dg = CubicSpline(a, cosmo.dGfa(a))
+        self.a_now = float(a_now)
+        self.gp_now = float(self._gp(self.a_now))
+
+    def factor(self, r):
+        at = self._a_of_chi(np.asarray(r, dtype=np.float64))
+        return (self._gp(at) - self.gp_now) / self._dg(np.sqrt(self.a_now * at))
+
+
 def paint_healpix_tiled(pos, mesh_shape, box, observer, rmin, rmax, nside, scheme="bilinear", mode="relative",
-                        weight=1.0, tiles=None):
+                        weight=1.0, tiles=None, vel=None, drift=None):
     """Sum over images (code, shift) of the shell paint of R_code X + shift, X the observer-centred,
-    box-wrapped position; tiles=None: the box alone. Normalisation as pm_oracle.paint_healpix."""
+    box-wrapped position; tiles=None: the box alone. With drift (a Drift) and vel, every image moves
+    by drift.factor(|X_t|) (R_code vel) first. Normalisation as pm_oracle.paint_healpix."""
     X, _ = _centred(pos, mesh_shape, box, observer, mode)
     n = X.shape[0]
     wt = np.broadcast_to(np.asarray(weight, dtype=np.float64).reshape(-1), (n,))
     tiles = [(0, np.zeros(3))] if tiles is None else [(int(c), np.asarray(s, dtype=np.float64)) for c, s in tiles]
+    V = None if vel is None else np.asarray(vel, dtype=np.float32).reshape(-1, 3).astype(np.float64)
     npix = 12 * nside * nside
     acc = np.zeros(npix)
     for code, sh in tiles:
         Xt = X @ cube_symmetry(code).T + sh
+        if drift is not None:
+            Xt = Xt + drift.factor(np.sqrt((Xt**2).sum(axis=1)))[:, None] * (V @ cube_symmetry(code).T)
         r = np.sqrt((Xt**2).sum(axis=1))
         for p in np.nonzero((r > rmin) & (r <= rmax))[0]:
             theta, phi = _angles(Xt[p])

@@ -174,3 +174,15 @@ def test_tophat_normalisation():
     z = np.linspace(0, 10, 257)
     assert (so.simps_weights(0, 10, 256) * nz(z)).sum() == pytest.approx(1.0, rel=1e-12)
     assert nz(np.array([0.5]))[0] == pytest.approx(nz(np.array([0.9]))[0]) and nz(np.array([1.01]))[0] == 0
+
+
+def test_drift_factor_closed_form_in_eds():
+    """Omega_m = 1: Gp = a, dGfa = 1.5 a^(1/2), chi = 2 c/H0 (1 - a^(1/2))."""
+    c = nbo.Cosmo(Omega_c=1.0 - 0.049, Omega_b=0.049, h=0.7)
+    d = so.Drift(c, 0.8)
+    r = np.array([10.0, 300.0, 900.0, 1500.0])
+    at = (1.0 - r / (2 * 2997.92458)) ** 2
+    expect = (at - 0.8) / (1.5 * (0.8 * at) ** 0.25)
+    assert np.allclose(d.factor(r), expect, rtol=1e-6, atol=1e-9)
+    r0 = 2 * 2997.92458 * (1 - np.sqrt(0.8))
+    assert abs(d.factor(np.array([r0]))[0]) < 1e-8
