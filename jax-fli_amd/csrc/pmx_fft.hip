@@ -14,6 +14,7 @@
 // back along x before they ever leave LDS. Transforms are unnormalised; the 1/N of the inverse is
 // folded into the P3 operator.
 #include "pmx_internal.h"
+#include "pmx_device.h"
 
 namespace {
 
@@ -419,29 +420,55 @@ __global__ void __launch_bounds__(384) k_c2r_z(const float2* __restrict__ inD, f
 // ---------------------------------------------------------------- force mesh from the potential
 // F_d = 8 (phi[+e_d] - phi[-e_d]) - (phi[+2 e_d] - phi[-2 e_d]) per cell, written interleaved (x,y,z,3):
 // the real-space form of the FastPM gradient i (8 sin k - sin 2k) / 6 (the 1/12 is folded into the
-// potential). One workgroup per (x, y) row; rows are remapped so each XCD streams one x slab and
-// the x +- 2 / y +- 2 neighbour rows come from its L2.
+// potential). Each thread owns 4 consecutive cells of a z row: one 16-byte load per neighbour row
+// (x +- 1, x +- 2, y +- 1, y +- 2), its own 4 values plus 2 + 2 z neighbours (8-byte loads, wrapped).
+// A workgroup holds rpb consecutive rows of tpr = NZ / 4 threads (whole waves), so its 12 outputs
+// per thread form one contiguous span, written from an LDS copy as 16-byte lanes. Rows are remapped
+// so each XCD streams one x slab and the neighbour rows come from its L2.
 __global__ void __launch_bounds__(256) k_grad4(const float* __restrict__ phi, float* __restrict__ F, int NX, int NY,
-                                               int NZ) {
-  const unsigned G = gridDim.x, b = blockIdx.x;
-  const unsigned row = (G % 8u == 0u) ? (b % 8u) * (G / 8u) + b / 8u : b;
-  const int x = (int)(row / (unsigned)NY), y = (int)(row % (unsigned)NY);
+                                               int NZ, int tpr, int rpb) {
+  __shared__ float4 stage[3 * 256];
+  const unsigned Lb = xcd_logical(blockIdx.x, gridDim.x);
+  const int lr = (int)threadIdx.x / tpr, t = (int)threadIdx.x - lr * tpr;
+  const unsigned nrows = (unsigned)NX * (unsigned)NY;
+  const unsigned row0 = Lb * (unsigned)rpb, row = row0 + (unsigned)lr;
+  const bool valid = row < nrows;
+  const int x = valid ? (int)(row / (unsigned)NY) : 0, y = valid ? (int)(row % (unsigned)NY) : 0;
   auto wr = [](int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); };
-  const size_t r0 = (size_t)row * NZ;
-  const size_t xp1 = ((size_t)wr(x + 1, NX) * NY + y) * NZ, xm1 = ((size_t)wr(x - 1, NX) * NY + y) * NZ;
-  const size_t xp2 = ((size_t)wr(x + 2, NX) * NY + y) * NZ, xm2 = ((size_t)wr(x - 2, NX) * NY + y) * NZ;
-  const size_t yp1 = ((size_t)x * NY + wr(y + 1, NY)) * NZ, ym1 = ((size_t)x * NY + wr(y - 1, NY)) * NZ;
-  const size_t yp2 = ((size_t)x * NY + wr(y + 2, NY)) * NZ, ym2 = ((size_t)x * NY + wr(y - 2, NY)) * NZ;
-  float* out = F + r0 * 3;
-  for (int z = threadIdx.x; z < NZ; z += blockDim.x) {
-    const float gx = 8.0f * (phi[xp1 + z] - phi[xm1 + z]) - (phi[xp2 + z] - phi[xm2 + z]);
-    const float gy = 8.0f * (phi[yp1 + z] - phi[ym1 + z]) - (phi[yp2 + z] - phi[ym2 + z]);
-    const float gz = 8.0f * (phi[r0 + wr(z + 1, NZ)] - phi[r0 + wr(z - 1, NZ)]) -
-                     (phi[r0 + wr(z + 2, NZ)] - phi[r0 + wr(z - 2, NZ)]);
-    out[3 * z] = gx;
-    out[3 * z + 1] = gy;
-    out[3 * z + 2] = gz;
+  const int z0 = 4 * t;
+  const unsigned rv = valid ? row : 0u;  // rows past the end load row 0 and store nothing
+  const size_t r0 = (size_t)rv * NZ + z0;
+  auto row_at = [&](int xx, int yy) { return *reinterpret_cast<const float4*>(phi + ((size_t)xx * NY + yy) * NZ + z0); };
+  const float4 xp1 = row_at(wr(x + 1, NX), y), xm1 = row_at(wr(x - 1, NX), y);
+  const float4 xp2 = row_at(wr(x + 2, NX), y), xm2 = row_at(wr(x - 2, NX), y);
+  const float4 yp1 = row_at(x, wr(y + 1, NY)), ym1 = row_at(x, wr(y - 1, NY));
+  const float4 yp2 = row_at(x, wr(y + 2, NY)), ym2 = row_at(x, wr(y - 2, NY));
+  const float4 c = *reinterpret_cast<const float4*>(phi + r0);
+  const float2 lo = *reinterpret_cast<const float2*>(phi + (size_t)rv * NZ + (z0 == 0 ? NZ - 2 : z0 - 2));
+  const float2 hi = *reinterpret_cast<const float2*>(phi + (size_t)rv * NZ + (z0 + 4 == NZ ? 0 : z0 + 4));
+  const float v[8] = {lo.x, lo.y, c.x, c.y, c.z, c.w, hi.x, hi.y};  // z0 - 2 .. z0 + 5
+  const float ax1[4] = {xp1.x, xp1.y, xp1.z, xp1.w}, bx1[4] = {xm1.x, xm1.y, xm1.z, xm1.w};
+  const float ax2[4] = {xp2.x, xp2.y, xp2.z, xp2.w}, bx2[4] = {xm2.x, xm2.y, xm2.z, xm2.w};
+  const float ay1[4] = {yp1.x, yp1.y, yp1.z, yp1.w}, by1[4] = {ym1.x, ym1.y, ym1.z, ym1.w};
+  const float ay2[4] = {yp2.x, yp2.y, yp2.z, yp2.w}, by2[4] = {ym2.x, ym2.y, ym2.z, ym2.w};
+  float o[12];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    o[3 * m] = 8.0f * (ax1[m] - bx1[m]) - (ax2[m] - bx2[m]);
+    o[3 * m + 1] = 8.0f * (ay1[m] - by1[m]) - (ay2[m] - by2[m]);
+    o[3 * m + 2] = 8.0f * (v[m + 3] - v[m + 1]) - (v[m + 4] - v[m]);
   }
+  // transpose through LDS: the block's outputs are one contiguous span (its rows are consecutive),
+  // stored as 16-byte lanes contiguous across each store instruction
+  const int T = (int)blockDim.x, me = (int)threadIdx.x;
+  stage[3 * me] = make_float4(o[0], o[1], o[2], o[3]);
+  stage[3 * me + 1] = make_float4(o[4], o[5], o[6], o[7]);
+  stage[3 * me + 2] = make_float4(o[8], o[9], o[10], o[11]);
+  __syncthreads();
+  const unsigned vrows = min((unsigned)rpb, nrows - row0);  // row0 < nrows for every launched block
+  const int nq = 3 * (int)vrows * tpr;
+  float4* out = reinterpret_cast<float4*>(F + (size_t)row0 * NZ * 3);
+  for (int q = me; q < nq; q += T) out[q] = stage[q];
 }
 
 // ---------------------------------------------------------------- host dispatch
@@ -566,9 +593,16 @@ int pmx_pass_c2r_z(pmx_plan_t* p, int ncomp, const float2* inD, float* out, hipS
 }
 
 int pmx_grad4(pmx_plan_t* p, const float* phi, float* F, hipStream_t s) {
+  // every supported axis length is a multiple of 4 and at most 1024 (tpr <= 256)
   const unsigned rows = (unsigned)p->nx * p->ny;
-  const int threads = p->nz < 256 ? p->nz : 256;
-  hipLaunchKernelGGL(k_grad4, dim3(rows), dim3(threads), 0, s, phi, F, p->nx, p->ny, p->nz);
+  const int tpr = p->nz / 4;
+  int rpb = 256 / tpr;
+  while (rpb > 1 && (tpr * rpb) % 64 != 0) --rpb;  // whole waves per block
+  if (p->nz % 4 != 0 || tpr > 256) return pmx_set_error(2, "grad4 needs nz a multiple of 4, at most 1024");
+  if ((reinterpret_cast<uintptr_t>(phi) | reinterpret_cast<uintptr_t>(F)) & 15u)
+    return pmx_set_error(2, "grad4 needs 16-byte aligned potential and force buffers");
+  hipLaunchKernelGGL(k_grad4, dim3((rows + rpb - 1) / rpb), dim3(tpr * rpb), 0, s, phi, F, p->nx, p->ny, p->nz, tpr,
+                     rpb);
   return pmx_check_launch("grad4");
 }
 

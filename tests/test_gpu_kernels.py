@@ -103,7 +103,7 @@ def test_rfft_matches_numpy(pkg, cuda, shape):
     assert rel(back / np.prod(shape), x) < 2e-6
 
 
-@pytest.mark.parametrize("shape", [(32, 32, 32), (32, 32, 64), (64, 96, 128)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (32, 32, 64), (64, 96, 128), (8, 8, 8), (8, 96, 192), (96, 8, 16)])
 @pytest.mark.parametrize("deconv", [False, True])
 def test_force_mesh(pkg, cuda, shape, deconv):
     rng = np.random.default_rng(6)
