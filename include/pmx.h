@@ -93,6 +93,12 @@ int pmx_forces(pmx_plan plan, const float* pos, int npx, int npy, int npz, int m
 int pmx_step(pmx_plan plan, float* x, float* v, int npx, int npy, int npz, int mode, float force_scale,
              float c_drift_pre, float c_kick, float c_drift_post, unsigned flags, void* stream);
 
+/* pmx_step, with the step's force mesh written to force_mesh (nx*ny*nz*3 floats, caller-owned,
+ * interleaved (x, y, z, 3)) instead of the plan's work buffer, where it stays for
+ * pmx_step_reverse_vjp_saved. Same bodies as pmx_step (solvers.py:151-202, 258-325). */
+int pmx_step_keep(pmx_plan plan, float* x, float* v, int npx, int npy, int npz, int mode, float force_scale,
+                  float c_drift_pre, float c_kick, float c_drift_post, unsigned flags, float* force_mesh, void* stream);
+
 /* Binned power spectrum sums of a real mesh (the estimator of _power,
  * src/jax_fli/_src/power/_compute.py:68-119): for bin b = #(kedges <= |k|), b in [0, nedges],
  * pk_sum[b] = sum |FFT(mesh)|^2, k_sum[b] = sum |k|, k_cnt[b] = mode count, over the full c2c grid.
@@ -234,6 +240,17 @@ int pmx_forces_vjp(pmx_plan plan, const float* pos, int npx, int npy, int npz, i
 int pmx_step_reverse_vjp(pmx_plan plan, float* x, float* v, int npx, int npy, int npz, int mode, float force_scale,
                          float c_kick, float c_drift, unsigned flags, float* x_bar, float* v_bar, double* coef_bar,
                          void* stream);
+
+/* pmx_step_reverse_vjp with the force mesh of that step taken from force_mesh (nx*ny*nz*3 floats,
+ * interleaved as the plan's own), kept by pmx_step_keep during the forward pass, instead of being
+ * recomputed at the restored positions: the same inverse and cotangent update without the paint,
+ * the five FFT passes and the gradient stencil of the force. The restored x differs from the
+ * forward's by the float32 rounding of x - v cd, as in pmx_step_reverse_vjp. Replaces the per-step
+ * reverse + jax.vjp(step) of _integrate_bwd (src/jax_fli/pm/integrate.py:304-358) when the HBM
+ * holds one force mesh per step. */
+int pmx_step_reverse_vjp_saved(pmx_plan plan, float* x, float* v, int npx, int npy, int npz, int mode,
+                               float force_scale, float c_kick, float c_drift, unsigned flags, const float* force_mesh,
+                               float* x_bar, float* v_bar, double* coef_bar, void* stream);
 
 /* ---- Born convergence ---------------------------------------------------------------------------
  * kappa[k][p] = sum_s coef[k][s] * (maps[s][p] / mean_s - 1) over `nshell` HEALPix (or any

@@ -42,6 +42,7 @@ SIGNATURES = {
     "pmx_force_mesh": (_i, [_vp, _vp, _f, _u, _vp, _vp]),
     "pmx_forces": (_i, [_vp, _vp, _i, _i, _i, _i, _f, _u, _vp, _vp]),
     "pmx_step": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _f, _u, _vp]),
+    "pmx_step_keep": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _f, _u, _vp, _vp]),
     "pmx_power": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "pmx_white_noise": (_i, [_vp, _c.c_uint64, _vp, _vp]),
     "pmx_philox4x32": (_i, [_c.c_uint64, _c.c_size_t, _vp, _vp]),
@@ -54,6 +55,7 @@ SIGNATURES = {
     "pmx_force_mesh_vjp": (_i, [_vp, _vp, _f, _u, _vp, _vp]),
     "pmx_forces_vjp": (_i, [_vp, _vp, _i, _i, _i, _i, _f, _u, _vp, _vp, _i, _vp, _vp]),
     "pmx_step_reverse_vjp": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _u, _vp, _vp, _vp, _vp]),
+    "pmx_step_reverse_vjp_saved": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _u, _vp, _vp, _vp, _vp, _vp]),
     "pmx_lpt1_lattice": (_i, [_vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp]),
     "pmx_lpt1_vjp": (_i, [_vp, _vp, _vp, _i, _i, _i, _f, _f, _vp, _vp]),
     "pmx_paint_healpix_tiled": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _dp, _dp, _d, _d, _i, _i, _i, _vp, _vp, _vp,

@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import weakref
 
 import numpy as np
 import torch
@@ -32,6 +33,24 @@ from . import engine
 
 
 # ---------------------------------------------------------------------------- parameters
+_PARAM_VALUES: dict = {}
+
+
+def _param_value(v: torch.Tensor) -> float:
+    """Host value of a scalar parameter tensor, read from the device once per tensor version (the
+    read waits for the device, so the stages of one forward model share it). Entries are keyed by
+    identity (a tensor's == is elementwise and would itself read the device)."""
+    hit = _PARAM_VALUES.get(id(v))
+    if hit is not None and hit[0]() is v and hit[1] == v._version:
+        return hit[2]
+    val = float(v.detach().cpu())
+    if len(_PARAM_VALUES) >= 64:
+        for key in [k for k, e in _PARAM_VALUES.items() if e[0]() is None]:
+            del _PARAM_VALUES[key]
+    _PARAM_VALUES[id(v)] = (weakref.ref(v), v._version, val)
+    return val
+
+
 def split_cosmo(cosmo):
     """(float-valued Cosmology, names, tensors) of the parameters given as tensors needing grad."""
     if cosmo is None:
@@ -40,7 +59,7 @@ def split_cosmo(cosmo):
     for f in dataclasses.fields(cosmo):
         v = getattr(cosmo, f.name)
         if isinstance(v, torch.Tensor):
-            vals[f.name] = float(v.detach().cpu())
+            vals[f.name] = _param_value(v)
             if v.requires_grad:
                 names.append(f.name)
                 tensors.append(v)
@@ -88,16 +107,35 @@ def _interp_apply(setup, fp):
     return f0 + t * (f1 - f0)
 
 
+_SETUPS: dict = {}
+
+
+def _colour_setup(mesh_size, box_size, k, device):
+    """Interpolation set-up of the table k at the r2c grid's |k|, the half-spectrum weights
+    (1 on the kz = 0 and Nyquist planes, 2 elsewhere) over N, cached per (mesh, box, table, device)."""
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    key = (tuple(mesh_size), tuple(box_size), k.tobytes(), str(device))
+    hit = _SETUPS.get(key)
+    if hit is None:
+        if len(_SETUPS) >= 16:
+            _SETUPS.clear()
+        setup = _interp_setup(_kgrid(mesh_size, box_size, device), engine.to_device(k, device))
+        nz = mesh_size[2]
+        wz = torch.full((nz // 2 + 1,), 2.0, dtype=torch.float64, device=device)
+        wz[0] = 1.0
+        if nz % 2 == 0:
+            wz[-1] = 1.0
+        hit = _SETUPS[key] = (setup, wz / float(np.prod(mesh_size)))
+    return hit
+
+
 class _Colour(torch.autograd.Function):
     @staticmethod
     def forward(ctx, noise, spec, *params):
         colour_fn, table_fn, cosmo, names, mesh_size, box_size = spec
         ctx.save_for_backward(noise)
         ctx.spec, ctx.params = spec, params
-        out = colour_fn(noise, cosmo)
-        # host derivatives of the P(k) table, computed while the device colours the field
-        ctx.dpts = [fd(lambda c: table_fn(c)[1], cosmo, name) for name in names]
-        return out
+        return colour_fn(noise, cosmo)
 
     @staticmethod
     def backward(ctx, g):
@@ -109,21 +147,18 @@ class _Colour(torch.autograd.Function):
         if names:
             # <g, irfft(rfft(noise) dA/dtheta)> = (1/N) sum_k w_k Re(conj(G_k) E_k) dA/dtheta(k) over the
             # half spectrum (w_k = 1 on the kz = 0 and Nyquist planes, 2 elsewhere), A = sqrt(interp(P N/V))
+            # device work is queued first (no host transfer waits on it); the host's float64
+            # derivatives of the table are computed while it runs
             dev = noise.device
             cross = (torch.conj(torch.fft.rfftn(g)) * torch.fft.rfftn(noise.to(torch.float32))).real.double()
             k, pt = table_fn(cosmo)
-            dpts = ctx.dpts
-            kt = torch.as_tensor(k, dtype=torch.float64, device=dev)
-            setup = _interp_setup(_kgrid(mesh_size, box_size, dev), kt)
-            amp = torch.sqrt(torch.clamp(_interp_apply(setup, torch.as_tensor(pt, device=dev)), min=0.0))
-            nz = mesh_size[2]
-            wz = torch.full((nz // 2 + 1,), 2.0, dtype=torch.float64, device=dev)
-            wz[0] = 1.0
-            if nz % 2 == 0:
-                wz[-1] = 1.0
-            cross = cross * wz / float(np.prod(mesh_size))
+            setup, wz = _colour_setup(mesh_size, box_size, k, dev)
+            amp = torch.sqrt(torch.clamp(_interp_apply(setup, engine.to_device(pt, dev)), min=0.0))
+            cross = cross * wz
             inv2a = torch.where(amp > 0, 0.5 / torch.where(amp > 0, amp, torch.ones_like(amp)), torch.zeros_like(amp))
-            sums = [(cross * _interp_apply(setup, torch.as_tensor(d, device=dev)) * inv2a).sum() for d in dpts]
+            cross = cross * inv2a
+            dpts = [fd(lambda c: table_fn(c)[1], cosmo, name) for name in names]
+            sums = [(cross * _interp_apply(setup, engine.to_device(d, dev))).sum() for d in dpts]
             grads = [_param_grad(t, v) for t, v in zip(ctx.params, sums)]
         return (gn, None, *grads)
 
@@ -152,7 +187,6 @@ class _LPT(torch.autograd.Function):
         ctx.save_for_backward(dx)
         ctx.spec, ctx.params, ctx.DV = spec, params, (D, vf)
         ctx.mesh = tuple(ic.shape)
-        ctx.ders = [fd(lambda c: _lpt_factors(c, a), cosmo, name) for name in names]  # overlaps the device
         return dx, p
 
     @staticmethod
@@ -171,7 +205,9 @@ class _LPT(torch.autograd.Function):
             gx = zero if dx_bar is None else (dx_bar * dx).sum(dtype=torch.float64)
             gv = zero if p_bar is None else (p_bar * dx).sum(dtype=torch.float64)
             gD = (gx + vf * gv) / D
-            for (dD, dv), t in zip(ctx.ders, ctx.params):
+            cosmo, names, a, _particles = ctx.spec
+            ders = [fd(lambda c: _lpt_factors(c, a), cosmo, name) for name in names]  # host, while the device runs
+            for (dD, dv), t in zip(ders, ctx.params):
                 grads.append(_param_grad(t, gD * dD + gv * dv))
         return (ic_bar, None, *grads)
 
@@ -193,12 +229,9 @@ class _NBodyReverse(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x0, v0, spec, *params):
         run, coef_fn, cosmo, names = spec
-        x, v, snaps, steps = run(x0, v0, cosmo)
+        x, v, snaps, steps, meshes = run(x0, v0, cosmo)
         ctx.save_for_backward(x.clone(), v.clone())
-        ctx.spec, ctx.params, ctx.steps = spec, params, steps
-        # host derivatives of the step coefficients, computed while the device runs the steps
-        ctx.co = np.asarray(coef_fn(cosmo)) if names else None
-        ctx.ders = [(fd(coef_fn, cosmo, name), float(fd(run.info["force_scale"], cosmo, name))) for name in names]
+        ctx.spec, ctx.params, ctx.steps, ctx.meshes = spec, params, steps, meshes
         return snaps
 
     @staticmethod
@@ -216,18 +249,22 @@ class _NBodyReverse(torch.autograd.Function):
                 xb += info["save_vjp"](ev[1], x, snaps_bar[ev[1]])
             else:
                 _, i, ck, cd = ev
-                cb = engine.step_reverse_vjp(x, v, xb, vb, mesh, mode=mode, force_scale=fs, c_kick=ck, c_drift=cd)
+                cb = engine.step_reverse_vjp(x, v, xb, vb, mesh, mode=mode, force_scale=fs, c_kick=ck, c_drift=cd,
+                                             force_mesh=None if ctx.meshes is None else ctx.meshes[i])
                 coef_bars.append((i, cb))
         grads = []
         if names:
-            co, ders = ctx.co, ctx.ders
+            # host derivatives of the step coefficients, computed while the device runs the reverse steps
+            co = np.asarray(coef_fn(cosmo))
+            ders = [(fd(coef_fn, cosmo, name), float(fd(info["force_scale"], cosmo, name))) for name in names]
             idx = [i for i, _ in coef_bars]
             allcb = torch.stack([cb for _, cb in coef_bars])  # (nsteps, 2) = (dL/dcd, dL/dck), on device
             for (dco, dfs), t in zip(ders, ctx.params):
                 # v' = v + F ck, x' = x + v' cd; F = fs F1 enters every kick
-                wck = torch.as_tensor(dco[idx, 0] + co[idx, 0] / fs * dfs, device=allcb.device)
-                wcd = torch.as_tensor(dco[idx, 1], device=allcb.device)
+                wck = engine.to_device(dco[idx, 0] + co[idx, 0] / fs * dfs, allcb.device)
+                wcd = engine.to_device(dco[idx, 1], allcb.device)
                 grads.append(_param_grad(t, (allcb[:, 1] * wck + allcb[:, 0] * wcd).sum()))
+        ctx.meshes = None  # the kept force meshes are released with the backward pass
         return (xb, vb, None, *grads)
 
 

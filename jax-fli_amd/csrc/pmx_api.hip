@@ -196,15 +196,16 @@ int pot_force_mesh_from(pmx_plan_t* p, const float* src, float force_scale, unsi
 // R^T is the sorted int64 deposit of the three components (scale from max |u| in p->ps_misc[3]);
 // S^T = -sum_d S_d; dot (+)= u . F. With vkick the same gather also applies v += F ckick (the
 // reverse step's un-kick).
+// M: the force mesh at the particles' positions (the plan's own, or one the caller kept)
 int force_adjoint(pmx_plan_t* p, const PartArgs& a, const float* u, float umul, float force_scale, unsigned flags,
-                  float* xbar, int accumulate, double* dot, float* vkick, float ckick, hipStream_t s) {
+                  const float* M, float* xbar, int accumulate, double* dot, float* vkick, float ckick, hipStream_t s) {
   float* U = reinterpret_cast<float*>(p->specC);  // 3 scalar meshes; consumed by div4 before the FFT reuses it
   PMX_TRY(pmx_launch_set_scale(p->ps_misc + 3, umul, p->ps_scale, s));
   PMX_TRY(pmx_paint_deposit(p, a, u, 3, umul, 0.f, 1, U, 1, 0, s, 3, p->ncell));
   PMX_TRY(pmx_launch_div4(p, U, U + p->ncell, U + 2 * p->ncell, 1, p->rho, s));
   PMX_TRY(pot_only(p, p->rho, force_scale, flags, s));
-  return pmx_launch_vjp_gather(p, a, 3, u, umul, p->fmesh, p->rho, nullptr, 1.f, xbar, accumulate, nullptr, dot, vkick,
-                               ckick, s);
+  return pmx_launch_vjp_gather(p, a, 3, u, umul, M, p->rho, nullptr, 1.f, xbar, accumulate, nullptr, dot, vkick, ckick,
+                               s);
 }
 
 // Events come from a pool created when timing is enabled, so a timed step creates none (event
@@ -439,6 +440,19 @@ int pmx_step(pmx_plan p, float* x, float* v, int npx, int npy, int npz, int mode
   return 0;
 }
 
+int pmx_step_keep(pmx_plan p, float* x, float* v, int npx, int npy, int npz, int mode, float force_scale,
+                  float c_drift_pre, float c_kick, float c_drift_post, unsigned flags, float* force_mesh, void* stream) {
+  PMX_TRY(check_plan(p));
+  if (!force_mesh) return pmx_set_error(1, "null force mesh");
+  // the launches capture the buffer pointer when they are queued, so the plan's own mesh can be
+  // restored at once
+  float* own = p->fmesh;
+  p->fmesh = force_mesh;
+  const int rc = pmx_step(p, x, v, npx, npy, npz, mode, force_scale, c_drift_pre, c_kick, c_drift_post, flags, stream);
+  p->fmesh = own;
+  return rc;
+}
+
 int pmx_pgd(pmx_plan p, float* x, float* v, int npx, int npy, int npz, int mode, float alpha, float kl, float ks,
             int target, float v_scale, void* stream) {
   PMX_TRY(check_plan(p));
@@ -477,7 +491,7 @@ int pmx_forces_vjp(pmx_plan p, const float* pos, int npx, int npy, int npz, int 
   PMX_TRY(pmx_paint_deposit(p, a, nullptr, 1, 1.f, 1.f, 0, p->rho, 1, 0, s));
   PMX_TRY(pot_force_mesh_from(p, p->rho, scale, flags, p->fmesh, s));
   PMX_TRY(pmx_launch_absmax(forces_bar, 3 * np, p->ps_misc + 3, s));
-  return force_adjoint(p, a, forces_bar, 1.f, scale, flags, pos_bar, accumulate, dot, nullptr, 0.f, s);
+  return force_adjoint(p, a, forces_bar, 1.f, scale, flags, p->fmesh, pos_bar, accumulate, dot, nullptr, 0.f, s);
 }
 
 int pmx_force_mesh_vjp(pmx_plan p, const float* fmesh_bar, float scale, unsigned flags, float* density_bar,
@@ -538,7 +552,25 @@ int pmx_step_reverse_vjp(pmx_plan p, float* x, float* v, int npx, int npy, int n
   PMX_TRY(pmx_paint_deposit(p, a, nullptr, 1, 1.f, 1.f, 0, p->rho, 1, 0, s));
   PMX_TRY(pot_force_mesh_from(p, p->rho, force_scale, flags, p->fmesh, s));
   // x_bar += J(x)^T (ck v_bar); coef_bar[1] = v_bar . F(x); the same gather un-kicks v -= F(x) ck
-  return force_adjoint(p, a, v_bar, c_kick, force_scale, flags, x_bar, 1, coef_bar + 1, v, -c_kick, s);
+  return force_adjoint(p, a, v_bar, c_kick, force_scale, flags, p->fmesh, x_bar, 1, coef_bar + 1, v, -c_kick, s);
+}
+
+int pmx_step_reverse_vjp_saved(pmx_plan p, float* x, float* v, int npx, int npy, int npz, int mode,
+                               float force_scale, float c_kick, float c_drift, unsigned flags, const float* force_mesh,
+                               float* x_bar, float* v_bar, double* coef_bar, void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, x, npx, npy, npz, mode));
+  if (!v || !x_bar || !v_bar || !coef_bar || !force_mesh) return pmx_set_error(1, "null buffer");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t n = (size_t)npx * npy * npz * 3;
+  PMX_TRY(pmx_launch_fill_d(coef_bar, 2, s));
+  PMX_TRY(pmx_launch_adj_prep(p, x_bar, v_bar, v, n, c_drift, coef_bar, p->ps_misc + 3, s));
+  // x -= v cd inside the sort's count pass; the sort serves the adjoint's deposit, and the force
+  // mesh at x is the one the forward step kept
+  PartArgs a = part_args(x, v, npx, npy, npz, mode, p);
+  PMX_TRY(pmx_paint_sort(p, a, -c_drift, nullptr, 1, 1.f, s));
+  a.v = nullptr;
+  return force_adjoint(p, a, v_bar, c_kick, force_scale, flags, force_mesh, x_bar, 1, coef_bar + 1, v, -c_kick, s);
 }
 
 int pmx_plan_timing(pmx_plan p, int enable) {

@@ -24,6 +24,19 @@ def _f32(t: torch.Tensor, name: str) -> torch.Tensor:
     return t.contiguous()
 
 
+def to_device(a, device, dtype=torch.float64) -> torch.Tensor:
+    """Host array or tensor -> contiguous tensor on `device`. From host memory to a GPU the copy goes
+    through a pinned staging buffer and is asynchronous on the current stream, so the host does not
+    wait for the work already queued there (a pageable copy would)."""
+    t = torch.as_tensor(a, dtype=dtype)
+    device = torch.device(device)
+    if t.device == device:
+        return t.contiguous()
+    if t.device.type == "cpu" and device.type == "cuda":
+        return t.contiguous().pin_memory().to(device, non_blocking=True)
+    return t.to(device).contiguous()
+
+
 def _pgrid(pos: torch.Tensor):
     if pos.ndim != 4 or pos.shape[-1] != 3:
         raise ValueError(f"particle array must have shape (X, Y, Z, 3), got {tuple(pos.shape)}")
@@ -110,8 +123,10 @@ def forces(pos: torch.Tensor, mesh_shape, mode="relative", scale=1.0, deconv=Fal
 
 
 def step(x: torch.Tensor, v: torch.Tensor, mesh_shape, mode="relative", force_scale=1.0, c_kick=0.0,
-         c_drift_pre=None, c_drift_post=None, deconv=False, atomic_paint=False, plan=None) -> None:
-    """In place: [x += v*c_drift_pre]; v += F(x)*force_scale*c_kick; [x += v*c_drift_post]."""
+         c_drift_pre=None, c_drift_post=None, deconv=False, atomic_paint=False, plan=None,
+         force_mesh: torch.Tensor | None = None) -> None:
+    """In place: [x += v*c_drift_pre]; v += F(x)*force_scale*c_kick; [x += v*c_drift_post]. With
+    force_mesh (float32 (nx, ny, nz, 3)), the step's force mesh is written there and kept."""
     if not (x.is_contiguous() and v.is_contiguous()):
         raise _lib.PmxError("state tensors must be contiguous")
     _f32(x, "x")
@@ -123,9 +138,21 @@ def step(x: torch.Tensor, v: torch.Tensor, mesh_shape, mode="relative", force_sc
     flags = (_lib.DECONV if deconv else 0) | (_lib.PAINT_ATOMIC if atomic_paint else 0)
     flags |= _lib.DRIFT_PRE if c_drift_pre is not None else 0
     flags |= _lib.DRIFT_POST if c_drift_post is not None else 0
+    if force_mesh is not None:
+        _force_mesh_arg(force_mesh, mesh_shape)
+        check(load().pmx_step_keep(plan.handle, ptr(x), ptr(v), npx, npy, npz, MODES[mode], float(force_scale),
+                                   float(c_drift_pre or 0.0), float(c_kick), float(c_drift_post or 0.0), flags,
+                                   ptr(force_mesh), stream_ptr()), "pmx_step_keep")
+        return
     check(load().pmx_step(plan.handle, ptr(x), ptr(v), npx, npy, npz, MODES[mode], float(force_scale),
                           float(c_drift_pre or 0.0), float(c_kick), float(c_drift_post or 0.0), flags,
                           stream_ptr()), "pmx_step")
+
+
+def _force_mesh_arg(force_mesh: torch.Tensor, mesh_shape) -> None:
+    _f32(force_mesh, "force_mesh")
+    if tuple(force_mesh.shape) != tuple(int(n) for n in mesh_shape) + (3,) or not force_mesh.is_contiguous():
+        raise ValueError("force_mesh must be a contiguous float32 (nx, ny, nz, 3) tensor")
 
 
 def pgd(x: torch.Tensor, v, mesh_shape, mode="relative", alpha=0.2, kl=0.3, ks=1.0, target="x", v_scale=1.0,
@@ -147,7 +174,7 @@ def power_sums(mesh: torch.Tensor, box, kedges: torch.Tensor):
     mesh = _f32(mesh, "mesh")
     plan = get_plan(mesh.shape, mesh.device)
     plan.set_box(box)
-    kedges = kedges.to(device=mesh.device, dtype=torch.float64).contiguous()
+    kedges = to_device(kedges, mesh.device)
     nb = kedges.numel() + 1
     pk = torch.empty(nb, dtype=torch.float64, device=mesh.device)
     ks = torch.empty_like(pk)
@@ -179,8 +206,8 @@ def color_field(field: torch.Tensor, box, ktab: torch.Tensor, ptab: torch.Tensor
     field = _f32(field, "field")
     plan = plan or get_plan(field.shape, field.device)
     plan.set_box(box)
-    ktab = ktab.to(device=field.device, dtype=torch.float64).contiguous()
-    ptab = ptab.to(device=field.device, dtype=torch.float64).contiguous()
+    ktab = to_device(ktab, field.device)
+    ptab = to_device(ptab, field.device)
     if out is None:
         out = torch.empty_like(field)
     check(load().pmx_color_field(plan.handle, ptr(field), ptr(ktab), ptr(ptab), ktab.numel(), ptr(out),
@@ -233,8 +260,8 @@ def _tiles(tiles, device):
     if tiles is None:
         return 1, None, None, None
     rot, shift = tiles
-    rot = torch.as_tensor(rot, dtype=torch.int32).reshape(-1).to(device).contiguous()
-    shift = torch.as_tensor(shift, dtype=torch.float64).reshape(-1, 3).to(device).contiguous()
+    rot = to_device(torch.as_tensor(rot, dtype=torch.int32).reshape(-1), device, torch.int32)
+    shift = to_device(torch.as_tensor(shift, dtype=torch.float64).reshape(-1, 3), device)
     if rot.numel() != shift.shape[0] or rot.numel() < 1:
         raise ValueError("image table: one rotation code and one shift per image")
     return rot.numel(), rot, shift, (rot, shift)
@@ -308,7 +335,7 @@ def _born_args(maps, coef):
     maps = _f32(maps, "maps")
     if maps.ndim != 2:
         raise ValueError("maps must be (nshell, npix)")
-    coef = torch.as_tensor(coef, dtype=torch.float64).to(maps.device).contiguous()
+    coef = to_device(coef, maps.device)
     if coef.ndim != 2 or coef.shape[1] != maps.shape[0]:
         raise ValueError("coef must be (nsrc, nshell)")
     return maps, coef
@@ -425,9 +452,11 @@ def forces_vjp(pos: torch.Tensor, mesh_shape, forces_bar: torch.Tensor, mode="re
 
 
 def step_reverse_vjp(x, v, x_bar, v_bar, mesh_shape, mode="relative", force_scale=1.0, c_kick=0.0, c_drift=0.0,
-                     deconv=False, coef_bar: torch.Tensor | None = None) -> torch.Tensor:
+                     deconv=False, coef_bar: torch.Tensor | None = None,
+                     force_mesh: torch.Tensor | None = None) -> torch.Tensor:
     """In place: undo one reversible step (v += F ck; x += v cd) on (x, v) and pull (x_bar, v_bar) back
-    through it. Returns coef_bar (float64 device [dL/dc_drift, dL/dc_kick])."""
+    through it. Returns coef_bar (float64 device [dL/dc_drift, dL/dc_kick]). force_mesh: that step's
+    force mesh kept by step(..., force_mesh=) (not recomputed)."""
     for t, nm in ((x, "x"), (v, "v"), (x_bar, "x_bar"), (v_bar, "v_bar")):
         _f32(t, nm)
         if not t.is_contiguous():
@@ -436,6 +465,13 @@ def step_reverse_vjp(x, v, x_bar, v_bar, mesh_shape, mode="relative", force_scal
     plan = get_plan(mesh_shape, x.device)
     if coef_bar is None:
         coef_bar = torch.empty(2, dtype=torch.float64, device=x.device)
+    if force_mesh is not None:
+        _force_mesh_arg(force_mesh, mesh_shape)
+        check(load().pmx_step_reverse_vjp_saved(plan.handle, ptr(x), ptr(v), npx, npy, npz, MODES[mode],
+                                                float(force_scale), float(c_kick), float(c_drift),
+                                                _lib.DECONV if deconv else 0, ptr(force_mesh), ptr(x_bar), ptr(v_bar),
+                                                ptr(coef_bar), stream_ptr()), "pmx_step_reverse_vjp_saved")
+        return coef_bar
     check(load().pmx_step_reverse_vjp(plan.handle, ptr(x), ptr(v), npx, npy, npz, MODES[mode], float(force_scale),
                                       float(c_kick), float(c_drift), _lib.DECONV if deconv else 0, ptr(x_bar),
                                       ptr(v_bar), ptr(coef_bar), stream_ptr()), "pmx_step_reverse_vjp")

@@ -70,11 +70,11 @@ class _Born(torch.autograd.Function):
     @staticmethod
     def forward(ctx, maps, spec, *params):
         coef_fn, cosmo, names = spec
-        coef = torch.as_tensor(coef_fn(cosmo), dtype=torch.float64).to(maps.device)
+        ctx.spec = spec
+        coef = engine.to_device(coef_fn(cosmo), maps.device)
         kappa = engine.born(maps.contiguous(), coef)
         ctx.save_for_backward(maps)
         ctx.coef, ctx.params = coef, params
-        ctx.ders = [torch.as_tensor(autodiff.fd(coef_fn, cosmo, n), device=maps.device) for n in names]
         return kappa
 
     @staticmethod
@@ -84,9 +84,12 @@ class _Born(torch.autograd.Function):
         maps_bar, dots = engine.born_vjp(maps, ctx.coef, kappa_bar.contiguous(), want_dots=names)
         grads = []
         if names:
+            coef_fn, cosmo, pnames = ctx.spec
+            # host derivatives of C, computed while the device runs the adjoint
+            ders = [engine.to_device(autodiff.fd(coef_fn, cosmo, n), maps.device) for n in pnames]
             D, E, mean = dots
             g = D / mean[None, :] - E[:, None]  # d<kappa_bar, kappa>/d C
-            for dC, t in zip(ctx.ders, ctx.params):
+            for dC, t in zip(ders, ctx.params):
                 grads.append(autodiff._param_grad(t, (dC * g).sum()))
         return (maps_bar if ctx.needs_input_grad[0] else None, None, *grads)
 

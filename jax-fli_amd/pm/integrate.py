@@ -58,6 +58,22 @@ def _stack(snaps):
     return first.replace(array=arr, scale_factors=sf, comoving_centers=rc, density_width=dw)
 
 
+# HBM the differentiable integration may spend on keeping each step's force mesh for the adjoint
+# (12 bytes per cell and step), at most a quarter of the free device memory; beyond it the reverse
+# pass recomputes the force at the restored positions.
+KEEP_FORCE_MESH_BYTES = 16 << 30
+
+
+def _force_mesh_store(nsteps, mesh_size, device):
+    """(nsteps, nx, ny, nz, 3) float32 buffer for the steps' force meshes, or None over budget."""
+    need = nsteps * int(np.prod(mesh_size)) * 12
+    if nsteps == 0 or need > KEEP_FORCE_MESH_BYTES or device.type != "cuda":
+        return None
+    if need > torch.cuda.mem_get_info(device)[0] // 4:
+        return None
+    return torch.empty((nsteps,) + tuple(mesh_size) + (3,), dtype=torch.float32, device=device)
+
+
 def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0):
     """Differentiable integration: the reversible solver's fused steps forward, the reverse adjoint
     (pmx_step_reverse_vjp per step, snapshot paint VJPs) backward (integrate.py:147-193, 288-538)."""
@@ -112,16 +128,18 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
         fs = force_scale(cosmo_)
         snaps = []
         saved_fields.clear()
+        meshes = _force_mesh_store(len(grid), mesh_size, x.device)
         for ev in events:
             if ev[0] == "step":
-                engine.step(x, v, mesh_size, mode=mode, force_scale=fs, c_kick=ev[2], c_drift_post=ev[3])
+                engine.step(x, v, mesh_size, mode=mode, force_scale=fs, c_kick=ev[2], c_drift_post=ev[3],
+                            force_mesh=None if meshes is None else meshes[ev[1]])
             else:
                 _, idx, tt = ev
                 fld = solver.save_at(displacements.replace(array=x, scale_factors=tt),
                                      velocities.replace(array=v, scale_factors=tt), tt, dt0, _state(idx), cosmo_)
                 saved_fields.append(fld)
                 snaps.append(fld.array)
-        return x, v, torch.stack(snaps), len(grid)
+        return x, v, torch.stack(snaps), len(grid), meshes
 
     run.info = {"mesh": mesh_size, "mode": mode, "events": events, "force_scale": force_scale, "save_vjp": save_vjp}
 

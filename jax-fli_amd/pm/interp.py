@@ -35,7 +35,7 @@ def drift_tables(cosmo, a_now: float, chi_max: float, device) -> dict:
     dg_t = np.asarray(_c.dGfa(cosmo, ag), dtype=np.float64)
     gp_now = float(np.asarray(_c.Gp(cosmo, a_now)).reshape(-1)[0])
     f = (gp_t - gp_now) / np.asarray(_c.dGfa(cosmo, np.sqrt(a_now * ag)), dtype=np.float64)
-    tab = torch.as_tensor(np.stack([a_chi, gp_t, dg_t]), dtype=torch.float64).to(device)
+    tab = engine.to_device(np.stack([a_chi, gp_t, dg_t]), device)
     return {"tables": tab.contiguous(), "chi_max": chi_max, "a_lo": a_lo, "a_hi": a_hi, "a_now": float(a_now),
             "gp_now": gp_now, "bound": float(np.abs(f).max()) * 1.01 + 1e-12}
 

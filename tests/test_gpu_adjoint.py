@@ -166,3 +166,37 @@ def test_step_reverse_vjp(pkg, cuda, npart):
     c = cb.cpu().numpy()
     assert _rel(float(c[0]), ref_cd) < 1e-5
     assert _rel(float(c[1]), ref_ck) < 1e-5
+
+
+def test_step_keep_and_reverse_with_the_kept_force_mesh(pkg, cuda):
+    """The forward step that keeps its force mesh moves the state exactly as the plain step, and the
+    reverse step that reuses that mesh restores the state and the cotangents as the one that
+    recomputes it (the restored positions differ from the forward's by float32 rounding only)."""
+    n = 32
+    mesh = (n, n, n)
+    x = torch.from_numpy(_disp(mesh, 1.2, 41)).to(cuda)
+    v = torch.from_numpy(_disp(mesh, 0.4, 42)).to(cuda)
+    fs, ck, cd = 0.45, 0.7, 0.3
+    M = torch.empty(mesh + (3,), dtype=torch.float32, device=cuda)
+    xa, va = x.clone(), v.clone()
+    pkg.engine.step(xa, va, mesh, force_scale=fs, c_kick=ck, c_drift_post=cd, force_mesh=M)
+    xb_, vb_ = x.clone(), v.clone()
+    pkg.engine.step(xb_, vb_, mesh, force_scale=fs, c_kick=ck, c_drift_post=cd)
+    assert torch.equal(xa, xb_) and torch.equal(va, vb_)
+    F = pkg.engine.readout(M, x)  # the kept mesh read at the step's positions is the kick's force
+    assert _rel_l2(((va - v) / ck).cpu().numpy(), F.cpu().numpy()) < 1e-5
+    rng = np.random.default_rng(43)
+    xbar = torch.from_numpy(rng.standard_normal(mesh + (3,)).astype(np.float32)).to(cuda)
+    vbar = torch.from_numpy(rng.standard_normal(mesh + (3,)).astype(np.float32)).to(cuda)
+    out = []
+    for keep in (True, False):
+        xs, vs, xbs, vbs = xa.clone(), va.clone(), xbar.clone(), vbar.clone()
+        cb = pkg.engine.step_reverse_vjp(xs, vs, xbs, vbs, mesh, force_scale=fs, c_kick=ck, c_drift=cd,
+                                         force_mesh=M if keep else None)
+        out.append((xs, vs, xbs, vbs, cb))
+    (xk, vk, xbk, vbk, cbk), (xr, vr, xbr, vbr, cbr) = out
+    assert (xk - x).abs().max() < 1e-4 and (vk - v).abs().max() < 1e-4
+    assert torch.equal(xk, xr) and torch.equal(vbk, vbr)
+    assert _rel_l2(vk.cpu().numpy(), vr.cpu().numpy()) < 1e-6
+    assert _rel_l2(xbk.cpu().numpy(), xbr.cpu().numpy()) < 1e-5
+    assert np.allclose(cbk.cpu().numpy(), cbr.cpu().numpy(), rtol=1e-5)

@@ -9,6 +9,8 @@ Tolerance 2e-3 relative (float32 forward and adjoint against a float64 finite di
 
 from __future__ import annotations
 
+import importlib
+
 import numpy as np
 import pytest
 import torch
@@ -74,8 +76,12 @@ def test_forward_model_gradient_matches_oracle_fd(pkg, cuda):
         assert abs(got - fd) / abs(fd) < 2e-3
 
 
-def test_nbody_gradient_is_the_reverse_adjoint(pkg, cuda):
-    """d L / d (dx, p) of the N-body stage alone, against the float64 oracle run by finite differences."""
+@pytest.mark.parametrize("keep", [True, False])
+def test_nbody_gradient_is_the_reverse_adjoint(pkg, cuda, keep, monkeypatch):
+    """d L / d (dx, p) of the N-body stage alone, against the float64 oracle run by finite differences;
+    with the steps' force meshes kept for the reverse pass and with the force recomputed there."""
+    if not keep:
+        monkeypatch.setattr(importlib.import_module(pkg.__name__ + ".pm.integrate"), "KEEP_FORCE_MESH_BYTES", 0)
     rng = np.random.default_rng(62)
     c = pkg.Cosmology(**P)
     o = nbo.Cosmo(**P)
