@@ -269,6 +269,25 @@ int pmx_born(pmx_plan plan, const float* maps, int nshell, int64_t npix, const d
 int pmx_born_vjp(pmx_plan plan, const float* maps, int nshell, int64_t npix, const double* coef, int nsrc,
                  const float* kappa_bar, float* maps_bar, int accumulate, double* dots, void* stream);
 
+/* Flat-sky Born convergence (_born_core_impl's flat branch, src/jax_fli/_src/lensing/_born.py:53-98,
+ * called by _born_flat, :131-190): planes maps (nshell, ny, nx); each plane is resampled at output
+ * pixel (i, j) by periodic bilinear interpolation (map_coordinates order 1, mode "wrap") at row
+ * coordinate j * scales[2 s] - 1/2 and column coordinate i * scales[2 s + 1] - 1/2 (the reference's
+ * angular grid: x angles index rows, y angles columns, times chi_s / pixel size), then
+ *   kappa[k] (+)= sum_s coef[k, s] (R_s / mean(maps[s]) - 1)        (nsrc, ny, nx)
+ * with the plane means of the unresampled maps (0 -> FLT_EPSILON) and coef as for pmx_born. */
+int pmx_born_flat(pmx_plan plan, const float* maps, int nshell, int ny, int nx, const double* coef, int nsrc,
+                  const double* scales, float* kappa, int accumulate, void* stream);
+
+/* Cotangent of pmx_born_flat: maps_bar (+)= d<kappa_bar, kappa>/d maps (the resampling's transpose by
+ * float atomics, plus the means' term). dots (optional, nsrc*nshell + nsrc + 2*nshell doubles) =
+ * {D[k, s] = <kappa_bar_k, R_s>, E[k] = sum kappa_bar_k, the means as used, P[s] = sum_p g_s(p)
+ * (dR_s/du j scales[2s] + dR_s/dw i scales[2s+1]) with g_s = sum_k coef[k, s] kappa_bar_k / mean_s}:
+ * dL/dcoef[k, s] = D[k, s] / mean_s - E[k] and dL/dchi_s through the sampling positions = P[s] / chi_s. */
+int pmx_born_flat_vjp(pmx_plan plan, const float* maps, int nshell, int ny, int nx, const double* coef, int nsrc,
+                      const double* scales, const float* kappa_bar, float* maps_bar, int accumulate, double* dots,
+                      void* stream);
+
 #ifdef __cplusplus
 }
 #endif

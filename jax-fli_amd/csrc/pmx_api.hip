@@ -35,6 +35,11 @@ int pmx_launch_born(pmx_plan_t* p, const float* maps, int S, long npix, const do
                     int accumulate, hipStream_t s);
 int pmx_launch_born_vjp(pmx_plan_t* p, const float* maps, int S, long npix, const double* C, int K,
                         const float* kappa_bar, float* maps_bar, int accumulate, double* dots, hipStream_t s);
+int pmx_launch_born_flat(pmx_plan_t* p, const float* maps, int S, int ny, int nx, const double* C, int K,
+                         const double* sc, float* kappa, int accumulate, hipStream_t s);
+int pmx_launch_born_flat_vjp(pmx_plan_t* p, const float* maps, int S, int ny, int nx, const double* C, int K,
+                             const double* sc, const float* kappa_bar, float* maps_bar, int accumulate, double* dots,
+                             hipStream_t s);
 
 namespace {
 thread_local std::string g_err;
@@ -780,6 +785,24 @@ int pmx_born_vjp(pmx_plan p, const float* maps, int nshell, int64_t npix, const 
   if (!kappa_bar || !maps_bar) return pmx_set_error(1, "null cotangent");
   return pmx_launch_born_vjp(p, maps, nshell, (long)npix, coef, nsrc, kappa_bar, maps_bar, accumulate, dots,
                              (hipStream_t)stream);
+}
+
+int pmx_born_flat(pmx_plan p, const float* maps, int nshell, int ny, int nx, const double* coef, int nsrc,
+                  const double* scales, float* kappa, int accumulate, void* stream) {
+  PMX_TRY(check_born(p, maps, nshell, (int64_t)ny * nx, coef, nsrc));
+  if (ny < 1 || nx < 1) return pmx_set_error(1, "need ny, nx >= 1");
+  if (!kappa || !scales) return pmx_set_error(1, "null buffer");
+  return pmx_launch_born_flat(p, maps, nshell, ny, nx, coef, nsrc, scales, kappa, accumulate, (hipStream_t)stream);
+}
+
+int pmx_born_flat_vjp(pmx_plan p, const float* maps, int nshell, int ny, int nx, const double* coef, int nsrc,
+                      const double* scales, const float* kappa_bar, float* maps_bar, int accumulate, double* dots,
+                      void* stream) {
+  PMX_TRY(check_born(p, maps, nshell, (int64_t)ny * nx, coef, nsrc));
+  if (ny < 1 || nx < 1) return pmx_set_error(1, "need ny, nx >= 1");
+  if (!kappa_bar || !maps_bar || !scales) return pmx_set_error(1, "null buffer");
+  return pmx_launch_born_flat_vjp(p, maps, nshell, ny, nx, coef, nsrc, scales, kappa_bar, maps_bar, accumulate, dots,
+                                  (hipStream_t)stream);
 }
 
 int pmx_paint_flat(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, const float* weights,

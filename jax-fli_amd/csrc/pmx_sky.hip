@@ -686,6 +686,163 @@ __global__ void __launch_bounds__(256) k_born_vjp(const float* __restrict__ kbar
   }
 }
 
+// ---------------------------------------------------------------- flat-sky Born
+// Plane s (ny rows, nx columns) is resampled at output pixel (i, j) by periodic bilinear
+// interpolation at row coordinate u = j rs_s - 1/2 and column coordinate w = i cs_s - 1/2: the
+// angular grid of _born_core_impl (src/jax_fli/_src/lensing/_born.py:53-98: x angles index rows,
+// y angles columns, both scaled by chi_s / pixel size) through map_coordinates(order=1, mode="wrap").
+// sc[2 s] = rs_s, sc[2 s + 1] = cs_s.
+struct FlatSample {
+  double val, du, dw;  // value and its derivatives along u and w
+};
+
+__device__ __forceinline__ long wrap_long(long i, long n) {
+  const long r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+__device__ __forceinline__ void flat_corners(int ny, int nx, double u, double w, long& r0, long& r1, long& c0, long& c1,
+                                             double& tu, double& tw) {
+  const double fu = floor(u), fw = floor(w);
+  tu = u - fu;
+  tw = w - fw;
+  r0 = wrap_long((long)fu, ny);
+  c0 = wrap_long((long)fw, nx);
+  r1 = (r0 + 1 == ny) ? 0 : r0 + 1;
+  c1 = (c0 + 1 == nx) ? 0 : c0 + 1;
+}
+
+__device__ __forceinline__ FlatSample flat_sample(const float* __restrict__ pl, int ny, int nx, double u, double w) {
+  long r0, r1, c0, c1;
+  double tu, tw;
+  flat_corners(ny, nx, u, w, r0, r1, c0, c1, tu, tw);
+  const double v00 = pl[r0 * nx + c0], v01 = pl[r0 * nx + c1], v10 = pl[r1 * nx + c0], v11 = pl[r1 * nx + c1];
+  FlatSample f;
+  f.val = (1.0 - tu) * ((1.0 - tw) * v00 + tw * v01) + tu * ((1.0 - tw) * v10 + tw * v11);
+  f.du = (1.0 - tw) * (v10 - v00) + tw * (v11 - v01);
+  f.dw = (1.0 - tu) * (v01 - v00) + tu * (v11 - v10);
+  return f;
+}
+
+// kappa[k][p] = sum_s f[k][s] R_s(p) - cb[k], R_s the resampled plane (f = C / mean of the plane)
+__global__ void __launch_bounds__(256) k_born_flat(const float* __restrict__ maps, int ny, int nx, int S, int K,
+                                                   const double* __restrict__ coef, const double* __restrict__ sc,
+                                                   float* __restrict__ kappa, int accumulate) {
+  const double* f = coef;
+  const double* cb = coef + K * S + S;
+  const long npix = (long)ny * nx;
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < npix; p += (long)gridDim.x * blockDim.x) {
+    const long i = p / nx, j = p - i * nx;
+    double acc[kBornMaxSources];
+#pragma unroll
+    for (int k = 0; k < kBornMaxSources; ++k) acc[k] = 0.0;
+    for (int s = 0; s < S; ++s) {
+      const double R = flat_sample(maps + (size_t)s * npix, ny, nx, (double)j * sc[2 * s] - 0.5,
+                                   (double)i * sc[2 * s + 1] - 0.5).val;
+#pragma unroll
+      for (int k = 0; k < kBornMaxSources; ++k)
+        if (k < K) acc[k] += f[k * S + s] * R;
+    }
+#pragma unroll
+    for (int k = 0; k < kBornMaxSources; ++k)
+      if (k < K) {
+        const float v = (float)(acc[k] - cb[k]);
+        float* o = kappa + (size_t)k * npix + p;
+        *o = accumulate ? *o + v : v;
+      }
+  }
+}
+
+// partials of D[k][s] = <kbar_k, R_s> (y = s < S), E[k] = sum kbar_k (y = S), as k_born_dots, and of
+// P[s] = sum_p g_s(p) (dR_s/du j rs_s + dR_s/dw i cs_s) = chi_s d<g_s, R_s>/d chi_s with
+// g_s = sum_k f[k][s] kbar_k, at part[((S + 1) K + s) nblk + b]
+__global__ void __launch_bounds__(256) k_born_flat_dots(const float* __restrict__ maps, const float* __restrict__ kbar,
+                                                        int ny, int nx, int S, int K, int nblk,
+                                                        const double* __restrict__ coef, const double* __restrict__ sc,
+                                                        double* __restrict__ part) {
+  const int s = blockIdx.y, b = blockIdx.x;
+  const long npix = (long)ny * nx;
+  long beg, end;
+  chunk_of(npix, nblk, b, beg, end);
+  const double* f = coef;
+  double acc[kBornMaxSources];
+#pragma unroll
+  for (int k = 0; k < kBornMaxSources; ++k) acc[k] = 0.0;
+  double accp = 0.0;
+  for (long p = beg + threadIdx.x; p < end; p += blockDim.x) {
+    if (s < S) {
+      const long i = p / nx, j = p - i * nx;
+      const double rs = sc[2 * s], cs = sc[2 * s + 1];
+      const FlatSample fs = flat_sample(maps + (size_t)s * npix, ny, nx, (double)j * rs - 0.5, (double)i * cs - 0.5);
+      double g = 0.0;
+#pragma unroll
+      for (int k = 0; k < kBornMaxSources; ++k)
+        if (k < K) {
+          const double kb = (double)kbar[(size_t)k * npix + p];
+          acc[k] += kb * fs.val;
+          g += f[k * S + s] * kb;
+        }
+      accp += g * (fs.du * (double)j * rs + fs.dw * (double)i * cs);
+    } else {
+#pragma unroll
+      for (int k = 0; k < kBornMaxSources; ++k)
+        if (k < K) acc[k] += (double)kbar[(size_t)k * npix + p];
+    }
+  }
+  for (int k = 0; k < K; ++k) block_sum_to(acc[k], part + ((size_t)s * K + k) * nblk + b);
+  if (s < S) block_sum_to(accp, part + ((size_t)(S + 1) * K + s) * nblk + b);
+}
+
+// dots[K S + K + S + s] = P[s]
+__global__ void __launch_bounds__(256) k_born_flat_p(const double* __restrict__ part, int nblk, int S, int K,
+                                                     double* __restrict__ dots) {
+  for (int s = threadIdx.x; s < S; s += blockDim.x) {
+    double t = 0.0;
+    for (int b = 0; b < nblk; ++b) t += part[((size_t)(S + 1) * K + s) * nblk + b];
+    dots[K * S + K + S + s] = t;
+  }
+}
+
+// maps_bar[s] (+)= -h[s] everywhere (the mean's term); the scatter pass adds the rest
+__global__ void __launch_bounds__(256) k_born_flat_vjp_fill(long npix, int S, int K, const double* __restrict__ coef,
+                                                            float* __restrict__ maps_bar, int accumulate) {
+  const double* h = coef + K * S + S + K;
+  const size_t n = (size_t)S * npix;
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    const float v = (float)(-h[q / npix]);
+    maps_bar[q] = accumulate ? maps_bar[q] + v : v;
+  }
+}
+
+// maps_bar[s] += R_s^T g_s: each output pixel adds g_s(p) times its 4 bilinear weights (float atomics)
+__global__ void __launch_bounds__(256) k_born_flat_scatter(const float* __restrict__ kbar, int ny, int nx, int S, int K,
+                                                           const double* __restrict__ coef,
+                                                           const double* __restrict__ sc, float* __restrict__ maps_bar) {
+  const double* f = coef;
+  const long npix = (long)ny * nx;
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < npix; p += (long)gridDim.x * blockDim.x) {
+    const long i = p / nx, j = p - i * nx;
+    double kb[kBornMaxSources];
+#pragma unroll
+    for (int k = 0; k < kBornMaxSources; ++k) kb[k] = (k < K) ? (double)kbar[(size_t)k * npix + p] : 0.0;
+    for (int s = 0; s < S; ++s) {
+      double g = 0.0;
+#pragma unroll
+      for (int k = 0; k < kBornMaxSources; ++k)
+        if (k < K) g += f[k * S + s] * kb[k];
+      if (g == 0.0) continue;
+      long r0, r1, c0, c1;
+      double tu, tw;
+      flat_corners(ny, nx, (double)j * sc[2 * s] - 0.5, (double)i * sc[2 * s + 1] - 0.5, r0, r1, c0, c1, tu, tw);
+      float* mb = maps_bar + (size_t)s * npix;
+      unsafeAtomicAdd(mb + r0 * nx + c0, (float)(g * (1.0 - tu) * (1.0 - tw)));
+      unsafeAtomicAdd(mb + r0 * nx + c1, (float)(g * (1.0 - tu) * tw));
+      unsafeAtomicAdd(mb + r1 * nx + c0, (float)(g * tu * (1.0 - tw)));
+      unsafeAtomicAdd(mb + r1 * nx + c1, (float)(g * tu * tw));
+    }
+  }
+}
+
 HpGeom hp_geom(int nside) {
   HpGeom g;
   g.nside = nside;
@@ -812,4 +969,46 @@ int pmx_launch_born_vjp(pmx_plan_t* p, const float* maps, int S, long npix, cons
   hipLaunchKernelGGL(k_born_vjp, dim3(grid_for((size_t)npix, 256)), dim3(256), 0, s, kappa_bar, npix, S, K, coef,
                      maps_bar, accumulate);
   return pmx_check_launch("born_vjp");
+}
+
+int pmx_launch_born_flat(pmx_plan_t* p, const float* maps, int S, int ny, int nx, const double* C, int K,
+                         const double* sc, float* kappa, int accumulate, hipStream_t s) {
+  const long npix = (long)ny * nx;
+  const int nblk = born_blocks(npix, S);
+  double* coef = p->red + kBornCoef;
+  hipLaunchKernelGGL(k_map_sums, dim3(nblk, S), dim3(256), 0, s, maps, npix, nblk, p->red);
+  PMX_CHECK_LAUNCH("born_sums");
+  hipLaunchKernelGGL(k_born_coef, dim3(1), dim3(256), 0, s, p->red, nblk, S, K, npix, C, coef);
+  PMX_CHECK_LAUNCH("born_coef");
+  hipLaunchKernelGGL(k_born_flat, dim3(grid_for((size_t)npix, 256)), dim3(256), 0, s, maps, ny, nx, S, K, coef, sc,
+                     kappa, accumulate);
+  return pmx_check_launch("born_flat");
+}
+
+int pmx_launch_born_flat_vjp(pmx_plan_t* p, const float* maps, int S, int ny, int nx, const double* C, int K,
+                             const double* sc, const float* kappa_bar, float* maps_bar, int accumulate, double* dots,
+                             hipStream_t s) {
+  const long npix = (long)ny * nx;
+  double* coef = p->red + kBornCoef;
+  int nblk = born_blocks(npix, S);
+  hipLaunchKernelGGL(k_map_sums, dim3(nblk, S), dim3(256), 0, s, maps, npix, nblk, p->red);
+  PMX_CHECK_LAUNCH("born_sums");
+  hipLaunchKernelGGL(k_born_coef, dim3(1), dim3(256), 0, s, p->red, nblk, S, K, npix, C, coef);
+  PMX_CHECK_LAUNCH("born_coef");
+  nblk = born_blocks(npix, (long)(S + 1) * K + S);
+  hipLaunchKernelGGL(k_born_flat_dots, dim3(nblk, S + 1), dim3(256), 0, s, maps, kappa_bar, ny, nx, S, K, nblk, coef,
+                     sc, p->red);
+  PMX_CHECK_LAUNCH("born_flat_dots");
+  hipLaunchKernelGGL(k_born_vjp_coef, dim3(1), dim3(256), 0, s, p->red, nblk, S, K, npix, C, coef, dots);
+  PMX_CHECK_LAUNCH("born_vjp_coef");
+  if (dots) {
+    hipLaunchKernelGGL(k_born_flat_p, dim3(1), dim3(256), 0, s, p->red, nblk, S, K, dots);
+    PMX_CHECK_LAUNCH("born_flat_p");
+  }
+  hipLaunchKernelGGL(k_born_flat_vjp_fill, dim3(grid_for((size_t)S * npix, 256)), dim3(256), 0, s, npix, S, K, coef,
+                     maps_bar, accumulate);
+  PMX_CHECK_LAUNCH("born_flat_vjp_fill");
+  hipLaunchKernelGGL(k_born_flat_scatter, dim3(grid_for((size_t)npix, 256)), dim3(256), 0, s, kappa_bar, ny, nx, S, K,
+                     coef, sc, maps_bar);
+  return pmx_check_launch("born_flat_scatter");
 }

@@ -374,6 +374,54 @@ def born_vjp(maps: torch.Tensor, coef, kappa_bar: torch.Tensor, want_dots=False)
     return maps_bar, (dots[:K * S].view(K, S), dots[K * S:K * S + K], dots[K * S + K:])
 
 
+def _born_flat_args(maps, coef, scales):
+    maps = _f32(maps, "maps")
+    if maps.ndim != 3:
+        raise ValueError("flat maps must be (nshell, ny, nx)")
+    coef = to_device(coef, maps.device)
+    if coef.ndim != 2 or coef.shape[1] != maps.shape[0]:
+        raise ValueError("coef must be (nsrc, nshell)")
+    scales = to_device(scales, maps.device)
+    if tuple(scales.shape) != (int(maps.shape[0]), 2):
+        raise ValueError("scales must be (nshell, 2): row and column coordinate per pixel index")
+    return maps, coef, scales
+
+
+def born_flat(maps: torch.Tensor, coef, scales, out=None, accumulate=False) -> torch.Tensor:
+    """kappa (nsrc, ny, nx) = sum_s coef[k, s] (R_s / mean(maps[s]) - 1), R_s the plane resampled at
+    (row j scales[s, 0] - 1/2, column i scales[s, 1] - 1/2) for output pixel (i, j)."""
+    maps, coef, scales = _born_flat_args(maps, coef, scales)
+    S, ny, nx = (int(n) for n in maps.shape)
+    K = int(coef.shape[0])
+    plan = get_plan((8, 8, 8), maps.device)
+    if out is None:
+        out = torch.empty((K, ny, nx), dtype=torch.float32, device=maps.device)
+    check(load().pmx_born_flat(plan.handle, ptr(maps), S, ny, nx, ptr(coef), K, ptr(scales), ptr(out),
+                               int(accumulate), stream_ptr()), "pmx_born_flat")
+    return out
+
+
+def born_flat_vjp(maps: torch.Tensor, coef, scales, kappa_bar: torch.Tensor, want_dots=False):
+    """(maps_bar, dots | None) for born_flat; dots = (D (nsrc, nshell), E (nsrc,), mean (nshell,),
+    P (nshell,)) with D = <kappa_bar[k], R_s>, P[s] = chi_s d<kappa_bar, kappa>/d chi_s through the
+    sampling positions (float64, device)."""
+    maps, coef, scales = _born_flat_args(maps, coef, scales)
+    kappa_bar = _f32(kappa_bar, "kappa_bar")
+    S, ny, nx = (int(n) for n in maps.shape)
+    K = int(coef.shape[0])
+    if tuple(kappa_bar.shape) != (K, ny, nx):
+        raise ValueError("kappa_bar must be (nsrc, ny, nx)")
+    plan = get_plan((8, 8, 8), maps.device)
+    maps_bar = torch.empty_like(maps)
+    dots = torch.empty(K * S + K + 2 * S, dtype=torch.float64, device=maps.device) if want_dots else None
+    check(load().pmx_born_flat_vjp(plan.handle, ptr(maps), S, ny, nx, ptr(coef), K, ptr(scales), ptr(kappa_bar),
+                                   ptr(maps_bar), 0, ptr(dots), stream_ptr()), "pmx_born_flat_vjp")
+    if dots is None:
+        return maps_bar, None
+    return maps_bar, (dots[:K * S].view(K, S), dots[K * S:K * S + K], dots[K * S + K:K * S + K + S],
+                      dots[K * S + K + S:])
+
+
 def paint_flat(pos: torch.Tensor, mesh_shape, flatsky_npix, zlo, zhi, scale, mode="absolute", weights=1.0,
                out=None, accumulate=False) -> torch.Tensor:
     pos = _f32(pos, "pos")

@@ -15,6 +15,10 @@ Parity status per function:
                                         periodic images of a uniform lattice fill a far shell with
                                         the lattice's mean density); normalisation as pm_oracle
   healpix_interp_weights_grad           pinned (float64 central differences of the weights)
+  flat_coords, map_coordinates_wrap     pinned (scipy.ndimage.map_coordinates order 1, mode grid-wrap;
+                                        integer coordinates pick pixels)
+  born_core_flat, born_flat, _vjp       pinned (constant planes give the spherical kappa; linear in
+                                        delta, blind to the mean; adjoint against differences)
   Drift                                 pinned (EdS closed form f = (a_t - a) / (1.5 (a a_t)^(1/4));
                                         zero drift at the distance of the snapshot time)
   paint_healpix_vjp                     pinned (central differences of paint_healpix_tiled)
@@ -268,6 +272,104 @@ def born_spherical(maps, r, a, sources, d_r, Om, chi_fn, min_z=0.01, max_z=1.5, 
         nzw = np.stack([nz(z) for nz in sources])
         return (nzw * w[None, :]) @ kg
     return born_core(maps, r, a, np.asarray(sources, dtype=np.float64), d_r, Om, chi_fn)
+
+
+def flat_coords(field_size, flatsky_npix, box_size, chi, shape_yx):
+    """Sampling coordinates (2, ny, nx) of one plane at distance chi (_born_core_impl, _born.py:53-66,
+    90-94, pixel_size of _born_flat, :154-159): meshgrid of linspace(0, f, n, endpoint=False) angles in
+    radians, stacked (x, y), divided by pixel_size = (box_y / ny, box_x / nx), minus 1/2."""
+    fx, fy = (field_size if isinstance(field_size, (tuple, list)) else (field_size, field_size))
+    ny_m, nx_m = flatsky_npix
+    pixel = np.array([box_size[1] / ny_m, box_size[0] / nx_m], dtype=np.float64)
+    ny, nx = shape_yx
+    xg, yg = np.meshgrid(np.linspace(0, fx, nx, endpoint=False), np.linspace(0, fy, ny, endpoint=False))
+    coords = np.stack([xg, yg], axis=0) * (np.pi / 180)
+    return coords * chi / pixel[:, None, None] - 0.5
+
+
+def map_coordinates_wrap(plane, coords):
+    """jax.scipy.ndimage.map_coordinates(plane, coords, order=1, mode="wrap"): bilinear interpolation
+    with every corner index taken modulo the axis length."""
+    plane = np.asarray(plane, dtype=np.float64)
+    n0, n1 = plane.shape
+    f0, f1 = np.floor(coords[0]), np.floor(coords[1])
+    t0, t1 = coords[0] - f0, coords[1] - f1
+    i0, j0 = f0.astype(np.int64), f1.astype(np.int64)
+    out = np.zeros(coords.shape[1:])
+    for d0, w0 in ((0, 1 - t0), (1, t0)):
+        for d1, w1 in ((0, 1 - t1), (1, t1)):
+            out += w0 * w1 * plane[(i0 + d0) % n0, (j0 + d1) % n1]
+    return out
+
+
+def map_coordinates_wrap_T(g, coords, shape):
+    """Transpose of map_coordinates_wrap: the (n0, n1) plane cotangent of output cotangent g."""
+    n0, n1 = shape
+    f0, f1 = np.floor(coords[0]), np.floor(coords[1])
+    t0, t1 = coords[0] - f0, coords[1] - f1
+    i0, j0 = f0.astype(np.int64), f1.astype(np.int64)
+    out = np.zeros(shape)
+    for d0, w0 in ((0, 1 - t0), (1, t0)):
+        for d1, w1 in ((0, 1 - t1), (1, t1)):
+            np.add.at(out, ((i0 + d0) % n0, (j0 + d1) % n1), w0 * w1 * g)
+    return out
+
+
+def born_core_flat(maps, r, a, z_source, d_r, Om, chi_fn, geom):
+    """_born_core_impl's flat branch (_born.py:53-98) in float64: each plane's contribution
+    delta_s d_r r_s / a_s 3/2 Om (H0/c)^2 is resampled by map_coordinates_wrap at flat_coords(chi = r_s),
+    then weighted by clip(1 - r_s / chi(z), 0, 1000). geom = (field_size, flatsky_npix, box_size).
+    Returns (nz, ny, nx)."""
+    maps = np.asarray(maps, dtype=np.float64)
+    S, ny, nx = maps.shape
+    r = np.asarray(r, dtype=np.float64).reshape(-1)
+    a = np.asarray(a, dtype=np.float64).reshape(-1)
+    zs = np.atleast_1d(np.asarray(z_source, dtype=np.float64))
+    const = 1.5 * Om * H0_OVER_C**2
+    chi_s = np.asarray(chi_fn(1.0 / (1.0 + zs)), dtype=np.float64)
+    mean = maps.mean(axis=(1, 2), keepdims=True)
+    mean = np.where(mean == 0, FLT_EPS, mean)
+    contrib = (maps / mean - 1.0) * (d_r * r / a)[:, None, None] * const
+    res = np.stack([map_coordinates_wrap(contrib[s], flat_coords(*geom, r[s], (ny, nx))) for s in range(S)])
+    eff = np.clip(1.0 - r[:, None] / chi_s[None, :], 0.0, 1000.0)  # (S, nz)
+    return np.einsum("sz,syx->zyx", eff, res)
+
+
+def born_flat(maps, r, a, sources, d_r, Om, chi_fn, geom, min_z=0.01, max_z=1.5, n_integrate=32):
+    """_born_flat (_born.py:131-190): as born_spherical with born_core_flat."""
+    if callable(sources[0]):
+        z = np.linspace(min_z, max_z, n_integrate + 1)
+        kg = born_core_flat(maps, r, a, z, d_r, Om, chi_fn, geom)
+        w = simps_weights(min_z, max_z, n_integrate)
+        nzw = np.stack([nz(z) for nz in sources])
+        return np.einsum("kz,zyx->kyx", nzw * w[None, :], kg)
+    return born_core_flat(maps, r, a, np.asarray(sources, dtype=np.float64), d_r, Om, chi_fn, geom)
+
+
+def born_flat_vjp(maps, kappa_bar, r, a, sources, d_r, Om, chi_fn, geom, **kw):
+    """d<kappa_bar, born_flat(maps)>/d maps (float64): kappa_k = sum_s C[k, s] (R_s rho_s / m_s - 1),
+    C read off constant planes, R_s^T by map_coordinates_wrap_T, and the means' term."""
+    maps = np.asarray(maps, dtype=np.float64)
+    S, ny, nx = maps.shape
+    kb = np.asarray(kappa_bar, dtype=np.float64)
+    # the flat branch shares the spherical coefficients (the resampling comes after them): read C off
+    # two-pixel spherical maps with delta_s = (+1, -1), as born_vjp does
+    C = np.zeros((kb.shape[0], S))
+    for s in range(S):
+        m2 = np.ones((S, 2))
+        m2[s] = (2.0, 0.0)
+        C[:, s] = born_spherical(m2, r, a, sources, d_r, Om, chi_fn, **kw)[:, 0]
+    out = np.zeros_like(maps)
+    for s in range(S):
+        coords = flat_coords(*geom, float(np.asarray(r).reshape(-1)[s]), (ny, nx))
+        m = maps[s].mean()
+        g = np.einsum("k,kyx->yx", C[:, s], kb)
+        if m == 0:
+            out[s] = map_coordinates_wrap_T(g, coords, (ny, nx)) / FLT_EPS
+        else:
+            Rr = map_coordinates_wrap(maps[s], coords)
+            out[s] = map_coordinates_wrap_T(g, coords, (ny, nx)) / m - (g * Rr).sum() / (m * m * ny * nx)
+    return out
 
 
 def born_vjp(maps, kappa_bar, r, a, sources, d_r, Om, chi_fn, **kw):

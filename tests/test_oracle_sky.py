@@ -186,3 +186,66 @@ def test_drift_factor_closed_form_in_eds():
     assert np.allclose(d.factor(r), expect, rtol=1e-6, atol=1e-9)
     r0 = 2 * 2997.92458 * (1 - np.sqrt(0.8))
     assert abs(d.factor(np.array([r0]))[0]) < 1e-8
+
+
+# ---------------------------------------------------------------- flat-sky Born
+def test_flat_resampling_is_scipy_grid_wrap():
+    from scipy import ndimage
+
+    rng = np.random.default_rng(8)
+    plane = rng.standard_normal((12, 20))
+    coords = np.stack([rng.uniform(-30, 40, (5, 7)), rng.uniform(-45, 70, (5, 7))])
+    ref = ndimage.map_coordinates(plane, coords.reshape(2, -1), order=1, mode="grid-wrap").reshape(5, 7)
+    assert np.allclose(so.map_coordinates_wrap(plane, coords), ref, rtol=1e-12, atol=1e-12)
+    ints = np.stack(np.meshgrid(np.arange(-12, 0), np.arange(20, 40), indexing="ij")).astype(np.float64)
+    assert np.array_equal(so.map_coordinates_wrap(plane, ints), plane)
+
+
+def test_flat_coords_put_x_angles_on_rows():
+    """x angles (linspace over nx) index rows over pixel_size[0] = box_y / ny, y angles columns."""
+    c = so.flat_coords((4.0, 2.0), (10, 20), (300.0, 100.0, 50.0), 250.0, (10, 20))
+    assert c.shape == (2, 10, 20)
+    j = np.arange(20)
+    assert np.allclose(c[0], (j * 4.0 / 20 * np.pi / 180 * 250.0 / (100.0 / 10) - 0.5)[None, :])
+    i = np.arange(10)
+    assert np.allclose(c[1], (i * 2.0 / 10 * np.pi / 180 * 250.0 / (300.0 / 20) - 0.5)[:, None])
+
+
+def test_flat_born_impulse_lands_on_its_bilinear_footprint():
+    """Unit sampling scale: output (i, j) averages rows j-1, j and columns i-1, i of the plane, so one
+    raised pixel at (r, c) raises exactly the outputs i in {c, c+1}, j in {r, r+1} by a quarter."""
+    c = nbo.Cosmo()
+    n, chi, box, A = 8, 200.0, 160.0, 3.0
+    fsz = (box / n) * n * 180 / (np.pi * chi)  # (f / n) (pi / 180) chi / pixel = 1
+    geom = ((fsz, fsz), (n, n), (box, box, box))
+    plane = np.ones((1, n, n))
+    plane[0, 2, 5] += A
+    a = np.array([0.95])
+    k = so.born_flat(plane, np.array([chi]), a, [1.0], 50.0, c.Omega_m, c.chi, geom)[0]
+    C = 1.5 * c.Omega_m * so.H0_OVER_C**2 * 50.0 * chi / a[0] * (1 - chi / c.chi(np.array([0.5]))[0])
+    m = 1 + A / n**2
+    hit = np.zeros((n, n))
+    for i in (5, 6):
+        for j in (2, 3):
+            hit[i, j] = 1
+    assert np.allclose(k, C * ((1 + A / 4 * hit) / m - 1), rtol=1e-12, atol=1e-15)
+
+
+def test_flat_born_linear_blind_to_mean_and_vjp():
+    c = nbo.Cosmo(Omega_c=0.25, Omega_b=0.0486, h=0.6774)
+    rng = np.random.default_rng(9)
+    S, ny, nx = 3, 10, 12
+    maps = rng.random((S, ny, nx)) + 0.5
+    r = np.array([150.0, 300.0, 450.0])
+    a = np.array([0.95, 0.9, 0.86])
+    geom = ((5.0, 4.0), (ny, nx), (400.0, 350.0, 400.0))
+    for src in ([so.tophat_pdf(0, 1)], [0.6, 1.2]):
+        k1 = so.born_flat(maps, r, a, src, 100.0, c.Omega_m, c.chi, geom)
+        assert np.allclose(so.born_flat(4.0 * maps, r, a, src, 100.0, c.Omega_m, c.chi, geom), k1, rtol=1e-12)
+        kb = rng.standard_normal(k1.shape)
+        g = so.born_flat_vjp(maps, kb, r, a, src, 100.0, c.Omega_m, c.chi, geom)
+        d = rng.standard_normal(maps.shape)
+        e = 1e-6
+        fd = ((so.born_flat(maps + e * d, r, a, src, 100.0, c.Omega_m, c.chi, geom) -
+               so.born_flat(maps - e * d, r, a, src, 100.0, c.Omega_m, c.chi, geom)) * kb).sum() / (2 * e)
+        assert (g * d).sum() == pytest.approx(fd, rel=1e-7)
