@@ -180,6 +180,20 @@ int pmx_paint_healpix_lightcone(pmx_plan plan, const float* pos, const float* ve
                                 double a_now, double gp_now, double drift_bound, float* map, int accumulate,
                                 void* stream);
 
+/* Cotangents of pmx_paint_healpix_lightcone: pos_bar and vel_bar (+)= d<map_bar, map>/d(pos, vel), the
+ * bilinear weights differentiated as in pmx_paint_healpix_vjp at the drifted image Z = Y + f(|Y|) R v
+ * and chained through the drift: dL/dY = G + f'(|Y|) (R v . G) Y / |Y|, dL/dv = f R^T G, f' the
+ * derivative of the tables' linear interpolation (the jax.vjp of DriftInterp / OnionTiler's drifted
+ * paint, src/jax_fli/pm/interp.py:110-121, 258-319, 476-493). Shell membership is a step and
+ * contributes nothing; ngp maps have zero cotangents. */
+int pmx_paint_healpix_lightcone_vjp(pmx_plan plan, const float* pos, const float* vel, int npx, int npy, int npz,
+                                    int mode, const float* weights, float weight, const double box[3],
+                                    const double observer[3], double rmin, double rmax, int nside, int scheme,
+                                    int ntiles, const int* tile_rot, const double* tile_shift,
+                                    const double* drift_tables, int ntab, double chi_max, double a_lo, double a_hi,
+                                    double a_now, double gp_now, double drift_bound, const float* map_bar,
+                                    float* pos_bar, float* vel_bar, int accumulate, void* stream);
+
 /* Flat-sky slab paint: absolute z in (zlo, zhi] (grid units), (x, y) rescaled to (fnx, fny) and
  * wrapped, 2D CIC, times `scale` (1 / pixel volume). _single_paint_2d (_painting.py:117-202). */
 int pmx_paint_flat(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, const float* weights,

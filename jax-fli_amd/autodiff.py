@@ -246,7 +246,10 @@ class _NBodyReverse(torch.autograd.Function):
         coef_bars = []
         for ev in reversed(info["events"]):
             if ev[0] == "save":
-                xb += info["save_vjp"](ev[1], x, snaps_bar[ev[1]])
+                gx, gv = info["save_vjp"](ev[1], x, v, snaps_bar[ev[1]])
+                xb += gx
+                if gv is not None:
+                    vb += gv
             else:
                 _, i, ck, cd = ev
                 cb = engine.step_reverse_vjp(x, v, xb, vb, mesh, mode=mode, force_scale=fs, c_kick=ck, c_drift=cd,

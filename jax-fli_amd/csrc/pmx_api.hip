@@ -30,7 +30,7 @@ int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w
 int pmx_launch_healpix_vjp(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, const double box[3],
                            const double obs[3], double rmin, double rmax, int nside, int scheme, int ntiles,
                            const int* rot, const double* shift, const float* map_bar, float* pos_bar, int accumulate,
-                           hipStream_t s);
+                           hipStream_t s, const PmxDrift* drift = nullptr, float* vel_bar = nullptr);
 int pmx_launch_born(pmx_plan_t* p, const float* maps, int S, long npix, const double* C, int K, float* kappa,
                     int accumulate, hipStream_t s);
 int pmx_launch_born_vjp(pmx_plan_t* p, const float* maps, int S, long npix, const double* C, int K,
@@ -757,6 +757,25 @@ int pmx_paint_healpix_vjp(pmx_plan p, const float* pos, int npx, int npy, int np
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
   return pmx_launch_healpix_vjp(p, a, weights, weight, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
                                 tile_shift, map_bar, pos_bar, accumulate, (hipStream_t)stream);
+}
+
+int pmx_paint_healpix_lightcone_vjp(pmx_plan p, const float* pos, const float* vel, int npx, int npy, int npz,
+                                    int mode, const float* weights, float weight, const double box[3],
+                                    const double observer[3], double rmin, double rmax, int nside, int scheme,
+                                    int ntiles, const int* tile_rot, const double* tile_shift,
+                                    const double* drift_tables, int ntab, double chi_max, double a_lo, double a_hi,
+                                    double a_now, double gp_now, double drift_bound, const float* map_bar,
+                                    float* pos_bar, float* vel_bar, int accumulate, void* stream) {
+  PMX_TRY(check_shell(p, pos, npx, npy, npz, mode, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
+                      tile_shift));
+  if (!map_bar || !pos_bar || !vel_bar) return pmx_set_error(1, "null cotangent");
+  if (!vel || !drift_tables) return pmx_set_error(1, "null argument");
+  if (ntab < 2 || !(chi_max > 0.0) || !(a_hi > a_lo) || !(a_now > 0.0) || !(drift_bound >= 0.0))
+    return pmx_set_error(1, "bad drift tables: need ntab >= 2, chi_max > 0, a_hi > a_lo, a_now > 0");
+  PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
+  const PmxDrift d{vel, drift_tables, ntab, chi_max, a_lo, a_hi, a_now, gp_now, drift_bound};
+  return pmx_launch_healpix_vjp(p, a, weights, weight, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
+                                tile_shift, map_bar, pos_bar, accumulate, (hipStream_t)stream, &d, vel_bar);
 }
 
 namespace {

@@ -292,13 +292,42 @@ __device__ __forceinline__ void image_vel(const ShellArgs& sa, int t, const doub
   for (int c = 0; c < 3; ++c) Vt[c] = R.s[c] * V[R.p[c]];
 }
 
-// _drift_to_time (src/jax_fli/pm/interp.py:110-121) at the image's own lightcone time
+// tab_lin and the slope of the segment it used (0 where the table is clamped)
+__device__ __forceinline__ double tab_lin_d(const double* t, int n, double x0, double x1, double x, double& slope) {
+  const double u = (x - x0) / (x1 - x0) * (double)(n - 1);
+  if (!(u > 0.0)) {
+    slope = 0.0;
+    return t[0];
+  }
+  if (u >= (double)(n - 1)) {
+    slope = 0.0;
+    return t[n - 1];
+  }
+  const int i = (int)u;
+  slope = (t[i + 1] - t[i]) * (double)(n - 1) / (x1 - x0);
+  return t[i] + (t[i + 1] - t[i]) * (u - (double)i);
+}
+
+// drift factor f(r) = (Gp(a_t) - Gp(a_now)) / dGfa(sqrt(a_now a_t)), a_t = a(chi = r), of
+// _drift_to_time (src/jax_fli/pm/interp.py:110-121), and its derivative df/dr (of the tables'
+// linear interpolation, so it is the exact derivative of what the paint computes)
+__device__ __forceinline__ double drift_factor(const ShellArgs& sa, double r, double& dfdr) {
+  double da, dgp, ddg;
+  const double at = tab_lin_d(sa.dtab, sa.ntab, 0.0, sa.chi_max, r, da);
+  const double gpt = tab_lin_d(sa.dtab + sa.ntab, sa.ntab, sa.a_lo, sa.a_hi, at, dgp);
+  const double bt = sqrt(sa.a_now * at);
+  const double dg = tab_lin_d(sa.dtab + 2 * sa.ntab, sa.ntab, sa.a_lo, sa.a_hi, bt, ddg);
+  const double num = gpt - sa.gp_now;
+  const double dfda = dgp / dg - num * ddg * (bt > 0.0 ? 0.5 * sa.a_now / bt : 0.0) / (dg * dg);
+  dfdr = dfda * da;
+  return num / dg;
+}
+
+// _drift_to_time at the image's own lightcone time: Xt += f(|Xt|) Vt
 __device__ __forceinline__ void drift_image(const ShellArgs& sa, double Xt[3], const double Vt[3]) {
   const double r = sqrt(Xt[0] * Xt[0] + Xt[1] * Xt[1] + Xt[2] * Xt[2]);
-  const double at = tab_lin(sa.dtab, sa.ntab, 0.0, sa.chi_max, r);
-  const double gpt = tab_lin(sa.dtab + sa.ntab, sa.ntab, sa.a_lo, sa.a_hi, at);
-  const double dg = tab_lin(sa.dtab + 2 * sa.ntab, sa.ntab, sa.a_lo, sa.a_hi, sqrt(sa.a_now * at));
-  const double f = (gpt - sa.gp_now) / dg;
+  double dfdr;
+  const double f = drift_factor(sa, r, dfdr);
   for (int c = 0; c < 3; ++c) Xt[c] += f * Vt[c];
 }
 
@@ -475,9 +504,13 @@ __global__ void __launch_bounds__(256) k_paint_healpix(PartArgs a, const float* 
 // Position cotangent of the bilinear shell paint: for every image of the particle inside the shell,
 // d/dX sum_m map_bar[pix_m] wgt_m(theta, phi), rotated back to the box frame, times
 // scale * weight * (Mpc/h per grid unit). The ngp scheme is piecewise constant: zero.
+// With drift (sa.v): Z = Y + f(|Y|) W for the image Y = R X + shift, W = R V, so
+// dL/dY = G + f'(|Y|) (W . G) Y / |Y| and dL/dW = f G, G = dL/dZ; both are rotated back by R^T.
+// vel_bar receives the velocity cotangent (drift only).
 __global__ void __launch_bounds__(256) k_healpix_vjp(PartArgs a, const float* __restrict__ w, float w0, ShellArgs sa,
                                                      HpGeom g, const float* __restrict__ map_bar, double scale,
-                                                     float* __restrict__ pos_bar, int accumulate) {
+                                                     float* __restrict__ pos_bar, float* __restrict__ vel_bar,
+                                                     int accumulate) {
   __shared__ unsigned short hits[HP_MAX_TILES];
   const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
   const int nhit = shell_images(a, sa, b, hits);
@@ -485,14 +518,27 @@ __global__ void __launch_bounds__(256) k_healpix_vjp(PartArgs a, const float* __
     int i, j, k;
     const long p = sb_particle(a, sa, b, r, i, j, k);
     if (p < 0) continue;
-    double X[3];
+    double X[3], V[3] = {0.0, 0.0, 0.0};
     centred(a, sa, p, i, j, k, X);
-    double gX[3] = {0.0, 0.0, 0.0};
+    if (sa.v) {
+      const float3 v = ld3(sa.v, (unsigned)p);
+      V[0] = v.x;
+      V[1] = v.y;
+      V[2] = v.z;
+    }
+    double gX[3] = {0.0, 0.0, 0.0}, gV[3] = {0.0, 0.0, 0.0};
     if (sa.scheme == 1) {
       for (int h = 0; h < nhit; ++h) {
         const int t = hits[h];
-        double Xt[3], rr;
-        image_of(sa, t, X, Xt);
+        double Y[3], Xt[3], W[3] = {0.0, 0.0, 0.0}, rr, f = 0.0, dfdr = 0.0, ry = 0.0;
+        image_of(sa, t, X, Y);
+        for (int c = 0; c < 3; ++c) Xt[c] = Y[c];
+        if (sa.v) {
+          image_vel(sa, t, V, W);
+          ry = sqrt(Y[0] * Y[0] + Y[1] * Y[1] + Y[2] * Y[2]);
+          f = drift_factor(sa, ry, dfdr);
+          for (int c = 0; c < 3; ++c) Xt[c] += f * W[c];
+        }
         if (!in_shell(sa, Xt, rr)) continue;
         const double rho2 = Xt[0] * Xt[0] + Xt[1] * Xt[1];
         if (!(rho2 > 0.0)) continue;  // on the polar axis phi is undefined
@@ -516,11 +562,25 @@ __global__ void __launch_bounds__(256) k_healpix_vjp(PartArgs a, const float* __
         G[0] = ct * Xt[0] - gp * Xt[1] / rho2;
         G[1] = ct * Xt[1] + gp * Xt[0] / rho2;
         G[2] = -gt * rho / r2;
+        double GY[3] = {G[0], G[1], G[2]}, GW[3] = {0.0, 0.0, 0.0};
+        if (sa.v) {
+          const double c = ry > 0.0 ? dfdr * (W[0] * G[0] + W[1] * G[1] + W[2] * G[2]) / ry : 0.0;
+          for (int q = 0; q < 3; ++q) {
+            GY[q] += c * Y[q];
+            GW[q] = f * G[q];
+          }
+        }
         if (sa.rot) {
           const Sym R = sym_of(sa.rot[t]);
-          for (int c = 0; c < 3; ++c) gX[R.p[c]] += R.s[c] * G[c];
+          for (int c = 0; c < 3; ++c) {
+            gX[R.p[c]] += R.s[c] * GY[c];
+            gV[R.p[c]] += R.s[c] * GW[c];
+          }
         } else {
-          for (int c = 0; c < 3; ++c) gX[c] += G[c];
+          for (int c = 0; c < 3; ++c) {
+            gX[c] += GY[c];
+            gV[c] += GW[c];
+          }
         }
       }
     }
@@ -532,6 +592,14 @@ __global__ void __launch_bounds__(256) k_healpix_vjp(PartArgs a, const float* __
       o = make_float3(q.x + o.x, q.y + o.y, q.z + o.z);
     }
     st3(pos_bar, (unsigned)p, o);
+    if (vel_bar) {
+      float3 ov = make_float3((float)(gV[0] * wt), (float)(gV[1] * wt), (float)(gV[2] * wt));
+      if (accumulate) {
+        const float3 q = ld3(vel_bar, (unsigned)p);
+        ov = make_float3(q.x + ov.x, q.y + ov.y, q.z + ov.z);
+      }
+      st3(vel_bar, (unsigned)p, ov);
+    }
   }
 }
 
@@ -930,12 +998,23 @@ int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w
 int pmx_launch_healpix_vjp(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, const double box[3],
                            const double obs[3], double rmin, double rmax, int nside, int scheme, int ntiles,
                            const int* rot, const double* shift, const float* map_bar, float* pos_bar, int accumulate,
-                           hipStream_t s) {
-  const ShellArgs sa = shell_args(p, a, box, obs, rmin, rmax, scheme, ntiles, rot, shift);
+                           hipStream_t s, const PmxDrift* drift, float* vel_bar) {
+  ShellArgs sa = shell_args(p, a, box, obs, rmin, rmax, scheme, ntiles, rot, shift);
+  if (drift) {
+    sa.v = drift->v;
+    sa.dtab = drift->tables;
+    sa.ntab = drift->ntab;
+    sa.chi_max = drift->chi_max;
+    sa.a_lo = drift->a_lo;
+    sa.a_hi = drift->a_hi;
+    sa.a_now = drift->a_now;
+    sa.gp_now = drift->gp_now;
+    sa.fbound = drift->fbound;
+  }
   const HpGeom g = hp_geom(nside);
   const double vol = (4.0 * M_PI / (double)g.npix) * (rmax * rmax * rmax - rmin * rmin * rmin) / 3.0;
   hipLaunchKernelGGL(k_healpix_vjp, dim3((unsigned)sa.nbx * sa.nby * sa.nbz), dim3(256), 0, s, a, w, w0, sa, g,
-                     map_bar, 1.0 / vol, pos_bar, accumulate);
+                     map_bar, 1.0 / vol, pos_bar, drift ? vel_bar : nullptr, accumulate);
   return pmx_check_launch("healpix_vjp");
 }
 

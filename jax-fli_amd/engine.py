@@ -309,6 +309,36 @@ def paint_healpix_drift(pos: torch.Tensor, vel: torch.Tensor, mesh_shape, box, o
     return out
 
 
+def paint_healpix_drift_vjp(pos: torch.Tensor, vel: torch.Tensor, map_bar: torch.Tensor, mesh_shape, box, observer,
+                            rmin, rmax, nside, drift, scheme="bilinear", mode="relative", weights=1.0, tiles=None,
+                            pos_bar=None, vel_bar=None, accumulate=False):
+    """(pos_bar, vel_bar) (+)= d<map_bar, paint_healpix_drift(pos, vel)>/d(pos, vel)."""
+    pos = _f32(pos, "pos")
+    vel = _f32(vel, "vel")
+    map_bar = _f32(map_bar, "map_bar")
+    if map_bar.numel() != 12 * nside * nside:
+        raise ValueError("map_bar must have 12 nside^2 pixels")
+    if tuple(vel.shape) != tuple(pos.shape):
+        raise ValueError("vel must have the shape of pos")
+    npx, npy, npz = _pgrid(pos)
+    plan = get_plan(mesh_shape, pos.device)
+    if pos_bar is None or vel_bar is None:
+        if accumulate:
+            raise ValueError("accumulate needs both pos_bar and vel_bar")
+        pos_bar = torch.empty_like(pos) if pos_bar is None else pos_bar
+        vel_bar = torch.empty_like(vel) if vel_bar is None else vel_bar
+    w, w0 = _weights(weights, npx * npy * npz)
+    nt, rot, shift, _keep = _tiles(tiles, pos.device)
+    tab = drift["tables"]
+    check(load().pmx_paint_healpix_lightcone_vjp(
+        plan.handle, ptr(pos), ptr(vel), npx, npy, npz, MODES[mode], ptr(w), w0, dbl3(box), dbl3(observer),
+        float(rmin), float(rmax), int(nside), _scheme(scheme), nt, ptr(rot), ptr(shift), ptr(tab), int(tab.shape[-1]),
+        float(drift["chi_max"]), float(drift["a_lo"]), float(drift["a_hi"]), float(drift["a_now"]),
+        float(drift["gp_now"]), float(drift["bound"]), ptr(map_bar), ptr(pos_bar), ptr(vel_bar), int(accumulate),
+        stream_ptr()), "pmx_paint_healpix_lightcone_vjp")
+    return pos_bar, vel_bar
+
+
 def paint_healpix_vjp(pos: torch.Tensor, map_bar: torch.Tensor, mesh_shape, box, observer, rmin, rmax, nside,
                       scheme="bilinear", mode="relative", weights=1.0, tiles=None, pos_bar=None,
                       accumulate=False) -> torch.Tensor:

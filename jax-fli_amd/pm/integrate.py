@@ -18,6 +18,7 @@ import numpy as np
 import torch
 
 from .. import autodiff, engine
+from .interp import paint_drifted_vjp
 from .solvers import AbstractNBodySolver, NBodyState, ReversibleDoubleKickDrift, StepOp, _mode, force_scale, run_op
 
 f32 = np.float32
@@ -83,8 +84,10 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
     if solver.pgd_kernel.op(1.0) is not None:
         raise NotImplementedError("gradients through PGD / sharpening corrections are not implemented")
     interp = solver.interp_kernel
-    if type(interp).__name__ == "DriftInterp" or getattr(interp, "drift_on_lightcone", False):
-        raise NotImplementedError("gradients through the drift to the lightcone are not implemented")
+    drifting = type(interp).__name__ == "DriftInterp" or bool(getattr(interp, "drift_on_lightcone", False))
+    if drifting and autodiff.split_cosmo(cosmo_in)[1]:
+        raise NotImplementedError("cosmology gradients through the drift to the lightcone are not implemented "
+                                  "(gradients with respect to the particles are)")
     target = interp.painting.target if interp.painting is not None else "particles"
     if target not in ("density", "particles", "spherical"):
         raise NotImplementedError(f"gradients through '{target}' snapshots are not implemented")
@@ -108,19 +111,26 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
         return NBodyState(interp_state=dataclasses.replace(base_state, rotation_idx=-1, shell_idx=idx),
                           t_initial=float(t0))
 
-    def save_vjp(idx, x, g):
-        """position cotangent of snapshot idx (the paint of x) for the snapshot cotangent g"""
+    save_time = {ev[1]: ev[2] for ev in events if ev[0] == "save"}
+
+    def save_vjp(idx, x, v, g):
+        """(position, velocity or None) cotangents of snapshot idx (the paint of x, and of v when the
+        particles are drifted to the lightcone) for the snapshot cotangent g"""
         if target == "particles":
-            return g
+            return g, None
         weights = 1.0 if interp.painting.weights is None else interp.painting.weights
         if target == "density":
-            return engine.paint_vjp(x, g.contiguous(), mode=mode, weights=weights)[0]
+            return engine.paint_vjp(x, g.contiguous(), mode=mode, weights=weights)[0], None
         st = _state(idx).interp_state
         rc, wd = interp.shell(st)
+        tiles = interp.images(st, displacements)
+        if drifting:
+            return paint_drifted_vjp(displacements.replace(array=x), velocities.replace(array=v), save_time[idx], cosmo,
+                                     rc, wd, interp.painting, g, tiles=tiles)
         return engine.paint_healpix_vjp(x, g.contiguous(), mesh_size, displacements.box_size,
                                         displacements.observer_position_mpc, rc - wd / 2, rc + wd / 2,
                                         displacements.nside, scheme=interp.painting.scheme, mode=mode,
-                                        weights=weights, tiles=interp.images(st, displacements))
+                                        weights=weights, tiles=tiles), None
 
     def run(x0, v0, cosmo_):
         x = x0.detach().contiguous().clone()

@@ -56,6 +56,16 @@ def paint_drifted(dx, p, t, cosmo, r_center, width, painting, tiles=None):
                                                 density_width=np.atleast_1d(width), unit=DensityUnit.DENSITY)
 
 
+def paint_drifted_vjp(dx, p, t, cosmo, r_center, width, painting, map_bar, tiles=None):
+    """(position, velocity) cotangents of paint_drifted for the map cotangent map_bar."""
+    rmin, rmax = r_center - width / 2, r_center + width / 2
+    drift = drift_tables(cosmo, float(t), 2.0 * rmax + 100.0, dx.array.device)
+    return engine.paint_healpix_drift_vjp(dx.array.contiguous(), p.array.contiguous(), map_bar.contiguous(),
+                                          dx.mesh_size, dx.box_size, dx.observer_position_mpc, rmin, rmax, dx.nside,
+                                          drift, scheme=painting.scheme, mode=_paint_mode(dx.unit),
+                                          weights=1.0 if painting.weights is None else painting.weights, tiles=tiles)
+
+
 @dataclasses.dataclass(frozen=True)
 class InterpTilerState:
     rotation_idx: int = -1

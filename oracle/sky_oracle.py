@@ -21,7 +21,8 @@ Parity status per function:
                                         delta, blind to the mean; adjoint against differences)
   Drift                                 pinned (EdS closed form f = (a_t - a) / (1.5 (a a_t)^(1/4));
                                         zero drift at the distance of the snapshot time)
-  paint_healpix_vjp                     pinned (central differences of paint_healpix_tiled)
+  paint_healpix_vjp                     pinned (central differences of paint_healpix_tiled, with and
+                                        without the drift, in positions and velocities)
   simps_weights                         pinned (exact for cubics; jax_cosmo simps restated)
   born_spherical                        pinned (thin-plane sum -> the Born integral
                                         3/2 Om (H0/c)^2 int chi (chi_s - chi) / (chi_s a) delta dchi
@@ -113,6 +114,15 @@ class Drift:
         at = self._a_of_chi(np.asarray(r, dtype=np.float64))
         return (self._gp(at) - self.gp_now) / self._dg(np.sqrt(self.a_now * at))
 
+    def factor_grad(self, r):
+        """df/dr by the chain rule through the splines."""
+        r = np.asarray(r, dtype=np.float64)
+        at = self._a_of_chi(r)
+        bt = np.sqrt(self.a_now * at)
+        num, den = self._gp(at) - self.gp_now, self._dg(bt)
+        dfda = self._gp(at, 1) / den - num * self._dg(bt, 1) * (0.5 * self.a_now / bt) / den**2
+        return dfda * self._a_of_chi(r, 1)
+
 
 def paint_healpix_tiled(pos, mesh_shape, box, observer, rmin, rmax, nside, scheme="bilinear", mode="relative",
                         weight=1.0, tiles=None, vel=None, drift=None):
@@ -123,7 +133,7 @@ def paint_healpix_tiled(pos, mesh_shape, box, observer, rmin, rmax, nside, schem
     n = X.shape[0]
     wt = np.broadcast_to(np.asarray(weight, dtype=np.float64).reshape(-1), (n,))
     tiles = [(0, np.zeros(3))] if tiles is None else [(int(c), np.asarray(s, dtype=np.float64)) for c, s in tiles]
-    V = None if vel is None else np.asarray(vel, dtype=np.float32).reshape(-1, 3).astype(np.float64)
+    V = None if vel is None else np.asarray(vel, dtype=np.float64).reshape(-1, 3)
     npix = 12 * nside * nside
     acc = np.zeros(npix)
     for code, sh in tiles:
@@ -188,14 +198,18 @@ def healpix_interp_weights_grad(nside, theta, phi):
 
 
 def paint_healpix_vjp(pos, map_bar, mesh_shape, box, observer, rmin, rmax, nside, scheme="bilinear",
-                      mode="relative", weight=1.0, tiles=None):
-    """pos_bar = d<map_bar, paint_healpix_tiled(pos)>/d pos (float64); zero for ngp."""
+                      mode="relative", weight=1.0, tiles=None, vel=None, drift=None):
+    """pos_bar = d<map_bar, paint_healpix_tiled(pos)>/d pos (float64); zero for ngp. With drift and
+    vel, returns (pos_bar, vel_bar) through Z = Y + f(|Y|) R vel: dL/dY = G + f'(|Y|) (R vel . G) Y/|Y|,
+    dL/dvel = f R^T G."""
     pos = np.asarray(pos)
     X, cell = _centred(pos, mesh_shape, box, observer, mode)
     n = X.shape[0]
     out = np.zeros((n, 3))
+    vout = np.zeros((n, 3))
+    V = None if vel is None else np.asarray(vel, dtype=np.float64).reshape(-1, 3)
     if scheme == "ngp":
-        return out.reshape(pos.shape)
+        return out.reshape(pos.shape) if drift is None else (out.reshape(pos.shape), vout.reshape(pos.shape))
     wt = np.broadcast_to(np.asarray(weight, dtype=np.float64).reshape(-1), (n,))
     tiles = [(0, np.zeros(3))] if tiles is None else [(int(c), np.asarray(s, dtype=np.float64)) for c, s in tiles]
     npix = 12 * nside * nside
@@ -203,7 +217,13 @@ def paint_healpix_vjp(pos, map_bar, mesh_shape, box, observer, rmin, rmax, nside
     mb = np.asarray(map_bar, dtype=np.float64)
     for code, sh in tiles:
         R = cube_symmetry(code)
-        Xt = X @ R.T + sh
+        Y = X @ R.T + sh
+        Xt, W, f, fp, ry = Y, None, None, None, None
+        if drift is not None:
+            W = V @ R.T
+            ry = np.sqrt((Y**2).sum(axis=1))
+            f, fp = drift.factor(ry), drift.factor_grad(ry)
+            Xt = Y + f[:, None] * W
         r = np.sqrt((Xt**2).sum(axis=1))
         for p in np.nonzero((r > rmin) & (r <= rmax))[0]:
             x, y, z = Xt[p]
@@ -217,7 +237,13 @@ def paint_healpix_vjp(pos, map_bar, mesh_shape, box, observer, rmin, rmax, nside
             rho, r2 = np.sqrt(rho2), r[p] ** 2
             G = np.array([gt * z * x / (rho * r2) - gp * y / rho2, gt * z * y / (rho * r2) + gp * x / rho2,
                           -gt * rho / r2])
-            out[p] += (R.T @ G) * wt[p] * scale
+            GY = G
+            if drift is not None:
+                GY = G + fp[p] * (W[p] @ G) * Y[p] / ry[p]
+                vout[p] += (R.T @ (f[p] * G)) * wt[p] * scale
+            out[p] += (R.T @ GY) * wt[p] * scale
+    if drift is not None:
+        return (out * cell).reshape(pos.shape), vout.reshape(pos.shape)
     return (out * cell).reshape(pos.shape)
 
 

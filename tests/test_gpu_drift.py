@@ -1,12 +1,15 @@
 """Drift to the lightcone on the GPU against the CPU oracle: the HEALPix shell paint with every
 particle (or every rotated image of it) drifted from the snapshot time to the time of its own
 distance from the observer (_drift_to_time, src/jax_fli/pm/interp.py:110-121, used by DriftInterp
-and OnionTiler with drift_on_lightcone), and both interpolation kernels inside a full N-body run.
+and OnionTiler with drift_on_lightcone), its adjoint in positions and velocities, both interpolation
+kernels inside a full N-body run, and the reverse-adjoint gradient through drifted shells.
 
 The device evaluates the drift factor from float64 tables (4096 points, linear interpolation) built
 by the package's own cosmology; the oracle uses its own distance quadrature and growth ODE through
 cubic splines. Tolerances: a single paint 1e-5 relative L2; lightcone shells of an N-body run 1e-4
-(float32 particle arithmetic over the run against float64)."""
+(float32 particle arithmetic over the run against float64); the paint adjoint 1e-5 (velocities) and
+1e-4 (positions: the slope of linear tables against spline derivatives); the N-body gradient 2e-3
+against float64 central differences."""
 
 from __future__ import annotations
 
@@ -123,13 +126,93 @@ def test_nbody_lightcone_with_drift_matches_oracle(pkg, cuda, kernel):
         assert rel(got[s], maps[s]) < 1e-4, s
 
 
-def test_gradient_through_the_drift_raises(pkg, cuda):
-    N, BOX = 8, 100.0
+@pytest.mark.parametrize("tiled", [False, True])
+def test_drifted_paint_vjp_matches_oracle(pkg, cuda, tiled):
+    mesh, box, obs = (16, 16, 16), (100.0, 100.0, 100.0), (50.0, 50.0, 50.0)
+    rng = np.random.default_rng(45)
+    dx = (rng.standard_normal(mesh + (3,)) * 1.2).astype(np.float32)
+    vel = (rng.standard_normal(mesh + (3,)) * 30.0).astype(np.float32)
+    rmin, rmax, a_now = (40.0, 90.0, 0.9) if tiled else (10.0, 45.0, 0.95)
+    tiles = None
+    if tiled:
+        rots = np.random.default_rng(5).integers(1, 48, size=27).astype(np.int32)
+        tiles = (rots, so.onion_shifts((0.5, 0.5, 0.5), box))
     c = pkg.Cosmology(**P)
-    noise = torch.from_numpy(np.random.default_rng(44).standard_normal((N,) * 3).astype(np.float32)).to(cuda)
-    noise.requires_grad_(True)
-    ic = pkg.interpolate_initial_conditions(noise, (N,) * 3, (BOX,) * 3, cosmo=c, nside=4)
-    dx, p = pkg.lpt(c, ic, ts=0.1)
+    tab = pkg.pm.interp.drift_tables(c, a_now, 2.0 * rmax + 100.0, cuda)
+    mb = rng.standard_normal(12 * 8 * 8).astype(np.float32)
+    gx, gv = pkg.engine.paint_healpix_drift_vjp(torch.from_numpy(dx).to(cuda), torch.from_numpy(vel).to(cuda),
+                                                torch.from_numpy(mb).to(cuda), mesh, box, obs, rmin, rmax, 8, tab,
+                                                tiles=tiles)
+    rx, rv = so.paint_healpix_vjp(dx, mb, mesh, box, obs, rmin, rmax, 8, tiles=None if tiles is None else
+                                  list(zip(*tiles)), vel=vel, drift=so.Drift(nbo.Cosmo(**P), a_now))
+    # the velocity cotangent carries f itself (tables agree to 1e-9); the position cotangent also
+    # f' = df/dr, the slope of the device's linear tables against the oracle's spline derivative
+    assert rel(gv.cpu().numpy(), rv) < 1e-5
+    assert rel(gx.cpu().numpy(), rx) < 1e-4
+
+
+def test_nbody_gradient_through_the_drifted_lightcone(pkg, cuda):
+    """d L / d (dx, p) for L = sum_s <W_s, shell_s> of OnionTiler(drift_on_lightcone=True) shells
+    (one inside the box, one from 27 rotated images) of a reversible N-body run, against float64
+    central differences of the oracle's run painting its drifted snapshots."""
+    N, BOX, A0, DT0, NSIDE = 16, 100.0, 0.1, 0.15, 8
+    c = pkg.Cosmology(**P)
+    o = nbo.Cosmo(**P)
+    edges = np.array([20.0, 45.0, 90.0])
+    r_c = 0.5 * (edges[1:] + edges[:-1])[::-1]
+    widths = (edges[1:] - edges[:-1])[::-1]
+    ts = np.stack([pkg.cosmology.a_of_chi(c, r_c - widths / 2), pkg.cosmology.a_of_chi(c, r_c + widths / 2)])
+    interp = pkg.OnionTiler(painting=pkg.PaintingOptions(target="spherical", scheme="bilinear"),
+                            drift_on_lightcone=True)
+    rots = interp.init().rotations
+    ic = pkg.gaussian_initial_conditions(11, (N,) * 3, (BOX,) * 3, cosmo=c, nside=NSIDE)
+    dx, p = pkg.lpt(c, ic, ts=A0)
+    x0, v0 = dx.array.clone(), p.array.clone()
+    dxf = dx.replace(array=x0.clone().requires_grad_(True))
+    pf = p.replace(array=v0.clone().requires_grad_(True))
+    solver = pkg.ReversibleDoubleKickDrift(interp_kernel=interp)
+    lc = pkg.nbody(c, dxf, pf, t0=A0, t1=1.0, dt0=DT0, ts=ts, solver=solver, adjoint="reverse")
+    rng = np.random.default_rng(46)
+    Wt = rng.standard_normal(tuple(lc.array.shape))
+    (lc.array.double() * torch.from_numpy(Wt).to(cuda)).sum().backward()
+    t_snap = np.asarray(lc.scale_factors, dtype=np.float64)
+    r_snap = np.asarray(lc.comoving_centers, dtype=np.float64)
+    w_snap = np.asarray(lc.density_width, dtype=np.float64)
+    order = np.argsort(t_snap)
+    max_r = lc.max_comoving_radius
+
+    def loss(xx, vv):
+        maps = {}
+
+        def save(i, t, x, v):
+            s = order[i]
+            rc, wd = r_snap[s], w_snap[s]
+            tiles = None if rc + wd / 2 <= max_r else list(zip(rots, so.onion_shifts((0.5, 0.5, 0.5), (BOX,) * 3)))
+            maps[s] = so.paint_healpix_tiled(x, (N,) * 3, (BOX,) * 3, (BOX / 2,) * 3, rc - wd / 2, rc + wd / 2, NSIDE,
+                                             tiles=tiles, vel=v, drift=so.Drift(o, float(t)))
+
+        nbo.run_nbody(o, xx, vv, A0, 1.0, DT0, np.sort(t_snap).astype(np.float32), solver="kkd", on_save=save)
+        return float(sum((maps[s] * Wt[s]).sum() for s in range(len(t_snap))))
+
+    x0n, v0n = x0.cpu().numpy().astype(np.float64), v0.cpu().numpy().astype(np.float64)
+    eps = 1e-6
+    for which, grad in (("x", dxf.array.grad), ("v", pf.array.grad)):
+        d = rng.standard_normal(x0n.shape)
+        if which == "x":
+            fd = (loss(x0n + eps * d, v0n) - loss(x0n - eps * d, v0n)) / (2 * eps)
+        else:
+            fd = (loss(x0n, v0n + eps * d) - loss(x0n, v0n - eps * d)) / (2 * eps)
+        got = float((grad.double().cpu().numpy() * d).sum())
+        print(f"d/d{which}: adjoint {got:.6e} fd {fd:.6e}")
+        assert abs(got - fd) / abs(fd) < 2e-3
+
+
+def test_cosmology_gradient_through_the_drift_raises(pkg, cuda):
+    N, BOX = 8, 100.0
+    Oc = torch.tensor(P["Omega_c"], dtype=torch.float64, requires_grad=True)
+    c = pkg.Cosmology(**{**P, "Omega_c": Oc})
+    ic = pkg.gaussian_initial_conditions(12, (N,) * 3, (BOX,) * 3, cosmo=pkg.Cosmology(**P), nside=4)
+    dx, p = pkg.lpt(pkg.Cosmology(**P), ic, ts=0.1)
     ts = np.array([[0.995], [0.98]])
     painting = pkg.PaintingOptions(target="spherical", scheme="bilinear")
     solver = pkg.ReversibleDoubleKickDrift(interp_kernel=pkg.DriftInterp(painting=painting))

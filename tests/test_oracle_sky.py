@@ -249,3 +249,30 @@ def test_flat_born_linear_blind_to_mean_and_vjp():
         fd = ((so.born_flat(maps + e * d, r, a, src, 100.0, c.Omega_m, c.chi, geom) -
                so.born_flat(maps - e * d, r, a, src, 100.0, c.Omega_m, c.chi, geom)) * kb).sum() / (2 * e)
         assert (g * d).sum() == pytest.approx(fd, rel=1e-7)
+
+
+def test_drifted_healpix_vjp_matches_differences():
+    c = nbo.Cosmo(Omega_c=0.25, Omega_b=0.0486, h=0.6774)
+    d = so.Drift(c, 0.9)
+    rng = np.random.default_rng(10)
+    args = ((8, 8, 8), (100.0, 100.0, 100.0), (50.0, 50.0, 50.0), 20.0, 150.0, 4)
+    tiles = [(0, (0.0, 0.0, 0.0)), (13, (100.0, 0.0, 0.0)), (40, (0.0, -100.0, 100.0))]
+    dx = rng.standard_normal((8, 8, 8, 3)) * 0.7
+    v = rng.standard_normal((8, 8, 8, 3)) * 30.0
+    m = so.paint_healpix_tiled(dx, *args, tiles=tiles, vel=v, drift=d)
+    mb = rng.standard_normal(m.shape)
+    gx, gv = so.paint_healpix_vjp(dx, mb, *args, tiles=tiles, vel=v, drift=d)
+    assert np.abs(gv).sum() > 0
+    for which in ("x", "v"):
+        dd = rng.standard_normal(dx.shape)
+        e = 1e-6
+        if which == "x":
+            hi = so.paint_healpix_tiled(dx + e * dd, *args, tiles=tiles, vel=v, drift=d)
+            lo = so.paint_healpix_tiled(dx - e * dd, *args, tiles=tiles, vel=v, drift=d)
+            got = (gx * dd).sum()
+        else:
+            hi = so.paint_healpix_tiled(dx, *args, tiles=tiles, vel=v + e * dd, drift=d)
+            lo = so.paint_healpix_tiled(dx, *args, tiles=tiles, vel=v - e * dd, drift=d)
+            got = (gv * dd).sum()
+        fd = ((hi - lo) * mb).sum() / (2 * e)
+        assert got == pytest.approx(fd, rel=1e-6), which
