@@ -269,6 +269,8 @@ int pmx_plan_create(pmx_plan* out, int nx, int ny, int nz, unsigned flags) {
   p->flags = flags;
   p->ncell = (size_t)nx * ny * nz;
   p->nspec = (size_t)p->nzh * nx * ny;
+  p->nzhp = (p->nzh + 7) & ~7;
+  p->nspecd = (size_t)p->nzhp * nx * ny;
   auto fail = [&](int rc) {
     pmx_plan_destroy(p);
     return rc;
@@ -287,7 +289,7 @@ int pmx_plan_create(pmx_plan* out, int nx, int ny, int nz, unsigned flags) {
       (rc = dev_alloc(&p->dec_x, nx, tot)) || (rc = dev_alloc(&p->dec_y, ny, tot)) ||
       (rc = dev_alloc(&p->dec_z, p->nzh, tot)) || (rc = dev_alloc(&p->rho, p->ncell, tot)) ||
       (rc = dev_alloc(&p->specA, p->nspec, tot)) || (rc = dev_alloc(&p->specB, p->nspec, tot)) ||
-      (rc = dev_alloc(&p->specC, 3 * p->nspec, tot)) || (rc = dev_alloc(&p->specD, 3 * p->nspec, tot)) ||
+      (rc = dev_alloc(&p->specC, 3 * p->nspec, tot)) || (rc = dev_alloc(&p->specD, 3 * p->nspecd, tot)) ||
       (rc = dev_alloc(&p->fmesh, 3 * p->ncell, tot)))
     return fail(rc);
   if ((rc = dev_alloc(&p->kphys_x, nx, tot)) || (rc = dev_alloc(&p->kphys_y, ny, tot)) ||

@@ -377,14 +377,15 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
 }
 
 // ---------------------------------------------------------------- P5: c2r along z
-// lines (c, l): D[c][x][y0+l][0..nzh) -> out[x][y0+l][z][c] (NC components interleaved)
+// lines (c, l): D[c][x][y0+l][0..nzh) (rows of NZHP complex) -> out[x][y0+l][z][c] (NC components
+// interleaved)
 template <int NZ, int NC>
 __global__ void __launch_bounds__(384) k_c2r_z(const float2* __restrict__ inD, float* __restrict__ out, int NY,
                                                long icomp, const float2* __restrict__ twM,
                                                const float2* __restrict__ twN) {
   constexpr int M = NZ / 2, T = TPL<M>();
   constexpr int L = (NC == 1) ? LPW<M>() : (LPW<M>() / 2 > 0 ? LPW<M>() / 2 : 1);
-  constexpr int LS = M + 1, NL = NC * L, NT = NL * T, NZH = M + 1;
+  constexpr int LS = M + 1, NL = NC * L, NT = NL * T, NZH = M + 1, NZHP = (NZH + 7) & ~7;
   __shared__ float2 s0[NL * LS], s1[NL * LS];
   const int x = blockIdx.y, y0 = blockIdx.x * L, tid = threadIdx.x;
   const int nl = min(L, NY - y0);
@@ -392,7 +393,7 @@ __global__ void __launch_bounds__(384) k_c2r_z(const float2* __restrict__ inD, f
   for (int i = tid; i < NC * nl * NZH; i += NT) {
     const int c = i / (nl * NZH), rem = i - c * (nl * NZH);
     const int ll = rem / NZH, k = rem - ll * NZH;
-    s1[(c * L + ll) * LS + k] = inD[c * icomp + ((size_t)x * NY + y0) * NZH + rem];
+    s1[(c * L + ll) * LS + k] = inD[c * icomp + ((size_t)x * NY + y0 + ll) * NZHP + k];
   }
   __syncthreads();
   // merge: Z'_k = (X_k + conj X_{M-k}) + i (X_k - conj X_{M-k}) conj(W^k)
@@ -504,7 +505,7 @@ template <int NZ, int NC> int launch_c2r_z(pmx_plan_t* p, const float2* in, floa
   constexpr int M = NZ / 2, T = TPL<M>();
   constexpr int L = (NC == 1) ? LPW<M>() : (LPW<M>() / 2 > 0 ? LPW<M>() / 2 : 1);
   dim3 grid((p->ny + L - 1) / L, p->nx), block(NC * L * T);
-  hipLaunchKernelGGL((k_c2r_z<NZ, NC>), grid, block, 0, s, in, out, p->ny, (long)p->nspec, p->tw_zh, p->tw_z);
+  hipLaunchKernelGGL((k_c2r_z<NZ, NC>), grid, block, 0, s, in, out, p->ny, (long)p->nspecd, p->tw_zh, p->tw_z);
   return pmx_check_launch("c2r_z");
 }
 
@@ -567,13 +568,14 @@ int pmx_xpass(pmx_plan_t* p, const KOp& op, const float2* inB, float2* out, hipS
 }
 
 int pmx_pass_c2c_y_inv(pmx_plan_t* p, int ncomp, const float2* inC, float2* outD, hipStream_t s) {
-  // P4: lines C[c][kz][x][*] for consecutive kz at fixed x -> D[c][x][ky][kz]
-  const long NX = p->nx, NY = p->ny, NZH = p->nzh;
+  // P4: lines C[c][kz][x][*] for consecutive kz at fixed x -> D[c][x][ky][kz], rows of nzhp complex so
+  // a workgroup's runs of consecutive kz start on 64-byte boundaries
+  const long NX = p->nx, NY = p->ny, NZH = p->nzhp;
   if (ncomp != 1 && ncomp != 3) return pmx_set_error(2, "ncomp must be 1 or 3");
   switch (p->ny) {
 #define M_(v)                                                                                         \
   case v:                                                                                             \
-    return launch_c2c<v, true>(inC, outD, NY, NX * NY, (long)p->nspec, NY * NZH, NZH, (long)p->nspec, \
+    return launch_c2c<v, true>(inC, outD, NY, NX * NY, (long)p->nspec, NY * NZH, NZH, (long)p->nspecd, \
                                p->nzh, p->nx, ncomp, p->tw_y, s);
     PMX_FOR_AXIS(M_)
 #undef M_

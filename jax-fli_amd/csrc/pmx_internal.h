@@ -25,6 +25,8 @@ struct pmx_plan_t {
   unsigned flags;
   size_t ncell;   // nx*ny*nz
   size_t nspec;   // nzh*nx*ny (complex)
+  int nzhp;       // row length of the D layout: nzh rounded up to 8 complex (64-byte aligned rows)
+  size_t nspecd;  // nzhp*nx*ny (complex), one component of D
   // twiddle tables exp(-2 pi i k / n), one per line length that the passes use
   float2 *tw_x, *tw_y, *tw_z, *tw_zh;
   // per-axis spectral tables, index = FFT bin along that axis (z: 0..nz/2)
