@@ -266,6 +266,18 @@ int pmx_step_reverse_vjp_saved(pmx_plan plan, float* x, float* v, int npx, int n
                                float force_scale, float c_kick, float c_drift, unsigned flags, const float* force_mesh,
                                float* x_bar, float* v_bar, double* coef_bar, void* stream);
 
+/* pmx_step_reverse_vjp for a ReversibleDoubleKickDrift step with the sharpening boost
+ * (SharpeningKernel, src/jax_fli/pm/correction.py:231-333; solvers.py:258-325, 339-411):
+ *   v <- v + F(x) c_kick + S(x) v_scale ;  x <- x + v c_drift     (S as in pmx_pgd, target 1)
+ * restored and pulled back in place: v_bar <- v_bar + c_drift x_bar ;
+ * x_bar <- x_bar + J_F(x)^T (c_kick v_bar) + J_S(x)^T (v_scale v_bar), with the force and the boost
+ * as one operator (Green's function times c_kick force_scale + v_scale alpha exp(-kl^2/k^2)
+ * exp(-k^4/ks^4)), so the step costs what pmx_step_reverse_vjp does. coef_bar = {dL/dc_drift,
+ * <v_bar, c_kick F + v_scale S>}. */
+int pmx_step_reverse_vjp_sharpen(pmx_plan plan, float* x, float* v, int npx, int npy, int npz, int mode,
+                                 float force_scale, float c_kick, float c_drift, unsigned flags, float alpha, float kl,
+                                 float ks, float v_scale, float* x_bar, float* v_bar, double* coef_bar, void* stream);
+
 /* ---- Born convergence ---------------------------------------------------------------------------
  * kappa[k][p] = sum_s coef[k][s] * (maps[s][p] / mean_s - 1) over `nshell` HEALPix (or any
  * same-length) density maps, mean_s the map's pixel mean (a zero mean is replaced by FLT_EPSILON).

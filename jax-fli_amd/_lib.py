@@ -56,6 +56,8 @@ SIGNATURES = {
     "pmx_forces_vjp": (_i, [_vp, _vp, _i, _i, _i, _i, _f, _u, _vp, _vp, _i, _vp, _vp]),
     "pmx_step_reverse_vjp": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _u, _vp, _vp, _vp, _vp]),
     "pmx_step_reverse_vjp_saved": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _u, _vp, _vp, _vp, _vp, _vp]),
+    "pmx_step_reverse_vjp_sharpen": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _u, _f, _f, _f, _f, _vp, _vp, _vp,
+                                          _vp]),
     "pmx_lpt1_lattice": (_i, [_vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp]),
     "pmx_lpt1_vjp": (_i, [_vp, _vp, _vp, _i, _i, _i, _f, _f, _vp, _vp]),
     "pmx_paint_healpix_tiled": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _dp, _dp, _d, _d, _i, _i, _i, _vp, _vp, _vp,

@@ -252,8 +252,10 @@ class _NBodyReverse(torch.autograd.Function):
                     vb += gv
             else:
                 _, i, ck, cd = ev
+                sh = info.get("sharpen")
                 cb = engine.step_reverse_vjp(x, v, xb, vb, mesh, mode=mode, force_scale=fs, c_kick=ck, c_drift=cd,
-                                             force_mesh=None if ctx.meshes is None else ctx.meshes[i])
+                                             force_mesh=None if ctx.meshes is None else ctx.meshes[i],
+                                             sharpen=None if sh is None else sh + (1.0 / cd,))
                 coef_bars.append((i, cb))
         grads = []
         if names:

@@ -167,16 +167,29 @@ int force_mesh_from(pmx_plan_t* p, const float* src, float scale, unsigned flags
 
 // The potential: p->rho = G src, G = ifft((force_scale / (12 N k^2)) fft) (optionally
 // CIC-compensated), a real symmetric operator, so it is also its own adjoint. `src` may be p->rho.
+// A force and a sharpening boost at the same positions as one operator: the Green's function times
+// a + b exp(-kl^2 / k^2) exp(-k^4 / ks^4), so its readout is c_kick F + v_scale S.
+struct PgdMix {
+  float a, b, kl2, ks4;
+};
+
 int pot_only(pmx_plan_t* p, const float* src, float force_scale, unsigned flags, hipStream_t s,
-             StepMarks* m = nullptr) {
+             StepMarks* m = nullptr, const PgdMix* mix = nullptr) {
   PMX_TRY(pmx_pass_r2c_z(p, src, p->specA, s));
   mark(m, 2, s);
   PMX_TRY(pmx_pass_c2c_y(p, p->specA, p->specB, s));
   mark(m, 3, s);
   KOp op{};
   op.mode = XP_POT;
-  op.scale = (float)((double)force_scale / (12.0 * (double)p->ncell));
+  op.scale = (float)((double)(mix ? 1.f : force_scale) / (12.0 * (double)p->ncell));
   op.flags = flags & PMX_DECONV;
+  if (mix) {
+    op.flags |= 4u;
+    op.mix_a = mix->a;
+    op.mix_b = mix->b;
+    op.pgd_kl2 = mix->kl2;
+    op.pgd_ks4 = mix->ks4;
+  }
   PMX_TRY(pmx_xpass(p, op, p->specB, p->specC, s));
   mark(m, 4, s);
   PMX_TRY(pmx_pass_c2c_y_inv(p, 1, p->specC, p->specD, s));
@@ -202,13 +215,15 @@ int pot_force_mesh_from(pmx_plan_t* p, const float* src, float force_scale, unsi
 // S^T = -sum_d S_d; dot (+)= u . F. With vkick the same gather also applies v += F ckick (the
 // reverse step's un-kick).
 // M: the force mesh at the particles' positions (the plan's own, or one the caller kept)
+// mix: the combined force + sharpening operator of M (then force_scale is not used)
 int force_adjoint(pmx_plan_t* p, const PartArgs& a, const float* u, float umul, float force_scale, unsigned flags,
-                  const float* M, float* xbar, int accumulate, double* dot, float* vkick, float ckick, hipStream_t s) {
+                  const float* M, float* xbar, int accumulate, double* dot, float* vkick, float ckick, hipStream_t s,
+                  const PgdMix* mix = nullptr) {
   float* U = reinterpret_cast<float*>(p->specC);  // 3 scalar meshes; consumed by div4 before the FFT reuses it
   PMX_TRY(pmx_launch_set_scale(p->ps_misc + 3, umul, p->ps_scale, s));
   PMX_TRY(pmx_paint_deposit(p, a, u, 3, umul, 0.f, 1, U, 1, 0, s, 3, p->ncell));
   PMX_TRY(pmx_launch_div4(p, U, U + p->ncell, U + 2 * p->ncell, 1, p->rho, s));
-  PMX_TRY(pot_only(p, p->rho, force_scale, flags, s));
+  PMX_TRY(pot_only(p, p->rho, force_scale, flags, s, nullptr, mix));
   return pmx_launch_vjp_gather(p, a, 3, u, umul, M, p->rho, nullptr, 1.f, xbar, accumulate, nullptr, dot, vkick, ckick,
                                s);
 }
@@ -560,6 +575,31 @@ int pmx_step_reverse_vjp(pmx_plan p, float* x, float* v, int npx, int npy, int n
   PMX_TRY(pot_force_mesh_from(p, p->rho, force_scale, flags, p->fmesh, s));
   // x_bar += J(x)^T (ck v_bar); coef_bar[1] = v_bar . F(x); the same gather un-kicks v -= F(x) ck
   return force_adjoint(p, a, v_bar, c_kick, force_scale, flags, p->fmesh, x_bar, 1, coef_bar + 1, v, -c_kick, s);
+}
+
+int pmx_step_reverse_vjp_sharpen(pmx_plan p, float* x, float* v, int npx, int npy, int npz, int mode,
+                                 float force_scale, float c_kick, float c_drift, unsigned flags, float alpha, float kl,
+                                 float ks, float v_scale, float* x_bar, float* v_bar, double* coef_bar, void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, x, npx, npy, npz, mode));
+  if (!v || !x_bar || !v_bar || !coef_bar) return pmx_set_error(1, "null buffer");
+  if (!(ks > 0.f) || !(kl >= 0.f)) return pmx_set_error(1, "need kl >= 0 and ks > 0");
+  if (flags & PMX_DECONV) return pmx_set_error(1, "the sharpening boost has no CIC deconvolution to combine with");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t n = (size_t)npx * npy * npz * 3;
+  const PgdMix mix{c_kick * force_scale, v_scale * alpha, kl * kl, (ks * ks) * (ks * ks)};
+  PMX_TRY(pmx_launch_fill_d(coef_bar, 2, s));
+  PMX_TRY(pmx_launch_adj_prep(p, x_bar, v_bar, v, n, c_drift, coef_bar, p->ps_misc + 3, s));
+  PartArgs a = part_args(x, v, npx, npy, npz, mode, p);
+  PMX_TRY(pmx_paint_sort(p, a, -c_drift, nullptr, 1, 1.f, s));
+  a.v = nullptr;
+  PMX_TRY(pmx_paint_deposit(p, a, nullptr, 1, 1.f, 1.f, 0, p->rho, 1, 0, s));
+  // M = the combined operator's mesh at x: its readout is c_kick F(x) + v_scale S(x)
+  PMX_TRY(pot_only(p, p->rho, 1.f, flags, s, nullptr, &mix));
+  PMX_TRY(pmx_grad4(p, p->rho, p->fmesh, s));
+  // x_bar += J_F^T (c_kick v_bar) + J_S^T (v_scale v_bar) in one gather, which also undoes the kick
+  // and the boost: v -= c_kick F(x) + v_scale S(x)
+  return force_adjoint(p, a, v_bar, 1.f, force_scale, flags, p->fmesh, x_bar, 1, coef_bar + 1, v, -1.f, s, &mix);
 }
 
 int pmx_step_reverse_vjp_saved(pmx_plan p, float* x, float* v, int npx, int npy, int npz, int mode,

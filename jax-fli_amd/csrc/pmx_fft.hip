@@ -313,6 +313,8 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
       float coef = (k2 > 0.f) ? op.scale / k2 : 0.f;
       if (op.flags & 1u) coef *= tb.dx[kx] * tb.dy[ky] * tb.dz[kz];
       if ((op.flags & 2u) && k2 > 0.f) coef *= expf(-op.pgd_kl2 / k2) * expf(-(k2 * k2) / op.pgd_ks4);
+      if ((op.flags & 4u) && k2 > 0.f)
+        coef *= op.mix_a + op.mix_b * (expf(-op.pgd_kl2 / k2) * expf(-(k2 * k2) / op.pgd_ks4));
       res[ll * LS + kx] = cscale(res[ll * LS + kx], coef);
     }
     __syncthreads();

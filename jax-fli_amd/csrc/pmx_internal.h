@@ -77,8 +77,10 @@ enum { XP_FWD = 0, XP_INV = 1, XP_FORCES = 2, XP_COLOR = 3, XP_POWER = 4, XP_POT
 struct KOp {
   int mode;               // see enum in pmx_fft.hip
   float scale;            // multiplies the spectrum (already divided by the cell count where relevant)
-  unsigned flags;         // bit0: CIC deconvolution; bit1 (XP_POT): PGD range filter
+  unsigned flags;         // bit0: CIC deconvolution; bit1 (XP_POT): PGD range filter; bit2 (XP_POT):
+                          // mix_a + mix_b * PGD range filter
   float pgd_kl2, pgd_ks4; // PGD filter exp(-kl^2 / k^2) exp(-k^4 / ks^4), k in cell units
+  float mix_a, mix_b;     // force and sharpening weights of one combined potential
   // colouring: amplitude = sqrt(P(|k|) * norm), P linearly interpolated in (ktab, ptab)
   const double* kphys_x;  // |k| components in h/Mpc per bin index, as float64
   const double* kphys_y;

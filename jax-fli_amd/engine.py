@@ -531,10 +531,12 @@ def forces_vjp(pos: torch.Tensor, mesh_shape, forces_bar: torch.Tensor, mode="re
 
 def step_reverse_vjp(x, v, x_bar, v_bar, mesh_shape, mode="relative", force_scale=1.0, c_kick=0.0, c_drift=0.0,
                      deconv=False, coef_bar: torch.Tensor | None = None,
-                     force_mesh: torch.Tensor | None = None) -> torch.Tensor:
+                     force_mesh: torch.Tensor | None = None, sharpen=None) -> torch.Tensor:
     """In place: undo one reversible step (v += F ck; x += v cd) on (x, v) and pull (x_bar, v_bar) back
     through it. Returns coef_bar (float64 device [dL/dc_drift, dL/dc_kick]). force_mesh: that step's
-    force mesh kept by step(..., force_mesh=) (not recomputed)."""
+    force mesh kept by step(..., force_mesh=) (not recomputed). sharpen = (alpha, kl, ks, v_scale): the
+    step also boosted v += alpha F_pgd(x) v_scale before the drift (coef_bar[1] is then
+    <v_bar, F ck + S v_scale>)."""
     for t, nm in ((x, "x"), (v, "v"), (x_bar, "x_bar"), (v_bar, "v_bar")):
         _f32(t, nm)
         if not t.is_contiguous():
@@ -543,6 +545,14 @@ def step_reverse_vjp(x, v, x_bar, v_bar, mesh_shape, mode="relative", force_scal
     plan = get_plan(mesh_shape, x.device)
     if coef_bar is None:
         coef_bar = torch.empty(2, dtype=torch.float64, device=x.device)
+    if sharpen is not None:
+        alpha, kl, ks, vs = (float(q) for q in sharpen)
+        check(load().pmx_step_reverse_vjp_sharpen(plan.handle, ptr(x), ptr(v), npx, npy, npz, MODES[mode],
+                                                  float(force_scale), float(c_kick), float(c_drift),
+                                                  _lib.DECONV if deconv else 0, alpha, kl, ks, vs, ptr(x_bar),
+                                                  ptr(v_bar), ptr(coef_bar), stream_ptr()),
+              "pmx_step_reverse_vjp_sharpen")
+        return coef_bar
     if force_mesh is not None:
         _force_mesh_arg(force_mesh, mesh_shape)
         check(load().pmx_step_reverse_vjp_saved(plan.handle, ptr(x), ptr(v), npx, npy, npz, MODES[mode],

@@ -81,8 +81,16 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
     if not isinstance(solver, ReversibleDoubleKickDrift):
         raise NotImplementedError("gradients need the reversible solver (ReversibleDoubleKickDrift) with "
                                   "adjoint='reverse'")
+    sharpen = None
     if solver.pgd_kernel.op(1.0) is not None:
-        raise NotImplementedError("gradients through PGD / sharpening corrections are not implemented")
+        kern = solver.pgd_kernel
+        if not kern.reversible:
+            raise NotImplementedError(f"{type(kern).__name__} is not reversible: the reverse adjoint needs the "
+                                      "sharpening kernel")
+        if autodiff.split_cosmo(cosmo_in)[1]:
+            raise NotImplementedError("cosmology gradients through the sharpening boost are not implemented "
+                                      "(gradients with respect to the particles are)")
+        sharpen = (float(kern.alpha), float(kern.kl), float(kern.ks))
     interp = solver.interp_kernel
     drifting = type(interp).__name__ == "DriftInterp" or bool(getattr(interp, "drift_on_lightcone", False))
     if drifting and autodiff.split_cosmo(cosmo_in)[1]:
@@ -138,9 +146,13 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
         fs = force_scale(cosmo_)
         snaps = []
         saved_fields.clear()
-        meshes = _force_mesh_store(len(grid), mesh_size, x.device)
+        # the sharpened reverse step recomputes its combined force + boost mesh, so none is kept
+        meshes = None if sharpen else _force_mesh_store(len(grid), mesh_size, x.device)
         for ev in events:
             if ev[0] == "step":
+                if sharpen:  # SharpeningKernel.apply before the kick and drift, as run_op does
+                    engine.pgd(x, v, mesh_size, mode=mode, alpha=sharpen[0], kl=sharpen[1], ks=sharpen[2],
+                               target="v", v_scale=1.0 / ev[3])
                 engine.step(x, v, mesh_size, mode=mode, force_scale=fs, c_kick=ev[2], c_drift_post=ev[3],
                             force_mesh=None if meshes is None else meshes[ev[1]])
             else:
@@ -151,7 +163,8 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
                 snaps.append(fld.array)
         return x, v, torch.stack(snaps), len(grid), meshes
 
-    run.info = {"mesh": mesh_size, "mode": mode, "events": events, "force_scale": force_scale, "save_vjp": save_vjp}
+    run.info = {"mesh": mesh_size, "mode": mode, "events": events, "force_scale": force_scale, "save_vjp": save_vjp,
+                "sharpen": sharpen}
 
     def coef_fn(c):
         return solver.step_coeffs(grid, dt0, float(t0), c, raw=True)

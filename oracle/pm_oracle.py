@@ -24,7 +24,8 @@ Parity status per function:
   pgd_range / pgd_displacement                 pinned (kl = 0, ks -> inf gives the force; a single
                                                plane wave is scaled by the filter at its k)
   cic_paint_vjp / cic_read_vjp /               pinned (float64 central differences along random
-  force_mesh_vjp / pm_forces_vjp                 directions; exact transpose identities)
+  force_mesh_vjp / pm_forces_vjp                 directions; exact transpose identities; also with the
+                                               PGD filter against pgd_displacement)
 """
 
 from __future__ import annotations
@@ -513,20 +514,22 @@ def cic_read_vjp(mesh, pos, out_bar, mode="relative"):
     return mb.reshape(mesh.shape), pb.reshape(pos.shape)
 
 
-def force_mesh_vjp(fmesh_bar, deconv=False):
+def force_mesh_vjp(fmesh_bar, deconv=False, pgd=None):
     """Density cotangent of force_mesh: each component is ifft(K_i fft(delta)) with K_i imaginary
-    and odd, whose transpose is its negative, so delta_bar = -sum_i force_mesh(fbar_i)[..., i]."""
+    and odd (times the real PGD filter when pgd is given), whose transpose is its negative, so
+    delta_bar = -sum_i force_mesh(fbar_i)[..., i]."""
     fb = np.asarray(fmesh_bar, dtype=np.float64)
-    return -sum(force_mesh(fb[..., i], deconv=deconv, dtype=np.float64)[..., i] for i in range(3))
+    return -sum(force_mesh(fb[..., i], deconv=deconv, dtype=np.float64, pgd=pgd)[..., i] for i in range(3))
 
 
-def pm_forces_vjp(pos, mesh_shape, forces_bar, mode="relative", scale=1.0, deconv=False):
+def pm_forces_vjp(pos, mesh_shape, forces_bar, mode="relative", scale=1.0, deconv=False, pgd=None):
     """pos_bar of <forces_bar, pm_forces(pos) * scale> (float64): readout-weight term plus the
-    paint-weight term through the transposed force operator."""
+    paint-weight term through the transposed force operator. pgd = (kl, ks): the PGD-filtered force
+    of pgd_displacement (scale = alpha)."""
     rho = cic_paint(pos, mesh_shape, mode, dtype=np.float64)
-    fm = force_mesh(rho, deconv=deconv, dtype=np.float64) * scale
+    fm = force_mesh(rho, deconv=deconv, dtype=np.float64, pgd=pgd) * scale
     mesh_bar, pb_read = cic_read_vjp(fm, pos, forces_bar, mode)
-    rho_bar = force_mesh_vjp(mesh_bar, deconv=deconv) * scale
+    rho_bar = force_mesh_vjp(mesh_bar, deconv=deconv, pgd=pgd) * scale
     pb_paint, _ = cic_paint_vjp(pos, rho_bar, mode)
     return pb_read + pb_paint
 

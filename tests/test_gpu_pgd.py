@@ -1,7 +1,9 @@
 """PGD and sharpening corrections (src/jax_fli/pm/correction.py:131-333) on the GPU against the
 float64 oracle: the displacement S(x) through pmx_pgd, full runs of the KDK solver with PGDKernel and
-of the reversible solver with SharpeningKernel, and the sharpened step's exact reverse.
-Tolerances: S 1e-5 relative L2 (the force bound), positions after a run 1e-4."""
+of the reversible solver with SharpeningKernel, the sharpened step's exact reverse, its adjoint, and
+the reverse-adjoint gradient through a sharpened run. Tolerances: S 1e-5 relative L2 (the force
+bound), positions after a run 1e-4, the step adjoint 1e-4, the run's gradient 2e-3 against float64
+central differences."""
 
 from __future__ import annotations
 
@@ -100,3 +102,79 @@ def test_correction_interfaces(pkg, cuda):
     vv = p.array.clone()
     pkg.engine.pgd(dx.array.clone(), vv, (16,) * 3, alpha=0.3, kl=0.3, ks=1.0, target="v", v_scale=2.0)
     assert torch.equal(v2.array, vv) and torch.equal(x2.array, dx.array)
+
+
+def test_sharpened_step_reverse_vjp(pkg, cuda):
+    """One sharpened KKD step (v += F ck + S vs; x += v cd): the reverse restores the state, and
+    x_bar = xb + J_F^T (ck v_bar) + J_S^T (vs v_bar) against the float64 oracle adjoints."""
+    n = 32
+    mesh = (n, n, n)
+    rng = np.random.default_rng(82)
+    x = (rng.standard_normal(mesh + (3,)) * 1.2).astype(np.float32)
+    v = (rng.standard_normal(mesh + (3,)) * 0.4).astype(np.float32)
+    fs, ck, cd, alpha, kl, ks = 0.45, 0.7, 0.3, 0.2, 0.3, 1.0
+    vs = 1.0 / cd
+    xd, vd = torch.from_numpy(x).to(cuda), torch.from_numpy(v).to(cuda)
+    pkg.engine.pgd(xd, vd, mesh, alpha=alpha, kl=kl, ks=ks, target="v", v_scale=vs)
+    pkg.engine.step(xd, vd, mesh, force_scale=fs, c_kick=ck, c_drift_post=cd)
+    xb = rng.standard_normal(x.shape).astype(np.float32)
+    vb = rng.standard_normal(x.shape).astype(np.float32)
+    xbd, vbd = torch.from_numpy(xb).to(cuda), torch.from_numpy(vb).to(cuda)
+    pkg.engine.step_reverse_vjp(xd, vd, xbd, vbd, mesh, force_scale=fs, c_kick=ck, c_drift=cd,
+                                sharpen=(alpha, kl, ks, vs))
+    assert np.abs(xd.cpu().numpy() - x).max() < 1e-4
+    assert np.abs(vd.cpu().numpy() - v).max() < 1e-4
+    vmid = vb.astype(np.float64) + cd * xb
+    ref = (xb + orc.pm_forces_vjp(x, mesh, ck * vmid, scale=fs)
+           + orc.pm_forces_vjp(x, mesh, vs * vmid, scale=alpha, pgd=(kl, ks)))
+    assert rel(vbd.cpu().numpy(), vmid) < 1e-6
+    err = rel(xbd.cpu().numpy(), ref)
+    print(f"sharpened reverse vjp: x_bar rel L2 {err:.2e}")
+    assert err < 1e-4
+
+
+def test_sharpened_nbody_gradient(pkg, cuda):
+    """d L / d (dx, p) through a reversible N-body run with the sharpening boost (reverse adjoint),
+    against float64 central differences of the oracle's sharpened run."""
+    c = pkg.Cosmology(**P)
+    o = nbo.Cosmo(**P)
+    n, box, a0, t1, dt0 = 16, 100.0, 0.1, 0.6, 0.1
+    ic = pkg.gaussian_initial_conditions(13, (n,) * 3, (box,) * 3, cosmo=c)
+    dx, p = pkg.lpt(c, ic, ts=a0)
+    x0, v0 = dx.array.clone(), p.array.clone()
+    dxf = dx.replace(array=x0.clone().requires_grad_(True))
+    pf = p.replace(array=v0.clone().requires_grad_(True))
+    solver = pkg.ReversibleDoubleKickDrift(interp_kernel=pkg.NoInterp(painting=pkg.PaintingOptions(target="particles")),
+                                           pgd_kernel=pkg.SharpeningKernel(alpha=0.2, kl=0.3, ks=1.0))
+    out = pkg.nbody(c, dxf, pf, t0=a0, t1=t1, dt0=dt0, ts=[t1], solver=solver, adjoint="reverse")
+    rng = np.random.default_rng(83)
+    xb = rng.standard_normal(tuple(x0.shape))
+    (out.array.double() * torch.from_numpy(xb).to(cuda).reshape(out.array.shape)).sum().backward()
+    x0n, v0n = x0.cpu().numpy().astype(np.float64), v0.cpu().numpy().astype(np.float64)
+
+    def loss(xx, vv):
+        x, _ = nbo.run_nbody(o, xx, vv, a0, t1, dt0, [t1], solver="kkd", correction=("sharpen", 0.2, 0.3, 1.0))
+        return float((x.reshape(xb.shape) * xb).sum())
+
+    eps = 1e-7
+    for which, grad in (("x", dxf.array.grad), ("v", pf.array.grad)):
+        d = rng.standard_normal(x0n.shape)
+        if which == "x":
+            fd = (loss(x0n + eps * d, v0n) - loss(x0n - eps * d, v0n)) / (2 * eps)
+        else:
+            fd = (loss(x0n, v0n + eps * d) - loss(x0n, v0n - eps * d)) / (2 * eps)
+        got = float((grad.double().cpu().numpy() * d).sum())
+        print(f"d/d{which}: adjoint {got:.6e} fd {fd:.6e}")
+        assert abs(got - fd) / abs(fd) < 2e-3
+
+
+def test_cosmology_gradient_through_sharpening_raises(pkg, cuda):
+    Oc = torch.tensor(P["Omega_c"], dtype=torch.float64, requires_grad=True)
+    c = pkg.Cosmology(**{**P, "Omega_c": Oc})
+    c0 = pkg.Cosmology(**P)
+    ic = pkg.gaussian_initial_conditions(14, (16,) * 3, (100.0,) * 3, cosmo=c0)
+    dx, p = pkg.lpt(c0, ic, ts=0.1)
+    solver = pkg.ReversibleDoubleKickDrift(interp_kernel=pkg.NoInterp(painting=pkg.PaintingOptions(target="particles")),
+                                           pgd_kernel=pkg.SharpeningKernel())
+    with pytest.raises(NotImplementedError):
+        pkg.nbody(c, dx, p, t0=0.1, t1=0.5, dt0=0.1, ts=[0.5], solver=solver, adjoint="reverse")

@@ -279,6 +279,12 @@ def test_adjoints_match_finite_differences(mode):
     pb = orc.pm_forces_vjp(pos, mesh, fb, mode, scale=0.7)
     ref = _dd64(lambda x: (orc.pm_forces(x, mesh, mode, scale=0.7) * fb).sum(), pos, d)
     assert abs((pb * d).sum() - ref) < 1e-7 * np.abs(pb).sum()
+    # the PGD-filtered force (the sharpening boost): transpose identity and differences
+    fm = orc.force_mesh(delta, pgd=(0.3, 1.0))
+    assert abs((fm * m3).sum() - (delta * orc.force_mesh_vjp(m3, pgd=(0.3, 1.0))).sum()) < 1e-9 * np.abs(fm * m3).sum()
+    pb = orc.pm_forces_vjp(pos, mesh, fb, mode, scale=0.2, pgd=(0.3, 1.0))
+    ref = _dd64(lambda x: (orc.pgd_displacement(x, mesh, 0.2, 0.3, 1.0, mode=mode) * fb).sum(), pos, d)
+    assert abs((pb * d).sum() - ref) < 1e-7 * np.abs(pb).sum()
 
 
 # ---------------------------------------------------------------- PGD / sharpening corrections
