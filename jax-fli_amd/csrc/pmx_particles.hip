@@ -354,11 +354,63 @@ __device__ __forceinline__ void for_particles(const PartArgs& a, const SortGeom&
   }
 }
 
+constexpr unsigned SORT_FAR = 0xFFFFFFFFu;  // pack entry of a particle outside its sub-block's window
+
+// Pass 1 for relative mode: the lattice cell plus floor(displacement) gives an unwrapped base cell
+// whose tile, unwrapped, differs from the window centre (the sub-block's first lattice tile) by a
+// plain subtraction; no periodic wrap or canonical difference is needed, because a slot names the
+// tile wrap(centre + d) and two slots naming one tile simply reserve two ranges of it. Each lane's
+// lattice coordinates follow from its index and the (unrolled) round: i = i0 + r / 2,
+// j = j0 + 4 (r % 2) + wave, k = k0 + lane. Particles outside the window take the far path with
+// the wrapped tile, as in for_particles.
+__device__ __forceinline__ void count_relative(const PartArgs& a, const SortGeom& g, int i0, int j0, int k0, int3 c,
+                                               unsigned* lcnt, unsigned* __restrict__ count,
+                                               unsigned* __restrict__ pack, unsigned& nfar) {
+  constexpr int PER = (SB_X * SB_Y * SB_Z) / 256, U = 4, h = SORT_WIN / 2;
+  const int wv = (int)threadIdx.x >> 6;
+  const int k = k0 + ((int)threadIdx.x & 63);
+  const bool kin = k < a.npz;
+  const int bz0 = __mul24(k, g.rz);
+#pragma unroll
+  for (int r0 = 0; r0 < PER; r0 += U) {
+    unsigned ps[U];
+    bool vs[U];
+    int is[U], js[U];
+    float3 xs[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + u;
+      is[u] = i0 + (r >> 1);
+      js[u] = j0 + (r & 1) * 4 + wv;
+      vs[u] = kin && is[u] < a.npx && js[u] < a.npy;
+      ps[u] = vs[u] ? ((unsigned)is[u] * a.npy + js[u]) * a.npz + k : 0u;
+      xs[u] = vs[u] ? ld3(a.x, ps[u]) : make_float3(0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int bx = __mul24(is[u], g.rx) + (int)floorf(xs[u].x);
+      const int by = __mul24(js[u], g.ry) + (int)floorf(xs[u].y);
+      const int bz = bz0 + (int)floorf(xs[u].z);
+      const int dx = (bx >> g.sx) - c.x + h, dy = (by >> g.sy) - c.y + h, dz = (bz >> g.sz) - c.z + h;
+      const bool inw =
+          (unsigned)dx < (unsigned)SORT_WIN && (unsigned)dy < (unsigned)SORT_WIN && (unsigned)dz < (unsigned)SORT_WIN;
+      const int slot = (dx * SORT_WIN + dy) * SORT_WIN + dz;
+      const bool on = vs[u];
+      const unsigned rk = wave_add(on && inw, slot, lcnt);  // rank within the slot's segment
+      if (on) pack[ps[u]] = inw ? ((unsigned)slot << 16) | rk : SORT_FAR;
+      if (on && !inw) {
+        const Base bb = base_of(a, g, is[u], js[u], k, xs[u]);
+        atomicAdd(count + (((unsigned)(bb.cx >> g.sx) * g.nty + (bb.cy >> g.sy)) * g.ntz + (bb.cz >> g.sz)), 1u);
+        ++nfar;
+      }
+    }
+  }
+}
+
 // Pass 1: optional pre-drift x += v * cd (written back), then per-tile particle counts. The window
 // histogram of each sub-block is kept in wgcnt and each particle's (window slot, rank within the
 // slot) in pack for the scatter pass. far: particles outside windows. With weights (wstride
 // values per particle) the largest |weight| goes to wmax_bits.
-constexpr unsigned SORT_FAR = 0xFFFFFFFFu;  // pack entry of a particle outside its sub-block's window
 
 __global__ void __launch_bounds__(256) k_sort_count(PartArgs a, SortGeom g, float cd, unsigned* __restrict__ count,
                                                     unsigned* __restrict__ wgcnt, unsigned* __restrict__ far,
@@ -383,14 +435,18 @@ __global__ void __launch_bounds__(256) k_sort_count(PartArgs a, SortGeom g, floa
   __syncthreads();
   const int3 c = cen;
   unsigned nfar = 0;
-  for_particles(a, g, i0, j0, k0, c, [&](bool on, unsigned p, unsigned tile, int slot) {
-    const unsigned r = wave_add(on && slot >= 0, slot, lcnt);  // rank within the slot's segment
-    if (on) pack[p] = slot >= 0 ? ((unsigned)slot << 16) | r : SORT_FAR;
-    if (on && slot < 0) {
-      atomicAdd(count + tile, 1u);
-      ++nfar;
-    }
-  });
+  if (a.mode == 0) {
+    count_relative(a, g, i0, j0, k0, c, lcnt, count, pack, nfar);
+  } else {
+    for_particles(a, g, i0, j0, k0, c, [&](bool on, unsigned p, unsigned tile, int slot) {
+      const unsigned r = wave_add(on && slot >= 0, slot, lcnt);  // rank within the slot's segment
+      if (on) pack[p] = slot >= 0 ? ((unsigned)slot << 16) | r : SORT_FAR;
+      if (on && slot < 0) {
+        atomicAdd(count + tile, 1u);
+        ++nfar;
+      }
+    });
+  }
   if (w) {  // largest |weight| for the fixed-point scale (non-negative floats order like their bits)
     float m = 0.f;
     for (int r = 0; r < (SB_X * SB_Y * SB_Z) / 256; ++r) {
