@@ -144,6 +144,12 @@ int pmx_lpt1_vjp(pmx_plan plan, const float* dx_bar, const float* p_bar, int npx
 int pmx_pgd(pmx_plan plan, float* x, float* v, int npx, int npy, int npz, int mode, float alpha, float kl, float ks,
             int target, float v_scale, void* stream);
 
+/* Cotangent of the PGD displacement S = alpha * read(x, F_pgd) of pmx_pgd: pos_bar (+)= J_S(pos)^T
+ * disp_bar; dot (optional, float64, +=) disp_bar . S. The jax.vjp of PGDKernel._compute_boost /
+ * SharpeningKernel (src/jax_fli/pm/correction.py:169-200, 301-333) for the checkpointed adjoint. */
+int pmx_pgd_vjp(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, float alpha, float kl, float ks,
+                const float* disp_bar, float* pos_bar, int accumulate, double* dot, void* stream);
+
 /* HEALPix shell paint: particles with rmin < |x - observer| <= rmax (Mpc/h, periodic box) painted
  * to RING pixels, scheme 0 = ngp, 1 = bilinear (HEALPix ring interpolation weights), map =
  * weighted count / (pixel solid angle * (rmax^3 - rmin^3) / 3). paint_particles_spherical as called

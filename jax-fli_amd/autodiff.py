@@ -276,3 +276,39 @@ class _NBodyReverse(torch.autograd.Function):
 def nbody_reverse(x0, v0, cosmo_in, run, coef_fn):
     cosmo, names, tensors = split_cosmo(cosmo_in)
     return _NBodyReverse.apply(x0, v0, (run, coef_fn, cosmo, names), *tensors)
+
+
+class _NBodyCheckpointed(torch.autograd.Function):
+    """Forward: the folded StepOps with checkpoints, returning the stacked snapshots. Backward: the
+    checkpointed adjoint (segments recomputed from their checkpoints, per-op VJPs)."""
+
+    @staticmethod
+    def forward(ctx, x0, v0, spec, *params):
+        run_fwd, run_bwd, coef_fn, fs_fn, cosmo, names = spec
+        snaps, ckpts, final = run_fwd(x0, v0, cosmo)
+        ctx.spec, ctx.params, ctx.ckpts, ctx.final = spec, params, ckpts, final
+        return snaps
+
+    @staticmethod
+    def backward(ctx, snaps_bar):
+        run_fwd, run_bwd, coef_fn, fs_fn, cosmo, names = ctx.spec
+        xb, vb, cbars = run_bwd(ctx.ckpts, ctx.final, snaps_bar)
+        ctx.ckpts = ctx.final = None  # the checkpoints are released with the backward pass
+        grads = []
+        if names:
+            # host derivatives of the op coefficients, computed while the device runs the adjoint
+            co = coef_fn(cosmo)
+            fs = float(fs_fn(cosmo))
+            ders = [(fd(coef_fn, cosmo, n), float(fd(fs_fn, cosmo, n))) for n in names]
+            idx = [i for i, _ in cbars]
+            allcb = torch.stack([cb for _, cb in cbars])  # (n_ops, 3) = dL/d(pre, kick, post), on device
+            for (dco, dfs), t in zip(ders, ctx.params):
+                w = dco[idx].copy()
+                w[:, 1] += co[idx, 1] / fs * dfs  # the force carries force_scale into every kick
+                grads.append(_param_grad(t, (allcb * engine.to_device(w, allcb.device)).sum()))
+        return (xb, vb, None, *grads)
+
+
+def nbody_checkpointed(x0, v0, cosmo_in, fns):
+    cosmo, names, tensors = split_cosmo(cosmo_in)
+    return _NBodyCheckpointed.apply(x0, v0, tuple(fns) + (cosmo, names), *tensors)

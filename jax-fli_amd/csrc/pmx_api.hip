@@ -516,6 +516,24 @@ int pmx_forces_vjp(pmx_plan p, const float* pos, int npx, int npy, int npz, int 
   return force_adjoint(p, a, forces_bar, 1.f, scale, flags, p->fmesh, pos_bar, accumulate, dot, nullptr, 0.f, s);
 }
 
+int pmx_pgd_vjp(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, float alpha, float kl, float ks,
+                const float* disp_bar, float* pos_bar, int accumulate, double* dot, void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
+  if (!disp_bar || !pos_bar) return pmx_set_error(1, "null cotangent");
+  if (!(ks > 0.f) || !(kl >= 0.f)) return pmx_set_error(1, "need kl >= 0 and ks > 0");
+  hipStream_t s = (hipStream_t)stream;
+  PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
+  const size_t np = (size_t)npx * npy * npz;
+  const PgdMix mix{0.f, alpha, kl * kl, (ks * ks) * (ks * ks)};  // the PGD operator alone
+  PMX_TRY(pmx_paint_sort(p, a, 0.f, nullptr, 1, 1.f, s));
+  PMX_TRY(pmx_paint_deposit(p, a, nullptr, 1, 1.f, 1.f, 0, p->rho, 1, 0, s));
+  PMX_TRY(pot_only(p, p->rho, 1.f, 0, s, nullptr, &mix));
+  PMX_TRY(pmx_grad4(p, p->rho, p->fmesh, s));
+  PMX_TRY(pmx_launch_absmax(disp_bar, 3 * np, p->ps_misc + 3, s));
+  return force_adjoint(p, a, disp_bar, 1.f, 1.f, 0, p->fmesh, pos_bar, accumulate, dot, nullptr, 0.f, s, &mix);
+}
+
 int pmx_force_mesh_vjp(pmx_plan p, const float* fmesh_bar, float scale, unsigned flags, float* density_bar,
                        void* stream) {
   PMX_TRY(check_plan(p));

@@ -529,6 +529,21 @@ def forces_vjp(pos: torch.Tensor, mesh_shape, forces_bar: torch.Tensor, mode="re
     return pos_bar
 
 
+def pgd_vjp(pos: torch.Tensor, mesh_shape, disp_bar: torch.Tensor, mode="relative", alpha=0.2, kl=0.3, ks=1.0,
+            pos_bar: torch.Tensor | None = None, accumulate=False, dot: torch.Tensor | None = None) -> torch.Tensor:
+    """pos_bar (+)= J_S(pos)^T disp_bar for S = alpha * PGD force (pgd); dot (float64 device) += disp_bar . S."""
+    pos = _f32(pos, "pos")
+    disp_bar = _f32(disp_bar, "disp_bar")
+    npx, npy, npz = _pgrid(pos)
+    plan = get_plan(mesh_shape, pos.device)
+    if pos_bar is None:
+        pos_bar = torch.empty_like(pos)
+        accumulate = False
+    check(load().pmx_pgd_vjp(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], float(alpha), float(kl), float(ks),
+                             ptr(disp_bar), ptr(pos_bar), int(accumulate), ptr(dot), stream_ptr()), "pmx_pgd_vjp")
+    return pos_bar
+
+
 def step_reverse_vjp(x, v, x_bar, v_bar, mesh_shape, mode="relative", force_scale=1.0, c_kick=0.0, c_drift=0.0,
                      deconv=False, coef_bar: torch.Tensor | None = None,
                      force_mesh: torch.Tensor | None = None, sharpen=None) -> torch.Tensor:

@@ -75,6 +75,77 @@ def _force_mesh_store(nsteps, mesh_size, device):
     return torch.empty((nsteps,) + tuple(mesh_size) + (3,), dtype=torch.float32, device=device)
 
 
+def _snapshot_vjp(solver, displacements, velocities, tg, cosmo, t0):
+    """(state_of(idx), save_vjp(idx, x, v, g)): the interpolation state of snapshot idx and the
+    (position, velocity or None) cotangents of its paint (of x, and of v when the particles are
+    drifted to the lightcone) for the snapshot cotangent g."""
+    interp = solver.interp_kernel
+    drifting = type(interp).__name__ == "DriftInterp" or bool(getattr(interp, "drift_on_lightcone", False))
+    target = interp.painting.target if interp.painting is not None else "particles"
+    if target not in ("density", "particles", "spherical"):
+        raise NotImplementedError(f"gradients through '{target}' snapshots are not implemented")
+    mesh_size = displacements.mesh_size
+    mode = _mode(displacements)
+    base_state = interp.init()
+    save_time = {ev[1]: ev[2] for ev in tg if ev[0] == "save"}
+
+    def state_of(idx):
+        return NBodyState(interp_state=dataclasses.replace(base_state, rotation_idx=-1, shell_idx=idx),
+                          t_initial=float(t0))
+
+    def save_vjp(idx, x, v, g):
+        if target == "particles":
+            return g, None
+        weights = 1.0 if interp.painting.weights is None else interp.painting.weights
+        if target == "density":
+            return engine.paint_vjp(x, g.contiguous(), mode=mode, weights=weights)[0], None
+        st = state_of(idx).interp_state
+        rc, wd = interp.shell(st)
+        tiles = interp.images(st, displacements)
+        if drifting:
+            return paint_drifted_vjp(displacements.replace(array=x), velocities.replace(array=v), save_time[idx], cosmo,
+                                     rc, wd, interp.painting, g, tiles=tiles)
+        return engine.paint_healpix_vjp(x, g.contiguous(), mesh_size, displacements.box_size,
+                                        displacements.observer_position_mpc, rc - wd / 2, rc + wd / 2,
+                                        displacements.nside, scheme=interp.painting.scheme, mode=mode,
+                                        weights=weights, tiles=tiles), None
+
+    return state_of, save_vjp
+
+
+def fold_ops(solver, cosmo, t0, ts, dt0, t1, raw: bool = False) -> list:
+    """The forward's item list: ("op", StepOp) with each step's leading drift folded into the
+    previous op when no snapshot separates them, and ("save", idx, t) at the snapshots (raw: float64
+    coefficients, for derivatives with respect to the cosmology)."""
+    ops: list = []
+    op0 = solver.init_op(t0, t0 + dt0, dt0, cosmo, raw=raw)
+    if op0 is not None:
+        ops.append(("op", op0))
+    tg = time_grid(t0, ts, dt0, t1)
+    step_ops = iter(solver.step_ops([(ev[1], ev[2]) for ev in tg if ev[0] == "step"], dt0, float(t0), cosmo,
+                                    raw=raw))
+    for ev in tg:
+        ops.append(("op", next(step_ops)) if ev[0] == "step" else ev)
+    items: list = []
+    pending: StepOp | None = None
+    for item in ops:
+        if item[0] == "op":
+            op = StepOp(item[1].pre, item[1].kick, item[1].post, item[1].corr)
+            if pending is not None and pending.post is None and op.pre is not None:
+                pending.post, op.pre = op.pre, None
+            if pending is not None:
+                items.append(("op", pending))
+            pending = op
+        else:
+            if pending is not None:
+                items.append(("op", pending))
+                pending = None
+            items.append(item)
+    if pending is not None:
+        items.append(("op", pending))
+    return items
+
+
 def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0):
     """Differentiable integration: the reversible solver's fused steps forward, the reverse adjoint
     (pmx_step_reverse_vjp per step, snapshot paint VJPs) backward (integrate.py:147-193, 288-538)."""
@@ -96,9 +167,6 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
     if drifting and autodiff.split_cosmo(cosmo_in)[1]:
         raise NotImplementedError("cosmology gradients through the drift to the lightcone are not implemented "
                                   "(gradients with respect to the particles are)")
-    target = interp.painting.target if interp.painting is not None else "particles"
-    if target not in ("density", "particles", "spherical"):
-        raise NotImplementedError(f"gradients through '{target}' snapshots are not implemented")
     cosmo, _, _ = autodiff.split_cosmo(cosmo_in)
     mesh_size = displacements.mesh_size
     mode = _mode(displacements)
@@ -113,32 +181,7 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
         else:
             events.append(("save", ev[1], ev[2]))
     saved_fields = []
-    base_state = interp.init()
-
-    def _state(idx):
-        return NBodyState(interp_state=dataclasses.replace(base_state, rotation_idx=-1, shell_idx=idx),
-                          t_initial=float(t0))
-
-    save_time = {ev[1]: ev[2] for ev in events if ev[0] == "save"}
-
-    def save_vjp(idx, x, v, g):
-        """(position, velocity or None) cotangents of snapshot idx (the paint of x, and of v when the
-        particles are drifted to the lightcone) for the snapshot cotangent g"""
-        if target == "particles":
-            return g, None
-        weights = 1.0 if interp.painting.weights is None else interp.painting.weights
-        if target == "density":
-            return engine.paint_vjp(x, g.contiguous(), mode=mode, weights=weights)[0], None
-        st = _state(idx).interp_state
-        rc, wd = interp.shell(st)
-        tiles = interp.images(st, displacements)
-        if drifting:
-            return paint_drifted_vjp(displacements.replace(array=x), velocities.replace(array=v), save_time[idx], cosmo,
-                                     rc, wd, interp.painting, g, tiles=tiles)
-        return engine.paint_healpix_vjp(x, g.contiguous(), mesh_size, displacements.box_size,
-                                        displacements.observer_position_mpc, rc - wd / 2, rc + wd / 2,
-                                        displacements.nside, scheme=interp.painting.scheme, mode=mode,
-                                        weights=weights, tiles=tiles), None
+    _state, save_vjp = _snapshot_vjp(solver, displacements, velocities, tg, cosmo, t0)
 
     def run(x0, v0, cosmo_):
         x = x0.detach().contiguous().clone()
@@ -174,6 +217,130 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
     return out.replace(array=arr)
 
 
+def _op_vjp(op: StepOp, x, v, v_after, xb, vb, fs, mesh_size, mode):
+    """Pull (xb, vb) back in place through one StepOp applied to the state (x, v) (v_after: the
+    velocity it produced) and return its coefficient cotangents [dL/dpre, dL/dkick, dL/dpost]
+    (float64 device). The op is x_a = x + pre v; [x_b = x_a + S(x_a) | v += v_scale S(x_a)];
+    v_c = v + kick F(x_b); x_d = x_b + post v_c."""
+    zero = torch.zeros((), dtype=torch.float64, device=x.device)
+    dpre = dkick = dpost = zero
+    x_a = x if op.pre is None else x + v * f32(op.pre)
+    x_b = x_a
+    corr = op.corr
+    if corr is not None and corr.target == "x":
+        x_b = x_a.clone()
+        corr.run(x_b, None, mesh_size, mode)
+    if op.post is not None:
+        dpost = (xb * v_after).sum(dtype=torch.float64)
+        vb.add_(xb * f32(op.post))
+    if op.kick is not None:
+        dot = torch.zeros(1, dtype=torch.float64, device=x.device)
+        g = engine.forces_vjp(x_b, mesh_size, vb, mode=mode, scale=fs, dot=dot)
+        xb.add_(g * f32(op.kick))
+        dkick = dot[0]
+    if corr is not None:
+        u = xb if corr.target == "x" else vb
+        g = engine.pgd_vjp(x_a, mesh_size, u, mode=mode, alpha=corr.alpha, kl=corr.kl, ks=corr.ks)
+        xb.add_(g if corr.target == "x" else g * f32(corr.v_scale))
+    if op.pre is not None:
+        dpre = (xb * v).sum(dtype=torch.float64)
+        vb.add_(xb * f32(op.pre))
+    return torch.stack([dpre, dkick, dpost])
+
+
+def _integrate_checkpointed(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0, checkpoints):
+    """Differentiable integration of any solver (integrate.py:147-193 with adjoint="checkpointed"):
+    the forward runs the same folded StepOps as the non-differentiable path and keeps the state
+    before every op (and snapshot) when that fits the HBM budget, else every stride-th op (stride =
+    ceil(n_ops / checkpoints), or about sqrt(n_ops)); the backward recomputes each segment from its
+    checkpoint and pulls the cotangents back through every op (the forces' and corrections' VJPs
+    at the positions they saw) and every snapshot paint."""
+    interp = solver.interp_kernel
+    drifting = type(interp).__name__ == "DriftInterp" or bool(getattr(interp, "drift_on_lightcone", False))
+    cosmo, names, _ = autodiff.split_cosmo(cosmo_in)
+    corrected = solver.pgd_kernel.op(1.0) is not None
+    if names and (drifting or corrected):
+        raise NotImplementedError("cosmology gradients through the drift to the lightcone or through PGD / "
+                                  "sharpening corrections are not implemented (gradients with respect to the "
+                                  "particles are)")
+    mesh_size = displacements.mesh_size
+    mode = _mode(displacements)
+    tg = time_grid(t0, ts, dt0, t1)
+    items = fold_ops(solver, cosmo, t0, ts, dt0, t1)
+    state_of, save_vjp = _snapshot_vjp(solver, displacements, velocities, tg, cosmo, t0)
+    op_items = [k for k, it in enumerate(items) if it[0] == "op"]
+    op_index = {k: j for j, k in enumerate(op_items)}
+    nops = len(op_items)
+    state_bytes = 2 * displacements.array.numel() * displacements.array.element_size()
+    dev = displacements.array.device
+    budget = KEEP_FORCE_MESH_BYTES
+    if dev.type == "cuda":
+        budget = min(budget, torch.cuda.mem_get_info(dev)[0] // 4)
+    if checkpoints is None and (len(items) + 1) * state_bytes <= budget:
+        ckpt_at = set(range(len(items)))  # every op's input state: no recomputation
+    else:
+        stride = max(1, math.ceil(nops / checkpoints)) if checkpoints else max(1, math.isqrt(max(nops, 1)))
+        ckpt_at = {0} | {op_items[j] for j in range(0, nops, stride)}
+    saved_fields = []
+
+    def forward(x0, v0, cosmo_):
+        x = x0.detach().contiguous().clone()
+        v = v0.detach().contiguous().clone()
+        ckpts, snaps = {}, []
+        saved_fields.clear()
+        for k, it in enumerate(items):
+            if k in ckpt_at:
+                ckpts[k] = (x.clone(), v.clone())
+            if it[0] == "op":
+                run_op(it[1], x, v, mesh_size, mode, cosmo_)
+            else:
+                _, idx, tt = it
+                fld = solver.save_at(displacements.replace(array=x, scale_factors=tt),
+                                     velocities.replace(array=v, scale_factors=tt), tt, dt0, state_of(idx), cosmo_)
+                saved_fields.append(fld)
+                snaps.append(fld.array)
+        return torch.stack(snaps), ckpts, (x, v)
+
+    def backward(ckpts, final, snaps_bar):
+        fs = force_scale(cosmo)
+        xb = torch.zeros_like(final[0])
+        vb = torch.zeros_like(final[1])
+        cbars = []
+        starts = sorted(ckpts)
+        for si in reversed(range(len(starts))):
+            k0 = starts[si]
+            k1 = starts[si + 1] if si + 1 < len(starts) else len(items)
+            after = ckpts[k1] if k1 < len(items) else final
+            x, v = (t.clone() for t in ckpts[k0])
+            states = []
+            for k in range(k0, k1):  # recompute the segment's states (none when every op is kept)
+                states.append((x.clone(), v.clone()) if k + 1 < k1 else (x, v))
+                if items[k][0] == "op" and k + 1 < k1:
+                    run_op(items[k][1], x, v, mesh_size, mode, cosmo)
+            for k in reversed(range(k0, k1)):
+                xs, vs = states[k - k0]
+                if items[k][0] == "save":
+                    gx, gv = save_vjp(items[k][1], xs, vs, snaps_bar[items[k][1]])
+                    xb += gx
+                    if gv is not None:
+                        vb += gv
+                else:
+                    cbars.append((op_index[k], _op_vjp(items[k][1], xs, vs, after[1], xb, vb, fs, mesh_size, mode)))
+                after = (xs, vs)
+        return xb, vb, cbars
+
+    def coef_fn(c):
+        return np.asarray([[o.pre or 0.0, o.kick or 0.0, o.post or 0.0]
+                           for it in fold_ops(solver, c, t0, ts, dt0, t1, raw=True) if it[0] == "op"
+                           for o in (it[1],)],
+                          dtype=np.float64)
+
+    arr = autodiff.nbody_checkpointed(displacements.array, velocities.array, cosmo_in,
+                                      (forward, backward, coef_fn, force_scale))
+    out = _stack(saved_fields)
+    return out.replace(array=arr)
+
+
 def integrate(displacements, velocities, cosmo, ts, solver: AbstractNBodySolver, t0: float, t1: float,
               dt0: float, adjoint: str = "checkpointed", checkpoints: int | None = None) -> Any:
     if adjoint not in ("checkpointed", "reverse", "lax"):
@@ -182,45 +349,24 @@ def integrate(displacements, velocities, cosmo, ts, solver: AbstractNBodySolver,
         raise ValueError(f"Cannot use adjoint='reverse' with {type(solver.pgd_kernel).__name__}.")
     _, _, grad_params = autodiff.split_cosmo(cosmo)
     if displacements.array.requires_grad or velocities.array.requires_grad or grad_params:
-        if adjoint != "reverse":
-            raise NotImplementedError("gradients are implemented for adjoint='reverse' (the reversible solver's "
-                                      "recomputation); use ReversibleDoubleKickDrift with adjoint='reverse'")
-        return _integrate_grad(displacements, velocities, cosmo, ts, solver, t0, t1, dt0)
+        if adjoint == "reverse":
+            return _integrate_grad(displacements, velocities, cosmo, ts, solver, t0, t1, dt0)
+        return _integrate_checkpointed(displacements, velocities, cosmo, ts, solver, t0, t1, dt0, checkpoints)
     cosmo, _, _ = autodiff.split_cosmo(cosmo)
     mesh_size = displacements.mesh_size
     mode = _mode(displacements)
     x = displacements.array.contiguous().clone()
     v = velocities.array.contiguous().clone()
-    t_init1 = t0 + dt0
     state = NBodyState(interp_state=solver.interp_kernel.init(), t_initial=float(t0))
-    ops: list = []
-    op0 = solver.init_op(t0, t_init1, dt0, cosmo)
-    if op0 is not None:
-        ops.append(("op", op0))
-    tg = time_grid(t0, ts, dt0, t1)
-    step_ops = iter(solver.step_ops([(ev[1], ev[2]) for ev in tg if ev[0] == "step"], dt0, state.t_initial, cosmo))
-    for ev in tg:
-        ops.append(("op", next(step_ops)) if ev[0] == "step" else ev)
-    pending: StepOp | None = None
     snaps = []
-    for item in ops:
+    for item in fold_ops(solver, cosmo, t0, ts, dt0, t1):
         if item[0] == "op":
-            op = StepOp(item[1].pre, item[1].kick, item[1].post, item[1].corr)
-            if pending is not None and pending.post is None and op.pre is not None:
-                pending.post, op.pre = op.pre, None
-            if pending is not None:
-                run_op(pending, x, v, mesh_size, mode, cosmo)
-            pending = op
+            run_op(item[1], x, v, mesh_size, mode, cosmo)
         else:
-            if pending is not None:
-                run_op(pending, x, v, mesh_size, mode, cosmo)
-                pending = None
             _, idx, tt = item
             st = NBodyState(interp_state=dataclasses.replace(state.interp_state, rotation_idx=-1, shell_idx=idx),
                             t_initial=state.t_initial)
             dx_f = displacements.replace(array=x, scale_factors=tt)
             p_f = velocities.replace(array=v, scale_factors=tt)
             snaps.append(solver.save_at(dx_f, p_f, tt, dt0, st, cosmo))
-    if pending is not None:
-        run_op(pending, x, v, mesh_size, mode, cosmo)
     return _stack(snaps)

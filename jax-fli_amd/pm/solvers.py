@@ -85,14 +85,15 @@ class AbstractNBodySolver:
     pgd_kernel: AbstractCorrection = NoCorrection()
 
     # coefficient records ------------------------------------------------------
-    def init_op(self, t0, t1, dt, cosmo) -> StepOp | None:
+    def init_op(self, t0, t1, dt, cosmo, raw: bool = False) -> StepOp | None:
         raise NotImplementedError
 
     def step_op(self, t0, t1, dt, t_initial, cosmo) -> StepOp:
         raise NotImplementedError
 
-    def step_ops(self, grid, dt, t_initial, cosmo) -> list:
-        """step_op for every (t0, t1) of grid."""
+    def step_ops(self, grid, dt, t_initial, cosmo, raw: bool = False) -> list:
+        """step_op for every (t0, t1) of grid (raw: float64 coefficients, for derivatives w.r.t. the
+        cosmology; otherwise rounded to float32 as the reference's traced arithmetic gives them)."""
         return [self.step_op(a, b, dt, t_initial, cosmo) for a, b in grid]
 
     # reference interface --------------------------------------------------------
@@ -114,12 +115,13 @@ class AbstractNBodySolver:
 class EfficientDriftDoubleKick(AbstractNBodySolver):
     """FastPM KDK (solvers.py:118-225): half kick at init, then drift -> force -> double kick."""
 
-    def init_op(self, t0, t1, dt, cosmo) -> StepOp:
+    def init_op(self, t0, t1, dt, cosmo, raw: bool = False) -> StepOp:
         # solvers.py:136-147: ai = ac = t0, af = sqrt(t0 t1) (host floats), kick with F(x0)
         ai = ac = f32(t0)
         af = f32((float(t0) * float(t1)) ** 0.5)
         kick = (_c.Gf(cosmo, af) - _c.Gf(cosmo, ai)) / _c.dGfa(cosmo, ac)
-        return StepOp(kick=float(f32(kick / (ac**2 * _c.E(cosmo, ac)))))
+        k = kick / (ac**2 * _c.E(cosmo, ac))
+        return StepOp(kick=float(k) if raw else float(f32(k)))
 
     def step_op(self, t0, t1, dt, t_initial, cosmo) -> StepOp:
         t0, t1 = f32(t0), f32(t1)
@@ -136,7 +138,7 @@ class EfficientDriftDoubleKick(AbstractNBodySolver):
         # solvers.py:175: the correction gets the drift factor without its 1/(a^3 E) prefactor
         return StepOp(pre=float(f32(cd)), kick=float(f32(ck)), corr=self.pgd_kernel.op(float(f32(drift))))
 
-    def step_ops(self, grid, dt, t_initial, cosmo) -> list:
+    def step_ops(self, grid, dt, t_initial, cosmo, raw: bool = False) -> list:
         """step_op vectorised over the steps (same float32 time arithmetic and float64 factors)."""
         if len(grid) == 0:
             return []
@@ -152,8 +154,8 @@ class EfficientDriftDoubleKick(AbstractNBodySolver):
         k1 = (gf1 - _c.Gf(cosmo, t0t1)) / dgf
         k2 = (_c.Gf(cosmo, t1t2) - gf1) / dgf
         ck = (k1 + k2) / (t1.astype(np.float64) ** 2 * _c.E(cosmo, t1))
-        return [StepOp(pre=float(f32(a)), kick=float(f32(b)), corr=self.pgd_kernel.op(float(f32(d))))
-                for a, b, d in zip(cd, ck, drift)]
+        r = (lambda q: float(q)) if raw else (lambda q: float(f32(q)))
+        return [StepOp(pre=r(a), kick=r(b), corr=self.pgd_kernel.op(r(d))) for a, b, d in zip(cd, ck, drift)]
 
     def init(self, displacement, velocities, t0, t1, dt, cosmo):
         op = self.init_op(t0, t1, dt, cosmo)
@@ -176,7 +178,7 @@ class EfficientDriftDoubleKick(AbstractNBodySolver):
 class ReversibleDoubleKickDrift(AbstractNBodySolver):
     """Reversible KKD (solvers.py:228-411): force at x_i -> K1 + K2 -> drift; init is a no-op."""
 
-    def init_op(self, t0, t1, dt, cosmo):
+    def init_op(self, t0, t1, dt, cosmo, raw: bool = False):
         return None
 
     def step_op(self, t0, t1, dt, t_initial, cosmo, raw: bool = False) -> StepOp:
@@ -216,9 +218,9 @@ class ReversibleDoubleKickDrift(AbstractNBodySolver):
         out = np.stack([ck, cd], axis=-1).astype(np.float64)
         return out if raw else out.astype(np.float32).astype(np.float64)
 
-    def step_ops(self, grid, dt, t_initial, cosmo) -> list:
+    def step_ops(self, grid, dt, t_initial, cosmo, raw: bool = False) -> list:
         return [StepOp(kick=float(a), post=float(b), corr=self.pgd_kernel.op(float(b)))
-                for a, b in self.step_coeffs(grid, dt, t_initial, cosmo)]
+                for a, b in self.step_coeffs(grid, dt, t_initial, cosmo, raw=raw)]
 
     def reverse_op(self, t0, t1, dt, t_initial, cosmo) -> StepOp:
         """Exact inverse of step_op(t0, t1): x -= v cd, then the boost is rewound at that x
