@@ -214,8 +214,34 @@ def bench_vjp(pkg, dev):
         h = 1e-3
         fd_oc = (float(loss(noise0, 0.25 + h, 0.8, target)) - float(loss(noise0, 0.25 - h, 0.8, target))) / (2 * h)
         fd_s8 = (float(loss(noise0, 0.25, 0.8 + h, target)) - float(loss(noise0, 0.25, 0.8 - h, target))) / (2 * h)
+    # the reference's default differentiable path: EfficientDriftDoubleKick with adjoint="checkpointed"
+    kdk = pkg.EfficientDriftDoubleKick(interp_kernel=pkg.NoInterp(painting=pkg.PaintingOptions(target="density")))
+
+    def loss_kdk(noise, Oc, s8):
+        c = pkg.Cosmology(Omega_c=Oc, sigma8=s8, **P)
+        ic = pkg.interpolate_initial_conditions(noise, (n,) * 3, (box,) * 3, cosmo=c)
+        dx, p = pkg.lpt(c, ic, ts=a0)
+        rho = pkg.nbody(c, dx, p, t0=a0, t1=1.0, dt0=dt0, ts=[1.0], solver=kdk).array.double()
+        return ((rho / rho.mean() - 1.0 - target) ** 2).sum()
+
+    def kdk_fwd():
+        with torch.no_grad():
+            return float(loss_kdk(noise0, 0.25, 0.8))
+
+    def kdk_fwd_bwd():
+        nz = noise0.clone().requires_grad_(True)
+        Oc = torch.tensor(0.25, dtype=torch.float64, device=dev, requires_grad=True)
+        s8 = torch.tensor(0.8, dtype=torch.float64, device=dev, requires_grad=True)
+        loss_kdk(nz, Oc, s8).backward()
+
+    kdk_fwd_ms, kdk_fwd_bwd_ms = timed(kdk_fwd), timed(kdk_fwd_bwd)
+    checkpointed = {"workload": "the same model with EfficientDriftDoubleKick and the default adjoint='checkpointed' "
+                                "(every op's state kept in HBM), gradient w.r.t. the noise and (Omega_c, sigma8)",
+                    "forward_ms": round(kdk_fwd_ms, 2), "forward_plus_vjp_ms": round(kdk_fwd_bwd_ms, 2),
+                    "vjp_over_forward": round(kdk_fwd_bwd_ms / kdk_fwd_ms, 2)}
     return {"workload": "128^3 particles, 128^3 mesh, box 500 Mpc/h, ReversibleDoubleKickDrift dt0=0.09 a=0.1->1 "
                         "(10 steps), reverse adjoint, L2 loss on the final density vs a second seed",
+            "checkpointed_kdk": checkpointed,
             "forward_ms": round(fwd_ms, 2), "forward_plus_vjp_ms": round(fwd_bwd_ms, 2),
             "vjp_over_forward": round(fwd_bwd_ms / fwd_ms, 2),
             "noise_only": {"forward_plus_vjp_ms": round(fwd_bwd_noise_ms, 2),
