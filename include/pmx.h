@@ -248,6 +248,14 @@ int pmx_force_mesh_vjp(pmx_plan plan, const float* fmesh_bar, float scale, unsig
 int pmx_forces_vjp(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, float scale, unsigned flags,
                    const float* forces_bar, float* pos_bar, int accumulate, double* dot, void* stream);
 
+/* pmx_forces_vjp with the force mesh at pos given (force_mesh, nx*ny*nz*3 floats, as kept by
+ * pmx_step_keep for the step that kicked with these positions) instead of recomputed: the sort of pos,
+ * the adjoint deposit, the potential pass and the gather only. For the checkpointed adjoint
+ * (integrate.py:147-193 with adjoint="checkpointed"). */
+int pmx_forces_vjp_saved(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, float scale,
+                         unsigned flags, const float* force_mesh, const float* forces_bar, float* pos_bar,
+                         int accumulate, double* dot, void* stream);
+
 /* One reverse-adjoint step of the reversible KKD solver, in place. The forward step was
  *   v <- v + F(x) * c_kick ;  x <- x + v * c_drift      (F = pm_forces(x) * force_scale)
  * (ReversibleDoubleKickDrift.step, solvers.py:258-325). Given the state after it and the cotangents

@@ -65,6 +65,7 @@ SIGNATURES = {
     "pmx_paint_healpix_vjp": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _dp, _dp, _d, _d, _i, _i, _i, _vp, _vp, _vp,
                                    _vp, _i, _vp]),
     "pmx_born": (_i, [_vp, _vp, _i, _c.c_int64, _vp, _i, _vp, _i, _vp]),
+    "pmx_forces_vjp_saved": (_i, [_vp, _vp, _i, _i, _i, _i, _f, _u, _vp, _vp, _vp, _i, _vp, _vp]),
     "pmx_pgd_vjp": (_i, [_vp, _vp, _i, _i, _i, _i, _f, _f, _f, _vp, _vp, _i, _vp, _vp]),
     "pmx_pgd": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _i, _f, _vp]),
     "pmx_paint_healpix_lightcone": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _vp, _f, _dp, _dp, _d, _d, _i, _i, _i, _vp,

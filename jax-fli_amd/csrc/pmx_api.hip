@@ -516,6 +516,20 @@ int pmx_forces_vjp(pmx_plan p, const float* pos, int npx, int npy, int npz, int 
   return force_adjoint(p, a, forces_bar, 1.f, scale, flags, p->fmesh, pos_bar, accumulate, dot, nullptr, 0.f, s);
 }
 
+int pmx_forces_vjp_saved(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, float scale,
+                         unsigned flags, const float* force_mesh, const float* forces_bar, float* pos_bar,
+                         int accumulate, double* dot, void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
+  if (!forces_bar || !pos_bar || !force_mesh) return pmx_set_error(1, "null buffer");
+  hipStream_t s = (hipStream_t)stream;
+  PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
+  const size_t np = (size_t)npx * npy * npz;
+  PMX_TRY(pmx_paint_sort(p, a, 0.f, nullptr, 1, 1.f, s));  // the adjoint's deposit reuses the sort
+  PMX_TRY(pmx_launch_absmax(forces_bar, 3 * np, p->ps_misc + 3, s));
+  return force_adjoint(p, a, forces_bar, 1.f, scale, flags, force_mesh, pos_bar, accumulate, dot, nullptr, 0.f, s);
+}
+
 int pmx_pgd_vjp(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, float alpha, float kl, float ks,
                 const float* disp_bar, float* pos_bar, int accumulate, double* dot, void* stream) {
   PMX_TRY(check_plan(p));

@@ -514,8 +514,10 @@ def force_mesh_vjp(fmesh_bar: torch.Tensor, scale=1.0, deconv=False) -> torch.Te
 
 
 def forces_vjp(pos: torch.Tensor, mesh_shape, forces_bar: torch.Tensor, mode="relative", scale=1.0, deconv=False,
-               pos_bar: torch.Tensor | None = None, accumulate=False, dot: torch.Tensor | None = None) -> torch.Tensor:
-    """pos_bar (+)= J(pos)^T forces_bar for forces(pos, scale); dot (float64 device scalar) += forces_bar . F."""
+               pos_bar: torch.Tensor | None = None, accumulate=False, dot: torch.Tensor | None = None,
+               force_mesh: torch.Tensor | None = None) -> torch.Tensor:
+    """pos_bar (+)= J(pos)^T forces_bar for forces(pos, scale); dot (float64 device scalar) += forces_bar . F.
+    force_mesh: the force mesh at pos, kept by step(..., force_mesh=) (not recomputed)."""
     pos = _f32(pos, "pos")
     forces_bar = _f32(forces_bar, "forces_bar")
     npx, npy, npz = _pgrid(pos)
@@ -523,6 +525,13 @@ def forces_vjp(pos: torch.Tensor, mesh_shape, forces_bar: torch.Tensor, mode="re
     if pos_bar is None:
         pos_bar = torch.empty_like(pos)
         accumulate = False
+    if force_mesh is not None:
+        _force_mesh_arg(force_mesh, mesh_shape)
+        check(load().pmx_forces_vjp_saved(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], float(scale),
+                                          _lib.DECONV if deconv else 0, ptr(force_mesh), ptr(forces_bar),
+                                          ptr(pos_bar), int(accumulate), ptr(dot), stream_ptr()),
+              "pmx_forces_vjp_saved")
+        return pos_bar
     check(load().pmx_forces_vjp(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], float(scale),
                                 _lib.DECONV if deconv else 0, ptr(forces_bar), ptr(pos_bar), int(accumulate),
                                 ptr(dot), stream_ptr()), "pmx_forces_vjp")

@@ -217,11 +217,11 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
     return out.replace(array=arr)
 
 
-def _op_vjp(op: StepOp, x, v, v_after, xb, vb, fs, mesh_size, mode):
+def _op_vjp(op: StepOp, x, v, v_after, xb, vb, fs, mesh_size, mode, force_mesh=None):
     """Pull (xb, vb) back in place through one StepOp applied to the state (x, v) (v_after: the
     velocity it produced) and return its coefficient cotangents [dL/dpre, dL/dkick, dL/dpost]
     (float64 device). The op is x_a = x + pre v; [x_b = x_a + S(x_a) | v += v_scale S(x_a)];
-    v_c = v + kick F(x_b); x_d = x_b + post v_c."""
+    v_c = v + kick F(x_b); x_d = x_b + post v_c. force_mesh: the kick's force mesh kept by the forward."""
     zero = torch.zeros((), dtype=torch.float64, device=x.device)
     dpre = dkick = dpost = zero
     x_a = x if op.pre is None else x + v * f32(op.pre)
@@ -235,7 +235,7 @@ def _op_vjp(op: StepOp, x, v, v_after, xb, vb, fs, mesh_size, mode):
         vb.add_(xb * f32(op.post))
     if op.kick is not None:
         dot = torch.zeros(1, dtype=torch.float64, device=x.device)
-        g = engine.forces_vjp(x_b, mesh_size, vb, mode=mode, scale=fs, dot=dot)
+        g = engine.forces_vjp(x_b, mesh_size, vb, mode=mode, scale=fs, dot=dot, force_mesh=force_mesh)
         xb.add_(g * f32(op.kick))
         dkick = dot[0]
     if corr is not None:
@@ -276,8 +276,13 @@ def _integrate_checkpointed(displacements, velocities, cosmo_in, ts, solver, t0,
     budget = KEEP_FORCE_MESH_BYTES
     if dev.type == "cuda":
         budget = min(budget, torch.cuda.mem_get_info(dev)[0] // 4)
+    kick_items = [k for k in op_items if items[k][1].kick is not None]
+    mesh_slot: dict = {}
     if checkpoints is None and (len(items) + 1) * state_bytes <= budget:
         ckpt_at = set(range(len(items)))  # every op's input state: no recomputation
+        mesh_bytes = int(np.prod(mesh_size)) * 12
+        if (len(items) + 1) * state_bytes + len(kick_items) * mesh_bytes <= budget:
+            mesh_slot = {k: j for j, k in enumerate(kick_items)}  # and every kick's force mesh
     else:
         stride = max(1, math.ceil(nops / checkpoints)) if checkpoints else max(1, math.isqrt(max(nops, 1)))
         ckpt_at = {0} | {op_items[j] for j in range(0, nops, stride)}
@@ -288,20 +293,24 @@ def _integrate_checkpointed(displacements, velocities, cosmo_in, ts, solver, t0,
         v = v0.detach().contiguous().clone()
         ckpts, snaps = {}, []
         saved_fields.clear()
+        meshes = (torch.empty((len(mesh_slot),) + tuple(mesh_size) + (3,), dtype=torch.float32, device=x.device)
+                  if mesh_slot else None)
         for k, it in enumerate(items):
             if k in ckpt_at:
                 ckpts[k] = (x.clone(), v.clone())
             if it[0] == "op":
-                run_op(it[1], x, v, mesh_size, mode, cosmo_)
+                run_op(it[1], x, v, mesh_size, mode, cosmo_,
+                       force_mesh=meshes[mesh_slot[k]] if k in mesh_slot else None)
             else:
                 _, idx, tt = it
                 fld = solver.save_at(displacements.replace(array=x, scale_factors=tt),
                                      velocities.replace(array=v, scale_factors=tt), tt, dt0, state_of(idx), cosmo_)
                 saved_fields.append(fld)
                 snaps.append(fld.array)
-        return torch.stack(snaps), ckpts, (x, v)
+        return torch.stack(snaps), (ckpts, meshes), (x, v)
 
-    def backward(ckpts, final, snaps_bar):
+    def backward(store, final, snaps_bar):
+        ckpts, meshes = store
         fs = force_scale(cosmo)
         xb = torch.zeros_like(final[0])
         vb = torch.zeros_like(final[1])
@@ -325,7 +334,9 @@ def _integrate_checkpointed(displacements, velocities, cosmo_in, ts, solver, t0,
                     if gv is not None:
                         vb += gv
                 else:
-                    cbars.append((op_index[k], _op_vjp(items[k][1], xs, vs, after[1], xb, vb, fs, mesh_size, mode)))
+                    fm = meshes[mesh_slot[k]] if k in mesh_slot else None
+                    cbars.append((op_index[k], _op_vjp(items[k][1], xs, vs, after[1], xb, vb, fs, mesh_size, mode,
+                                                       force_mesh=fm)))
                 after = (xs, vs)
         return xb, vb, cbars
 

@@ -62,8 +62,8 @@ def _mode(field: ParticleField) -> str:
     raise ValueError("PM forces need GRID_RELATIVE or GRID_ABSOLUTE particles")
 
 
-def run_op(op: StepOp, x, v, mesh_size, mode, cosmo, deconv=False) -> None:
-    """Apply one StepOp in place on device arrays x, v."""
+def run_op(op: StepOp, x, v, mesh_size, mode, cosmo, deconv=False, force_mesh=None) -> None:
+    """Apply one StepOp in place on device arrays x, v (force_mesh: keep the kick's force mesh there)."""
     if op.kick is None:
         for c in (op.pre, op.post):
             if c is not None:
@@ -76,7 +76,7 @@ def run_op(op: StepOp, x, v, mesh_size, mode, cosmo, deconv=False) -> None:
             pre = None
         op.corr.run(x, v, mesh_size, mode)
     engine.step(x, v, mesh_size, mode=mode, force_scale=force_scale(cosmo), c_kick=op.kick, c_drift_pre=pre,
-                c_drift_post=op.post, deconv=deconv)
+                c_drift_post=op.post, deconv=deconv, force_mesh=force_mesh)
 
 
 @dataclasses.dataclass(frozen=True)
