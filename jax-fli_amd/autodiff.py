@@ -256,6 +256,13 @@ class _NBodyReverse(torch.autograd.Function):
                 cb = engine.step_reverse_vjp(x, v, xb, vb, mesh, mode=mode, force_scale=fs, c_kick=ck, c_drift=cd,
                                              force_mesh=None if ctx.meshes is None else ctx.meshes[i],
                                              sharpen=None if sh is None else sh + (1.0 / cd,))
+                if sh is not None and names:
+                    # the combined step returns <v_bar, ck F + vs S>; split it with B = <v_bar, S(x)> at the
+                    # restored x, and carry vs = 1 / cd into dL/dcd
+                    S = torch.zeros_like(v)
+                    engine.pgd(x, S, mesh, mode=mode, alpha=sh[0], kl=sh[1], ks=sh[2], target="v", v_scale=1.0)
+                    B = (vb * S).sum(dtype=torch.float64)
+                    cb = torch.stack([cb[0] - B / (cd * cd), (cb[1] - B / cd) / ck])
                 coef_bars.append((i, cb))
         grads = []
         if names:
@@ -301,7 +308,7 @@ class _NBodyCheckpointed(torch.autograd.Function):
             fs = float(fs_fn(cosmo))
             ders = [(fd(coef_fn, cosmo, n), float(fd(fs_fn, cosmo, n))) for n in names]
             idx = [i for i, _ in cbars]
-            allcb = torch.stack([cb for _, cb in cbars])  # (n_ops, 3) = dL/d(pre, kick, post), on device
+            allcb = torch.stack([cb for _, cb in cbars])  # (n_ops, 4) = dL/d(pre, kick, post, v_scale), on device
             for (dco, dfs), t in zip(ders, ctx.params):
                 w = dco[idx].copy()
                 w[:, 1] += co[idx, 1] / fs * dfs  # the force carries force_scale into every kick

@@ -158,9 +158,6 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
         if not kern.reversible:
             raise NotImplementedError(f"{type(kern).__name__} is not reversible: the reverse adjoint needs the "
                                       "sharpening kernel")
-        if autodiff.split_cosmo(cosmo_in)[1]:
-            raise NotImplementedError("cosmology gradients through the sharpening boost are not implemented "
-                                      "(gradients with respect to the particles are)")
         sharpen = (float(kern.alpha), float(kern.kl), float(kern.ks))
     interp = solver.interp_kernel
     drifting = type(interp).__name__ == "DriftInterp" or bool(getattr(interp, "drift_on_lightcone", False))
@@ -219,11 +216,11 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
 
 def _op_vjp(op: StepOp, x, v, v_after, xb, vb, fs, mesh_size, mode, force_mesh=None):
     """Pull (xb, vb) back in place through one StepOp applied to the state (x, v) (v_after: the
-    velocity it produced) and return its coefficient cotangents [dL/dpre, dL/dkick, dL/dpost]
-    (float64 device). The op is x_a = x + pre v; [x_b = x_a + S(x_a) | v += v_scale S(x_a)];
+    velocity it produced) and return its coefficient cotangents [dL/dpre, dL/dkick, dL/dpost,
+    dL/dv_scale] (float64 device; v_scale: the sharpening boost's factor). The op is x_a = x + pre v; [x_b = x_a + S(x_a) | v += v_scale S(x_a)];
     v_c = v + kick F(x_b); x_d = x_b + post v_c. force_mesh: the kick's force mesh kept by the forward."""
     zero = torch.zeros((), dtype=torch.float64, device=x.device)
-    dpre = dkick = dpost = zero
+    dpre = dkick = dpost = dvs = zero
     x_a = x if op.pre is None else x + v * f32(op.pre)
     x_b = x_a
     corr = op.corr
@@ -240,12 +237,17 @@ def _op_vjp(op: StepOp, x, v, v_after, xb, vb, fs, mesh_size, mode, force_mesh=N
         dkick = dot[0]
     if corr is not None:
         u = xb if corr.target == "x" else vb
-        g = engine.pgd_vjp(x_a, mesh_size, u, mode=mode, alpha=corr.alpha, kl=corr.kl, ks=corr.ks)
-        xb.add_(g if corr.target == "x" else g * f32(corr.v_scale))
+        dot = torch.zeros(1, dtype=torch.float64, device=x.device)
+        g = engine.pgd_vjp(x_a, mesh_size, u, mode=mode, alpha=corr.alpha, kl=corr.kl, ks=corr.ks, dot=dot)
+        if corr.target == "x":
+            xb.add_(g)
+        else:
+            xb.add_(g * f32(corr.v_scale))
+            dvs = dot[0]  # <v_bar, S(x_a)>
     if op.pre is not None:
         dpre = (xb * v).sum(dtype=torch.float64)
         vb.add_(xb * f32(op.pre))
-    return torch.stack([dpre, dkick, dpost])
+    return torch.stack([dpre, dkick, dpost, dvs])
 
 
 def _integrate_checkpointed(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0, checkpoints):
@@ -258,11 +260,9 @@ def _integrate_checkpointed(displacements, velocities, cosmo_in, ts, solver, t0,
     interp = solver.interp_kernel
     drifting = type(interp).__name__ == "DriftInterp" or bool(getattr(interp, "drift_on_lightcone", False))
     cosmo, names, _ = autodiff.split_cosmo(cosmo_in)
-    corrected = solver.pgd_kernel.op(1.0) is not None
-    if names and (drifting or corrected):
-        raise NotImplementedError("cosmology gradients through the drift to the lightcone or through PGD / "
-                                  "sharpening corrections are not implemented (gradients with respect to the "
-                                  "particles are)")
+    if names and drifting:
+        raise NotImplementedError("cosmology gradients through the drift to the lightcone are not implemented "
+                                  "(gradients with respect to the particles are)")
     mesh_size = displacements.mesh_size
     mode = _mode(displacements)
     tg = time_grid(t0, ts, dt0, t1)
@@ -341,7 +341,8 @@ def _integrate_checkpointed(displacements, velocities, cosmo_in, ts, solver, t0,
         return xb, vb, cbars
 
     def coef_fn(c):
-        return np.asarray([[o.pre or 0.0, o.kick or 0.0, o.post or 0.0]
+        return np.asarray([[o.pre or 0.0, o.kick or 0.0, o.post or 0.0,
+                            o.corr.v_scale if o.corr is not None and o.corr.target == "v" else 0.0]
                            for it in fold_ops(solver, c, t0, ts, dt0, t1, raw=True) if it[0] == "op"
                            for o in (it[1],)],
                           dtype=np.float64)
