@@ -191,14 +191,17 @@ int pmx_paint_healpix_lightcone(pmx_plan plan, const float* pos, const float* ve
  * and chained through the drift: dL/dY = G + f'(|Y|) (R v . G) Y / |Y|, dL/dv = f R^T G, f' the
  * derivative of the tables' linear interpolation (the jax.vjp of DriftInterp / OnionTiler's drifted
  * paint, src/jax_fli/pm/interp.py:110-121, 258-319, 476-493). Shell membership is a step and
- * contributes nothing; ngp maps have zero cotangents. */
+ * contributes nothing; ngp maps have zero cotangents. With nparam (<= 4) tables of df/dtheta_q on
+ * the a(chi) grid (param_tables, nparam x ntab), param_dots[q] += sum over particle images of
+ * (R v . G) df/dtheta_q(|Y|) times the weight: the cotangent of theta_q through the drift factor. */
 int pmx_paint_healpix_lightcone_vjp(pmx_plan plan, const float* pos, const float* vel, int npx, int npy, int npz,
                                     int mode, const float* weights, float weight, const double box[3],
                                     const double observer[3], double rmin, double rmax, int nside, int scheme,
                                     int ntiles, const int* tile_rot, const double* tile_shift,
                                     const double* drift_tables, int ntab, double chi_max, double a_lo, double a_hi,
                                     double a_now, double gp_now, double drift_bound, const float* map_bar,
-                                    float* pos_bar, float* vel_bar, int accumulate, void* stream);
+                                    float* pos_bar, float* vel_bar, int accumulate, int nparam,
+                                    const double* param_tables, double* param_dots, void* stream);
 
 /* Flat-sky slab paint: absolute z in (zlo, zhi] (grid units), (x, y) rescaled to (fnx, fny) and
  * wrapped, 2D CIC, times `scale` (1 / pixel volume). _single_paint_2d (_painting.py:117-202). */

@@ -71,7 +71,8 @@ SIGNATURES = {
     "pmx_paint_healpix_lightcone": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _vp, _f, _dp, _dp, _d, _d, _i, _i, _i, _vp,
                                          _vp, _vp, _i, _d, _d, _d, _d, _d, _d, _vp, _i, _vp]),
     "pmx_paint_healpix_lightcone_vjp": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _vp, _f, _dp, _dp, _d, _d, _i, _i, _i,
-                                             _vp, _vp, _vp, _i, _d, _d, _d, _d, _d, _d, _vp, _vp, _vp, _i, _vp]),
+                                             _vp, _vp, _vp, _i, _d, _d, _d, _d, _d, _d, _vp, _vp, _vp, _i, _i, _vp,
+                                             _vp, _vp]),
     "pmx_born_flat": (_i, [_vp, _vp, _i, _i, _i, _vp, _i, _vp, _vp, _i, _vp]),
     "pmx_born_flat_vjp": (_i, [_vp, _vp, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _vp, _vp]),
     "pmx_born_vjp": (_i, [_vp, _vp, _i, _c.c_int64, _vp, _i, _vp, _vp, _i, _vp, _vp]),

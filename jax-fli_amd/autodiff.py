@@ -244,12 +244,15 @@ class _NBodyReverse(torch.autograd.Function):
         xb = torch.zeros_like(x)
         vb = torch.zeros_like(v)
         coef_bars = []
+        pdots = None  # parameter cotangents from the snapshots (the drift factor)
         for ev in reversed(info["events"]):
             if ev[0] == "save":
-                gx, gv = info["save_vjp"](ev[1], x, v, snaps_bar[ev[1]])
+                gx, gv, pd = info["save_vjp"](ev[1], x, v, snaps_bar[ev[1]])
                 xb += gx
                 if gv is not None:
                     vb += gv
+                if pd is not None:
+                    pdots = pd if pdots is None else pdots + pd
             else:
                 _, i, ck, cd = ev
                 sh = info.get("sharpen")
@@ -276,6 +279,8 @@ class _NBodyReverse(torch.autograd.Function):
                 wck = engine.to_device(dco[idx, 0] + co[idx, 0] / fs * dfs, allcb.device)
                 wcd = engine.to_device(dco[idx, 1], allcb.device)
                 grads.append(_param_grad(t, (allcb[:, 1] * wck + allcb[:, 0] * wcd).sum()))
+            if pdots is not None:
+                grads = [g + _param_grad(t, pdots[j]) for j, (g, t) in enumerate(zip(grads, ctx.params))]
         ctx.meshes = None  # the kept force meshes are released with the backward pass
         return (xb, vb, None, *grads)
 
@@ -299,7 +304,7 @@ class _NBodyCheckpointed(torch.autograd.Function):
     @staticmethod
     def backward(ctx, snaps_bar):
         run_fwd, run_bwd, coef_fn, fs_fn, cosmo, names = ctx.spec
-        xb, vb, cbars = run_bwd(ctx.ckpts, ctx.final, snaps_bar)
+        xb, vb, cbars, pdots = run_bwd(ctx.ckpts, ctx.final, snaps_bar)
         ctx.ckpts = ctx.final = None  # the checkpoints are released with the backward pass
         grads = []
         if names:
@@ -313,6 +318,8 @@ class _NBodyCheckpointed(torch.autograd.Function):
                 w = dco[idx].copy()
                 w[:, 1] += co[idx, 1] / fs * dfs  # the force carries force_scale into every kick
                 grads.append(_param_grad(t, (allcb * engine.to_device(w, allcb.device)).sum()))
+            if pdots is not None:
+                grads = [g + _param_grad(t, pdots[j]) for j, (g, t) in enumerate(zip(grads, ctx.params))]
         return (xb, vb, None, *grads)
 
 

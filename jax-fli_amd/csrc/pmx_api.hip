@@ -839,7 +839,8 @@ int pmx_paint_healpix_lightcone_vjp(pmx_plan p, const float* pos, const float* v
                                     int ntiles, const int* tile_rot, const double* tile_shift,
                                     const double* drift_tables, int ntab, double chi_max, double a_lo, double a_hi,
                                     double a_now, double gp_now, double drift_bound, const float* map_bar,
-                                    float* pos_bar, float* vel_bar, int accumulate, void* stream) {
+                                    float* pos_bar, float* vel_bar, int accumulate, int nparam,
+                                    const double* param_tables, double* param_dots, void* stream) {
   PMX_TRY(check_shell(p, pos, npx, npy, npz, mode, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
                       tile_shift));
   if (!map_bar || !pos_bar || !vel_bar) return pmx_set_error(1, "null cotangent");
@@ -847,7 +848,12 @@ int pmx_paint_healpix_lightcone_vjp(pmx_plan p, const float* pos, const float* v
   if (ntab < 2 || !(chi_max > 0.0) || !(a_hi > a_lo) || !(a_now > 0.0) || !(drift_bound >= 0.0))
     return pmx_set_error(1, "bad drift tables: need ntab >= 2, chi_max > 0, a_hi > a_lo, a_now > 0");
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
-  const PmxDrift d{vel, drift_tables, ntab, chi_max, a_lo, a_hi, a_now, gp_now, drift_bound};
+  if (nparam < 0 || nparam > 4 || (nparam > 0 && (!param_tables || !param_dots)))
+    return pmx_set_error(1, "need 0 <= nparam <= 4 with parameter tables and dots");
+  PmxDrift d{vel, drift_tables, ntab, chi_max, a_lo, a_hi, a_now, gp_now, drift_bound};
+  d.ptab = param_tables;
+  d.nparam = nparam;
+  d.pdots = param_dots;
   return pmx_launch_healpix_vjp(p, a, weights, weight, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
                                 tile_shift, map_bar, pos_bar, accumulate, (hipStream_t)stream, &d, vel_bar);
 }

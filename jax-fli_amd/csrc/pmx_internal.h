@@ -121,6 +121,10 @@ struct PmxDrift {
   const double* tables;  // a(chi) on [0, chi_max]; Gp(a), dGfa(a) on [a_lo, a_hi]; ntab points each
   int ntab;
   double chi_max, a_lo, a_hi, a_now, gp_now, fbound;
+  // adjoint only: per-parameter tables of df/dtheta on the a(chi) grid, and the dots they feed
+  const double* ptab = nullptr;
+  int nparam = 0;
+  double* pdots = nullptr;
 };
 // paint tiles of a plan (log2 of the tile dims, tile count)
 void pmx_paint_tiles(const pmx_plan_t* p, int& sx, int& sy, int& sz, int& ntiles);

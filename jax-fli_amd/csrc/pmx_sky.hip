@@ -231,6 +231,9 @@ struct ShellArgs {
   int ntab;
   double chi_max, a_lo, a_hi, a_now, gp_now;
   double fbound;    // bound on |f| over the tables' range (widens the image culling boxes)
+  const double* ptab;  // adjoint: nparam tables of df/dtheta on the a(chi) grid
+  int nparam;
+  double* pdots;       // adjoint: += sum over particle images of (W . G) df/dtheta(|Y|) weight
 };
 
 constexpr int HP_HASH = 4096, HP_PROBES = 32, HP_SB = 16, HP_PER_THREAD = HP_SB * HP_SB * HP_SB / 256;
@@ -504,6 +507,22 @@ __global__ void __launch_bounds__(256) k_paint_healpix(PartArgs a, const float* 
 // Position cotangent of the bilinear shell paint: for every image of the particle inside the shell,
 // d/dX sum_m map_bar[pix_m] wgt_m(theta, phi), rotated back to the box frame, times
 // scale * weight * (Mpc/h per grid unit). The ngp scheme is piecewise constant: zero.
+constexpr int kDriftParams = 4;  // cosmological parameters the drift adjoint differentiates at once
+
+// *dst += the workgroup's sum of v (float64 atomic: the order of the workgroups' additions varies)
+__device__ __forceinline__ void block_sum_add(double v, double* dst) {
+  __shared__ double part[16];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if (__lane_id() == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += part[i];
+    unsafeAtomicAdd(dst, t);
+  }
+  __syncthreads();
+}
+
 // With drift (sa.v): Z = Y + f(|Y|) W for the image Y = R X + shift, W = R V, so
 // dL/dY = G + f'(|Y|) (W . G) Y / |Y| and dL/dW = f G, G = dL/dZ; both are rotated back by R^T.
 // vel_bar receives the velocity cotangent (drift only).
@@ -514,6 +533,7 @@ __global__ void __launch_bounds__(256) k_healpix_vjp(PartArgs a, const float* __
   __shared__ unsigned short hits[HP_MAX_TILES];
   const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
   const int nhit = shell_images(a, sa, b, hits);
+  double pacc[kDriftParams] = {0.0, 0.0, 0.0, 0.0};
   for (int r = 0; r < HP_PER_THREAD; ++r) {
     int i, j, k;
     const long p = sb_particle(a, sa, b, r, i, j, k);
@@ -526,7 +546,7 @@ __global__ void __launch_bounds__(256) k_healpix_vjp(PartArgs a, const float* __
       V[1] = v.y;
       V[2] = v.z;
     }
-    double gX[3] = {0.0, 0.0, 0.0}, gV[3] = {0.0, 0.0, 0.0};
+    double gX[3] = {0.0, 0.0, 0.0}, gV[3] = {0.0, 0.0, 0.0}, pp[kDriftParams] = {0.0, 0.0, 0.0, 0.0};
     if (sa.scheme == 1) {
       for (int h = 0; h < nhit; ++h) {
         const int t = hits[h];
@@ -564,11 +584,13 @@ __global__ void __launch_bounds__(256) k_healpix_vjp(PartArgs a, const float* __
         G[2] = -gt * rho / r2;
         double GY[3] = {G[0], G[1], G[2]}, GW[3] = {0.0, 0.0, 0.0};
         if (sa.v) {
-          const double c = ry > 0.0 ? dfdr * (W[0] * G[0] + W[1] * G[1] + W[2] * G[2]) / ry : 0.0;
+          const double wg = W[0] * G[0] + W[1] * G[1] + W[2] * G[2];
+          const double c = ry > 0.0 ? dfdr * wg / ry : 0.0;
           for (int q = 0; q < 3; ++q) {
             GY[q] += c * Y[q];
             GW[q] = f * G[q];
           }
+          for (int q = 0; q < sa.nparam; ++q) pp[q] += wg * tab_lin(sa.ptab + q * sa.ntab, sa.ntab, 0.0, sa.chi_max, ry);
         }
         if (sa.rot) {
           const Sym R = sym_of(sa.rot[t]);
@@ -585,6 +607,7 @@ __global__ void __launch_bounds__(256) k_healpix_vjp(PartArgs a, const float* __
       }
     }
     const double wt = (w ? (double)w[p] : (double)w0) * scale;
+    for (int q = 0; q < sa.nparam; ++q) pacc[q] += pp[q] * wt;
     float3 o = make_float3((float)(gX[0] * wt * sa.cell[0]), (float)(gX[1] * wt * sa.cell[1]),
                            (float)(gX[2] * wt * sa.cell[2]));
     if (accumulate) {
@@ -601,6 +624,7 @@ __global__ void __launch_bounds__(256) k_healpix_vjp(PartArgs a, const float* __
       st3(vel_bar, (unsigned)p, ov);
     }
   }
+  for (int q = 0; q < sa.nparam; ++q) block_sum_add(pacc[q], sa.pdots + q);
 }
 
 __global__ void k_finalize_map_d(const double* __restrict__ acc, float* __restrict__ map, long n, double scale,
@@ -948,6 +972,9 @@ ShellArgs shell_args(const pmx_plan_t* p, const PartArgs& a, const double box[3]
   sa.dtab = nullptr;
   sa.ntab = 0;
   sa.chi_max = sa.a_lo = sa.a_hi = sa.a_now = sa.gp_now = sa.fbound = 0.0;
+  sa.ptab = nullptr;
+  sa.nparam = 0;
+  sa.pdots = nullptr;
   return sa;
 }
 
@@ -1010,6 +1037,9 @@ int pmx_launch_healpix_vjp(pmx_plan_t* p, const PartArgs& a, const float* w, flo
     sa.a_now = drift->a_now;
     sa.gp_now = drift->gp_now;
     sa.fbound = drift->fbound;
+    sa.ptab = drift->ptab;
+    sa.nparam = drift->ptab ? drift->nparam : 0;
+    sa.pdots = drift->pdots;
   }
   const HpGeom g = hp_geom(nside);
   const double vol = (4.0 * M_PI / (double)g.npix) * (rmax * rmax * rmax - rmin * rmin * rmin) / 3.0;

@@ -311,8 +311,10 @@ def paint_healpix_drift(pos: torch.Tensor, vel: torch.Tensor, mesh_shape, box, o
 
 def paint_healpix_drift_vjp(pos: torch.Tensor, vel: torch.Tensor, map_bar: torch.Tensor, mesh_shape, box, observer,
                             rmin, rmax, nside, drift, scheme="bilinear", mode="relative", weights=1.0, tiles=None,
-                            pos_bar=None, vel_bar=None, accumulate=False):
-    """(pos_bar, vel_bar) (+)= d<map_bar, paint_healpix_drift(pos, vel)>/d(pos, vel)."""
+                            pos_bar=None, vel_bar=None, accumulate=False, param_tables=None, param_dots=None):
+    """(pos_bar, vel_bar) (+)= d<map_bar, paint_healpix_drift(pos, vel)>/d(pos, vel). param_tables
+    (float64 device (nparam, ntab): df/dtheta on the drift's a(chi) grid) and param_dots (float64 device
+    (nparam,), added to): the cotangents of up to 4 parameters through the drift factor."""
     pos = _f32(pos, "pos")
     vel = _f32(vel, "vel")
     map_bar = _f32(map_bar, "map_bar")
@@ -335,6 +337,7 @@ def paint_healpix_drift_vjp(pos: torch.Tensor, vel: torch.Tensor, map_bar: torch
         float(rmin), float(rmax), int(nside), _scheme(scheme), nt, ptr(rot), ptr(shift), ptr(tab), int(tab.shape[-1]),
         float(drift["chi_max"]), float(drift["a_lo"]), float(drift["a_hi"]), float(drift["a_now"]),
         float(drift["gp_now"]), float(drift["bound"]), ptr(map_bar), ptr(pos_bar), ptr(vel_bar), int(accumulate),
+        0 if param_tables is None else int(param_tables.shape[0]), ptr(param_tables), ptr(param_dots),
         stream_ptr()), "pmx_paint_healpix_lightcone_vjp")
     return pos_bar, vel_bar
 

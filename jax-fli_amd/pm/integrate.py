@@ -75,10 +75,11 @@ def _force_mesh_store(nsteps, mesh_size, device):
     return torch.empty((nsteps,) + tuple(mesh_size) + (3,), dtype=torch.float32, device=device)
 
 
-def _snapshot_vjp(solver, displacements, velocities, tg, cosmo, t0):
+def _snapshot_vjp(solver, displacements, velocities, tg, cosmo, t0, param_names=()):
     """(state_of(idx), save_vjp(idx, x, v, g)): the interpolation state of snapshot idx and the
-    (position, velocity or None) cotangents of its paint (of x, and of v when the particles are
-    drifted to the lightcone) for the snapshot cotangent g."""
+    (position, velocity or None, parameter dots or None) cotangents of its paint (of x, and of v and
+    the cosmological parameters param_names through the drift factor when the particles are drifted
+    to the lightcone) for the snapshot cotangent g."""
     interp = solver.interp_kernel
     drifting = type(interp).__name__ == "DriftInterp" or bool(getattr(interp, "drift_on_lightcone", False))
     target = interp.painting.target if interp.painting is not None else "particles"
@@ -95,20 +96,20 @@ def _snapshot_vjp(solver, displacements, velocities, tg, cosmo, t0):
 
     def save_vjp(idx, x, v, g):
         if target == "particles":
-            return g, None
+            return g, None, None
         weights = 1.0 if interp.painting.weights is None else interp.painting.weights
         if target == "density":
-            return engine.paint_vjp(x, g.contiguous(), mode=mode, weights=weights)[0], None
+            return engine.paint_vjp(x, g.contiguous(), mode=mode, weights=weights)[0], None, None
         st = state_of(idx).interp_state
         rc, wd = interp.shell(st)
         tiles = interp.images(st, displacements)
         if drifting:
             return paint_drifted_vjp(displacements.replace(array=x), velocities.replace(array=v), save_time[idx], cosmo,
-                                     rc, wd, interp.painting, g, tiles=tiles)
+                                     rc, wd, interp.painting, g, tiles=tiles, param_names=param_names)
         return engine.paint_healpix_vjp(x, g.contiguous(), mesh_size, displacements.box_size,
                                         displacements.observer_position_mpc, rc - wd / 2, rc + wd / 2,
                                         displacements.nside, scheme=interp.painting.scheme, mode=mode,
-                                        weights=weights, tiles=tiles), None
+                                        weights=weights, tiles=tiles), None, None
 
     return state_of, save_vjp
 
@@ -160,10 +161,6 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
                                       "sharpening kernel")
         sharpen = (float(kern.alpha), float(kern.kl), float(kern.ks))
     interp = solver.interp_kernel
-    drifting = type(interp).__name__ == "DriftInterp" or bool(getattr(interp, "drift_on_lightcone", False))
-    if drifting and autodiff.split_cosmo(cosmo_in)[1]:
-        raise NotImplementedError("cosmology gradients through the drift to the lightcone are not implemented "
-                                  "(gradients with respect to the particles are)")
     cosmo, _, _ = autodiff.split_cosmo(cosmo_in)
     mesh_size = displacements.mesh_size
     mode = _mode(displacements)
@@ -178,7 +175,8 @@ def _integrate_grad(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0
         else:
             events.append(("save", ev[1], ev[2]))
     saved_fields = []
-    _state, save_vjp = _snapshot_vjp(solver, displacements, velocities, tg, cosmo, t0)
+    _state, save_vjp = _snapshot_vjp(solver, displacements, velocities, tg, cosmo, t0,
+                                     param_names=tuple(autodiff.split_cosmo(cosmo_in)[1]))
 
     def run(x0, v0, cosmo_):
         x = x0.detach().contiguous().clone()
@@ -257,17 +255,12 @@ def _integrate_checkpointed(displacements, velocities, cosmo_in, ts, solver, t0,
     ceil(n_ops / checkpoints), or about sqrt(n_ops)); the backward recomputes each segment from its
     checkpoint and pulls the cotangents back through every op (the forces' and corrections' VJPs
     at the positions they saw) and every snapshot paint."""
-    interp = solver.interp_kernel
-    drifting = type(interp).__name__ == "DriftInterp" or bool(getattr(interp, "drift_on_lightcone", False))
     cosmo, names, _ = autodiff.split_cosmo(cosmo_in)
-    if names and drifting:
-        raise NotImplementedError("cosmology gradients through the drift to the lightcone are not implemented "
-                                  "(gradients with respect to the particles are)")
     mesh_size = displacements.mesh_size
     mode = _mode(displacements)
     tg = time_grid(t0, ts, dt0, t1)
     items = fold_ops(solver, cosmo, t0, ts, dt0, t1)
-    state_of, save_vjp = _snapshot_vjp(solver, displacements, velocities, tg, cosmo, t0)
+    state_of, save_vjp = _snapshot_vjp(solver, displacements, velocities, tg, cosmo, t0, param_names=tuple(names))
     op_items = [k for k, it in enumerate(items) if it[0] == "op"]
     op_index = {k: j for j, k in enumerate(op_items)}
     nops = len(op_items)
@@ -315,6 +308,7 @@ def _integrate_checkpointed(displacements, velocities, cosmo_in, ts, solver, t0,
         xb = torch.zeros_like(final[0])
         vb = torch.zeros_like(final[1])
         cbars = []
+        pdots = None  # parameter cotangents from the snapshots (the drift factor)
         starts = sorted(ckpts)
         for si in reversed(range(len(starts))):
             k0 = starts[si]
@@ -329,16 +323,18 @@ def _integrate_checkpointed(displacements, velocities, cosmo_in, ts, solver, t0,
             for k in reversed(range(k0, k1)):
                 xs, vs = states[k - k0]
                 if items[k][0] == "save":
-                    gx, gv = save_vjp(items[k][1], xs, vs, snaps_bar[items[k][1]])
+                    gx, gv, pd = save_vjp(items[k][1], xs, vs, snaps_bar[items[k][1]])
                     xb += gx
                     if gv is not None:
                         vb += gv
+                    if pd is not None:
+                        pdots = pd if pdots is None else pdots + pd
                 else:
                     fm = meshes[mesh_slot[k]] if k in mesh_slot else None
                     cbars.append((op_index[k], _op_vjp(items[k][1], xs, vs, after[1], xb, vb, fs, mesh_size, mode,
                                                        force_mesh=fm)))
                 after = (xs, vs)
-        return xb, vb, cbars
+        return xb, vb, cbars, pdots
 
     def coef_fn(c):
         return np.asarray([[o.pre or 0.0, o.kick or 0.0, o.post or 0.0,

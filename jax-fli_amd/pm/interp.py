@@ -56,14 +56,36 @@ def paint_drifted(dx, p, t, cosmo, r_center, width, painting, tiles=None):
                                                 density_width=np.atleast_1d(width), unit=DensityUnit.DENSITY)
 
 
-def paint_drifted_vjp(dx, p, t, cosmo, r_center, width, painting, map_bar, tiles=None):
-    """(position, velocity) cotangents of paint_drifted for the map cotangent map_bar."""
+def drift_factor_table(cosmo, a_now: float, chi_max: float) -> np.ndarray:
+    """f(chi) = (Gp(a(chi)) - Gp(a_now)) / dGfa(sqrt(a_now a(chi))) on the drift tables' chi grid."""
+    chi = np.linspace(0.0, chi_max, DRIFT_TABLE)
+    a_chi = np.asarray(_c.a_of_chi(cosmo, chi), dtype=np.float64)
+    gp_now = float(np.asarray(_c.Gp(cosmo, a_now)).reshape(-1)[0])
+    return (np.asarray(_c.Gp(cosmo, a_chi), dtype=np.float64) - gp_now) / np.asarray(
+        _c.dGfa(cosmo, np.sqrt(a_now * a_chi)), dtype=np.float64)
+
+
+def paint_drifted_vjp(dx, p, t, cosmo, r_center, width, painting, map_bar, tiles=None, param_names=()):
+    """(position, velocity, parameter dots or None) cotangents of paint_drifted for the map cotangent
+    map_bar; param_names: cosmological parameters differentiated through the drift factor (float64
+    differences of its table)."""
+    from .. import autodiff
+
     rmin, rmax = r_center - width / 2, r_center + width / 2
-    drift = drift_tables(cosmo, float(t), 2.0 * rmax + 100.0, dx.array.device)
-    return engine.paint_healpix_drift_vjp(dx.array.contiguous(), p.array.contiguous(), map_bar.contiguous(),
-                                          dx.mesh_size, dx.box_size, dx.observer_position_mpc, rmin, rmax, dx.nside,
-                                          drift, scheme=painting.scheme, mode=_paint_mode(dx.unit),
-                                          weights=1.0 if painting.weights is None else painting.weights, tiles=tiles)
+    chi_max = 2.0 * rmax + 100.0
+    dev = dx.array.device
+    drift = drift_tables(cosmo, float(t), chi_max, dev)
+    ptab = pdots = None
+    if param_names:
+        ptab = engine.to_device(np.stack([autodiff.fd(lambda c: drift_factor_table(c, float(t), chi_max), cosmo, n)
+                                          for n in param_names]), dev)
+        pdots = torch.zeros(len(param_names), dtype=torch.float64, device=dev)
+    gx, gv = engine.paint_healpix_drift_vjp(dx.array.contiguous(), p.array.contiguous(), map_bar.contiguous(),
+                                            dx.mesh_size, dx.box_size, dx.observer_position_mpc, rmin, rmax, dx.nside,
+                                            drift, scheme=painting.scheme, mode=_paint_mode(dx.unit),
+                                            weights=1.0 if painting.weights is None else painting.weights,
+                                            tiles=tiles, param_tables=ptab, param_dots=pdots)
+    return gx, gv, pdots
 
 
 @dataclasses.dataclass(frozen=True)
