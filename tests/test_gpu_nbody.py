@@ -1,6 +1,6 @@
 """End-to-end parity of the reference-shaped API on the GPU against the CPU oracle:
 initial conditions, LPT, full nbody runs (both solvers), snapshot/lightcone painting, P(k),
-reversibility. Bounds (north star): mesh relative L2 <= 1e-5 against the oracle's float64 run,
+reversibility. Bounds: mesh relative L2 <= 1e-5 against the oracle's float64 run,
 particle positions <= 1e-4, P(k) within 0.1% below k_Nyq/2. Each test also reports the oracle's own
 float32-vs-float64 error for scale."""
 
