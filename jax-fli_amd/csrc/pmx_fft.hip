@@ -47,6 +47,13 @@ template <int N> constexpr int LPW() {
   return l < 1 ? 1 : (l > 64 ? 64 : l);
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin by linear id, i.e. by blockIdx.x when
+// gridDim.x is a multiple of 8. Remapping x so that each XCD takes one contiguous range of line
+// groups puts the workgroups whose transposed stores share cache lines on one L2, which merges them.
+__device__ __forceinline__ unsigned xcd_x() {
+  return (gridDim.x % 8u == 0u) ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
+}
+
 // ---------------------------------------------------------------- complex helpers
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -169,7 +176,7 @@ __global__ void __launch_bounds__(256) k_r2c_z(const float* __restrict__ in, flo
                                                const float2* __restrict__ twN) {
   constexpr int M = NZ / 2, T = TPL<M>(), L = LPW<M>(), LS = M + 1, NT = L * T;
   __shared__ float2 s0[L * LS], s1[L * LS];
-  const int x = blockIdx.y, y0 = blockIdx.x * L, tid = threadIdx.x;
+  const int x = blockIdx.y, y0 = (int)xcd_x() * L, tid = threadIdx.x;
   const int nl = min(L, NY - y0);
   const float2* src = reinterpret_cast<const float2*>(in + ((size_t)x * NY + y0) * NZ);
   for (int i = tid; i < nl * M; i += NT) {
@@ -208,7 +215,8 @@ __global__ void __launch_bounds__(256) k_c2c_lines(const float2* __restrict__ in
                                                    int nlines, const float2* __restrict__ tw) {
   constexpr int T = TPL<N>(), L = LPW<N>(), LS = N + 1, NT = L * T;
   __shared__ float2 s0[L * LS], s1[L * LS];
-  const int by = blockIdx.y, bz = blockIdx.z, l0 = blockIdx.x * L, tid = threadIdx.x;
+  const int by = blockIdx.y, bz = blockIdx.z, l0 = (int)xcd_x() * L, tid = threadIdx.x;
+  if (l0 >= nlines) return;  // a padded grid's extra workgroups exit whole
   const int nl = min(L, nlines - l0);
   const float2* src = in + by * isy + bz * isz + (long)l0 * isl;
   for (int i = tid; i < nl * N; i += NT) {
@@ -255,7 +263,7 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
   __shared__ float2 sm[NBUF * L * LS];
   float2* s0 = sm;
   float2* s1 = sm + L * LS;
-  const int kz = blockIdx.y, ky0 = blockIdx.x * L, tid = threadIdx.x;
+  const int kz = blockIdx.y, ky0 = (int)xcd_x() * L, tid = threadIdx.x;
   const int nl = min(L, NY - ky0);
   const float2* src = in + ((size_t)kz * NY + ky0) * N;
   for (int i = tid; i < nl * N; i += NT) {
@@ -489,7 +497,11 @@ template <int N, bool INV>
 int launch_c2c(const float2* in, float2* out, long isy, long isl, long isz, long osy, long osk, long osz,
                int nlines, int ny_blocks, int nz_blocks, const float2* tw, hipStream_t s) {
   constexpr int L = LPW<N>(), T = TPL<N>();
-  dim3 grid((nlines + L - 1) / L, ny_blocks, nz_blocks), block(L * T);
+  // runs of fewer than 8 lines store partial 64-byte sectors: pad the grid's x extent to a multiple
+  // of 8 so the XCD remap gathers the workgroups sharing those sectors on one L2
+  const int g0 = (nlines + L - 1) / L;
+  const int gx = L < 8 ? (g0 + 7) / 8 * 8 : g0;
+  dim3 grid(gx, ny_blocks, nz_blocks), block(L * T);
   hipLaunchKernelGGL((k_c2c_lines<N, INV>), grid, block, 0, s, in, out, isy, isl, isz, osy, osk, osz, nlines, tw);
   return pmx_check_launch("c2c_lines");
 }
