@@ -41,7 +41,7 @@ int pmx_plan_create(pmx_plan* plan, int nx, int ny, int nz, unsigned flags);
 int pmx_plan_destroy(pmx_plan plan);
 size_t pmx_plan_workspace_bytes(pmx_plan plan);
 /* Entries of the last sorted paint on this plan that fell outside their particle sub-block's
- * 5x5x5-tile window and were counted and placed one by one with global atomics (particles displaced
+ * 5x5x8-tile window and were counted and placed one by one with global atomics (particles displaced
  * by more than about two tiles). Synchronises the device. */
 int pmx_plan_escapes(pmx_plan plan, uint64_t* count);
 /* Per-kernel timing of pmx_step on this plan. enable > 0 clears previous records, creates the HIP

@@ -177,11 +177,16 @@ __global__ void __launch_bounds__(256) k_paint(PartArgs a, const float* __restri
 // 2^30 (exact, order-independent sums); each deposit is truncated to that grid.
 //
 // Particles are read in Lagrangian sub-blocks of 8 x 8 x 64 lattice sites; a sub-block's tiles lie
-// within +-2 tiles of the tile of its first site, so they are histogrammed in a 5x5x5 LDS window
-// and reserved with one global atomic per window slot. Particles outside the window (displaced by
-// more than about two tiles) count and reserve one by one with global atomics.
+// within a window of 5 x 5 x 8 tiles around the tile of its first site (-2..+2 in x and y, -2..+5
+// in z), so they are histogrammed in an LDS window and reserved with one global atomic per window
+// slot. Particles outside the window (displaced by more than about two tiles) count and reserve one
+// by one with global atomics.
 constexpr int SB_X = 8, SB_Y = 8, SB_Z = 64;  // particle sub-block (4096 sites, 256 threads x 16)
-constexpr int SORT_WIN = 5, SORT_SLOTS = SORT_WIN * SORT_WIN * SORT_WIN;
+// window of tiles around the tile of a sub-block's first site: SW_* tiles per axis, SL_* of them
+// below it. z is longer and reaches further up because a sub-block is 64 sites long in z (two tiles
+// at one particle per cell, four at the 2-cell spacing of a particle sub-lattice).
+constexpr int SW_X = 5, SW_Y = 5, SW_Z = 8, SL_X = 2, SL_Y = 2, SL_Z = 2;
+constexpr int SORT_SLOTS = SW_X * SW_Y * SW_Z;
 constexpr int TILE_PAD_MAX = 17 * 17 * 33;    // padded tile cells (int64 LDS: 76 KiB)
 
 // Division by a runtime constant: q = floor(n / d) for all 32-bit n (round-up multiplier).
@@ -295,8 +300,7 @@ __device__ __forceinline__ int3 window_centre(const PartArgs& a, const SortGeom&
 }
 
 __device__ __forceinline__ unsigned slot_tile(const SortGeom& g, int3 c, int s) {
-  const int h = SORT_WIN / 2;
-  const int dz = s % SORT_WIN - h, dy = (s / SORT_WIN) % SORT_WIN - h, dx = s / (SORT_WIN * SORT_WIN) - h;
+  const int dz = s % SW_Z - SL_Z, dy = (s / SW_Z) % SW_Y - SL_Y, dx = s / (SW_Y * SW_Z) - SL_X;
   const int tx = wrap_idx(c.x + dx, g.ntx), ty = wrap_idx(c.y + dy, g.nty), tz = wrap_idx(c.z + dz, g.ntz);
   return ((unsigned)tx * g.nty + ty) * g.ntz + tz;
 }
@@ -326,7 +330,7 @@ __device__ __forceinline__ unsigned wave_add(bool on, int key, unsigned* cnt) {
 template <class Fn>
 __device__ __forceinline__ void for_particles(const PartArgs& a, const SortGeom& g, int i0, int j0, int k0, int3 c,
                                               Fn&& fn) {
-  constexpr int PER = (SB_X * SB_Y * SB_Z) / 256, U = 4, h = SORT_WIN / 2;
+  constexpr int PER = (SB_X * SB_Y * SB_Z) / 256, U = 4;
   for (int r0 = 0; r0 < PER; r0 += U) {
     unsigned ps[U];
     bool vs[U];
@@ -348,8 +352,9 @@ __device__ __forceinline__ void for_particles(const PartArgs& a, const SortGeom&
       const int tx = b.cx >> g.sx, ty = b.cy >> g.sy, tz = b.cz >> g.sz;
       const unsigned tile = ((unsigned)tx * g.nty + ty) * g.ntz + tz;
       const int dx = tdiff(tx, c.x, g.ntx), dy = tdiff(ty, c.y, g.nty), dz = tdiff(tz, c.z, g.ntz);
-      const bool inw = dx >= -h && dx <= h && dy >= -h && dy <= h && dz >= -h && dz <= h;
-      fn(vs[u], ps[u], tile, inw ? ((dx + h) * SORT_WIN + (dy + h)) * SORT_WIN + (dz + h) : -1);
+      const bool inw = (unsigned)(dx + SL_X) < (unsigned)SW_X && (unsigned)(dy + SL_Y) < (unsigned)SW_Y &&
+                       (unsigned)(dz + SL_Z) < (unsigned)SW_Z;
+      fn(vs[u], ps[u], tile, inw ? ((dx + SL_X) * SW_Y + (dy + SL_Y)) * SW_Z + (dz + SL_Z) : -1);
     }
   }
 }
@@ -366,7 +371,7 @@ constexpr unsigned SORT_FAR = 0xFFFFFFFFu;  // pack entry of a particle outside 
 __device__ __forceinline__ void count_relative(const PartArgs& a, const SortGeom& g, int i0, int j0, int k0, int3 c,
                                                unsigned* lcnt, unsigned* __restrict__ count,
                                                unsigned* __restrict__ pack, unsigned& nfar) {
-  constexpr int PER = (SB_X * SB_Y * SB_Z) / 256, U = 4, h = SORT_WIN / 2;
+  constexpr int PER = (SB_X * SB_Y * SB_Z) / 256, U = 4;
   const int wv = (int)threadIdx.x >> 6;
   const int k = k0 + ((int)threadIdx.x & 63);
   const bool kin = k < a.npz;
@@ -391,10 +396,9 @@ __device__ __forceinline__ void count_relative(const PartArgs& a, const SortGeom
       const int bx = __mul24(is[u], g.rx) + (int)floorf(xs[u].x);
       const int by = __mul24(js[u], g.ry) + (int)floorf(xs[u].y);
       const int bz = bz0 + (int)floorf(xs[u].z);
-      const int dx = (bx >> g.sx) - c.x + h, dy = (by >> g.sy) - c.y + h, dz = (bz >> g.sz) - c.z + h;
-      const bool inw =
-          (unsigned)dx < (unsigned)SORT_WIN && (unsigned)dy < (unsigned)SORT_WIN && (unsigned)dz < (unsigned)SORT_WIN;
-      const int slot = (dx * SORT_WIN + dy) * SORT_WIN + dz;
+      const int dx = (bx >> g.sx) - c.x + SL_X, dy = (by >> g.sy) - c.y + SL_Y, dz = (bz >> g.sz) - c.z + SL_Z;
+      const bool inw = (unsigned)dx < (unsigned)SW_X && (unsigned)dy < (unsigned)SW_Y && (unsigned)dz < (unsigned)SW_Z;
+      const int slot = (dx * SW_Y + dy) * SW_Z + dz;
       const bool on = vs[u];
       const unsigned rk = wave_add(on && inw, slot, lcnt);  // rank within the slot's segment
       if (on) pack[ps[u]] = inw ? ((unsigned)slot << 16) | rk : SORT_FAR;
