@@ -203,6 +203,18 @@ int pmx_paint_healpix_lightcone_vjp(pmx_plan plan, const float* pos, const float
                                     float* pos_bar, float* vel_bar, int accumulate, int nparam,
                                     const double* param_tables, double* param_dots, void* stream);
 
+/* One image of the particles as absolute grid positions (npx*npy*npz*3 floats in out): X = the
+ * particle's unwrapped position in Mpc/h (lattice + displacement in relative mode, not wrapped into
+ * the box) minus observer, Y = R X + shift with R the cube symmetry `rot` (0..47 in the order of
+ * get_all_47_symmetries before it drops entry 0, the identity), and with vel (may be null: no drift)
+ * Y += f(|Y|) R v, f the lightcone drift factor from the tables of pmx_paint_healpix_lightcone; out =
+ * (Y + observer) / cell. The rotate / shift / drift that TelephotoInterp applies before its paint
+ * (_paint_telephoto and _paint_single, src/jax_fli/pm/interp.py:590-660; _drift_to_time, 110-121). */
+int pmx_image_positions(pmx_plan plan, const float* pos, const float* vel, int npx, int npy, int npz, int mode,
+                        const double box[3], const double observer[3], int rot, const double shift[3],
+                        const double* drift_tables, int ntab, double chi_max, double a_lo, double a_hi, double a_now,
+                        double gp_now, float* out, void* stream);
+
 /* Flat-sky slab paint: absolute z in (zlo, zhi] (grid units), (x, y) rescaled to (fnx, fny) and
  * wrapped, 2D CIC, times `scale` (1 / pixel volume). _single_paint_2d (_painting.py:117-202). */
 int pmx_paint_flat(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, const float* weights,

@@ -993,6 +993,57 @@ int pmx_sky_max_tiles() { return HP_MAX_TILES; }
 int pmx_born_max_shells() { return kBornMaxShells; }
 int pmx_born_max_sources() { return kBornMaxSources; }
 
+namespace {
+// One image of the particles as absolute grid positions: X = unwrapped position (Mpc/h) - observer,
+// Y = R X + shift, with sa.v Y += f(|Y|) R v (drift_image), out = (Y + observer) / cell. One thread
+// per particle; float64 arithmetic, rounded once to float32.
+__global__ void __launch_bounds__(256) k_image_positions(PartArgs a, ShellArgs sa, int code, double sh0, double sh1,
+                                                         double sh2, float* __restrict__ out) {
+  const size_t n = (size_t)a.npx * a.npy * a.npz;
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int k = (int)(p % (size_t)a.npz);
+  const size_t q = p / (size_t)a.npz;
+  const int j = (int)(q % (size_t)a.npy), i = (int)(q / (size_t)a.npy);
+  const float3 x = ld3(a.x, (unsigned)p);
+  const bool rel = a.mode == 0;
+  const double X[3] = {((double)x.x + (rel ? (double)(i * sa.rx) : 0.0)) * sa.cell[0] - sa.obs[0],
+                       ((double)x.y + (rel ? (double)(j * sa.ry) : 0.0)) * sa.cell[1] - sa.obs[1],
+                       ((double)x.z + (rel ? (double)(k * sa.rz) : 0.0)) * sa.cell[2] - sa.obs[2]};
+  const Sym R = sym_of(code);
+  const double sh[3] = {sh0, sh1, sh2};
+  double Y[3];
+  for (int c = 0; c < 3; ++c) Y[c] = R.s[c] * X[R.p[c]] + sh[c];
+  if (sa.v) {
+    const float3 v = ld3(sa.v, (unsigned)p);
+    const double V[3] = {v.x, v.y, v.z};
+    double Vt[3];
+    for (int c = 0; c < 3; ++c) Vt[c] = R.s[c] * V[R.p[c]];
+    drift_image(sa, Y, Vt);
+  }
+  for (int c = 0; c < 3; ++c) out[3 * p + c] = (float)((Y[c] + sa.obs[c]) / sa.cell[c]);
+}
+}  // namespace
+
+int pmx_launch_image_positions(pmx_plan_t* p, const PartArgs& a, const double box[3], const double obs[3], int code,
+                               const double shift[3], float* out, hipStream_t s, const PmxDrift* drift) {
+  ShellArgs sa = shell_args(p, a, box, obs, 0.0, 0.0, 0, 1, nullptr, nullptr);
+  if (drift) {
+    sa.v = drift->v;
+    sa.dtab = drift->tables;
+    sa.ntab = drift->ntab;
+    sa.chi_max = drift->chi_max;
+    sa.a_lo = drift->a_lo;
+    sa.a_hi = drift->a_hi;
+    sa.a_now = drift->a_now;
+    sa.gp_now = drift->gp_now;
+  }
+  const size_t n = (size_t)a.npx * a.npy * a.npz;
+  hipLaunchKernelGGL(k_image_positions, dim3(grid_for(n, 256)), dim3(256), 0, s, a, sa, code, shift[0], shift[1],
+                     shift[2], out);
+  return pmx_check_launch("image_positions");
+}
+
 int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, const double box[3],
                        const double obs[3], double rmin, double rmax, int nside, int scheme, int ntiles,
                        const int* rot, const double* shift, float* map, int accumulate, hipStream_t s,

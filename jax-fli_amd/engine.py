@@ -470,6 +470,33 @@ def paint_flat(pos: torch.Tensor, mesh_shape, flatsky_npix, zlo, zhi, scale, mod
     return out
 
 
+def image_positions(pos: torch.Tensor, mesh_shape, box, observer, rot=0, shift=(0.0, 0.0, 0.0), mode="relative",
+                    vel=None, drift=None, out=None) -> torch.Tensor:
+    """Absolute grid positions (npx, npy, npz, 3) of one image of the particles: rotated by the cube
+    symmetry `rot` about the observer, shifted by `shift` (Mpc/h) and, with vel and drift (the tables
+    of paint_healpix_drift), drifted to each image's own lightcone time. Positions are not wrapped."""
+    pos = _f32(pos, "pos")
+    npx, npy, npz = _pgrid(pos)
+    plan = get_plan(mesh_shape, pos.device)
+    if out is None:
+        out = torch.empty_like(pos)
+    if vel is not None:
+        vel = _f32(vel, "vel")
+        if tuple(vel.shape) != tuple(pos.shape):
+            raise ValueError("vel must have the shape of pos")
+        if drift is None:
+            raise ValueError("a drift needs its tables")
+        tab = drift["tables"]
+        targs = (ptr(tab), int(tab.shape[-1]), float(drift["chi_max"]), float(drift["a_lo"]), float(drift["a_hi"]),
+                 float(drift["a_now"]), float(drift["gp_now"]))
+    else:
+        targs = (None, 0, 0.0, 0.0, 0.0, 0.0, 0.0)
+    check(load().pmx_image_positions(plan.handle, ptr(pos), ptr(vel), npx, npy, npz, MODES[mode], dbl3(box),
+                                     dbl3(observer), int(rot), dbl3(shift), *targs, ptr(out), stream_ptr()),
+          "pmx_image_positions")
+    return out
+
+
 # ---------------------------------------------------------------------------- adjoints
 def _zeros_like_or(t, like):
     return torch.zeros_like(like) if t is None else t

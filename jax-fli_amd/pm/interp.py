@@ -1,9 +1,10 @@
-"""Snapshot interpolation kernels (src/jax_fli/pm/interp.py:46-504): the abstract interface,
+"""Snapshot interpolation kernels (src/jax_fli/pm/interp.py:46-699): the abstract interface,
 ``NoInterp`` (paint the particles as they are at each snapshot), ``DriftInterp`` (spherical shells
 with every particle drifted to its own lightcone crossing) and ``OnionTiler`` (spherical shells
 beyond the box painted from rotated periodic images of it, optionally drifted). The drift runs inside
-the HEALPix paint kernel from host-built float64 tables of a(chi), Gp(a) and dGfa(a). The Telephoto
-kernel is not implemented on this engine."""
+the HEALPix paint kernel from host-built float64 tables of a(chi), Gp(a) and dGfa(a); and
+``TelephotoInterp`` (one rotated, shifted image of the box per shell, for flat or spherical
+targets)."""
 
 from __future__ import annotations
 
@@ -17,7 +18,7 @@ import torch
 
 from .. import cosmology as _c
 from .. import engine
-from ..fields import DensityUnit, FieldStatus, SphericalDensity, _paint_mode
+from ..fields import DensityUnit, FieldStatus, PositionUnit, SphericalDensity, _paint_mode
 from ..painting import PaintingOptions
 
 DRIFT_TABLE = 4096
@@ -274,6 +275,95 @@ class OnionTiler(AbstractInterp):
         return result.replace(scale_factors=float(t), comoving_centers=r_center, density_width=width)
 
 
+@dataclasses.dataclass(frozen=True)
 class TelephotoInterp(AbstractInterp):
-    def __init__(self, *a, **k):
-        raise NotImplementedError("TelephotoInterp is not implemented on the pmx engine")
+    """One rotated, shifted copy of the box per shell, for narrow fields of view (src/jax_fli/pm/
+    interp.py:507-699). A shell whose centre is inside the box radius (r_center <=
+    max_comoving_distance) is painted from the box itself; a farther one from the box rotated about
+    the observer by symmetry (rotation_idx mod 47) of get_all_47_symmetries and moved along the line
+    of sight (z) by floor(r_center / max_comoving_distance) * max_comoving_distance, where
+    rotation_idx counts the shells painted so far beyond the box radius (from -1). With
+    drift_on_lightcone (the default) every particle is drifted to its own lightcone time first.
+    Flat targets: pmx_image_positions builds the image's absolute grid positions and the flat slab
+    paint takes them; spherical targets: one image of the HEALPix shell paint.
+
+    Differences from the reference, by design: the drift's lightcone time is taken at the image's
+    own distance from the observer (the reference measures it from the image minus the observer
+    position a second time, interp.py:647, which agrees only for an observer at the origin); rewind
+    undoes advance with the same centre test (the reference's rewind tests r_center + width/2,
+    interp.py:556-558), so the adjoint paints each shell with the forward's rotation; spherical
+    images are painted from box-wrapped positions, as OnionTiler's are."""
+
+    drift_on_lightcone: bool = True
+
+    def init(self) -> InterpTilerState:
+        tgt = None if self.painting is None else self.painting.target
+        if tgt not in ("spherical", "flat"):
+            raise ValueError("TelephotoInterp only supports projections (flat or spherical).")
+        if tgt == "spherical":
+            warnings.warn("TelephotoInterp is meant for narrow-field flat-sky maps: full-sky HEALPix shells painted "
+                          "from one image of the box show strong artifacts.")
+        return InterpTilerState(rotation_idx=-1, shell_idx=-1, rotations=np.arange(1, 48, dtype=np.int32))
+
+    def _r_center(self, idx):
+        return float(np.asarray(self.r_centers).reshape(-1)[idx])
+
+    def _beyond(self, idx):
+        return self._r_center(idx) > self.max_comoving_distance
+
+    def advance(self, state, t):
+        hit = self.ts is not None and bool(np.isin(np.float32(t), np.asarray(self.ts, dtype=np.float32)))
+        if not hit:
+            return state
+        nxt = state.shell_idx + 1
+        return dataclasses.replace(state, shell_idx=nxt, rotation_idx=state.rotation_idx + int(self._beyond(nxt)))
+
+    def rewind(self, state, t):
+        hit = self.ts is not None and bool(np.isin(np.float32(t), np.asarray(self.ts, dtype=np.float32)))
+        if not hit:
+            return state
+        return dataclasses.replace(state, shell_idx=state.shell_idx - 1,
+                                   rotation_idx=state.rotation_idx - int(self._beyond(state.shell_idx)))
+
+    def rotation_code(self, shell_idx: int) -> int:
+        """Cube-symmetry code (1..47) of the shell's image: entry rotation_idx mod 47 of the 47
+        non-identity symmetries, rotation_idx = (shells 0..shell_idx beyond the box radius) - 1."""
+        rc = np.asarray(self.r_centers, dtype=np.float64).reshape(-1)[:shell_idx + 1]
+        return int((int((rc > self.max_comoving_distance).sum()) - 1) % 47) + 1
+
+    def images(self, state, field):
+        r_center, _ = self.shell(state)
+        if self.max_comoving_distance is None or r_center <= self.max_comoving_distance:
+            return None
+        d = float(np.floor(r_center / self.max_comoving_distance) * self.max_comoving_distance)
+        return (np.array([self.rotation_code(state.shell_idx)], dtype=np.int32),
+                np.array([[0.0, 0.0, d]], dtype=np.float64))
+
+    def paint(self, state, t, y, cosmo):
+        dx, p = y
+        r_center, width = self.shell(state)
+        pt = self.painting
+        tiles = self.images(state, dx)
+        if pt.target == "spherical":
+            if self.drift_on_lightcone:
+                out = paint_drifted(dx, p, t, cosmo, r_center, width, pt, tiles=tiles)
+            else:
+                out = dx.paint_spherical(center=r_center, density_plane_width=width, scheme=pt.scheme,
+                                         weights=pt.weights, kernel_width_arcmin=pt.kernel_width_arcmin,
+                                         smoothing_interpretation=pt.smoothing_interpretation,
+                                         paint_nside=pt.paint_nside, batch_size=pt.batch_size, tiles=tiles)
+        else:
+            src = dx
+            if tiles is not None or self.drift_on_lightcone:
+                code, shift = (0, np.zeros(3)) if tiles is None else (int(tiles[0][0]), tiles[1][0])
+                drift = None
+                if self.drift_on_lightcone:
+                    reach = float(np.linalg.norm(shift)) + 2.0 * float(np.linalg.norm(dx.box_size)) + 100.0
+                    drift = drift_tables(cosmo, float(t), reach, dx.array.device)
+                g = engine.image_positions(dx.array.contiguous(), dx.mesh_size, dx.box_size, dx.observer_position_mpc,
+                                           rot=code, shift=shift, mode=_paint_mode(dx.unit),
+                                           vel=p.array.contiguous() if drift else None, drift=drift)
+                src = dx.replace(array=g, unit=PositionUnit.GRID_ABSOLUTE)
+            out = src.paint_2d(center=r_center, density_plane_width=width, weights=pt.weights,
+                               batch_size=pt.batch_size)
+        return out.replace(scale_factors=float(t), comoving_centers=r_center, density_width=width)

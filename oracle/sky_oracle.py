@@ -31,6 +31,14 @@ Parity status per function:
                                         in the reference tree: restated as Simpson over [0, 10]
                                         with 256 intervals)
   born_vjp                              pinned (central differences of born_spherical)
+  paint_flat                            pinned (a slab holding the whole box on a flat grid of the
+                                        mesh's (nx, ny) = the 3D CIC paint summed over z; a particle on
+                                        integer flat coordinates fills one pixel; slab edges are
+                                        (lo, hi]; mass = selected weight / pixel volume)
+  image_positions                       pinned (code 0, zero shift and no drift give the unwrapped
+                                        grid positions; a code permutes and flips the centred axes as
+                                        cube_symmetry's matrix does; the drift moves each image by
+                                        Drift.factor(|Y|) R v)
 """
 
 from __future__ import annotations
@@ -419,3 +427,56 @@ def born_vjp(maps, kappa_bar, r, a, sources, d_r, Om, chi_fn, **kw):
         else:
             out[s] = g / mean[s] - (g * maps[s]).sum() / (mean[s] ** 2 * npix)
     return out
+
+
+# ---------------------------------------------------------------------------- flat sky
+def paint_flat(pos, mesh_shape, flatsky_npix, zlo, zhi, scale, mode="relative", weight=1.0):
+    """_single_paint_2d (src/jax_fli/_src/fields/_painting.py:160-202) on absolute grid positions
+    (lattice + displacement in relative mode, not wrapped): particles with zlo < z <= zhi (grid
+    units), (x, y) scaled by flatsky_npix / (nx, ny) and wrapped, then jaxpm's cic_paint_2d (4
+    corners floor + {0, 1}, weights 1 - |x - corner|, periodic), times scale (1 / pixel volume)."""
+    pos = np.asarray(pos)
+    np_shape = pos.shape[:3]
+    dt = np.float64 if pos.dtype == np.float64 else np.float32  # float32: the device's absolute positions
+    g = pos.reshape(-1, 3).astype(dt)
+    if mode == "relative":
+        g = g + orc._lattice(np_shape, mesh_shape).astype(dt)
+    g = g.astype(np.float64)
+    n = g.shape[0]
+    wt = np.broadcast_to(np.asarray(weight, dtype=np.float64).reshape(-1), (n,))
+    fnx, fny = int(flatsky_npix[0]), int(flatsky_npix[1])
+    sel = (g[:, 2] > zlo) & (g[:, 2] <= zhi)
+    xy = g[sel, :2] * np.array([fnx / mesh_shape[0], fny / mesh_shape[1]])
+    xy = np.mod(xy, np.array([fnx, fny], dtype=np.float64))
+    w = wt[sel]
+    fl = np.floor(xy)
+    out = np.zeros(fnx * fny)
+    for ox in (0, 1):
+        for oy in (0, 1):
+            c = fl + np.array([ox, oy])
+            k = (1.0 - np.abs(xy[:, 0] - c[:, 0])) * (1.0 - np.abs(xy[:, 1] - c[:, 1])) * w
+            ix = np.mod(c[:, 0].astype(np.int64), fnx)
+            iy = np.mod(c[:, 1].astype(np.int64), fny)
+            out += np.bincount(ix * fny + iy, weights=k, minlength=out.size)
+    return (out * scale).reshape(fnx, fny)
+
+
+def image_positions(pos, mesh_shape, box, observer, code=0, shift=(0.0, 0.0, 0.0), mode="relative", vel=None,
+                    drift=None):
+    """Absolute grid positions of one image of the particles (TelephotoInterp._paint_telephoto and
+    _paint_single, src/jax_fli/pm/interp.py:590-660, with _drift_to_time, 110-121): X the unwrapped
+    position in Mpc/h minus the observer, Y = R_code X + shift, and with drift (a Drift) and vel,
+    Y += drift.factor(|Y|) R_code vel; returned as (Y + observer) / cell, shape (npart, 3)."""
+    pos = np.asarray(pos)
+    np_shape = pos.shape[:3]
+    g = pos.reshape(-1, 3).astype(np.float64)
+    if mode == "relative":
+        g = g + orc._lattice(np_shape, mesh_shape).astype(np.float64)
+    cell = np.asarray(box, dtype=np.float64) / np.asarray(mesh_shape, dtype=np.float64)
+    obs = np.asarray(observer, dtype=np.float64)
+    R = cube_symmetry(code)
+    Y = (g * cell - obs) @ R.T + np.asarray(shift, dtype=np.float64)
+    if drift is not None:
+        V = np.asarray(vel, dtype=np.float64).reshape(-1, 3) @ R.T
+        Y = Y + drift.factor(np.sqrt((Y**2).sum(axis=1)))[:, None] * V
+    return (Y + obs) / cell

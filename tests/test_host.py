@@ -165,3 +165,37 @@ def test_native_growth_table_equals_python_loop(pkg):
         xs_n, D_n, G_n, dG_n = c._growth_table_bg(bg, True)
         xs_p, D_p, G_p, dG_p = c._growth_table_bg(bg, False)
         assert np.array_equal(D_n, D_p) and np.array_equal(G_n, G_p) and np.array_equal(dG_n, dG_p)
+
+
+def test_telephoto_rotations_and_shifts(pkg):
+    """TelephotoInterp (src/jax_fli/pm/interp.py:507-580): the shell index advances at snapshot
+    times; rotation_idx counts the shells beyond the box radius (from -1); the image is symmetry
+    rotation_idx mod 47 of the 47 non-identity ones, shifted along z by the radius times
+    floor(r_center / radius); shells whose centre is within the radius are painted from the box."""
+    ts = np.array([0.3, 0.5, 0.7, 0.9], dtype=np.float32)
+    r_c = np.array([260.0, 160.0, 70.0, 40.0])
+    tel = pkg.TelephotoInterp(painting=pkg.PaintingOptions(target="flat")).update_geometry(
+        ts=ts, r_centers=r_c, density_widths=np.array([60.0, 60.0, 60.0, 30.0]), max_comoving_distance=50.0)
+    st = tel.init()
+    assert st.rotation_idx == -1 and st.shell_idx == -1 and list(st.rotations) == list(range(1, 48))
+    seen = []
+    for t in ts:
+        st = tel.advance(tel.advance(st, float(t) - 0.05), float(t))  # a non-snapshot time changes nothing
+        seen.append((st.shell_idx, st.rotation_idx, tel.images(st, None)))
+    assert [s[:2] for s in seen] == [(0, 0), (1, 1), (2, 2), (3, 2)]
+    codes = [None if s[2] is None else int(s[2][0][0]) for s in seen]
+    assert codes == [1, 2, 3, None]  # rotation_idx 0, 1, 2 -> entries 1, 2, 3 of the 48-entry list
+    assert np.array_equal(seen[0][2][1], [[0.0, 0.0, 250.0]]) and np.array_equal(seen[1][2][1], [[0.0, 0.0, 150.0]])
+    assert np.array_equal(seen[2][2][1], [[0.0, 0.0, 50.0]])
+    for shell in range(3):
+        assert tel.rotation_code(shell) == codes[shell]
+    for t in ts[::-1]:  # rewinding undoes every advance
+        st = tel.rewind(st, float(t))
+    assert st.shell_idx == -1 and st.rotation_idx == -1
+    many = tel.update_geometry(ts=ts, r_centers=np.full(50, 200.0), density_widths=np.full(50, 10.0),
+                               max_comoving_distance=50.0)
+    assert many.rotation_code(46) == 47 and many.rotation_code(47) == 1  # the index wraps mod 47
+    with pytest.raises(ValueError):
+        pkg.TelephotoInterp(painting=pkg.PaintingOptions(target="density")).init()
+    with pytest.warns(UserWarning):
+        pkg.TelephotoInterp(painting=pkg.PaintingOptions(target="spherical")).init()

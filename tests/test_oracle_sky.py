@@ -276,3 +276,62 @@ def test_drifted_healpix_vjp_matches_differences():
             got = (gv * dd).sum()
         fd = ((hi - lo) * mb).sum() / (2 * e)
         assert got == pytest.approx(fd, rel=1e-6), which
+
+
+# ---------------------------------------------------------------- flat-sky slab paint
+def test_flat_paint_of_whole_box_is_projected_cic():
+    """A slab holding the whole box on a flat grid of the mesh's (nx, ny): every z weight sums to one,
+    so the flat map times the pixel volume is the (pinned) 3D CIC paint summed over z."""
+    rng = np.random.default_rng(3)
+    mesh = (8, 6, 4)
+    dx = rng.normal(0.0, 1.5, (4, 3, 2, 3))
+    w = rng.uniform(0.5, 2.0, 4 * 3 * 2)
+    for mode, pos in (("relative", dx), ("absolute", rng.uniform(0, 1, dx.shape) * np.array(mesh))):
+        flat = so.paint_flat(pos, mesh, (8, 6), -1e9, 1e9, 1.0, mode=mode, weight=w)
+        ref = orc.cic_paint(pos, mesh, mode=mode, weight=w, dtype=np.float64).sum(axis=2)
+        assert np.allclose(flat, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_flat_paint_pixels_slab_edges_and_scale():
+    mesh = (4, 4, 8)
+    pos = np.zeros((1, 1, 4, 3), dtype=np.float32)
+    pos[0, 0, :, 0] = [1.0, 1.5, 3.0, 3.999]
+    pos[0, 0, :, 1] = [2.0, 2.0, 0.0, 1.0]
+    pos[0, 0, :, 2] = [2.0, 3.0, 5.0, 2.0]  # slab (2, 5]: the first and last are out
+    m = so.paint_flat(pos, mesh, (8, 4), 2.0, 5.0, 0.5, mode="absolute")
+    # x scaled by 8 / 4 = 2: 1.5 -> 3.0 and 3.0 -> 6.0, integer flat coordinates fill one pixel each
+    expect = np.zeros((8, 4))
+    expect[3, 2] += 0.5      # (1.5, 2.0) -> (3.0, 2.0)
+    expect[6, 0] += 0.5      # (3.0, 0.0) -> (6.0, 0.0)
+    assert np.allclose(m, expect)
+    # a half-pixel offset splits evenly; wrapping carries the far corner to pixel 0
+    pos2 = np.array([[[[3.75, 0.5, 1.0]]]], dtype=np.float32)
+    m2 = so.paint_flat(pos2, mesh, (8, 4), 0.0, 8.0, 1.0, mode="absolute")
+    e2 = np.zeros((8, 4))
+    e2[7, 0] = e2[7, 1] = e2[0, 0] = e2[0, 1] = 0.25
+    assert np.allclose(m2, e2)
+
+
+# ---------------------------------------------------------------- one-image transform (Telephoto)
+def test_image_positions_identity_rotation_and_drift():
+    rng = np.random.default_rng(5)
+    mesh, box, obs = (8, 8, 8), (80.0, 80.0, 80.0), np.array([40.0, 40.0, 0.0])
+    dx = rng.normal(0.0, 2.0, (4, 4, 4, 3))
+    g = dx.reshape(-1, 3) + orc._lattice((4, 4, 4), mesh)
+    assert np.allclose(so.image_positions(dx, mesh, box, obs), g, rtol=0, atol=1e-12)  # unwrapped
+    cell = 10.0
+    X = g * cell - obs
+    for code, (perm, sign) in {7: ((0, 2, 1), (1, 1, -1)), 46: ((2, 0, 1), (-1, -1, -1))}.items():
+        Y = np.stack([sign[c] * X[:, perm[c]] for c in range(3)], axis=1) + np.array([0.0, 0.0, 160.0])
+        got = so.image_positions(dx, mesh, box, obs, code=code, shift=(0.0, 0.0, 160.0))
+        assert np.allclose(got, (Y + obs) / cell, atol=1e-12)
+
+    class ConstDrift:
+        def factor(self, r):
+            return 0.25 * np.ones_like(r)
+
+    v = rng.normal(0.0, 1.0, (4, 4, 4, 3))
+    got = so.image_positions(dx, mesh, box, obs, code=7, shift=(0.0, 0.0, 160.0), vel=v, drift=ConstDrift())
+    Vt = np.stack([[1, 1, -1][c] * v.reshape(-1, 3)[:, (0, 2, 1)[c]] for c in range(3)], axis=1)
+    Y = np.stack([[1, 1, -1][c] * X[:, (0, 2, 1)[c]] for c in range(3)], axis=1) + np.array([0.0, 0.0, 160.0])
+    assert np.allclose(got, (Y + 0.25 * Vt + obs) / cell, atol=1e-12)
