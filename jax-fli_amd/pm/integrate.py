@@ -85,6 +85,8 @@ def _snapshot_vjp(solver, displacements, velocities, tg, cosmo, t0, param_names=
     target = interp.painting.target if interp.painting is not None else "particles"
     if target not in ("density", "particles", "spherical"):
         raise NotImplementedError(f"gradients through '{target}' snapshots are not implemented")
+    if drifting and target != "spherical":
+        raise NotImplementedError(f"gradients through drifted '{target}' snapshots are not implemented")
     mesh_size = displacements.mesh_size
     mode = _mode(displacements)
     base_state = interp.init()

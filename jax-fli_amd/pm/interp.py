@@ -178,8 +178,9 @@ class NoInterp(AbstractInterp):
 @dataclasses.dataclass(frozen=True)
 class DriftInterp(AbstractInterp):
     """No tiling; each particle is drifted from the snapshot time to the time its distance from the
-    observer corresponds to before painting (interp.py:212-320). On this engine the drift is applied
-    to spherical (HEALPix) targets; other targets raise."""
+    observer corresponds to before painting (interp.py:212-320): spherical shells drift inside the
+    HEALPix paint kernel; flat, density and particle targets take the drifted positions from
+    pmx_image_positions (particles are returned in Mpc/h, as the reference returns them)."""
 
     def init(self) -> InterpTilerState:
         if self.r_centers is not None and self.density_widths is not None and self.max_comoving_distance is not None:
@@ -201,10 +202,25 @@ class DriftInterp(AbstractInterp):
 
     def paint(self, state, t, y, cosmo):
         dx, p = y
-        if self.painting is None or self.painting.target != "spherical":
-            raise NotImplementedError("DriftInterp paints spherical shells on the pmx engine")
         r_center, width = self.shell(state)
-        out = paint_drifted(dx, p, t, cosmo, r_center, width, self.painting)
+        pt = self.painting
+        tgt = None if pt is None else pt.target
+        if tgt == "spherical":
+            out = paint_drifted(dx, p, t, cosmo, r_center, width, pt)
+            return out.replace(scale_factors=float(t), comoving_centers=r_center, density_width=width)
+        # the drifted particles as absolute grid positions (interp.py:277-289), then the target's paint
+        drift = drift_tables(cosmo, float(t), 2.0 * float(np.linalg.norm(dx.box_size)) + 100.0, dx.array.device)
+        g = engine.image_positions(dx.array.contiguous(), dx.mesh_size, dx.box_size, dx.observer_position_mpc,
+                                   mode=_paint_mode(dx.unit), vel=p.array.contiguous(), drift=drift)
+        drifted = dx.replace(array=g, unit=PositionUnit.GRID_ABSOLUTE)
+        if tgt == "flat":
+            out = drifted.paint_2d(center=r_center, density_plane_width=width, weights=pt.weights,
+                                   batch_size=pt.batch_size)
+        elif tgt == "density":
+            out = drifted.paint(weights=1.0 if pt.weights is None else pt.weights, chunk_size=pt.chunk_size,
+                                batch_size=pt.batch_size)
+        else:
+            out = drifted.to(PositionUnit.MPC_H)
         return out.replace(scale_factors=float(t), comoving_centers=r_center, density_width=width)
 
 

@@ -6,9 +6,10 @@ pmx_image_positions (one rotated, shifted and optionally drifted image of the pa
 grid positions) against sky_oracle.image_positions; pmx_paint_flat against sky_oracle.paint_flat in
 both position modes, on a particle sub-lattice, with per-particle weights and a flat grid finer than
 the mesh; and full N-body runs that save flat (with and without the drift) and HEALPix shells through
-TelephotoInterp, against the oracle's own run painting its snapshots the same way.
+TelephotoInterp, and through DriftInterp with flat, density and particle targets, against the oracle's
+own run painting its snapshots the same way.
 
-Tolerances: image positions 1e-6 of a grid unit without the drift and 1e-4 with it (float64 tables
+Tolerances: image positions 1e-5 grid units per 16 of coordinate size without the drift and 1e-4 with it (float64 tables
 with linear interpolation against the oracle's splines, times velocities of tens of grid units); a
 single flat paint 1e-6 relative L2 (float32 atomics); lightcone maps of an N-body run 1e-4 (float32
 particle arithmetic over the run against float64)."""
@@ -152,6 +153,62 @@ def test_nbody_telephoto_matches_oracle(pkg, cuda, target, drift):
     for s in range(len(t_snap)):
         assert np.abs(maps[s]).sum() > 0, s
         assert rel(got[s], maps[s]) < 1e-4, s
+
+
+@pytest.mark.parametrize("target", ["flat", "density", "particles"])
+def test_nbody_drift_interp_targets_match_oracle(pkg, cuda, target):
+    """DriftInterp (src/jax_fli/pm/interp.py:212-320) with flat, density and particle targets: every
+    particle drifted to its own lightcone time, then the target's paint (or the positions in Mpc/h),
+    against the oracle's run drifting and painting its own snapshots."""
+    from oracle import pm_oracle as orc
+
+    N, BOX, A0, DT0, FLAT = 16, 100.0, 0.1, 0.15, (16, 16)
+    KT = np.logspace(-4, 1, 128)
+    c = pkg.Cosmology(**P)
+    o = nbo.Cosmo(**P)
+    edges = np.array([10.0, 25.0, 45.0])
+    r_c = 0.5 * (edges[1:] + edges[:-1])[::-1]
+    widths = (edges[1:] - edges[:-1])[::-1]
+    ts = np.stack([pkg.cosmology.a_of_chi(c, r_c - widths / 2), pkg.cosmology.a_of_chi(c, r_c + widths / 2)])
+    interp = pkg.DriftInterp(painting=pkg.PaintingOptions(target=target))
+    noise = np.random.default_rng(53).standard_normal((N,) * 3).astype(np.float32)
+    ic = pkg.interpolate_initial_conditions(torch.from_numpy(noise).to(cuda), (N,) * 3, (BOX,) * 3, cosmo=c,
+                                            flatsky_npix=FLAT)
+    dx, p = pkg.lpt(c, ic, ts=A0)
+    solver = pkg.ReversibleDoubleKickDrift(interp_kernel=interp)
+    lc = pkg.nbody(c, dx, p, t0=A0, t1=1.0, dt0=DT0, ts=ts, solver=solver)
+    got = lc.array.reshape(len(r_c), -1).double().cpu().numpy()
+
+    t_snap = np.asarray(lc.scale_factors, dtype=np.float64).reshape(-1)
+    r_snap = np.asarray(lc.comoving_centers, dtype=np.float64).reshape(-1)
+    w_snap = np.asarray(lc.density_width, dtype=np.float64).reshape(-1)
+    order = np.argsort(t_snap)
+    ic0 = nbo.color_field(noise.astype(np.float64), (BOX,) * 3, KT, nbo.linear_matter_power(o, KT))
+    x0, v0 = nbo.lpt1(o, ic0, A0)
+    obs = (BOX / 2,) * 3
+    out = {}
+
+    def save(i, t, x, v):
+        s = order[i]
+        rc, wd = r_snap[s], w_snap[s]
+        g = so.image_positions(x, (N,) * 3, (BOX,) * 3, obs, vel=v, drift=so.Drift(o, float(t)))
+        if target == "flat":
+            cg, wg = rc * N / BOX, wd * N / BOX
+            out[s] = so.paint_flat(g.reshape((N,) * 3 + (3,)), (N,) * 3, FLAT, np.float32(cg - wg / 2),
+                                   np.float32(cg + wg / 2), 1.0 / ((BOX / FLAT[0]) * (BOX / FLAT[1]) * wd),
+                                   mode="absolute").reshape(-1)
+        elif target == "density":
+            out[s] = orc.cic_paint(g.reshape((N,) * 3 + (3,)), (N,) * 3, mode="absolute",
+                                   dtype=np.float64).reshape(-1)
+        else:
+            out[s] = (g * (BOX / N)).reshape(-1)
+
+    nbo.run_nbody(o, x0, v0, A0, 1.0, DT0, np.sort(t_snap).astype(np.float32), solver="kkd", on_save=save)
+    for s in range(len(t_snap)):
+        assert np.abs(out[s]).sum() > 0, s
+        assert rel(got[s], out[s]) < 1e-4, s
+    if target == "particles":
+        assert lc.unit == pkg.PositionUnit.MPC_H
 
 
 class _nullcontext:
