@@ -257,19 +257,33 @@ int pmx_paint_healpix_vjp(pmx_plan plan, const float* pos, int npx, int npy, int
 int pmx_force_mesh_vjp(pmx_plan plan, const float* fmesh_bar, float scale, unsigned flags, float* density_bar,
                        void* stream);
 
-/* Position cotangent of pmx_forces: pos_bar (+)= J(pos)^T forces_bar, J the Jacobian of
- * pm_forces(pos) * scale (readout-weight and paint-weight terms); dot (optional) += forces_bar . F.
- * Recomputes the force mesh at pos. Replaces jax.vjp of _forces (src/jax_fli/pm/solvers.py:19-32). */
+/* Position cotangent of pmx_forces: pos_bar (+)= J(pos)^T (bar_scale forces_bar), J the Jacobian of
+ * pm_forces(pos) * scale (readout-weight and paint-weight terms); dot (optional) += forces_bar . F
+ * (forces_bar unscaled). Recomputes the force mesh at pos. Replaces jax.vjp of _forces
+ * (src/jax_fli/pm/solvers.py:19-32); bar_scale folds a kick coefficient into the pull-back. */
 int pmx_forces_vjp(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, float scale, unsigned flags,
-                   const float* forces_bar, float* pos_bar, int accumulate, double* dot, void* stream);
+                   const float* forces_bar, float bar_scale, float* pos_bar, int accumulate, double* dot,
+                   void* stream);
 
 /* pmx_forces_vjp with the force mesh at pos given (force_mesh, nx*ny*nz*3 floats, as kept by
  * pmx_step_keep for the step that kicked with these positions) instead of recomputed: the sort of pos,
  * the adjoint deposit, the potential pass and the gather only. For the checkpointed adjoint
  * (integrate.py:147-193 with adjoint="checkpointed"). */
 int pmx_forces_vjp_saved(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, float scale,
-                         unsigned flags, const float* force_mesh, const float* forces_bar, float* pos_bar,
-                         int accumulate, double* dot, void* stream);
+                         unsigned flags, const float* force_mesh, const float* forces_bar, float bar_scale,
+                         float* pos_bar, int accumulate, double* dot, void* stream);
+
+/* out = x + c v over n floats (out may alias x): the drift x_a = x + c_pre v that the checkpointed
+ * adjoint recomputes from a kept state before pulling a kick back (the solvers' drift,
+ * src/jax_fli/pm/solvers.py:165-172, 305-312). */
+int pmx_axpy(float* out, const float* x, const float* v, float c, size_t n, void* stream);
+
+/* Transpose of a drift x <- x + c v for the cotangents (xbar, vbar) of its output: vbar += c xbar,
+ * and dot (float64 device scalar) += <xbar, v>, the cotangent of c (per-workgroup partials summed in a
+ * fixed order). The per-op drift pull-back of the checkpointed adjoint (jax.vjp of the solvers'
+ * drifts, src/jax_fli/pm/integrate.py:147-193 with adjoint="checkpointed"). */
+int pmx_adjoint_drift(pmx_plan plan, const float* xbar, float* vbar, const float* v, size_t n, float c, double* dot,
+                      void* stream);
 
 /* One reverse-adjoint step of the reversible KKD solver, in place. The forward step was
  *   v <- v + F(x) * c_kick ;  x <- x + v * c_drift      (F = pm_forces(x) * force_scale)

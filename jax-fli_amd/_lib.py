@@ -24,6 +24,7 @@ _f = _c.c_float
 _d = _c.c_double
 _u = _c.c_uint
 _dp = _c.POINTER(_c.c_double)
+_sz = _c.c_size_t
 SIGNATURES = {
     "pmx_abi_version": (_i, []),
     "pmx_last_error": (_c.c_char_p, []),
@@ -55,7 +56,7 @@ SIGNATURES = {
     "pmx_paint_cic_vjp": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _vp, _vp, _vp, _i, _vp]),
     "pmx_readout_cic_vjp": (_i, [_vp, _vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp]),
     "pmx_force_mesh_vjp": (_i, [_vp, _vp, _f, _u, _vp, _vp]),
-    "pmx_forces_vjp": (_i, [_vp, _vp, _i, _i, _i, _i, _f, _u, _vp, _vp, _i, _vp, _vp]),
+    "pmx_forces_vjp": (_i, [_vp, _vp, _i, _i, _i, _i, _f, _u, _vp, _f, _vp, _i, _vp, _vp]),
     "pmx_step_reverse_vjp": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _u, _vp, _vp, _vp, _vp]),
     "pmx_step_reverse_vjp_saved": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _u, _vp, _vp, _vp, _vp, _vp]),
     "pmx_step_reverse_vjp_sharpen": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _u, _f, _f, _f, _f, _vp, _vp, _vp,
@@ -67,7 +68,9 @@ SIGNATURES = {
     "pmx_paint_healpix_vjp": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _dp, _dp, _d, _d, _i, _i, _i, _vp, _vp, _vp,
                                    _vp, _i, _vp]),
     "pmx_born": (_i, [_vp, _vp, _i, _c.c_int64, _vp, _i, _vp, _i, _vp]),
-    "pmx_forces_vjp_saved": (_i, [_vp, _vp, _i, _i, _i, _i, _f, _u, _vp, _vp, _vp, _i, _vp, _vp]),
+    "pmx_forces_vjp_saved": (_i, [_vp, _vp, _i, _i, _i, _i, _f, _u, _vp, _vp, _f, _vp, _i, _vp, _vp]),
+    "pmx_axpy": (_i, [_vp, _vp, _vp, _f, _sz, _vp]),
+    "pmx_adjoint_drift": (_i, [_vp, _vp, _vp, _vp, _sz, _f, _vp, _vp]),
     "pmx_pgd_vjp": (_i, [_vp, _vp, _i, _i, _i, _i, _f, _f, _f, _vp, _vp, _i, _vp, _vp]),
     "pmx_pgd": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _i, _f, _vp]),
     "pmx_paint_healpix_lightcone": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _vp, _f, _dp, _dp, _d, _d, _i, _i, _i, _vp,

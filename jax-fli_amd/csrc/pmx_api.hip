@@ -504,10 +504,12 @@ int pmx_pgd(pmx_plan p, float* x, float* v, int npx, int npy, int npz, int mode,
 }
 
 int pmx_forces_vjp(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, float scale, unsigned flags,
-                   const float* forces_bar, float* pos_bar, int accumulate, double* dot, void* stream) {
+                   const float* forces_bar, float bar_scale, float* pos_bar, int accumulate, double* dot,
+                   void* stream) {
   PMX_TRY(check_plan(p));
   PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
   if (!forces_bar || !pos_bar) return pmx_set_error(1, "null cotangent");
+  if (!std::isfinite(bar_scale)) return pmx_set_error(1, "bar_scale must be finite");
   hipStream_t s = (hipStream_t)stream;
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
   const size_t np = (size_t)npx * npy * npz;
@@ -515,21 +517,38 @@ int pmx_forces_vjp(pmx_plan p, const float* pos, int npx, int npy, int npz, int 
   PMX_TRY(pmx_paint_deposit(p, a, nullptr, 1, 1.f, 1.f, 0, p->rho, 1, 0, s));
   PMX_TRY(pot_force_mesh_from(p, p->rho, scale, flags, p->fmesh, s));
   PMX_TRY(pmx_launch_absmax(forces_bar, 3 * np, p->ps_misc + 3, s));
-  return force_adjoint(p, a, forces_bar, 1.f, scale, flags, p->fmesh, pos_bar, accumulate, dot, nullptr, 0.f, s);
+  return force_adjoint(p, a, forces_bar, bar_scale, scale, flags, p->fmesh, pos_bar, accumulate, dot, nullptr, 0.f,
+                       s);
 }
 
 int pmx_forces_vjp_saved(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, float scale,
-                         unsigned flags, const float* force_mesh, const float* forces_bar, float* pos_bar,
-                         int accumulate, double* dot, void* stream) {
+                         unsigned flags, const float* force_mesh, const float* forces_bar, float bar_scale,
+                         float* pos_bar, int accumulate, double* dot, void* stream) {
   PMX_TRY(check_plan(p));
   PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
   if (!forces_bar || !pos_bar || !force_mesh) return pmx_set_error(1, "null buffer");
+  if (!std::isfinite(bar_scale)) return pmx_set_error(1, "bar_scale must be finite");
   hipStream_t s = (hipStream_t)stream;
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
   const size_t np = (size_t)npx * npy * npz;
   PMX_TRY(pmx_paint_sort(p, a, 0.f, nullptr, 1, 1.f, s));  // the adjoint's deposit reuses the sort
   PMX_TRY(pmx_launch_absmax(forces_bar, 3 * np, p->ps_misc + 3, s));
-  return force_adjoint(p, a, forces_bar, 1.f, scale, flags, force_mesh, pos_bar, accumulate, dot, nullptr, 0.f, s);
+  return force_adjoint(p, a, forces_bar, bar_scale, scale, flags, force_mesh, pos_bar, accumulate, dot, nullptr, 0.f,
+                       s);
+}
+
+int pmx_axpy(float* out, const float* x, const float* v, float c, size_t n, void* stream) {
+  if (!out || !x || !v) return pmx_set_error(1, "null buffer");
+  if (n == 0) return 0;
+  return pmx_launch_axpy(out, x, v, c, n, (hipStream_t)stream);
+}
+
+int pmx_adjoint_drift(pmx_plan p, const float* xbar, float* vbar, const float* v, size_t n, float c, double* dot,
+                      void* stream) {
+  PMX_TRY(check_plan(p));
+  if (!xbar || !vbar || !v || !dot) return pmx_set_error(1, "null buffer");
+  if (n == 0) return 0;
+  return pmx_launch_adj_prep(p, xbar, vbar, v, n, c, dot, nullptr, (hipStream_t)stream);
 }
 
 int pmx_pgd_vjp(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, float alpha, float kl, float ks,

@@ -148,6 +148,7 @@ int pmx_launch_vjp_gather(pmx_plan_t* p, const PartArgs& a, int ncomp, const flo
                           const float* rhob, const float* w, float w0, float* xbar, int accumulate, float* wbar,
                           double* dot, float* vkick, float ckick, hipStream_t s);
 
+int pmx_launch_axpy(float* out, const float* x, const float* v, float c, size_t n, hipStream_t s);
 int pmx_launch_adj_prep(pmx_plan_t* p, const float* xbar, float* vbar, const float* v, size_t n, float cd,
                         double* dot, unsigned* bits, hipStream_t s);
 int pmx_launch_fill_d(double* a, size_t n, hipStream_t s);

@@ -869,7 +869,8 @@ __global__ void __launch_bounds__(256) k_vjp_gather(PartArgs a, const float* __r
 }
 
 // Reverse-adjoint preparation of a KKD step (v += F ck; x += v cd) before the state is reversed:
-// dot += <xbar, v>, vbar += cd * xbar, *bits = max |vbar| (for the deposit scale).
+// dot += <xbar, v>, vbar += cd * xbar, *bits = max |vbar| (for the deposit scale; skipped when bits
+// is null). Also the transpose of any drift x += v cd on its own.
 __global__ void __launch_bounds__(256) k_adj_prep(const float* __restrict__ xbar, float* __restrict__ vbar,
                                                   const float* __restrict__ v, size_t n, float cd,
                                                   double* __restrict__ part, unsigned* __restrict__ bits) {
@@ -882,8 +883,15 @@ __global__ void __launch_bounds__(256) k_adj_prep(const float* __restrict__ xbar
     vbar[i] = vb;
     m = fmaxf(m, fabsf(vb));
   }
-  block_max_to(m, bits);
+  if (bits) block_max_to(m, bits);
   block_sum_store(local, part);
+}
+
+// out <- x + v*c (out may alias x), as the fused step's drift computes it
+__global__ void __launch_bounds__(256) k_axpy(float* out, const float* x, const float* __restrict__ v, size_t n,
+                                              float c) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = x[i] + v[i] * c;
 }
 
 // x <- x + v*c
@@ -1527,6 +1535,11 @@ int pmx_launch_vjp_gather(pmx_plan_t* p, const PartArgs& a, int ncomp, const flo
   if (rc || !dot || !u) return rc;
   hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, s, p->red, (int)G.x, dot);
   return pmx_check_launch("finish_sum");
+}
+int pmx_launch_axpy(float* out, const float* x, const float* v, float c, size_t n, hipStream_t s) {
+  const size_t g = grid_for(n, 256);
+  hipLaunchKernelGGL(k_axpy, dim3((unsigned)(g < 8192 ? g : 8192)), dim3(256), 0, s, out, x, v, n, c);
+  return pmx_check_launch("axpy");
 }
 int pmx_launch_adj_prep(pmx_plan_t* p, const float* xbar, float* vbar, const float* v, size_t n, float cd, double* dot,
                         unsigned* bits, hipStream_t s) {

@@ -545,9 +545,9 @@ def force_mesh_vjp(fmesh_bar: torch.Tensor, scale=1.0, deconv=False) -> torch.Te
 
 def forces_vjp(pos: torch.Tensor, mesh_shape, forces_bar: torch.Tensor, mode="relative", scale=1.0, deconv=False,
                pos_bar: torch.Tensor | None = None, accumulate=False, dot: torch.Tensor | None = None,
-               force_mesh: torch.Tensor | None = None) -> torch.Tensor:
-    """pos_bar (+)= J(pos)^T forces_bar for forces(pos, scale); dot (float64 device scalar) += forces_bar . F.
-    force_mesh: the force mesh at pos, kept by step(..., force_mesh=) (not recomputed)."""
+               force_mesh: torch.Tensor | None = None, bar_scale=1.0) -> torch.Tensor:
+    """pos_bar (+)= J(pos)^T (bar_scale forces_bar) for forces(pos, scale); dot (float64 device scalar) +=
+    forces_bar . F. force_mesh: the force mesh at pos, kept by step(..., force_mesh=) (not recomputed)."""
     pos = _f32(pos, "pos")
     forces_bar = _f32(forces_bar, "forces_bar")
     npx, npy, npz = _pgrid(pos)
@@ -559,13 +559,43 @@ def forces_vjp(pos: torch.Tensor, mesh_shape, forces_bar: torch.Tensor, mode="re
         _force_mesh_arg(force_mesh, mesh_shape)
         check(load().pmx_forces_vjp_saved(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], float(scale),
                                           _lib.DECONV if deconv else 0, ptr(force_mesh), ptr(forces_bar),
-                                          ptr(pos_bar), int(accumulate), ptr(dot), stream_ptr()),
+                                          float(bar_scale), ptr(pos_bar), int(accumulate), ptr(dot), stream_ptr()),
               "pmx_forces_vjp_saved")
         return pos_bar
     check(load().pmx_forces_vjp(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], float(scale),
-                                _lib.DECONV if deconv else 0, ptr(forces_bar), ptr(pos_bar), int(accumulate),
-                                ptr(dot), stream_ptr()), "pmx_forces_vjp")
+                                _lib.DECONV if deconv else 0, ptr(forces_bar), float(bar_scale), ptr(pos_bar),
+                                int(accumulate), ptr(dot), stream_ptr()), "pmx_forces_vjp")
     return pos_bar
+
+
+def axpy(x: torch.Tensor, v: torch.Tensor, c: float, out: torch.Tensor | None = None) -> torch.Tensor:
+    """out = x + c v (float32, same shapes; out may be x)."""
+    x = _f32(x, "x")
+    v = _f32(v, "v")
+    if v.shape != x.shape:
+        raise ValueError("x and v must have the same shape")
+    if out is None:
+        out = torch.empty_like(x)
+    check(load().pmx_axpy(ptr(out), ptr(x), ptr(v), float(c), x.numel(), stream_ptr()), "pmx_axpy")
+    return out
+
+
+def adjoint_drift(xbar: torch.Tensor, vbar: torch.Tensor, v: torch.Tensor, c: float, dot: torch.Tensor,
+                  mesh_shape) -> None:
+    """Transpose of the drift x <- x + c v: vbar += c xbar (in place) and dot (float64 device, added to)
+    += <xbar, v>. mesh_shape names the plan whose reduction buffer the dot uses."""
+    if not vbar.is_contiguous():
+        raise ValueError("vbar is updated in place and must be contiguous")
+    xbar = _f32(xbar, "xbar")
+    vbar = _f32(vbar, "vbar")
+    v = _f32(v, "v")
+    if not (xbar.shape == vbar.shape == v.shape):
+        raise ValueError("xbar, vbar and v must have the same shape")
+    if dot.dtype != torch.float64 or dot.numel() < 1 or not dot.is_contiguous():
+        raise ValueError("dot must be a contiguous float64 device tensor")
+    plan = get_plan(mesh_shape, xbar.device)
+    check(load().pmx_adjoint_drift(plan.handle, ptr(xbar), ptr(vbar), ptr(v), xbar.numel(), float(c), ptr(dot),
+                                   stream_ptr()), "pmx_adjoint_drift")
 
 
 def pgd_vjp(pos: torch.Tensor, mesh_shape, disp_bar: torch.Tensor, mode="relative", alpha=0.2, kl=0.3, ks=1.0,

@@ -219,35 +219,29 @@ def _op_vjp(op: StepOp, x, v, v_after, xb, vb, fs, mesh_size, mode, force_mesh=N
     velocity it produced) and return its coefficient cotangents [dL/dpre, dL/dkick, dL/dpost,
     dL/dv_scale] (float64 device; v_scale: the sharpening boost's factor). The op is x_a = x + pre v; [x_b = x_a + S(x_a) | v += v_scale S(x_a)];
     v_c = v + kick F(x_b); x_d = x_b + post v_c. force_mesh: the kick's force mesh kept by the forward."""
-    zero = torch.zeros((), dtype=torch.float64, device=x.device)
-    dpre = dkick = dpost = dvs = zero
-    x_a = x if op.pre is None else x + v * f32(op.pre)
+    dots = torch.zeros(4, dtype=torch.float64, device=x.device)  # [pre, kick, post, v_scale]
+    x_a = x if op.pre is None else engine.axpy(x, v, f32(op.pre))
     x_b = x_a
     corr = op.corr
     if corr is not None and corr.target == "x":
         x_b = x_a.clone()
         corr.run(x_b, None, mesh_size, mode)
-    if op.post is not None:
-        dpost = (xb * v_after).sum(dtype=torch.float64)
-        vb.add_(xb * f32(op.post))
-    if op.kick is not None:
-        dot = torch.zeros(1, dtype=torch.float64, device=x.device)
-        g = engine.forces_vjp(x_b, mesh_size, vb, mode=mode, scale=fs, dot=dot, force_mesh=force_mesh)
-        xb.add_(g * f32(op.kick))
-        dkick = dot[0]
+    if op.post is not None:  # dots[2] += <xb, v_after>; vb += post xb
+        engine.adjoint_drift(xb, vb, v_after, f32(op.post), dots[2:3], mesh_size)
+    if op.kick is not None:  # xb += kick J^T vb; dots[1] += <vb, F(x_b)>
+        engine.forces_vjp(x_b, mesh_size, vb, mode=mode, scale=fs, dot=dots[1:2], force_mesh=force_mesh,
+                          pos_bar=xb, accumulate=True, bar_scale=f32(op.kick))
     if corr is not None:
         u = xb if corr.target == "x" else vb
-        dot = torch.zeros(1, dtype=torch.float64, device=x.device)
-        g = engine.pgd_vjp(x_a, mesh_size, u, mode=mode, alpha=corr.alpha, kl=corr.kl, ks=corr.ks, dot=dot)
+        g = engine.pgd_vjp(x_a, mesh_size, u, mode=mode, alpha=corr.alpha, kl=corr.kl, ks=corr.ks,
+                           dot=None if corr.target == "x" else dots[3:4])  # <v_bar, S(x_a)>
         if corr.target == "x":
             xb.add_(g)
         else:
             xb.add_(g * f32(corr.v_scale))
-            dvs = dot[0]  # <v_bar, S(x_a)>
-    if op.pre is not None:
-        dpre = (xb * v).sum(dtype=torch.float64)
-        vb.add_(xb * f32(op.pre))
-    return torch.stack([dpre, dkick, dpost, dvs])
+    if op.pre is not None:  # dots[0] += <xb, v>; vb += pre xb
+        engine.adjoint_drift(xb, vb, v, f32(op.pre), dots[0:1], mesh_size)
+    return dots
 
 
 def _integrate_checkpointed(displacements, velocities, cosmo_in, ts, solver, t0, t1, dt0, checkpoints):

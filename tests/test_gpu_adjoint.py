@@ -115,6 +115,46 @@ def test_forces_vjp_directional(pkg, cuda, mode):
     assert _rel(float(dot.cpu()), ref_dot) < 1e-5
 
 
+@pytest.mark.parametrize("saved", [False, True])
+def test_forces_vjp_bar_scale_and_accumulate(pkg, cuda, saved):
+    """bar_scale folds a kick coefficient into the pull-back (pos_bar += c J^T u) and leaves the dot
+    <u, F> unscaled; accumulate adds to pos_bar."""
+    n = 32
+    mesh = (n, n, n)
+    pos = torch.from_numpy(_disp(mesh, 1.5, 38)).to(cuda)
+    u = torch.from_numpy(np.random.default_rng(39).standard_normal(tuple(pos.shape)).astype(np.float32)).to(cuda)
+    fm = None
+    if saved:
+        fm = torch.empty(mesh + (3,), dtype=torch.float32, device=cuda)
+        xs, vs = pos.clone(), torch.zeros_like(pos)
+        pkg.engine.step(xs, vs, mesh, mode="relative", force_scale=0.45, c_kick=1.0, force_mesh=fm)
+    d1 = torch.zeros(1, dtype=torch.float64, device=cuda)
+    d2 = torch.zeros(1, dtype=torch.float64, device=cuda)
+    g1 = pkg.engine.forces_vjp(pos, mesh, u, scale=0.45, dot=d1, force_mesh=fm)
+    base = torch.from_numpy(np.random.default_rng(40).standard_normal(tuple(pos.shape)).astype(np.float32)).to(cuda)
+    acc = base.clone()
+    pkg.engine.forces_vjp(pos, mesh, u, scale=0.45, dot=d2, force_mesh=fm, pos_bar=acc, accumulate=True,
+                          bar_scale=0.37)
+    expect = base.double() + 0.37 * g1.double()
+    assert _rel_l2(acc.cpu().numpy(), expect.cpu().numpy()) < 1e-5
+    assert _rel(float(d2.cpu()), float(d1.cpu())) < 1e-6
+
+
+def test_axpy_and_adjoint_drift(pkg, cuda):
+    rng = np.random.default_rng(44)
+    x, v, xb, vb = (rng.standard_normal((16, 16, 16, 3)).astype(np.float32) for _ in range(4))
+    out = pkg.engine.axpy(torch.from_numpy(x).to(cuda), torch.from_numpy(v).to(cuda), 0.3).cpu().numpy()
+    ref = x.astype(np.float64) + 0.3 * v.astype(np.float64)
+    assert np.abs(out - ref).max() <= 2e-7 * np.abs(ref).max()
+    vbt = torch.from_numpy(vb).to(cuda)
+    dot = torch.full((1,), 1.5, dtype=torch.float64, device=cuda)
+    pkg.engine.adjoint_drift(torch.from_numpy(xb).to(cuda), vbt, torch.from_numpy(v).to(cuda), -0.7, dot, (16, 16, 16))
+    ref_vb = vb.astype(np.float64) - 0.7 * xb.astype(np.float64)
+    assert np.abs(vbt.cpu().numpy() - ref_vb).max() <= 2e-7 * np.abs(ref_vb).max()
+    ref_dot = 1.5 + float((xb.astype(np.float64) * v.astype(np.float64)).sum())
+    assert _rel(float(dot.cpu()), ref_dot) < 1e-12
+
+
 @pytest.mark.parametrize("npart", [32, 16])
 def test_step_reverse_vjp(pkg, cuda, npart):
     """One KKD step (v += F ck; x += v cd): the reverse restores the state and the cotangents match
