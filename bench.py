@@ -380,6 +380,8 @@ def main():
     ap.add_argument("--no-vjp", action="store_true", help="skip the forward+VJP measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-large", action="store_true", help="skip the 512^3-on-1024^3 forward + VJP measurement")
+    ap.add_argument("--large-only", action="store_true",
+                    help="run only the 512^3-on-1024^3 forward + VJP timing (no differences) and print its JSON")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the per-phase HBM traffic from two rocprofv3 --pmc runs of this script and exit")
     args = ap.parse_args()
@@ -399,6 +401,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.large_only:
+        print(json.dumps({"large": bench_large(pkg, dev, fd_dirs=0)}))
+        return
     cosmo = pkg.Cosmology(Omega_c=0.25, Omega_b=0.0486, h=0.6774, n_s=0.9667, sigma8=0.8)
     mesh = (N, N, N)
 

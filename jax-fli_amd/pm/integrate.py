@@ -310,12 +310,15 @@ def _integrate_checkpointed(displacements, velocities, cosmo_in, ts, solver, t0,
             k0 = starts[si]
             k1 = starts[si + 1] if si + 1 < len(starts) else len(items)
             after = ckpts[k1] if k1 < len(items) else final
-            x, v = (t.clone() for t in ckpts[k0])
-            states = []
-            for k in range(k0, k1):  # recompute the segment's states (none when every op is kept)
-                states.append((x.clone(), v.clone()) if k + 1 < k1 else (x, v))
-                if items[k][0] == "op" and k + 1 < k1:
-                    run_op(items[k][1], x, v, mesh_size, mode, cosmo)
+            if k1 == k0 + 1:  # a kept state: the pull-backs only read it
+                states = [ckpts[k0]]
+            else:  # recompute the segment's states from its checkpoint
+                x, v = (t.clone() for t in ckpts[k0])
+                states = []
+                for k in range(k0, k1):
+                    states.append((x.clone(), v.clone()) if k + 1 < k1 else (x, v))
+                    if items[k][0] == "op" and k + 1 < k1:
+                        run_op(items[k][1], x, v, mesh_size, mode, cosmo)
             for k in reversed(range(k0, k1)):
                 xs, vs = states[k - k0]
                 if items[k][0] == "save":
