@@ -818,7 +818,7 @@ int pmx_paint_healpix_tiled(pmx_plan p, const float* pos, int npx, int npy, int 
   PMX_TRY(check_shell(p, pos, npx, npy, npz, mode, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
                       tile_shift));
   if (!map) return pmx_set_error(1, "null map");
-  PMX_TRY(ensure_acc(p, (size_t)12 * nside * nside));
+  PMX_TRY(ensure_acc(p, (size_t)12 * nside * nside + 4 * (size_t)nside));  // maps + ring table
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
   return pmx_launch_healpix(p, a, weights, weight, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
                             tile_shift, map, accumulate, (hipStream_t)stream);
@@ -835,7 +835,7 @@ int pmx_paint_healpix_lightcone(pmx_plan p, const float* pos, const float* vel, 
   if (!map || !vel || !drift_tables) return pmx_set_error(1, "null argument");
   if (ntab < 2 || !(chi_max > 0.0) || !(a_hi > a_lo) || !(a_now > 0.0) || !(drift_bound >= 0.0))
     return pmx_set_error(1, "bad drift tables: need ntab >= 2, chi_max > 0, a_hi > a_lo, a_now > 0");
-  PMX_TRY(ensure_acc(p, (size_t)12 * nside * nside));
+  PMX_TRY(ensure_acc(p, (size_t)12 * nside * nside + 4 * (size_t)nside));  // maps + ring table
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
   const PmxDrift d{vel, drift_tables, ntab, chi_max, a_lo, a_hi, a_now, gp_now, drift_bound};
   return pmx_launch_healpix(p, a, weights, weight, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
@@ -849,6 +849,7 @@ int pmx_paint_healpix_vjp(pmx_plan p, const float* pos, int npx, int npy, int np
   PMX_TRY(check_shell(p, pos, npx, npy, npz, mode, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
                       tile_shift));
   if (!map_bar || !pos_bar) return pmx_set_error(1, "null cotangent");
+  PMX_TRY(ensure_acc(p, (size_t)12 * nside * nside + 4 * (size_t)nside));  // the ring table after the maps
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
   return pmx_launch_healpix_vjp(p, a, weights, weight, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
                                 tile_shift, map_bar, pos_bar, accumulate, (hipStream_t)stream);
@@ -871,6 +872,7 @@ int pmx_paint_healpix_lightcone_vjp(pmx_plan p, const float* pos, const float* v
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
   if (nparam < 0 || nparam > 4 || (nparam > 0 && (!param_tables || !param_dots)))
     return pmx_set_error(1, "need 0 <= nparam <= 4 with parameter tables and dots");
+  PMX_TRY(ensure_acc(p, (size_t)12 * nside * nside + 4 * (size_t)nside));  // the ring table after the maps
   PmxDrift d{vel, drift_tables, ntab, chi_max, a_lo, a_hi, a_now, gp_now, drift_bound};
   d.ptab = param_tables;
   d.nparam = nparam;

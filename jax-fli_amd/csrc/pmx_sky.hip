@@ -25,6 +25,7 @@ struct HpGeom {
   int nside;
   long npix, ncap;
   double fact1, fact2;
+  const double* rth;  // colatitude of each ring 1 .. 4 nside - 1 (index = ring), built by k_ring_theta
 };
 
 __device__ __forceinline__ long hp_ang2pix(const HpGeom& g, double z, double phi) {
@@ -60,28 +61,38 @@ __device__ __forceinline__ long hp_ring_above(const HpGeom& g, double z) {
   return (z > 0) ? iring : 4 * (long)g.nside - iring - 1;
 }
 
-__device__ __forceinline__ void hp_ring_info(const HpGeom& g, long ring, long& startpix, long& ringpix, double& theta,
-                                             bool& shifted) {
+// colatitude of ring `ring` (1 .. 4 nside - 1)
+__device__ __forceinline__ double hp_ring_theta(const HpGeom& g, long ring) {
   const long ns = g.nside;
   const long northring = (ring > 2 * ns) ? 4 * ns - ring : ring;
+  double theta;
   if (northring < ns) {
     const double tmp = (double)northring * northring * g.fact2;
     const double costheta = 1.0 - tmp;
     const double sintheta = sqrt(tmp * (2.0 - tmp));
     theta = atan2(sintheta, costheta);
+  } else {
+    theta = acos((2 * ns - northring) * g.fact1);
+  }
+  return northring != ring ? M_PI - theta : theta;
+}
+
+// first pixel, pixel count and shift of ring `ring`; its colatitude from the table g.rth
+__device__ __forceinline__ void hp_ring_info(const HpGeom& g, long ring, long& startpix, long& ringpix, double& theta,
+                                             bool& shifted) {
+  const long ns = g.nside;
+  const long northring = (ring > 2 * ns) ? 4 * ns - ring : ring;
+  if (northring < ns) {
     ringpix = 4 * northring;
     shifted = true;
     startpix = 2 * northring * (northring - 1);
   } else {
-    theta = acos((2 * ns - northring) * g.fact1);
     ringpix = 4 * ns;
     shifted = ((northring - ns) & 1) == 0;
     startpix = g.ncap + (northring - ns) * ringpix;
   }
-  if (northring != ring) {
-    theta = M_PI - theta;
-    startpix = g.npix - startpix - ringpix;
-  }
+  if (northring != ring) startpix = g.npix - startpix - ringpix;
+  theta = g.rth[ring];
 }
 
 // pixels on one ring around phi and the phi weight of the second one; dphi = ring pixel width
@@ -104,9 +115,9 @@ __device__ __forceinline__ void hp_ring_pair(const HpGeom& g, long ring, double 
 // Bilinear weights of the four RING pixels around (theta, phi); DER also gives d wgt / d theta and
 // d wgt / d phi (the integer ring/pixel choices are locally constant).
 template <bool DER>
-__device__ __forceinline__ void hp_interpol(const HpGeom& g, double theta, double phi, long pix[4], double wgt[4],
-                                            double dth[4], double dph[4]) {
-  const double z = cos(theta);
+__device__ __forceinline__ void hp_interpol(const HpGeom& g, double theta, double z, double phi, long pix[4],
+                                            double wgt[4], double dth[4], double dph[4]) {
+  // z = cos(theta), passed in as the caller's z / r
   const long ir1 = hp_ring_above(g, z);
   const long ir2 = ir1 + 1;
   const long nring = 4 * (long)g.nside;
@@ -236,7 +247,7 @@ struct ShellArgs {
   double* pdots;       // adjoint: += sum over particle images of (W . G) df/dtheta(|Y|) weight
 };
 
-constexpr int HP_HASH = 4096, HP_PROBES = 32, HP_SB = 16, HP_PER_THREAD = HP_SB * HP_SB * HP_SB / 256;
+constexpr int HP_HASH = 4096, HP_PROBES = 32, HP_PROBES_FULL = 4, HP_SB = 16, HP_PER_THREAD = HP_SB * HP_SB * HP_SB / 256;
 constexpr int HP_MAX_TILES = 4096;
 constexpr unsigned HP_EMPTY = 0xFFFFFFFFu;
 
@@ -417,12 +428,15 @@ __device__ int shell_images(const PartArgs& a, const ShellArgs& sa, unsigned b, 
 
 // Per-workgroup aggregation: the (pixel, weight) deposits of the sub-block go through an LDS hash
 // table, and each distinct pixel then reaches the map with one float64 atomic. A deposit that
-// finds no free slot within HP_PROBES probes goes straight to the map.
+// finds no free slot within HP_PROBES probes goes straight to the map and marks the table full;
+// after that a deposit probes only HP_PROBES_FULL slots (a sub-block near the observer spans more
+// pixels than the table holds, and long probe runs would cost more than the atomics they save).
 __device__ __forceinline__ void hp_table_add(volatile unsigned* keys, double* vals, unsigned pix, double v,
-                                             double* __restrict__ acc) {
+                                             double* __restrict__ acc, volatile int* full) {
   unsigned h = (pix * 2654435761u) >> 20;  // 12-bit multiplicative hash
+  const int probes = *full ? HP_PROBES_FULL : HP_PROBES;
 #pragma unroll 1
-  for (int t = 0; t < HP_PROBES; ++t) {
+  for (int t = 0; t < probes; ++t) {
     const unsigned k = keys[h];
     if (k == pix) {
       atomicAdd(&vals[h], v);
@@ -437,6 +451,7 @@ __device__ __forceinline__ void hp_table_add(volatile unsigned* keys, double* va
     }
     h = (h + 1) & (HP_HASH - 1);
   }
+  *full = 1;
   unsafeAtomicAdd(acc + pix, v);
 }
 
@@ -456,10 +471,12 @@ __global__ void __launch_bounds__(256) k_paint_healpix(PartArgs a, const float* 
   __shared__ unsigned keys[HP_HASH];
   __shared__ double vals[HP_HASH];
   __shared__ unsigned short hits[HP_MAX_TILES];
+  __shared__ int full;
   for (int i = threadIdx.x; i < HP_HASH; i += blockDim.x) {
     keys[i] = HP_EMPTY;
     vals[i] = 0.0;
   }
+  if (threadIdx.x == 0) full = 0;
   const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
   const int nhit = shell_images(a, sa, b, hits);  // synchronises the workgroup
   for (int r = 0; r < HP_PER_THREAD; ++r) {
@@ -486,14 +503,14 @@ __global__ void __launch_bounds__(256) k_paint_healpix(PartArgs a, const float* 
       if (!in_shell(sa, Xt, rr)) continue;
       const double phi = phi_of(Xt);
       if (sa.scheme == 0) {
-        hp_table_add(keys, vals, (unsigned)hp_ang2pix(g, Xt[2] / rr, phi), wt, acc);
+        hp_table_add(keys, vals, (unsigned)hp_ang2pix(g, Xt[2] / rr, phi), wt, acc, &full);
       } else {
         const double theta = atan2(sqrt(Xt[0] * Xt[0] + Xt[1] * Xt[1]), Xt[2]);
         long pix[4];
         double wg[4], d0[4], d1[4];
-        hp_interpol<false>(g, theta, phi, pix, wg, d0, d1);
+        hp_interpol<false>(g, theta, Xt[2] / rr, phi, pix, wg, d0, d1);
 #pragma unroll
-        for (int m = 0; m < 4; ++m) hp_table_add(keys, vals, (unsigned)pix[m], wt * wg[m], acc);
+        for (int m = 0; m < 4; ++m) hp_table_add(keys, vals, (unsigned)pix[m], wt * wg[m], acc, &full);
       }
     }
   }
@@ -567,7 +584,7 @@ __global__ void __launch_bounds__(256) k_healpix_vjp(PartArgs a, const float* __
         const double theta = atan2(rho, Xt[2]);
         long pix[4];
         double wg[4], dth[4], dph[4];
-        hp_interpol<true>(g, theta, phi, pix, wg, dth, dph);
+        hp_interpol<true>(g, theta, Xt[2] / rr, phi, pix, wg, dth, dph);
         double gt = 0.0, gp = 0.0;
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
@@ -942,6 +959,25 @@ HpGeom hp_geom(int nside) {
   g.ncap = 2L * nside * (nside - 1);
   g.fact2 = 4.0 / (double)g.npix;
   g.fact1 = (double)(2 * nside) * g.fact2;
+  g.rth = nullptr;
+  return g;
+}
+
+namespace {
+__global__ void k_ring_theta(HpGeom g, double* __restrict__ out) {
+  const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (r < 4L * g.nside) out[r] = r == 0 ? 0.0 : hp_ring_theta(g, r);
+}
+}  // namespace
+
+// HEALPix geometry with its ring colatitude table, built on `s` after the npix map accumulators in
+// p->acc (the caller sized p->acc for npix + 4 nside doubles)
+static HpGeom hp_geom_table(pmx_plan_t* p, int nside, hipStream_t s, int& rc) {
+  HpGeom g = hp_geom(nside);
+  double* tab = p->acc + g.npix;
+  hipLaunchKernelGGL(k_ring_theta, dim3((unsigned)((4L * nside + 255) / 256)), dim3(256), 0, s, g, tab);
+  rc = pmx_check_launch("ring_theta");
+  g.rth = tab;
   return g;
 }
 
@@ -1060,7 +1096,9 @@ int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w
     sa.gp_now = drift->gp_now;
     sa.fbound = drift->fbound;
   }
-  const HpGeom g = hp_geom(nside);
+  int rc = 0;
+  const HpGeom g = hp_geom_table(p, nside, s, rc);
+  if (rc) return rc;
   double* acc = p->acc;
   hipLaunchKernelGGL(k_zero_d, dim3(grid_for((size_t)g.npix, 256)), dim3(256), 0, s, acc, g.npix);
   PMX_CHECK_LAUNCH("healpix_zero");
@@ -1092,7 +1130,9 @@ int pmx_launch_healpix_vjp(pmx_plan_t* p, const PartArgs& a, const float* w, flo
     sa.nparam = drift->ptab ? drift->nparam : 0;
     sa.pdots = drift->pdots;
   }
-  const HpGeom g = hp_geom(nside);
+  int rc = 0;
+  const HpGeom g = hp_geom_table(p, nside, s, rc);
+  if (rc) return rc;
   const double vol = (4.0 * M_PI / (double)g.npix) * (rmax * rmax * rmax - rmin * rmin * rmin) / 3.0;
   hipLaunchKernelGGL(k_healpix_vjp, dim3((unsigned)sa.nbx * sa.nby * sa.nbz), dim3(256), 0, s, a, w, w0, sa, g,
                      map_bar, 1.0 / vol, pos_bar, drift ? vel_bar : nullptr, accumulate);
