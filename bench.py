@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """PM step throughput of the fused MI355X engine on the single-GPU forward workload.
 
-Workload (the BASELINE single-GPU forward configuration): 256^3 particles on a 256^3 mesh, box
+Workload (the reference's headline single-GPU forward configuration): 256^3 particles on a 256^3 mesh, box
 1000 Mpc/h, float32, seeded Gaussian ICs (Philox white noise x EH98 P(k), Omega_c=0.25,
 sigma8=0.8, Omega_b=0.0486, h=0.6774, n_s=0.9667), 1LPT to a=0.1, EfficientDriftDoubleKick with
 dt0=0.05 to a=1, snapshots at 4 HEALPix shells (nside 256, bilinear, equal comoving width out to
