@@ -221,6 +221,14 @@ int pmx_paint_flat(pmx_plan plan, const float* pos, int npx, int npy, int npz, i
                    float weight, int fnx, int fny, float zlo, float zhi, double scale, float* map, int accumulate,
                    void* stream);
 
+/* Position cotangent of pmx_paint_flat for the cotangent map_bar of its (scaled) map: pos_bar (+)=
+ * scale * w * d/dpos sum_c w_c(pos) map_bar[c] over the 2D CIC corners (x and y chained through the
+ * flat-grid scaling); the slab selection is a step, so z gets nothing. Replaces jax.vjp of
+ * _single_paint_2d (src/jax_fli/_src/fields/_painting.py:160-202). */
+int pmx_paint_flat_vjp(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, const float* weights,
+                       float weight, int fnx, int fny, float zlo, float zhi, double scale, const float* map_bar,
+                       float* pos_bar, int accumulate, void* stream);
+
 /* ---- adjoints (vector-Jacobian products) --------------------------------------------------------
  * Hand-written transposes of the entry points above, for reverse-mode gradients of the reference's
  * `adjoint="reverse"` / `"checkpointed"` integration (src/jax_fli/pm/integrate.py:147-193, 288-538).

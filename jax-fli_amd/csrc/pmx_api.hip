@@ -11,6 +11,8 @@
 int pmx_launch_fill(float* a, size_t n, float v, hipStream_t s);
 int pmx_launch_fill_d(double* a, size_t n, hipStream_t s);
 int pmx_launch_finalize_map(const float* acc, float* map, long n, double scale, int accumulate, hipStream_t s);
+int pmx_launch_flat_vjp(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, int fnx, int fny, float zlo,
+                        float zhi, const float* map_bar, float scale, float* pos_bar, int accumulate, hipStream_t s);
 int pmx_launch_flat(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, int fnx, int fny, float zlo, float zhi,
                     float* acc, hipStream_t s);
 int pmx_launch_philox_raw(uint32_t* out, size_t nblocks, uint64_t seed, hipStream_t s);
@@ -925,6 +927,17 @@ int pmx_born_flat_vjp(pmx_plan p, const float* maps, int nshell, int ny, int nx,
   if (!kappa_bar || !maps_bar || !scales) return pmx_set_error(1, "null buffer");
   return pmx_launch_born_flat_vjp(p, maps, nshell, ny, nx, coef, nsrc, scales, kappa_bar, maps_bar, accumulate, dots,
                                   (hipStream_t)stream);
+}
+
+int pmx_paint_flat_vjp(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, const float* weights,
+                       float weight, int fnx, int fny, float zlo, float zhi, double scale, const float* map_bar,
+                       float* pos_bar, int accumulate, void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
+  if (!map_bar || !pos_bar || fnx < 1 || fny < 1) return pmx_set_error(1, "bad flat-sky arguments");
+  PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
+  return pmx_launch_flat_vjp(p, a, weights, weight, fnx, fny, zlo, zhi, map_bar, (float)scale, pos_bar, accumulate,
+                             (hipStream_t)stream);
 }
 
 int pmx_image_positions(pmx_plan p, const float* pos, const float* vel, int npx, int npy, int npz, int mode,

@@ -1146,6 +1146,50 @@ __global__ void __launch_bounds__(256) k_paint_flat(PartArgs a, const float* __r
   }
 }
 
+// Position cotangent of k_paint_flat for the map cotangent map_bar (the finalised map's, so the
+// 1 / pixel-volume scale enters here): pos_bar (+)= scale * wt * d/dpos sum_c w_c map_bar[c] over
+// the 4 corners, the weight derivatives -sgn(p - corner) as in k_vjp_gather, x and y chained through
+// the flat-grid scaling; the slab selection is a step and z gets nothing.
+__global__ void __launch_bounds__(256) k_paint_flat_vjp(PartArgs a, const float* __restrict__ w, float w0, FlatArgs fa,
+                                                        const float* __restrict__ map_bar, float scale,
+                                                        float* __restrict__ pos_bar, int accumulate, int rx, int ry,
+                                                        int rz) {
+  const unsigned np = (unsigned)a.npx * a.npy * a.npz;
+  const Chunk ch = my_chunk(np);
+  LatticeWalk lw;
+  lw.init(ch.begin + threadIdx.x, blockDim.x, a.npy, a.npz);
+  for (unsigned p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x, lw.step()) {
+    float3 x = ld3(a.x, p);
+    if (a.mode == 0) x = make_float3(x.x + (float)(lw.i * rx), x.y + (float)(lw.j * ry), x.z + (float)(lw.k * rz));
+    float gx = 0.f, gy = 0.f;
+    if (x.z > fa.zlo && x.z <= fa.zhi) {
+      const float wt = (w ? w[p] : w0) * scale;
+      float px = x.x * fa.sxf, py = x.y * fa.syf;
+      px = px - floorf(px / fa.fnx) * fa.fnx;
+      py = py - floorf(py / fa.fny) * fa.fny;
+      const Corner1D cx = corner_abs(px, fa.fnx), cy = corner_abs(py, fa.fny);
+      const float fx = px - floorf(px), fy = py - floorf(py);
+      const float dwx[2] = {-sgnf(fx), -sgnf(fx - 1.0f)}, dwy[2] = {-sgnf(fy), -sgnf(fy - 1.0f)};
+      const float wxv[2] = {cx.w0, cx.w1}, wyv[2] = {cy.w0, cy.w1};
+#pragma unroll
+      for (int o = 0; o < 4; ++o) {
+        const int ox = o & 1, oy = o >> 1;
+        const float m = map_bar[(size_t)(ox ? cx.i1 : cx.i0) * fa.fny + (oy ? cy.i1 : cy.i0)];
+        gx += dwx[ox] * wyv[oy] * m;
+        gy += wxv[ox] * dwy[oy] * m;
+      }
+      gx *= wt * fa.sxf;
+      gy *= wt * fa.syf;
+    }
+    float3 g = make_float3(gx, gy, 0.f);
+    if (accumulate) {
+      const float3 o = ld3(pos_bar, p);
+      g = make_float3(o.x + g.x, o.y + g.y, o.z);
+    }
+    st3(pos_bar, p, g);
+  }
+}
+
 // ---------------------------------------------------------------- Philox4x32-10 white noise
 // Salmon et al. 2011 (Random123). counter = (idx/4 lo, idx/4 hi, 0, 0), key = (seed lo, seed hi).
 // Box-Muller on u = ((r >> 8) + 1) 2^-24 in (0, 1], evaluated in float64 then rounded.
@@ -1448,6 +1492,23 @@ int pmx_launch_readout(pmx_plan_t* p, const PartArgs& a, const float* mesh, int 
 int pmx_launch_finalize_map(const float* acc, float* map, long n, double scale, int accumulate, hipStream_t s) {
   hipLaunchKernelGGL(k_finalize_map, dim3(grid_for(n, 256)), dim3(256), 0, s, acc, map, n, scale, accumulate);
   return pmx_check_launch("finalize_map");
+}
+
+int pmx_launch_flat_vjp(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, int fnx, int fny, float zlo,
+                        float zhi, const float* map_bar, float scale, float* pos_bar, int accumulate, hipStream_t s) {
+  int rx, ry, rz;
+  ratios(p, a, rx, ry, rz);
+  FlatArgs fa;
+  fa.fnx = fnx;
+  fa.fny = fny;
+  fa.sxf = (float)fnx / (float)p->nx;
+  fa.syf = (float)fny / (float)p->ny;
+  fa.zlo = zlo;
+  fa.zhi = zhi;
+  const size_t np = (size_t)a.npx * a.npy * a.npz;
+  hipLaunchKernelGGL(k_paint_flat_vjp, dim3(grid_particles(np)), dim3(256), 0, s, a, w, w0, fa, map_bar, scale, pos_bar,
+                     accumulate, rx, ry, rz);
+  return pmx_check_launch("paint_flat_vjp");
 }
 
 int pmx_launch_flat(pmx_plan_t* p, const PartArgs& a, const float* w, float w0, int fnx, int fny, float zlo, float zhi,

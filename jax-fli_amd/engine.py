@@ -470,6 +470,26 @@ def paint_flat(pos: torch.Tensor, mesh_shape, flatsky_npix, zlo, zhi, scale, mod
     return out
 
 
+def paint_flat_vjp(pos: torch.Tensor, map_bar: torch.Tensor, mesh_shape, zlo, zhi, scale, mode="absolute",
+                   weights=1.0, pos_bar: torch.Tensor | None = None, accumulate=False) -> torch.Tensor:
+    """pos_bar (+)= d<map_bar, paint_flat(pos, ...)>/d pos (x and y; z gets nothing)."""
+    pos = _f32(pos, "pos")
+    map_bar = _f32(map_bar, "map_bar")
+    if map_bar.dim() != 2:
+        raise ValueError("map_bar must be a (fnx, fny) map")
+    npx, npy, npz = _pgrid(pos)
+    plan = get_plan(mesh_shape, pos.device)
+    fnx, fny = int(map_bar.shape[0]), int(map_bar.shape[1])
+    if pos_bar is None:
+        pos_bar = torch.empty_like(pos)
+        accumulate = False
+    w, w0 = _weights(weights, npx * npy * npz)
+    check(load().pmx_paint_flat_vjp(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], ptr(w), w0, fnx, fny,
+                                    float(zlo), float(zhi), float(scale), ptr(map_bar), ptr(pos_bar), int(accumulate),
+                                    stream_ptr()), "pmx_paint_flat_vjp")
+    return pos_bar
+
+
 def image_positions(pos: torch.Tensor, mesh_shape, box, observer, rot=0, shift=(0.0, 0.0, 0.0), mode="relative",
                     vel=None, drift=None, out=None) -> torch.Tensor:
     """Absolute grid positions (npx, npy, npz, 3) of one image of the particles: rotated by the cube

@@ -169,6 +169,16 @@ class AbstractField:
         return self.replace(array=-self.array)
 
 
+def flat_slab(field, center, width):
+    """(zlo, zhi, scale) of the flat-sky slab paint (_painting.py:164-202): the slab
+    (c - w/2, c + w/2] in grid units of z, float32 bounds, and 1 / pixel volume."""
+    nz, Lz = field.mesh_size[2], field.box_size[2]
+    fnx, fny = field.flatsky_npix
+    wg, cg = float(width) * nz / Lz, float(center) * nz / Lz
+    pixvol = (field.box_size[0] / fnx) * (field.box_size[1] / fny) * float(width)
+    return float(np.float32(cg - wg / 2)), float(np.float32(cg + wg / 2)), 1.0 / pixvol
+
+
 def _paint_mode(unit: PositionUnit) -> str:
     if unit == PositionUnit.MPC_H:
         raise ValueError("Cannot paint ParticleField with unit MPC_H; convert to GRID_RELATIVE or GRID_ABSOLUTE first.")
@@ -266,17 +276,11 @@ class ParticleField(AbstractField):
             data = data[None]
         elif data.ndim != 5 or data.shape[0] != centers.size:
             raise ValueError("batched input needs one center/width per snapshot")
-        nx, ny, nz = self.mesh_size
-        Lx, Ly, Lz = self.box_size
-        fnx, fny = self.flatsky_npix
         maps = []
         for b in range(data.shape[0]):
-            c, wdt = centers[b], widths[b]
-            wg, cg = wdt * nz / Lz, c * nz / Lz
-            pixvol = (Lx / fnx) * (Ly / fny) * wdt
-            maps.append(engine.paint_flat(data[b].contiguous(), self.mesh_size, self.flatsky_npix,
-                                          float(np.float32(cg - wg / 2)), float(np.float32(cg + wg / 2)),
-                                          1.0 / pixvol, mode=mode, weights=1.0 if weights is None else weights))
+            zlo, zhi, scale = flat_slab(self, centers[b], widths[b])
+            maps.append(engine.paint_flat(data[b].contiguous(), self.mesh_size, self.flatsky_npix, zlo, zhi, scale,
+                                          mode=mode, weights=1.0 if weights is None else weights))
         arr = torch.stack(maps).squeeze(0) if len(maps) > 1 else maps[0]
         return FlatDensity.FromDensityMetadata(array=arr, field=self, status=FieldStatus.LIGHTCONE,
                                                comoving_centers=centers, density_width=widths,

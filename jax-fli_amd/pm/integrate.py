@@ -18,6 +18,7 @@ import numpy as np
 import torch
 
 from .. import autodiff, engine
+from ..fields import flat_slab
 from .interp import paint_drifted_vjp
 from .solvers import AbstractNBodySolver, NBodyState, ReversibleDoubleKickDrift, StepOp, _mode, force_scale, run_op
 
@@ -83,10 +84,12 @@ def _snapshot_vjp(solver, displacements, velocities, tg, cosmo, t0, param_names=
     interp = solver.interp_kernel
     drifting = type(interp).__name__ == "DriftInterp" or bool(getattr(interp, "drift_on_lightcone", False))
     target = interp.painting.target if interp.painting is not None else "particles"
-    if target not in ("density", "particles", "spherical"):
+    if target not in ("density", "particles", "spherical", "flat"):
         raise NotImplementedError(f"gradients through '{target}' snapshots are not implemented")
     if drifting and target != "spherical":
         raise NotImplementedError(f"gradients through drifted '{target}' snapshots are not implemented")
+    if target == "flat" and type(interp).__name__ == "TelephotoInterp":
+        raise NotImplementedError("gradients through TelephotoInterp's flat shells are not implemented")
     mesh_size = displacements.mesh_size
     mode = _mode(displacements)
     base_state = interp.init()
@@ -104,6 +107,10 @@ def _snapshot_vjp(solver, displacements, velocities, tg, cosmo, t0, param_names=
             return engine.paint_vjp(x, g.contiguous(), mode=mode, weights=weights)[0], None, None
         st = state_of(idx).interp_state
         rc, wd = interp.shell(st)
+        if target == "flat":
+            zlo, zhi, scale = flat_slab(displacements, rc, wd)
+            return engine.paint_flat_vjp(x, g.contiguous(), mesh_size, zlo, zhi, scale, mode=mode,
+                                         weights=weights), None, None
         tiles = interp.images(st, displacements)
         if drifting:
             return paint_drifted_vjp(displacements.replace(array=x), velocities.replace(array=v), save_time[idx], cosmo,

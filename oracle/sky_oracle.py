@@ -35,6 +35,8 @@ Parity status per function:
                                         mesh's (nx, ny) = the 3D CIC paint summed over z; a particle on
                                         integer flat coordinates fills one pixel; slab edges are
                                         (lo, hi]; mass = selected weight / pixel volume)
+  paint_flat_vjp                        pinned (float64 central differences of paint_flat along random
+                                        directions; z gets nothing)
   image_positions                       pinned (code 0, zero shift and no drift give the unwrapped
                                         grid positions; a code permutes and flips the centred axes as
                                         cube_symmetry's matrix does; the drift moves each image by
@@ -459,6 +461,41 @@ def paint_flat(pos, mesh_shape, flatsky_npix, zlo, zhi, scale, mode="relative", 
             iy = np.mod(c[:, 1].astype(np.int64), fny)
             out += np.bincount(ix * fny + iy, weights=k, minlength=out.size)
     return (out * scale).reshape(fnx, fny)
+
+
+def paint_flat_vjp(pos, map_bar, mesh_shape, zlo, zhi, scale, mode="relative", weight=1.0):
+    """d<map_bar, paint_flat(pos)>/d pos (jax.vjp of _single_paint_2d, _painting.py:160-202): the 2D CIC
+    weight derivatives -sgn(p - corner) times the other axis' weight, chained through the flat-grid
+    scaling; the slab selection is a step (zero derivative), so z gets nothing. Shape (npart, 3)."""
+    pos = np.asarray(pos)
+    np_shape = pos.shape[:3]
+    dt = np.float64 if pos.dtype == np.float64 else np.float32
+    g = pos.reshape(-1, 3).astype(dt)
+    if mode == "relative":
+        g = g + orc._lattice(np_shape, mesh_shape).astype(dt)
+    g = g.astype(np.float64)
+    n = g.shape[0]
+    wt = np.broadcast_to(np.asarray(weight, dtype=np.float64).reshape(-1), (n,)) * scale
+    mb = np.asarray(map_bar, dtype=np.float64)
+    fnx, fny = mb.shape
+    sxy = np.array([fnx / mesh_shape[0], fny / mesh_shape[1]])
+    out = np.zeros((n, 3))
+    sel = (g[:, 2] > zlo) & (g[:, 2] <= zhi)
+    xy = np.mod(g[sel, :2] * sxy, np.array([fnx, fny], dtype=np.float64))
+    fl = np.floor(xy)
+    gx = np.zeros(xy.shape[0])
+    gy = np.zeros(xy.shape[0])
+    for ox in (0, 1):
+        for oy in (0, 1):
+            c = fl + np.array([ox, oy])
+            wx = 1.0 - np.abs(xy[:, 0] - c[:, 0])
+            wy = 1.0 - np.abs(xy[:, 1] - c[:, 1])
+            m = mb[np.mod(c[:, 0].astype(np.int64), fnx), np.mod(c[:, 1].astype(np.int64), fny)]
+            gx += -np.sign(xy[:, 0] - c[:, 0]) * wy * m
+            gy += wx * -np.sign(xy[:, 1] - c[:, 1]) * m
+    out[sel, 0] = gx * wt[sel] * sxy[0]
+    out[sel, 1] = gy * wt[sel] * sxy[1]
+    return out
 
 
 def image_positions(pos, mesh_shape, box, observer, code=0, shift=(0.0, 0.0, 0.0), mode="relative", vel=None,

@@ -335,3 +335,23 @@ def test_image_positions_identity_rotation_and_drift():
     Vt = np.stack([[1, 1, -1][c] * v.reshape(-1, 3)[:, (0, 2, 1)[c]] for c in range(3)], axis=1)
     Y = np.stack([[1, 1, -1][c] * X[:, (0, 2, 1)[c]] for c in range(3)], axis=1) + np.array([0.0, 0.0, 160.0])
     assert np.allclose(got, (Y + 0.25 * Vt + obs) / cell, atol=1e-12)
+
+
+def test_flat_paint_vjp_matches_differences():
+    rng = np.random.default_rng(8)
+    mesh, flat = (8, 8, 8), (12, 10)
+    for mode, pos in (("relative", rng.normal(0.0, 1.2, (4, 4, 4, 3))),
+                      ("absolute", rng.uniform(0.0, 8.0, (4, 4, 4, 3)))):
+        w = rng.uniform(0.5, 1.5, 64)
+        mb = rng.standard_normal(flat)
+        zlo, zhi, scale = 1.3, 6.7, 0.37
+        g = so.paint_flat_vjp(pos, mb, mesh, zlo, zhi, scale, mode=mode, weight=w)
+        assert np.all(g[:, 2] == 0.0)
+        eps = 1e-6
+        for _ in range(3):
+            d = rng.standard_normal(pos.shape)
+            d[..., 2] = 0.0  # the slab selection is a step: keep z fixed
+            fp = (so.paint_flat(pos + eps * d, mesh, flat, zlo, zhi, scale, mode=mode, weight=w) * mb).sum()
+            fm = (so.paint_flat(pos - eps * d, mesh, flat, zlo, zhi, scale, mode=mode, weight=w) * mb).sum()
+            fd = (fp - fm) / (2 * eps)
+            assert (g * d.reshape(-1, 3)).sum() == pytest.approx(fd, rel=1e-6, abs=1e-9)
