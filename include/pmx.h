@@ -215,6 +215,19 @@ int pmx_image_positions(pmx_plan plan, const float* pos, const float* vel, int n
                         const double* drift_tables, int ntab, double chi_max, double a_lo, double a_hi, double a_now,
                         double gp_now, float* out, void* stream);
 
+/* Cotangents of pmx_image_positions for out_bar (npx*npy*npz*3 floats, of its grid positions):
+ * pos_bar (+)= cell R^T Ybar with Ybar = out_bar / cell + f'(|Y|) (R v . out_bar / cell) Y / |Y|, and with
+ * vel (the drift) vel_bar (+)= f R^T out_bar / cell; with nparam (<= 4) tables of df/dtheta_q on the
+ * a(chi) grid, param_dots[q] += sum (R v . out_bar / cell) df/dtheta_q(|Y|). The jax.vjp of
+ * TelephotoInterp's and DriftInterp's image transform and drift (src/jax_fli/pm/interp.py:110-121,
+ * 277-289, 590-660). */
+int pmx_image_positions_vjp(pmx_plan plan, const float* pos, const float* vel, int npx, int npy, int npz, int mode,
+                            const double box[3], const double observer[3], int rot, const double shift[3],
+                            const double* drift_tables, int ntab, double chi_max, double a_lo, double a_hi,
+                            double a_now, double gp_now, const float* out_bar, float* pos_bar, float* vel_bar,
+                            int accumulate, int nparam, const double* param_tables, double* param_dots,
+                            void* stream);
+
 /* Flat-sky slab paint: absolute z in (zlo, zhi] (grid units), (x, y) rescaled to (fnx, fny) and
  * wrapped, 2D CIC, times `scale` (1 / pixel volume). _single_paint_2d (_painting.py:117-202). */
 int pmx_paint_flat(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, const float* weights,

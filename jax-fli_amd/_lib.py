@@ -52,6 +52,8 @@ SIGNATURES = {
     "pmx_paint_healpix": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _dp, _dp, _d, _d, _i, _i, _vp, _i, _vp]),
     "pmx_paint_flat": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _i, _i, _f, _f, _d, _vp, _i, _vp]),
     "pmx_paint_flat_vjp": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _i, _i, _f, _f, _d, _vp, _vp, _i, _vp]),
+    "pmx_image_positions_vjp": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _dp, _dp, _i, _dp, _vp, _i, _d, _d, _d, _d, _d,
+                                     _vp, _vp, _vp, _i, _i, _vp, _vp, _vp]),
     "pmx_image_positions": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _dp, _dp, _i, _dp, _vp, _i, _d, _d, _d, _d, _d, _vp,
                                  _vp]),
     "pmx_paint_cic_vjp": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _vp, _vp, _vp, _i, _vp]),

@@ -35,6 +35,9 @@ int pmx_launch_healpix_vjp(pmx_plan_t* p, const PartArgs& a, const float* w, flo
                            hipStream_t s, const PmxDrift* drift = nullptr, float* vel_bar = nullptr);
 int pmx_launch_image_positions(pmx_plan_t* p, const PartArgs& a, const double box[3], const double obs[3], int code,
                                const double shift[3], float* out, hipStream_t s, const PmxDrift* drift);
+int pmx_launch_image_positions_vjp(pmx_plan_t* p, const PartArgs& a, const double box[3], const double obs[3],
+                                   int code, const double shift[3], const float* gbar, float* xbar, float* vbar,
+                                   int accumulate, hipStream_t s, const PmxDrift* drift);
 int pmx_launch_born(pmx_plan_t* p, const float* maps, int S, long npix, const double* C, int K, float* kappa,
                     int accumulate, hipStream_t s);
 int pmx_launch_born_vjp(pmx_plan_t* p, const float* maps, int S, long npix, const double* C, int K,
@@ -958,6 +961,36 @@ int pmx_image_positions(pmx_plan p, const float* pos, const float* vel, int npx,
   const PmxDrift d{vel, drift_tables, ntab, chi_max, a_lo, a_hi, a_now, gp_now, 0.0};
   return pmx_launch_image_positions(p, part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p), box,
                                     observer, rot, shift, out, (hipStream_t)stream, &d);
+}
+
+int pmx_image_positions_vjp(pmx_plan p, const float* pos, const float* vel, int npx, int npy, int npz, int mode,
+                            const double box[3], const double observer[3], int rot, const double shift[3],
+                            const double* drift_tables, int ntab, double chi_max, double a_lo, double a_hi,
+                            double a_now, double gp_now, const float* out_bar, float* pos_bar, float* vel_bar,
+                            int accumulate, int nparam, const double* param_tables, double* param_dots,
+                            void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
+  if (!out_bar || !pos_bar || !box || !observer || !shift) return pmx_set_error(1, "null argument");
+  if (rot < 0 || rot >= 48) return pmx_set_error(1, "rotation code must be in [0, 48)");
+  for (int c = 0; c < 3; ++c)
+    if (!(box[c] > 0.0)) return pmx_set_error(1, "box sizes must be positive");
+  if ((size_t)npx * npy * npz * 3 >= ((size_t)1 << 32)) return pmx_set_error(1, "too many particles");
+  PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
+  if (!vel)
+    return pmx_launch_image_positions_vjp(p, a, box, observer, rot, shift, out_bar, pos_bar, nullptr, accumulate,
+                                          (hipStream_t)stream, nullptr);
+  if (!vel_bar) return pmx_set_error(1, "a drifted image needs vel_bar");
+  if (!drift_tables || ntab < 2 || !(chi_max > 0.0) || !(a_hi > a_lo) || !(a_now > 0.0))
+    return pmx_set_error(1, "bad drift tables: need ntab >= 2, chi_max > 0, a_hi > a_lo, a_now > 0");
+  if (nparam < 0 || nparam > 4 || (nparam > 0 && (!param_tables || !param_dots)))
+    return pmx_set_error(1, "need 0 <= nparam <= 4 with parameter tables and dots");
+  PmxDrift d{vel, drift_tables, ntab, chi_max, a_lo, a_hi, a_now, gp_now, 0.0};
+  d.ptab = param_tables;
+  d.nparam = nparam;
+  d.pdots = param_dots;
+  return pmx_launch_image_positions_vjp(p, a, box, observer, rot, shift, out_bar, pos_bar, vel_bar, accumulate,
+                                        (hipStream_t)stream, &d);
 }
 
 int pmx_paint_flat(pmx_plan p, const float* pos, int npx, int npy, int npz, int mode, const float* weights,

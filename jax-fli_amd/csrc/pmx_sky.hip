@@ -1059,7 +1059,97 @@ __global__ void __launch_bounds__(256) k_image_positions(PartArgs a, ShellArgs s
   }
   for (int c = 0; c < 3; ++c) out[3 * p + c] = (float)((Y[c] + sa.obs[c]) / sa.cell[c]);
 }
+
+// Cotangents of k_image_positions for gbar (per particle, of the absolute grid positions out):
+// Zbar = gbar / cell; with the drift, Ybar = Zbar + f'(|Y|) (W . Zbar) Y / |Y| and Wbar = f Zbar
+// (W = R v); xbar (+)= cell * R^T Ybar, vbar (+)= R^T Wbar, and pdots[q] += sum (W . Zbar)
+// df/dtheta_q(|Y|) from the parameter tables.
+__global__ void __launch_bounds__(256) k_image_positions_vjp(PartArgs a, ShellArgs sa, int code, double sh0,
+                                                             double sh1, double sh2, const float* __restrict__ gbar,
+                                                             float* __restrict__ xbar, float* __restrict__ vbar,
+                                                             int accumulate) {
+  const size_t n = (size_t)a.npx * a.npy * a.npz;
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double pp[kDriftParams] = {0.0, 0.0, 0.0, 0.0};
+  if (p < n) {
+    const int k = (int)(p % (size_t)a.npz);
+    const size_t q = p / (size_t)a.npz;
+    const int j = (int)(q % (size_t)a.npy), i = (int)(q / (size_t)a.npy);
+    const Sym R = sym_of(code);
+    const float3 gb = ld3(gbar, (unsigned)p);
+    const double Zb[3] = {gb.x / sa.cell[0], gb.y / sa.cell[1], gb.z / sa.cell[2]};
+    double Yb[3] = {Zb[0], Zb[1], Zb[2]}, Wb[3] = {0.0, 0.0, 0.0};
+    if (sa.v) {
+      const float3 x = ld3(a.x, (unsigned)p);
+      const bool rel = a.mode == 0;
+      const double X[3] = {((double)x.x + (rel ? (double)(i * sa.rx) : 0.0)) * sa.cell[0] - sa.obs[0],
+                           ((double)x.y + (rel ? (double)(j * sa.ry) : 0.0)) * sa.cell[1] - sa.obs[1],
+                           ((double)x.z + (rel ? (double)(k * sa.rz) : 0.0)) * sa.cell[2] - sa.obs[2]};
+      const double sh[3] = {sh0, sh1, sh2};
+      const float3 v = ld3(sa.v, (unsigned)p);
+      const double V[3] = {v.x, v.y, v.z};
+      double Y[3], W[3];
+      for (int c = 0; c < 3; ++c) {
+        Y[c] = R.s[c] * X[R.p[c]] + sh[c];
+        W[c] = R.s[c] * V[R.p[c]];
+      }
+      const double ry = sqrt(Y[0] * Y[0] + Y[1] * Y[1] + Y[2] * Y[2]);
+      double dfdr;
+      const double f = drift_factor(sa, ry, dfdr);
+      const double wz = W[0] * Zb[0] + W[1] * Zb[1] + W[2] * Zb[2];
+      const double cc = ry > 0.0 ? dfdr * wz / ry : 0.0;
+      for (int c = 0; c < 3; ++c) {
+        Yb[c] += cc * Y[c];
+        Wb[c] = f * Zb[c];
+      }
+      for (int t = 0; t < sa.nparam; ++t) pp[t] = wz * tab_lin(sa.ptab + t * sa.ntab, sa.ntab, 0.0, sa.chi_max, ry);
+    }
+    double Xb[3] = {0.0, 0.0, 0.0}, Vb[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < 3; ++c) {
+      Xb[R.p[c]] += R.s[c] * Yb[c];
+      Vb[R.p[c]] += R.s[c] * Wb[c];
+    }
+    float3 o = make_float3((float)(Xb[0] * sa.cell[0]), (float)(Xb[1] * sa.cell[1]), (float)(Xb[2] * sa.cell[2]));
+    if (accumulate) {
+      const float3 t = ld3(xbar, (unsigned)p);
+      o = make_float3(t.x + o.x, t.y + o.y, t.z + o.z);
+    }
+    st3(xbar, (unsigned)p, o);
+    if (vbar) {
+      float3 ov = make_float3((float)Vb[0], (float)Vb[1], (float)Vb[2]);
+      if (accumulate) {
+        const float3 t = ld3(vbar, (unsigned)p);
+        ov = make_float3(t.x + ov.x, t.y + ov.y, t.z + ov.z);
+      }
+      st3(vbar, (unsigned)p, ov);
+    }
+  }
+  for (int t = 0; t < sa.nparam; ++t) block_sum_add(pp[t], sa.pdots + t);
+}
 }  // namespace
+
+int pmx_launch_image_positions_vjp(pmx_plan_t* p, const PartArgs& a, const double box[3], const double obs[3],
+                                   int code, const double shift[3], const float* gbar, float* xbar, float* vbar,
+                                   int accumulate, hipStream_t s, const PmxDrift* drift) {
+  ShellArgs sa = shell_args(p, a, box, obs, 0.0, 0.0, 0, 1, nullptr, nullptr);
+  if (drift) {
+    sa.v = drift->v;
+    sa.dtab = drift->tables;
+    sa.ntab = drift->ntab;
+    sa.chi_max = drift->chi_max;
+    sa.a_lo = drift->a_lo;
+    sa.a_hi = drift->a_hi;
+    sa.a_now = drift->a_now;
+    sa.gp_now = drift->gp_now;
+    sa.ptab = drift->ptab;
+    sa.nparam = drift->ptab ? drift->nparam : 0;
+    sa.pdots = drift->pdots;
+  }
+  const size_t n = (size_t)a.npx * a.npy * a.npz;
+  hipLaunchKernelGGL(k_image_positions_vjp, dim3(grid_for(n, 256)), dim3(256), 0, s, a, sa, code, shift[0], shift[1],
+                     shift[2], gbar, xbar, drift ? vbar : nullptr, accumulate);
+  return pmx_check_launch("image_positions_vjp");
+}
 
 int pmx_launch_image_positions(pmx_plan_t* p, const PartArgs& a, const double box[3], const double obs[3], int code,
                                const double shift[3], float* out, hipStream_t s, const PmxDrift* drift) {

@@ -517,6 +517,40 @@ def image_positions(pos: torch.Tensor, mesh_shape, box, observer, rot=0, shift=(
     return out
 
 
+def image_positions_vjp(pos: torch.Tensor, out_bar: torch.Tensor, mesh_shape, box, observer, rot=0,
+                        shift=(0.0, 0.0, 0.0), mode="relative", vel=None, drift=None, param_tables=None,
+                        param_dots=None):
+    """(pos_bar, vel_bar | None): cotangents of image_positions for out_bar; param_tables (float64 device
+    (nparam, ntab): df/dtheta on the drift's a(chi) grid) and param_dots (float64 device, added to) give
+    up to 4 parameter cotangents through the drift factor."""
+    pos = _f32(pos, "pos")
+    out_bar = _f32(out_bar, "out_bar")
+    if out_bar.shape != pos.shape:
+        raise ValueError("out_bar must have the shape of pos")
+    npx, npy, npz = _pgrid(pos)
+    plan = get_plan(mesh_shape, pos.device)
+    pos_bar = torch.empty_like(pos)
+    vel_bar = None
+    nparam = 0
+    if vel is not None:
+        vel = _f32(vel, "vel")
+        if drift is None:
+            raise ValueError("a drift needs its tables")
+        vel_bar = torch.empty_like(vel)
+        tab = drift["tables"]
+        targs = (ptr(tab), int(tab.shape[-1]), float(drift["chi_max"]), float(drift["a_lo"]), float(drift["a_hi"]),
+                 float(drift["a_now"]), float(drift["gp_now"]))
+        if param_tables is not None:
+            nparam = int(param_tables.shape[0])
+    else:
+        targs = (None, 0, 0.0, 0.0, 0.0, 0.0, 0.0)
+    check(load().pmx_image_positions_vjp(plan.handle, ptr(pos), ptr(vel), npx, npy, npz, MODES[mode], dbl3(box),
+                                         dbl3(observer), int(rot), dbl3(shift), *targs, ptr(out_bar), ptr(pos_bar),
+                                         ptr(vel_bar), 0, nparam, ptr(param_tables) if nparam else None,
+                                         ptr(param_dots) if nparam else None, stream_ptr()), "pmx_image_positions_vjp")
+    return pos_bar, vel_bar
+
+
 # ---------------------------------------------------------------------------- adjoints
 def _zeros_like_or(t, like):
     return torch.zeros_like(like) if t is None else t

@@ -19,7 +19,7 @@ import torch
 
 from .. import autodiff, engine
 from ..fields import flat_slab
-from .interp import paint_drifted_vjp
+from .interp import image_snapshot_vjp, paint_drifted_vjp
 from .solvers import AbstractNBodySolver, NBodyState, ReversibleDoubleKickDrift, StepOp, _mode, force_scale, run_op
 
 f32 = np.float32
@@ -86,10 +86,8 @@ def _snapshot_vjp(solver, displacements, velocities, tg, cosmo, t0, param_names=
     target = interp.painting.target if interp.painting is not None else "particles"
     if target not in ("density", "particles", "spherical", "flat"):
         raise NotImplementedError(f"gradients through '{target}' snapshots are not implemented")
-    if drifting and target != "spherical":
-        raise NotImplementedError(f"gradients through drifted '{target}' snapshots are not implemented")
-    if target == "flat" and type(interp).__name__ == "TelephotoInterp":
-        raise NotImplementedError("gradients through TelephotoInterp's flat shells are not implemented")
+    # flat, density and particle snapshots of a drifted or rotated image go through the image's adjoint
+    imaged = target != "spherical" and (drifting or type(interp).__name__ == "TelephotoInterp")
     mesh_size = displacements.mesh_size
     mode = _mode(displacements)
     base_state = interp.init()
@@ -100,12 +98,15 @@ def _snapshot_vjp(solver, displacements, velocities, tg, cosmo, t0, param_names=
                           t_initial=float(t0))
 
     def save_vjp(idx, x, v, g):
+        st = state_of(idx).interp_state
+        if imaged:
+            return image_snapshot_vjp(interp, st, displacements.replace(array=x), velocities.replace(array=v),
+                                      save_time[idx], cosmo, g, param_names=param_names)
         if target == "particles":
             return g, None, None
         weights = 1.0 if interp.painting.weights is None else interp.painting.weights
         if target == "density":
             return engine.paint_vjp(x, g.contiguous(), mode=mode, weights=weights)[0], None, None
-        st = state_of(idx).interp_state
         rc, wd = interp.shell(st)
         if target == "flat":
             zlo, zhi, scale = flat_slab(displacements, rc, wd)

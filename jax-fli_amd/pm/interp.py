@@ -18,7 +18,7 @@ import torch
 
 from .. import cosmology as _c
 from .. import engine
-from ..fields import DensityUnit, FieldStatus, PositionUnit, SphericalDensity, _paint_mode
+from ..fields import DensityUnit, FieldStatus, PositionUnit, SphericalDensity, _paint_mode, flat_slab
 from ..painting import PaintingOptions
 
 DRIFT_TABLE = 4096
@@ -86,6 +86,62 @@ def paint_drifted_vjp(dx, p, t, cosmo, r_center, width, painting, map_bar, tiles
                                             drift, scheme=painting.scheme, mode=_paint_mode(dx.unit),
                                             weights=1.0 if painting.weights is None else painting.weights,
                                             tiles=tiles, param_tables=ptab, param_dots=pdots)
+    return gx, gv, pdots
+
+
+def _image_spec(interp, state, dx, t, cosmo):
+    """(rotation code, shift, drift tables or None) of the image a flat, density or particle snapshot of
+    DriftInterp / TelephotoInterp is painted from."""
+    tiles = interp.images(state, dx) if isinstance(interp, TelephotoInterp) else None
+    code, shift = (0, np.zeros(3)) if tiles is None else (int(tiles[0][0]), np.asarray(tiles[1][0], np.float64))
+    drift = None
+    if isinstance(interp, DriftInterp) or getattr(interp, "drift_on_lightcone", False):
+        reach = float(np.linalg.norm(shift)) + 2.0 * float(np.linalg.norm(dx.box_size)) + 100.0
+        drift = drift_tables(cosmo, float(t), reach, dx.array.device)
+    return code, shift, drift
+
+
+def _image_field(interp, state, dx, p, t, cosmo):
+    """The snapshot's particles as one (rotated, shifted, drifted) image in absolute grid units."""
+    code, shift, drift = _image_spec(interp, state, dx, t, cosmo)
+    g = engine.image_positions(dx.array.contiguous(), dx.mesh_size, dx.box_size, dx.observer_position_mpc, rot=code,
+                               shift=shift, mode=_paint_mode(dx.unit), vel=p.array.contiguous() if drift else None,
+                               drift=drift)
+    return dx.replace(array=g, unit=PositionUnit.GRID_ABSOLUTE)
+
+
+def image_snapshot_vjp(interp, state, dx, p, t, cosmo, g, param_names=()):
+    """(position, velocity or None, parameter dots or None) cotangents of a flat, density or particle
+    snapshot painted from an image (DriftInterp, TelephotoInterp) for its cotangent g: the target's
+    paint adjoint at the image positions, then pmx_image_positions_vjp through the transform and drift
+    (and the drift factor's dependence on the cosmological parameters param_names)."""
+    from .. import autodiff
+
+    code, shift, drift = _image_spec(interp, state, dx, t, cosmo)
+    vel = p.array.contiguous() if drift else None
+    img = engine.image_positions(dx.array.contiguous(), dx.mesh_size, dx.box_size, dx.observer_position_mpc,
+                                 rot=code, shift=shift, mode=_paint_mode(dx.unit), vel=vel, drift=drift)
+    pt = interp.painting
+    weights = 1.0 if pt.weights is None else pt.weights
+    if pt.target == "flat":
+        r_center, width = interp.shell(state)
+        zlo, zhi, scale = flat_slab(dx, r_center, width)
+        gi = engine.paint_flat_vjp(img, g.contiguous(), dx.mesh_size, zlo, zhi, scale, mode="absolute",
+                                   weights=weights)
+    elif pt.target == "density":
+        gi = engine.paint_vjp(img, g.contiguous(), mode="absolute", weights=weights)[0]
+    else:  # particles, returned in Mpc/h
+        cell = torch.tensor([b / m for b, m in zip(dx.box_size, dx.mesh_size)], dtype=torch.float32, device=img.device)
+        gi = (g * cell).contiguous()
+    ptab = pdots = None
+    if drift is not None and param_names:
+        chi_max = drift["chi_max"]
+        ptab = engine.to_device(np.stack([autodiff.fd(lambda c: drift_factor_table(c, float(t), chi_max), cosmo, n)
+                                          for n in param_names]), dx.array.device)
+        pdots = torch.zeros(len(param_names), dtype=torch.float64, device=dx.array.device)
+    gx, gv = engine.image_positions_vjp(dx.array.contiguous(), gi, dx.mesh_size, dx.box_size,
+                                        dx.observer_position_mpc, rot=code, shift=shift, mode=_paint_mode(dx.unit),
+                                        vel=vel, drift=drift, param_tables=ptab, param_dots=pdots)
     return gx, gv, pdots
 
 
@@ -209,10 +265,7 @@ class DriftInterp(AbstractInterp):
             out = paint_drifted(dx, p, t, cosmo, r_center, width, pt)
             return out.replace(scale_factors=float(t), comoving_centers=r_center, density_width=width)
         # the drifted particles as absolute grid positions (interp.py:277-289), then the target's paint
-        drift = drift_tables(cosmo, float(t), 2.0 * float(np.linalg.norm(dx.box_size)) + 100.0, dx.array.device)
-        g = engine.image_positions(dx.array.contiguous(), dx.mesh_size, dx.box_size, dx.observer_position_mpc,
-                                   mode=_paint_mode(dx.unit), vel=p.array.contiguous(), drift=drift)
-        drifted = dx.replace(array=g, unit=PositionUnit.GRID_ABSOLUTE)
+        drifted = _image_field(self, state, dx, p, t, cosmo)
         if tgt == "flat":
             out = drifted.paint_2d(center=r_center, density_plane_width=width, weights=pt.weights,
                                    batch_size=pt.batch_size)
@@ -371,15 +424,7 @@ class TelephotoInterp(AbstractInterp):
         else:
             src = dx
             if tiles is not None or self.drift_on_lightcone:
-                code, shift = (0, np.zeros(3)) if tiles is None else (int(tiles[0][0]), tiles[1][0])
-                drift = None
-                if self.drift_on_lightcone:
-                    reach = float(np.linalg.norm(shift)) + 2.0 * float(np.linalg.norm(dx.box_size)) + 100.0
-                    drift = drift_tables(cosmo, float(t), reach, dx.array.device)
-                g = engine.image_positions(dx.array.contiguous(), dx.mesh_size, dx.box_size, dx.observer_position_mpc,
-                                           rot=code, shift=shift, mode=_paint_mode(dx.unit),
-                                           vel=p.array.contiguous() if drift else None, drift=drift)
-                src = dx.replace(array=g, unit=PositionUnit.GRID_ABSOLUTE)
+                src = _image_field(self, state, dx, p, t, cosmo)
             out = src.paint_2d(center=r_center, density_plane_width=width, weights=pt.weights,
                                batch_size=pt.batch_size)
         return out.replace(scale_factors=float(t), comoving_centers=r_center, density_width=width)

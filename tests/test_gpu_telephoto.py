@@ -286,3 +286,105 @@ def test_nbody_gradient_through_a_flat_snapshot(pkg, cuda, adjoint):
         got = float((grad.double().cpu().numpy() * d).sum())
         print(f"{adjoint} d/d{which}: adjoint {got:.6e} fd {fd:.6e}")
         assert abs(got - fd) / abs(fd) < 2e-3
+
+
+@pytest.mark.parametrize("drift", [False, True])
+def test_image_positions_vjp_matches_differences(pkg, cuda, drift):
+    """pmx_image_positions_vjp against float64 central differences of the oracle's image transform
+    (positions, and velocities through the drift)."""
+    mesh, box, obs = (16, 16, 16), (100.0, 100.0, 100.0), (50.0, 50.0, 50.0)
+    rng = np.random.default_rng(71)
+    pos = (rng.standard_normal((8, 8, 8, 3)) * 1.5).astype(np.float32)
+    vel = (rng.standard_normal(pos.shape) * 20.0).astype(np.float32)
+    gbar = rng.standard_normal(pos.shape).astype(np.float32)
+    c = pkg.Cosmology(**P)
+    a_now, code, shift = 0.8, 46, (0.0, 0.0, 150.0)
+    tab = pkg.pm.interp.drift_tables(c, a_now, 600.0, cuda) if drift else None
+    gx, gv = pkg.engine.image_positions_vjp(torch.from_numpy(pos).to(cuda), torch.from_numpy(gbar).to(cuda), mesh, box,
+                                            obs, rot=code, shift=shift,
+                                            vel=torch.from_numpy(vel).to(cuda) if drift else None, drift=tab)
+    dr = so.Drift(nbo.Cosmo(**P), a_now) if drift else None
+
+    def f(pp, vv):
+        return float((so.image_positions(pp, mesh, box, obs, code=code, shift=shift, vel=vv if drift else None,
+                                         drift=dr).reshape(pos.shape) * gbar).sum())
+
+    p64, v64 = pos.astype(np.float64), vel.astype(np.float64)
+    eps = 1e-5
+    d = rng.standard_normal(pos.shape)
+    fd = (f(p64 + eps * d, v64) - f(p64 - eps * d, v64)) / (2 * eps)
+    assert float((gx.double().cpu().numpy() * d).sum()) == pytest.approx(fd, rel=1e-4)
+    if drift:
+        fdv = (f(p64, v64 + eps * d) - f(p64, v64 - eps * d)) / (2 * eps)
+        assert float((gv.double().cpu().numpy() * d).sum()) == pytest.approx(fdv, rel=1e-4)
+    else:
+        assert gv is None
+
+
+@pytest.mark.parametrize("kind,target,adjoint", [("drift", "flat", "reverse"), ("drift", "density", "reverse"),
+                                                 ("drift", "particles", "checkpointed"),
+                                                 ("telephoto", "flat", "reverse")])
+def test_nbody_gradient_through_image_snapshots(pkg, cuda, kind, target, adjoint):
+    """d <W, snapshot> / d (dx, p) for one shell painted from a drifted (DriftInterp) or rotated, shifted
+    and drifted (TelephotoInterp, a shell beyond the box radius) image, against float64 central
+    differences of the oracle's run, image transform and paint."""
+    from oracle import pm_oracle as orc
+
+    N, BOX, A0, DT0, FLAT = 16, 100.0, 0.1, 0.2, (16, 16)
+    c = pkg.Cosmology(**P)
+    o = nbo.Cosmo(**P)
+    ic = pkg.gaussian_initial_conditions(11, (N,) * 3, (BOX,) * 3, cosmo=c, flatsky_npix=FLAT)
+    dx, p = pkg.lpt(c, ic, ts=A0)
+    x0, v0 = dx.array.clone(), p.array.clone()
+    rng = np.random.default_rng(64)
+    center, width = (30.0, 30.0) if kind == "drift" else (70.0, 30.0)
+    painting = pkg.PaintingOptions(target=target)
+    interp = (pkg.DriftInterp(painting=painting) if kind == "drift"
+              else pkg.TelephotoInterp(painting=painting, drift_on_lightcone=True))
+    solver = pkg.ReversibleDoubleKickDrift(interp_kernel=interp)
+    ts = np.stack([pkg.cosmology.a_of_chi(c, np.array([center - width / 2])),
+                   pkg.cosmology.a_of_chi(c, np.array([center + width / 2]))])
+    dxf = dx.replace(array=x0.clone().requires_grad_(True))
+    pf = p.replace(array=v0.clone().requires_grad_(True))
+    out = pkg.nbody(c, dxf, pf, t0=A0, t1=1.0, dt0=DT0, ts=ts, solver=solver, adjoint=adjoint)
+    W = rng.standard_normal(tuple(out.array.shape))
+    (out.array.double() * torch.from_numpy(W).to(cuda)).sum().backward()
+    r_c = float(np.asarray(out.comoving_centers).reshape(-1)[0])
+    w_c = float(np.asarray(out.density_width).reshape(-1)[0])
+    t_snap = float(np.asarray(out.scale_factors, dtype=np.float64).reshape(-1)[0])
+    radius = dx.max_comoving_radius
+    code, shift = (1, (0.0, 0.0, float(np.floor(r_c / radius) * radius))) if kind == "telephoto" else (0, (0.0,) * 3)
+    assert kind == "drift" or r_c > radius
+    x0n, v0n = x0.cpu().numpy().astype(np.float64), v0.cpu().numpy().astype(np.float64)
+    obs = (BOX / 2,) * 3
+
+    def loss(xx, vv):
+        snap = {}
+
+        def save(i, t, x, v):
+            snap["xv"] = (x, v)
+
+        nbo.run_nbody(o, xx, vv, A0, 1.0, DT0, np.array([t_snap], np.float32), solver="kkd", on_save=save)
+        x, v = snap["xv"]
+        g = so.image_positions(x, (N,) * 3, (BOX,) * 3, obs, code=code, shift=shift, vel=v,
+                               drift=so.Drift(o, t_snap)).reshape((N,) * 3 + (3,))
+        if target == "flat":
+            zlo, zhi, scale = pkg.fields.flat_slab(dx, r_c, w_c)
+            res = so.paint_flat(g, (N,) * 3, FLAT, zlo, zhi, scale, mode="absolute")
+        elif target == "density":
+            res = orc.cic_paint(g, (N,) * 3, mode="absolute", dtype=np.float64)
+        else:
+            res = g * (BOX / N)
+        return float((res.reshape(W.shape) * W).sum())
+
+    assert loss(x0n, v0n) != 0.0
+    eps = 1e-6
+    for which, grad in (("x", dxf.array.grad), ("v", pf.array.grad)):
+        d = rng.standard_normal(x0n.shape)
+        if which == "x":
+            fd = (loss(x0n + eps * d, v0n) - loss(x0n - eps * d, v0n)) / (2 * eps)
+        else:
+            fd = (loss(x0n, v0n + eps * d) - loss(x0n, v0n - eps * d)) / (2 * eps)
+        got = float((grad.double().cpu().numpy() * d).sum())
+        print(f"{kind} {target} {adjoint} d/d{which}: adjoint {got:.6e} fd {fd:.6e}")
+        assert abs(got - fd) / abs(fd) < 2e-3
