@@ -247,7 +247,7 @@ struct ShellArgs {
   double* pdots;       // adjoint: += sum over particle images of (W . G) df/dtheta(|Y|) weight
 };
 
-constexpr int HP_HASH = 4096, HP_PROBES = 16, HP_PROBES_FULL = 2, HP_SB = 16, HP_PER_THREAD = HP_SB * HP_SB * HP_SB / 256;
+constexpr int HP_HASH = 4096, HP_PROBES = 8, HP_PROBES_FULL = 1, HP_SB = 16, HP_PER_THREAD = HP_SB * HP_SB * HP_SB / 256;
 constexpr int HP_MAX_TILES = 4096;
 constexpr unsigned HP_EMPTY = 0xFFFFFFFFu;
 
