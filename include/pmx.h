@@ -31,15 +31,31 @@ typedef struct pmx_plan_t* pmx_plan;
 #define PMX_DRIFT_POST 4u  /* pmx_step: x += v * c_drift_post after the kick */
 #define PMX_PAINT_ATOMIC 8u /* paint with the one-pass global-atomic scatter instead of the sorted
                                owner-computes tiles (plan flag or per-call flag) */
+#define PMX_PLAN_NO_RESERVE 16u /* pmx_plan_create: reserve no paint-sort buffers (pmx_plan_reserve sizes them) */
 
 int pmx_abi_version(void);
 const char* pmx_last_error(void);
 int pmx_fft_supported(int n);
+/* Workgroup cap of the grid-stride kernels (fills, noise, LPT, image transforms, map finalisation,
+ * Born): 0 restores the default 2^20; smaller values make every such kernel loop over its range
+ * (a check that large inputs are covered). Process-wide; not a reference interface. */
+int pmx_set_grid_cap(unsigned max_blocks);
 
-/* Plan for one mesh shape; allocates its workspace (synchronous). */
+/* Plan for one mesh shape; allocates its workspace (synchronous): FFT tables and spectra, density
+ * and force meshes, and, unless flags has PMX_PLAN_NO_RESERVE, the paint-sort buffers for one
+ * particle per cell (an (nx, ny, nz) particle grid). */
 int pmx_plan_create(pmx_plan* plan, int nx, int ny, int nz, unsigned flags);
 int pmx_plan_destroy(pmx_plan plan);
+/* Bytes of device memory the plan holds (everything pmx_plan_create and pmx_plan_reserve allocated). */
 size_t pmx_plan_workspace_bytes(pmx_plan plan);
+/* Grow the plan's buffers so that later calls allocate nothing (synchronous; never shrinks):
+ *   npx, npy, npz > 0: paint-sort buffers for that particle grid (and any grid with no more particles
+ *                      and particle sub-blocks of 8 x 8 x 64);
+ *   acc_count > 0:     the float64 map accumulator of the HEALPix and flat-sky paints, which need
+ *                      12 nside^2 + 4 nside values (HEALPix) or fnx * fny (flat).
+ * Entry points never allocate: a paint whose grid or map exceeds the reservation fails with code 4
+ * and names the size to reserve. No reference counterpart (XLA allocates per call). */
+int pmx_plan_reserve(pmx_plan plan, int npx, int npy, int npz, size_t acc_count);
 /* Entries of the last sorted paint on this plan that fell outside their particle sub-block's
  * 5x5x8-tile window and were counted and placed one by one with global atomics (particles displaced
  * by more than about two tiles). Synchronises the device. */

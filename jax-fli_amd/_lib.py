@@ -29,6 +29,8 @@ SIGNATURES = {
     "pmx_abi_version": (_i, []),
     "pmx_last_error": (_c.c_char_p, []),
     "pmx_fft_supported": (_i, [_i]),
+    "pmx_set_grid_cap": (_i, [_u]),
+    "pmx_plan_reserve": (_i, [_vp, _i, _i, _i, _sz]),
     "pmx_plan_create": (_i, [_c.POINTER(_vp), _i, _i, _i, _u]),
     "pmx_plan_destroy": (_i, [_vp]),
     "pmx_plan_workspace_bytes": (_c.c_size_t, [_vp]),
@@ -90,6 +92,7 @@ DECONV = 1
 DRIFT_PRE = 2
 DRIFT_POST = 4
 PAINT_ATOMIC = 8
+PLAN_NO_RESERVE = 16
 
 
 class PmxError(RuntimeError):
@@ -153,6 +156,23 @@ class Plan:
             check(lib.pmx_plan_create(ctypes.byref(handle), *self.mesh_shape, int(flags)), "pmx_plan_create")
         self.handle = handle
         self._box = None
+        self._grids = set()
+        self._acc = 0
+
+    def reserve(self, npx: int = 0, npy: int = 0, npz: int = 0, acc: int = 0) -> None:
+        """Size the paint-sort buffers for a particle grid and the map accumulator for `acc` values
+        before the first call that needs them (synchronous, once per new grid or larger map): the
+        entry points themselves never allocate."""
+        grid = (int(npx), int(npy), int(npz)) if npx else None
+        acc = int(acc) if acc > self._acc else 0
+        if (grid is None or grid in self._grids) and not acc:
+            return
+        with torch.cuda.device(self.device):
+            check(load().pmx_plan_reserve(self.handle, *(grid or (0, 0, 0)), acc), "pmx_plan_reserve")
+        if grid is not None:
+            self._grids.add(grid)
+        if acc:
+            self._acc = acc
 
     def escapes(self) -> int:
         """Particles the last sorted paint placed one by one (outside their sub-block's tile window;
@@ -211,7 +231,7 @@ def get_plan(mesh_shape, device: torch.device | None = None) -> Plan:
     key = (dev.index, tuple(int(n) for n in mesh_shape))
     p = _plans.get(key)
     if p is None:
-        p = Plan(mesh_shape, dev)
+        p = Plan(mesh_shape, dev, PLAN_NO_RESERVE)
         _plans[key] = p
     return p
 

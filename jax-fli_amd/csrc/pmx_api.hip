@@ -138,14 +138,25 @@ PartArgs part_args(float* x, float* v, int npx, int npy, int npz, int mode, pmx_
   return a;
 }
 
+// The map accumulator is sized by pmx_plan_reserve; entry points only check it.
 int ensure_acc(pmx_plan_t* p, size_t count) {
   const size_t bytes = count * sizeof(double);
   if (p->acc_bytes >= bytes) return 0;
+  return pmx_set_error(4, "map accumulator holds " + std::to_string(p->acc_bytes / sizeof(double)) +
+                              " values, this call needs " + std::to_string(count) +
+                              ": call pmx_plan_reserve with acc_count >= " + std::to_string(count) + " first");
+}
+
+int grow_acc(pmx_plan_t* p, size_t count) {
+  const size_t bytes = count * sizeof(double);
+  if (p->acc_bytes >= bytes) return 0;
   if (p->acc) PMX_HIP(hipFree(p->acc));
+  p->workspace_bytes -= p->acc_bytes;
   p->acc = nullptr;
   p->acc_bytes = 0;
   PMX_HIP(hipMalloc((void**)&p->acc, bytes));
   p->acc_bytes = bytes;
+  p->workspace_bytes += bytes;
   return 0;
 }
 
@@ -277,6 +288,15 @@ int pmx_abi_version(void) { return 1; }
 const char* pmx_last_error(void) { return g_err.c_str(); }
 int pmx_fft_supported(int n) { return pmx_fft_len_ok(n) ? 1 : 0; }
 
+unsigned pmx_grid_cap = 1u << 20;
+
+int pmx_set_grid_cap(unsigned max_blocks) {
+  if (max_blocks == 0) max_blocks = 1u << 20;
+  if (max_blocks > (1u << 20)) return pmx_set_error(1, "grid cap above 2^20 workgroups");
+  pmx_grid_cap = max_blocks;
+  return 0;
+}
+
 int pmx_plan_create(pmx_plan* out, int nx, int ny, int nz, unsigned flags) {
   if (!out) return pmx_set_error(1, "null output handle");
   *out = nullptr;
@@ -331,6 +351,12 @@ int pmx_plan_create(pmx_plan* out, int nx, int ny, int nz, unsigned flags) {
       hipMemset(p->ps_misc, 0, 4 * sizeof(unsigned)) != hipSuccess)
     return fail(pmx_set_error(3, "plan: clearing paint counters failed"));
   p->workspace_bytes = tot;
+  if (!(flags & PMX_PLAN_NO_RESERVE)) {
+    size_t grown = 0;
+    rc = pmx_sort_reserve(p, nx, ny, nz, grown);
+    p->workspace_bytes += grown;
+    if (rc) return fail(rc);
+  }
   std::vector<float> g, w2, dec;
   if ((rc = upload(p->tw_x, twiddles(nx))) || (rc = upload(p->tw_y, twiddles(ny))) || (rc = upload(p->tw_z, twiddles(nz))) ||
       (rc = upload(p->tw_zh, twiddles(nz / 2))))
@@ -342,6 +368,21 @@ int pmx_plan_create(pmx_plan* out, int nx, int ny, int nz, unsigned flags) {
   axis_tables(nz, p->nzh, g, w2, dec);
   if ((rc = upload(p->grad_z, g)) || (rc = upload(p->w2_z, w2)) || (rc = upload(p->dec_z, dec))) return fail(rc);
   *out = p;
+  return 0;
+}
+
+int pmx_plan_reserve(pmx_plan p, int npx, int npy, int npz, size_t acc_count) {
+  PMX_TRY(check_plan(p));
+  if (npx < 0 || npy < 0 || npz < 0) return pmx_set_error(1, "negative particle grid");
+  if (npx > 0 && npy > 0 && npz > 0) {
+    if ((size_t)npx * npy * npz * 3 >= ((size_t)1 << 32))
+      return pmx_set_error(2, "particle grid too large (3 * count must be below 2^32)");
+    size_t grown = 0;
+    const int rc = pmx_sort_reserve(p, npx, npy, npz, grown);
+    p->workspace_bytes += grown;
+    if (rc) return rc;
+  }
+  if (acc_count > 0) PMX_TRY(grow_acc(p, acc_count));
   return 0;
 }
 

@@ -37,10 +37,15 @@ __device__ __forceinline__ void block_sum_to(double v, double* dst) {
   __syncthreads();
 }
 
+}  // namespace
+
+// Workgroup cap of the grid-stride launches (pmx_set_grid_cap; 2^20 by default).
+extern unsigned pmx_grid_cap;
+
+namespace {
 inline int grid_for(size_t n, int block) {
   size_t g = (n + block - 1) / block;
-  const size_t gmax = 1u << 20;
+  const size_t gmax = pmx_grid_cap;
   return (int)(g < 1 ? 1 : (g > gmax ? gmax : g));
 }
-
 }  // namespace

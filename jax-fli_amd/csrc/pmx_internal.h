@@ -135,6 +135,8 @@ int pmx_launch_fill(float* a, size_t n, float v, hipStream_t s);
 // the sorted paint in two halves: sort (count, scan, scatter) then deposit(s) of the sorted particles
 int pmx_paint_sort(pmx_plan_t* p, const PartArgs& a, float cd_pre, const float* w, int wstride, float w0,
                    hipStream_t s);
+// size the sort buffers for a particle grid (synchronous, grow only; adds the new bytes to grown)
+int pmx_sort_reserve(pmx_plan_t* p, int npx, int npy, int npz, size_t& grown);
 // ncomp > 1: component c carries w[p * wstride + c] and goes to mesh + c * mcomp
 int pmx_paint_deposit(pmx_plan_t* p, const PartArgs& a, const float* w, int wstride, float wmul, float w0,
                       int is_signed, float* mesh, int mstride, int accumulate, hipStream_t s, int ncomp = 1,

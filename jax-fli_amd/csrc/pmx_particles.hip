@@ -1356,6 +1356,40 @@ static SortGeom sort_geom(const pmx_plan_t* p, const PartArgs& a, int& ntiles) {
   return g;
 }
 
+// Size the sort buffers for a particle grid of npx x npy x npz (grow only; synchronous). Adds the
+// bytes newly allocated to `grown`.
+int pmx_sort_reserve(pmx_plan_t* p, int npx, int npy, int npz, size_t& grown) {
+  const size_t np = (size_t)npx * npy * npz;
+  const size_t nsb = (size_t)((npx + SB_X - 1) / SB_X) * ((npy + SB_Y - 1) / SB_Y) * ((npz + SB_Z - 1) / SB_Z);
+  if (p->ps_wg_cap < nsb * SORT_SLOTS) {
+    if (p->ps_wgcnt) (void)hipFree(p->ps_wgcnt);
+    grown -= p->ps_wg_cap * sizeof(unsigned);
+    p->ps_wgcnt = nullptr;
+    p->ps_wg_cap = 0;
+    hipError_t e = hipMalloc((void**)&p->ps_wgcnt, nsb * SORT_SLOTS * sizeof(unsigned));
+    if (e != hipSuccess) return pmx_set_error(4, std::string("paint sort counts: ") + hipGetErrorString(e));
+    p->ps_wg_cap = nsb * SORT_SLOTS;
+    grown += p->ps_wg_cap * sizeof(unsigned);
+  }
+  if (p->ps_seg_cap < np) {
+    if (p->ps_seg) (void)hipFree(p->ps_seg);
+    if (p->ps_pack) (void)hipFree(p->ps_pack);
+    grown -= 2 * p->ps_seg_cap * sizeof(unsigned);
+    p->ps_seg = p->ps_pack = nullptr;
+    p->ps_seg_cap = 0;
+    hipError_t e = hipMalloc((void**)&p->ps_seg, np * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->ps_pack, np * sizeof(unsigned));
+    if (e != hipSuccess) {
+      if (p->ps_seg) (void)hipFree(p->ps_seg);
+      p->ps_seg = nullptr;
+      return pmx_set_error(4, std::string("paint sort segments: ") + hipGetErrorString(e));
+    }
+    p->ps_seg_cap = np;
+    grown += 2 * np * sizeof(unsigned);
+  }
+  return 0;
+}
+
 // Sort the particles of `a` by paint tile ([a.v] pre-drift x += v * cd_pre first). The fixed-point
 // scale for the following deposits comes from |w0| or, with w (wstride values per particle), from
 // the largest |w|.
@@ -1365,24 +1399,11 @@ int pmx_paint_sort(pmx_plan_t* p, const PartArgs& a, float cd_pre, const float* 
   const SortGeom g = sort_geom(p, a, ntiles);
   const size_t np = (size_t)a.npx * a.npy * a.npz;
   const size_t nsb = (size_t)g.nbx * g.nby * g.nbz;
-  if (p->ps_wg_cap < nsb * SORT_SLOTS) {
-    if (p->ps_wgcnt) (void)hipFree(p->ps_wgcnt);
-    p->ps_wgcnt = nullptr;
-    p->ps_wg_cap = 0;
-    hipError_t e = hipMalloc((void**)&p->ps_wgcnt, nsb * SORT_SLOTS * sizeof(unsigned));
-    if (e != hipSuccess) return pmx_set_error(4, std::string("paint sort counts: ") + hipGetErrorString(e));
-    p->ps_wg_cap = nsb * SORT_SLOTS;
-  }
-  if (p->ps_seg_cap < np) {
-    if (p->ps_seg) (void)hipFree(p->ps_seg);
-    if (p->ps_pack) (void)hipFree(p->ps_pack);
-    p->ps_seg = p->ps_pack = nullptr;
-    p->ps_seg_cap = 0;
-    hipError_t e = hipMalloc((void**)&p->ps_seg, np * sizeof(unsigned));
-    if (e == hipSuccess) e = hipMalloc((void**)&p->ps_pack, np * sizeof(unsigned));
-    if (e != hipSuccess) return pmx_set_error(4, std::string("paint sort segments: ") + hipGetErrorString(e));
-    p->ps_seg_cap = np;
-  }
+  if (p->ps_wg_cap < nsb * SORT_SLOTS || p->ps_seg_cap < np)
+    return pmx_set_error(4, "paint sort buffers hold " + std::to_string(p->ps_seg_cap) + " particles / " +
+                                std::to_string(p->ps_wg_cap / SORT_SLOTS) + " sub-blocks, this grid needs " +
+                                std::to_string(np) + " / " + std::to_string(nsb) +
+                                ": call pmx_plan_reserve with its particle grid first");
   unsigned* wmax_bits = p->ps_misc;
   unsigned* far = p->ps_misc + 1;
   unsigned* far_last = p->ps_misc + 2;

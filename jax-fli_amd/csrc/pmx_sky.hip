@@ -1031,33 +1031,33 @@ int pmx_born_max_sources() { return kBornMaxSources; }
 
 namespace {
 // One image of the particles as absolute grid positions: X = unwrapped position (Mpc/h) - observer,
-// Y = R X + shift, with sa.v Y += f(|Y|) R v (drift_image), out = (Y + observer) / cell. One thread
-// per particle; float64 arithmetic, rounded once to float32.
+// Y = R X + shift, with sa.v Y += f(|Y|) R v (drift_image), out = (Y + observer) / cell. Grid-stride
+// over particles; float64 arithmetic, rounded once to float32.
 __global__ void __launch_bounds__(256) k_image_positions(PartArgs a, ShellArgs sa, int code, double sh0, double sh1,
                                                          double sh2, float* __restrict__ out) {
   const size_t n = (size_t)a.npx * a.npy * a.npz;
-  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int k = (int)(p % (size_t)a.npz);
-  const size_t q = p / (size_t)a.npz;
-  const int j = (int)(q % (size_t)a.npy), i = (int)(q / (size_t)a.npy);
-  const float3 x = ld3(a.x, (unsigned)p);
-  const bool rel = a.mode == 0;
-  const double X[3] = {((double)x.x + (rel ? (double)(i * sa.rx) : 0.0)) * sa.cell[0] - sa.obs[0],
-                       ((double)x.y + (rel ? (double)(j * sa.ry) : 0.0)) * sa.cell[1] - sa.obs[1],
-                       ((double)x.z + (rel ? (double)(k * sa.rz) : 0.0)) * sa.cell[2] - sa.obs[2]};
-  const Sym R = sym_of(code);
-  const double sh[3] = {sh0, sh1, sh2};
-  double Y[3];
-  for (int c = 0; c < 3; ++c) Y[c] = R.s[c] * X[R.p[c]] + sh[c];
-  if (sa.v) {
-    const float3 v = ld3(sa.v, (unsigned)p);
-    const double V[3] = {v.x, v.y, v.z};
-    double Vt[3];
-    for (int c = 0; c < 3; ++c) Vt[c] = R.s[c] * V[R.p[c]];
-    drift_image(sa, Y, Vt);
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(p % (size_t)a.npz);
+    const size_t q = p / (size_t)a.npz;
+    const int j = (int)(q % (size_t)a.npy), i = (int)(q / (size_t)a.npy);
+    const float3 x = ld3(a.x, (unsigned)p);
+    const bool rel = a.mode == 0;
+    const double X[3] = {((double)x.x + (rel ? (double)(i * sa.rx) : 0.0)) * sa.cell[0] - sa.obs[0],
+                         ((double)x.y + (rel ? (double)(j * sa.ry) : 0.0)) * sa.cell[1] - sa.obs[1],
+                         ((double)x.z + (rel ? (double)(k * sa.rz) : 0.0)) * sa.cell[2] - sa.obs[2]};
+    const Sym R = sym_of(code);
+    const double sh[3] = {sh0, sh1, sh2};
+    double Y[3];
+    for (int c = 0; c < 3; ++c) Y[c] = R.s[c] * X[R.p[c]] + sh[c];
+    if (sa.v) {
+      const float3 v = ld3(sa.v, (unsigned)p);
+      const double V[3] = {v.x, v.y, v.z};
+      double Vt[3];
+      for (int c = 0; c < 3; ++c) Vt[c] = R.s[c] * V[R.p[c]];
+      drift_image(sa, Y, Vt);
+    }
+    for (int c = 0; c < 3; ++c) out[3 * p + c] = (float)((Y[c] + sa.obs[c]) / sa.cell[c]);
   }
-  for (int c = 0; c < 3; ++c) out[3 * p + c] = (float)((Y[c] + sa.obs[c]) / sa.cell[c]);
 }
 
 // Cotangents of k_image_positions for gbar (per particle, of the absolute grid positions out):
@@ -1069,9 +1069,8 @@ __global__ void __launch_bounds__(256) k_image_positions_vjp(PartArgs a, ShellAr
                                                              float* __restrict__ xbar, float* __restrict__ vbar,
                                                              int accumulate) {
   const size_t n = (size_t)a.npx * a.npy * a.npz;
-  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   double pp[kDriftParams] = {0.0, 0.0, 0.0, 0.0};
-  if (p < n) {
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
     const int k = (int)(p % (size_t)a.npz);
     const size_t q = p / (size_t)a.npz;
     const int j = (int)(q % (size_t)a.npy), i = (int)(q / (size_t)a.npy);
@@ -1102,7 +1101,7 @@ __global__ void __launch_bounds__(256) k_image_positions_vjp(PartArgs a, ShellAr
         Yb[c] += cc * Y[c];
         Wb[c] = f * Zb[c];
       }
-      for (int t = 0; t < sa.nparam; ++t) pp[t] = wz * tab_lin(sa.ptab + t * sa.ntab, sa.ntab, 0.0, sa.chi_max, ry);
+      for (int t = 0; t < sa.nparam; ++t) pp[t] += wz * tab_lin(sa.ptab + t * sa.ntab, sa.ntab, 0.0, sa.chi_max, ry);
     }
     double Xb[3] = {0.0, 0.0, 0.0}, Vb[3] = {0.0, 0.0, 0.0};
     for (int c = 0; c < 3; ++c) {

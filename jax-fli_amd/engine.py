@@ -64,6 +64,7 @@ def paint(pos: torch.Tensor, mesh_shape, mode="relative", weights=1.0, out: torc
     if out is None:
         out = torch.empty(tuple(mesh_shape), dtype=torch.float32, device=pos.device)
     w, w0 = _weights(weights, npx * npy * npz)
+    plan.reserve(npx, npy, npz)
     check(load().pmx_paint_cic(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], ptr(w), w0, ptr(out),
                                int(accumulate), stream_ptr()), "pmx_paint_cic")
     return out
@@ -117,6 +118,7 @@ def forces(pos: torch.Tensor, mesh_shape, mode="relative", scale=1.0, deconv=Fal
     plan = get_plan(mesh_shape, pos.device)
     out = torch.empty_like(pos)
     flags = (_lib.DECONV if deconv else 0) | (_lib.PAINT_ATOMIC if atomic_paint else 0)
+    plan.reserve(npx, npy, npz)
     check(load().pmx_forces(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], float(scale), flags, ptr(out),
                             stream_ptr()), "pmx_forces")
     return out
@@ -140,10 +142,12 @@ def step(x: torch.Tensor, v: torch.Tensor, mesh_shape, mode="relative", force_sc
     flags |= _lib.DRIFT_POST if c_drift_post is not None else 0
     if force_mesh is not None:
         _force_mesh_arg(force_mesh, mesh_shape)
+        plan.reserve(npx, npy, npz)
         check(load().pmx_step_keep(plan.handle, ptr(x), ptr(v), npx, npy, npz, MODES[mode], float(force_scale),
                                    float(c_drift_pre or 0.0), float(c_kick), float(c_drift_post or 0.0), flags,
                                    ptr(force_mesh), stream_ptr()), "pmx_step_keep")
         return
+    plan.reserve(npx, npy, npz)
     check(load().pmx_step(plan.handle, ptr(x), ptr(v), npx, npy, npz, MODES[mode], float(force_scale),
                           float(c_drift_pre or 0.0), float(c_kick), float(c_drift_post or 0.0), flags,
                           stream_ptr()), "pmx_step")
@@ -164,6 +168,7 @@ def pgd(x: torch.Tensor, v, mesh_shape, mode="relative", alpha=0.2, kl=0.3, ks=1
     tv = {"x": 0, "v": 1}[target]
     if tv == 1:
         v = _f32(v, "v")
+    plan.reserve(npx, npy, npz)
     check(load().pmx_pgd(plan.handle, ptr(x), ptr(v) if tv == 1 else None, npx, npy, npz, MODES[mode], float(alpha),
                          float(kl), float(ks), tv, float(v_scale), stream_ptr()), "pmx_pgd")
     return x, v
@@ -277,6 +282,7 @@ def paint_healpix(pos: torch.Tensor, mesh_shape, box, observer, rmin, rmax, nsid
         out = torch.empty(12 * nside * nside, dtype=torch.float32, device=pos.device)
     w, w0 = _weights(weights, npx * npy * npz)
     nt, rot, shift, _keep = _tiles(tiles, pos.device)
+    plan.reserve(acc=12 * int(nside) ** 2 + 4 * int(nside))
     check(load().pmx_paint_healpix_tiled(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], ptr(w), w0, dbl3(box),
                                          dbl3(observer), float(rmin), float(rmax), int(nside), _scheme(scheme), nt,
                                          ptr(rot), ptr(shift), ptr(out), int(accumulate), stream_ptr()),
@@ -300,6 +306,7 @@ def paint_healpix_drift(pos: torch.Tensor, vel: torch.Tensor, mesh_shape, box, o
     w, w0 = _weights(weights, npx * npy * npz)
     nt, rot, shift, _keep = _tiles(tiles, pos.device)
     tab = drift["tables"]
+    plan.reserve(acc=12 * int(nside) ** 2 + 4 * int(nside))
     check(load().pmx_paint_healpix_lightcone(
         plan.handle, ptr(pos), ptr(vel), npx, npy, npz, MODES[mode], ptr(w), w0, dbl3(box), dbl3(observer),
         float(rmin), float(rmax), int(nside), _scheme(scheme), nt, ptr(rot), ptr(shift), ptr(tab),
@@ -332,6 +339,7 @@ def paint_healpix_drift_vjp(pos: torch.Tensor, vel: torch.Tensor, map_bar: torch
     w, w0 = _weights(weights, npx * npy * npz)
     nt, rot, shift, _keep = _tiles(tiles, pos.device)
     tab = drift["tables"]
+    plan.reserve(acc=12 * int(nside) ** 2 + 4 * int(nside))
     check(load().pmx_paint_healpix_lightcone_vjp(
         plan.handle, ptr(pos), ptr(vel), npx, npy, npz, MODES[mode], ptr(w), w0, dbl3(box), dbl3(observer),
         float(rmin), float(rmax), int(nside), _scheme(scheme), nt, ptr(rot), ptr(shift), ptr(tab), int(tab.shape[-1]),
@@ -357,6 +365,7 @@ def paint_healpix_vjp(pos: torch.Tensor, map_bar: torch.Tensor, mesh_shape, box,
         accumulate = False
     w, w0 = _weights(weights, npx * npy * npz)
     nt, rot, shift, _keep = _tiles(tiles, pos.device)
+    plan.reserve(acc=12 * int(nside) ** 2 + 4 * int(nside))
     check(load().pmx_paint_healpix_vjp(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], ptr(w), w0, dbl3(box),
                                        dbl3(observer), float(rmin), float(rmax), int(nside), _scheme(scheme), nt,
                                        ptr(rot), ptr(shift), ptr(map_bar), ptr(pos_bar), int(accumulate),
@@ -464,6 +473,7 @@ def paint_flat(pos: torch.Tensor, mesh_shape, flatsky_npix, zlo, zhi, scale, mod
     if out is None:
         out = torch.empty((fnx, fny), dtype=torch.float32, device=pos.device)
     w, w0 = _weights(weights, npx * npy * npz)
+    plan.reserve(acc=fnx * fny)
     check(load().pmx_paint_flat(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], ptr(w), w0, fnx, fny,
                                 float(zlo), float(zhi), float(scale), ptr(out), int(accumulate), stream_ptr()),
           "pmx_paint_flat")
@@ -484,6 +494,7 @@ def paint_flat_vjp(pos: torch.Tensor, map_bar: torch.Tensor, mesh_shape, zlo, zh
         pos_bar = torch.empty_like(pos)
         accumulate = False
     w, w0 = _weights(weights, npx * npy * npz)
+    plan.reserve(acc=fnx * fny)
     check(load().pmx_paint_flat_vjp(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], ptr(w), w0, fnx, fny,
                                     float(zlo), float(zhi), float(scale), ptr(map_bar), ptr(pos_bar), int(accumulate),
                                     stream_ptr()), "pmx_paint_flat_vjp")
@@ -565,6 +576,7 @@ def paint_vjp(pos: torch.Tensor, mesh_bar: torch.Tensor, mode="relative", weight
     w, w0 = _weights(weights, npx * npy * npz)
     pos_bar = torch.empty_like(pos)
     wbar = torch.empty((npx, npy, npz), dtype=torch.float32, device=pos.device) if want_weights else None
+    plan.reserve(npx, npy, npz)
     check(load().pmx_paint_cic_vjp(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], ptr(w), w0, ptr(mesh_bar),
                                    ptr(pos_bar), ptr(wbar), 0, stream_ptr()), "pmx_paint_cic_vjp")
     return pos_bar, wbar
@@ -581,6 +593,7 @@ def readout_vjp(mesh: torch.Tensor, pos: torch.Tensor, out_bar: torch.Tensor, mo
     plan = get_plan(tuple(mesh.shape[:3]), pos.device)
     mesh_bar = torch.empty_like(mesh) if want_mesh else None
     pos_bar = torch.empty_like(pos) if want_pos else None
+    plan.reserve(npx, npy, npz)
     check(load().pmx_readout_cic_vjp(plan.handle, ptr(mesh), ncomp, ptr(pos), npx, npy, npz, MODES[mode],
                                      ptr(out_bar), ptr(mesh_bar), ptr(pos_bar), 0, stream_ptr()),
           "pmx_readout_cic_vjp")
@@ -611,11 +624,13 @@ def forces_vjp(pos: torch.Tensor, mesh_shape, forces_bar: torch.Tensor, mode="re
         accumulate = False
     if force_mesh is not None:
         _force_mesh_arg(force_mesh, mesh_shape)
+        plan.reserve(npx, npy, npz)
         check(load().pmx_forces_vjp_saved(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], float(scale),
                                           _lib.DECONV if deconv else 0, ptr(force_mesh), ptr(forces_bar),
                                           float(bar_scale), ptr(pos_bar), int(accumulate), ptr(dot), stream_ptr()),
               "pmx_forces_vjp_saved")
         return pos_bar
+    plan.reserve(npx, npy, npz)
     check(load().pmx_forces_vjp(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], float(scale),
                                 _lib.DECONV if deconv else 0, ptr(forces_bar), float(bar_scale), ptr(pos_bar),
                                 int(accumulate), ptr(dot), stream_ptr()), "pmx_forces_vjp")
@@ -662,6 +677,7 @@ def pgd_vjp(pos: torch.Tensor, mesh_shape, disp_bar: torch.Tensor, mode="relativ
     if pos_bar is None:
         pos_bar = torch.empty_like(pos)
         accumulate = False
+    plan.reserve(npx, npy, npz)
     check(load().pmx_pgd_vjp(plan.handle, ptr(pos), npx, npy, npz, MODES[mode], float(alpha), float(kl), float(ks),
                              ptr(disp_bar), ptr(pos_bar), int(accumulate), ptr(dot), stream_ptr()), "pmx_pgd_vjp")
     return pos_bar
@@ -685,6 +701,7 @@ def step_reverse_vjp(x, v, x_bar, v_bar, mesh_shape, mode="relative", force_scal
         coef_bar = torch.empty(2, dtype=torch.float64, device=x.device)
     if sharpen is not None:
         alpha, kl, ks, vs = (float(q) for q in sharpen)
+        plan.reserve(npx, npy, npz)
         check(load().pmx_step_reverse_vjp_sharpen(plan.handle, ptr(x), ptr(v), npx, npy, npz, MODES[mode],
                                                   float(force_scale), float(c_kick), float(c_drift),
                                                   _lib.DECONV if deconv else 0, alpha, kl, ks, vs, ptr(x_bar),
@@ -693,11 +710,13 @@ def step_reverse_vjp(x, v, x_bar, v_bar, mesh_shape, mode="relative", force_scal
         return coef_bar
     if force_mesh is not None:
         _force_mesh_arg(force_mesh, mesh_shape)
+        plan.reserve(npx, npy, npz)
         check(load().pmx_step_reverse_vjp_saved(plan.handle, ptr(x), ptr(v), npx, npy, npz, MODES[mode],
                                                 float(force_scale), float(c_kick), float(c_drift),
                                                 _lib.DECONV if deconv else 0, ptr(force_mesh), ptr(x_bar), ptr(v_bar),
                                                 ptr(coef_bar), stream_ptr()), "pmx_step_reverse_vjp_saved")
         return coef_bar
+    plan.reserve(npx, npy, npz)
     check(load().pmx_step_reverse_vjp(plan.handle, ptr(x), ptr(v), npx, npy, npz, MODES[mode], float(force_scale),
                                       float(c_kick), float(c_drift), _lib.DECONV if deconv else 0, ptr(x_bar),
                                       ptr(v_bar), ptr(coef_bar), stream_ptr()), "pmx_step_reverse_vjp")

@@ -27,7 +27,7 @@ class UnfusedPM:
 
     def __init__(self, mesh_shape, device):
         self.mesh_shape = tuple(int(n) for n in mesh_shape)
-        self.plan = _lib.Plan(self.mesh_shape, device, flags=_lib.PAINT_ATOMIC)
+        self.plan = _lib.Plan(self.mesh_shape, device, flags=_lib.PAINT_ATOMIC | _lib.PLAN_NO_RESERVE)
         ks = [2.0 * math.pi * torch.fft.fftfreq(n, d=1.0, device=device, dtype=torch.float64) for n in self.mesh_shape]
         kx, ky, kz = torch.meshgrid(*ks, indexing="ij")
         k2 = kx**2 + ky**2 + kz**2

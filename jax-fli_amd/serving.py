@@ -128,7 +128,7 @@ class _Slot:
 
     def bytes(self) -> int:
         return sum(t.numel() * t.element_size() for t in (self.noise, self.ic, self.x, self.v, self.rho)) + \
-            int(self.plan.workspace_bytes) + 4 * self.n**3  # + the paint sort's index buffer
+            int(self.plan.workspace_bytes)  # the plan's workspace includes its paint-sort buffers
 
 
 class Server:

@@ -199,3 +199,46 @@ def test_telephoto_rotations_and_shifts(pkg):
         pkg.TelephotoInterp(painting=pkg.PaintingOptions(target="density")).init()
     with pytest.warns(UserWarning):
         pkg.TelephotoInterp(painting=pkg.PaintingOptions(target="spherical")).init()
+
+
+def test_resolve_ts_geometry_cases(pkg, cosmo):
+    """The four input forms of the lightcone geometry and its refusals (pm/_resolve_geometry.py:12-114
+    of the reference), with distances from the oracle's independent quadrature."""
+    import torch
+
+    from jax_fli_amd.pm._resolve_geometry import resolve_ts_geometry
+    from jax_fli_amd.painting import PaintingOptions
+
+    p = dict(Omega_c=0.25, Omega_b=0.0486)
+    c, o = cosmo.Cosmology(**p), nbo.Cosmo(**p)
+    f = pkg.DensityField(array=torch.zeros(8, 8, 8), mesh_size=(8, 8, 8), box_size=(1000.0,) * 3)
+    dens, sph = PaintingOptions(target="density"), PaintingOptions(target="spherical")
+    # shell count: far to near, widths = spacing, float32 times
+    ts, r, w, lc = resolve_ts_geometry(c, f, dens, nb_shells=4)
+    assert lc and ts.dtype == np.float32 and np.allclose(r, [437.5, 312.5, 187.5, 62.5]) and np.allclose(w, 125.0)
+    assert np.allclose(o.chi(ts.astype(np.float64)), r, rtol=2e-6)
+    _, _, w2, _ = resolve_ts_geometry(c, f, dens, nb_shells=4, density_widths=30.0)
+    assert np.array_equal(w2, np.full(4, 30.0))
+    # one time: a snapshot, width 0, or the given scalar width
+    ts, r, w, lc = resolve_ts_geometry(c, f, dens, ts=0.5)
+    assert not lc and ts.shape == (1,) and np.allclose(r, o.chi(np.array([0.5])), rtol=1e-8) and w.tolist() == [0.0]
+    ts, r, w, lc = resolve_ts_geometry(c, f, sph, ts=[0.5], density_widths=[[80.0]])
+    assert not lc and w.tolist() == [80.0]
+    # several times: shell centres
+    a = np.array([0.6, 0.8, 0.95])
+    ts, r, w, lc = resolve_ts_geometry(c, f, dens, ts=a)
+    assert lc and np.allclose(r, o.chi(a), rtol=1e-8) and np.allclose(w, pkg.distances(r, 0.0))
+    # (2, n): near/far edges, centre in distance, time back from the centre
+    e = np.array([[0.9, 0.7], [0.8, 0.6]])
+    ts, r, w, lc = resolve_ts_geometry(c, f, sph, ts=e)
+    rn, rf = o.chi(e[0]), o.chi(e[1])
+    assert lc and np.allclose(r, 0.5 * (rn + rf), rtol=1e-8) and np.allclose(w, np.abs(rf - rn), rtol=1e-8)
+    assert np.allclose(o.chi(ts.astype(np.float64)), r, rtol=2e-5)
+    # refusals
+    for kw in (dict(), dict(ts=0.5, nb_shells=3), dict(ts=0.5, density_widths=[1.0, 2.0]),
+               dict(ts=[0.5, 0.6], density_widths=[1.0, 2.0, 3.0]), dict(nb_shells=3, density_widths=[1.0, 2.0]),
+               dict(ts=np.ones((3, 2)))):
+        with pytest.raises(ValueError):
+            resolve_ts_geometry(c, f, dens, **kw)
+    with pytest.raises(ValueError):
+        resolve_ts_geometry(c, f, sph, ts=0.5)
