@@ -186,6 +186,19 @@ int pmx_paint_healpix_tiled(pmx_plan plan, const float* pos, int npx, int npy, i
                             double rmin, double rmax, int nside, int scheme, int ntiles, const int* tile_rot,
                             const double* tile_shift, float* map, int accumulate, void* stream);
 
+/* pmx_paint_healpix_tiled without a plan and without allocating: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). (nx, ny, nz) is the mesh the positions refer to (the plan's mesh
+ * in pmx_paint_healpix_tiled); everything else as there. The accumulator and ring table live in a
+ * static device array of 12 * 512^2 + 4 * 512 float64 values (25.2 MB) that every GPU the library
+ * runs on holds until the process ends: nside must be at most 512 (code 2 otherwise). The array is
+ * shared by all calls on one GPU and cleared by each call, so calls that overlap on one GPU (other
+ * streams, other threads) race: serialise them. */
+int pmx_paint_healpix_noalloc(int nx, int ny, int nz, const float* pos, int npx, int npy, int npz, int mode,
+                              const float* weights, float weight, const double box[3], const double observer[3],
+                              double rmin, double rmax, int nside, int scheme, int ntiles, const int* tile_rot,
+                              const double* tile_shift, float* map, int accumulate, void* stream);
+
 /* pmx_paint_healpix_tiled with the drift to the lightcone of DriftInterp / OnionTiler
  * (pm/interp.py:110-121, 212-504): every image position X_t (observer-centred, Mpc/h) first moves by
  * f (R_t vel) with f = (Gp(a_t) - gp_now) / dGfa(sqrt(a_now a_t)), a_t = a(chi = |X_t|), and the shell

@@ -70,6 +70,8 @@ SIGNATURES = {
     "pmx_lpt1_vjp": (_i, [_vp, _vp, _vp, _i, _i, _i, _f, _f, _vp, _vp]),
     "pmx_paint_healpix_tiled": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _dp, _dp, _d, _d, _i, _i, _i, _vp, _vp, _vp,
                                      _i, _vp]),
+    "pmx_paint_healpix_noalloc": (_i, [_i, _i, _i, _vp, _i, _i, _i, _i, _vp, _f, _dp, _dp, _d, _d, _i, _i, _i, _vp,
+                                       _vp, _vp, _i, _vp]),
     "pmx_paint_healpix_vjp": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _f, _dp, _dp, _d, _d, _i, _i, _i, _vp, _vp, _vp,
                                    _vp, _i, _vp]),
     "pmx_born": (_i, [_vp, _vp, _i, _c.c_int64, _vp, _i, _vp, _i, _vp]),

@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 // launchers from pmx_particles.hip
@@ -867,6 +868,66 @@ int pmx_paint_healpix_tiled(pmx_plan p, const float* pos, int npx, int npy, int 
   PMX_TRY(ensure_acc(p, (size_t)12 * nside * nside + 4 * (size_t)nside));  // maps + ring table
   PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, p);
   return pmx_launch_healpix(p, a, weights, weight, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
+                            tile_shift, map, accumulate, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------ HEALPix shell paint without allocation
+// The map accumulator (12 nside^2 float64 values) and the ring colatitude table (4 nside) of the
+// shell paint live in one file-scope device array per GPU. Host code never passes `scratch` itself
+// (its host-side name is not a device address): each device's address comes from
+// hipGetSymbolAddress on the entry point's first call on that device.
+namespace {
+constexpr int kStaticMaxNside = 512;
+constexpr size_t kStaticScratch = (size_t)12 * kStaticMaxNside * kStaticMaxNside + 4 * (size_t)kStaticMaxNside;
+constexpr int kStaticMaxDevices = 64;
+__device__ __align__(256) double scratch[kStaticScratch];
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kStaticMaxDevices] = {};
+
+int static_scratch(double** out) {
+  int dev = -1;
+  PMX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kStaticMaxDevices) return pmx_set_error(2, "device index beyond the static scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return pmx_set_error(3, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
+int pmx_paint_healpix_noalloc(int nx, int ny, int nz, const float* pos, int npx, int npy, int npz, int mode,
+                              const float* weights, float weight, const double box[3], const double observer[3],
+                              double rmin, double rmax, int nside, int scheme, int ntiles, const int* tile_rot,
+                              const double* tile_shift, float* map, int accumulate, void* stream) {
+  if (!pmx_fft_len_ok(nx) || !pmx_fft_len_ok(ny) || !pmx_fft_len_ok(nz))
+    return pmx_set_error(2, "unsupported mesh size (each axis must be one of 8,16,32,64,96,128,192,256,512,1024)");
+  if (nside < 1 || nside > kStaticMaxNside)
+    return pmx_set_error(2, "pmx_paint_healpix_noalloc: nside must be in [1, " + std::to_string(kStaticMaxNside) +
+                                "] (its static scratch holds " + std::to_string(kStaticScratch) + " doubles)");
+  double* ptr = nullptr;
+  PMX_TRY(static_scratch(&ptr));
+  // a plan-shaped view of the mesh geometry and the scratch: nothing here owns device memory
+  pmx_plan_t v;
+  std::memset(&v, 0, sizeof(v));
+  v.nx = nx;
+  v.ny = ny;
+  v.nz = nz;
+  v.nzh = nz / 2 + 1;
+  v.ncell = (size_t)nx * ny * nz;
+  PMX_HIP(hipGetDevice(&v.device));
+  v.acc = ptr;
+  v.acc_bytes = kStaticScratch * sizeof(double);
+  PMX_TRY(check_shell(&v, pos, npx, npy, npz, mode, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
+                      tile_shift));
+  if (!map) return pmx_set_error(1, "null map");
+  PartArgs a = part_args(const_cast<float*>(pos), nullptr, npx, npy, npz, mode, &v);
+  return pmx_launch_healpix(&v, a, weights, weight, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
                             tile_shift, map, accumulate, (hipStream_t)stream);
 }
 
