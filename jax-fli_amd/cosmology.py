@@ -302,7 +302,7 @@ def eisenstein_hu(cosmo: Cosmology, k):
     k_eq = 7.46e-2 * w_m / th2 / h
     b1 = 0.313 * w_m**-0.419 * (1.0 + 0.607 * w_m**0.674)
     b2 = 0.238 * w_m**0.223
-    z_d = 1291.0 * w_m**0.251 / (1.0 + 0.659 * w_m**0.828) * (1.0 + b1 * w_m**b2)
+    z_d = 1291.0 * w_m**0.251 / (1.0 + 0.659 * w_m**0.828) * (1.0 + b1 * w_b**b2)  # EH98 eq. 4: Omega_b h^2
     z_eq = 2.50e4 * w_m / (th2 * th2)
     R_eq = 31.5 * w_b / (th2 * th2) * (1.0e3 / z_eq)
     R_d = 31.5 * w_b / (th2 * th2) * (1.0e3 / z_d)

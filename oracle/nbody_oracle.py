@@ -10,9 +10,13 @@ src/jax_fli/pm/solvers.py line by line.
 Parity status:
   growth / Gf / gp / dGfa / E          pinned (EdS closed forms; growth-equation identity)
   comoving distance                     pinned (EdS closed form 2 c/H0 (1 - sqrt(a)))
-  EH98 transfer, sigma8 normalisation   pinned on invariants only (T -> 1 as k -> 0, sigma(8) = sigma8
-                                        by independent quadrature); the jax_cosmo discretisation is
-                                        parity unpinned (not in the reference tree)
+  EH98 transfer, sigma8 normalisation   pinned by the paper's own limits and fits: T -> 1 as k -> 0;
+                                        f_b -> 0 reduces it to T0(k, 1, 1) exactly and to the
+                                        independent zero-baryon fit (eqs. 28-31) within 1%; the sound
+                                        horizon at the drag epoch (eqs. 4, 6) within 2% of the fit of
+                                        eq. 26; sigma(8) = sigma8 by adaptive quadrature. The jax_cosmo
+                                        discretisation (its k table, its sigma8 integral) is parity
+                                        unpinned (not in the reference tree)
   time_grid                             pinned (hand enumeration of integrate.py semantics)
   kdk / kkd coefficient algebra         pinned (linear-force runs reproduce D(a)/D(a0) exactly in
                                         EdS and LCDM; KKD step/reverse round trip)
@@ -104,20 +108,29 @@ class Cosmo:
 
 
 # ------------------------------------------------------------------------------ EH98 + sigma8
-def eisenstein_hu(c: Cosmo, k):
-    k = np.atleast_1d(np.asarray(k, dtype=np.float64))
+def eh98_scales(c: Cosmo):
+    """EH98 (Eisenstein & Hu 1998, ApJ 496, 605) eqs. 2-7: k_eq and the Silk scale in h/Mpc, z_eq, the
+    drag redshift z_d, R at z_eq and z_d, and the sound horizon s at the drag epoch in Mpc/h."""
     T2 = (T_CMB / 2.7) ** 2
     wm, wb = c.Omega_m * c.h**2, c.Omega_b * c.h**2
-    fb, fc = c.Omega_b / c.Omega_m, c.Omega_c / c.Omega_m
     keq = 7.46e-2 * wm / T2 / c.h
     zeq = 2.50e4 * wm / T2**2
     b1 = 0.313 * wm**-0.419 * (1 + 0.607 * wm**0.674)
     b2 = 0.238 * wm**0.223
-    zd = 1291 * wm**0.251 / (1 + 0.659 * wm**0.828) * (1 + b1 * wm**b2)
+    zd = 1291 * wm**0.251 / (1 + 0.659 * wm**0.828) * (1 + b1 * wb**b2)  # eq. 4: the last power is of Omega_b h^2
     Req = 31.5 * wb / T2**2 * (1e3 / zeq)
     Rd = 31.5 * wb / T2**2 * (1e3 / zd)
     s = 2 / (3 * keq) * math.sqrt(6 / Req) * math.log((math.sqrt(1 + Rd) + math.sqrt(Rd + Req)) / (1 + math.sqrt(Req)))
     ksilk = 1.6 * wb**0.52 * wm**0.73 * (1 + (10.4 * wm) ** -0.95) / c.h
+    return dict(keq=keq, zeq=zeq, zd=zd, Req=Req, Rd=Rd, s=s, ksilk=ksilk)
+
+
+def eisenstein_hu(c: Cosmo, k):
+    k = np.atleast_1d(np.asarray(k, dtype=np.float64))
+    wm = c.Omega_m * c.h**2
+    fb, fc = c.Omega_b / c.Omega_m, c.Omega_c / c.Omega_m
+    sc = eh98_scales(c)
+    keq, zeq, zd, Rd, s, ksilk = sc["keq"], sc["zeq"], sc["zd"], sc["Rd"], sc["s"], sc["ksilk"]
     q = k / (13.41 * keq)
 
     def T0(alpha, beta):
@@ -172,12 +185,12 @@ def jc_interp(x, xp, fp):
 def color_field(noise, box, ktab, ptab):
     """interpolate_initial_conditions (initial.py:175-182) with the P(k) table given."""
     shape = noise.shape
-    field = np.fft.fftn(np.asarray(noise, dtype=np.float64))
+    field = orc.fftn(np.asarray(noise, dtype=np.float64))
     kv = orc.fftk(shape)
     kmesh = np.sqrt(sum((kk / box[i] * shape[i]) ** 2 for i, kk in enumerate(kv)))
     pkm = jc_interp(kmesh.reshape(-1), np.asarray(ktab), np.asarray(ptab)).reshape(kmesh.shape)
     pkm = pkm * np.prod(shape) / np.prod(box)
-    return np.fft.ifftn(field * np.sqrt(np.maximum(pkm, 0.0))).real
+    return orc.ifftn(field * np.sqrt(np.maximum(pkm, 0.0))).real
 
 
 def lpt1(c: Cosmo, ic, a):

@@ -30,7 +30,60 @@ Parity status per function:
 
 from __future__ import annotations
 
+import ctypes
+import os
+
 import numpy as np
+
+# The per-particle loops (CIC paint/read, HEALPix deposits) also exist in C (oracle/oracle_c.c, the same
+# operations in the same order; built by __graft_entry__.build() into oracle/liboracle_c.so). They are
+# used when that library is present and USE_C is true; tests/test_oracle.py checks both agree.
+USE_C = True
+_C = None
+
+
+def _clib():
+    global _C
+    if _C is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liboracle_c.so")
+        if not os.path.exists(path):
+            _C = False
+        else:
+            lib = ctypes.CDLL(path)
+            i64, vp, d, f = ctypes.c_int64, ctypes.c_void_p, ctypes.c_double, ctypes.c_float
+            for name, wt in (("cic_paint_f64", d), ("cic_paint_f32", f)):
+                fn = getattr(lib, name)
+                fn.argtypes = [vp, i64, i64, i64, ctypes.c_int, i64, i64, i64, vp, wt, vp]
+                fn.restype = ctypes.c_int
+            for name in ("cic_read_f64", "cic_read_f32"):
+                fn = getattr(lib, name)
+                fn.argtypes = [vp, ctypes.c_int, vp, i64, i64, i64, ctypes.c_int, i64, i64, i64, vp]
+                fn.restype = ctypes.c_int
+            lib.healpix_deposit.argtypes = [i64, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, i64, vp]
+            lib.healpix_deposit.restype = ctypes.c_int
+            _C = lib
+    return _C if (USE_C and _C) else None
+
+
+def _cptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _workers() -> int:
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def fftn(a, **kw):
+    """Complex 3D FFT (pocketfft through scipy.fft, threaded over the host cores)."""
+    import scipy.fft
+
+    return scipy.fft.fftn(a, workers=_workers(), **kw)
+
+
+def ifftn(a, **kw):
+    import scipy.fft
+
+    return scipy.fft.ifftn(a, workers=_workers(), **kw)
 
 # ---------------------------------------------------------------------------- CIC
 # jaxpm.painting.cic_paint_dx (called at src/jax_fli/_src/fields/_painting.py:67-73): particle
@@ -70,6 +123,17 @@ def cic_paint(pos, mesh_shape, mode="relative", weight=1.0, dtype=np.float32):
     """CIC scatter; accumulation in float64, result cast to `dtype`."""
     pos = np.asarray(pos)
     np_shape = pos.shape[:3]
+    lib = _clib()
+    if lib is not None and dtype in (np.float32, np.float64):
+        x = np.ascontiguousarray(pos, dtype=dtype).reshape(-1, 3)
+        wa = np.asarray(weight, dtype=dtype).reshape(-1)
+        wp = None if wa.size == 1 else np.ascontiguousarray(np.broadcast_to(wa, (x.shape[0],)))
+        mesh = np.empty(int(np.prod(mesh_shape)), dtype=np.float64)
+        fn = lib.cic_paint_f64 if dtype == np.float64 else lib.cic_paint_f32
+        rc = fn(_cptr(x), *[int(n) for n in np_shape], 0 if mode == "relative" else 1, *[int(n) for n in mesh_shape],
+                None if wp is None else _cptr(wp), float(wa[0]) if wp is None else 0.0, _cptr(mesh))
+        assert rc == 0
+        return mesh.reshape(mesh_shape).astype(dtype)
     base, w0, w1 = _corners(pos, mesh_shape, mode, np_shape, dtype)
     wt = np.broadcast_to(np.asarray(weight, dtype=dtype).reshape(-1), (base.shape[0],))
     mesh = np.zeros(int(np.prod(mesh_shape)), dtype=np.float64)
@@ -95,6 +159,17 @@ def cic_read(mesh, pos, mode="relative", dtype=np.float32):
     m = mesh.reshape(-1, ncomp)
     pos = np.asarray(pos)
     np_shape = pos.shape[:3]
+    lib = _clib()
+    if lib is not None and dtype in (np.float32, np.float64) and ncomp <= 8:
+        x = np.ascontiguousarray(pos, dtype=dtype).reshape(-1, 3)
+        mc = np.ascontiguousarray(m)
+        out = np.empty((x.shape[0], ncomp), dtype=dtype)
+        fn = lib.cic_read_f64 if dtype == np.float64 else lib.cic_read_f32
+        rc = fn(_cptr(mc), int(ncomp), _cptr(x), *[int(n) for n in np_shape], 0 if mode == "relative" else 1,
+                *[int(n) for n in mesh_shape], _cptr(out))
+        assert rc == 0
+        out = out.reshape(tuple(np_shape) + (ncomp,))
+        return out[..., 0] if mesh.ndim == 3 else out
     base, w0, w1 = _corners(pos, mesh_shape, mode, np_shape, dtype)
     nx, ny, nz = mesh_shape
     out = np.zeros((base.shape[0], ncomp), dtype=dtype)
@@ -161,7 +236,7 @@ def force_mesh(delta, deconv=False, dtype=np.float64, pgd=None):
     the PGD filter."""
     mesh_shape = delta.shape
     cdt = np.complex128 if dtype == np.float64 else np.complex64
-    dk = np.fft.fftn(np.asarray(delta, dtype=dtype)).astype(cdt)
+    dk = fftn(np.asarray(delta, dtype=dtype)).astype(cdt)
     kvec = fftk(mesh_shape, np.float64)
     pot = dk * invlaplace_kernel(kvec)
     if pgd is not None:
@@ -177,7 +252,7 @@ def force_mesh(delta, deconv=False, dtype=np.float64, pgd=None):
             sl[i] = n // 2
             g = np.array(np.broadcast_to(g, kvec[i].shape))
             g[tuple(sl)] = 0.0
-        out[..., i] = np.fft.ifftn(-g * pot).real.astype(dtype)
+        out[..., i] = ifftn(-g * pot).real.astype(dtype)
     return out
 
 
@@ -224,7 +299,7 @@ def power(mesh, box_shape, kedges=None):
     kcount = np.bincount(dig, minlength=nb)
     kavg = np.bincount(dig, weights=np.broadcast_to(kmesh, mesh_shape).reshape(-1), minlength=nb)
     kavg = (kavg / np.where(kcount == 0, 1, kcount))[1:-1]
-    meshk = np.fft.fftn(mesh, norm="ortho")
+    meshk = fftn(mesh, norm="ortho")
     mmk = meshk.real**2 + meshk.imag**2
     psum = np.bincount(dig, weights=mmk.reshape(-1), minlength=nb)
     pk = (psum / np.where(kcount > 0, kcount, 1))[1:-1] * np.prod(box / ms)
@@ -417,6 +492,23 @@ def healpix_interp_weights(nside, theta, phi):
     return pix, wgt
 
 
+def healpix_deposit(nside, scheme, wrap_guard, X, r, wt, sel, acc) -> bool:
+    """The per-particle loop of the shell paints in C (True), or False when the C library is absent and
+    the caller runs its own loop."""
+    lib = _clib()
+    if lib is None:
+        return False
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    wt = np.ascontiguousarray(wt, dtype=np.float64)
+    sel = np.ascontiguousarray(sel, dtype=np.int64)
+    assert acc.dtype == np.float64 and acc.flags.c_contiguous
+    rc = lib.healpix_deposit(int(nside), 0 if scheme == "ngp" else 1, int(wrap_guard), _cptr(X), _cptr(r), _cptr(wt),
+                             _cptr(sel), int(sel.size), _cptr(acc))
+    assert rc == 0
+    return True
+
+
 def paint_healpix(pos, mesh_shape, box, observer, rmin, rmax, nside, scheme="bilinear", mode="relative", weight=1.0):
     """Shell paint (build-defined normalisation: weighted count / (pixel solid angle x shell
     volume per steradian)); positions wrapped into the periodic box; float64 geometry."""
@@ -433,6 +525,8 @@ def paint_healpix(pos, mesh_shape, box, observer, rmin, rmax, nside, scheme="bil
     wt = np.broadcast_to(np.asarray(weight, dtype=np.float64).reshape(-1), (g.shape[0],))
     npix = 12 * nside * nside
     acc = np.zeros(npix)
+    if healpix_deposit(nside, scheme, 0, rel, r, wt, sel, acc):
+        sel = ()
     for p in sel:
         X, Y, Z = rel[p]
         phi = np.arctan2(Y, X)

@@ -151,7 +151,10 @@ def paint_healpix_tiled(pos, mesh_shape, box, observer, rmin, rmax, nside, schem
         if drift is not None:
             Xt = Xt + drift.factor(np.sqrt((Xt**2).sum(axis=1)))[:, None] * (V @ cube_symmetry(code).T)
         r = np.sqrt((Xt**2).sum(axis=1))
-        for p in np.nonzero((r > rmin) & (r <= rmax))[0]:
+        sel = np.nonzero((r > rmin) & (r <= rmax))[0]
+        if orc.healpix_deposit(nside, scheme, 1, Xt, r, wt, sel, acc):
+            continue
+        for p in sel:
             theta, phi = _angles(Xt[p])
             if scheme == "ngp":
                 acc[orc.healpix_ang2pix(nside, Xt[p, 2] / r[p], phi)] += wt[p]

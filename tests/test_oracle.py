@@ -186,10 +186,60 @@ def test_growth_equation_identity_lcdm():
 
 
 def test_eh98_and_sigma8():
+    """T -> 1 as k -> 0, and sigma(8 Mpc/h) of the normalised P(k) by scipy's adaptive quadrature (not the
+    Gauss-Legendre rule the normalisation itself uses)."""
+    from scipy import integrate
+
     c = nbo.Cosmo(Omega_c=0.25, Omega_b=0.0486, h=0.6774, n_s=0.9667, sigma8=0.8)
     assert abs(nbo.eisenstein_hu(c, 1e-6)[0] - 1) < 1e-3
-    pk = lambda k: nbo.linear_matter_power(c, k)
-    assert nbo.sigma_R(c, pk) == pytest.approx(0.8, rel=1e-6)
+
+    def integrand(lnk):
+        k = np.exp(lnk)
+        x = 8.0 * k
+        w = 3 * (np.sin(x) - x * np.cos(x)) / x**3
+        return float(k**3 * nbo.linear_matter_power(c, k)[0] * w * w / (2 * np.pi**2))
+
+    s2 = sum(integrate.quad(integrand, lo, hi, limit=400, epsrel=1e-10)[0]
+             for lo, hi in zip(np.linspace(np.log(1e-5), np.log(1e3), 9)[:-1], np.linspace(np.log(1e-5), np.log(1e3), 9)[1:]))
+    assert np.sqrt(s2) == pytest.approx(0.8, rel=1e-6)
+
+
+def _eh98_T0(k, Om, h, alpha_c=1.0, beta_c=1.0):
+    """EH98 eqs. 3, 10, 19, 20: the CDM transfer shape T0(k, alpha_c, beta_c), k in h/Mpc."""
+    th = nbo.T_CMB / 2.7
+    keq = 7.46e-2 * Om * h * h / th**2 / h
+    q = k / (13.41 * keq)
+    L = np.log(np.e + 1.8 * beta_c * q)
+    return L / (L + (14.2 / alpha_c + 386.0 / (1.0 + 69.9 * q**1.08)) * q * q)
+
+
+def test_eh98_zero_baryon_limit():
+    """With f_b -> 0 the full EH98 transfer function must reduce to T0(k, 1, 1) (alpha_c, beta_c -> 1, the
+    baryon part weighted by f_b -> 0), and stay within 1% of the independent zero-baryon fit of EH98
+    eqs. 28-31 (other constants: ln(2e + 1.8 q), 14.2 + 731 / (1 + 62.5 q), q = k Theta^2 / (Omega_m h))."""
+    k = np.logspace(-4, 1, 60)
+    for h in (0.6774, 0.72):
+        c = nbo.Cosmo(Omega_c=0.3, Omega_b=1e-10, h=h)
+        T = nbo.eisenstein_hu(c, k)
+        np.testing.assert_allclose(T, _eh98_T0(k, c.Omega_m, h), rtol=1e-8)
+        q = k * (nbo.T_CMB / 2.7) ** 2 / (c.Omega_m * h)
+        L0 = np.log(2 * np.e + 1.8 * q)
+        Tnw = L0 / (L0 + (14.2 + 731.0 / (1 + 62.5 * q)) * q * q)
+        assert np.max(np.abs(T / Tnw - 1)) < 0.01
+
+
+@pytest.mark.parametrize("Oc,Ob,h", [(0.25, 0.0486, 0.6774), (0.2, 0.04, 0.7), (0.35, 0.06, 0.65)])
+def test_eh98_sound_horizon_and_drag_epoch(Oc, Ob, h):
+    """The sound horizon at the drag epoch (EH98 eq. 6 through the drag redshift of eq. 4) against the
+    paper's independent fit, eq. 26: s = 44.5 ln(9.83 / (Om h^2)) / sqrt(1 + 10 (Ob h^2)^(3/4)) Mpc,
+    stated accurate to 2%. A drag redshift taking Omega_m h^2 in place of Omega_b h^2 in its last factor
+    puts s 5% low here."""
+    c = nbo.Cosmo(Omega_c=Oc, Omega_b=Ob, h=h)
+    wm, wb = c.Omega_m * h * h, Ob * h * h
+    s_fit = 44.5 * np.log(9.83 / wm) / np.sqrt(1 + 10 * wb**0.75)
+    sc = nbo.eh98_scales(c)
+    assert sc["s"] / h == pytest.approx(s_fit, rel=0.02)
+    assert 950 < sc["zd"] < 1100 and sc["zd"] < sc["zeq"]
 
 
 # ---------------------------------------------------------------- time grid + solver algebra
@@ -326,3 +376,58 @@ def test_sharpened_kkd_is_reversible_in_float64():
     vr = v1 - corr(xr) / cd
     vr = vr - force(xr) * ck
     assert np.abs(xr - x0).max() < 1e-12 and np.abs(vr - v0).max() < 1e-12
+
+
+@pytest.mark.parametrize("mode", ["relative", "absolute"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_c_loops_equal_numpy_loops(mode, dtype):
+    """oracle_c.c performs the NumPy loops' operations in the same order: bitwise equal CIC paint and
+    readout (scalar and per-particle weights, 1 and 3 components, sub-lattice and absolute positions)."""
+    if orc._clib() is None:
+        pytest.skip("oracle/liboracle_c.so not built")
+    rng = np.random.default_rng(5)
+    n = 12
+    x = rng.standard_normal((n, n, n, 3)) * 1.7
+    mesh_shape = (2 * n, 2 * n, 3 * n) if mode == "relative" else (n + 4, n, n + 2)
+    if mode == "absolute":
+        x = x + rng.random((n, n, n, 3)) * np.array(mesh_shape)
+    x = x.astype(dtype)
+    w = rng.random((n, n, n)).astype(dtype)
+    mesh = rng.standard_normal(mesh_shape + (3,)).astype(dtype)
+    got, ref = [], []
+    for use_c, out in ((True, got), (False, ref)):
+        orc.USE_C = use_c
+        try:
+            out.append(orc.cic_paint(x, mesh_shape, mode, weight=w, dtype=dtype))
+            out.append(orc.cic_paint(x, mesh_shape, mode, weight=0.7, dtype=dtype))
+            out.append(orc.cic_read(mesh, x, mode, dtype=dtype))
+            out.append(orc.cic_read(mesh[..., 1], x, mode, dtype=dtype))
+        finally:
+            orc.USE_C = True
+    for a, b in zip(got, ref):
+        assert a.dtype == b.dtype and np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("scheme", ["ngp", "bilinear"])
+def test_c_healpix_deposits_equal_python_loop(scheme):
+    """The C shell-paint loop against the Python one (both polar caps and the equator), with rotated images."""
+    from oracle import sky_oracle as so
+
+    if orc._clib() is None:
+        pytest.skip("oracle/liboracle_c.so not built")
+    rng = np.random.default_rng(8)
+    n, nside = 8, 8
+    pos = (rng.standard_normal((n, n, n, 3)) * 2.0).astype(np.float32)
+    tiles = [(0, (0.0, 0.0, 0.0)), (7, (100.0, 0.0, 0.0)), (29, (0.0, -100.0, 100.0))]
+    outs = []
+    for use_c in (True, False):
+        orc.USE_C = use_c
+        try:
+            outs.append((orc.paint_healpix(pos, (n,) * 3, (100.0,) * 3, (50.0, 47.0, 52.0), 5.0, 60.0, nside, scheme),
+                         so.paint_healpix_tiled(pos, (n,) * 3, (100.0,) * 3, (50.0, 47.0, 52.0), 20.0, 150.0, nside,
+                                                scheme, tiles=tiles)))
+        finally:
+            orc.USE_C = True
+    for a, b in zip(*outs):
+        assert np.sum(b) > 0
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12 * float(np.abs(b).max()))
