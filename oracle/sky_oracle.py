@@ -134,11 +134,26 @@ class Drift:
         return dfda * self._a_of_chi(r, 1)
 
 
+def shell_members(pos, mesh_shape, box, observer, rmin, rmax, mode="relative", tiles=None):
+    """Per image (in the order of `tiles`), the indices of the particles whose image lies in the shell
+    rmin < r <= rmax: the shell membership that paint_healpix_tiled's `members` can hold fixed."""
+    X, _ = _centred(pos, mesh_shape, box, observer, mode)
+    tiles = [(0, np.zeros(3))] if tiles is None else [(int(c), np.asarray(s, dtype=np.float64)) for c, s in tiles]
+    out = []
+    for code, sh in tiles:
+        r = np.sqrt(((X @ cube_symmetry(code).T + sh) ** 2).sum(axis=1))
+        out.append(np.nonzero((r > rmin) & (r <= rmax))[0])
+    return out
+
+
 def paint_healpix_tiled(pos, mesh_shape, box, observer, rmin, rmax, nside, scheme="bilinear", mode="relative",
-                        weight=1.0, tiles=None, vel=None, drift=None):
+                        weight=1.0, tiles=None, vel=None, drift=None, members=None):
     """Sum over images (code, shift) of the shell paint of R_code X + shift, X the observer-centred,
     box-wrapped position; tiles=None: the box alone. With drift (a Drift) and vel, every image moves
-    by drift.factor(|X_t|) (R_code vel) first. Normalisation as pm_oracle.paint_healpix."""
+    by drift.factor(|X_t|) (R_code vel) first. Normalisation as pm_oracle.paint_healpix. members (from
+    shell_members): paint exactly these particles of each image, whatever their radius now -- the
+    paint with its shell membership frozen, whose derivative is the one an adjoint that treats
+    membership as a step (zero derivative) computes."""
     X, _ = _centred(pos, mesh_shape, box, observer, mode)
     n = X.shape[0]
     wt = np.broadcast_to(np.asarray(weight, dtype=np.float64).reshape(-1), (n,))
@@ -146,12 +161,12 @@ def paint_healpix_tiled(pos, mesh_shape, box, observer, rmin, rmax, nside, schem
     V = None if vel is None else np.asarray(vel, dtype=np.float64).reshape(-1, 3)
     npix = 12 * nside * nside
     acc = np.zeros(npix)
-    for code, sh in tiles:
+    for ti, (code, sh) in enumerate(tiles):
         Xt = X @ cube_symmetry(code).T + sh
         if drift is not None:
             Xt = Xt + drift.factor(np.sqrt((Xt**2).sum(axis=1)))[:, None] * (V @ cube_symmetry(code).T)
         r = np.sqrt((Xt**2).sum(axis=1))
-        sel = np.nonzero((r > rmin) & (r <= rmax))[0]
+        sel = np.nonzero((r > rmin) & (r <= rmax))[0] if members is None else members[ti]
         if orc.healpix_deposit(nside, scheme, 1, Xt, r, wt, sel, acc):
             continue
         for p in sel:
