@@ -356,25 +356,32 @@ def bench_large(pkg, dev, fd_dirs=2, sweep=(1e-2, 3e-3, 1e-3, 3e-4)):
                     "Born; the VJP runs the reverse adjoint with the shell-paint and Born adjoints"}
 
 
-def bench_serving(pkg, dev, count=512):
+def bench_serving(pkg, dev, count=512, slots=None):
     """Serving workload: `count` forward-simulation requests, mesh uniform over {64, 96, 128, 192}^3
     (box 3.9 N Mpc/h), own seed, Omega_c ~ U(0.2, 0.35), sigma8 ~ U(0.7, 0.9), 1LPT + KDK dt0 = 0.09 to
-    a = 1 (10 steps), float32. Reports the saturated throughput (all requests at t = 0), the same
-    requests one at a time (no batching), and p50/p99 latency of a Poisson stream offered at 70% of the
-    saturated throughput."""
-    srv = pkg.serving.Server(device=dev, max_active=48)
-    srv.prewarm({64: 12, 96: 12, 128: 12, 192: 12})
-    burst = pkg.serving.poisson_requests(count, float("inf"), seed=11)
-    srv.serve(burst[:16])  # warm-up
-    _, sat = srv.serve(burst)
-    one = pkg.serving.Server(device=dev, max_active=1)
-    one._pool, one._slot_bytes = srv._pool, srv._slot_bytes
-    _, serial = one.serve(burst)
-    rate = 0.7 * sat["throughput_req_s"]
-    _, pois = srv.serve(pkg.serving.poisson_requests(count, rate, seed=12))
-    r3 = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
+    a = 1 (10 steps), float32, through the native server (batched graph-replayed steps per size class).
+    Reports the saturated throughput (all requests at t = 0), the same requests one at a time (each
+    submitted when the previous one is done), and p50/p99 latency of a Poisson stream offered at 70% of
+    the saturated throughput."""
+    meshes = (64, 96, 128, 192)
+    slots = slots or {64: 32, 96: 24, 128: 16, 192: 8}
+    srv = pkg.serving.Server(device=dev, meshes=meshes, slots=slots)
+    try:
+        burst = pkg.serving.poisson_requests(count, float("inf"), seed=11)
+        srv.serve(burst[:32])  # warm-up
+        _, sat = srv.serve(burst)
+        _, serial = srv.serve(burst, sequential=True)
+        rate = 0.7 * sat["throughput_req_s"]
+        _, pois = srv.serve(pkg.serving.poisson_requests(count, rate, seed=12))
+        device_gb = srv.stats()["device_bytes"] / 1e9
+    finally:
+        srv.close()
+    r3 = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items() if k != "device_bytes"}
     return {"workload": f"{count} requests, mesh uniform over 64/96/128/192^3, box 3.9 N Mpc/h, 1LPT + KDK "
                         "dt0=0.09 a=0.1->1, per-request seed and (Omega_c, sigma8)",
+            "server": {"slots": {str(m): int(slots[m]) for m in meshes}, "device_GB": round(device_gb, 2),
+                       "form": "native server: one batched launch per step phase over a size class's live slots, "
+                               "replayed from a captured HIP graph; host preparation on 4 native threads"},
             "saturated": r3(sat), "one_at_a_time": r3(serial),
             "poisson": {"offered_req_s": round(rate, 1), **r3(pois)}}
 

@@ -407,6 +407,58 @@ int pmx_born_flat_vjp(pmx_plan plan, const float* maps, int nshell, int ny, int 
                       const double* scales, const float* kappa_bar, float* maps_bar, int accumulate, double* dots,
                       void* stream);
 
+/* ------------------------------------------------------------------------------------------------
+ * Forward-simulation server: continuous batching of seeded forward simulations of mixed mesh sizes
+ * and cosmologies on one GPU. In place of the reference's one-simulation-per-call loop
+ * (run_simulations, src/jax_fli/scripts/fli_simulate.py:512-585, each call lpt + nbody with
+ * EfficientDriftDoubleKick, pm/nbody.py:28-171). A request: white noise from `seed` (Philox, as
+ * pmx_white_noise), EH98 P(k) of its flat LCDM cosmology (Omega_k = 0, w0 = -1, wa = 0) normalised to
+ * sigma8 colouring it in a cubic box, first-order LPT at a0, the KDK loop with dt0 to a1; results: the
+ * final GRID_RELATIVE displacements (mesh^3 x 3 floats) and the painted density (mesh^3 floats), written
+ * into the caller's device buffers. One size class per mesh size, each with `slots` concurrent requests
+ * (0: as many as hbm_budget / nclass allows, at most 64); a request is admitted when a slot of its size
+ * is free, in arrival order per size. Steps of all live slots of a class run as one batched launch per
+ * phase, replayed from a captured HIP graph. Host preparation runs on prep_threads threads (0: 4). */
+typedef struct pmx_server_t* pmx_server;
+typedef struct {
+  int mesh;        /* n: n^3 particles on an n^3 mesh, one of the server's sizes */
+  uint64_t seed;   /* white-noise seed */
+  double Omega_c, Omega_b, h, n_s, sigma8;
+  double box;      /* Mpc/h */
+  double a0, a1, dt0;
+} pmx_request_t;
+typedef struct {
+  uint64_t submitted, completed, failed, running, queued;
+  int peak_running;                        /* most slots held at once since the server was created */
+  double p50_ms, p99_ms, mean_ms, max_ms;  /* submit -> completion latency of completed requests */
+  double wall_s;                           /* first submission to last completion */
+  double throughput_req_s;                 /* completed / wall_s */
+  size_t device_bytes;                     /* the server's device memory (slot arenas and tables) */
+} pmx_server_stats_t;
+#define PMX_REQ_QUEUED 0
+#define PMX_REQ_RUNNING 1
+#define PMX_REQ_DONE 2
+#define PMX_REQ_FAILED (-1)
+/* Allocates every class's slots and captures its step graph (synchronous); starts the server threads. */
+int pmx_server_create(pmx_server* server, const int* meshes, const int* slots, int nclass, size_t hbm_budget,
+                      int prep_threads);
+/* Slots of the class for `mesh` and the device bytes of one slot. */
+int pmx_server_class(pmx_server server, int mesh, int* slots, size_t* slot_bytes);
+/* Queue a request (returns at once). displacement / density: device buffers the result is written
+ * into; valid once the request is PMX_REQ_DONE. */
+int pmx_server_submit(pmx_server server, const pmx_request_t* request, float* displacement, float* density,
+                      uint64_t* id);
+/* State of a request (PMX_REQ_*), without waiting. */
+int pmx_server_poll(pmx_server server, uint64_t id, int* state);
+/* Block until the request is done (timeout_s <= 0: no limit; code 5 on timeout). */
+int pmx_server_wait(pmx_server server, uint64_t id, double timeout_s);
+/* Submission, admission and completion times of a request (seconds since the server was created). */
+int pmx_server_request_times(pmx_server server, uint64_t id, double times[3]);
+/* Counts and latency percentiles over the requests with id >= since_id. */
+int pmx_server_stats(pmx_server server, uint64_t since_id, pmx_server_stats_t* stats);
+/* Finishes the admitted requests, stops the threads and frees everything (synchronous). */
+int pmx_server_destroy(pmx_server server);
+
 #ifdef __cplusplus
 }
 #endif

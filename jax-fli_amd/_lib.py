@@ -88,6 +88,14 @@ SIGNATURES = {
     "pmx_born_flat": (_i, [_vp, _vp, _i, _i, _i, _vp, _i, _vp, _vp, _i, _vp]),
     "pmx_born_flat_vjp": (_i, [_vp, _vp, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _vp, _vp]),
     "pmx_born_vjp": (_i, [_vp, _vp, _i, _c.c_int64, _vp, _i, _vp, _vp, _i, _vp, _vp]),
+    "pmx_server_create": (_i, [_c.POINTER(_vp), _vp, _vp, _i, _sz, _i]),
+    "pmx_server_class": (_i, [_vp, _i, _vp, _vp]),
+    "pmx_server_submit": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "pmx_server_poll": (_i, [_vp, _c.c_uint64, _vp]),
+    "pmx_server_wait": (_i, [_vp, _c.c_uint64, _d]),
+    "pmx_server_request_times": (_i, [_vp, _c.c_uint64, _vp]),
+    "pmx_server_stats": (_i, [_vp, _c.c_uint64, _vp]),
+    "pmx_server_destroy": (_i, [_vp]),
 }
 
 DECONV = 1

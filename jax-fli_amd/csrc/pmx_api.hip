@@ -285,6 +285,14 @@ int pmx_check_launch(const char* what) {
 
 extern "C" {
 
+}  // extern "C"
+
+int pmx_pot_force_mesh(pmx_plan_t* p, const float* src, float force_scale, hipStream_t s) {
+  return pot_force_mesh_from(p, src, force_scale, 0u, p->fmesh, s);
+}
+
+extern "C" {
+
 int pmx_abi_version(void) { return 1; }
 const char* pmx_last_error(void) { return g_err.c_str(); }
 int pmx_fft_supported(int n) { return pmx_fft_len_ok(n) ? 1 : 0; }
@@ -305,6 +313,7 @@ int pmx_plan_create(pmx_plan* out, int nx, int ny, int nz, unsigned flags) {
     return pmx_set_error(2, "unsupported mesh size (each axis must be one of 8,16,32,64,96,128,192,256,512,1024)");
   pmx_plan_t* p = new pmx_plan_t();
   std::memset(p, 0, sizeof(*p));
+  p->nslots = 1;
   p->nx = nx;
   p->ny = ny;
   p->nz = nz;
@@ -387,8 +396,177 @@ int pmx_plan_reserve(pmx_plan p, int npx, int npy, int npz, size_t acc_count) {
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+// Regions of one slot of a batched plan (byte offsets, each 256-byte aligned): density, spectra A, B,
+// C (3), D (3), force mesh, the colouring's k tables (x, y, z), paint-sort counters, offsets, cursors,
+// misc, scale, faces, window counts, segments, packs, and the caller's region.
+struct BatchLayout {
+  size_t o[20];
+  size_t stride;
+};
+BatchLayout batch_layout(int n, size_t extra) {
+  pmx_plan_t t;
+  std::memset(&t, 0, sizeof(t));
+  t.nx = t.ny = t.nz = n;
+  int sx, sy, sz, ntiles;
+  pmx_paint_tiles(&t, sx, sy, sz, ntiles);
+  const size_t ncell = (size_t)n * n * n, nzh = (size_t)n / 2 + 1, nspec = nzh * n * n,
+               nspecd = (size_t)((nzh + 7) & ~(size_t)7) * n * n;
+  const size_t bytes[20] = {ncell * 4, nspec * 8, nspec * 8, 3 * nspec * 8, 3 * nspecd * 8, 3 * ncell * 4,
+                            (size_t)n * 8, (size_t)n * 8, nzh * 8, (size_t)ntiles * 4, (size_t)ntiles * 4,
+                            (size_t)ntiles * 4, (size_t)ntiles * 4, 16, 8,
+                            (size_t)ntiles * pmx_paint_face_cells(&t) * 3 * 4, pmx_sort_wg_entries(n, n, n) * 4,
+                            ncell * 4, ncell * 4, extra};
+  BatchLayout L;
+  size_t off = 0;
+  for (int i = 0; i < 20; ++i) {
+    L.o[i] = off;
+    off += (bytes[i] + 255) & ~(size_t)255;
+  }
+  L.stride = off;
+  return L;
+}
+}  // namespace
+
+size_t pmx_batched_slot_bytes(int n, size_t extra) { return batch_layout(n, extra).stride; }
+
+// A batched plan for `nslots` cubic meshes of side n (the request server's size classes): the FFT
+// and k-space tables are allocated once and shared; every slot gets, in one arena, the work buffers
+// of an ordinary plan (density, spectra, force mesh, k tables for the colouring, paint sort for n^3
+// particles) plus `extra` bytes for the caller, each region 256-byte aligned, slots `stride` bytes
+// apart. *extra_off is the offset of the caller's region inside a slot. The plan's pointers are slot
+// 0's; pmx_slot_view gives an ordinary (nslots = 1) view of one slot.
+int pmx_plan_create_batched(pmx_plan_t** out, int n, int nslots, size_t extra, size_t* extra_off) {
+  *out = nullptr;
+  if (!pmx_fft_len_ok(n)) return pmx_set_error(2, "unsupported mesh size");
+  if (nslots < 1) return pmx_set_error(1, "need at least one slot");
+  pmx_plan_t* p = new pmx_plan_t();
+  std::memset(p, 0, sizeof(*p));
+  p->nx = p->ny = p->nz = n;
+  p->nzh = n / 2 + 1;
+  p->ncell = (size_t)n * n * n;
+  p->nspec = (size_t)p->nzh * n * n;
+  p->nzhp = (p->nzh + 7) & ~7;
+  p->nspecd = (size_t)p->nzhp * n * n;
+  auto fail = [&](int rc) {
+    pmx_plan_destroy(p);
+    return rc;
+  };
+  hipError_t e = hipGetDevice(&p->device);
+  if (e != hipSuccess) {
+    delete p;
+    return pmx_set_error(3, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  }
+  size_t tot = 0;
+  int rc = 0;
+  if ((rc = dev_alloc(&p->tw_x, n, tot)) || (rc = dev_alloc(&p->tw_y, n, tot)) || (rc = dev_alloc(&p->tw_z, n, tot)) ||
+      (rc = dev_alloc(&p->tw_zh, n / 2, tot)) || (rc = dev_alloc(&p->grad_x, n, tot)) ||
+      (rc = dev_alloc(&p->grad_y, n, tot)) || (rc = dev_alloc(&p->grad_z, p->nzh, tot)) ||
+      (rc = dev_alloc(&p->w2_x, n, tot)) || (rc = dev_alloc(&p->w2_y, n, tot)) || (rc = dev_alloc(&p->w2_z, p->nzh, tot)) ||
+      (rc = dev_alloc(&p->dec_x, n, tot)) || (rc = dev_alloc(&p->dec_y, n, tot)) ||
+      (rc = dev_alloc(&p->dec_z, p->nzh, tot)) || (rc = dev_alloc(&p->red, (size_t)kRedMax, tot)))
+    return fail(rc);
+  // one slot's regions
+  const BatchLayout L = batch_layout(n, extra);
+  const size_t o_rho = L.o[0], o_A = L.o[1], o_B = L.o[2], o_C = L.o[3], o_D = L.o[4], o_F = L.o[5], o_kx = L.o[6],
+               o_ky = L.o[7], o_kz = L.o[8], o_cnt = L.o[9], o_tcnt = L.o[10], o_off = L.o[11], o_cur = L.o[12],
+               o_misc = L.o[13], o_scale = L.o[14], o_faces = L.o[15], o_wg = L.o[16], o_seg = L.o[17],
+               o_pack = L.o[18];
+  *extra_off = L.o[19];
+  const size_t stride = L.stride;
+  char* arena = nullptr;
+  e = hipMalloc((void**)&arena, stride * (size_t)nslots);
+  if (e != hipSuccess) return fail(pmx_set_error(4, std::string("batched plan arena: ") + hipGetErrorString(e)));
+  p->arena = arena;
+  tot += stride * (size_t)nslots;
+  p->rho = reinterpret_cast<float*>(arena + o_rho);
+  p->specA = reinterpret_cast<float2*>(arena + o_A);
+  p->specB = reinterpret_cast<float2*>(arena + o_B);
+  p->specC = reinterpret_cast<float2*>(arena + o_C);
+  p->specD = reinterpret_cast<float2*>(arena + o_D);
+  p->fmesh = reinterpret_cast<float*>(arena + o_F);
+  p->kphys_x = reinterpret_cast<double*>(arena + o_kx);
+  p->kphys_y = reinterpret_cast<double*>(arena + o_ky);
+  p->kphys_z = reinterpret_cast<double*>(arena + o_kz);
+  p->ps_count = reinterpret_cast<unsigned*>(arena + o_cnt);
+  p->ps_tcount = reinterpret_cast<unsigned*>(arena + o_tcnt);
+  p->ps_offset = reinterpret_cast<unsigned*>(arena + o_off);
+  p->ps_cursor = reinterpret_cast<unsigned*>(arena + o_cur);
+  p->ps_misc = reinterpret_cast<unsigned*>(arena + o_misc);
+  p->ps_scale = reinterpret_cast<float*>(arena + o_scale);
+  p->ps_faces = reinterpret_cast<float*>(arena + o_faces);
+  p->ps_wgcnt = reinterpret_cast<unsigned*>(arena + o_wg);
+  p->ps_seg = reinterpret_cast<unsigned*>(arena + o_seg);
+  p->ps_pack = reinterpret_cast<unsigned*>(arena + o_pack);
+  p->ps_wg_cap = pmx_sort_wg_entries(n, n, n);
+  p->ps_seg_cap = p->ncell;
+  p->nslots = nslots;
+  p->slots.stride = stride;
+  p->workspace_bytes = tot;
+  // the count pass accumulates into zeroed counters (its deposit pass re-zeroes them); misc holds the
+  // far-particle counters
+  if (hipMemset(arena, 0, stride * (size_t)nslots) != hipSuccess)
+    return fail(pmx_set_error(3, "batched plan: clearing the arena failed"));
+  std::vector<float> g, w2, dec;
+  if ((rc = upload(p->tw_x, twiddles(n))) || (rc = upload(p->tw_y, twiddles(n))) || (rc = upload(p->tw_z, twiddles(n))) ||
+      (rc = upload(p->tw_zh, twiddles(n / 2))))
+    return fail(rc);
+  axis_tables(n, n, g, w2, dec);
+  if ((rc = upload(p->grad_x, g)) || (rc = upload(p->w2_x, w2)) || (rc = upload(p->dec_x, dec)) ||
+      (rc = upload(p->grad_y, g)) || (rc = upload(p->w2_y, w2)) || (rc = upload(p->dec_y, dec)))
+    return fail(rc);
+  axis_tables(n, p->nzh, g, w2, dec);
+  if ((rc = upload(p->grad_z, g)) || (rc = upload(p->w2_z, w2)) || (rc = upload(p->dec_z, dec))) return fail(rc);
+  *out = p;
+  return 0;
+}
+
+// An ordinary plan (nslots = 1) over slot z of a batched plan: the same shared tables, the slot's
+// buffers. It owns nothing.
+pmx_plan_t pmx_slot_view(const pmx_plan_t* b, int z) {
+  pmx_plan_t v = *b;
+  const size_t st = b->slots.stride;
+  v.nslots = 1;
+  v.slots = Slots{};
+  v.arena = nullptr;
+  v.marks = nullptr;
+  v.timing = 0;
+  v.rho = at_slot(b->rho, st, z);
+  v.specA = at_slot(b->specA, st, z);
+  v.specB = at_slot(b->specB, st, z);
+  v.specC = at_slot(b->specC, st, z);
+  v.specD = at_slot(b->specD, st, z);
+  v.fmesh = at_slot(b->fmesh, st, z);
+  v.kphys_x = at_slot(b->kphys_x, st, z);
+  v.kphys_y = at_slot(b->kphys_y, st, z);
+  v.kphys_z = at_slot(b->kphys_z, st, z);
+  v.ps_count = at_slot(b->ps_count, st, z);
+  v.ps_tcount = at_slot(b->ps_tcount, st, z);
+  v.ps_offset = at_slot(b->ps_offset, st, z);
+  v.ps_cursor = at_slot(b->ps_cursor, st, z);
+  v.ps_misc = at_slot(b->ps_misc, st, z);
+  v.ps_scale = at_slot(b->ps_scale, st, z);
+  v.ps_faces = at_slot(b->ps_faces, st, z);
+  v.ps_wgcnt = at_slot(b->ps_wgcnt, st, z);
+  v.ps_seg = at_slot(b->ps_seg, st, z);
+  v.ps_pack = at_slot(b->ps_pack, st, z);
+  return v;
+}
+
+extern "C" {
+
 int pmx_plan_destroy(pmx_plan p) {
   if (!p) return 0;
+  if (p->arena) {  // a batched plan: the shared tables and the arena (slot buffers point into it)
+    void* bufs[] = {p->tw_x, p->tw_y, p->tw_z, p->tw_zh, p->grad_x, p->grad_y, p->grad_z, p->w2_x, p->w2_y,
+                    p->w2_z, p->dec_x, p->dec_y, p->dec_z, p->red, p->arena};
+    for (void* b : bufs)
+      if (b) (void)hipFree(b);
+    delete p;
+    return 0;
+  }
   void* bufs[] = {p->tw_x, p->tw_y, p->tw_z, p->tw_zh, p->grad_x, p->grad_y, p->grad_z, p->w2_x, p->w2_y, p->w2_z,
                   p->dec_x, p->dec_y, p->dec_z, p->rho, p->specA, p->specB, p->specC, p->specD, p->fmesh, p->acc,
                   p->kphys_x, p->kphys_y, p->kphys_z, p->ps_count, p->ps_tcount, p->ps_offset, p->ps_cursor, p->ps_wgcnt,
@@ -915,6 +1093,7 @@ int pmx_paint_healpix_noalloc(int nx, int ny, int nz, const float* pos, int npx,
   // a plan-shaped view of the mesh geometry and the scratch: nothing here owns device memory
   pmx_plan_t v;
   std::memset(&v, 0, sizeof(v));
+  v.nslots = 1;
   v.nx = nx;
   v.ny = ny;
   v.nz = nz;

@@ -173,9 +173,13 @@ template <int N> constexpr bool res_in_b() { return (FP<N>::S & 1) != 0; }
 template <int NZ>
 __global__ void __launch_bounds__(256) k_r2c_z(const float* __restrict__ in, float2* __restrict__ outA,
                                                int NX, int NY, const float2* __restrict__ twM,
-                                               const float2* __restrict__ twN) {
+                                               const float2* __restrict__ twN, Slots sl) {
   constexpr int M = NZ / 2, T = TPL<M>(), L = LPW<M>(), LS = M + 1, NT = L * T;
   __shared__ float2 s0[L * LS], s1[L * LS];
+  const unsigned zs = blockIdx.z;
+  if (!slot_live(sl, zs)) return;
+  in = at_slot(in, sl.stride, zs);
+  outA = at_slot(outA, sl.stride, zs);
   const int x = blockIdx.y, y0 = (int)xcd_x() * L, tid = threadIdx.x;
   const int nl = min(L, NY - y0);
   const float2* src = reinterpret_cast<const float2*>(in + ((size_t)x * NY + y0) * NZ);
@@ -212,11 +216,15 @@ __global__ void __launch_bounds__(256) k_r2c_z(const float* __restrict__ in, flo
 template <int N, bool INV>
 __global__ void __launch_bounds__(256) k_c2c_lines(const float2* __restrict__ in, float2* __restrict__ out,
                                                    long isy, long isl, long isz, long osy, long osk, long osz,
-                                                   int nlines, const float2* __restrict__ tw) {
+                                                   int nlines, const float2* __restrict__ tw, int nzb, Slots sl) {
   constexpr int T = TPL<N>(), L = LPW<N>(), LS = N + 1, NT = L * T;
   __shared__ float2 s0[L * LS], s1[L * LS];
-  const int by = blockIdx.y, bz = blockIdx.z, l0 = (int)xcd_x() * L, tid = threadIdx.x;
+  const unsigned zs = blockIdx.z / (unsigned)nzb;  // grid z: slot * nzb + the pass's own z block
+  const int by = blockIdx.y, bz = (int)(blockIdx.z - zs * (unsigned)nzb), l0 = (int)xcd_x() * L, tid = threadIdx.x;
   if (l0 >= nlines) return;  // a padded grid's extra workgroups exit whole
+  if (!slot_live(sl, zs)) return;
+  in = at_slot(in, sl.stride, zs);
+  out = at_slot(out, sl.stride, zs);
   const int nl = min(L, nlines - l0);
   const float2* src = in + by * isy + bz * isz + (long)l0 * isl;
   for (int i = tid; i < nl * N; i += NT) {
@@ -257,10 +265,15 @@ __device__ __forceinline__ double interp_lin(double x, const double* __restrict_
 template <int N, int MODE>
 __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, float2* __restrict__ out, int NY,
                                                int NZ, long ocomp, KOp op, XTabs tb,
-                                               const float2* __restrict__ tw) {
+                                               const float2* __restrict__ tw, Slots sl) {
   constexpr int T = TPL<N>(), L = LPW<N>(), LS = N + 1, NT = L * T;
   constexpr int NBUF = (MODE == XP_FORCES) ? 3 : 2;
   __shared__ float2 sm[NBUF * L * LS];
+  const unsigned zs = blockIdx.z;
+  if (!slot_live(sl, zs)) return;
+  in = at_slot(in, sl.stride, zs);
+  out = at_slot(out, sl.stride, zs);
+  if (sl.coef) op.scale = sl.coef[4 * zs + 3];
   float2* s0 = sm;
   float2* s1 = sm + L * LS;
   const int kz = blockIdx.y, ky0 = (int)xcd_x() * L, tid = threadIdx.x;
@@ -392,11 +405,15 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
 template <int NZ, int NC>
 __global__ void __launch_bounds__(384) k_c2r_z(const float2* __restrict__ inD, float* __restrict__ out, int NY,
                                                long icomp, const float2* __restrict__ twM,
-                                               const float2* __restrict__ twN) {
+                                               const float2* __restrict__ twN, Slots sl) {
   constexpr int M = NZ / 2, T = TPL<M>();
   constexpr int L = (NC == 1) ? LPW<M>() : (LPW<M>() / 2 > 0 ? LPW<M>() / 2 : 1);
   constexpr int LS = M + 1, NL = NC * L, NT = NL * T, NZH = M + 1, NZHP = (NZH + 7) & ~7;
   __shared__ float2 s0[NL * LS], s1[NL * LS];
+  const unsigned zs = blockIdx.z;
+  if (!slot_live(sl, zs)) return;
+  inD = at_slot(inD, sl.stride, zs);
+  out = at_slot(out, sl.stride, zs);
   const int x = blockIdx.y, y0 = blockIdx.x * L, tid = threadIdx.x;
   const int nl = min(L, NY - y0);
   // load X[k], k = 0..M
@@ -437,8 +454,12 @@ __global__ void __launch_bounds__(384) k_c2r_z(const float2* __restrict__ inD, f
 // per thread form one contiguous span, written from an LDS copy as 16-byte lanes. Rows are remapped
 // so each XCD streams one x slab and the neighbour rows come from its L2.
 __global__ void __launch_bounds__(256) k_grad4(const float* __restrict__ phi, float* __restrict__ F, int NX, int NY,
-                                               int NZ, int tpr, int rpb) {
+                                               int NZ, int tpr, int rpb, Slots sl) {
   __shared__ float4 stage[3 * 256];
+  const unsigned zs = blockIdx.z;
+  if (!slot_live(sl, zs)) return;
+  phi = at_slot(phi, sl.stride, zs);
+  F = at_slot(F, sl.stride, zs);
   const unsigned Lb = xcd_logical(blockIdx.x, gridDim.x);
   const int lr = (int)threadIdx.x / tpr, t = (int)threadIdx.x - lr * tpr;
   const unsigned nrows = (unsigned)NX * (unsigned)NY;
@@ -488,38 +509,42 @@ __global__ void __launch_bounds__(256) k_grad4(const float* __restrict__ phi, fl
 
 template <int NZ> int launch_r2c_z(pmx_plan_t* p, const float* in, float2* out, hipStream_t s) {
   constexpr int M = NZ / 2, L = LPW<M>(), T = TPL<M>();
-  dim3 grid((p->ny + L - 1) / L, p->nx), block(L * T);
-  hipLaunchKernelGGL(k_r2c_z<NZ>, grid, block, 0, s, in, out, p->nx, p->ny, p->tw_zh, p->tw_z);
+  dim3 grid((p->ny + L - 1) / L, p->nx, p->nslots), block(L * T);
+  hipLaunchKernelGGL(k_r2c_z<NZ>, grid, block, 0, s, in, out, p->nx, p->ny, p->tw_zh, p->tw_z, p->slots);
   return pmx_check_launch("r2c_z");
 }
 
 template <int N, bool INV>
 int launch_c2c(const float2* in, float2* out, long isy, long isl, long isz, long osy, long osk, long osz,
-               int nlines, int ny_blocks, int nz_blocks, const float2* tw, hipStream_t s) {
+               int nlines, int ny_blocks, int nz_blocks, const float2* tw, hipStream_t s, int nslots,
+               const Slots& sl) {
   constexpr int L = LPW<N>(), T = TPL<N>();
   // runs of fewer than 8 lines store partial 64-byte sectors: pad the grid's x extent to a multiple
   // of 8 so the XCD remap gathers the workgroups sharing those sectors on one L2
   const int g0 = (nlines + L - 1) / L;
   const int gx = L < 8 ? (g0 + 7) / 8 * 8 : g0;
-  dim3 grid(gx, ny_blocks, nz_blocks), block(L * T);
-  hipLaunchKernelGGL((k_c2c_lines<N, INV>), grid, block, 0, s, in, out, isy, isl, isz, osy, osk, osz, nlines, tw);
+  dim3 grid(gx, ny_blocks, nz_blocks * nslots), block(L * T);
+  hipLaunchKernelGGL((k_c2c_lines<N, INV>), grid, block, 0, s, in, out, isy, isl, isz, osy, osk, osz, nlines, tw,
+                     nz_blocks, sl);
   return pmx_check_launch("c2c_lines");
 }
 
 template <int N, int MODE> int launch_xpass(pmx_plan_t* p, const KOp& op, const float2* in, float2* out, hipStream_t s) {
   constexpr int L = LPW<N>(), T = TPL<N>();
-  dim3 grid((p->ny + L - 1) / L, p->nzh), block(L * T);
+  dim3 grid((p->ny + L - 1) / L, p->nzh, p->nslots), block(L * T);
   XTabs tb{p->grad_x, p->grad_y, p->grad_z, p->w2_x, p->w2_y, p->w2_z, p->dec_x, p->dec_y, p->dec_z};
   size_t shm = (MODE == XP_POWER) ? 3 * (size_t)(op.nedges + 1) * sizeof(double) : 0;
-  hipLaunchKernelGGL((k_xpass<N, MODE>), grid, block, shm, s, in, out, p->ny, p->nz, (long)p->nspec, op, tb, p->tw_x);
+  hipLaunchKernelGGL((k_xpass<N, MODE>), grid, block, shm, s, in, out, p->ny, p->nz, (long)p->nspec, op, tb, p->tw_x,
+                     p->slots);
   return pmx_check_launch("xpass");
 }
 
 template <int NZ, int NC> int launch_c2r_z(pmx_plan_t* p, const float2* in, float* out, hipStream_t s) {
   constexpr int M = NZ / 2, T = TPL<M>();
   constexpr int L = (NC == 1) ? LPW<M>() : (LPW<M>() / 2 > 0 ? LPW<M>() / 2 : 1);
-  dim3 grid((p->ny + L - 1) / L, p->nx), block(NC * L * T);
-  hipLaunchKernelGGL((k_c2r_z<NZ, NC>), grid, block, 0, s, in, out, p->ny, (long)p->nspecd, p->tw_zh, p->tw_z);
+  dim3 grid((p->ny + L - 1) / L, p->nx, p->nslots), block(NC * L * T);
+  hipLaunchKernelGGL((k_c2r_z<NZ, NC>), grid, block, 0, s, in, out, p->ny, (long)p->nspecd, p->tw_zh, p->tw_z,
+                     p->slots);
   return pmx_check_launch("c2r_z");
 }
 
@@ -549,7 +574,10 @@ int pmx_pass_c2c_y(pmx_plan_t* p, const float2* inA, float2* outB, hipStream_t s
   // lines: A[kz][x][*] (kz = block y, x = line), out B[kz][ky][x]
   const long NX = p->nx, NY = p->ny;
   switch (p->ny) {
-#define M_(v) case v: return launch_c2c<v, false>(inA, outB, NX * NY, NY, 0, NY * NX, NX, 0, p->nx, p->nzh, 1, p->tw_y, s);
+#define M_(v)                                                                                                \
+  case v:                                                                                                    \
+    return launch_c2c<v, false>(inA, outB, NX * NY, NY, 0, NY * NX, NX, 0, p->nx, p->nzh, 1, p->tw_y, s, p->nslots, \
+                                p->slots);
     PMX_FOR_AXIS(M_)
 #undef M_
   }
@@ -590,7 +618,7 @@ int pmx_pass_c2c_y_inv(pmx_plan_t* p, int ncomp, const float2* inC, float2* outD
 #define M_(v)                                                                                         \
   case v:                                                                                             \
     return launch_c2c<v, true>(inC, outD, NY, NX * NY, (long)p->nspec, NY * NZH, NZH, (long)p->nspecd, \
-                               p->nzh, p->nx, ncomp, p->tw_y, s);
+                               p->nzh, p->nx, ncomp, p->tw_y, s, p->nslots, p->slots);
     PMX_FOR_AXIS(M_)
 #undef M_
   }
@@ -617,8 +645,8 @@ int pmx_grad4(pmx_plan_t* p, const float* phi, float* F, hipStream_t s) {
   if (p->nz % 4 != 0 || tpr > 256) return pmx_set_error(2, "grad4 needs nz a multiple of 4, at most 1024");
   if ((reinterpret_cast<uintptr_t>(phi) | reinterpret_cast<uintptr_t>(F)) & 15u)
     return pmx_set_error(2, "grad4 needs 16-byte aligned potential and force buffers");
-  hipLaunchKernelGGL(k_grad4, dim3((rows + rpb - 1) / rpb), dim3(tpr * rpb), 0, s, phi, F, p->nx, p->ny, p->nz, tpr,
-                     rpb);
+  hipLaunchKernelGGL(k_grad4, dim3((rows + rpb - 1) / rpb, 1, p->nslots), dim3(tpr * rpb), 0, s, phi, F, p->nx, p->ny,
+                     p->nz, tpr, rpb, p->slots);
   return pmx_check_launch("grad4");
 }
 

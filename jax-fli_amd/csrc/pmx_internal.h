@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -19,8 +20,30 @@ struct StepMarks {
   hipEvent_t e[kStepMarks];
 };
 
+// Slot batching. A batched plan holds nslots copies of its work buffers (and of the particles it
+// steps), each slot's `stride` bytes after the previous one's, and shares its FFT and k-space
+// tables. Its launches add the slot to the grid (blockIdx.z, times any z extent the kernel already
+// uses) and every kernel moves its slot-owned pointers by slot * stride. Device arrays, per slot:
+// live (nonzero: the slot runs this launch; null: every slot runs) and coef (4 floats: unused, kick,
+// drift after the kick, potential scale; null: the launch arguments hold the coefficients).
+struct Slots {
+  size_t stride = 0;
+  const int* live = nullptr;
+  const float* coef = nullptr;
+};
+template <class T>
+__host__ __device__ __forceinline__ T* at_slot(T* p, size_t stride, unsigned z) {
+  return p ? reinterpret_cast<T*>(reinterpret_cast<char*>(const_cast<typename std::remove_const<T>::type*>(p)) +
+                                  (size_t)z * stride)
+           : p;
+}
+__device__ __forceinline__ bool slot_live(const Slots& s, unsigned z) { return !s.live || s.live[z] != 0; }
+
 struct pmx_plan_t {
   int nx, ny, nz, nzh;
+  int nslots;       // slots of a batched plan (1: an ordinary plan)
+  Slots slots;      // the batched plan's slot stride and per-slot control arrays
+  void* arena;      // a batched plan's slot buffers (one allocation); null for an ordinary plan
   int device;
   unsigned flags;
   size_t ncell;   // nx*ny*nz
@@ -61,6 +84,12 @@ struct pmx_plan_t {
 };
 
 constexpr int kRedMax = 65536;  // partial-sum slots: the largest particle-kernel grid
+
+// batched plans of the request server (pmx_api.hip)
+int pmx_plan_create_batched(pmx_plan_t** out, int n, int nslots, size_t extra, size_t* extra_off);
+size_t pmx_batched_slot_bytes(int n, size_t extra);  // the slot stride pmx_plan_create_batched would use
+pmx_plan_t pmx_slot_view(const pmx_plan_t* b, int z);
+int pmx_pot_force_mesh(pmx_plan_t* p, const float* src, float force_scale, hipStream_t s);
 
 // error reporting (pmx_api.hip)
 int pmx_set_error(int code, const std::string& msg);
@@ -135,6 +164,7 @@ int pmx_launch_fill(float* a, size_t n, float v, hipStream_t s);
 // the sorted paint in two halves: sort (count, scan, scatter) then deposit(s) of the sorted particles
 int pmx_paint_sort(pmx_plan_t* p, const PartArgs& a, float cd_pre, const float* w, int wstride, float w0,
                    hipStream_t s);
+size_t pmx_sort_wg_entries(int npx, int npy, int npz);
 // size the sort buffers for a particle grid (synchronous, grow only; adds the new bytes to grown)
 int pmx_sort_reserve(pmx_plan_t* p, int npx, int npy, int npz, size_t& grown);
 // ncomp > 1: component c carries w[p * wstride + c] and goes to mesh + c * mcomp

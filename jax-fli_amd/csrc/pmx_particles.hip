@@ -419,9 +419,20 @@ __device__ __forceinline__ void count_relative(const PartArgs& a, const SortGeom
 __global__ void __launch_bounds__(256) k_sort_count(PartArgs a, SortGeom g, float cd, unsigned* __restrict__ count,
                                                     unsigned* __restrict__ wgcnt, unsigned* __restrict__ far,
                                                     const float* __restrict__ w, int wstride,
-                                                    unsigned* __restrict__ wmax_bits, unsigned* __restrict__ pack) {
+                                                    unsigned* __restrict__ wmax_bits, unsigned* __restrict__ pack,
+                                                    Slots sl) {
   __shared__ unsigned lcnt[SORT_SLOTS];
   __shared__ int3 cen;
+  const unsigned z = blockIdx.z;
+  if (!slot_live(sl, z)) return;
+  a.x = at_slot(a.x, sl.stride, z);
+  a.v = at_slot(a.v, sl.stride, z);
+  count = at_slot(count, sl.stride, z);
+  wgcnt = at_slot(wgcnt, sl.stride, z);
+  far = at_slot(far, sl.stride, z);
+  w = at_slot(w, sl.stride, z);
+  wmax_bits = at_slot(wmax_bits, sl.stride, z);
+  pack = at_slot(pack, sl.stride, z);
   const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
   int i0, j0, k0;
   subblock_origin(g, b, i0, j0, k0);
@@ -478,8 +489,19 @@ __global__ void __launch_bounds__(1024) k_sort_scan(unsigned* __restrict__ count
                                                     unsigned* __restrict__ offset,
                                                     unsigned* __restrict__ cursor, int ntiles, float wmax_host,
                                                     unsigned* __restrict__ wmax_bits, float* __restrict__ scale,
-                                                    unsigned* __restrict__ far, unsigned* __restrict__ far_last) {
+                                                    unsigned* __restrict__ far, unsigned* __restrict__ far_last,
+                                                    Slots sl) {
   __shared__ unsigned part[1024];
+  const unsigned z = blockIdx.z;
+  if (!slot_live(sl, z)) return;
+  count = at_slot(count, sl.stride, z);
+  tcount = at_slot(tcount, sl.stride, z);
+  offset = at_slot(offset, sl.stride, z);
+  cursor = at_slot(cursor, sl.stride, z);
+  wmax_bits = at_slot(wmax_bits, sl.stride, z);
+  scale = at_slot(scale, sl.stride, z);
+  far = at_slot(far, sl.stride, z);
+  far_last = at_slot(far_last, sl.stride, z);
   const int T = blockDim.x, t = threadIdx.x;
   const int per = (ntiles + T - 1) / T;
   const int beg = min(ntiles, t * per), end = min(ntiles, beg + per);
@@ -520,9 +542,17 @@ __global__ void __launch_bounds__(1024) k_sort_scan(unsigned* __restrict__ count
 // outside the window take one global cursor increment on their tile.
 __global__ void __launch_bounds__(256) k_sort_scatter(PartArgs a, SortGeom g, unsigned* __restrict__ cursor,
                                                       const unsigned* __restrict__ wgcnt,
-                                                      const unsigned* __restrict__ pack, unsigned* __restrict__ seg) {
+                                                      const unsigned* __restrict__ pack, unsigned* __restrict__ seg,
+                                                      Slots sl) {
   __shared__ unsigned lbase[SORT_SLOTS];
   __shared__ int3 cen;
+  const unsigned z = blockIdx.z;
+  if (!slot_live(sl, z)) return;
+  a.x = at_slot(a.x, sl.stride, z);
+  cursor = at_slot(cursor, sl.stride, z);
+  wgcnt = at_slot(wgcnt, sl.stride, z);
+  pack = at_slot(pack, sl.stride, z);
+  seg = at_slot(seg, sl.stride, z);
   const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
   int i0, j0, k0;
   subblock_origin(g, b, i0, j0, k0);
@@ -571,8 +601,18 @@ __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, c
                                                        const unsigned* __restrict__ seg,
                                                        const float* __restrict__ scale, float* __restrict__ mesh,
                                                        int mstride, size_t mcomp, float* __restrict__ faces,
-                                                       int face_stride, size_t fcomp, int accumulate) {
+                                                       int face_stride, size_t fcomp, int accumulate, Slots sl) {
   __shared__ unsigned long long tile[TILE_PAD_MAX];
+  const unsigned zs = blockIdx.z;
+  if (!slot_live(sl, zs)) return;
+  a.x = at_slot(a.x, sl.stride, zs);
+  w = at_slot(w, sl.stride, zs);
+  tcount = at_slot(tcount, sl.stride, zs);
+  offset = at_slot(offset, sl.stride, zs);
+  seg = at_slot(seg, sl.stride, zs);
+  scale = at_slot(scale, sl.stride, zs);
+  mesh = at_slot(mesh, sl.stride, zs);
+  faces = at_slot(faces, sl.stride, zs);
   const int BX = 1 << g.sx, BY = 1 << g.sy, BZ = 1 << g.sz;
   const int PY = BY + 1, PZ = BZ + 1;
   const int npad = (BX + 1) * PY * PZ;
@@ -663,7 +703,11 @@ __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, c
 // xy, xz, yz, xyz, in that order). One thread per (tile, lower-shell cell).
 __global__ void __launch_bounds__(256) k_paint_faces(SortGeom g, const float* __restrict__ faces, int face_stride,
                                                      size_t fcomp, float* __restrict__ mesh, int mstride, size_t mcomp,
-                                                     int ncomp, int shell) {
+                                                     int ncomp, int shell, Slots sl) {
+  const unsigned zs = blockIdx.z;
+  if (!slot_live(sl, zs)) return;
+  faces = at_slot(faces, sl.stride, zs);
+  mesh = at_slot(mesh, sl.stride, zs);
   const int BX = 1 << g.sx, BY = 1 << g.sy, BZ = 1 << g.sz;
   const size_t gid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   const size_t ntiles = (size_t)g.ntx * g.nty * g.ntz;
@@ -1004,9 +1048,19 @@ __device__ __forceinline__ Axis1D axis_of(int mode, int lattice, float xc) {
 template <int NC>
 __global__ void __launch_bounds__(256) k_readout_lds(PartArgs a, const float* __restrict__ mesh, float* __restrict__ out,
                                                      float ck, float cd, int what, int NX, int NY, int NZ, int rx,
-                                                     int ry, int rz, int nbj, int nbk, FDiv dj, FDiv dk) {
+                                                     int ry, int rz, int nbj, int nbk, FDiv dj, FDiv dk, Slots sl) {
   __shared__ float sm[RO_CAP * NC];
   __shared__ int red[6][4];
+  const unsigned zs = blockIdx.z;
+  if (!slot_live(sl, zs)) return;
+  a.x = at_slot(a.x, sl.stride, zs);
+  a.v = at_slot(a.v, sl.stride, zs);
+  mesh = at_slot(mesh, sl.stride, zs);
+  out = at_slot(out, sl.stride, zs);
+  if (sl.coef) {
+    ck = sl.coef[4 * zs + 1];
+    cd = sl.coef[4 * zs + 2];
+  }
   const unsigned b = xcd_logical(blockIdx.x, gridDim.x);
   const unsigned bq = fdiv(b, dk), bi_ = fdiv(bq, dj);
   const int bk = (int)(b - bq * (unsigned)nbk), bj = (int)(bq - bi_ * (unsigned)nbj), bi = (int)bi_;
@@ -1356,6 +1410,11 @@ static SortGeom sort_geom(const pmx_plan_t* p, const PartArgs& a, int& ntiles) {
   return g;
 }
 
+// Window-count entries the sort keeps for a particle grid (one per window slot and sub-block).
+size_t pmx_sort_wg_entries(int npx, int npy, int npz) {
+  return (size_t)((npx + SB_X - 1) / SB_X) * ((npy + SB_Y - 1) / SB_Y) * ((npz + SB_Z - 1) / SB_Z) * SORT_SLOTS;
+}
+
 // Size the sort buffers for a particle grid of npx x npy x npz (grow only; synchronous). Adds the
 // bytes newly allocated to `grown`.
 int pmx_sort_reserve(pmx_plan_t* p, int npx, int npy, int npz, size_t& grown) {
@@ -1407,17 +1466,19 @@ int pmx_paint_sort(pmx_plan_t* p, const PartArgs& a, float cd_pre, const float* 
   unsigned* wmax_bits = p->ps_misc;
   unsigned* far = p->ps_misc + 1;
   unsigned* far_last = p->ps_misc + 2;
-  hipLaunchKernelGGL(k_sort_count, dim3((unsigned)nsb), dim3(256), 0, s, a, g, cd_pre, p->ps_count, p->ps_wgcnt, far, w,
-                     wstride, wmax_bits, p->ps_pack);
+  const unsigned S = (unsigned)p->nslots;
+  if (S > 1 && (a.v || w)) return pmx_set_error(2, "a batched paint takes no pre-drift and no per-particle weights");
+  hipLaunchKernelGGL(k_sort_count, dim3((unsigned)nsb, 1, S), dim3(256), 0, s, a, g, cd_pre, p->ps_count, p->ps_wgcnt,
+                     far, w, wstride, wmax_bits, p->ps_pack, p->slots);
   int rc = pmx_check_launch("sort_count");
   if (rc) return rc;
   PartArgs b = a;
   b.v = nullptr;
-  hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, p->ps_count, p->ps_tcount, p->ps_offset, p->ps_cursor,
-                     ntiles, w ? -1.0f : fabsf(w0), wmax_bits, p->ps_scale, far, far_last);
+  hipLaunchKernelGGL(k_sort_scan, dim3(1, 1, S), dim3(1024), 0, s, p->ps_count, p->ps_tcount, p->ps_offset,
+                     p->ps_cursor, ntiles, w ? -1.0f : fabsf(w0), wmax_bits, p->ps_scale, far, far_last, p->slots);
   if ((rc = pmx_check_launch("sort_scan"))) return rc;
-  hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)nsb), dim3(256), 0, s, b, g, p->ps_cursor, p->ps_wgcnt, p->ps_pack,
-                     p->ps_seg);
+  hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)nsb, 1, S), dim3(256), 0, s, b, g, p->ps_cursor, p->ps_wgcnt,
+                     p->ps_pack, p->ps_seg, p->slots);
   return pmx_check_launch("sort_scatter");
 }
 
@@ -1433,20 +1494,21 @@ int pmx_paint_deposit(pmx_plan_t* p, const PartArgs& a_in, const float* w, int w
   const int BX = 1 << g.sx, BY = 1 << g.sy, BZ = 1 << g.sz;
   const int face_stride = pmx_paint_face_cells(p);
   const size_t fcomp = (size_t)ntiles * face_stride;
+  const unsigned S = (unsigned)p->nslots;
   if (is_signed)
-    hipLaunchKernelGGL(k_paint_deposit<true>, dim3(ntiles), dim3(512), 0, s, a, g, w, wstride, wmul, w0, ncomp,
+    hipLaunchKernelGGL(k_paint_deposit<true>, dim3(ntiles, 1, S), dim3(512), 0, s, a, g, w, wstride, wmul, w0, ncomp,
                        p->ps_tcount, p->ps_offset, p->ps_seg, p->ps_scale, mesh, mstride, mcomp, p->ps_faces,
-                       face_stride, fcomp, accumulate);
+                       face_stride, fcomp, accumulate, p->slots);
   else
-    hipLaunchKernelGGL(k_paint_deposit<false>, dim3(ntiles), dim3(512), 0, s, a, g, w, wstride, wmul, w0, ncomp,
+    hipLaunchKernelGGL(k_paint_deposit<false>, dim3(ntiles, 1, S), dim3(512), 0, s, a, g, w, wstride, wmul, w0, ncomp,
                        p->ps_tcount, p->ps_offset, p->ps_seg, p->ps_scale, mesh, mstride, mcomp, p->ps_faces,
-                       face_stride, fcomp, accumulate);
+                       face_stride, fcomp, accumulate, p->slots);
   int rc = pmx_check_launch("paint_deposit");
   if (rc) return rc;
   const int shell = BX * BY * BZ - (BX - 1) * (BY - 1) * (BZ - 1);
   const size_t nthreads = (size_t)ntiles * shell;
-  hipLaunchKernelGGL(k_paint_faces, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s, g, p->ps_faces,
-                     face_stride, fcomp, mesh, mstride, mcomp, ncomp, shell);
+  hipLaunchKernelGGL(k_paint_faces, dim3((unsigned)((nthreads + 255) / 256), 1, S), dim3(256), 0, s, g, p->ps_faces,
+                     face_stride, fcomp, mesh, mstride, mcomp, ncomp, shell, p->slots);
   return pmx_check_launch("paint_faces");
 }
 
@@ -1460,6 +1522,7 @@ int pmx_launch_paint(pmx_plan_t* p, const PartArgs& a_in, const float* w, float 
   ratios(p, a, rx, ry, rz);
   const size_t np = (size_t)a.npx * a.npy * a.npz;
   if (atomic) {
+    if (p->nslots > 1) return pmx_set_error(2, "a batched plan paints with the sorted deposit only");
     if (a.v != nullptr) {
       hipLaunchKernelGGL(k_drift, dim3(grid_particles(3 * np)), dim3(256), 0, s, a.x, a.v, (unsigned)(3 * np),
                          cd_pre);
@@ -1493,14 +1556,16 @@ int pmx_launch_readout(pmx_plan_t* p, const PartArgs& a, const float* mesh, int 
     const int nbj = a.npy / RB_J, nbk = a.npz / RB_K;
     const FDiv dj = make_fdiv((unsigned)nbj), dk = make_fdiv((unsigned)nbk);
     const unsigned G = (unsigned)(np / (RB_I * RB_J * RB_K));
+    const unsigned S = (unsigned)p->nslots;
     if (ncomp == 3)
-      hipLaunchKernelGGL(k_readout_lds<3>, dim3(G), dim3(256), 0, s, a, mesh, out, ck, cd_post, what, p->nx, p->ny,
-                         p->nz, rx, ry, rz, nbj, nbk, dj, dk);
+      hipLaunchKernelGGL(k_readout_lds<3>, dim3(G, 1, S), dim3(256), 0, s, a, mesh, out, ck, cd_post, what, p->nx,
+                         p->ny, p->nz, rx, ry, rz, nbj, nbk, dj, dk, p->slots);
     else
-      hipLaunchKernelGGL(k_readout_lds<1>, dim3(G), dim3(256), 0, s, a, mesh, out, ck, cd_post, what & 1, p->nx,
-                         p->ny, p->nz, rx, ry, rz, nbj, nbk, dj, dk);
+      hipLaunchKernelGGL(k_readout_lds<1>, dim3(G, 1, S), dim3(256), 0, s, a, mesh, out, ck, cd_post, what & 1,
+                         p->nx, p->ny, p->nz, rx, ry, rz, nbj, nbk, dj, dk, p->slots);
     return pmx_check_launch("readout");
   }
+  if (p->nslots > 1) return pmx_set_error(2, "a batched readout needs the staged (LDS) form");
   if (ncomp == 3)
     hipLaunchKernelGGL(k_readout<3>, dim3(grid_particles(np)), dim3(256), 0, s, a, mesh, out, ck, cd_post, what, p->nx,
                        p->ny, p->nz, rx, ry, rz);

@@ -1,42 +1,41 @@
-"""Continuous-batching server for forward simulations on one GPU.
+"""Continuous-batching forward-simulation server on one GPU (Python side of pmx_server_* in libpmx).
 
 A request is one forward simulation of the PM path: seeded Gaussian ICs (white noise coloured by the
 EH98 P(k) of its own cosmology), first-order LPT, then the EfficientDriftDoubleKick time loop
 (src/jax_fli/pm/solvers.py:118-225, integrate.py:55-119) to the final scale factor, returning the final
 displacements and the painted density. The reference runs such simulations one at a time
-(fli_simulate.py); this server keeps many in flight:
+(run_simulations, src/jax_fli/scripts/fli_simulate.py:512-585); the native server keeps many in flight:
 
-* admission by HBM budget: a request is admitted when a slot is free and its device memory (plan
-  workspace + particle and mesh buffers, measured per mesh size) fits the budget; requests that
-  arrive while others run wait in arrival order and are admitted as memory frees up;
-* every admitted request owns a stream and a plan (pooled per mesh size), and the scheduler issues
-  the requests' steps round-robin, at most `lookahead` steps ahead of the device per request, so the
-  steps of different requests overlap on the GPU (small meshes fill the GPU only together) and the
-  queued steps cover the host's preparation of newly admitted requests;
-* all host work of a request (cosmology tables, time grid, kick/drift coefficients) is done once at
-  admission; the device work of a step is one fused pmx_step call.
+* one size class per mesh size, each with a fixed number of slots sized from the HBM budget; a slot
+  holds a request's particles and meshes in HBM from its ICs to its final paint, and a request is
+  admitted (in arrival order per size) as soon as a slot of its size frees up;
+* every step phase of all live slots of a class is one batched launch, replayed from a captured HIP
+  graph; classes run on their own streams, so different sizes overlap on the device;
+* the host preparation of a request (P(k) table and sigma8 integral, growth table, time grid, KDK
+  coefficients) is native code on a pool of host threads, off the scheduling thread.
 
-The kernels are deterministic (integer-accumulated paint, fixed-order FFTs and gathers), so a
-request's result is the same bit for bit alone and in any mix.
+The kernels are deterministic and slots do not interact, so a request's result is the same bit for
+bit alone and in any mix.
 """
 
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 import math
 import time
-from collections import deque
 
 import numpy as np
 import torch
 
 from . import _lib
 from . import cosmology as _c
-from . import engine
 from .pm.integrate import time_grid
 from .pm.interp import NoInterp
 from .pm.solvers import EfficientDriftDoubleKick, StepOp
 from .painting import PaintingOptions
+
+QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, -1
 
 
 @dataclasses.dataclass
@@ -71,6 +70,7 @@ class SimResult:
     arrival: float
     admitted: float
     finished: float
+    request_id: int = 0         # the server's id (pmx_server_request_times)
 
     @property
     def latency(self) -> float:
@@ -78,9 +78,10 @@ class SimResult:
 
 
 def forward_ops(req: SimRequest):
-    """(D, vel_factor, ktab, ptab, ops): the host side of one request -- LPT factors, the colouring's
-    P(k) table (times N / V), and the KDK time loop as StepOps with each step's leading drift fused
-    into the previous step's readout (as integrate() does)."""
+    """(D, vel_factor, ktab, ptab, ops, force_scale): the host side of one request in the Python layer --
+    LPT factors, the colouring's P(k) table (times N / V), and the KDK time loop as StepOps with each
+    step's leading drift fused into the previous step's readout (as integrate() does). The native
+    server prepares the same quantities (pmx_host_request_plan); tests compare the two."""
     cosmo = req.cosmology()
     n, box = req.mesh, req.box_size()
     k = np.logspace(-4, 1, 128)
@@ -104,168 +105,144 @@ def forward_ops(req: SimRequest):
     return D, vf, k, ptab, ops, 1.5 * cosmo.Omega_m
 
 
-class _Slot:
-    """Device state of one request in flight."""
+class _Request(ctypes.Structure):
+    _fields_ = [("mesh", ctypes.c_int), ("seed", ctypes.c_uint64), ("Omega_c", ctypes.c_double),
+                ("Omega_b", ctypes.c_double), ("h", ctypes.c_double), ("n_s", ctypes.c_double),
+                ("sigma8", ctypes.c_double), ("box", ctypes.c_double), ("a0", ctypes.c_double),
+                ("a1", ctypes.c_double), ("dt0", ctypes.c_double)]
 
-    def __init__(self, n, device, plan):
-        self.n = n
-        self.plan = plan
-        self.stream = torch.cuda.Stream(device=device)
-        shape = (n, n, n)
-        self.noise = torch.empty(shape, dtype=torch.float32, device=device)
-        self.ic = torch.empty(shape, dtype=torch.float32, device=device)
-        self.x = torch.empty(shape + (3,), dtype=torch.float32, device=device)
-        self.v = torch.empty(shape + (3,), dtype=torch.float32, device=device)
-        self.rho = torch.empty(shape, dtype=torch.float32, device=device)
-        self.ktab = torch.empty(128, dtype=torch.float64, device=device)
-        self.ptab = torch.empty(128, dtype=torch.float64, device=device)
-        self.host_tab = torch.empty((2, 128), dtype=torch.float64, pin_memory=True)
-        self.req = None
-        self.stages = None
-        self.next = 0
-        self.events = deque()
-        self.admitted = 0.0
 
-    def bytes(self) -> int:
-        return sum(t.numel() * t.element_size() for t in (self.noise, self.ic, self.x, self.v, self.rho)) + \
-            int(self.plan.workspace_bytes)  # the plan's workspace includes its paint-sort buffers
+class _Stats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("running", ctypes.c_uint64), ("queued", ctypes.c_uint64), ("peak_running", ctypes.c_int),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
+                ("max_ms", ctypes.c_double), ("wall_s", ctypes.c_double), ("throughput_req_s", ctypes.c_double),
+                ("device_bytes", ctypes.c_size_t)]
 
 
 class Server:
-    """Continuous-batching forward-simulation server (one GPU, one host thread)."""
+    """The native continuous-batching server (pmx_server_*). meshes: the size classes; slots: per-size
+    slot counts (dict or one int; None: from hbm_budget, default 60% of the free device memory)."""
 
-    def __init__(self, device=None, hbm_budget: int | None = None, max_active: int = 64, lookahead: int = 4,
-                 admit_per_pass: int = 4):
+    def __init__(self, device=None, meshes=(64, 96, 128, 192), slots=None, hbm_budget: int | None = None,
+                 prep_threads: int = 4):
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        self.meshes = tuple(int(m) for m in meshes)
+        lib = _lib.load()
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info(self.device)
-            hbm_budget = int(free * 0.8)
-        self.hbm_budget = int(hbm_budget)
-        self.max_active = int(max_active)
-        self.lookahead = int(lookahead)
-        self.admit_per_pass = int(admit_per_pass)
-        self._pool: dict[int, list[_Slot]] = {}
-        self._slot_bytes: dict[int, int] = {}
-
-    # -- slots -----------------------------------------------------------------
-    def prewarm(self, counts: dict) -> None:
-        """Create `counts[n]` pooled slots for mesh size n now (plan creation allocates synchronously)."""
-        for n, c in counts.items():
-            have = len(self._pool.get(n, []))
-            for _ in range(max(0, int(c) - have)):
-                self._new_slot(int(n), keep=True)
-            self._slot_bytes.setdefault(int(n), self._pool[int(n)][0].bytes())
-        torch.cuda.synchronize(self.device)
-
-    def _slot_size(self, n: int) -> int:
-        if n not in self._slot_bytes:
-            self._slot_bytes[n] = self._new_slot(n, keep=True).bytes()
-        return self._slot_bytes[n]
-
-    def _new_slot(self, n: int, keep: bool = False) -> _Slot:
+            hbm_budget = int(free * 0.6)
+        if slots is None:
+            sl = [0] * len(self.meshes)
+        elif isinstance(slots, dict):
+            sl = [int(slots.get(m, 0)) for m in self.meshes]
+        else:
+            sl = [int(slots)] * len(self.meshes)
+        ms = (ctypes.c_int * len(self.meshes))(*self.meshes)
+        ss = (ctypes.c_int * len(sl))(*sl)
+        handle = ctypes.c_void_p()
         with torch.cuda.device(self.device):
-            plan = _lib.Plan((n, n, n), self.device)
-            s = _Slot(n, self.device, plan)
-        if keep:
-            self._pool.setdefault(n, []).append(s)
-        return s
+            _lib.check(lib.pmx_server_create(ctypes.byref(handle), ms, ss, len(self.meshes), int(hbm_budget),
+                                             int(prep_threads)), "pmx_server_create")
+        self._h = handle
+        self._keep = {}
 
-    def _take_slot(self, n: int) -> _Slot:
-        pool = self._pool.get(n)
-        return pool.pop() if pool else self._new_slot(n)
+    # -- classes -----------------------------------------------------------------
+    def slots(self, mesh: int) -> int:
+        n = ctypes.c_int()
+        _lib.check(_lib.load().pmx_server_class(self._h, int(mesh), ctypes.byref(n), None), "pmx_server_class")
+        return int(n.value)
 
-    def _give_back(self, s: _Slot) -> None:
-        s.req, s.stages, s.next = None, None, 0
-        s.events.clear()
-        self._pool.setdefault(s.n, []).append(s)
+    def slot_bytes(self, mesh: int) -> int:
+        b = ctypes.c_size_t()
+        _lib.check(_lib.load().pmx_server_class(self._h, int(mesh), None, ctypes.byref(b)), "pmx_server_class")
+        return int(b.value)
 
-    # -- request stages ----------------------------------------------------------
-    def _stages(self, s: _Slot, req: SimRequest):
-        D, vf, k, ptab, ops, fs = forward_ops(req)
-        n, box, plan, mesh = req.mesh, req.box_size(), s.plan, (req.mesh,) * 3
-        s.host_tab[0].copy_(torch.from_numpy(k))
-        s.host_tab[1].copy_(torch.from_numpy(ptab))
+    # -- requests ----------------------------------------------------------------
+    def submit(self, req: SimRequest) -> int:
+        """Queue one request; the result buffers are allocated here and handed to the server."""
+        n = int(req.mesh)
+        disp = torch.empty((n, n, n, 3), dtype=torch.float32, device=self.device)
+        dens = torch.empty((n, n, n), dtype=torch.float32, device=self.device)
+        r = _Request(n, int(req.seed) & (2**64 - 1), float(req.Omega_c), float(req.Omega_b), float(req.h),
+                     float(req.n_s), float(req.sigma8), req.box_size(), float(req.a0), float(req.a1), float(req.dt0))
+        rid = ctypes.c_uint64()
+        _lib.check(_lib.load().pmx_server_submit(self._h, ctypes.byref(r), disp.data_ptr(), dens.data_ptr(),
+                                                 ctypes.byref(rid)), "pmx_server_submit")
+        self._keep[int(rid.value)] = (req, disp, dens)
+        return int(rid.value)
 
-        def ics():
-            s.ktab.copy_(s.host_tab[0], non_blocking=True)
-            s.ptab.copy_(s.host_tab[1], non_blocking=True)
-            engine.white_noise(mesh, req.seed, out=s.noise, plan=plan)
-            engine.color_field(s.noise, (box,) * 3, s.ktab, s.ptab, out=s.ic, plan=plan)
-            engine.lpt1(s.ic, D, vf, out=(s.x, s.v), plan=plan)
+    def state(self, rid: int) -> int:
+        st = ctypes.c_int()
+        _lib.check(_lib.load().pmx_server_poll(self._h, ctypes.c_uint64(rid), ctypes.byref(st)), "pmx_server_poll")
+        return int(st.value)
 
-        def step(op):
-            return lambda: engine.step(s.x, s.v, mesh, force_scale=fs, c_kick=op.kick, c_drift_pre=op.pre,
-                                       c_drift_post=op.post, plan=plan)
+    def wait(self, rid: int, timeout_s: float = 0.0) -> None:
+        _lib.check(_lib.load().pmx_server_wait(self._h, ctypes.c_uint64(rid), float(timeout_s)), "pmx_server_wait")
 
-        def final():
-            engine.paint(s.x, mesh, out=s.rho, plan=plan)
+    def times(self, rid: int):
+        t = (ctypes.c_double * 3)()
+        _lib.check(_lib.load().pmx_server_request_times(self._h, ctypes.c_uint64(rid), t), "pmx_server_request_times")
+        return tuple(t)
 
-        return [ics] + [step(op) for op in ops] + [final]
+    def stats(self, since_id: int = 0) -> dict:
+        s = _Stats()
+        _lib.check(_lib.load().pmx_server_stats(self._h, ctypes.c_uint64(since_id), ctypes.byref(s)),
+                   "pmx_server_stats")
+        return {"requests": int(s.completed), "failed": int(s.failed), "wall_s": s.wall_s,
+                "throughput_req_s": s.throughput_req_s, "p50_ms": s.p50_ms, "p99_ms": s.p99_ms,
+                "mean_ms": s.mean_ms, "max_ms": s.max_ms, "peak_active": int(s.peak_running),
+                "device_bytes": int(s.device_bytes)}
 
-    # -- main loop ---------------------------------------------------------------
-    def serve(self, requests, clock=time.perf_counter):
-        """Run `requests` (arrival times relative to the start); returns ({rid: SimResult}, stats)."""
-        pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
-        for n in {r.mesh for r in pending}:  # measure slot sizes up front (plan creation is synchronous)
-            if self._slot_size(n) > self.hbm_budget:
-                raise ValueError(f"a {n}^3 request needs {self._slot_size(n)} bytes, above the HBM budget")
-        active: list[_Slot] = []
-        results: dict[int, SimResult] = {}
-        used = 0
-        peak_active = 0
+    def _result(self, rid: int) -> SimResult:
+        req, disp, dens = self._keep.pop(rid)
+        t_sub, t_adm, t_done = self.times(rid)
+        return SimResult(req.rid, req.mesh, disp, dens, req.arrival, req.arrival + (t_adm - t_sub),
+                         req.arrival + (t_done - t_sub), rid)
+
+    def serve(self, requests, sequential: bool = False, clock=time.perf_counter):
+        """Submit `requests` at their arrival times (seconds after the call) and collect the results:
+        ({rid: SimResult}, stats over these requests). sequential: one request at a time (each submitted
+        when the previous one is done), the no-batching baseline."""
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        ids, first = [], None
         t0 = clock()
-        while pending or active:
+        i = 0
+        while i < len(pending):
             now = clock() - t0
-            # admission: arrival order, by free slot and HBM budget; a few per pass so that the host
-            # preparation of new requests interleaves with issuing the queued ones' steps
-            admitted = 0
-            while (pending and pending[0].arrival <= now and len(active) < self.max_active
-                   and admitted < self.admit_per_pass):
-                req = pending[0]
-                need = self._slot_size(req.mesh)
-                if used + need > self.hbm_budget:
-                    break
-                pending.popleft()
-                s = self._take_slot(req.mesh)
-                s.req, s.admitted = req, now
-                s.stages = self._stages(s, req)
-                used += need
-                active.append(s)
-                admitted += 1
-            peak_active = max(peak_active, len(active))
-            issued = False
-            for s in active:  # round-robin: one stage per request per pass
-                while s.events and s.events[0].query():
-                    s.events.popleft()
-                if s.next < len(s.stages) and len(s.events) < self.lookahead:
-                    with torch.cuda.stream(s.stream):
-                        s.stages[s.next]()
-                        ev = torch.cuda.Event()
-                        ev.record(s.stream)
-                    s.events.append(ev)
-                    s.next += 1
-                    issued = True
-            done = [s for s in active if s.next == len(s.stages) and all(e.query() for e in s.events)]
-            for s in done:
-                fin = clock() - t0
-                r = s.req
-                results[r.rid] = SimResult(r.rid, r.mesh, s.x.clone(), s.rho.clone(), r.arrival, s.admitted, fin)
-                active.remove(s)
-                used -= self._slot_size(s.n)
-                self._give_back(s)
-            if not issued and not done:
-                if active:
-                    active[0].events[0].synchronize() if active[0].events else None
-                elif pending:
-                    time.sleep(max(0.0, min(1e-3, pending[0].arrival - (clock() - t0))))
-        torch.cuda.synchronize(self.device)
-        wall = clock() - t0
-        lat = np.asarray([r.latency for r in results.values()])
-        stats = {"requests": len(results), "wall_s": wall, "throughput_req_s": len(results) / wall if wall > 0 else 0.0,
-                 "p50_ms": float(np.percentile(lat, 50) * 1e3) if lat.size else None,
-                 "p99_ms": float(np.percentile(lat, 99) * 1e3) if lat.size else None,
-                 "mean_ms": float(lat.mean() * 1e3) if lat.size else None, "peak_active": peak_active}
+            if sequential:
+                rid = self.submit(pending[i])
+                self.wait(rid)
+                ids.append(rid)
+                i += 1
+            elif pending[i].arrival <= now:
+                ids.append(self.submit(pending[i]))
+                i += 1
+            else:
+                time.sleep(min(1e-3, pending[i].arrival - now))
+            if first is None and ids:
+                first = ids[0]
+        for rid in ids:
+            self.wait(rid)
+        results = {}
+        for rid in ids:
+            res = self._result(rid)
+            results[res.rid] = res
+        stats = self.stats(first if first is not None else 0)
         return results, stats
+
+    def close(self) -> None:
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            with torch.cuda.device(self.device):
+                _lib._lib.pmx_server_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def poisson_requests(count: int, rate_per_s: float, seed: int = 0, meshes=(64, 96, 128, 192)):

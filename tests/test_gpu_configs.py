@@ -129,7 +129,7 @@ def test_serving_mix_matches_oracle(pkg, cuda):
         if m not in sizes:
             reqs.append(pkg.serving.SimRequest(rid=len(reqs), mesh=m, seed=900 + m, Omega_c=0.27, sigma8=0.82,
                                                arrival=reqs[-1].arrival + 1e-3))
-    srv = pkg.serving.Server(device=cuda, max_active=16)
+    srv = pkg.serving.Server(device=cuda, meshes=(64, 96, 128, 192), slots=4)
     mixed, stats = srv.serve(reqs)
     assert stats["requests"] == len(reqs)
     print(f"served {len(reqs)} requests, peak {stats['peak_active']} in flight, p50 {stats['p50_ms']:.1f} ms "

@@ -242,3 +242,36 @@ def test_resolve_ts_geometry_cases(pkg, cosmo):
             resolve_ts_geometry(c, f, dens, **kw)
     with pytest.raises(ValueError):
         resolve_ts_geometry(c, f, sph, ts=0.5)
+
+
+@pytest.mark.parametrize("oc,s8,n", [(0.25, 0.8, 64), (0.3137, 0.7123, 96), (0.21, 0.88, 192)])
+def test_native_request_plan_equals_python(pkg, oc, s8, n):
+    """The request server's native host preparation (pmx_host_request_plan: P(k) table, LPT factors, the
+    fused KDK coefficients) against the Python layer's forward_ops: coefficients and LPT factors bitwise
+    (float32, as the device takes them), the P(k) table to float64 rounding."""
+    import ctypes
+    import os
+
+    path = os.path.join(os.path.dirname(pkg.__file__), "libpmx_host.so")
+    if not os.path.exists(path):
+        pytest.skip("libpmx_host.so not built")
+    lib = ctypes.CDLL(path)
+    d, i, vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+    lib.pmx_host_request_plan.argtypes = [vp, i, d, d, d, d, vp, vp, vp, vp, i, vp, vp]
+    req = pkg.serving.SimRequest(rid=0, mesh=n, seed=1, Omega_c=oc, sigma8=s8)
+    c = req.cosmology()
+    params = np.array([c.Omega_c, c.Omega_b, c.h, c.n_s, c.sigma8, c.Omega_k, c.w0, c.wa])
+    kt, pt = np.zeros(128), np.zeros(128)
+    lpt, ops = np.zeros(2, np.float32), np.zeros((64, 3), np.float32)
+    nops, fs = ctypes.c_int(), ctypes.c_double()
+    rc = lib.pmx_host_request_plan(params.ctypes.data, n, req.box_size(), req.a0, req.a1, req.dt0, kt.ctypes.data,
+                                   pt.ctypes.data, lpt.ctypes.data, ops.ctypes.data, 64, ctypes.byref(nops),
+                                   ctypes.byref(fs))
+    assert rc == 0
+    D, vf, k, ptab, pops, fsp = pkg.serving.forward_ops(req)
+    np.testing.assert_allclose(kt, k, rtol=1e-15)
+    np.testing.assert_allclose(pt, ptab, rtol=2e-15)
+    assert lpt[0] == np.float32(D) and lpt[1] == np.float32(vf) and fs.value == fsp
+    ref = np.array([[np.nan if o.pre is None else o.pre, o.kick, np.nan if o.post is None else o.post] for o in pops],
+                   np.float32)
+    assert nops.value == len(pops) and np.array_equal(ops[: nops.value], ref, equal_nan=True)
