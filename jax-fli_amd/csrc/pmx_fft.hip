@@ -39,6 +39,29 @@ PMX_FP(512, 3, 8, 8, 8, 0)
 PMX_FP(1024, 4, 8, 8, 4, 4)
 #undef PMX_FP
 
+// LDS layout of one line: complex element e sits at px(e) = e + e/16 (a skipped slot every 16
+// values breaks the bank conflicts of the Stockham strides), and the lines of a workgroup are LSP<N>
+// slots apart, chosen (by a model of the 32-bank LDS) so that the line loads and the transposed
+// stores of a workgroup also spread over the banks.
+__device__ __forceinline__ int px(int e) { return e + (e >> 4); }
+template <int N> constexpr int LSP() {
+  switch (N) {
+    case 4: return 5;
+    case 8: return 9;
+    case 16: return 19;
+    case 32: return 35;
+    case 48: return 59;
+    case 64: return 71;
+    case 96: return 118;
+    case 128: return 137;
+    case 192: return 212;
+    case 256: return 274;
+    case 512: return 548;
+    case 1024: return 1096;
+    default: return N + (N >> 4) + 1;
+  }
+}
+
 // threads per line and lines per workgroup
 template <int N> constexpr int TPL() { return N / FP<N>::R[0]; }
 constexpr int pow2_floor(int v) { int p = 1; while (p * 2 <= v) p *= 2; return p; }
@@ -140,7 +163,7 @@ __device__ __forceinline__ void stage(const float2* __restrict__ src, float2* __
       const int k = j % NS;
       float2 v[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[r] = src[j + r * NB];
+      for (int r = 0; r < R; ++r) v[r] = src[px(j + r * NB)];
       if (NS > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
@@ -151,7 +174,7 @@ __device__ __forceinline__ void stage(const float2* __restrict__ src, float2* __
       Bfly<R, INV>::run(v);
       const int d = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[d + r * NS] = v[r];
+      for (int r = 0; r < R; ++r) dst[px(d + r * NS)] = v[r];
     }
   }
 }
@@ -174,39 +197,41 @@ template <int NZ>
 __global__ void __launch_bounds__(256) k_r2c_z(const float* __restrict__ in, float2* __restrict__ outA,
                                                int NX, int NY, const float2* __restrict__ twM,
                                                const float2* __restrict__ twN, Slots sl) {
-  constexpr int M = NZ / 2, T = TPL<M>(), L = LPW<M>(), LS = M + 1, NT = L * T;
+  constexpr int M = NZ / 2, T = TPL<M>(), L = LPW<M>(), LS = LSP<M>(), NT = L * T;
   __shared__ float2 s0[L * LS], s1[L * LS];
   const unsigned zs = blockIdx.z;
   if (!slot_live(sl, zs)) return;
   in = at_slot(in, sl.stride, zs);
   outA = at_slot(outA, sl.stride, zs);
+  __shared__ float2 stw[M];  // the line twiddles, read from LDS by every stage
   const int x = blockIdx.y, y0 = (int)xcd_x() * L, tid = threadIdx.x;
   const int nl = min(L, NY - y0);
   const float2* src = reinterpret_cast<const float2*>(in + ((size_t)x * NY + y0) * NZ);
+  for (int i = tid; i < M; i += NT) stw[i] = twM[i];
   for (int i = tid; i < nl * M; i += NT) {
     const int l = i / M, k = i - l * M;
-    s0[l * LS + k] = src[i];
+    s0[l * LS + px(k)] = src[i];
   }
   __syncthreads();
   const int l = tid / T, t = tid - l * T;
-  fft_run<M, false>(s0 + l * LS, s1 + l * LS, t, twM);
+  fft_run<M, false>(s0 + l * LS, s1 + l * LS, t, stw);
   const float2* res = res_in_b<M>() ? s1 : s0;
   float2* oth = res_in_b<M>() ? s0 : s1;
   // split: X_k = E_k + W^k O_k,  E = (Z_k + conj Z_{M-k})/2,  O = -i (Z_k - conj Z_{M-k})/2
   for (int i = tid; i < L * (M + 1); i += NT) {
     const int ll = i / (M + 1), k = i - ll * (M + 1);
-    const float2 zk = res[ll * LS + (k == M ? 0 : k)];
-    const float2 zc = cconj(res[ll * LS + (k == 0 ? 0 : M - k)]);
+    const float2 zk = res[ll * LS + px((k == M ? 0 : k))];
+    const float2 zc = cconj(res[ll * LS + px((k == 0 ? 0 : M - k))]);
     const float2 e = cscale(cadd(zk, zc), 0.5f);
     const float2 o = cscale(mul_mi<false>(csub(zk, zc)), 0.5f);
-    oth[ll * LS + k] = cadd(e, cmul(twN[k], o));
+    oth[ll * LS + px(k)] = cadd(e, cmul(twN[k], o));
   }
   __syncthreads();
   float2* dst = outA + (size_t)x * NY + y0;
   const size_t osk = (size_t)NX * NY;
   for (int i = tid; i < (M + 1) * L; i += NT) {
     const int k = i / L, ll = i - k * L;
-    if (ll < nl) dst[k * osk + ll] = oth[ll * LS + k];
+    if (ll < nl) dst[k * osk + ll] = oth[ll * LS + px(k)];
   }
 }
 
@@ -217,7 +242,7 @@ template <int N, bool INV>
 __global__ void __launch_bounds__(256) k_c2c_lines(const float2* __restrict__ in, float2* __restrict__ out,
                                                    long isy, long isl, long isz, long osy, long osk, long osz,
                                                    int nlines, const float2* __restrict__ tw, int nzb, Slots sl) {
-  constexpr int T = TPL<N>(), L = LPW<N>(), LS = N + 1, NT = L * T;
+  constexpr int T = TPL<N>(), L = LPW<N>(), LS = LSP<N>(), NT = L * T;
   __shared__ float2 s0[L * LS], s1[L * LS];
   const unsigned zs = blockIdx.z / (unsigned)nzb;  // grid z: slot * nzb + the pass's own z block
   const int by = blockIdx.y, bz = (int)(blockIdx.z - zs * (unsigned)nzb), l0 = (int)xcd_x() * L, tid = threadIdx.x;
@@ -225,20 +250,22 @@ __global__ void __launch_bounds__(256) k_c2c_lines(const float2* __restrict__ in
   if (!slot_live(sl, zs)) return;
   in = at_slot(in, sl.stride, zs);
   out = at_slot(out, sl.stride, zs);
+  __shared__ float2 stw[N];
   const int nl = min(L, nlines - l0);
   const float2* src = in + by * isy + bz * isz + (long)l0 * isl;
+  for (int i = tid; i < N; i += NT) stw[i] = tw[i];
   for (int i = tid; i < nl * N; i += NT) {
     const int l = i / N, k = i - l * N;
-    s0[l * LS + k] = src[l * isl + k];
+    s0[l * LS + px(k)] = src[l * isl + k];
   }
   __syncthreads();
   const int l = tid / T, t = tid - l * T;
-  fft_run<N, INV>(s0 + l * LS, s1 + l * LS, t, tw);
+  fft_run<N, INV>(s0 + l * LS, s1 + l * LS, t, stw);
   const float2* res = res_in_b<N>() ? s1 : s0;
   float2* dst = out + by * osy + bz * osz + l0;
   for (int i = tid; i < N * L; i += NT) {
     const int k = i / L, ll = i - k * L;
-    if (ll < nl) dst[k * osk + ll] = res[ll * LS + k];
+    if (ll < nl) dst[k * osk + ll] = res[ll * LS + px(k)];
   }
 }
 
@@ -266,7 +293,7 @@ template <int N, int MODE>
 __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, float2* __restrict__ out, int NY,
                                                int NZ, long ocomp, KOp op, XTabs tb,
                                                const float2* __restrict__ tw, Slots sl) {
-  constexpr int T = TPL<N>(), L = LPW<N>(), LS = N + 1, NT = L * T;
+  constexpr int T = TPL<N>(), L = LPW<N>(), LS = LSP<N>(), NT = L * T;
   constexpr int NBUF = (MODE == XP_FORCES) ? 3 : 2;
   __shared__ float2 sm[NBUF * L * LS];
   const unsigned zs = blockIdx.z;
@@ -274,19 +301,21 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
   in = at_slot(in, sl.stride, zs);
   out = at_slot(out, sl.stride, zs);
   if (sl.coef) op.scale = sl.coef[4 * zs + 3];
+  __shared__ float2 stw[N];
   float2* s0 = sm;
   float2* s1 = sm + L * LS;
   const int kz = blockIdx.y, ky0 = (int)xcd_x() * L, tid = threadIdx.x;
   const int nl = min(L, NY - ky0);
   const float2* src = in + ((size_t)kz * NY + ky0) * N;
+  for (int i = tid; i < N; i += NT) stw[i] = tw[i];
   for (int i = tid; i < nl * N; i += NT) {
     const int l = i / N, k = i - l * N;
-    s0[l * LS + k] = src[i];
+    s0[l * LS + px(k)] = src[i];
   }
   __syncthreads();
   const int l = tid / T, t = tid - l * T;
   constexpr bool FWD = (MODE != XP_INV);
-  fft_run<N, !FWD>(s0 + l * LS, s1 + l * LS, t, tw);
+  fft_run<N, !FWD>(s0 + l * LS, s1 + l * LS, t, stw);
   float2* res = res_in_b<N>() ? s1 : s0;
   float2* oth = res_in_b<N>() ? s0 : s1;
 
@@ -294,13 +323,13 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
     float2* dst = out + ((size_t)kz * NY + ky0) * N;
     for (int i = tid; i < nl * N; i += NT) {
       const int ll = i / N, k = i - ll * N;
-      dst[i] = res[ll * LS + k];
+      dst[i] = res[ll * LS + px(k)];
     }
   } else if constexpr (MODE == XP_INV) {
     float2* dst = out + (size_t)kz * N * NY + ky0;
     for (int i = tid; i < N * L; i += NT) {
       const int k = i / L, ll = i - k * L;
-      if (ll < nl) dst[(size_t)k * NY + ll] = res[ll * LS + k];
+      if (ll < nl) dst[(size_t)k * NY + ll] = res[ll * LS + px(k)];
     }
   } else if constexpr (MODE == XP_FORCES) {
     float2* s2 = sm + 2 * L * LS;
@@ -312,16 +341,16 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
         const float g = (c == 0) ? tb.gx[kx] : (c == 1 ? tb.gy[ky] : tb.gz[kz]);
         float coef = (k2 > 0.f) ? g * op.scale / k2 : 0.f;
         if (op.flags & 1u) coef *= tb.dx[kx] * tb.dy[ky] * tb.dz[kz];
-        const float2 d = res[ll * LS + kx];
-        s2[ll * LS + kx] = make_float2(-coef * d.y, coef * d.x);  // i * coef * d
+        const float2 d = res[ll * LS + px(kx)];
+        s2[ll * LS + px(kx)] = make_float2(-coef * d.y, coef * d.x);  // i * coef * d
       }
       __syncthreads();
-      fft_run<N, true>(s2 + l * LS, oth + l * LS, t, tw);
+      fft_run<N, true>(s2 + l * LS, oth + l * LS, t, stw);
       const float2* r2 = res_in_b<N>() ? oth : s2;
       float2* dst = out + c * ocomp + (size_t)kz * N * NY + ky0;
       for (int i = tid; i < N * L; i += NT) {
         const int k = i / L, ll = i - k * L;
-        if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + k];
+        if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + px(k)];
       }
       __syncthreads();
     }
@@ -336,15 +365,15 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
       if ((op.flags & 2u) && k2 > 0.f) coef *= expf(-op.pgd_kl2 / k2) * expf(-(k2 * k2) / op.pgd_ks4);
       if ((op.flags & 4u) && k2 > 0.f)
         coef *= op.mix_a + op.mix_b * (expf(-op.pgd_kl2 / k2) * expf(-(k2 * k2) / op.pgd_ks4));
-      res[ll * LS + kx] = cscale(res[ll * LS + kx], coef);
+      res[ll * LS + px(kx)] = cscale(res[ll * LS + px(kx)], coef);
     }
     __syncthreads();
-    fft_run<N, true>(res + l * LS, oth + l * LS, t, tw);
+    fft_run<N, true>(res + l * LS, oth + l * LS, t, stw);
     const float2* r2 = res_in_b<N>() ? oth : res;
     float2* dst = out + (size_t)kz * N * NY + ky0;
     for (int i = tid; i < N * L; i += NT) {
       const int k = i / L, ll = i - k * L;
-      if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + k];
+      if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + px(k)];
     }
   } else if constexpr (MODE == XP_COLOR) {
     for (int i = tid; i < L * N; i += NT) {
@@ -354,15 +383,15 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
                              op.kphys_z[kz] * op.kphys_z[kz]);
       double pk = interp_lin(kk, op.ktab, op.ptab, op.ntab) * op.cnorm;
       const float amp = (float)(sqrt(pk > 0.0 ? pk : 0.0)) * op.scale;
-      res[ll * LS + kx] = cscale(res[ll * LS + kx], amp);
+      res[ll * LS + px(kx)] = cscale(res[ll * LS + px(kx)], amp);
     }
     __syncthreads();
-    fft_run<N, true>(res + l * LS, oth + l * LS, t, tw);
+    fft_run<N, true>(res + l * LS, oth + l * LS, t, stw);
     const float2* r2 = res_in_b<N>() ? oth : res;
     float2* dst = out + (size_t)kz * N * NY + ky0;
     for (int i = tid; i < N * L; i += NT) {
       const int k = i / L, ll = i - k * L;
-      if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + k];
+      if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + px(k)];
     }
   } else if constexpr (MODE == XP_POWER) {
     // bins: 0 .. nedges (nedges + 1 entries), bin = number of edges <= |k|
@@ -384,7 +413,7 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
         int mid = (lo + hi) >> 1;
         if (op.kedges[mid] <= kk) lo = mid + 1; else hi = mid;
       }
-      const float2 d = res[ll * LS + kx];
+      const float2 d = res[ll * LS + px(kx)];
       const double p = ((double)d.x * d.x + (double)d.y * d.y) * mult;
       atomicAdd(&hist[lo], p);
       atomicAdd(&hist[nb + lo], kk * mult);
@@ -408,39 +437,41 @@ __global__ void __launch_bounds__(384) k_c2r_z(const float2* __restrict__ inD, f
                                                const float2* __restrict__ twN, Slots sl) {
   constexpr int M = NZ / 2, T = TPL<M>();
   constexpr int L = (NC == 1) ? LPW<M>() : (LPW<M>() / 2 > 0 ? LPW<M>() / 2 : 1);
-  constexpr int LS = M + 1, NL = NC * L, NT = NL * T, NZH = M + 1, NZHP = (NZH + 7) & ~7;
+  constexpr int LS = LSP<M>(), NL = NC * L, NT = NL * T, NZH = M + 1, NZHP = (NZH + 7) & ~7;
   __shared__ float2 s0[NL * LS], s1[NL * LS];
   const unsigned zs = blockIdx.z;
   if (!slot_live(sl, zs)) return;
   inD = at_slot(inD, sl.stride, zs);
   out = at_slot(out, sl.stride, zs);
+  __shared__ float2 stw[M];
   const int x = blockIdx.y, y0 = blockIdx.x * L, tid = threadIdx.x;
   const int nl = min(L, NY - y0);
+  for (int i = tid; i < M; i += NT) stw[i] = twM[i];
   // load X[k], k = 0..M
   for (int i = tid; i < NC * nl * NZH; i += NT) {
     const int c = i / (nl * NZH), rem = i - c * (nl * NZH);
     const int ll = rem / NZH, k = rem - ll * NZH;
-    s1[(c * L + ll) * LS + k] = inD[c * icomp + ((size_t)x * NY + y0 + ll) * NZHP + k];
+    s1[(c * L + ll) * LS + px(k)] = inD[c * icomp + ((size_t)x * NY + y0 + ll) * NZHP + k];
   }
   __syncthreads();
   // merge: Z'_k = (X_k + conj X_{M-k}) + i (X_k - conj X_{M-k}) conj(W^k)
   for (int i = tid; i < NL * M; i += NT) {
     const int ln = i / M, k = i - ln * M;
-    const float2 xk = s1[ln * LS + k];
-    const float2 xc = cconj(s1[ln * LS + (M - k)]);
+    const float2 xk = s1[ln * LS + px(k)];
+    const float2 xc = cconj(s1[ln * LS + px((M - k))]);
     const float2 a = cadd(xk, xc);
     const float2 b = cmulc(csub(xk, xc), twN[k]);
-    s0[ln * LS + k] = make_float2(a.x - b.y, a.y + b.x);
+    s0[ln * LS + px(k)] = make_float2(a.x - b.y, a.y + b.x);
   }
   __syncthreads();
   const int ln = tid / T, t = tid - ln * T;
-  fft_run<M, true>(s0 + ln * LS, s1 + ln * LS, t, twM);
+  fft_run<M, true>(s0 + ln * LS, s1 + ln * LS, t, stw);
   const float2* res = res_in_b<M>() ? s1 : s0;
   float* dst = out + ((size_t)x * NY + y0) * NZ * NC;
   for (int i = tid; i < nl * NZ * NC; i += NT) {
     const int ll = i / (NZ * NC), rem = i - ll * (NZ * NC);
     const int z = rem / NC, c = rem - z * NC;
-    const float2 v = res[(c * L + ll) * LS + (z >> 1)];
+    const float2 v = res[(c * L + ll) * LS + px((z >> 1))];
     dst[i] = (z & 1) ? v.y : v.x;
   }
 }
