@@ -123,6 +123,19 @@ int pmx_step_keep(pmx_plan plan, float* x, float* v, int npx, int npy, int npz, 
 int pmx_power(pmx_plan plan, const float* mesh, const double* kedges, int nedges, double* pk_sum, double* k_sum,
               double* k_cnt, void* stream);
 
+/* Cross spectra and Legendre multipoles of the same estimator (_power with mesh2 and multipoles,
+ * src/jax_fli/_src/power/_compute.py:5-20, 47-63, 92-112). For each of the npoles poles l = poles[j]
+ * (host array, 0 <= l <= 16, 1 <= npoles <= 16) and bin b:
+ *   p_re[j*(nedges+1) + b] = sum Re(m1 conj(m2)) (2l+1) P_l(mu),  p_im[...] = the same sum of Im,
+ * m1 = FFT(mesh), m2 = FFT(mesh2) (m2 = m1 when mesh2 is NULL: then p_im is not written and may be
+ * NULL), mu = k.los / |k| (0 at k = 0) with los (host double[3]) normalised here; los may be NULL
+ * when every pole is 0. k_sum, k_cnt as for pmx_power. The caller forms |sum| = sqrt(re^2 + im^2) for
+ * cross spectra, as the reference does. float64 device outputs, overwritten; 1 <= nedges <= 512.
+ * Uses the plan's spectrum buffers: calls on one plan must not overlap. Needs pmx_plan_set_box. */
+int pmx_power_multipoles(pmx_plan plan, const float* mesh, const float* mesh2, const double* kedges, int nedges,
+                         const int* poles, int npoles, const double* los, double* p_re, double* p_im,
+                         double* k_sum, double* k_cnt, void* stream);
+
 /* Counter-based white noise N(0,1) (Philox4x32-10 + Box-Muller) filling (nx,ny,nz) -- stands in for
  * normal_field(key) (initial.py:93). pmx_philox4x32 writes raw 4x32 blocks for known-answer tests. */
 int pmx_white_noise(pmx_plan plan, uint64_t seed, float* out, void* stream);

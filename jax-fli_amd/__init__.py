@@ -17,7 +17,7 @@ from .painting import PaintingOptions
 from .pm import (AbstractCorrection, AbstractInterp, AbstractNBodySolver, DriftInterp, EfficientDriftDoubleKick,
                  InterpTilerState, NBodyState, NoCorrection, NoInterp, OnionTiler, PGDKernel,
                  ReversibleDoubleKickDrift, SharpeningKernel, TelephotoInterp, lpt, nbody)
-from .power import PowerSpectrum, power
+from .power import PowerSpectrum, coherence, power, transfer
 from .utils import centers, compute_lightcone_shells, distances, edges
 
 __version__ = "0.1.0"

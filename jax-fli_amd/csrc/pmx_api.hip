@@ -3,6 +3,7 @@
 #include "pmx_internal.h"
 #include "../../include/pmx.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -933,26 +934,66 @@ int pmx_plan_timing_read(pmx_plan p, double* ms, int* nsteps) {
 
 int pmx_power(pmx_plan p, const float* mesh, const double* kedges, int nedges, double* pk_sum, double* k_sum,
               double* k_cnt, void* stream) {
+  const int pole0 = 0;
+  return pmx_power_multipoles(p, mesh, nullptr, kedges, nedges, &pole0, 1, nullptr, pk_sum, nullptr, k_sum, k_cnt,
+                              stream);
+}
+
+int pmx_power_multipoles(pmx_plan p, const float* mesh, const float* mesh2, const double* kedges, int nedges,
+                         const int* poles, int npoles, const double* los, double* p_re, double* p_im,
+                         double* k_sum, double* k_cnt, void* stream) {
   PMX_TRY(check_plan(p));
   if (!p->box_set) return pmx_set_error(1, "pmx_plan_set_box must be called before pmx_power");
-  if (!mesh || !kedges || !pk_sum || !k_sum || !k_cnt || nedges < 1 || nedges > 512)
+  if (!mesh || !kedges || !p_re || !k_sum || !k_cnt || nedges < 1 || nedges > 512)
     return pmx_set_error(1, "bad power-spectrum arguments");
+  if (!poles || npoles < 1 || npoles > 16) return pmx_set_error(1, "npoles must be in [1, 16]");
+  int lmax = 0;
+  for (int j = 0; j < npoles; ++j) {
+    if (poles[j] < 0 || poles[j] > 16) return pmx_set_error(1, "multipole orders must be in [0, 16]");
+    lmax = std::max(lmax, poles[j]);
+  }
+  if (mesh2 && !p_im) return pmx_set_error(1, "a cross spectrum needs p_im");
+  double u[3] = {0.0, 0.0, 0.0};
+  if (lmax > 0) {
+    if (!los) return pmx_set_error(1, "multipoles need a line of sight");
+    const double nrm = std::sqrt(los[0] * los[0] + los[1] * los[1] + los[2] * los[2]);
+    if (!(nrm > 0.0) || !std::isfinite(nrm)) return pmx_set_error(1, "line of sight must be a nonzero vector");
+    for (int c = 0; c < 3; ++c) u[c] = los[c] / nrm;
+  }
+  const int per = pmx_xpass_power_poles(p, nedges);
+  if (per < 1) return pmx_set_error(1, "too many k bins for this mesh");
   hipStream_t s = (hipStream_t)stream;
-  PMX_TRY(pmx_launch_fill_d(pk_sum, nedges + 1, s));
-  PMX_TRY(pmx_launch_fill_d(k_sum, nedges + 1, s));
-  PMX_TRY(pmx_launch_fill_d(k_cnt, nedges + 1, s));
-  PMX_TRY(pmx_fft_forward_to_B(p, mesh, s));
+  const int nb = nedges + 1;
+  PMX_TRY(pmx_launch_fill_d(p_re, (size_t)npoles * nb, s));
+  if (mesh2) PMX_TRY(pmx_launch_fill_d(p_im, (size_t)npoles * nb, s));
+  PMX_TRY(pmx_launch_fill_d(k_sum, nb, s));
+  PMX_TRY(pmx_launch_fill_d(k_cnt, nb, s));
   KOp op{};
+  if (mesh2) {  // m2 in full [kz][ky][kx] layout in specC; specA/B are then reused for m1
+    PMX_TRY(pmx_fft_forward_to_B(p, mesh2, s));
+    op.mode = XP_FWD;
+    PMX_TRY(pmx_xpass(p, op, p->specB, p->specC, s));
+  }
+  PMX_TRY(pmx_fft_forward_to_B(p, mesh, s));
   op.mode = XP_POWER;
   op.kphys_x = p->kphys_x;
   op.kphys_y = p->kphys_y;
   op.kphys_z = p->kphys_z;
   op.kedges = kedges;
   op.nedges = nedges;
-  op.pk_sum = pk_sum;
   op.k_sum = k_sum;
   op.k_cnt = k_cnt;
-  return pmx_xpass(p, op, p->specB, nullptr, s);
+  op.spec2 = mesh2 ? p->specC : nullptr;
+  for (int c = 0; c < 3; ++c) op.los[c] = u[c];
+  for (int j0 = 0; j0 < npoles; j0 += per) {
+    op.npoles = std::min(per, npoles - j0);
+    for (int j = 0; j < op.npoles; ++j) op.poles[j] = poles[j0 + j];
+    op.pk_sum = p_re + (size_t)j0 * nb;
+    op.pk_im = mesh2 ? p_im + (size_t)j0 * nb : nullptr;
+    op.acc_k = (j0 == 0);
+    PMX_TRY(pmx_xpass(p, op, p->specB, nullptr, s));
+  }
+  return 0;
 }
 
 int pmx_white_noise(pmx_plan p, uint64_t seed, float* out, void* stream) {

@@ -289,6 +289,18 @@ __device__ __forceinline__ double interp_lin(double x, const double* __restrict_
   return fp[ind] + (x - xi) * (fp[ind + 1] - fp[ind]) / (xp[ind + 1] - xi);
 }
 
+// Legendre polynomial P_l(mu) by the three-term recurrence n P_n = (2n-1) mu P_{n-1} - (n-1) P_{n-2}
+__device__ __forceinline__ double legendre(double mu, int ell) {
+  if (ell == 0) return 1.0;
+  double p0 = 1.0, p1 = mu;
+  for (int n = 2; n <= ell; ++n) {
+    const double pn = ((2 * n - 1) * mu * p1 - (n - 1) * p0) / n;
+    p0 = p1;
+    p1 = pn;
+  }
+  return p1;
+}
+
 template <int N, int MODE>
 __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, float2* __restrict__ out, int NY,
                                                int NZ, long ocomp, KOp op, XTabs tb,
@@ -394,19 +406,27 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
       if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + px(k)];
     }
   } else if constexpr (MODE == XP_POWER) {
-    // bins: 0 .. nedges (nedges + 1 entries), bin = number of edges <= |k|
-    extern __shared__ double hist[];  // 3 * (nedges + 1)
-    const int nb = op.nedges + 1;
-    for (int i = tid; i < 3 * nb; i += NT) hist[i] = 0.0;
+    // bins: 0 .. nedges (nedges + 1 entries), bin = number of edges <= |k|. The half spectrum holds
+    // one of each pair (k, k') of an interior kz plane, k' the index-mirror of k: the spectrum there is
+    // the conjugate, |k'| = |k|, and k' is -k except on the x/y Nyquist lines, whose fftfreq component
+    // keeps its sign; mu' is taken from k' itself. Planes kz = 0 and kz = nz/2 hold every mode once.
+    extern __shared__ double hist[];  // [npoles re][npoles im][k sum][count] x (nedges + 1)
+    const int nb = op.nedges + 1, np = op.npoles;
+    for (int i = tid; i < (2 * np + 2) * nb; i += NT) hist[i] = 0.0;
     __syncthreads();
-    const double mult = (kz == 0 || 2 * kz == NZ) ? 1.0 : 2.0;
+    const bool interior = !(kz == 0 || 2 * kz == NZ);
+    const double mult = interior ? 2.0 : 1.0;
+    const double kzp = op.kphys_z[kz];
+    int lmax = 0;
+    for (int j = 0; j < np; ++j) lmax = max(lmax, op.poles[j]);
     for (int i = tid; i < nl * N; i += NT) {
       const int ll = i / N, kx = i - ll * N;
       const int ky = ky0 + ll;
+      const double kxp = op.kphys_x[kx], kyp = op.kphys_y[ky];
       double s = 0.0;
-      s = __dadd_rn(s, __dmul_rn(op.kphys_x[kx], op.kphys_x[kx]));
-      s = __dadd_rn(s, __dmul_rn(op.kphys_y[ky], op.kphys_y[ky]));
-      s = __dadd_rn(s, __dmul_rn(op.kphys_z[kz], op.kphys_z[kz]));
+      s = __dadd_rn(s, __dmul_rn(kxp, kxp));
+      s = __dadd_rn(s, __dmul_rn(kyp, kyp));
+      s = __dadd_rn(s, __dmul_rn(kzp, kzp));
       const double kk = __dsqrt_rn(s);
       int lo = 0, hi = op.nedges;  // upper_bound
       while (lo < hi) {
@@ -414,16 +434,53 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
         if (op.kedges[mid] <= kk) lo = mid + 1; else hi = mid;
       }
       const float2 d = res[ll * LS + px(kx)];
-      const double p = ((double)d.x * d.x + (double)d.y * d.y) * mult;
-      atomicAdd(&hist[lo], p);
-      atomicAdd(&hist[nb + lo], kk * mult);
-      atomicAdd(&hist[2 * nb + lo], mult);
+      double re, im = 0.0;
+      if (op.spec2) {  // m1 conj(m2)
+        const float2 e = op.spec2[((size_t)kz * NY + ky) * N + kx];
+        re = (double)d.x * e.x + (double)d.y * e.y;
+        im = (double)d.y * e.x - (double)d.x * e.y;
+      } else {
+        re = (double)d.x * d.x + (double)d.y * d.y;
+      }
+      if (lmax == 0) {
+        atomicAdd(&hist[lo], re * mult);
+        if (op.spec2 && !interior) atomicAdd(&hist[np * nb + lo], im);
+      } else {
+        const double inv = (kk == 0.0) ? 0.0 : 1.0 / kk;
+        const double mu = (kk == 0.0) ? 0.0 : (kxp * op.los[0] + kyp * op.los[1] + kzp * op.los[2]) * inv;
+        double mu2 = 0.0;
+        if (interior) {
+          const double kxq = op.kphys_x[kx ? N - kx : 0], kyq = op.kphys_y[ky ? NY - ky : 0];
+          mu2 = (kk == 0.0) ? 0.0 : (kxq * op.los[0] + kyq * op.los[1] - kzp * op.los[2]) * inv;
+        }
+        for (int j = 0; j < np; ++j) {
+          const int ell = op.poles[j];
+          double w = legendre(mu, ell) * (2 * ell + 1);
+          double wr = w, wi = w;
+          if (interior) {
+            const double w2 = legendre(mu2, ell) * (2 * ell + 1);
+            wr = w + w2;
+            wi = w - w2;
+          }
+          atomicAdd(&hist[j * nb + lo], re * wr);
+          if (op.spec2) atomicAdd(&hist[(np + j) * nb + lo], im * wi);
+        }
+      }
+      if (op.acc_k) {
+        atomicAdd(&hist[2 * np * nb + lo], kk * mult);
+        atomicAdd(&hist[(2 * np + 1) * nb + lo], mult);
+      }
     }
     __syncthreads();
-    for (int i = tid; i < nb; i += NT) {
+    for (int i = tid; i < np * nb; i += NT) {
       if (hist[i] != 0.0) atomicAdd(&op.pk_sum[i], hist[i]);
-      if (hist[nb + i] != 0.0) atomicAdd(&op.k_sum[i], hist[nb + i]);
-      if (hist[2 * nb + i] != 0.0) atomicAdd(&op.k_cnt[i], hist[2 * nb + i]);
+      if (op.spec2 && hist[np * nb + i] != 0.0) atomicAdd(&op.pk_im[i], hist[np * nb + i]);
+    }
+    if (op.acc_k) {
+      for (int i = tid; i < nb; i += NT) {
+        if (hist[2 * np * nb + i] != 0.0) atomicAdd(&op.k_sum[i], hist[2 * np * nb + i]);
+        if (hist[(2 * np + 1) * nb + i] != 0.0) atomicAdd(&op.k_cnt[i], hist[(2 * np + 1) * nb + i]);
+      }
     }
   }
 }
@@ -564,7 +621,7 @@ template <int N, int MODE> int launch_xpass(pmx_plan_t* p, const KOp& op, const 
   constexpr int L = LPW<N>(), T = TPL<N>();
   dim3 grid((p->ny + L - 1) / L, p->nzh, p->nslots), block(L * T);
   XTabs tb{p->grad_x, p->grad_y, p->grad_z, p->w2_x, p->w2_y, p->w2_z, p->dec_x, p->dec_y, p->dec_z};
-  size_t shm = (MODE == XP_POWER) ? 3 * (size_t)(op.nedges + 1) * sizeof(double) : 0;
+  size_t shm = (MODE == XP_POWER) ? (2 * (size_t)op.npoles + 2) * (op.nedges + 1) * sizeof(double) : 0;
   hipLaunchKernelGGL((k_xpass<N, MODE>), grid, block, shm, s, in, out, p->ny, p->nz, (long)p->nspec, op, tb, p->tw_x,
                      p->slots);
   return pmx_check_launch("xpass");
@@ -619,6 +676,26 @@ int pmx_fft_forward_to_B(pmx_plan_t* p, const float* in, hipStream_t s) {
   int rc = pmx_pass_r2c_z(p, in, p->specA, s);
   if (rc) return rc;
   return pmx_pass_c2c_y(p, p->specA, p->specB, s);
+}
+
+namespace {
+template <int N> constexpr size_t xpass_static_lds() {
+  return (2 * (size_t)LPW<N>() * LSP<N>() + N) * sizeof(float2);
+}
+}  // namespace
+
+int pmx_xpass_power_poles(const pmx_plan_t* p, int nedges) {
+  // per-workgroup LDS kept within 64 KiB: the x-line buffers plus the (2 npoles + 2) histograms
+  size_t stat = 0;
+  switch (p->nx) {
+#define M_(v) case v: stat = xpass_static_lds<v>(); break;
+    PMX_FOR_AXIS(M_)
+#undef M_
+    default: return 0;
+  }
+  const size_t budget = 65536 - stat, per = (size_t)(nedges + 1) * sizeof(double);
+  const long np = (long)(budget / per) / 2 - 1;
+  return np < 1 ? 0 : (np > kMaxPolesLaunch ? kMaxPolesLaunch : (int)np);
 }
 
 int pmx_xpass(pmx_plan_t* p, const KOp& op, const float2* inB, float2* out, hipStream_t s) {

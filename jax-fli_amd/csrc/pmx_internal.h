@@ -118,16 +118,26 @@ struct KOp {
   const double* ptab;
   int ntab;
   double cnorm;
-  // power spectrum binning
+  // power spectrum binning: per pole, sums of Re and Im of m1 conj(m2) (2l+1) P_l(mu) over the
+  // nedges+1 bins of |k|; the auto spectrum has m2 = m1 and no Im sums
   const double* kedges;
   int nedges;
-  double* pk_sum;         // nedges+1 bins
-  double* k_sum;
+  double* pk_sum;         // npoles x (nedges+1) real sums
+  double* pk_im;          // npoles x (nedges+1) imaginary sums, cross spectra only (else null)
+  double* k_sum;          // nedges+1 (accumulated only when acc_k)
   double* k_cnt;
+  const float2* spec2;    // second mesh's full spectrum [kz][ky][kx] (cross spectrum) or null
+  int npoles;             // 1..kMaxPolesLaunch
+  int poles[8];
+  double los[3];          // unit line of sight; mu = k.los / |k| (0 at k = 0)
+  int acc_k;
 };
+constexpr int kMaxPolesLaunch = 8;
 
 int pmx_fft_forward_to_B(pmx_plan_t* p, const float* in, hipStream_t s);                 // R -> specB
 int pmx_xpass(pmx_plan_t* p, const KOp& op, const float2* inB, float2* out, hipStream_t s);  // specB -> out
+// poles one XP_POWER launch can bin for nedges edges on this plan (0: too many edges)
+int pmx_xpass_power_poles(const pmx_plan_t* p, int nedges);
 int pmx_inverse_from_C(pmx_plan_t* p, int ncomp, const float2* inC, float* out, hipStream_t s);  // specC -> real
 int pmx_pass_r2c_z(pmx_plan_t* p, const float* in, float2* outA, hipStream_t s);
 int pmx_pass_c2c_y(pmx_plan_t* p, const float2* inA, float2* outB, hipStream_t s);

@@ -189,6 +189,32 @@ def power_sums(mesh: torch.Tensor, box, kedges: torch.Tensor):
     return pk, ks, kc
 
 
+def power_multipole_sums(mesh: torch.Tensor, mesh2: torch.Tensor | None, box, kedges: torch.Tensor, poles, los):
+    """Device sums (re, im, k_sum, k_cnt): re/im of shape (npoles, nedges+1) (im None for an auto
+    spectrum), float64 (pmx_power_multipoles)."""
+    mesh = _f32(mesh, "mesh")
+    if mesh2 is not None:
+        mesh2 = _f32(mesh2, "mesh2")
+        if tuple(mesh2.shape) != tuple(mesh.shape):
+            raise ValueError("mesh2 must have the mesh's shape")
+    plan = get_plan(mesh.shape, mesh.device)
+    plan.set_box(box)
+    kedges = to_device(kedges, mesh.device)
+    nb = kedges.numel() + 1
+    poles = [int(l) for l in poles]
+    re = torch.empty((len(poles), nb), dtype=torch.float64, device=mesh.device)
+    im = torch.empty_like(re) if mesh2 is not None else None
+    ks = torch.empty(nb, dtype=torch.float64, device=mesh.device)
+    kc = torch.empty_like(ks)
+    cp = (ctypes.c_int * len(poles))(*poles)
+    cl = (ctypes.c_double * 3)(*[float(v) for v in los]) if los is not None else None
+    check(load().pmx_power_multipoles(plan.handle, ptr(mesh), ptr(mesh2) if mesh2 is not None else None, ptr(kedges),
+                                      kedges.numel(), cp, len(poles), cl, ptr(re),
+                                      ptr(im) if im is not None else None, ptr(ks), ptr(kc), stream_ptr()),
+          "pmx_power_multipoles")
+    return re, im, ks, kc
+
+
 def white_noise(mesh_shape, seed: int, device=None, out: torch.Tensor | None = None, plan=None) -> torch.Tensor:
     plan = plan or get_plan(mesh_shape, device)
     if out is None:

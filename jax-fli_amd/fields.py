@@ -331,19 +331,66 @@ class DensityField(AbstractField):
                 FieldStatus.LPT1, FieldStatus.LPT2, FieldStatus.PARTICLES):
             raise ValueError(f"DensityField array shape {s} does not match mesh_size {self.mesh_size}")
 
+    def _pairs(self, other, what):
+        a = self.array
+        if a.ndim not in (3, 4):
+            raise ValueError(f"DensityField.{what} expects array shape (X,Y,Z) or (B,X,Y,Z)")
+        if other is None:
+            return [(x, None) for x in ([a] if a.ndim == 3 else list(a))]
+        b = other.array
+        if tuple(b.shape) != tuple(a.shape):
+            raise ValueError(f"other array must match shape for {what}")
+        return [(a, b)] if a.ndim == 3 else list(zip(list(a), list(b)))
+
     def power(self, mesh2=None, *, kedges=None, multipoles=0, los=(0.0, 0.0, 1.0), batch_size=None):
+        """3D power spectrum P(k) (fields/density.py:276-320): auto, or cross with mesh2 (pairwise over
+        a batch), for the pole(s) `multipoles` about `los`."""
+        from .power import PowerSpectrum, power
+
+        ks, pks = [], []
+        for x, y in self._pairs(mesh2, "power"):
+            k, pk = power(x, y, box_shape=self.box_size, kedges=kedges, multipoles=multipoles, los=los)
+            ks.append(k)
+            pks.append(pk)
+        pk = pks[0] if self.array.ndim == 3 else np.stack(pks)
+        return PowerSpectrum(wavenumber=ks[0], spectra=pk, name="pk", scale_factors=self.scale_factors)
+
+    def cross_power(self, *, kedges=None, multipoles=0, los=(0.0, 0.0, 1.0), batch_size=None):
+        """All B(B+1)/2 spectra of a batch of B >= 2 fields in upper-triangular order (0,0), (0,1), ...,
+        (B-1,B-1) (fields/density.py:322-402)."""
         from .power import PowerSpectrum, power
 
         data = self.array
-        arrs = [data] if data.ndim == 3 else list(data)
-        ks, pks = [], []
-        for a in arrs:
-            k, pk = power(a, None if mesh2 is None else mesh2.array, box_shape=self.box_size, kedges=kedges,
-                          multipoles=multipoles, los=los)
-            ks.append(k)
-            pks.append(pk)
-        pk = pks[0] if data.ndim == 3 else np.stack(pks)
-        return PowerSpectrum(wavenumber=ks[0], spectra=pk, name="pk", scale_factors=self.scale_factors)
+        if data.ndim != 4:
+            raise ValueError(f"cross_power requires batched array with shape (B, X, Y, Z), got array with "
+                             f"{data.ndim} dimensions. Use power() for single fields.")
+        nb = data.shape[0]
+        if nb < 2:
+            raise ValueError(f"cross_power requires at least 2 maps in batch dimension, got {nb}. "
+                             "Use power() for single fields.")
+        poles = tuple(multipoles) if isinstance(multipoles, (list, tuple)) else (multipoles,)
+        k, pks = None, []
+        for i in range(nb):
+            for j in range(i, nb):
+                k, pk = power(data[i], data[j], box_shape=self.box_size, kedges=kedges, multipoles=poles, los=los)
+                pks.append(pk)
+        return PowerSpectrum(wavenumber=k, spectra=np.stack(pks), name="cross_pk", scale_factors=self.scale_factors)
+
+    def transfer(self, other, *, kedges=None, batch_size=None):
+        """Monopole transfer function sqrt(P_other / P_self) (fields/density.py:404-438)."""
+        from .power import PowerSpectrum, transfer
+
+        out = [transfer(x, y, box_shape=self.box_size, kedges=kedges) for x, y in self._pairs(other, "transfer")]
+        sp = out[0][1] if self.array.ndim == 3 else np.stack([o[1] for o in out])
+        return PowerSpectrum(wavenumber=out[0][0], spectra=sp, name="transfer", scale_factors=self.scale_factors)
+
+    def coherence(self, other, *, kedges=None, batch_size=None):
+        """Monopole coherence P01 / sqrt(P0 P1) (fields/density.py:440-475)."""
+        from .power import PowerSpectrum, coherence
+
+        out = [coherence(x, y, box_shape=self.box_size, kedges=kedges) for x, y in self._pairs(other, "coherence")]
+        sp = out[0][1] if self.array.ndim == 3 else np.stack([o[1] for o in out])
+        return PowerSpectrum(wavenumber=out[0][0], spectra=sp, name="coherence", scale_factors=self.scale_factors)
 
 
 @dataclasses.dataclass(frozen=True, eq=False)

@@ -276,11 +276,29 @@ def pgd_displacement(pos, mesh_shape, alpha, kl, ks, mode="relative", dtype=np.f
 # _power / _initialize_pk (src/jax_fli/_src/power/_compute.py:23-119), auto spectrum, monopole.
 
 
-def power(mesh, box_shape, kedges=None):
+def legendre(mu, ell):
+    """P_ell(mu) by the three-term recurrence (src/jax_fli/_src/power/_compute.py:8-20)."""
+    mu = np.asarray(mu, dtype=np.float64)
+    if ell == 0:
+        return np.ones_like(mu)
+    p0, p1 = np.ones_like(mu), mu
+    for n in range(2, ell + 1):
+        p0, p1 = p1, ((2 * n - 1) * mu * p1 - (n - 1) * p0) / n
+    return p1
+
+
+def power(mesh, box_shape, kedges=None, mesh2=None, multipoles=0, los=(0.0, 0.0, 1.0)):
+    """_power (src/jax_fli/_src/power/_compute.py:23-119) in float64 over the full c2c grid: bins by
+    digitize(|k|, kedges); auto |m|^2 or cross m1 conj(m2) (ortho FFTs), weighted by (2l+1) P_l(mu),
+    mu = k.los/|k| (0 at k = 0; no weighting when multipoles == 0); the cross sum per bin is
+    sqrt(re^2 + im^2); P = sum / count * prod(box / mesh), under- and overflow bins dropped. One pole:
+    (kavg, pk); several: (kavg, pk of shape (npoles, nbins))."""
     mesh = np.asarray(mesh, dtype=np.float64)
     mesh_shape = mesh.shape
     box = np.asarray(box_shape, dtype=np.float64)
     ms = np.asarray(mesh_shape)
+    poles = tuple(multipoles) if isinstance(multipoles, (list, tuple)) else (multipoles,)
+    use_los = not (isinstance(multipoles, int) and multipoles == 0)
     if kedges is None:
         kmax = np.pi * np.min(ms / box)
         dk = 2 * np.pi / np.min(box) * 2
@@ -293,17 +311,51 @@ def power(mesh, box_shape, kedges=None):
         shape = [1, 1, 1]
         shape[i] = m
         kvec.append(((2 * np.pi * m / b) * np.fft.fftfreq(m)).reshape(shape))
-    kmesh = np.sqrt(sum(k**2 for k in kvec))
-    dig = np.digitize(np.broadcast_to(kmesh, mesh_shape).reshape(-1), kedges)
+    kmesh = np.broadcast_to(np.sqrt(sum(k**2 for k in kvec)), mesh_shape)
+    dig = np.digitize(kmesh.reshape(-1), kedges)
     nb = kedges.size + 1
     kcount = np.bincount(dig, minlength=nb)
-    kavg = np.bincount(dig, weights=np.broadcast_to(kmesh, mesh_shape).reshape(-1), minlength=nb)
+    kavg = np.bincount(dig, weights=kmesh.reshape(-1), minlength=nb)
     kavg = (kavg / np.where(kcount == 0, 1, kcount))[1:-1]
+    if use_los:
+        u = np.asarray(los, dtype=np.float64)
+        u = u / np.linalg.norm(u)
+        mu = np.broadcast_to(sum(k * c for k, c in zip(kvec, u)), mesh_shape)
+        mu = np.where(kmesh == 0, 0.0, mu / np.where(kmesh == 0, 1.0, kmesh))
+    else:
+        mu = np.ones(mesh_shape)
     meshk = fftn(mesh, norm="ortho")
-    mmk = meshk.real**2 + meshk.imag**2
-    psum = np.bincount(dig, weights=mmk.reshape(-1), minlength=nb)
-    pk = (psum / np.where(kcount > 0, kcount, 1))[1:-1] * np.prod(box / ms)
-    return kavg, pk
+    if mesh2 is None:
+        mmk = meshk.real**2 + meshk.imag**2
+    else:
+        mmk = meshk * np.conj(fftn(np.asarray(mesh2, dtype=np.float64), norm="ortho"))
+    out = []
+    for ell in poles:
+        w = (mmk * (2 * int(ell) + 1) * legendre(mu, int(ell))).reshape(-1)
+        if mesh2 is None:
+            psum = np.bincount(dig, weights=w, minlength=nb)
+        else:
+            re = np.bincount(dig, weights=w.real, minlength=nb)
+            im = np.bincount(dig, weights=w.imag, minlength=nb)
+            psum = np.sqrt(re**2 + im**2)
+        out.append(psum)
+    pk = (np.stack(out) / np.where(kcount > 0, kcount, 1))[:, 1:-1] * np.prod(box / ms)
+    return (kavg, pk[0]) if len(poles) == 1 else (kavg, pk)
+
+
+def transfer(mesh0, mesh1, box_shape, kedges=None):
+    """sqrt(P1 / P0), monopoles (src/jax_fli/_src/power/_compute.py:240-244)."""
+    k, p0 = power(mesh0, box_shape, kedges)
+    _, p1 = power(mesh1, box_shape, kedges)
+    return k, (p1 / p0) ** 0.5
+
+
+def coherence(mesh0, mesh1, box_shape, kedges=None):
+    """P01 / sqrt(P0 P1), monopoles (src/jax_fli/_src/power/_compute.py:247-252)."""
+    k, p01 = power(mesh0, box_shape, kedges, mesh2=mesh1)
+    _, p0 = power(mesh0, box_shape, kedges)
+    _, p1 = power(mesh1, box_shape, kedges)
+    return k, p01 / (p0 * p1) ** 0.5
 
 
 # ---------------------------------------------------------------------------- Philox4x32-10

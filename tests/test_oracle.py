@@ -115,6 +115,75 @@ def test_power_plane_wave_single_bin():
     assert pk[b] == pytest.approx((n**3 / 2) / count * (L / n) ** 3, rel=1e-12)
 
 
+def test_legendre_matches_numpy_legval():
+    mu = np.random.default_rng(3).uniform(-1, 1, 50)
+    for ell in range(9):
+        c = np.zeros(ell + 1)
+        c[ell] = 1.0
+        assert np.allclose(orc.legendre(mu, ell), np.polynomial.legendre.legval(mu, c), atol=1e-14, rtol=0)
+
+
+def _plane_wave(n, m, phase=0.0, axis=0):
+    x = 2 * np.pi * m * np.arange(n) / n + phase
+    shape = [1, 1, 1]
+    shape[axis] = n
+    return np.broadcast_to(np.cos(x).reshape(shape), (n, n, n))
+
+
+def test_power_multipoles_of_a_plane_wave():
+    """A plane wave along x holds two modes, mu = +-1 for a line of sight along x and mu = 0 across it:
+    P_l = (2l+1) P_0 for even l along, 0 for odd l; across, P_l = (2l+1) P_l(0) P_0."""
+    n, L = 32, 100.0
+    mesh = _plane_wave(n, 4)
+    kedges = np.linspace(0.01, 1.5, 40)
+    _, p0 = orc.power(mesh, (L, L, L), kedges)
+    b = int(np.argmax(p0))
+    _, along = orc.power(mesh, (L, L, L), kedges, multipoles=(0, 1, 2, 3, 4), los=(2.0, 0.0, 0.0))
+    assert np.allclose(along[:, b] / p0[b], [1, 0, 5, 0, 9], atol=1e-12)
+    _, across = orc.power(mesh, (L, L, L), kedges, multipoles=(0, 2, 4), los=(0.0, 0.0, 1.0))
+    assert np.allclose(across[:, b] / p0[b], [1, 5 * -0.5, 9 * 3 / 8], atol=1e-12)
+    _, tilted = orc.power(mesh, (L, L, L), kedges, multipoles=[2], los=(1.0, 1.0, 0.0))  # mu^2 = 1/2
+    assert tilted[b] / p0[b] == pytest.approx(5 * (1.5 * 0.5 - 0.5), abs=1e-12)
+
+
+def test_cross_power_of_shifted_plane_waves():
+    """cos(kx) against cos(kx + phi): the two modes give m1 conj(m2) = A e^{-+i phi}, so the cross
+    monopole is |cos phi| P_0 and the dipole along x is 3 |sin phi| P_0 (the imaginary parts)."""
+    n, L, phi = 32, 100.0, 0.7
+    m1, m2 = _plane_wave(n, 4), _plane_wave(n, 4, phase=phi)
+    kedges = np.linspace(0.01, 1.5, 40)
+    _, p0 = orc.power(m1, (L, L, L), kedges)
+    b = int(np.argmax(p0))
+    _, c0 = orc.power(m1, (L, L, L), kedges, mesh2=m2)
+    assert c0[b] / p0[b] == pytest.approx(np.cos(phi), abs=1e-12)
+    _, c1 = orc.power(m1, (L, L, L), kedges, mesh2=m2, multipoles=(0, 1), los=(1.0, 0.0, 0.0))
+    assert c1[1][b] / p0[b] == pytest.approx(3 * np.sin(phi), abs=1e-12)
+    _, cm = orc.power(m1, (L, L, L), kedges, mesh2=-m1)  # the magnitude of the sum
+    assert np.allclose(cm, p0, rtol=1e-12, atol=0)
+    k, coh = orc.coherence(m1, m2, (L, L, L), kedges)
+    assert coh[b] == pytest.approx(np.cos(phi), abs=1e-12)
+
+
+def test_cross_power_identities_on_a_random_field():
+    rng = np.random.default_rng(11)
+    n, box = (16, 12, 8), (50.0, 40.0, 30.0)
+    a = rng.standard_normal(n)
+    k, pa = orc.power(a, box)
+    _, paa = orc.power(a, box, mesh2=a)
+    assert np.allclose(paa, pa, rtol=1e-12)
+    _, p2 = orc.power(a, box, mesh2=2.5 * a)
+    assert np.allclose(p2, 2.5 * pa, rtol=1e-12)
+    _, odd = orc.power(a, box, multipoles=(1, 3), los=(0.3, -0.2, 0.9))  # a real field's auto spectrum is even
+    assert np.abs(odd).max() < 1e-12 * pa.max()
+    _, tr = orc.transfer(a, 3.0 * a, box)
+    assert np.allclose(tr, 3.0, rtol=1e-12)
+    _, coh = orc.coherence(a, a + 1e-9 * rng.standard_normal(n), box)
+    assert np.allclose(coh, 1.0, atol=1e-6)
+    # multipoles == 0 applies no line of sight; [0] with any line of sight is the same monopole
+    _, m0 = orc.power(a, box, multipoles=[0], los=(1.0, 2.0, 3.0))
+    assert np.allclose(m0, pa, rtol=1e-12)
+
+
 # ---------------------------------------------------------------- Philox
 def test_philox_known_answers():
     # Random123 kat_vectors (philox4x32 10 rounds)
