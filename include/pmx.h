@@ -158,6 +158,19 @@ int pmx_lpt1(pmx_plan plan, const float* ic, float growth, float vel_factor, flo
 int pmx_lpt1_lattice(pmx_plan plan, const float* ic, int npx, int npy, int npz, float growth, float vel_factor,
                      float* dx, float* p, void* stream);
 
+/* LPT of order 1 or 2 at the particle lattice (jaxpm.pm.lpt as called by lpt, src/jax_fli/pm/lpt.py:119-127;
+ * upstream JaxPM: Jenkins 2010 eqs. 2-3):
+ *   dx = D1 F(ic) [+ D2 F(delta2)],  pv = c1 (D1 F(ic)) [+ c2 (D2 F(delta2))],
+ * F(.) the FastPM force of a density at the lattice, delta2 = sum_{i<j} (H_ii H_jj - H_ij^2),
+ * H_ij = ifft(g_i g_j fft(ic) / k^2) with g the FastPM gradient kernel. coef (float64 device array) holds
+ * the columns D1, c1, D2, c2 of ntab rows each: ntab == 1 constants (snapshot LPT: D1 = D(a), c1 = a^2 f E,
+ * D2 = 3/7 D2(a)/D2(1), c2 = a^2 f2 E); ntab >= 2 a table over the comoving distance r of each lattice
+ * particle, r = |lattice cell * box / mesh - observer| (Mpc/h), on the uniform grid [0, r_max], linearly
+ * interpolated (lightcone LPT, per-particle scale factors a(r)). order 1 reads columns D1, c1 only.
+ * Uses the plan's density and force buffers. */
+int pmx_lpt(pmx_plan plan, const float* ic, int npx, int npy, int npz, int order, const double* coef, int ntab,
+            double r_max, const double box[3], const double observer[3], float* dx, float* pv, void* stream);
+
 /* Cotangent of the initial field for pmx_lpt1 / pmx_lpt1_lattice: ic_bar = the transpose of
  * F(ic) applied to growth * (dx_bar + vel_factor * p_bar) placed on the lattice cells. dx_bar or
  * p_bar may be null (zero). Replaces jax.vjp of jaxpm.pm.lpt(order=1). */

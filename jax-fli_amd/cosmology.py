@@ -243,6 +243,64 @@ def dGfa(cosmo: Cosmology, a):
     return 1.5 * cosmo.Omega_m * growth_factor(cosmo, a) / (a**2 * E(cosmo, a))
 
 
+# ---------------------------------------------------------------------- second-order growth
+def _growth2_rhs(cosmo, x, y):
+    # y = (D, G, D2, G2) in ln a; the second-order mode obeys the linear operator with the source
+    # -1.5 Om(a) D^2:  dG2/dln a = -(2 + dlnE/dlna) G2 + 1.5 Om(a) (D2 - D^2)
+    D, G, D2, G2 = y
+    a = math.exp(x)
+    p = -3.0 * (1.0 + cosmo.w0 + cosmo.wa)
+    fde = a**p * math.exp(-3.0 * cosmo.wa * (1.0 - a))
+    om = cosmo.Omega_m * a**-3
+    ok = cosmo.Omega_k * a**-2
+    e2 = om + ok + cosmo.Omega_de * fde
+    dlne = 0.5 * (-3.0 * om - 2.0 * ok + cosmo.Omega_de * fde * (p + 3.0 * cosmo.wa * a)) / e2
+    r = 1.5 * om / e2
+    return (G, -(2.0 + dlne) * G + r * D, G2, -(2.0 + dlne) * G2 + r * (D2 - D * D))
+
+
+@functools.lru_cache(maxsize=256)
+def _growth2_table_bg(cosmo: _Background):
+    """RK4 in ln a of (D, G, D2, G2) from the matter-dominated growing modes D = a,
+    D2 = -3/7 a^2 at a = 1e-5, on the grid of _growth_table_bg; D2 and dD2/dln a normalised by
+    D2(a = 1), as jaxpm.growth.growth_factor_second returns it."""
+    x0 = math.log(_A_INIT)
+    h = -x0 / _NSTEP
+    n_tot = _NSTEP + int(math.ceil(math.log(_A_MAX) / h))
+    xs = x0 + h * np.arange(n_tot + 1)
+    y = (_A_INIT, _A_INIT, -3.0 / 7.0 * _A_INIT**2, -6.0 / 7.0 * _A_INIT**2)
+    ys = [y]
+    for i in range(n_tot):
+        x = x0 + h * i
+        k1 = _growth2_rhs(cosmo, x, y)
+        k2 = _growth2_rhs(cosmo, x + 0.5 * h, tuple(v + 0.5 * h * d for v, d in zip(y, k1)))
+        k3 = _growth2_rhs(cosmo, x + 0.5 * h, tuple(v + 0.5 * h * d for v, d in zip(y, k2)))
+        k4 = _growth2_rhs(cosmo, x + h, tuple(v + h * d for v, d in zip(y, k3)))
+        y = tuple(v + (h / 6.0) * (a + 2 * b + 2 * c + d) for v, a, b, c, d in zip(y, k1, k2, k3, k4))
+        ys.append(y)
+    Y = np.array(ys)
+    dG2 = np.array([_growth2_rhs(cosmo, xs[i], Y[i])[3] for i in range(n_tot + 1)])
+    norm = Y[_NSTEP, 2]
+    return xs, Y[:, 2] / norm, Y[:, 3] / norm, dG2 / norm
+
+
+def _growth2_DG(cosmo: Cosmology, a):
+    xs, D2, G2, dG2 = _growth2_table_bg(_background(cosmo))
+    return _hermite2(xs, D2, G2, G2, dG2, np.log(np.asarray(a, dtype=np.float64)))
+
+
+def growth_factor_second(cosmo: Cosmology, a):
+    """Second-order growth D2(a) / D2(1) (positive; the 2LPT displacement is 3/7 of it times the
+    second-order force)."""
+    return _growth2_DG(cosmo, a)[0]
+
+
+def growth_rate_second(cosmo: Cosmology, a):
+    """f2 = dln D2 / dln a."""
+    D2, G2 = _growth2_DG(cosmo, a)
+    return G2 / D2
+
+
 # ---------------------------------------------------------------------- distances
 _NCHI = 8192
 _A_MIN_CHI = 1e-3

@@ -21,6 +21,12 @@ int pmx_launch_philox_raw(uint32_t* out, size_t nblocks, uint64_t seed, hipStrea
 int pmx_launch_white_noise(float* out, size_t n, uint64_t seed, hipStream_t s);
 int pmx_launch_lpt1(pmx_plan_t* p, const float* F, float* dx, float* pv, int npx, int npy, int npz, float D, float c,
                     hipStream_t s);
+// delta2 of 2LPT from three Hessian components (interleaved (x,y,z,3)): diagonal pass
+// rho = H11 H00 + H22 (H00 + H11), off-diagonal pass rho -= H01^2 + H02^2 + H12^2
+int pmx_launch_delta2(const float* H, float* rho, size_t ncell, int offdiag, hipStream_t s);
+// dx (+)= D F, pv (+)= c D F at the lattice
+int pmx_launch_lpt_apply(pmx_plan_t* p, const float* F, float* dx, float* pv, int npx, int npy, int npz,
+                         const LptGeom& g, int term, int accumulate, hipStream_t s);
 int pmx_launch_lpt1_vjp(pmx_plan_t* p, const float* dxb, const float* pvb, int npx, int npy, int npz, float D, float c,
                         float* G, hipStream_t s);
 // launchers from pmx_sky.hip
@@ -1041,6 +1047,46 @@ int pmx_lpt1_lattice(pmx_plan p, const float* ic, int npx, int npy, int npz, flo
   hipStream_t s = (hipStream_t)stream;
   PMX_TRY(force_mesh_from(p, ic, 1.0f, 0u, p->fmesh, s));
   return pmx_launch_lpt1(p, p->fmesh, dx, pv, npx, npy, npz, growth, vel_factor, s);
+}
+
+int pmx_lpt(pmx_plan p, const float* ic, int npx, int npy, int npz, int order, const double* coef, int ntab,
+            double r_max, const double box[3], const double observer[3], float* dx, float* pv, void* stream) {
+  PMX_TRY(check_plan(p));
+  PMX_TRY(check_particles(p, dx, npx, npy, npz, 0));
+  if (!ic || !pv || !coef) return pmx_set_error(1, "null buffer");
+  if (order != 1 && order != 2) return pmx_set_error(1, "order must be 1 or 2");
+  if (ntab < 1 || ntab > (1 << 20)) return pmx_set_error(1, "ntab must be in [1, 2^20]");
+  LptGeom g{};
+  g.coef = coef;
+  g.ntab = ntab;
+  if (ntab > 1) {
+    if (!box || !observer || !(r_max > 0.0)) return pmx_set_error(1, "a coefficient table needs box, observer, r_max > 0");
+    const int n[3] = {p->nx, p->ny, p->nz};
+    for (int c = 0; c < 3; ++c) {
+      if (!(box[c] > 0.0)) return pmx_set_error(1, "box must be positive");
+      g.cell[c] = box[c] / (double)n[c];
+      g.obs[c] = observer[c];
+    }
+    g.r_max = r_max;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (order == 2) {
+    // delta2 into rho from the six Hessian components (two passes of three through fmesh)
+    PMX_TRY(pmx_fft_forward_to_B(p, ic, s));
+    KOp op{};
+    op.mode = XP_HESS;
+    op.scale = (float)(1.0 / (double)p->ncell);
+    for (int off = 0; off < 2; ++off) {
+      op.flags = off ? kHessOffDiag : 0u;
+      PMX_TRY(pmx_xpass(p, op, p->specB, p->specC, s));
+      PMX_TRY(pmx_inverse_from_C(p, 3, p->specC, p->fmesh, s));
+      PMX_TRY(pmx_launch_delta2(p->fmesh, p->rho, p->ncell, off, s));
+    }
+    PMX_TRY(force_mesh_from(p, p->rho, 1.0f, 0u, p->fmesh, s));
+    PMX_TRY(pmx_launch_lpt_apply(p, p->fmesh, dx, pv, npx, npy, npz, g, 1, 0, s));
+  }
+  PMX_TRY(force_mesh_from(p, ic, 1.0f, 0u, p->fmesh, s));
+  return pmx_launch_lpt_apply(p, p->fmesh, dx, pv, npx, npy, npz, g, 0, order == 2 ? 1 : 0, s);
 }
 
 int pmx_lpt1_vjp(pmx_plan p, const float* dx_bar, const float* p_bar, int npx, int npy, int npz, float growth,

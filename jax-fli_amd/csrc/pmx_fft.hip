@@ -306,7 +306,7 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
                                                int NZ, long ocomp, KOp op, XTabs tb,
                                                const float2* __restrict__ tw, Slots sl) {
   constexpr int T = TPL<N>(), L = LPW<N>(), LS = LSP<N>(), NT = L * T;
-  constexpr int NBUF = (MODE == XP_FORCES) ? 3 : 2;
+  constexpr int NBUF = (MODE == XP_FORCES || MODE == XP_HESS) ? 3 : 2;
   __shared__ float2 sm[NBUF * L * LS];
   const unsigned zs = blockIdx.z;
   if (!slot_live(sl, zs)) return;
@@ -355,6 +355,33 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
         if (op.flags & 1u) coef *= tb.dx[kx] * tb.dy[ky] * tb.dz[kz];
         const float2 d = res[ll * LS + px(kx)];
         s2[ll * LS + px(kx)] = make_float2(-coef * d.y, coef * d.x);  // i * coef * d
+      }
+      __syncthreads();
+      fft_run<N, true>(s2 + l * LS, oth + l * LS, t, stw);
+      const float2* r2 = res_in_b<N>() ? oth : s2;
+      float2* dst = out + c * ocomp + (size_t)kz * N * NY + ky0;
+      for (int i = tid; i < N * L; i += NT) {
+        const int k = i / L, ll = i - k * L;
+        if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + px(k)];
+      }
+      __syncthreads();
+    }
+  } else if constexpr (MODE == XP_HESS) {
+    // H_ij = ifft(g_i g_j delta / k^2): the FastPM gradient kernel twice on the potential -delta/k^2,
+    // (i g_i)(i g_j)(-delta / k^2); real coefficients, 0 at k = 0 and on Nyquist planes (g = 0 there)
+    float2* s2 = sm + 2 * L * LS;
+    const bool off = (op.flags & kHessOffDiag) != 0u;
+    for (int c = 0; c < 3; ++c) {
+      const int ci = off ? (c == 2 ? 1 : 0) : c, cj = off ? (c == 0 ? 1 : 2) : c;
+      for (int i = tid; i < L * N; i += NT) {
+        const int ll = i / N, kx = i - ll * N;
+        const int ky = min(ky0 + ll, NY - 1);
+        const float k2 = tb.w2x[kx] + tb.w2y[ky] + tb.w2z[kz];
+        const float gi = (ci == 0) ? tb.gx[kx] : (ci == 1 ? tb.gy[ky] : tb.gz[kz]);
+        const float gj = (cj == 0) ? tb.gx[kx] : (cj == 1 ? tb.gy[ky] : tb.gz[kz]);
+        const float coef = (k2 > 0.f) ? gi * gj * op.scale / k2 : 0.f;
+        const float2 d = res[ll * LS + px(kx)];
+        s2[ll * LS + px(kx)] = make_float2(coef * d.x, coef * d.y);
       }
       __syncthreads();
       fft_run<N, true>(s2 + l * LS, oth + l * LS, t, stw);
@@ -709,6 +736,7 @@ int pmx_xpass(pmx_plan_t* p, const KOp& op, const float2* inB, float2* out, hipS
       case XP_COLOR: return launch_xpass<v, XP_COLOR>(p, op, inB, out, s);  \
       case XP_POWER: return launch_xpass<v, XP_POWER>(p, op, inB, out, s);  \
       case XP_POT: return launch_xpass<v, XP_POT>(p, op, inB, out, s);      \
+      case XP_HESS: return launch_xpass<v, XP_HESS>(p, op, inB, out, s);    \
     }                                                                       \
     return pmx_set_error(2, "bad k-space mode");
     PMX_FOR_AXIS(M_)

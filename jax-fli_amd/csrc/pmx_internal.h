@@ -100,7 +100,17 @@ bool pmx_fft_len_ok(int n);
 
 // k-space operators of the x pass: forward only, inverse only, 3 force components (Green x FastPM
 // gradient), colouring by sqrt(P(k)), power-spectrum binning, potential delta/k^2
-enum { XP_FWD = 0, XP_INV = 1, XP_FORCES = 2, XP_COLOR = 3, XP_POWER = 4, XP_POT = 5 };
+enum { XP_FWD = 0, XP_INV = 1, XP_FORCES = 2, XP_COLOR = 3, XP_POWER = 4, XP_POT = 5, XP_HESS = 6 };
+// XP_HESS: three components of the Hessian of the potential, H_ij = ifft(g_i g_j fft(src) / k^2):
+// the diagonal (00, 11, 22), or with this flag the off-diagonal (01, 02, 12)
+constexpr unsigned kHessOffDiag = 8u;
+// LPT term at the lattice (pmx_particles.hip): (D, c) = columns (2 term, 2 term + 1) of coef; ntab == 1:
+// constants, else a table over the particle's comoving distance r on the uniform grid [0, r_max]
+struct LptGeom {
+  const double* coef;
+  int ntab;
+  double r_max, cell[3], obs[3];
+};
 
 // FFT passes (pmx_fft.hip). All asynchronous on `stream`. Return 0 or an error code.
 struct KOp {

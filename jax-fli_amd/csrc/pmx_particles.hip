@@ -1307,6 +1307,60 @@ __global__ void k_lpt1(const float* __restrict__ F, float* __restrict__ dx, floa
   }
 }
 
+// 2LPT source delta2 = sum_{i<j} (H_ii H_jj - H_ij^2) from the interleaved Hessian components
+__global__ void k_delta2(const float* __restrict__ H, float* __restrict__ rho, size_t n, int offdiag) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    const float h0 = H[3 * q], h1 = H[3 * q + 1], h2 = H[3 * q + 2];
+    if (offdiag)
+      rho[q] = rho[q] - (h0 * h0 + h1 * h1 + h2 * h2);
+    else
+      rho[q] = h1 * h0 + h2 * (h0 + h1);
+  }
+}
+
+// One LPT term at the lattice: dx (+)= D F, pv (+)= c (D F). (D, c) are columns 2 term and 2 term + 1
+// of g.coef: constants (ntab == 1) or linear interpolation in the particle's comoving distance
+// r = |lattice position * cell - observer| on the uniform grid of ntab points over [0, r_max].
+template <bool ACC>
+__global__ void k_lpt_apply(const float* __restrict__ F, float* __restrict__ dx, float* __restrict__ pv, int npx,
+                            int npy, int npz, int rx, int ry, int rz, int NY, int NZ, LptGeom g, int term) {
+  const size_t n = (size_t)npx * npy * npz;
+  const double* cD = g.coef + (size_t)(2 * term) * g.ntab;
+  const double* cV = g.coef + (size_t)(2 * term + 1) * g.ntab;
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+    const unsigned k = (unsigned)(p % (unsigned)npz), j = (unsigned)((p / (unsigned)npz) % (unsigned)npy),
+                   i = (unsigned)(p / ((size_t)npz * npy));
+    const size_t cell = ((size_t)i * rx * NY + (size_t)j * ry) * NZ + (size_t)k * rz;
+    float D, c;
+    if (g.ntab == 1) {
+      D = (float)cD[0];
+      c = (float)cV[0];
+    } else {
+      const double px_ = (double)(i * rx) * g.cell[0] - g.obs[0];
+      const double py_ = (double)(j * ry) * g.cell[1] - g.obs[1];
+      const double pz_ = (double)(k * rz) * g.cell[2] - g.obs[2];
+      const double r = sqrt(px_ * px_ + py_ * py_ + pz_ * pz_);
+      double t = r / g.r_max * (double)(g.ntab - 1);
+      t = t < 0.0 ? 0.0 : (t > (double)(g.ntab - 1) ? (double)(g.ntab - 1) : t);
+      int it = (int)t;
+      if (it > g.ntab - 2) it = g.ntab - 2;
+      const double fr = t - (double)it;
+      D = (float)(cD[it] + fr * (cD[it + 1] - cD[it]));
+      c = (float)(cV[it] + fr * (cV[it + 1] - cV[it]));
+    }
+    for (int d = 0; d < 3; ++d) {
+      const float v = F[3 * cell + d] * D;
+      if (ACC) {
+        dx[3 * p + d] += v;
+        pv[3 * p + d] += v * c;
+      } else {
+        dx[3 * p + d] = v;
+        pv[3 * p + d] = v * c;
+      }
+    }
+  }
+}
+
 // transpose of k_lpt1 onto the force mesh: G[cell] = D (dx_bar + c pv_bar) of the particle on that
 // lattice cell, 0 on the cells between lattice sites (every cell is written)
 __global__ void k_lpt1_vjp(const float* __restrict__ dxb, const float* __restrict__ pvb, float* __restrict__ G,
@@ -1631,6 +1685,21 @@ int pmx_launch_lpt1(pmx_plan_t* p, const float* F, float* dx, float* pv, int npx
   hipLaunchKernelGGL(k_lpt1, dim3(grid_for(n, 256)), dim3(256), 0, s, F, dx, pv, npx, npy, npz, p->nx / npx,
                      p->ny / npy, p->nz / npz, p->ny, p->nz, D, c);
   return pmx_check_launch("lpt1");
+}
+int pmx_launch_delta2(const float* H, float* rho, size_t ncell, int offdiag, hipStream_t s) {
+  hipLaunchKernelGGL(k_delta2, dim3(grid_for(ncell, 256)), dim3(256), 0, s, H, rho, ncell, offdiag);
+  return pmx_check_launch("delta2");
+}
+int pmx_launch_lpt_apply(pmx_plan_t* p, const float* F, float* dx, float* pv, int npx, int npy, int npz,
+                         const LptGeom& g, int term, int accumulate, hipStream_t s) {
+  const size_t n = (size_t)npx * npy * npz;
+  if (accumulate)
+    hipLaunchKernelGGL(k_lpt_apply<true>, dim3(grid_for(n, 256)), dim3(256), 0, s, F, dx, pv, npx, npy, npz,
+                       p->nx / npx, p->ny / npy, p->nz / npz, p->ny, p->nz, g, term);
+  else
+    hipLaunchKernelGGL(k_lpt_apply<false>, dim3(grid_for(n, 256)), dim3(256), 0, s, F, dx, pv, npx, npy, npz,
+                       p->nx / npx, p->ny / npy, p->nz / npz, p->ny, p->nz, g, term);
+  return pmx_check_launch("lpt_apply");
 }
 int pmx_launch_lpt1_vjp(pmx_plan_t* p, const float* dxb, const float* pvb, int npx, int npy, int npz, float D, float c,
                         float* G, hipStream_t s) {

@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -259,6 +260,25 @@ def lpt1(ic: torch.Tensor, growth: float, vel_factor: float, out=None, plan=None
         dx, pv = out
     check(load().pmx_lpt1_lattice(plan.handle, ptr(ic), npx, npy, npz, float(growth), float(vel_factor), ptr(dx),
                                   ptr(pv), stream_ptr()), "pmx_lpt1_lattice")
+    return dx, pv
+
+
+def lpt(ic: torch.Tensor, order: int, coef, particles=None, r_max: float = 0.0, box=None, observer=None, plan=None):
+    """LPT of order 1 or 2 at the particle lattice (pmx_lpt): coef is (4,) = (D1, c1, D2, c2) constants or
+    (4, ntab) columns over the comoving distance grid [0, r_max] of a lightcone (box, observer in Mpc/h)."""
+    ic = _f32(ic, "ic")
+    plan = plan or get_plan(ic.shape, ic.device)
+    npx, npy, npz = tuple(ic.shape) if particles is None else tuple(int(n) for n in particles)
+    coef = np.asarray(coef, dtype=np.float64)
+    ntab = 1 if coef.ndim == 1 else coef.shape[1]
+    if coef.shape[0] != 4:
+        raise ValueError("coef must hold the four columns D1, c1, D2, c2")
+    cdev = torch.from_numpy(np.ascontiguousarray(coef.reshape(4, ntab))).to(ic.device)
+    dx = torch.empty((npx, npy, npz, 3), dtype=torch.float32, device=ic.device)
+    pv = torch.empty_like(dx)
+    check(load().pmx_lpt(plan.handle, ptr(ic), npx, npy, npz, int(order), ptr(cdev), ntab, float(r_max),
+                         dbl3(box) if box is not None else None, dbl3(observer) if observer is not None else None,
+                         ptr(dx), ptr(pv), stream_ptr()), "pmx_lpt")
     return dx, pv
 
 

@@ -134,11 +134,17 @@ class AbstractField:
     def __getitem__(self, key):
         if not self.is_batched():
             raise ValueError("indexing needs a batched field")
+        n = self.array.shape[0]
+        if isinstance(key, slice) and key.step is not None and key.step < 0:  # torch has no negative steps
+            key = np.arange(n)[key].copy()
+
         def pick(v):
-            if isinstance(v, (torch.Tensor, np.ndarray)) and np.ndim(v) >= 1 and v.shape[0] == self.array.shape[0]:
+            if isinstance(v, (torch.Tensor, np.ndarray)) and np.ndim(v) >= 1 and v.shape[0] == n:
+                if isinstance(v, torch.Tensor) and isinstance(key, np.ndarray):
+                    return v[torch.as_tensor(key, device=v.device)]
                 return v[key]
             return v
-        return self.replace(array=self.array[key], scale_factors=pick(self.scale_factors),
+        return self.replace(array=pick(self.array), scale_factors=pick(self.scale_factors),
                             z_sources=pick(self.z_sources), comoving_centers=pick(self.comoving_centers),
                             density_width=pick(self.density_width))
 
@@ -261,27 +267,44 @@ class ParticleField(AbstractField):
                             scale_factors=self.scale_factors, unit=DensityUnit.DENSITY)
 
     # -------------------------------------------------------------- flat-sky (particles.py:234-322)
+    def _shell_sets(self, centers, widths, what):
+        """Particle sets painted into every shell (particles.py:256-283, 363-392): a 5D batch paints all
+        of its S sets into each of its S shells (the reference's lightcone mode); a 4D field paints its
+        one set into one shell, or into several when it is a lightcone with per-particle scale factors."""
+        data = self.array
+        if centers.shape != widths.shape:
+            raise ValueError(f"center and density_plane_width must have the same shape; got {centers.shape} and "
+                             f"{widths.shape}")
+        if data.ndim == 5:
+            if centers.size != data.shape[0]:
+                raise ValueError(f"Batched input: center must have {data.shape[0]} elements, got {centers.size}")
+            return [data[s] for s in range(data.shape[0])]
+        if data.ndim == 4:
+            if centers.size != 1:
+                sf = self.scale_factors
+                if sf is None or tuple(np.shape(np.squeeze(np.asarray(sf)))) != tuple(self.mesh_size) or \
+                        self.status != FieldStatus.LIGHTCONE:
+                    raise ValueError("Painting with mutiple centers/widths requires batched input data ")
+            return [data]
+        raise ValueError(f"{what}() expects 4D or 5D array, got shape {tuple(data.shape)}")
+
     def paint_2d(self, center, density_plane_width, *, weights=None, batch_size=None) -> "FlatDensity":
         if self.flatsky_npix is None:
             raise ValueError("ParticleField requires `flatsky_npix` for paint_2d.")
         mode = _paint_mode(self.unit)
-        data = self.array
         centers = np.atleast_1d(np.asarray(center, dtype=np.float64))
         widths = np.atleast_1d(np.asarray(density_plane_width, dtype=np.float64))
-        if centers.shape != widths.shape:
-            raise ValueError("center and density_plane_width must have the same shape")
-        if data.ndim == 4:
-            if centers.size != 1:
-                raise ValueError("Painting with mutiple centers/widths requires batched input data ")
-            data = data[None]
-        elif data.ndim != 5 or data.shape[0] != centers.size:
-            raise ValueError("batched input needs one center/width per snapshot")
+        sets = self._shell_sets(centers, widths, "paint_2d")
         maps = []
-        for b in range(data.shape[0]):
+        for b in range(centers.size):
             zlo, zhi, scale = flat_slab(self, centers[b], widths[b])
-            maps.append(engine.paint_flat(data[b].contiguous(), self.mesh_size, self.flatsky_npix, zlo, zhi, scale,
-                                          mode=mode, weights=1.0 if weights is None else weights))
-        arr = torch.stack(maps).squeeze(0) if len(maps) > 1 else maps[0]
+            m = None
+            for k, part in enumerate(sets):
+                m = engine.paint_flat(part.contiguous(), self.mesh_size, self.flatsky_npix, zlo, zhi, scale,
+                                      mode=mode, weights=1.0 if weights is None else weights, out=m,
+                                      accumulate=k > 0)
+            maps.append(m)
+        arr = torch.stack(maps) if len(maps) > 1 else maps[0]
         return FlatDensity.FromDensityMetadata(array=arr, field=self, status=FieldStatus.LIGHTCONE,
                                                comoving_centers=centers, density_width=widths,
                                                unit=DensityUnit.DENSITY)
@@ -298,24 +321,19 @@ class ParticleField(AbstractField):
         if kernel_width_arcmin is not None or paint_nside not in (None, self.nside):
             raise NotImplementedError("smoothing kernels and paint_nside resampling are not supported")
         mode = _paint_mode(self.unit)
-        data = self.array
         centers = np.atleast_1d(np.asarray(center, dtype=np.float64))
         widths = np.atleast_1d(np.asarray(density_plane_width, dtype=np.float64))
-        if centers.shape != widths.shape:
-            raise ValueError("center and density_plane_width must have the same shape")
-        if data.ndim == 4:
-            if centers.size != 1:
-                raise ValueError("Painting with mutiple centers/widths requires batched input data ")
-            data = data[None]
-        elif data.ndim != 5 or data.shape[0] != centers.size:
-            raise ValueError("batched input needs one center/width per shell")
+        sets = self._shell_sets(centers, widths, "paint_spherical")
         maps = []
-        for b in range(data.shape[0]):
+        for b in range(centers.size):
             c, wdt = centers[b], widths[b]
-            maps.append(engine.paint_healpix(data[b].contiguous(), self.mesh_size, self.box_size,
-                                             self.observer_position_mpc, c - wdt / 2, c + wdt / 2, self.nside,
-                                             scheme=scheme, mode=mode, weights=1.0 if weights is None else weights,
-                                             tiles=tiles))
+            m = None
+            for k, part in enumerate(sets):
+                m = engine.paint_healpix(part.contiguous(), self.mesh_size, self.box_size, self.observer_position_mpc,
+                                         c - wdt / 2, c + wdt / 2, self.nside, scheme=scheme, mode=mode,
+                                         weights=1.0 if weights is None else weights, tiles=tiles, out=m,
+                                         accumulate=k > 0)
+            maps.append(m)
         arr = torch.stack(maps) if len(maps) > 1 else maps[0]
         return SphericalDensity.FromDensityMetadata(array=arr, field=self, status=FieldStatus.LIGHTCONE,
                                                     comoving_centers=centers, density_width=widths,

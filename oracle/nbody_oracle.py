@@ -101,6 +101,32 @@ class Cosmo:
         D, Dp, Dpp = self.D_and_derivs(a)
         return Dpp * a**3 * self.E(a) + Dp * a**3 * self.dEda(a) + 3 * a**2 * self.E(a) * Dp
 
+    # second order: D2'' + (3/a + E'/E) D2' = 1.5 Om (D2 - D^2) / (a^5 E^2), D2 = -3/7 a^2 in matter
+    # domination, integrated with D (unnormalised) and normalised by D2(1)
+    def _rhs2(self, a, y):
+        D, Dp, D2, D2p = y
+        dlnE = float(self.dEda(a)[0] / self.E(a)[0])
+        r = 1.5 * self.Omega_m / (a**5 * float(self.Esqr(a)))
+        q = 3.0 / a + dlnE
+        return [Dp, -q * Dp + r * D, D2p, -q * D2p + r * (D2 - D * D)]
+
+    def _solve2(self):
+        if getattr(self, "_sol2", None) is None:
+            a0 = 1e-5
+            self._sol2 = _si.solve_ivp(self._rhs2, (a0, 2.1), [a0, 1.0, -3.0 / 7.0 * a0**2, -6.0 / 7.0 * a0],
+                                       method="DOP853", rtol=1e-12, atol=1e-20, dense_output=True)
+            self._norm2 = self._sol2.sol(1.0)[2]
+        return self._sol2
+
+    def growth_factor_second(self, a):
+        a = np.atleast_1d(np.asarray(a, dtype=np.float64))
+        return self._solve2().sol(a)[2] / self._norm2
+
+    def growth_rate_second(self, a):
+        a = np.atleast_1d(np.asarray(a, dtype=np.float64))
+        y = self._solve2().sol(a)
+        return a * y[3] / y[2]
+
     def chi(self, a):
         a = np.atleast_1d(np.asarray(a, dtype=np.float64))
         f = lambda x: C_OVER_H0 / (x * x * float(self.E(x)[0]))
@@ -198,8 +224,53 @@ def lpt1(c: Cosmo, ic, a):
     F = orc.force_mesh(np.asarray(ic, dtype=np.float64))
     D = float(c.growth_factor(a)[0])
     dx = D * F
-    p = a**2 * float(c.growth_rate(a)[0]) * float(c.E(a)) * dx
+    p = a**2 * float(c.growth_rate(a)[0]) * float(c.E(a)[0]) * dx
     return dx, p
+
+
+def lpt(c: Cosmo, ic, a, order=1):
+    """jaxpm.pm.lpt at the lattice (src/jax_fli/pm/lpt.py:119-127; upstream JaxPM, Jenkins 2010 eqs.
+    2-3): dx = D F(ic) [+ 3/7 D2(a)/D2(1) F(delta2)], p = a^2 f E D F [+ a^2 f2 E 3/7 D2 F(delta2)].
+    `a` is a scalar or an array broadcasting against (X, Y, Z, 1) (per-particle scale factors)."""
+    ic = np.asarray(ic, dtype=np.float64)
+    a = np.asarray(a, dtype=np.float64)
+    F = orc.force_mesh(ic)
+    shp = a.shape
+    af = a.reshape(-1)
+    D = c.growth_factor(af).reshape(shp)
+    E = c.E(af).reshape(shp)
+    f = c.growth_rate(af).reshape(shp)
+    dx = D * F
+    p = a**2 * f * E * dx
+    if order == 2:
+        F2 = orc.force_mesh(orc.lpt2_source(ic))
+        dx2 = 3.0 / 7.0 * c.growth_factor_second(af).reshape(shp) * F2
+        p2 = a**2 * c.growth_rate_second(af).reshape(shp) * E * dx2
+        dx = dx + dx2
+        p = p + p2
+    return dx, p
+
+
+def a_of_chi(c: Cosmo, chi, a_min=1e-3, n=4001):
+    """Inverse of the comoving distance: chi(a) on a geometric grid of a (per-segment quadrature),
+    ln a as a cubic spline of chi."""
+    key = (a_min, n)
+    if getattr(c, "_achi", None) is None or c._achi[0] != key:
+        a = np.geomspace(a_min, 1.0, n)
+        f = lambda x: C_OVER_H0 / (x * x * float(c.E(x)[0]))
+        seg = np.array([_si.quad(f, a[i], a[i + 1], epsabs=1e-13, epsrel=1e-13)[0] for i in range(n - 1)])
+        chi_tab = np.concatenate([np.cumsum(seg[::-1])[::-1], [0.0]])
+        from scipy.interpolate import CubicSpline
+        c._achi = (key, CubicSpline(chi_tab[::-1], np.log(a)[::-1]))
+    return np.exp(c._achi[1](np.asarray(chi, dtype=np.float64)))
+
+
+def lightcone_scale_factors(c: Cosmo, mesh_shape, box, observer):
+    """compute_particle_scale_factors (src/jax_fli/utils.py:234-244) for the lattice particles:
+    a(|cell * box / mesh - observer|), shape (X, Y, Z, 1)."""
+    ax = [np.arange(n) * (b / n) - o for n, b, o in zip(mesh_shape, box, observer)]
+    r = np.sqrt(ax[0][:, None, None] ** 2 + ax[1][None, :, None] ** 2 + ax[2][None, None, :] ** 2)
+    return a_of_chi(c, r)[..., None]
 
 
 # ------------------------------------------------------------------------------ time loop + solvers

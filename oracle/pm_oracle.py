@@ -256,6 +256,31 @@ def force_mesh(delta, deconv=False, dtype=np.float64, pgd=None):
     return out
 
 
+def hessian(delta):
+    """H_ij = ifft(grad_i grad_j invlaplace fft(delta)) = ifft(g_i g_j delta_k / k^2), the shear of the
+    potential with the FastPM gradient kernel (upstream JaxPM lpt, order 2). Returns {(i, j): mesh}."""
+    mesh_shape = delta.shape
+    dk = fftn(np.asarray(delta, dtype=np.float64))
+    kvec = fftk(mesh_shape, np.float64)
+    pot = dk * invlaplace_kernel(kvec)
+    g = [gradient_kernel(kvec, i) for i in range(3)]
+    return {(i, j): ifftn(g[i] * g[j] * pot).real for i in range(3) for j in range(i, 3)}
+
+
+def lpt2_source(delta):
+    """delta2 = sum_{i<j} (H_ii H_jj - H_ij^2), accumulated as the upstream loop does: per i, add
+    H_ii times the running sum of the earlier diagonals, subtract the squares H_ij^2 for j > i."""
+    H = hessian(delta)
+    d2 = np.zeros(delta.shape)
+    acc = np.zeros(delta.shape)
+    for i in range(3):
+        d2 = d2 + H[i, i] * acc
+        acc = acc + H[i, i]
+        for j in range(i + 1, 3):
+            d2 = d2 - H[i, j] ** 2
+    return d2
+
+
 def pm_forces(pos, mesh_shape, mode="relative", scale=1.0, deconv=False, dtype=np.float64):
     """jaxpm.pm.pm_forces(pos) * scale (scale = 1.5 Omega_m in _forces, solvers.py:29-30)."""
     rho = cic_paint(pos, mesh_shape, mode, dtype=dtype)

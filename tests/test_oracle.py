@@ -254,6 +254,71 @@ def test_growth_equation_identity_lcdm():
     assert np.allclose(c.dGfa(a), 1.5 * c.Omega_m * c.growth_factor(a) / (a**2 * c.E(a)), rtol=1e-6)
 
 
+def test_second_order_growth_eds_and_bouchet():
+    """EdS: the growing mode D2 = -3/7 a^2 gives D2/D2(1) = a^2 and f2 = 2 exactly. LCDM: D2(1) against
+    -3/7 Om^(-1/143) D(1)^2 and f2 against 2 Om(a)^(6/11) (Bouchet et al. 1995 fits, ~0.1-1%)."""
+    a = np.array([0.05, 0.2, 0.5, 1.0, 1.4])
+    e = nbo.Cosmo(Omega_c=0.95, Omega_b=0.05)
+    assert np.allclose(e.growth_factor_second(a), a**2, rtol=1e-9)
+    assert np.allclose(e.growth_rate_second(a), 2.0, rtol=1e-9)
+    c = nbo.Cosmo()
+    y = c._solve2().sol(1.0)
+    assert y[2] / (-3.0 / 7.0 * y[0] ** 2) == pytest.approx(c.Omega_m ** (-1.0 / 143.0), rel=5e-4)
+    om_a = c.Omega_m / (a**3 * c.Esqr(a))
+    assert np.allclose(c.growth_rate_second(a), 2 * om_a ** (6.0 / 11.0), rtol=6e-3)
+    assert c.growth_factor_second(1.0)[0] == pytest.approx(1.0, abs=1e-14)
+
+
+def _gk(w):
+    return (8 * np.sin(w) - np.sin(2 * w)) / 6
+
+
+def test_lpt2_source_closed_forms():
+    """cos(w1 x) + cos(w2 y): only H_xx H_yy survives, delta2 = ab g1^2 g2^2 / (w1^2 w2^2) cos cos; a
+    single oblique plane wave has no second-order source (H_xx H_yy = H_xy^2)."""
+    n = 16
+    x = np.arange(n)
+    w1, w2 = 2 * np.pi * 3 / n, 2 * np.pi * 2 / n
+    al, be = 0.7, -1.3
+    d = al * np.cos(w1 * x)[:, None, None] + be * np.cos(w2 * x)[None, :, None] + 0 * x[None, None, :]
+    d2 = orc.lpt2_source(d)
+    ref = al * be * _gk(w1) ** 2 * _gk(w2) ** 2 / (w1**2 * w2**2) * np.cos(w1 * x)[:, None, None] * np.cos(w2 * x)[None, :, None]
+    assert np.allclose(d2, np.broadcast_to(ref, d2.shape), atol=1e-13)
+    w3 = 2 * np.pi * 5 / n
+    one = np.cos(w1 * x[:, None, None] + w2 * x[None, :, None] - w3 * x[None, None, :])
+    assert np.abs(orc.lpt2_source(one)).max() < 1e-13
+
+
+def test_lpt2_displacement_closed_form():
+    """dx(order 2) - dx(order 1) = 3/7 D2 F(delta2); for delta2 = C cos(w1 x) cos(w2 y) the FastPM force
+    is -C/2 g(k) [sin(w1 x + w2 y) +- sin(w1 x - w2 y)] / (w1^2 + w2^2) per axis."""
+    n, a = 16, 0.4
+    c = nbo.Cosmo()
+    x = np.arange(n)
+    w1, w2 = 2 * np.pi * 3 / n, 2 * np.pi * 2 / n
+    d = (np.cos(w1 * x)[:, None, None] + np.cos(w2 * x)[None, :, None]) * np.ones((1, 1, n))
+    dx1, p1 = nbo.lpt(c, d, a, order=1)
+    dx2, p2 = nbo.lpt(c, d, a, order=2)
+    C = _gk(w1) ** 2 * _gk(w2) ** 2 / (w1**2 * w2**2)
+    sp = np.sin(w1 * x[:, None] + w2 * x[None, :])[..., None]
+    sm = np.sin(w1 * x[:, None] - w2 * x[None, :])[..., None]
+    k2 = w1**2 + w2**2
+    F2 = np.zeros((n, n, n, 3))
+    F2[..., 0] = -C / 2 * _gk(w1) * (sp + sm) / k2
+    F2[..., 1] = -C / 2 * _gk(w2) * (sp - sm) / k2
+    D2 = 3.0 / 7.0 * c.growth_factor_second(a)[0]
+    assert np.allclose(dx2 - dx1, D2 * F2, atol=1e-13)
+    v2 = a**2 * c.growth_rate_second(a)[0] * c.E(a)[0]
+    assert np.allclose(p2 - p1, v2 * D2 * F2, atol=1e-13)
+
+
+def test_a_of_chi_inverts_chi():
+    c = nbo.Cosmo()
+    a = np.array([0.15, 0.4, 0.77, 0.999])
+    assert np.allclose(nbo.a_of_chi(c, c.chi(a)), a, rtol=1e-9)
+    assert nbo.a_of_chi(c, 0.0) == pytest.approx(1.0, abs=1e-12)
+
+
 def test_eh98_and_sigma8():
     """T -> 1 as k -> 0, and sigma(8 Mpc/h) of the normalised P(k) by scipy's adaptive quadrature (not the
     Gauss-Legendre rule the normalisation itself uses)."""
