@@ -1009,18 +1009,20 @@ __device__ __forceinline__ int base_cell(int mode, int lattice, float xc) {
   return (mode == 0 ? lattice : 0) + (int)floorf(xc);
 }
 
-// Wave-wide min / max through DPP lane moves (quad swaps, row mirrors, row broadcasts 15 and 31):
-// lane 63 ends with the result of all 64 lanes, which every lane reads back. All lanes must be active.
+// Wave-wide min / max: four DPP steps fused into v_min/v_max (quad swaps, half-row and row mirrors;
+// bound_ctrl is harmless as every lane has a source) leave each 16-lane row's extreme in all of its
+// lanes; the four rows are then combined from lanes 0, 16, 32, 48 on the scalar unit. All lanes
+// must be active.
 template <bool MAX>
 __device__ __forceinline__ int wave_minmax(int v) {
   auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
-  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
-  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
-  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
-  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));  // row_mirror
-  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));  // row_bcast:15 into rows 1, 3
-  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));  // row_bcast:31 into rows 2, 3
-  return __builtin_amdgcn_readlane(v, 63);
+  v = op(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));   // quad_perm [1,0,3,2]
+  v = op(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));   // quad_perm [2,3,0,1]
+  v = op(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));  // row_half_mirror
+  v = op(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true));  // row_mirror
+  const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+  const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+  return op(op(r0, r1), op(r2, r3));
 }
 
 // base cell (unwrapped) and the two CIC weights along one axis, with the expressions of corner_rel
@@ -1086,26 +1088,30 @@ __global__ void __launch_bounds__(256) k_readout_lds(PartArgs a, const float* __
   int mn[3], mx[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    mn[c] = min(min(red[c][0], red[c][1]), min(red[c][2], red[c][3]));
-    mx[c] = max(max(red[3 + c][0], red[3 + c][1]), max(red[3 + c][2], red[3 + c][3])) + 1;  // upper corner
+    mn[c] = __builtin_amdgcn_readfirstlane(min(min(red[c][0], red[c][1]), min(red[c][2], red[c][3])));
+    mx[c] = __builtin_amdgcn_readfirstlane(
+                max(max(red[3 + c][0], red[3 + c][1]), max(red[3 + c][2], red[3 + c][3]))) + 1;  // upper corner
   }
-  const long long ex = (long long)mx[0] - mn[0] + 1, ey = (long long)mx[1] - mn[1] + 1,
-                  ez = (long long)mx[2] - mn[2] + 1;
+  // box extents; each is checked against the cap before the product so that 32 bits suffice
+  const unsigned ex = (unsigned)(mx[0] - mn[0] + 1), ey = (unsigned)(mx[1] - mn[1] + 1),
+                 ez = (unsigned)(mx[2] - mn[2] + 1);
+  const bool fits = ex <= (unsigned)RO_CAP && ey <= (unsigned)RO_CAP && ez <= (unsigned)RO_CAP &&
+                    ex * ey <= (unsigned)RO_CAP && ex * ey * ez <= (unsigned)RO_CAP;
   float f[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) f[c] = 0.f;
-  if (ex * ey * ez <= RO_CAP) {
+  if (fits) {
     const int EY = (int)ey, EZ = (int)ez, ncell = (int)(ex * ey * ez);
-    const float iy = 1.0f / (float)EY, iz = 1.0f / (float)EZ;
+    // v_rcp is within 1 ulp of 1/e: div_small's margin holds for q < 2^21
+    const float iy = __builtin_amdgcn_rcpf((float)EY), iz = __builtin_amdgcn_rcpf((float)EZ);
     // the box origin wrapped once; inside the box one conditional subtraction wraps each axis when
     // the box is no longer than the mesh, an exact modulo otherwise (meshes smaller than the box)
     const int ox = wrap_cell(mn[0], NX), oy = wrap_cell(mn[1], NY), oz = wrap_cell(mn[2], NZ);
-    const bool short_box = ex <= NX && ey <= NY && ez <= NZ;
-    for (int q = t; q < ncell; q += 256) {
+    auto cell_of = [&](int q, bool exact) {
       int qz, qy;
       const int qx = div_small(div_small(q, iz, EZ, qz), iy, EY, qy);
       int gx = ox + qx, gy = oy + qy, gz = oz + qz;
-      if (short_box) {
+      if (!exact) {
         gx -= gx >= NX ? NX : 0;
         gy -= gy >= NY ? NY : 0;
         gz -= gz >= NZ ? NZ : 0;
@@ -1114,9 +1120,37 @@ __global__ void __launch_bounds__(256) k_readout_lds(PartArgs a, const float* __
         gy %= NY;
         gz %= NZ;
       }
-      const float* m = mesh + (size_t)(((unsigned)gx * NY + (unsigned)gy) * NZ + (unsigned)gz) * NC;
+      return mesh + (size_t)(((unsigned)gx * NY + (unsigned)gy) * NZ + (unsigned)gz) * NC;
+    };
+    if (ex <= (unsigned)NX && ey <= (unsigned)NY && ez <= (unsigned)NZ) {
+      // two cells per thread and trip, both loads issued before either LDS store
+      int q = t;
+      for (; q + 256 < ncell; q += 512) {
+        const float* m0 = cell_of(q, false);
+        const float* m1 = cell_of(q + 256, false);
+        float a0[NC], a1[NC];
 #pragma unroll
-      for (int c = 0; c < NC; ++c) sm[q * NC + c] = m[c];
+        for (int c = 0; c < NC; ++c) {
+          a0[c] = m0[c];
+          a1[c] = m1[c];
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          sm[q * NC + c] = a0[c];
+          sm[(q + 256) * NC + c] = a1[c];
+        }
+      }
+      if (q < ncell) {
+        const float* m0 = cell_of(q, false);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) sm[q * NC + c] = m0[c];
+      }
+    } else {
+      for (int q = t; q < ncell; q += 256) {
+        const float* m = cell_of(q, true);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) sm[q * NC + c] = m[c];
+      }
     }
     __syncthreads();
     const int lx = ax.b - mn[0], ly = ay.b - mn[1], lz = az.b - mn[2];
