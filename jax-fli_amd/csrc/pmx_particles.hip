@@ -207,6 +207,7 @@ struct SortGeom {
   int nbx, nby, nbz;     // particle sub-blocks per axis
   int lpz, lpy;          // log2 of npz, npy when both are powers of two, else -1
   FDiv div_npz, div_npy;
+  int wmask;             // NX - 1 when NX, NY, NZ are all powers of two (wrap by masking), else -1
 };
 
 // periodic wrap, one compare per side for the usual |b| < N, exact modulo otherwise
@@ -218,6 +219,16 @@ __device__ __forceinline__ int wrap_cell(int b, int n) {
     c += c < 0 ? n : 0;
   }
   return c;
+}
+
+// (q / e, q % e) for small box and face indices, without an integer division:
+// q = a e + r, 0 <= r < e, puts (q + 1/2) / e at least 1/(2e) away from an integer, and the float
+// product (q + 1/2) * fl(1/e) is within about 2^-23 (q + 1) / e of it, inside that margin for any
+// q < 2^20 (the boxes hold at most a few thousand cells).
+__device__ __forceinline__ int div_small(int q, float inv_e, int e, int& rem) {
+  const int a = (int)(((float)q + 0.5f) * inv_e);
+  rem = q - a * e;
+  return a;
 }
 
 // lattice coordinates of particle p
@@ -252,9 +263,15 @@ __device__ __forceinline__ Base base_of(const PartArgs& a, const SortGeom& g, in
     bz += __mul24(k, g.rz);
   }
   Base b;
-  b.cx = wrap_cell(bx, g.NX);
-  b.cy = wrap_cell(by, g.NY);
-  b.cz = wrap_cell(bz, g.NZ);
+  if (g.wmask >= 0) {  // two's complement: b & (N - 1) is b mod N for any b
+    b.cx = bx & (g.NX - 1);
+    b.cy = by & (g.NY - 1);
+    b.cz = bz & (g.NZ - 1);
+  } else {
+    b.cx = wrap_cell(bx, g.NX);
+    b.cy = wrap_cell(by, g.NY);
+    b.cz = wrap_cell(bz, g.NZ);
+  }
   b.fx = x.x - flx;
   b.fy = x.y - fly;
   b.fz = x.z - flz;
@@ -676,21 +693,19 @@ __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, c
       *dst = accumulate ? *dst + v : v;
     }
     float* f = faces + comp * fcomp + (size_t)L * face_stride;
+    const float ipz = __builtin_amdgcn_rcpf((float)PZ);
     for (int q = threadIdx.x; q < nface; q += blockDim.x) {
       int lx, ly, lz;
       if (q < PY * PZ) {
         lx = BX;
-        ly = q / PZ;
-        lz = q % PZ;
+        ly = div_small(q, ipz, PZ, lz);
       } else if (q < PY * PZ + BX * PZ) {
-        const int r = q - PY * PZ;
-        lx = r / PZ;
+        lx = div_small(q - PY * PZ, ipz, PZ, lz);
         ly = BY;
-        lz = r % PZ;
       } else {
         const int r = q - PY * PZ - BX * PZ;
-        lx = r / BY;
-        ly = r % BY;
+        lx = r >> g.sy;
+        ly = r & (BY - 1);
         lz = BZ;
       }
       f[q] = (float)((double)(long long)tile[(lx * PY + ly) * PZ + lz] * inv);
@@ -994,16 +1009,6 @@ __global__ void __launch_bounds__(256) k_readout(PartArgs a, const float* __rest
 // larger boxes (shell crossings, wide spreads) gather from global memory as k_readout does. The
 // weights and the summation order are those of k_readout: the results are bitwise identical.
 constexpr int RB_I = 4, RB_J = 8, RB_K = 8, RO_CAP = 1664;
-
-// (q / e, q % e) for the box indices of the staging loops, without an integer division:
-// q = a e + r, 0 <= r < e, puts (q + 1/2) / e at least 1/(2e) away from an integer, and the float
-// product (q + 1/2) * fl(1/e) is within about 2^-23 (q + 1) / e of it, inside that margin for any
-// q < 2^20 (the boxes hold at most a few thousand cells).
-__device__ __forceinline__ int div_small(int q, float inv_e, int e, int& rem) {
-  const int a = (int)(((float)q + 0.5f) * inv_e);
-  rem = q - a * e;
-  return a;
-}
 
 __device__ __forceinline__ int base_cell(int mode, int lattice, float xc) {
   return (mode == 0 ? lattice : 0) + (int)floorf(xc);
@@ -1495,6 +1500,8 @@ static SortGeom sort_geom(const pmx_plan_t* p, const PartArgs& a, int& ntiles) {
   const bool pow2 = (a.npz & (a.npz - 1)) == 0 && (a.npy & (a.npy - 1)) == 0;
   g.lpz = pow2 ? ilog2(a.npz) : -1;
   g.lpy = pow2 ? ilog2(a.npy) : -1;
+  auto is_pow2 = [](int n) { return (n & (n - 1)) == 0; };
+  g.wmask = (is_pow2(p->nx) && is_pow2(p->ny) && is_pow2(p->nz)) ? p->nx - 1 : -1;
   return g;
 }
 
