@@ -356,6 +356,75 @@ def bench_large(pkg, dev, fd_dirs=2, sweep=(1e-2, 3e-3, 1e-3, 3e-4)):
                     "Born; the VJP runs the reverse adjoint with the shell-paint and Born adjoints"}
 
 
+def cpu_threads():
+    """Host threads of the CPU baseline: the process's CPU share (OMP_NUM_THREADS when set, as on a box
+    that shares its host between GPUs), else every core in the affinity mask."""
+    aff = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(aff, int(env))) if env and env.isdigit() else aff
+
+
+def bench_config1(pkg, dev, cpu=True):
+    """The 64^3 workload (64^3 particles on a 64^3 mesh, box 250 Mpc/h, 1LPT to a = 0.1, then
+    EfficientDriftDoubleKick with dt0 = 0.09 to a = 1): PM steps/s of the fused step, the unfused per-op
+    GPU step and the CPU restatement on all threads and on one."""
+    import torch
+    from importlib import import_module
+
+    from oracle import pm_oracle as orc
+
+    n, box = 64, 250.0
+    req = pkg.serving.SimRequest(rid=0, mesh=n, seed=0, Omega_c=0.25, sigma8=0.8, box=box, dt0=0.09)
+    cosmo = req.cosmology()
+    _, _, _, _, ops, force_scale = pkg.serving.forward_ops(req)
+    ic = pkg.gaussian_initial_conditions(0, (n,) * 3, (box,) * 3, cosmo=cosmo)
+    dx, p = pkg.lpt(cosmo, ic, ts=req.a0)
+    x0, v0 = dx.array.contiguous(), p.array.contiguous()
+    x, v = x0.clone(), v0.clone()
+    unf = import_module("jax_fli_amd.pm.unfused").UnfusedPM((n,) * 3, dev)
+
+    def fused(o):
+        pkg.engine.step(x, v, (n,) * 3, force_scale=force_scale, c_kick=o.kick, c_drift_pre=o.pre, c_drift_post=o.post)
+
+    def unfused(o):
+        unf.step(x, v, force_scale, o.kick, o.post or 0.0, c_drift_pre=o.pre)
+
+    def per_step(fn, reps=20):
+        for _ in range(2):
+            x.copy_(x0)
+            v.copy_(v0)
+            for o in ops:
+                fn(o)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            x.copy_(x0)
+            v.copy_(v0)
+            for o in ops:
+                fn(o)
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / (reps * len(ops))
+
+    out = {"workload": "64^3 particles on 64^3 mesh, box 250 Mpc/h, KDK dt0 0.09 from a=0.1 to 1",
+           "steps": len(ops), "fused_gpu_steps_per_s": round(1e3 / per_step(fused), 1),
+           "unfused_gpu_steps_per_s": round(1e3 / per_step(unfused), 1)}
+    if cpu:
+        xc0, vc0 = x0.cpu().numpy(), v0.cpu().numpy()
+        for label, w in (("cpu_steps_per_s", cpu_threads()), ("cpu_1core_steps_per_s", 1)):
+            xc, vc = xc0, vc0
+            orc.cpu_pm_step(xc, vc, force_scale, ops[1].kick, ops[1].post or 0.0, workers=w)  # plans, pages
+            t = time.perf_counter()
+            for o in ops:
+                if o.pre is not None:
+                    xc = xc + vc * np.float32(o.pre)
+                xc, vc = orc.cpu_pm_step(xc, vc, force_scale, o.kick, o.post or 0.0, workers=w)
+            out[label] = round(len(ops) / (time.perf_counter() - t), 2)
+        out["cpu_threads"] = cpu_threads()
+    return out
+
+
 def bench_serving(pkg, dev, count=512, slots=None):
     """Serving workload: `count` forward-simulation requests, mesh uniform over {64, 96, 128, 192}^3
     (box 3.9 N Mpc/h), own seed, Omega_c ~ U(0.2, 0.35), sigma8 ~ U(0.7, 0.9), 1LPT + KDK dt0 = 0.09 to
@@ -585,22 +654,31 @@ def main():
     large = None if args.no_large else bench_large(pkg, dev)
     serving = None if args.no_serving else bench_serving(pkg, dev)
 
-    # CPU baseline: the NumPy/SciPy restatement of the reference step on the host cores
+    # CPU baseline: the NumPy/SciPy/C restatement of the reference step on the host's CPU share, and on
+    # one core; the 64^3 workload beside it (GPU fused, GPU unfused, CPU)
     cpu = None
+    config1 = bench_config1(pkg, dev, cpu=args.cpu_steps > 0)
     if args.cpu_steps > 0:
         from oracle import pm_oracle as orc
 
-        cores = min(len(os.sched_getaffinity(0)), 16)
-        xc, vc = x0.cpu().numpy(), v0.cpu().numpy()
+        cores = cpu_threads()
         o = ops[1][1]
-        t = time.perf_counter()
-        for _ in range(args.cpu_steps):
-            xc, vc = orc.cpu_pm_step(xc, vc, force_scale, o.kick, o.post or 0.0, workers=cores)
-        cpu_s = (time.perf_counter() - t) / args.cpu_steps
+
+        def cpu_time(workers, steps):
+            xc, vc = x0.cpu().numpy(), v0.cpu().numpy()
+            t = time.perf_counter()
+            for _ in range(steps):
+                xc, vc = orc.cpu_pm_step(xc, vc, force_scale, o.kick, o.post or 0.0, workers=workers)
+            return (time.perf_counter() - t) / steps
+
+        cpu_s = cpu_time(cores, args.cpu_steps)
+        one_s = cpu_time(1, 1)
         cpu = {"value": round(1.0 / cpu_s, 5), "unit": "PM steps/s", "cores": cores, "kind": "port",
-               "sample": f"{args.cpu_steps} full 256^3 PM step(s) of the NumPy/SciPy restatement of the reference "
-                         f"(c2c scipy.fft on {cores} threads, single-threaded CIC scatter/gather), "
-                         f"{cpu_s:.1f} s/step"}
+               "sample": f"{args.cpu_steps} full 256^3 PM step(s) of the NumPy/SciPy/C restatement of the reference "
+                         f"(c2c scipy.fft, OpenMP CIC scatter/gather, {cores} threads): {cpu_s:.2f} s/step; "
+                         f"one 256^3 step on 1 thread: {one_s:.1f} s",
+               "one_core": {"value": round(1.0 / one_s, 5), "s_per_step": round(one_s, 2)},
+               "config1_64cubed": config1}
 
     out = {
         "metric": METRIC,
@@ -628,6 +706,7 @@ def main():
         "phases": per_phase,
         "shell_paint_ms": shell_ms,
         "cpu_baseline": cpu,
+        "config1_64cubed": config1,
         "unfused_gpu": {"ms_per_step": round(unf_ms, 4), "fused_ms_per_step": round(fused_ms, 4),
                         "speedup": round(unf_ms / fused_ms, 2), "force_rel_l2_vs_fused": force_err,
                         "form": "atomic scatter paint + rocFFT c2c fwd + 3 x (k-space multiply, rocFFT c2c inv, "

@@ -11,12 +11,16 @@
  *                         src/jax_fli/_src/fields/_painting.py:66-82)
  *   cic_read_{f64,f32}    pm_oracle.cic_read: per particle, out = out + mesh[corner] * w in T, corners in
  *                         x-fastest order (cic_read_dx / cic_read, fields/particles.py:183-229)
+ *   cic_paint_mt,         the CPU baseline's paint and 3-component readout (pm_oracle.cpu_pm_step): one
+ *   cic_read3_mt          pass over the particles on `nthreads` OpenMP threads, float32 weights, deposits
+ *                         added with atomic float adds (summation order, hence rounding, varies between
+ *                         runs: not used where bitwise equality is checked)
  *   healpix_deposit       the per-particle body of pm_oracle.paint_healpix / sky_oracle.paint_healpix_tiled:
  *                         RING angles, ngp pixel or the four bilinear pixels and weights of HEALPix
  *                         get_interpol (Gorski et al. 2005), added to acc in particle order
  *
- * Build: gcc -O2 -ffp-contract=off -fPIC -shared (no fast-math, no FMA contraction: the NumPy side
- * rounds every product and sum).
+ * Build: gcc -O2 -ffp-contract=off -fopenmp -fPIC -shared (no fast-math, no FMA contraction: the NumPy
+ * side rounds every product and sum).
  */
 #include <math.h>
 #include <stdint.h>
@@ -115,6 +119,53 @@ DEF_PAINT(cic_paint_f32, float)
 
 DEF_READ(cic_read_f64, double)
 DEF_READ(cic_read_f32, float)
+
+/* relative-mode paint and readout for the CPU baseline, parallel over particles */
+int cic_paint_mt(const float* pos, int64_t npx, int64_t npy, int64_t npz, int64_t nx, int64_t ny, int64_t nz,
+                 float* mesh, int nthreads) {
+  const int64_t np_ = npx * npy * npz, nc = nx * ny * nz;
+  const int64_t rx = nx / npx, ry = ny / npy, rz = nz / npz;
+  memset(mesh, 0, sizeof(float) * nc);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int64_t p = 0; p < np_; ++p) {
+    const int64_t k = p % npz, j = (p / npz) % npy, i = p / (npz * npy);
+    const int64_t lat[3] = {i * rx, j * ry, k * rz};
+    int64_t b[3];
+    float w[2][3];
+    for (int d = 0; d < 3; ++d) CIC_CORNERS(float, pos[3 * p + d], lat[d], 0, b[d], w[0][d], w[1][d]);
+    for (int o = 0; o < 8; ++o) {
+      const int ox = o & 1, oy = (o >> 1) & 1, oz = (o >> 2) & 1;
+      const float v = (w[ox][0] * w[oy][1]) * w[oz][2];
+      const int64_t c = (pmod(b[0] + ox, nx) * ny + pmod(b[1] + oy, ny)) * nz + pmod(b[2] + oz, nz);
+#pragma omp atomic
+      mesh[c] += v;
+    }
+  }
+  return 0;
+}
+
+int cic_read3_mt(const float* mesh, const float* pos, int64_t npx, int64_t npy, int64_t npz, int64_t nx, int64_t ny,
+                 int64_t nz, float* out, int nthreads) {
+  const int64_t np_ = npx * npy * npz;
+  const int64_t rx = nx / npx, ry = ny / npy, rz = nz / npz;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int64_t p = 0; p < np_; ++p) {
+    const int64_t k = p % npz, j = (p / npz) % npy, i = p / (npz * npy);
+    const int64_t lat[3] = {i * rx, j * ry, k * rz};
+    int64_t b[3];
+    float w[2][3];
+    for (int d = 0; d < 3; ++d) CIC_CORNERS(float, pos[3 * p + d], lat[d], 0, b[d], w[0][d], w[1][d]);
+    float acc[3] = {0.f, 0.f, 0.f};
+    for (int o = 0; o < 8; ++o) {
+      const int ox = o & 1, oy = (o >> 1) & 1, oz = (o >> 2) & 1;
+      const float ww = (w[ox][0] * w[oy][1]) * w[oz][2];
+      const float* m = mesh + ((pmod(b[0] + ox, nx) * ny + pmod(b[1] + oy, ny)) * nz + pmod(b[2] + oz, nz)) * 3;
+      for (int c = 0; c < 3; ++c) acc[c] = acc[c] + m[c] * ww;
+    }
+    for (int c = 0; c < 3; ++c) out[p * 3 + c] = acc[c];
+  }
+  return 0;
+}
 
 /* ------------------------------------------------------------------------------------ HEALPix RING */
 static const double PI = 3.141592653589793;

@@ -59,6 +59,10 @@ def _clib():
                 fn = getattr(lib, name)
                 fn.argtypes = [vp, ctypes.c_int, vp, i64, i64, i64, ctypes.c_int, i64, i64, i64, vp]
                 fn.restype = ctypes.c_int
+            lib.cic_paint_mt.argtypes = [vp, i64, i64, i64, i64, i64, i64, vp, ctypes.c_int]
+            lib.cic_paint_mt.restype = ctypes.c_int
+            lib.cic_read3_mt.argtypes = [vp, vp, i64, i64, i64, i64, i64, i64, vp, ctypes.c_int]
+            lib.cic_read3_mt.restype = ctypes.c_int
             lib.healpix_deposit.argtypes = [i64, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, i64, vp]
             lib.healpix_deposit.restype = ctypes.c_int
             _C = lib
@@ -708,22 +712,32 @@ def pm_forces_vjp(pos, mesh_shape, forces_bar, mode="relative", scale=1.0, decon
 # ---------------------------------------------------------------------------- CPU baseline step
 def cpu_pm_step(x, v, force_scale, c_kick, c_drift, workers=1):
     """One PM step of the reference's op structure on the host, float32, for the CPU baseline:
-    CIC paint (8 scatter passes), complex-to-complex 3D FFT, Green x gradient per component,
-    3 inverse c2c FFTs, 3 separate CIC readouts, kick, drift (pm_forces as called by _forces,
-    solvers.py:19-32, then the kick/drift of solvers.py:296-318). FFTs use scipy.fft with
-    `workers` threads; the scatter/gather passes are single-threaded NumPy."""
+    CIC paint, complex-to-complex 3D FFT, Green x gradient per component, 3 inverse c2c FFTs, a
+    3-component CIC readout, kick, drift (pm_forces as called by _forces, solvers.py:19-32, then the
+    kick/drift of solvers.py:296-318). FFTs use scipy.fft and the paint/readout the C loops
+    (cic_paint_mt, cic_read3_mt), all on `workers` threads; NumPy loops when the C library is absent."""
     import scipy.fft as sfft
 
     mesh_shape = x.shape[:3]
-    rho = cic_paint(x, mesh_shape, "relative", dtype=np.float32)
+    lib = _clib()
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    if lib is not None:
+        rho = np.empty(mesh_shape, dtype=np.float32)
+        lib.cic_paint_mt(_cptr(x), *x.shape[:3], *mesh_shape, _cptr(rho), int(workers))
+    else:
+        rho = cic_paint(x, mesh_shape, "relative", dtype=np.float32)
     dk = sfft.fftn(rho.astype(np.complex64), workers=workers)
     kvec = fftk(mesh_shape, np.float32)
     pot = dk * invlaplace_kernel(kvec).astype(np.float32)
-    f = np.empty(x.shape, dtype=np.float32)
+    fm = np.empty(tuple(mesh_shape) + (3,), dtype=np.float32)
     for i in range(3):
         g = gradient_kernel(kvec, i).astype(np.complex64)
-        fm = sfft.ifftn(-g * pot, workers=workers).real.astype(np.float32)
-        f[..., i] = cic_read(fm, x, "relative", dtype=np.float32)
+        fm[..., i] = sfft.ifftn(-g * pot, workers=workers).real
+    if lib is not None:
+        f = np.empty(x.shape, dtype=np.float32)
+        lib.cic_read3_mt(_cptr(fm), _cptr(x), *x.shape[:3], *mesh_shape, _cptr(f), int(workers))
+    else:
+        f = np.stack([cic_read(fm[..., i], x, "relative", dtype=np.float32) for i in range(3)], axis=-1)
     f *= np.float32(force_scale)
     v = v + f * np.float32(c_kick)
     x = x + v * np.float32(c_drift)

@@ -542,6 +542,30 @@ def test_c_loops_equal_numpy_loops(mode, dtype):
         assert a.dtype == b.dtype and np.array_equal(a, b)
 
 
+def test_cpu_baseline_loops_match_the_oracle():
+    """The multi-threaded paint/readout of the CPU baseline: paint equal to the oracle's to float32 rounding
+    (atomic adds reorder the sums), readout bitwise equal; one CPU step equal to the oracle's forces."""
+    lib = orc._clib()
+    if lib is None:
+        pytest.skip("liboracle_c.so not built")
+    rng = np.random.default_rng(4)
+    n = 16
+    x = (rng.standard_normal((n, n, n, 3)) * 1.3).astype(np.float32)
+    rho = np.empty((n, n, n), np.float32)
+    lib.cic_paint_mt(orc._cptr(x), n, n, n, n, n, n, orc._cptr(rho), 4)
+    ref = orc.cic_paint(x, (n, n, n), "relative", dtype=np.float64)
+    assert np.linalg.norm(rho - ref) / np.linalg.norm(ref) < 1e-6
+    fm = rng.standard_normal((n, n, n, 3)).astype(np.float32)
+    f = np.empty_like(x)
+    lib.cic_read3_mt(orc._cptr(fm), orc._cptr(x), n, n, n, n, n, n, orc._cptr(f), 4)
+    refr = np.stack([orc.cic_read(fm[..., i], x, "relative", dtype=np.float32) for i in range(3)], -1)
+    assert np.array_equal(f, refr)
+    v = np.zeros_like(x)
+    x1, v1 = orc.cpu_pm_step(x, v, 1.0, 1.0, 0.0, workers=4)
+    F = orc.pm_forces(x, (n, n, n), "relative", dtype=np.float64)
+    assert np.linalg.norm(v1 - F) / np.linalg.norm(F) < 1e-5
+
+
 @pytest.mark.parametrize("scheme", ["ngp", "bilinear"])
 def test_c_healpix_deposits_equal_python_loop(scheme):
     """The C shell-paint loop against the Python one (both polar caps and the equator), with rotated images."""
