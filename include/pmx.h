@@ -136,6 +136,15 @@ int pmx_power_multipoles(pmx_plan plan, const float* mesh, const float* mesh2, c
                          const int* poles, int npoles, const double* los, double* p_re, double* p_im,
                          double* k_sum, double* k_cnt, void* stream);
 
+/* HEALPix resolution change with healpy.ud_grade's semantics (as applied after painting at paint_nside by
+ * jaxpm.spherical.paint_particles_spherical, called at src/jax_fli/_src/fields/_painting.py:329-347):
+ * in NESTED order a degraded pixel averages its finite, not-UNSEEN (-1.6375e30) children (pess: UNSEEN
+ * unless all are good, else UNSEEN when none is) and an upgraded pixel copies its parent; a non-zero power
+ * scales by (nside_out / nside_in)^power. order_in / order_out: 0 RING, 1 NESTED. nside: powers of two
+ * up to 8192. Device float32 maps of 12 nside^2 values; in and out must not overlap. */
+int pmx_healpix_ud_grade(const float* in, int nside_in, float* out, int nside_out, float power, int order_in,
+                         int order_out, int pess, void* stream);
+
 /* Counter-based white noise N(0,1) (Philox4x32-10 + Box-Muller) filling (nx,ny,nz) -- stands in for
  * normal_field(key) (initial.py:93). pmx_philox4x32 writes raw 4x32 blocks for known-answer tests. */
 int pmx_white_noise(pmx_plan plan, uint64_t seed, float* out, void* stream);

@@ -27,6 +27,8 @@ int pmx_launch_delta2(const float* H, float* rho, size_t ncell, int offdiag, hip
 // dx (+)= D F, pv (+)= c D F at the lattice
 int pmx_launch_lpt_apply(pmx_plan_t* p, const float* F, float* dx, float* pv, int npx, int npy, int npz,
                          const LptGeom& g, int term, int accumulate, hipStream_t s);
+int pmx_launch_ud_grade(const float* in, int nside_in, float* out, int nside_out, double ratio, int ring_in,
+                        int ring_out, int pess, hipStream_t s);
 int pmx_launch_lpt1_vjp(pmx_plan_t* p, const float* dxb, const float* pvb, int npx, int npy, int npz, float D, float c,
                         float* G, hipStream_t s);
 // launchers from pmx_sky.hip
@@ -1000,6 +1002,18 @@ int pmx_power_multipoles(pmx_plan p, const float* mesh, const float* mesh2, cons
     PMX_TRY(pmx_xpass(p, op, p->specB, nullptr, s));
   }
   return 0;
+}
+
+int pmx_healpix_ud_grade(const float* in, int nside_in, float* out, int nside_out, float power, int order_in,
+                         int order_out, int pess, void* stream) {
+  auto pow2 = [](int n) { return n >= 1 && n <= (1 << 13) && (n & (n - 1)) == 0; };
+  if (!in || !out || in == out) return pmx_set_error(1, "ud_grade needs distinct input and output maps");
+  if (!pow2(nside_in) || !pow2(nside_out)) return pmx_set_error(1, "ud_grade: nside must be a power of two <= 8192");
+  if ((order_in != 0 && order_in != 1) || (order_out != 0 && order_out != 1))
+    return pmx_set_error(1, "ud_grade: order must be 0 (RING) or 1 (NESTED)");
+  const double ratio = power != 0.f ? std::pow((double)nside_out / (double)nside_in, (double)power) : 1.0;
+  return pmx_launch_ud_grade(in, nside_in, out, nside_out, ratio, order_in == 0, order_out == 0, pess != 0,
+                             (hipStream_t)stream);
 }
 
 int pmx_white_noise(pmx_plan p, uint64_t seed, float* out, void* stream) {

@@ -1024,6 +1024,137 @@ int born_blocks(long npix, long rows) {
 
 }  // namespace
 
+// ------------------------------------------------------------------ HEALPix orderings and ud_grade
+namespace {
+// Gorski et al. (2005): face f has corner ring jrll[f] nside and phi index jpll[f]; a NESTED index
+// holds the face above the bit-interleaved (ix, iy) of the pixel inside it.
+__constant__ int kJrll[12] = {2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4};
+__constant__ int kJpll[12] = {1, 3, 5, 7, 0, 2, 4, 6, 1, 3, 5, 7};
+
+__device__ __forceinline__ long long spread_bits(long long v) {
+  long long o = 0;
+  for (int b = 0; b < 31; ++b) o |= ((v >> b) & 1ll) << (2 * b);
+  return o;
+}
+__device__ __forceinline__ long long compact_bits(long long v) {
+  long long o = 0;
+  for (int b = 0; b < 31; ++b) o |= ((v >> (2 * b)) & 1ll) << b;
+  return o;
+}
+__device__ __forceinline__ long long isqrt_ll(long long v) {
+  long long r = (long long)sqrt((double)v);
+  while (r * r > v) --r;
+  while ((r + 1) * (r + 1) <= v) ++r;
+  return r;
+}
+
+__device__ long long hp_nest2ring(long long n, long long pix) {
+  const long long npface = n * n, face = pix / npface, r = pix - face * npface;
+  const long long ix = compact_bits(r), iy = compact_bits(r >> 1);
+  const long long jr = kJrll[face] * n - ix - iy - 1;
+  long long nr, kshift, before;
+  const long long ncap = 2 * n * (n - 1), npix = 12 * n * n;
+  if (jr < n) {
+    nr = jr;
+    kshift = 0;
+    before = 2 * nr * (nr - 1);
+  } else if (jr > 3 * n) {
+    nr = 4 * n - jr;
+    kshift = 0;
+    before = npix - 2 * (nr + 1) * nr;
+  } else {
+    nr = n;
+    kshift = (jr - n) & 1;
+    before = ncap + (jr - n) * 4 * n;
+  }
+  long long jp = (kJpll[face] * nr + ix - iy + 1 + kshift) / 2;
+  if (jp > 4 * n) jp -= 4 * n;
+  if (jp < 1) jp += 4 * n;
+  return before + jp - 1;
+}
+
+__device__ long long hp_ring2nest(long long n, long long pix) {
+  const long long ncap = 2 * n * (n - 1), npix = 12 * n * n, nl2 = 2 * n;
+  long long iring, iphi, kshift, nr, face;
+  if (pix < ncap) {
+    iring = (1 + isqrt_ll(1 + 2 * pix)) >> 1;
+    iphi = (pix + 1) - 2 * iring * (iring - 1);
+    kshift = 0;
+    nr = iring;
+    face = (iphi - 1) / nr;
+  } else if (pix < npix - ncap) {
+    const long long ip = pix - ncap, tmp = ip / (4 * n);
+    iring = tmp + n;
+    iphi = ip - tmp * 4 * n + 1;
+    kshift = (iring + n) & 1;
+    nr = n;
+    const long long ire = iring - n + 1, irm = nl2 + 2 - ire;
+    const long long ifm = (iphi - ire / 2 + n - 1) / n, ifp = (iphi - irm / 2 + n - 1) / n;
+    face = (ifp == ifm) ? (ifp | 4) : ((ifp < ifm) ? ifp : ifm + 8);
+  } else {
+    const long long ip = npix - pix;
+    iring = (1 + isqrt_ll(2 * ip - 1)) >> 1;
+    iphi = 4 * iring + 1 - (ip - 2 * iring * (iring - 1));
+    kshift = 0;
+    nr = iring;
+    iring = 2 * nl2 - iring;
+    face = 8 + (iphi - 1) / nr;
+  }
+  const long long irt = iring - kJrll[face] * n + 1;
+  long long ipt = 2 * iphi - kJpll[face] * nr - kshift - 1;
+  if (ipt >= nl2) ipt -= 8 * n;
+  const long long ix = (ipt - irt) >> 1, iy = (-ipt - irt) >> 1;
+  return face * n * n + spread_bits(ix) + (spread_bits(iy) << 1);
+}
+
+__device__ __forceinline__ bool hp_bad(float v) {
+  const double unseen = -1.6375e30;
+  return !isfinite(v) || fabs((double)v - unseen) <= 1e-8 + 1e-5 * 1.6375e30;
+}
+
+// out pixel q (in order_out) from the map at nside_in (in order_in): the NESTED children average
+// (finite, not UNSEEN ones; pess: UNSEEN unless all are good, else UNSEEN when none is) or the
+// parent's value, scaled by ratio = (nside_out / nside_in)^power
+__global__ void __launch_bounds__(256) k_ud_grade(const float* __restrict__ in, long long nin, float* __restrict__ out,
+                                                  long long nout, int ring_in, int ring_out, int pess, double ratio) {
+  const long long npix = 12 * nout * nout;
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < npix;
+       q += (long long)gridDim.x * blockDim.x) {
+    const long long nq = ring_out ? hp_ring2nest(nout, q) : q;
+    float v;
+    if (nout >= nin) {
+      const long long r2 = (nout / nin) * (nout / nin), par = nq / r2;
+      v = (float)((double)in[ring_in ? hp_nest2ring(nin, par) : par] * ratio);
+    } else {
+      const long long r2 = (nin / nout) * (nin / nout);
+      double sum = 0.0;
+      long long nhit = 0;
+      for (long long c = nq * r2; c < (nq + 1) * r2; ++c) {
+        const float x = in[ring_in ? hp_nest2ring(nin, c) : c];
+        if (!hp_bad(x)) {
+          sum += (double)x;
+          ++nhit;
+        }
+      }
+      const bool badout = pess ? (nhit != r2) : (nhit == 0);
+      const double hits = (double)nhit / ratio;
+      v = badout ? -1.6375e30f : (float)(sum / hits);
+    }
+    out[q] = v;
+  }
+}
+}  // namespace
+
+int pmx_launch_ud_grade(const float* in, int nside_in, float* out, int nside_out, double ratio, int ring_in,
+                        int ring_out, int pess, hipStream_t s) {
+  const long long npix = 12ll * nside_out * nside_out;
+  long long g = (npix + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_ud_grade, dim3((unsigned)g), dim3(256), 0, s, in, (long long)nside_in, out,
+                     (long long)nside_out, ring_in, ring_out, pess, ratio);
+  return pmx_check_launch("ud_grade");
+}
+
 // ------------------------------------------------------------------ launchers used by pmx_api.hip
 int pmx_sky_max_tiles() { return HP_MAX_TILES; }
 int pmx_born_max_shells() { return kBornMaxShells; }

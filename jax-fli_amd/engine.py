@@ -318,6 +318,25 @@ def _tiles(tiles, device):
     return rot.numel(), rot, shift, (rot, shift)
 
 
+_ORDERS = {"RING": 0, "NESTED": 1}
+
+
+def healpix_ud_grade(m: torch.Tensor, nside_out: int, power: float = 0.0, order_in: str = "RING",
+                     order_out: str = "RING", pess: bool = False) -> torch.Tensor:
+    """healpy.ud_grade of a float32 HEALPix map on the device (pmx_healpix_ud_grade)."""
+    m = _f32(m, "map")
+    nside_in = int(round((m.numel() / 12) ** 0.5))
+    if 12 * nside_in * nside_in != m.numel():
+        raise ValueError("not a HEALPix map")
+    oi, oo = _ORDERS.get(str(order_in).upper()), _ORDERS.get(str(order_out).upper())
+    if oi is None or oo is None:
+        raise ValueError("ud_grade orders are 'RING' or 'NESTED'")
+    out = torch.empty(12 * int(nside_out) ** 2, dtype=torch.float32, device=m.device)
+    check(load().pmx_healpix_ud_grade(ptr(m), nside_in, ptr(out), int(nside_out), float(power or 0.0), oi, oo,
+                                      int(bool(pess)), stream_ptr()), "pmx_healpix_ud_grade")
+    return out
+
+
 def paint_healpix(pos: torch.Tensor, mesh_shape, box, observer, rmin, rmax, nside, scheme="bilinear",
                   mode="relative", weights=1.0, out=None, accumulate=False, tiles=None) -> torch.Tensor:
     """HEALPix shell map; `tiles` = (rotation codes, shifts (n, 3) Mpc/h) sums the box's images."""

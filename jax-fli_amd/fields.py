@@ -318,8 +318,9 @@ class ParticleField(AbstractField):
         (n, 3) Mpc/h) sums the shell paint over that table of images of the periodic box."""
         if self.nside is None:
             raise ValueError("Spherical painting requires `nside`.")
-        if kernel_width_arcmin is not None or paint_nside not in (None, self.nside):
-            raise NotImplementedError("smoothing kernels and paint_nside resampling are not supported")
+        if kernel_width_arcmin is not None:
+            raise NotImplementedError("smoothing kernels are not supported")
+        pn = self.nside if paint_nside is None else int(paint_nside)
         mode = _paint_mode(self.unit)
         centers = np.atleast_1d(np.asarray(center, dtype=np.float64))
         widths = np.atleast_1d(np.asarray(density_plane_width, dtype=np.float64))
@@ -330,9 +331,12 @@ class ParticleField(AbstractField):
             m = None
             for k, part in enumerate(sets):
                 m = engine.paint_healpix(part.contiguous(), self.mesh_size, self.box_size, self.observer_position_mpc,
-                                         c - wdt / 2, c + wdt / 2, self.nside, scheme=scheme, mode=mode,
+                                         c - wdt / 2, c + wdt / 2, pn, scheme=scheme, mode=mode,
                                          weights=1.0 if weights is None else weights, tiles=tiles, out=m,
                                          accumulate=k > 0)
+            if paint_nside is not None:  # painted at paint_nside, then resampled to the field's nside
+                m = engine.healpix_ud_grade(m, self.nside, power=ud_grade_power, order_in=ud_grade_order_in,
+                                            order_out=ud_grade_order_out, pess=ud_grade_pess)
             maps.append(m)
         arr = torch.stack(maps) if len(maps) > 1 else maps[0]
         return SphericalDensity.FromDensityMetadata(array=arr, field=self, status=FieldStatus.LIGHTCONE,

@@ -19,7 +19,7 @@ import torch
 
 from .. import autodiff, engine
 from ..fields import flat_slab
-from .interp import image_snapshot_vjp, paint_drifted_vjp
+from .interp import image_snapshot_vjp, paint_drifted_vjp, paint_nside_of, ud_grade_adjoint
 from .solvers import AbstractNBodySolver, NBodyState, ReversibleDoubleKickDrift, StepOp, _mode, force_scale, run_op
 
 f32 = np.float32
@@ -116,10 +116,11 @@ def _snapshot_vjp(solver, displacements, velocities, tg, cosmo, t0, param_names=
         if drifting:
             return paint_drifted_vjp(displacements.replace(array=x), velocities.replace(array=v), save_time[idx], cosmo,
                                      rc, wd, interp.painting, g, tiles=tiles, param_names=param_names)
+        g = ud_grade_adjoint(g.contiguous(), displacements, interp.painting)
         return engine.paint_healpix_vjp(x, g.contiguous(), mesh_size, displacements.box_size,
                                         displacements.observer_position_mpc, rc - wd / 2, rc + wd / 2,
-                                        displacements.nside, scheme=interp.painting.scheme, mode=mode,
-                                        weights=weights, tiles=tiles), None, None
+                                        paint_nside_of(displacements, interp.painting), scheme=interp.painting.scheme,
+                                        mode=mode, weights=weights, tiles=tiles), None, None
 
     return state_of, save_vjp
 

@@ -41,6 +41,31 @@ def drift_tables(cosmo, a_now: float, chi_max: float, device) -> dict:
             "gp_now": gp_now, "bound": float(np.abs(f).max()) * 1.01 + 1e-12}
 
 
+def paint_nside_of(field, painting) -> int:
+    """nside a shell is painted at: the painting's paint_nside, else the field's."""
+    pn = getattr(painting, "paint_nside", None)
+    return int(field.nside if pn is None else pn)
+
+
+def ud_grade_to(m, field, painting):
+    """A map painted at paint_nside brought to the field's nside (healpy.ud_grade with the painting's
+    power, orders and pess); unchanged when no paint_nside is set."""
+    if getattr(painting, "paint_nside", None) is None:
+        return m
+    return engine.healpix_ud_grade(m, field.nside, power=painting.ud_grade_power, order_in=painting.ud_grade_order_in,
+                                   order_out=painting.ud_grade_order_out, pess=painting.ud_grade_pess)
+
+
+def ud_grade_adjoint(map_bar, field, painting):
+    """Transpose of ud_grade_to for maps without bad pixels: ud_grade back to paint_nside with power
+    -power - 2 and the orders swapped (degrading averages the children, upgrading copies the parent)."""
+    if getattr(painting, "paint_nside", None) is None:
+        return map_bar
+    return engine.healpix_ud_grade(map_bar, paint_nside_of(field, painting),
+                                   power=-float(painting.ud_grade_power) - 2.0,
+                                   order_in=painting.ud_grade_order_out, order_out=painting.ud_grade_order_in)
+
+
 def paint_drifted(dx, p, t, cosmo, r_center, width, painting, tiles=None):
     """Spherical shell map of the particles each drifted to the lightcone (DriftInterp.paint and
     OnionTiler with drift_on_lightcone, interp.py:258-319, 375-419, 476-493)."""
@@ -49,9 +74,10 @@ def paint_drifted(dx, p, t, cosmo, r_center, width, painting, tiles=None):
     rmin, rmax = r_center - width / 2, r_center + width / 2
     drift = drift_tables(cosmo, float(t), 2.0 * rmax + 100.0, dx.array.device)
     arr = engine.paint_healpix_drift(dx.array.contiguous(), p.array.contiguous(), dx.mesh_size, dx.box_size,
-                                     dx.observer_position_mpc, rmin, rmax, dx.nside, drift, scheme=painting.scheme,
-                                     mode=_paint_mode(dx.unit),
+                                     dx.observer_position_mpc, rmin, rmax, paint_nside_of(dx, painting), drift,
+                                     scheme=painting.scheme, mode=_paint_mode(dx.unit),
                                      weights=1.0 if painting.weights is None else painting.weights, tiles=tiles)
+    arr = ud_grade_to(arr, dx, painting)
     return SphericalDensity.FromDensityMetadata(array=arr, field=dx, status=FieldStatus.LIGHTCONE,
                                                 comoving_centers=np.atleast_1d(r_center),
                                                 density_width=np.atleast_1d(width), unit=DensityUnit.DENSITY)
@@ -81,9 +107,11 @@ def paint_drifted_vjp(dx, p, t, cosmo, r_center, width, painting, map_bar, tiles
         ptab = engine.to_device(np.stack([autodiff.fd(lambda c: drift_factor_table(c, float(t), chi_max), cosmo, n)
                                           for n in param_names]), dev)
         pdots = torch.zeros(len(param_names), dtype=torch.float64, device=dev)
+    map_bar = ud_grade_adjoint(map_bar.contiguous(), dx, painting)
     gx, gv = engine.paint_healpix_drift_vjp(dx.array.contiguous(), p.array.contiguous(), map_bar.contiguous(),
-                                            dx.mesh_size, dx.box_size, dx.observer_position_mpc, rmin, rmax, dx.nside,
-                                            drift, scheme=painting.scheme, mode=_paint_mode(dx.unit),
+                                            dx.mesh_size, dx.box_size, dx.observer_position_mpc, rmin, rmax,
+                                            paint_nside_of(dx, painting), drift, scheme=painting.scheme,
+                                            mode=_paint_mode(dx.unit),
                                             weights=1.0 if painting.weights is None else painting.weights,
                                             tiles=tiles, param_tables=ptab, param_dots=pdots)
     return gx, gv, pdots
@@ -219,7 +247,10 @@ class NoInterp(AbstractInterp):
             result = dx.paint_spherical(center=r_center, density_plane_width=width, scheme=pt.scheme,
                                         weights=pt.weights, kernel_width_arcmin=pt.kernel_width_arcmin,
                                         smoothing_interpretation=pt.smoothing_interpretation,
-                                        paint_nside=pt.paint_nside, batch_size=pt.batch_size)
+                                        paint_nside=pt.paint_nside, ud_grade_power=pt.ud_grade_power,
+                                        ud_grade_order_in=pt.ud_grade_order_in,
+                                        ud_grade_order_out=pt.ud_grade_order_out,
+                                        ud_grade_pess=pt.ud_grade_pess, batch_size=pt.batch_size)
         elif tgt == "flat":
             result = dx.paint_2d(center=r_center, density_plane_width=width, weights=self.painting.weights,
                                  batch_size=self.painting.batch_size)
@@ -339,7 +370,9 @@ class OnionTiler(AbstractInterp):
         result = dx.paint_spherical(center=r_center, density_plane_width=width, scheme=pt.scheme, weights=pt.weights,
                                     kernel_width_arcmin=pt.kernel_width_arcmin,
                                     smoothing_interpretation=pt.smoothing_interpretation,
-                                    paint_nside=pt.paint_nside, batch_size=pt.batch_size,
+                                    paint_nside=pt.paint_nside, ud_grade_power=pt.ud_grade_power,
+                                    ud_grade_order_in=pt.ud_grade_order_in, ud_grade_order_out=pt.ud_grade_order_out,
+                                    ud_grade_pess=pt.ud_grade_pess, batch_size=pt.batch_size,
                                     tiles=self.images(state, dx))
         return result.replace(scale_factors=float(t), comoving_centers=r_center, density_width=width)
 
@@ -420,7 +453,10 @@ class TelephotoInterp(AbstractInterp):
                 out = dx.paint_spherical(center=r_center, density_plane_width=width, scheme=pt.scheme,
                                          weights=pt.weights, kernel_width_arcmin=pt.kernel_width_arcmin,
                                          smoothing_interpretation=pt.smoothing_interpretation,
-                                         paint_nside=pt.paint_nside, batch_size=pt.batch_size, tiles=tiles)
+                                         paint_nside=pt.paint_nside, ud_grade_power=pt.ud_grade_power,
+                                         ud_grade_order_in=pt.ud_grade_order_in,
+                                        ud_grade_order_out=pt.ud_grade_order_out,
+                                         ud_grade_pess=pt.ud_grade_pess, batch_size=pt.batch_size, tiles=tiles)
         else:
             src = dx
             if tiles is not None or self.drift_on_lightcone:

@@ -276,6 +276,118 @@ def paint_healpix_vjp(pos, map_bar, mesh_shape, box, observer, rmin, rmax, nside
 
 
 # ---------------------------------------------------------------------------- Born
+# ---------------------------------------------------------------------------- HEALPix orderings
+# Gorski et al. (2005): a NESTED index holds the base face (0..11) in its top bits and the
+# bit-interleaved (ix, iy) position inside the face below; the RING index counts pixels ring by ring
+# from the north pole. Face f has its corner ring jrll[f] * nside and its phi index jpll[f].
+_JRLL = np.array([2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4], dtype=np.int64)
+_JPLL = np.array([1, 3, 5, 7, 0, 2, 4, 6, 1, 3, 5, 7], dtype=np.int64)
+
+
+def _spread_bits(v):
+    v = np.asarray(v, dtype=np.int64)
+    out = np.zeros_like(v)
+    for b in range(30):
+        out |= ((v >> b) & 1) << (2 * b)
+    return out
+
+
+def _compact_bits(v):
+    v = np.asarray(v, dtype=np.int64)
+    out = np.zeros_like(v)
+    for b in range(30):
+        out |= ((v >> (2 * b)) & 1) << b
+    return out
+
+
+def nest2xyf(nside, pix):
+    pix = np.asarray(pix, dtype=np.int64)
+    npface = nside * nside
+    face = pix // npface
+    r = pix % npface
+    return _compact_bits(r), _compact_bits(r >> 1), face
+
+
+def xyf2nest(nside, ix, iy, face):
+    return np.asarray(face, np.int64) * nside * nside + _spread_bits(ix) + (_spread_bits(iy) << 1)
+
+
+def _ring_of_xyf(nside, ix, iy, face):
+    """(ring number jr in 1..4 nside - 1, pixels in that ring's quarter nr, phi index jp in 1..4 nr,
+    kshift) of the pixel at (ix, iy) on face."""
+    jr = _JRLL[face] * nside - ix - iy - 1
+    nr = np.where(jr < nside, jr, np.where(jr > 3 * nside, 4 * nside - jr, nside))
+    kshift = np.where((jr < nside) | (jr > 3 * nside), 0, (jr - nside) & 1)
+    jp = (_JPLL[face] * nr + ix - iy + 1 + kshift) // 2
+    jp = np.where(jp > 4 * nside, jp - 4 * nside, jp)
+    jp = np.where(jp < 1, jp + 4 * nside, jp)
+    return jr, nr, jp, kshift
+
+
+def nest2ring(nside, pix):
+    ix, iy, face = nest2xyf(nside, pix)
+    jr, nr, jp, _ = _ring_of_xyf(nside, ix, iy, face)
+    ncap, npix = 2 * nside * (nside - 1), 12 * nside * nside
+    before = np.where(jr < nside, 2 * nr * (nr - 1),
+                      np.where(jr > 3 * nside, npix - 2 * (nr + 1) * nr, ncap + (jr - nside) * 4 * nside))
+    return before + jp - 1
+
+
+def ring2nest(nside, pix):
+    """Inverse of nest2ring, by table: every ring index is reached once (a bijection, checked in tests)."""
+    npix = 12 * nside * nside
+    inv = np.empty(npix, dtype=np.int64)
+    inv[nest2ring(nside, np.arange(npix))] = np.arange(npix)
+    return inv[np.asarray(pix, dtype=np.int64)]
+
+
+def healpix_reorder(m, nside, r2n):
+    """A map in RING order to NESTED (r2n) or back."""
+    m = np.asarray(m)
+    idx = nest2ring(nside, np.arange(12 * nside * nside))
+    return m[idx] if r2n else _scatter(m, idx)
+
+
+def _scatter(m, idx):
+    out = np.empty_like(m)
+    out[idx] = m
+    return out
+
+
+UNSEEN = -1.6375e30
+
+
+def ud_grade(map_in, nside_out, pess=False, order_in="RING", order_out="RING", power=0.0):
+    """healpy.ud_grade's published algorithm: in NESTED order, a degraded pixel averages its finite,
+    not-UNSEEN children (pess: UNSEEN unless all are good; else UNSEEN when none is), an upgraded
+    pixel copies its parent; a non-zero power scales by (nside_out / nside_in)^power."""
+    m = np.asarray(map_in, dtype=np.float64)
+    nside_in = int(round(np.sqrt(m.size / 12)))
+    if order_in == "RING":
+        m = healpix_reorder(m, nside_in, True)
+    ratio = (nside_out / nside_in) ** power if power else 1.0
+    if nside_out > nside_in:
+        r2 = (nside_out // nside_in) ** 2
+        out = np.repeat(m * ratio, r2)
+    elif nside_out < nside_in:
+        r2 = (nside_in // nside_out) ** 2
+        mr = m.reshape(-1, r2)
+        bad = (np.abs(mr - UNSEEN) <= 1e-8 + 1e-5 * abs(UNSEEN)) | ~np.isfinite(mr)
+        good = ~bad
+        out = np.where(good, mr, 0.0).sum(axis=1)
+        nhit = good.sum(axis=1).astype(np.float64)
+        badout = (nhit != r2) if pess else (nhit == 0)
+        if power:
+            nhit = nhit / ratio
+        out = np.where(nhit != 0, out / np.where(nhit != 0, nhit, 1.0), out)
+        out[badout] = UNSEEN
+    else:
+        out = m
+    if order_out == "RING":
+        out = healpix_reorder(out, nside_out, False)
+    return out
+
+
 def simps_weights(a, b, N):
     """_simps_weights (src/jax_fli/_src/lensing/_born.py:16-26): dx/3 [1, 4, 2, ..., 4, 1]."""
     dx = (b - a) / N

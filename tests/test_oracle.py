@@ -236,6 +236,51 @@ def test_healpix_interp_weights_identities():
         assert wd.get(pix, 0) == pytest.approx(1.0, abs=1e-9)
 
 
+def test_healpix_nest_ring_orderings():
+    """nest2ring is a bijection at every nside; NESTED is hierarchical (the four children 4p..4p+3 at
+    2 nside have their centres inside parent p, located by the RING ang2pix pinned above); at nside 1
+    both orderings coincide."""
+    from oracle import sky_oracle as so
+
+    assert np.array_equal(so.nest2ring(1, np.arange(12)), np.arange(12))
+    for n in (1, 2, 4, 8, 16):
+        r = so.nest2ring(n, np.arange(12 * n * n))
+        assert np.array_equal(np.sort(r), np.arange(12 * n * n))
+        assert np.array_equal(so.ring2nest(n, r), np.arange(12 * n * n))
+    for n in (1, 2, 4):
+        for p in range(12 * n * n):
+            parent = so.nest2ring(n, p)
+            for c in range(4 * p, 4 * p + 4):
+                th, ph = orc.healpix_pix2ang(2 * n, int(so.nest2ring(2 * n, c)))
+                assert orc.healpix_ang2pix(n, np.cos(th), ph) == parent
+
+
+def test_ud_grade_properties():
+    from oracle import sky_oracle as so
+
+    rng = np.random.default_rng(6)
+    m = rng.standard_normal(12 * 8 * 8) + 3.0
+    # degrade averages: the mean is kept; power -2 keeps the sum; upgrade then degrade is the identity
+    d = so.ud_grade(m, 4)
+    assert np.mean(d) == pytest.approx(np.mean(m), rel=1e-12)
+    assert np.sum(so.ud_grade(m, 2, power=-2)) == pytest.approx(np.sum(m), rel=1e-12)
+    assert np.allclose(so.ud_grade(so.ud_grade(m, 16), 8), m, rtol=1e-13, atol=0)
+    assert np.allclose(so.ud_grade(m, 16, power=2), np.repeat(so.healpix_reorder(m, 8, True) * 4.0, 4)[
+        so.ring2nest(16, np.arange(12 * 16 * 16))], rtol=1e-13)
+    # NESTED output is the reordered RING output; a NESTED input is read as such
+    assert np.array_equal(so.ud_grade(m, 4, order_out="NESTED"), so.healpix_reorder(d, 4, True))
+    mn = so.healpix_reorder(m, 8, True)
+    assert np.allclose(so.ud_grade(mn, 4, order_in="NESTED"), d, rtol=1e-13)
+    # bad children: skipped (average of the good ones), or the whole pixel with pess
+    mb = so.healpix_reorder(m, 8, True).copy()
+    mb[0] = so.UNSEEN
+    mb[5] = np.nan
+    got = so.ud_grade(mb, 4, order_in="NESTED", order_out="NESTED")
+    assert got[0] == pytest.approx(np.mean(mb[1:4]), rel=1e-12) and got[1] == pytest.approx(np.mean(mb[[4, 6, 7]]))
+    gp = so.ud_grade(mb, 4, pess=True, order_in="NESTED", order_out="NESTED")
+    assert gp[0] == so.UNSEEN and gp[1] == so.UNSEEN and gp[2] == pytest.approx(np.mean(mb[8:12]))
+
+
 # ---------------------------------------------------------------- cosmology
 def test_eds_closed_forms():
     c = nbo.Cosmo(Omega_c=0.95, Omega_b=0.05)
