@@ -136,6 +136,14 @@ int pmx_power_multipoles(pmx_plan plan, const float* mesh, const float* mesh2, c
                          const int* poles, int npoles, const double* los, double* p_re, double* p_im,
                          double* k_sum, double* k_cnt, void* stream);
 
+/* Gaussian width (radians) of the rbf_neighbor shell paint (scheme 2 of the HEALPix paints): each particle
+ * is spread over its pixel and the pixels of the eight directions one pixel resolution away, weighted by
+ * exp(-d^2 / (2 sigma^2)) of the angular distance d to each pixel centre and normalised to sum 1 (the
+ * "rbf_neighbor" option of PaintingOptions, fields/painting.py:82; the upstream kernel in
+ * jaxpm.spherical is not in the reference, so this definition is this library's own). 0 (the default):
+ * the pixel resolution sqrt(4 pi / npix) read as a FWHM. The scheme has no adjoint. */
+int pmx_plan_set_rbf_width(pmx_plan plan, double sigma_rad);
+
 /* HEALPix resolution change with healpy.ud_grade's semantics (as applied after painting at paint_nside by
  * jaxpm.spherical.paint_particles_spherical, called at src/jax_fli/_src/fields/_painting.py:329-347):
  * in NESTED order a degraded pixel averages its finite, not-UNSEEN (-1.6375e30) children (pess: UNSEEN
@@ -202,8 +210,8 @@ int pmx_pgd_vjp(pmx_plan plan, const float* pos, int npx, int npy, int npz, int 
                 const float* disp_bar, float* pos_bar, int accumulate, double* dot, void* stream);
 
 /* HEALPix shell paint: particles with rmin < |x - observer| <= rmax (Mpc/h, periodic box) painted
- * to RING pixels, scheme 0 = ngp, 1 = bilinear (HEALPix ring interpolation weights), map =
- * weighted count / (pixel solid angle * (rmax^3 - rmin^3) / 3). paint_particles_spherical as called
+ * to RING pixels, scheme 0 = ngp, 1 = bilinear (HEALPix ring interpolation weights), 2 = rbf_neighbor
+ * (pmx_plan_set_rbf_width), map = weighted count / (pixel solid angle * (rmax^3 - rmin^3) / 3). paint_particles_spherical as called
  * by _single_paint_spherical (src/jax_fli/_src/fields/_painting.py:249-348). */
 int pmx_paint_healpix(pmx_plan plan, const float* pos, int npx, int npy, int npz, int mode, const float* weights,
                       float weight, const double box[3], const double observer[3], double rmin, double rmax,

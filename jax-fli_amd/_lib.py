@@ -47,6 +47,7 @@ SIGNATURES = {
     "pmx_step": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _f, _u, _vp]),
     "pmx_step_keep": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _f, _f, _u, _vp, _vp]),
     "pmx_power": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "pmx_plan_set_rbf_width": (_i, [_vp, _d]),
     "pmx_healpix_ud_grade": (_i, [_vp, _i, _vp, _i, _f, _i, _i, _i, _vp]),
     "pmx_lpt": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _i, _d, _vp, _vp, _vp, _vp, _vp]),
     "pmx_power_multipoles": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),

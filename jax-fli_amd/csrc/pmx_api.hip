@@ -1004,6 +1004,13 @@ int pmx_power_multipoles(pmx_plan p, const float* mesh, const float* mesh2, cons
   return 0;
 }
 
+int pmx_plan_set_rbf_width(pmx_plan p, double sigma_rad) {
+  PMX_TRY(check_plan(p));
+  if (!(sigma_rad >= 0.0) || !std::isfinite(sigma_rad)) return pmx_set_error(1, "rbf width must be finite and >= 0");
+  p->rbf_sigma = sigma_rad;
+  return 0;
+}
+
 int pmx_healpix_ud_grade(const float* in, int nside_in, float* out, int nside_out, float power, int order_in,
                          int order_out, int pess, void* stream) {
   auto pow2 = [](int n) { return n >= 1 && n <= (1 << 13) && (n & (n - 1)) == 0; };
@@ -1121,7 +1128,7 @@ int check_shell(pmx_plan_t* p, const float* pos, int npx, int npy, int npz, int 
   PMX_TRY(check_particles(p, pos, npx, npy, npz, mode));
   if (!box || !observer) return pmx_set_error(1, "null argument");
   if (nside < 1 || nside > 8192) return pmx_set_error(1, "nside out of range");
-  if (scheme != 0 && scheme != 1) return pmx_set_error(2, "scheme must be 0 (ngp) or 1 (bilinear)");
+  if (scheme < 0 || scheme > 2) return pmx_set_error(2, "scheme must be 0 (ngp), 1 (bilinear) or 2 (rbf_neighbor)");
   if (!(rmax > rmin) || rmin < 0) return pmx_set_error(1, "need 0 <= rmin < rmax");
   if (rot && (ntiles < 1 || ntiles > pmx_sky_max_tiles() || !shift))
     return pmx_set_error(1, "image table: need 1 <= ntiles <= " + std::to_string(pmx_sky_max_tiles()) +
@@ -1233,6 +1240,7 @@ int pmx_paint_healpix_vjp(pmx_plan p, const float* pos, int npx, int npy, int np
                           float weight, const double box[3], const double observer[3], double rmin, double rmax,
                           int nside, int scheme, int ntiles, const int* tile_rot, const double* tile_shift,
                           const float* map_bar, float* pos_bar, int accumulate, void* stream) {
+  if (scheme == 2) return pmx_set_error(2, "the rbf_neighbor scheme has no adjoint");
   PMX_TRY(check_shell(p, pos, npx, npy, npz, mode, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
                       tile_shift));
   if (!map_bar || !pos_bar) return pmx_set_error(1, "null cotangent");
@@ -1250,6 +1258,7 @@ int pmx_paint_healpix_lightcone_vjp(pmx_plan p, const float* pos, const float* v
                                     double a_now, double gp_now, double drift_bound, const float* map_bar,
                                     float* pos_bar, float* vel_bar, int accumulate, int nparam,
                                     const double* param_tables, double* param_dots, void* stream) {
+  if (scheme == 2) return pmx_set_error(2, "the rbf_neighbor scheme has no adjoint");
   PMX_TRY(check_shell(p, pos, npx, npy, npz, mode, box, observer, rmin, rmax, nside, scheme, ntiles, tile_rot,
                       tile_shift));
   if (!map_bar || !pos_bar || !vel_bar) return pmx_set_error(1, "null cotangent");

@@ -65,6 +65,7 @@ struct pmx_plan_t {
   float* fmesh;     // force mesh [x][y][z][3]
   double *kphys_x, *kphys_y, *kphys_z;  // estimator wavevectors (h/Mpc) per bin, set by pmx_plan_set_box
   bool box_set;
+  double rbf_sigma = 0.0;  // rbf_neighbor shell paints: Gaussian width in radians (0: the pixel resolution as FWHM)
   // sorted paint (pmx_particles.hip): per paint tile particle counts (zero between paints), the
   // last sort's counts, segment offsets, reservation cursors; per particle sub-block window counts; segments of particle
   // indices; misc = {max |weight| bits, far entries, far entries of the last paint}; fixed-point

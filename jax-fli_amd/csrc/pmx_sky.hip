@@ -54,6 +54,92 @@ __device__ __forceinline__ long hp_ang2pix(const HpGeom& g, double z, double phi
   return (z > 0) ? 2 * ir * (ir - 1) + ip : g.npix - 2 * ir * (ir + 1) + ip;
 }
 
+// unit vector of the centre of RING pixel `pix`
+__device__ void hp_pix2vec(const HpGeom& g, long pix, double v[3]) {
+  const long ns = g.nside;
+  double z, st, phi;
+  if (pix < g.ncap) {
+    long iring = (long)((1.0 + sqrt(1.0 + 2.0 * (double)pix)) * 0.5);
+    while (2 * iring * (iring - 1) > pix) --iring;
+    while (2 * (iring + 1) * iring <= pix) ++iring;
+    const long iphi = pix - 2 * iring * (iring - 1);
+    const double t = (double)iring * iring * 4.0 / (double)g.npix;
+    z = 1.0 - t;
+    st = sqrt(t * (2.0 - t));
+    phi = ((double)iphi + 0.5) * M_PI / (2.0 * iring);
+  } else if (pix < g.npix - g.ncap) {
+    const long ip = pix - g.ncap, iring = ip / (4 * ns) + ns, iphi = ip % (4 * ns) + 1;
+    const double fodd = ((iring + ns) & 1) ? 1.0 : 0.5;
+    z = (double)(2 * ns - iring) * 2.0 / (3.0 * ns);
+    st = sqrt((1.0 - z) * (1.0 + z));
+    phi = ((double)iphi - fodd) * M_PI / (2.0 * ns);
+  } else {
+    const long ip = g.npix - pix;
+    long iring = (long)((1.0 + sqrt(2.0 * (double)ip - 1.0)) * 0.5);
+    while (2 * iring * (iring - 1) >= ip) --iring;
+    while (2 * (iring + 1) * iring < ip) ++iring;
+    const long iphi = 4 * iring + 1 - (ip - 2 * iring * (iring - 1));
+    const double t = (double)iring * iring * 4.0 / (double)g.npix;
+    z = -1.0 + t;
+    st = sqrt(t * (2.0 - t));
+    phi = ((double)iphi - 0.5) * M_PI / (2.0 * iring);
+  }
+  v[0] = st * cos(phi);
+  v[1] = st * sin(phi);
+  v[2] = z;
+}
+
+// rbf_neighbor candidates and weights of a direction n (unit): its pixel and the pixels of the eight
+// directions one pixel resolution away (azimuths k pi/4 in the tangent plane), each weighted by
+// exp(-d^2 / (2 sigma^2)), d the angular distance from n to the pixel centre, normalised to sum 1.
+// Returns the number of distinct pixels (<= 9).
+__device__ int hp_rbf(const HpGeom& g, const double n[3], double sigma, long pix[9], double wgt[9]) {
+  const double st = sqrt(n[0] * n[0] + n[1] * n[1]);
+  double ct = n[2], cp = 1.0, sp = 0.0;
+  if (st > 0.0) {
+    cp = n[0] / st;
+    sp = n[1] / st;
+  }
+  const double eth[3] = {ct * cp, ct * sp, -st}, eph[3] = {-sp, cp, 0.0};
+  const double step = sqrt(4.0 * M_PI / (double)g.npix), cs = cos(step), ss = sin(step);
+  int m = 0;
+  for (int k = -1; k < 8; ++k) {
+    double u[3];
+    if (k < 0) {
+      u[0] = n[0];
+      u[1] = n[1];
+      u[2] = n[2];
+    } else {
+      const double ca = cos(0.25 * M_PI * k), sa = sin(0.25 * M_PI * k);
+      for (int c = 0; c < 3; ++c) u[c] = cs * n[c] + ss * (ca * eth[c] + sa * eph[c]);
+    }
+    const double z = fmin(1.0, fmax(-1.0, u[2]));
+    double ph = atan2(u[1], u[0]);
+    if (ph < 0.0) ph += 2.0 * M_PI;
+    if (ph >= 2.0 * M_PI) ph = 0.0;
+    const long q = hp_ang2pix(g, z, ph);
+    bool seen = false;
+    for (int j = 0; j < m; ++j) seen |= (pix[j] == q);
+    if (!seen) pix[m++] = q;
+  }
+  // exp(-(d^2 - dmin^2) / (2 sigma^2)): the nearest candidate weighs 1, so no width underflows to 0 / 0
+  double q2[9], qmin = 1e300, sum = 0.0;
+  for (int j = 0; j < m; ++j) {
+    double c[3];
+    hp_pix2vec(g, pix[j], c);
+    const double dx = n[0] - c[0], dy = n[1] - c[1], dz = n[2] - c[2];
+    const double d = 2.0 * asin(fmin(1.0, 0.5 * sqrt(dx * dx + dy * dy + dz * dz)));
+    q2[j] = (d / sigma) * (d / sigma);
+    qmin = fmin(qmin, q2[j]);
+  }
+  for (int j = 0; j < m; ++j) {
+    wgt[j] = exp(-0.5 * (q2[j] - qmin));
+    sum += wgt[j];
+  }
+  for (int j = 0; j < m; ++j) wgt[j] /= sum;
+  return m;
+}
+
 __device__ __forceinline__ long hp_ring_above(const HpGeom& g, double z) {
   const double az = fabs(z);
   if (az <= 2.0 / 3.0) return (long)(g.nside * (2.0 - 1.5 * z));
@@ -227,7 +313,8 @@ struct ShellArgs {
   double cell[3];   // Mpc/h per grid unit
   double obs[3];    // observer, Mpc/h
   double rmin, rmax;
-  int scheme;       // 0 ngp, 1 bilinear
+  int scheme;       // 0 ngp, 1 bilinear, 2 rbf_neighbor
+  double rbf_sigma;  // rbf_neighbor: Gaussian width (radians)
   int NX, NY, NZ;
   int rx, ry, rz;   // lattice spacing of the particle grid in cells
   int nbx, nby, nbz;
@@ -504,6 +591,12 @@ __global__ void __launch_bounds__(256) k_paint_healpix(PartArgs a, const float* 
       const double phi = phi_of(Xt);
       if (sa.scheme == 0) {
         hp_table_add(keys, vals, (unsigned)hp_ang2pix(g, Xt[2] / rr, phi), wt, acc, &full);
+      } else if (sa.scheme == 2) {
+        const double n[3] = {Xt[0] / rr, Xt[1] / rr, Xt[2] / rr};
+        long pix[9];
+        double wg[9];
+        const int m = hp_rbf(g, n, sa.rbf_sigma, pix, wg);
+        for (int q = 0; q < m; ++q) hp_table_add(keys, vals, (unsigned)pix[q], wt * wg[q], acc, &full);
       } else {
         const double theta = atan2(sqrt(Xt[0] * Xt[0] + Xt[1] * Xt[1]), Xt[2]);
         long pix[4];
@@ -995,6 +1088,7 @@ ShellArgs shell_args(const pmx_plan_t* p, const PartArgs& a, const double box[3]
   sa.rmin = rmin;
   sa.rmax = rmax;
   sa.scheme = scheme;
+  sa.rbf_sigma = p->rbf_sigma;
   sa.rx = p->nx / a.npx;
   sa.ry = p->ny / a.npy;
   sa.rz = p->nz / a.npz;
@@ -1319,6 +1413,8 @@ int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w
   int rc = 0;
   const HpGeom g = hp_geom_table(p, nside, s, rc);
   if (rc) return rc;
+  if (!(sa.rbf_sigma > 0.0))  // default width: the pixel resolution, read as a FWHM
+    sa.rbf_sigma = sqrt(4.0 * M_PI / (double)g.npix) / (2.0 * sqrt(2.0 * log(2.0)));
   double* acc = p->acc;
   hipLaunchKernelGGL(k_zero_d, dim3(grid_for((size_t)g.npix, 256)), dim3(256), 0, s, acc, g.npix);
   PMX_CHECK_LAUNCH("healpix_zero");

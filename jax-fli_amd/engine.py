@@ -7,6 +7,7 @@ All work is enqueued on torch's current stream; nothing synchronises.
 from __future__ import annotations
 
 import ctypes
+import math
 
 import numpy as np
 import torch
@@ -296,13 +297,13 @@ def lpt1_vjp(dx_bar, p_bar, mesh_shape, growth: float, vel_factor: float) -> tor
     return out
 
 
-_SCHEMES = {"ngp": 0, "bilinear": 1}
+_SCHEMES = {"ngp": 0, "bilinear": 1, "rbf_neighbor": 2}
 
 
 def _scheme(scheme):
     sch = _SCHEMES.get(scheme)
     if sch is None:
-        raise NotImplementedError(f"spherical painting scheme {scheme!r} is not supported (ngp, bilinear)")
+        raise NotImplementedError(f"spherical painting scheme {scheme!r} is not supported (ngp, bilinear, rbf_neighbor)")
     return sch
 
 
@@ -337,12 +338,27 @@ def healpix_ud_grade(m: torch.Tensor, nside_out: int, power: float = 0.0, order_
     return out
 
 
+def rbf_sigma(kernel_width_arcmin=None, smoothing_interpretation="fwhm") -> float:
+    """Gaussian width (radians) of the rbf_neighbor paint: kernel_width_arcmin read as a FWHM or a sigma;
+    0 (no width given) selects the pixel resolution as FWHM."""
+    if kernel_width_arcmin is None:
+        return 0.0
+    w = float(kernel_width_arcmin) * math.pi / (180.0 * 60.0)
+    if smoothing_interpretation == "fwhm":
+        return w / (2.0 * math.sqrt(2.0 * math.log(2.0)))
+    if smoothing_interpretation == "sigma":
+        return w
+    raise ValueError(f"smoothing_interpretation must be 'fwhm' or 'sigma', got {smoothing_interpretation!r}")
+
+
 def paint_healpix(pos: torch.Tensor, mesh_shape, box, observer, rmin, rmax, nside, scheme="bilinear",
-                  mode="relative", weights=1.0, out=None, accumulate=False, tiles=None) -> torch.Tensor:
-    """HEALPix shell map; `tiles` = (rotation codes, shifts (n, 3) Mpc/h) sums the box's images."""
+                  mode="relative", weights=1.0, out=None, accumulate=False, tiles=None, sigma=0.0) -> torch.Tensor:
+    """HEALPix shell map; `tiles` = (rotation codes, shifts (n, 3) Mpc/h) sums the box's images; `sigma`:
+    the rbf_neighbor width in radians (rbf_sigma)."""
     pos = _f32(pos, "pos")
     npx, npy, npz = _pgrid(pos)
     plan = get_plan(mesh_shape, pos.device)
+    check(load().pmx_plan_set_rbf_width(plan.handle, float(sigma)), "pmx_plan_set_rbf_width")
     if out is None:
         out = torch.empty(12 * nside * nside, dtype=torch.float32, device=pos.device)
     w, w0 = _weights(weights, npx * npy * npz)
@@ -356,7 +372,8 @@ def paint_healpix(pos: torch.Tensor, mesh_shape, box, observer, rmin, rmax, nsid
 
 
 def paint_healpix_drift(pos: torch.Tensor, vel: torch.Tensor, mesh_shape, box, observer, rmin, rmax, nside, drift,
-                        scheme="bilinear", mode="relative", weights=1.0, out=None, accumulate=False, tiles=None):
+                        scheme="bilinear", mode="relative", weights=1.0, out=None, accumulate=False, tiles=None,
+                        sigma=0.0):
     """paint_healpix with every (image) position drifted to its own lightcone time first. drift: dict with
     "tables" (device float64 (3, ntab): a(chi), Gp(a), dGfa(a)), "chi_max", "a_lo", "a_hi", "a_now",
     "gp_now", "bound"."""
@@ -366,6 +383,7 @@ def paint_healpix_drift(pos: torch.Tensor, vel: torch.Tensor, mesh_shape, box, o
     if tuple(vel.shape) != tuple(pos.shape):
         raise ValueError("vel must have the shape of pos")
     plan = get_plan(mesh_shape, pos.device)
+    check(load().pmx_plan_set_rbf_width(plan.handle, float(sigma)), "pmx_plan_set_rbf_width")
     if out is None:
         out = torch.empty(12 * nside * nside, dtype=torch.float32, device=pos.device)
     w, w0 = _weights(weights, npx * npy * npz)

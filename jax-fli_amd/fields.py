@@ -318,8 +318,8 @@ class ParticleField(AbstractField):
         (n, 3) Mpc/h) sums the shell paint over that table of images of the periodic box."""
         if self.nside is None:
             raise ValueError("Spherical painting requires `nside`.")
-        if kernel_width_arcmin is not None:
-            raise NotImplementedError("smoothing kernels are not supported")
+        # kernel_width_arcmin / smoothing_interpretation set the rbf_neighbor width (unused by ngp, bilinear)
+        sigma = engine.rbf_sigma(kernel_width_arcmin, smoothing_interpretation) if scheme == "rbf_neighbor" else 0.0
         pn = self.nside if paint_nside is None else int(paint_nside)
         mode = _paint_mode(self.unit)
         centers = np.atleast_1d(np.asarray(center, dtype=np.float64))
@@ -333,7 +333,7 @@ class ParticleField(AbstractField):
                 m = engine.paint_healpix(part.contiguous(), self.mesh_size, self.box_size, self.observer_position_mpc,
                                          c - wdt / 2, c + wdt / 2, pn, scheme=scheme, mode=mode,
                                          weights=1.0 if weights is None else weights, tiles=tiles, out=m,
-                                         accumulate=k > 0)
+                                         accumulate=k > 0, sigma=sigma)
             if paint_nside is not None:  # painted at paint_nside, then resampled to the field's nside
                 m = engine.healpix_ud_grade(m, self.nside, power=ud_grade_power, order_in=ud_grade_order_in,
                                             order_out=ud_grade_order_out, pess=ud_grade_pess)

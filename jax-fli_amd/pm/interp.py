@@ -76,7 +76,10 @@ def paint_drifted(dx, p, t, cosmo, r_center, width, painting, tiles=None):
     arr = engine.paint_healpix_drift(dx.array.contiguous(), p.array.contiguous(), dx.mesh_size, dx.box_size,
                                      dx.observer_position_mpc, rmin, rmax, paint_nside_of(dx, painting), drift,
                                      scheme=painting.scheme, mode=_paint_mode(dx.unit),
-                                     weights=1.0 if painting.weights is None else painting.weights, tiles=tiles)
+                                     weights=1.0 if painting.weights is None else painting.weights, tiles=tiles,
+                                     sigma=engine.rbf_sigma(painting.kernel_width_arcmin,
+                                                            painting.smoothing_interpretation)
+                                     if painting.scheme == "rbf_neighbor" else 0.0)
     arr = ud_grade_to(arr, dx, painting)
     return SphericalDensity.FromDensityMetadata(array=arr, field=dx, status=FieldStatus.LIGHTCONE,
                                                 comoving_centers=np.atleast_1d(r_center),

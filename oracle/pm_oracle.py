@@ -577,7 +577,7 @@ def healpix_deposit(nside, scheme, wrap_guard, X, r, wt, sel, acc) -> bool:
     """The per-particle loop of the shell paints in C (True), or False when the C library is absent and
     the caller runs its own loop."""
     lib = _clib()
-    if lib is None:
+    if lib is None or scheme not in ("ngp", "bilinear"):
         return False
     X = np.ascontiguousarray(X, dtype=np.float64)
     r = np.ascontiguousarray(r, dtype=np.float64)
@@ -590,9 +590,45 @@ def healpix_deposit(nside, scheme, wrap_guard, X, r, wt, sel, acc) -> bool:
     return True
 
 
-def paint_healpix(pos, mesh_shape, box, observer, rmin, rmax, nside, scheme="bilinear", mode="relative", weight=1.0):
+def healpix_pix2vec(nside, pix):
+    th, ph = healpix_pix2ang(nside, int(pix))
+    return np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+
+
+def healpix_rbf(nside, n, sigma):
+    """rbf_neighbor: the pixel of unit direction n and those of the eight directions one pixel resolution
+    sqrt(4 pi / npix) away (tangent-plane azimuths k pi/4, from the theta and phi unit vectors), each
+    weighted by exp(-d^2 / (2 sigma^2)) of the angular distance d = 2 asin(|n - c| / 2) to its centre c,
+    normalised to sum 1 (evaluated as exp(-(d^2 - dmin^2) / (2 sigma^2)), which cannot underflow). Returns (pixels, weights), distinct pixels in first-seen order."""
+    n = np.asarray(n, dtype=np.float64)
+    st = np.hypot(n[0], n[1])
+    cp, sp = (n[0] / st, n[1] / st) if st > 0 else (1.0, 0.0)
+    eth = np.array([n[2] * cp, n[2] * sp, -st])
+    eph = np.array([-sp, cp, 0.0])
+    step = np.sqrt(4 * np.pi / (12 * nside * nside))
+    pix = []
+    for k in range(-1, 8):
+        if k < 0:
+            u = n
+        else:
+            a = 0.25 * np.pi * k
+            u = np.cos(step) * n + np.sin(step) * (np.cos(a) * eth + np.sin(a) * eph)
+        ph = np.arctan2(u[1], u[0])
+        ph = ph + 2 * np.pi if ph < 0 else ph
+        ph = 0.0 if ph >= 2 * np.pi else ph
+        q = healpix_ang2pix(nside, min(1.0, max(-1.0, u[2])), ph)
+        if q not in pix:
+            pix.append(q)
+    d = np.array([2 * np.arcsin(min(1.0, 0.5 * np.linalg.norm(n - healpix_pix2vec(nside, q)))) for q in pix])
+    w = np.exp(-0.5 * ((d / sigma) ** 2 - (d.min() / sigma) ** 2))  # relative to the nearest: no underflow
+    return np.array(pix), w / w.sum()
+
+
+def paint_healpix(pos, mesh_shape, box, observer, rmin, rmax, nside, scheme="bilinear", mode="relative", weight=1.0,
+                  sigma=None):
     """Shell paint (build-defined normalisation: weighted count / (pixel solid angle x shell
-    volume per steradian)); positions wrapped into the periodic box; float64 geometry."""
+    volume per steradian)); positions wrapped into the periodic box; float64 geometry. rbf_neighbor:
+    healpix_rbf with width sigma (radians; default the pixel resolution read as a FWHM)."""
     pos = np.asarray(pos, dtype=np.float32)
     np_shape = pos.shape[:3]
     g = pos.reshape(-1, 3).astype(np.float64)
@@ -615,6 +651,11 @@ def paint_healpix(pos, mesh_shape, box, observer, rmin, rmax, nside, scheme="bil
             phi += 2 * np.pi
         if scheme == "ngp":
             acc[healpix_ang2pix(nside, Z / r[p], phi)] += wt[p]
+        elif scheme == "rbf_neighbor":
+            sg = sigma if sigma else np.sqrt(4 * np.pi / npix) / (2 * np.sqrt(2 * np.log(2)))
+            pix, w = healpix_rbf(nside, rel[p] / r[p], sg)
+            for q, wq in zip(pix, w):
+                acc[q] += wt[p] * wq
         else:
             theta = np.arctan2(np.sqrt(X * X + Y * Y), Z)
             pix, w = healpix_interp_weights(nside, theta, phi)

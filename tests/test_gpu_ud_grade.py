@@ -1,8 +1,10 @@
-"""HEALPix resolution changes on the GPU (pmx_healpix_ud_grade) against the oracle's restatement of
-healpy.ud_grade: degrade and upgrade over RING/NESTED input and output orders, powers, bad pixels with
-and without pess; spherical shells painted at paint_nside and resampled to the field's nside (the
-painting options of jaxpm.spherical.paint_particles_spherical, _src/fields/_painting.py:329-347); and
-the adjoint used by reverse-mode gradients (a dot-product test).
+"""Spherical painting options on the GPU (those jaxpm.spherical.paint_particles_spherical takes,
+_src/fields/_painting.py:329-347; fields/painting.py:82): HEALPix resolution changes
+(pmx_healpix_ud_grade) against the oracle's restatement of healpy.ud_grade -- degrade and upgrade over
+RING/NESTED input and output orders, powers, bad pixels with and without pess; shells painted at
+paint_nside and resampled to the field's nside; the adjoint used by reverse-mode gradients (a
+dot-product test); and the rbf_neighbor scheme against the oracle's restatement of this library's
+definition of it (parity with the upstream kernel, which the reference does not contain, is unpinned).
 
 Tolerances: maps 1e-6 relative (float64 sums on both sides, float32 storage); UNSEEN pixels equal;
 painted shells 1e-5 relative L2 against the oracle's paint and ud_grade."""
@@ -88,3 +90,37 @@ def test_ud_grade_refuses_bad_arguments(pkg, cuda):
         pkg.engine.healpix_ud_grade(m, 6)
     with pytest.raises(ValueError):
         pkg.engine.healpix_ud_grade(m, 4, order_in="SPIRAL")
+
+
+@pytest.mark.parametrize("kw", [{}, {"kernel_width_arcmin": 400.0}, {"kernel_width_arcmin": 150.0,
+                                                                       "smoothing_interpretation": "sigma"},
+                                {"kernel_width_arcmin": 300.0, "paint_nside": 16}])
+def test_rbf_neighbor_paint_matches_oracle(pkg, cuda, kw):
+    mesh, box = (16, 16, 16), (100.0,) * 3
+    rng = np.random.default_rng(23)
+    dx = (rng.standard_normal(mesh + (3,)) * 1.5).astype(np.float32)
+    fld = pkg.ParticleField(array=torch.from_numpy(dx).to(cuda), mesh_size=mesh, box_size=box, nside=8,
+                            unit=pkg.PositionUnit.GRID_RELATIVE, status=pkg.FieldStatus.PARTICLES)
+    sph = fld.paint_spherical(center=40.0, density_plane_width=30.0, scheme="rbf_neighbor", **kw)
+    pn = kw.get("paint_nside", 8)
+    w = kw.get("kernel_width_arcmin")
+    sigma = None
+    if w is not None:
+        sigma = np.deg2rad(w / 60.0)
+        if kw.get("smoothing_interpretation", "fwhm") == "fwhm":
+            sigma /= 2.0 * np.sqrt(2.0 * np.log(2.0))
+    ref = orc.paint_healpix(dx, mesh, box, fld.observer_position_mpc, 25.0, 55.0, pn, scheme="rbf_neighbor",
+                            sigma=sigma)
+    if pn != 8:
+        ref = so.ud_grade(ref, 8)
+    got = sph.array.cpu().numpy()
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
+    ngp = fld.paint_spherical(center=40.0, density_plane_width=30.0, scheme="ngp").array.cpu().numpy()
+    assert got.sum() == pytest.approx(ngp.sum(), rel=1e-5)  # the kernel moves weight, it does not create it
+
+
+def test_rbf_neighbor_has_no_adjoint(pkg, cuda):
+    x = torch.zeros((8, 8, 8, 3), device=cuda)
+    with pytest.raises(pkg._lib.PmxError):
+        pkg.engine.paint_healpix_vjp(x, torch.ones(12 * 4 * 4, device=cuda), (8, 8, 8), (40.0,) * 3, (20.0,) * 3, 2.0,
+                                     18.0, 4, scheme="rbf_neighbor")

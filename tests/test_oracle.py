@@ -281,6 +281,34 @@ def test_ud_grade_properties():
     assert gp[0] == so.UNSEEN and gp[1] == so.UNSEEN and gp[2] == pytest.approx(np.mean(mb[8:12]))
 
 
+def test_rbf_neighbor_weights():
+    """The rbf_neighbor weights sum to 1 over distinct candidates that start with the ngp pixel; as sigma -> 0
+    the weight goes to the nearest pixel centre (brute force over all pixels), as sigma -> inf it is uniform;
+    a shell paint keeps the total weight."""
+    rng = np.random.default_rng(8)
+    for nside in (4, 8):
+        npix = 12 * nside * nside
+        centres = np.array([orc.healpix_pix2vec(nside, q) for q in range(npix)])
+        for _ in range(40):
+            n = rng.standard_normal(3)
+            n /= np.linalg.norm(n)
+            ph = np.arctan2(n[1], n[0]) % (2 * np.pi)
+            pix, w = orc.healpix_rbf(nside, n, 0.02)
+            assert len(set(pix.tolist())) == len(pix) and abs(w.sum() - 1) < 1e-12
+            assert pix[0] == orc.healpix_ang2pix(nside, n[2], ph)
+            _, wt = orc.healpix_rbf(nside, n, 1e-3)
+            nearest = int(np.argmax(centres @ n))
+            if nearest in pix.tolist():
+                assert pix[int(np.argmax(wt))] == nearest
+            _, wu = orc.healpix_rbf(nside, n, 1e3)
+            assert np.allclose(wu, 1.0 / len(wu), atol=1e-9)
+    pos = (rng.standard_normal((8, 8, 8, 3)) * 2).astype(np.float32)
+    box, obs, rmin, rmax, nside = (40.0,) * 3, (20.0,) * 3, 3.0, 18.0, 4
+    m = orc.paint_healpix(pos, (8, 8, 8), box, obs, rmin, rmax, nside, scheme="rbf_neighbor").astype(np.float64)
+    ng = orc.paint_healpix(pos, (8, 8, 8), box, obs, rmin, rmax, nside, scheme="ngp").astype(np.float64)
+    assert m.sum() == pytest.approx(ng.sum(), rel=1e-6) and np.abs(m - ng).max() > 0
+
+
 # ---------------------------------------------------------------- cosmology
 def test_eds_closed_forms():
     c = nbo.Cosmo(Omega_c=0.95, Omega_b=0.05)
