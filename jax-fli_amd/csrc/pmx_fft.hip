@@ -39,25 +39,36 @@ PMX_FP(512, 3, 8, 8, 8, 0)
 PMX_FP(1024, 4, 8, 8, 4, 4)
 #undef PMX_FP
 
-// LDS layout of one line: complex element e sits at px(e) = e + e/16 (a skipped slot every 16
-// values breaks the bank conflicts of the Stockham strides), and the lines of a workgroup are LSP<N>
-// slots apart, chosen (by a model of the 32-bank LDS) so that the line loads and the transposed
-// stores of a workgroup also spread over the banks.
-__device__ __forceinline__ int px(int e) { return e + (e >> 4); }
+// LDS layout of one line of N complex values: element e sits at slot px<N>(e), and the lines of a
+// workgroup are LSP<N> slots apart. Both come from a model of gfx950's LDS -- 64 banks of 4 bytes, a
+// wave's 8-byte accesses served 32 lanes at a time -- which reproduces the bank-conflict counter of
+// the FFT passes, applied to every Stockham stage's strided reads and writes and to the line loads and
+// transposed stores: for 256, 512 and 1024 points an XOR of the low five bits with bits 3..7 makes the
+// stages conflict-free (half the LDS cycles of a skipped slot every 16 values, e + e/16), and the other
+// XORs below cut 15-50% for the shorter lines; 8 and 128 points keep e + e/16. Every map is a
+// permutation of the line's slots (the XOR forms) or injective (e + e/16) within LSP<N>.
+template <int N> __device__ __forceinline__ int px(int e) {
+  if constexpr (N == 256 || N == 512 || N == 1024) return e ^ ((e >> 3) & 31);
+  else if constexpr (N == 96 || N == 192) return e ^ ((e >> 5) & 3);
+  else if constexpr (N == 64) return e ^ (((e >> 3) & 7) << 2);
+  else if constexpr (N == 32) return e ^ ((e >> 3) & 3);
+  else if constexpr (N == 16) return e ^ ((e >> 2) & 3);
+  else return e + (e >> 4);
+}
 template <int N> constexpr int LSP() {
   switch (N) {
     case 4: return 5;
     case 8: return 9;
-    case 16: return 19;
-    case 32: return 35;
+    case 16: return 20;
+    case 32: return 36;
     case 48: return 59;
-    case 64: return 71;
-    case 96: return 118;
+    case 64: return 69;
+    case 96: return 108;
     case 128: return 137;
-    case 192: return 212;
-    case 256: return 274;
-    case 512: return 548;
-    case 1024: return 1096;
+    case 192: return 216;
+    case 256: return 260;
+    case 512: return 520;
+    case 1024: return 1040;
     default: return N + (N >> 4) + 1;
   }
 }
@@ -149,48 +160,55 @@ template <bool INV> struct Bfly<8, INV> {
 
 // ---------------------------------------------------------------- Stockham stage
 // One radix-R stage of an N-point transform held in LDS, NS = product of the radices already
-// applied. Butterfly j reads src[j + r N/R], twiddles by w^(r (j mod NS) N/(NS R)), writes
-// dst[(j/NS) NS R + j mod NS + r NS] (self-sorting: natural order out after the last stage).
+// applied. Butterfly j reads a[j + r N/R], twiddles by w^(r (j mod NS) N/(NS R)), writes
+// a[(j/NS) NS R + j mod NS + r NS] (self-sorting: natural order out after the last stage). The stage
+// runs in place in one line buffer: each thread reads all of its butterflies' inputs into registers,
+// the workgroup synchronises, and the outputs are written over the inputs -- half the LDS of a
+// two-buffer (ping-pong) form, so twice the workgroups per CU, for one more barrier per stage. Every
+// thread of the workgroup must call; the result is in `a`, visible to the whole workgroup on return.
 template <int N, int R, int NS, bool INV>
-__device__ __forceinline__ void stage(const float2* __restrict__ src, float2* __restrict__ dst, int t,
-                                      const float2* __restrict__ tw) {
+__device__ __forceinline__ void stage_inplace(float2* a, int t, const float2* __restrict__ tw) {
   constexpr int T = TPL<N>();
-  constexpr int NB = N / R;
+  constexpr int NB = N / R, NJ = (NB + T - 1) / T;
+  float2 v[NJ][R];
 #pragma unroll
-  for (int jj = 0; jj < (NB + T - 1) / T; ++jj) {
+  for (int jj = 0; jj < NJ; ++jj) {
     const int j = t + jj * T;
     if ((NB % T == 0) || j < NB) {
       const int k = j % NS;
-      float2 v[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[r] = src[px(j + r * NB)];
+      for (int r = 0; r < R; ++r) v[jj][r] = a[px<N>(j + r * NB)];
       if (NS > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
           const float2 w = tw[r * k * (N / (NS * R))];
-          v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
+          v[jj][r] = INV ? cmulc(v[jj][r], w) : cmul(v[jj][r], w);
         }
       }
-      Bfly<R, INV>::run(v);
-      const int d = (j / NS) * NS * R + k;
-#pragma unroll
-      for (int r = 0; r < R; ++r) dst[px(d + r * NS)] = v[r];
+      Bfly<R, INV>::run(v[jj]);
     }
   }
+  __syncthreads();
+#pragma unroll
+  for (int jj = 0; jj < NJ; ++jj) {
+    const int j = t + jj * T;
+    if ((NB % T == 0) || j < NB) {
+      const int k = j % NS;
+      const int d = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[px<N>(d + r * NS)] = v[jj][r];
+    }
+  }
+  __syncthreads();
 }
-
-// All stages; every thread of the workgroup must call (barriers inside). The result ends in `a`
-// when the stage count is even, in `b` when odd (see res_in_b).
 template <int N, bool INV, int S = 0, int NS = 1>
-__device__ __forceinline__ void fft_run(float2* a, float2* b, int t, const float2* __restrict__ tw) {
+__device__ __forceinline__ void fft_inplace(float2* a, int t, const float2* __restrict__ tw) {
   if constexpr (S < FP<N>::S) {
     constexpr int R = FP<N>::R[S];
-    stage<N, R, NS, INV>(a, b, t, tw);
-    __syncthreads();
-    fft_run<N, INV, S + 1, NS * R>(b, a, t, tw);
+    stage_inplace<N, R, NS, INV>(a, t, tw);
+    fft_inplace<N, INV, S + 1, NS * R>(a, t, tw);
   }
 }
-template <int N> constexpr bool res_in_b() { return (FP<N>::S & 1) != 0; }
 
 // ---------------------------------------------------------------- P1: r2c along z
 template <int NZ>
@@ -198,7 +216,7 @@ __global__ void __launch_bounds__(256) k_r2c_z(const float* __restrict__ in, flo
                                                int NX, int NY, const float2* __restrict__ twM,
                                                const float2* __restrict__ twN, Slots sl) {
   constexpr int M = NZ / 2, T = TPL<M>(), L = LPW<M>(), LS = LSP<M>(), NT = L * T;
-  __shared__ float2 s0[L * LS], s1[L * LS];
+  __shared__ float2 s0[L * LS];
   const unsigned zs = blockIdx.z;
   if (!slot_live(sl, zs)) return;
   in = at_slot(in, sl.stride, zs);
@@ -210,28 +228,30 @@ __global__ void __launch_bounds__(256) k_r2c_z(const float* __restrict__ in, flo
   for (int i = tid; i < M; i += NT) stw[i] = twM[i];
   for (int i = tid; i < nl * M; i += NT) {
     const int l = i / M, k = i - l * M;
-    s0[l * LS + px(k)] = src[i];
+    s0[l * LS + px<M>(k)] = src[i];
   }
   __syncthreads();
   const int l = tid / T, t = tid - l * T;
-  fft_run<M, false>(s0 + l * LS, s1 + l * LS, t, stw);
-  const float2* res = res_in_b<M>() ? s1 : s0;
-  float2* oth = res_in_b<M>() ? s0 : s1;
-  // split: X_k = E_k + W^k O_k,  E = (Z_k + conj Z_{M-k})/2,  O = -i (Z_k - conj Z_{M-k})/2
-  for (int i = tid; i < L * (M + 1); i += NT) {
-    const int ll = i / (M + 1), k = i - ll * (M + 1);
-    const float2 zk = res[ll * LS + px((k == M ? 0 : k))];
-    const float2 zc = cconj(res[ll * LS + px((k == 0 ? 0 : M - k))]);
-    const float2 e = cscale(cadd(zk, zc), 0.5f);
-    const float2 o = cscale(mul_mi<false>(csub(zk, zc)), 0.5f);
-    oth[ll * LS + px(k)] = cadd(e, cmul(twN[k], o));
+  fft_inplace<M, false>(s0 + l * LS, t, stw);
+  // split: X_k = E_k + W^k O_k,  E = (Z_k + conj Z_{M-k})/2,  O = -i (Z_k - conj Z_{M-k})/2, in place:
+  // one thread takes the pair (p, M - p) (Z_M is Z_0), reads both inputs and writes both outputs
+  constexpr int NP = M / 2 + 1;
+  for (int i = tid; i < L * NP; i += NT) {
+    const int ll = i / NP, p = i - ll * NP, q = M - p;
+    float2* line = s0 + ll * LS;
+    const float2 a = line[px<M>(p)], b = line[px<M>(q == M ? 0 : q)];
+    const float2 ea = cscale(cadd(a, cconj(b)), 0.5f), oa = cscale(mul_mi<false>(csub(a, cconj(b))), 0.5f);
+    const float2 eb = cscale(cadd(b, cconj(a)), 0.5f), ob = cscale(mul_mi<false>(csub(b, cconj(a))), 0.5f);
+    const float2 xp = cadd(ea, cmul(twN[p], oa)), xq = cadd(eb, cmul(twN[q], ob));
+    line[px<M>(p)] = xp;
+    if (q != p) line[px<M>(q)] = xq;
   }
   __syncthreads();
   float2* dst = outA + (size_t)x * NY + y0;
   const size_t osk = (size_t)NX * NY;
   for (int i = tid; i < (M + 1) * L; i += NT) {
     const int k = i / L, ll = i - k * L;
-    if (ll < nl) dst[k * osk + ll] = oth[ll * LS + px(k)];
+    if (ll < nl) dst[k * osk + ll] = s0[ll * LS + px<M>(k)];
   }
 }
 
@@ -243,7 +263,7 @@ __global__ void __launch_bounds__(256) k_c2c_lines(const float2* __restrict__ in
                                                    long isy, long isl, long isz, long osy, long osk, long osz,
                                                    int nlines, const float2* __restrict__ tw, int nzb, Slots sl) {
   constexpr int T = TPL<N>(), L = LPW<N>(), LS = LSP<N>(), NT = L * T;
-  __shared__ float2 s0[L * LS], s1[L * LS];
+  __shared__ float2 s0[L * LS];
   const unsigned zs = blockIdx.z / (unsigned)nzb;  // grid z: slot * nzb + the pass's own z block
   const int by = blockIdx.y, bz = (int)(blockIdx.z - zs * (unsigned)nzb), l0 = (int)xcd_x() * L, tid = threadIdx.x;
   if (l0 >= nlines) return;  // a padded grid's extra workgroups exit whole
@@ -256,16 +276,16 @@ __global__ void __launch_bounds__(256) k_c2c_lines(const float2* __restrict__ in
   for (int i = tid; i < N; i += NT) stw[i] = tw[i];
   for (int i = tid; i < nl * N; i += NT) {
     const int l = i / N, k = i - l * N;
-    s0[l * LS + px(k)] = src[l * isl + k];
+    s0[l * LS + px<N>(k)] = src[l * isl + k];
   }
   __syncthreads();
   const int l = tid / T, t = tid - l * T;
-  fft_run<N, INV>(s0 + l * LS, s1 + l * LS, t, stw);
-  const float2* res = res_in_b<N>() ? s1 : s0;
+  fft_inplace<N, INV>(s0 + l * LS, t, stw);
+  const float2* res = s0;
   float2* dst = out + by * osy + bz * osz + l0;
   for (int i = tid; i < N * L; i += NT) {
     const int k = i / L, ll = i - k * L;
-    if (ll < nl) dst[k * osk + ll] = res[ll * LS + px(k)];
+    if (ll < nl) dst[k * osk + ll] = res[ll * LS + px<N>(k)];
   }
 }
 
@@ -306,7 +326,7 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
                                                int NZ, long ocomp, KOp op, XTabs tb,
                                                const float2* __restrict__ tw, Slots sl) {
   constexpr int T = TPL<N>(), L = LPW<N>(), LS = LSP<N>(), NT = L * T;
-  constexpr int NBUF = (MODE == XP_FORCES || MODE == XP_HESS) ? 3 : 2;
+  constexpr int NBUF = (MODE == XP_FORCES || MODE == XP_HESS) ? 2 : 1;
   __shared__ float2 sm[NBUF * L * LS];
   const unsigned zs = blockIdx.z;
   if (!slot_live(sl, zs)) return;
@@ -315,36 +335,34 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
   if (sl.coef) op.scale = sl.coef[4 * zs + 3];
   __shared__ float2 stw[N];
   float2* s0 = sm;
-  float2* s1 = sm + L * LS;
   const int kz = blockIdx.y, ky0 = (int)xcd_x() * L, tid = threadIdx.x;
   const int nl = min(L, NY - ky0);
   const float2* src = in + ((size_t)kz * NY + ky0) * N;
   for (int i = tid; i < N; i += NT) stw[i] = tw[i];
   for (int i = tid; i < nl * N; i += NT) {
     const int l = i / N, k = i - l * N;
-    s0[l * LS + px(k)] = src[i];
+    s0[l * LS + px<N>(k)] = src[i];
   }
   __syncthreads();
   const int l = tid / T, t = tid - l * T;
   constexpr bool FWD = (MODE != XP_INV);
-  fft_run<N, !FWD>(s0 + l * LS, s1 + l * LS, t, stw);
-  float2* res = res_in_b<N>() ? s1 : s0;
-  float2* oth = res_in_b<N>() ? s0 : s1;
+  fft_inplace<N, !FWD>(s0 + l * LS, t, stw);
+  float2* res = s0;
 
   if constexpr (MODE == XP_FWD) {
     float2* dst = out + ((size_t)kz * NY + ky0) * N;
     for (int i = tid; i < nl * N; i += NT) {
       const int ll = i / N, k = i - ll * N;
-      dst[i] = res[ll * LS + px(k)];
+      dst[i] = res[ll * LS + px<N>(k)];
     }
   } else if constexpr (MODE == XP_INV) {
     float2* dst = out + (size_t)kz * N * NY + ky0;
     for (int i = tid; i < N * L; i += NT) {
       const int k = i / L, ll = i - k * L;
-      if (ll < nl) dst[(size_t)k * NY + ll] = res[ll * LS + px(k)];
+      if (ll < nl) dst[(size_t)k * NY + ll] = res[ll * LS + px<N>(k)];
     }
   } else if constexpr (MODE == XP_FORCES) {
-    float2* s2 = sm + 2 * L * LS;
+    float2* s2 = sm + L * LS;
     for (int c = 0; c < 3; ++c) {
       for (int i = tid; i < L * N; i += NT) {
         const int ll = i / N, kx = i - ll * N;
@@ -353,23 +371,23 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
         const float g = (c == 0) ? tb.gx[kx] : (c == 1 ? tb.gy[ky] : tb.gz[kz]);
         float coef = (k2 > 0.f) ? g * op.scale / k2 : 0.f;
         if (op.flags & 1u) coef *= tb.dx[kx] * tb.dy[ky] * tb.dz[kz];
-        const float2 d = res[ll * LS + px(kx)];
-        s2[ll * LS + px(kx)] = make_float2(-coef * d.y, coef * d.x);  // i * coef * d
+        const float2 d = res[ll * LS + px<N>(kx)];
+        s2[ll * LS + px<N>(kx)] = make_float2(-coef * d.y, coef * d.x);  // i * coef * d
       }
       __syncthreads();
-      fft_run<N, true>(s2 + l * LS, oth + l * LS, t, stw);
-      const float2* r2 = res_in_b<N>() ? oth : s2;
+      fft_inplace<N, true>(s2 + l * LS, t, stw);
+      const float2* r2 = s2;
       float2* dst = out + c * ocomp + (size_t)kz * N * NY + ky0;
       for (int i = tid; i < N * L; i += NT) {
         const int k = i / L, ll = i - k * L;
-        if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + px(k)];
+        if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + px<N>(k)];
       }
       __syncthreads();
     }
   } else if constexpr (MODE == XP_HESS) {
     // H_ij = ifft(g_i g_j delta / k^2): the FastPM gradient kernel twice on the potential -delta/k^2,
     // (i g_i)(i g_j)(-delta / k^2); real coefficients, 0 at k = 0 and on Nyquist planes (g = 0 there)
-    float2* s2 = sm + 2 * L * LS;
+    float2* s2 = sm + L * LS;
     const bool off = (op.flags & kHessOffDiag) != 0u;
     for (int c = 0; c < 3; ++c) {
       const int ci = off ? (c == 2 ? 1 : 0) : c, cj = off ? (c == 0 ? 1 : 2) : c;
@@ -380,16 +398,16 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
         const float gi = (ci == 0) ? tb.gx[kx] : (ci == 1 ? tb.gy[ky] : tb.gz[kz]);
         const float gj = (cj == 0) ? tb.gx[kx] : (cj == 1 ? tb.gy[ky] : tb.gz[kz]);
         const float coef = (k2 > 0.f) ? gi * gj * op.scale / k2 : 0.f;
-        const float2 d = res[ll * LS + px(kx)];
-        s2[ll * LS + px(kx)] = make_float2(coef * d.x, coef * d.y);
+        const float2 d = res[ll * LS + px<N>(kx)];
+        s2[ll * LS + px<N>(kx)] = make_float2(coef * d.x, coef * d.y);
       }
       __syncthreads();
-      fft_run<N, true>(s2 + l * LS, oth + l * LS, t, stw);
-      const float2* r2 = res_in_b<N>() ? oth : s2;
+      fft_inplace<N, true>(s2 + l * LS, t, stw);
+      const float2* r2 = s2;
       float2* dst = out + c * ocomp + (size_t)kz * N * NY + ky0;
       for (int i = tid; i < N * L; i += NT) {
         const int k = i / L, ll = i - k * L;
-        if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + px(k)];
+        if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + px<N>(k)];
       }
       __syncthreads();
     }
@@ -404,15 +422,15 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
       if ((op.flags & 2u) && k2 > 0.f) coef *= expf(-op.pgd_kl2 / k2) * expf(-(k2 * k2) / op.pgd_ks4);
       if ((op.flags & 4u) && k2 > 0.f)
         coef *= op.mix_a + op.mix_b * (expf(-op.pgd_kl2 / k2) * expf(-(k2 * k2) / op.pgd_ks4));
-      res[ll * LS + px(kx)] = cscale(res[ll * LS + px(kx)], coef);
+      res[ll * LS + px<N>(kx)] = cscale(res[ll * LS + px<N>(kx)], coef);
     }
     __syncthreads();
-    fft_run<N, true>(res + l * LS, oth + l * LS, t, stw);
-    const float2* r2 = res_in_b<N>() ? oth : res;
+    fft_inplace<N, true>(res + l * LS, t, stw);
+    const float2* r2 = res;
     float2* dst = out + (size_t)kz * N * NY + ky0;
     for (int i = tid; i < N * L; i += NT) {
       const int k = i / L, ll = i - k * L;
-      if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + px(k)];
+      if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + px<N>(k)];
     }
   } else if constexpr (MODE == XP_COLOR) {
     for (int i = tid; i < L * N; i += NT) {
@@ -422,15 +440,15 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
                              op.kphys_z[kz] * op.kphys_z[kz]);
       double pk = interp_lin(kk, op.ktab, op.ptab, op.ntab) * op.cnorm;
       const float amp = (float)(sqrt(pk > 0.0 ? pk : 0.0)) * op.scale;
-      res[ll * LS + px(kx)] = cscale(res[ll * LS + px(kx)], amp);
+      res[ll * LS + px<N>(kx)] = cscale(res[ll * LS + px<N>(kx)], amp);
     }
     __syncthreads();
-    fft_run<N, true>(res + l * LS, oth + l * LS, t, stw);
-    const float2* r2 = res_in_b<N>() ? oth : res;
+    fft_inplace<N, true>(res + l * LS, t, stw);
+    const float2* r2 = res;
     float2* dst = out + (size_t)kz * N * NY + ky0;
     for (int i = tid; i < N * L; i += NT) {
       const int k = i / L, ll = i - k * L;
-      if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + px(k)];
+      if (ll < nl) dst[(size_t)k * NY + ll] = r2[ll * LS + px<N>(k)];
     }
   } else if constexpr (MODE == XP_POWER) {
     // bins: 0 .. nedges (nedges + 1 entries), bin = number of edges <= |k|. The half spectrum holds
@@ -460,7 +478,7 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
         int mid = (lo + hi) >> 1;
         if (op.kedges[mid] <= kk) lo = mid + 1; else hi = mid;
       }
-      const float2 d = res[ll * LS + px(kx)];
+      const float2 d = res[ll * LS + px<N>(kx)];
       double re, im = 0.0;
       if (op.spec2) {  // m1 conj(m2)
         const float2 e = op.spec2[((size_t)kz * NY + ky) * N + kx];
@@ -522,7 +540,7 @@ __global__ void __launch_bounds__(384) k_c2r_z(const float2* __restrict__ inD, f
   constexpr int M = NZ / 2, T = TPL<M>();
   constexpr int L = (NC == 1) ? LPW<M>() : (LPW<M>() / 2 > 0 ? LPW<M>() / 2 : 1);
   constexpr int LS = LSP<M>(), NL = NC * L, NT = NL * T, NZH = M + 1, NZHP = (NZH + 7) & ~7;
-  __shared__ float2 s0[NL * LS], s1[NL * LS];
+  __shared__ float2 s0[NL * LS];
   const unsigned zs = blockIdx.z;
   if (!slot_live(sl, zs)) return;
   inD = at_slot(inD, sl.stride, zs);
@@ -535,27 +553,32 @@ __global__ void __launch_bounds__(384) k_c2r_z(const float2* __restrict__ inD, f
   for (int i = tid; i < NC * nl * NZH; i += NT) {
     const int c = i / (nl * NZH), rem = i - c * (nl * NZH);
     const int ll = rem / NZH, k = rem - ll * NZH;
-    s1[(c * L + ll) * LS + px(k)] = inD[c * icomp + ((size_t)x * NY + y0 + ll) * NZHP + k];
+    s0[(c * L + ll) * LS + px<M>(k)] = inD[c * icomp + ((size_t)x * NY + y0 + ll) * NZHP + k];
   }
   __syncthreads();
-  // merge: Z'_k = (X_k + conj X_{M-k}) + i (X_k - conj X_{M-k}) conj(W^k)
-  for (int i = tid; i < NL * M; i += NT) {
-    const int ln = i / M, k = i - ln * M;
-    const float2 xk = s1[ln * LS + px(k)];
-    const float2 xc = cconj(s1[ln * LS + px((M - k))]);
-    const float2 a = cadd(xk, xc);
-    const float2 b = cmulc(csub(xk, xc), twN[k]);
-    s0[ln * LS + px(k)] = make_float2(a.x - b.y, a.y + b.x);
+  // merge: Z'_k = (X_k + conj X_{M-k}) + i (X_k - conj X_{M-k}) conj(W^k), in place: one thread takes
+  // the pair (p, M - p), reads X_p and X_{M-p} and writes Z'_p and Z'_{M-p} (X_M only feeds Z'_0)
+  constexpr int NP = M / 2 + 1;
+  for (int i = tid; i < NL * NP; i += NT) {
+    const int ln = i / NP, p = i - ln * NP, q = M - p;
+    float2* line = s0 + ln * LS;
+    const float2 xp = line[px<M>(p)], xq = line[px<M>(q)];
+    const float2 ap = cadd(xp, cconj(xq)), bp = cmulc(csub(xp, cconj(xq)), twN[p]);
+    line[px<M>(p)] = make_float2(ap.x - bp.y, ap.y + bp.x);
+    if (p != 0 && q != p) {
+      const float2 aq = cadd(xq, cconj(xp)), bq = cmulc(csub(xq, cconj(xp)), twN[q]);
+      line[px<M>(q)] = make_float2(aq.x - bq.y, aq.y + bq.x);
+    }
   }
   __syncthreads();
   const int ln = tid / T, t = tid - ln * T;
-  fft_run<M, true>(s0 + ln * LS, s1 + ln * LS, t, stw);
-  const float2* res = res_in_b<M>() ? s1 : s0;
+  fft_inplace<M, true>(s0 + ln * LS, t, stw);
+  const float2* res = s0;
   float* dst = out + ((size_t)x * NY + y0) * NZ * NC;
   for (int i = tid; i < nl * NZ * NC; i += NT) {
     const int ll = i / (NZ * NC), rem = i - ll * (NZ * NC);
     const int z = rem / NC, c = rem - z * NC;
-    const float2 v = res[(c * L + ll) * LS + px((z >> 1))];
+    const float2 v = res[(c * L + ll) * LS + px<M>((z >> 1))];
     dst[i] = (z & 1) ? v.y : v.x;
   }
 }
@@ -707,7 +730,7 @@ int pmx_fft_forward_to_B(pmx_plan_t* p, const float* in, hipStream_t s) {
 
 namespace {
 template <int N> constexpr size_t xpass_static_lds() {
-  return (2 * (size_t)LPW<N>() * LSP<N>() + N) * sizeof(float2);
+  return ((size_t)LPW<N>() * LSP<N>() + N) * sizeof(float2);  // one line buffer + the twiddles
 }
 }  // namespace
 
