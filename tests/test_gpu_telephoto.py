@@ -388,3 +388,71 @@ def test_nbody_gradient_through_image_snapshots(pkg, cuda, kind, target, adjoint
         got = float((grad.double().cpu().numpy() * d).sum())
         print(f"{kind} {target} {adjoint} d/d{which}: adjoint {got:.6e} fd {fd:.6e}")
         assert abs(got - fd) / abs(fd) < 2e-3
+
+
+@pytest.mark.parametrize("kind,adjoint", [("drift", "reverse"), ("drift", "checkpointed"), ("telephoto", "reverse")])
+def test_cosmology_gradient_through_flat_image_snapshots(pkg, cuda, kind, adjoint):
+    """d L / d (Omega_c, sigma8) through white noise -> ICs -> LPT -> N-body with one flat-sky snapshot
+    painted from a drifted (DriftInterp) or rotated, shifted and drifted (TelephotoInterp, a shell beyond
+    the box radius) image. Omega_c reaches the map through the ICs, the growth, the step coefficients and
+    the drift factor f(chi) of the image transform (pmx_image_positions_vjp's parameter tables); sigma8
+    through the IC amplitude. Against float64 central differences of the oracle's forward model at the
+    snapshot's fixed geometry (scale factor, slab centre and width)."""
+    N, BOX, A0, DT0, FLAT = 16, 100.0, 0.1, 0.2, (16, 16)
+    KT = np.logspace(-4, 1, 128)
+    c0 = pkg.Cosmology(**P)
+    center, width = (30.0, 30.0) if kind == "drift" else (70.0, 30.0)
+    ts = np.stack([pkg.cosmology.a_of_chi(c0, np.array([center - width / 2])),
+                   pkg.cosmology.a_of_chi(c0, np.array([center + width / 2]))])
+    rng = np.random.default_rng(83)
+    noise = rng.standard_normal((N,) * 3).astype(np.float32)
+    Oc = torch.tensor(P["Omega_c"], dtype=torch.float64, requires_grad=True)
+    s8 = torch.tensor(P["sigma8"], dtype=torch.float64, requires_grad=True)
+    c = pkg.Cosmology(**{**P, "Omega_c": Oc, "sigma8": s8})
+    ic = pkg.interpolate_initial_conditions(torch.from_numpy(noise).to(cuda), (N,) * 3, (BOX,) * 3, cosmo=c,
+                                            flatsky_npix=FLAT)
+    dx, p = pkg.lpt(c, ic, ts=A0)
+    painting = pkg.PaintingOptions(target="flat")
+    interp = (pkg.DriftInterp(painting=painting) if kind == "drift"
+              else pkg.TelephotoInterp(painting=painting, drift_on_lightcone=True))
+    solver = pkg.ReversibleDoubleKickDrift(interp_kernel=interp)
+    out = pkg.nbody(c, dx, p, t0=A0, t1=1.0, dt0=DT0, ts=ts, solver=solver, adjoint=adjoint)
+    W = rng.standard_normal(tuple(out.array.shape))
+    L = (out.array.double() * torch.from_numpy(W).to(cuda)).sum()
+    L.backward()
+    r_c = float(np.asarray(out.comoving_centers).reshape(-1)[0])
+    w_c = float(np.asarray(out.density_width).reshape(-1)[0])
+    t_snap = float(np.asarray(out.scale_factors, dtype=np.float64).reshape(-1)[0])
+    radius = dx.max_comoving_radius
+    code, shift = (1, (0.0, 0.0, float(np.floor(r_c / radius) * radius))) if kind == "telephoto" else (0, (0.0,) * 3)
+    assert kind == "drift" or r_c > radius
+    zlo, zhi, scale = pkg.fields.flat_slab(dx, r_c, w_c)
+    obs = (BOX / 2,) * 3
+
+    def oracle_loss(oc, sig):
+        o = nbo.Cosmo(**{**P, "Omega_c": oc, "sigma8": sig})
+        ic0 = nbo.color_field(noise.astype(np.float64), (BOX,) * 3, KT, nbo.linear_matter_power(o, KT))
+        x0, v0 = nbo.lpt1(o, ic0, A0)
+        snap = {}
+
+        def save(i, t, x, v):
+            snap["xv"] = (x, v)
+
+        nbo.run_nbody(o, x0, v0, A0, 1.0, DT0, np.array([t_snap], np.float32), solver="kkd", on_save=save)
+        x, v = snap["xv"]
+        g = so.image_positions(x, (N,) * 3, (BOX,) * 3, obs, code=code, shift=shift, vel=v,
+                               drift=so.Drift(o, t_snap)).reshape((N,) * 3 + (3,))
+        res = so.paint_flat(g, (N,) * 3, FLAT, zlo, zhi, scale, mode="absolute")
+        return float((res.reshape(W.shape) * W).sum())
+
+    l0 = oracle_loss(P["Omega_c"], P["sigma8"])
+    assert l0 != 0.0
+    sc = float((out.array.detach().double().abs() * torch.from_numpy(np.abs(W)).to(cuda)).sum())
+    assert abs(float(L) - l0) / sc < 1e-4
+    h = 1e-6
+    fd_oc = (oracle_loss(P["Omega_c"] + h, P["sigma8"]) - oracle_loss(P["Omega_c"] - h, P["sigma8"])) / (2 * h)
+    fd_s8 = (oracle_loss(P["Omega_c"], P["sigma8"] + h) - oracle_loss(P["Omega_c"], P["sigma8"] - h)) / (2 * h)
+    print(f"{kind} {adjoint}: d/dOmega_c adjoint {float(Oc.grad):.6e} fd {fd_oc:.6e}; "
+          f"d/dsigma8 adjoint {float(s8.grad):.6e} fd {fd_s8:.6e}")
+    assert abs(float(Oc.grad) - fd_oc) / abs(fd_oc) < 2e-3
+    assert abs(float(s8.grad) - fd_s8) / abs(fd_s8) < 2e-3
