@@ -464,6 +464,8 @@ def main():
     ap.add_argument("--no-vjp", action="store_true", help="skip the forward+VJP measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-large", action="store_true", help="skip the 512^3-on-1024^3 forward + VJP measurement")
+    ap.add_argument("--no-config1", action="store_true",
+                    help="skip the 64^3 workload (keeps profiler runs to 256^3 dispatches)")
     ap.add_argument("--large-only", action="store_true",
                     help="run only the 512^3-on-1024^3 forward + VJP timing (no differences) and print its JSON")
     ap.add_argument("--large-sweep", action="store_true",
@@ -657,7 +659,7 @@ def main():
     # CPU baseline: the NumPy/SciPy/C restatement of the reference step on the host's CPU share, and on
     # one core; the 64^3 workload beside it (GPU fused, GPU unfused, CPU)
     cpu = None
-    config1 = bench_config1(pkg, dev, cpu=args.cpu_steps > 0)
+    config1 = None if args.no_config1 else bench_config1(pkg, dev, cpu=args.cpu_steps > 0)
     if args.cpu_steps > 0:
         from oracle import pm_oracle as orc
 
