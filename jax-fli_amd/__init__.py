@@ -5,7 +5,7 @@ registers it). The compute path is libpmx.so (hand-written HIP for gfx950, C ABI
 PyTorch only provides device memory and streams. Names follow ``import jax_fli as jfli``.
 """
 
-from . import _lib, autodiff, cosmology, engine, fields, initial, io, lensing, pm, power, redshift, serving, utils
+from . import _lib, autodiff, cosmology, engine, fields, initial, io, lensing, pm, power, prng, redshift, serving, utils
 from .cosmology import Cosmology, Planck18
 from .fields import (AbstractField, ConvergenceUnit, DensityField, DensityUnit, FieldStatus, FlatDensity,
                      FlatKappaField, ParticleField, PhysicalUnit, PositionUnit, SphericalDensity,

@@ -18,6 +18,7 @@ import torch
 
 from .. import cosmology as _c
 from .. import engine
+from .. import prng
 from ..fields import DensityUnit, FieldStatus, PositionUnit, SphericalDensity, _paint_mode, flat_slab
 from ..painting import PaintingOptions
 
@@ -319,18 +320,22 @@ class OnionTiler(AbstractInterp):
     on axes where the observer sits on the face at 0 / 1), each rotated by a random cube symmetry,
     all summed into one map by a single pmx_paint_healpix_tiled launch.
 
-    Differences from the reference, by design: the rotations come from numpy's PCG64 seeded with
-    `seed` (the reference draws jax.random.randint(PRNGKey(42), (27,), 0, 47), whose bits this
-    engine does not reproduce); particles are wrapped into the box before imaging (the reference
-    images unwrapped positions); `layers` > 1 extends the offsets to [-layers, layers]^3 so shells
-    beyond 1.5 boxes are covered, and `rotate=False` gives plain periodic replication. With
-    drift_on_lightcone (the reference default) every image is drifted to its own lightcone time
-    inside the same paint kernel."""
+    The rotations are the reference's draw, jax.random.randint(PRNGKey(seed), (n,), 0, 47) with
+    seed 42 (interp.py:334-335), reproduced on the host by jax_fli_amd.prng (Threefry-2x32 with JAX's
+    partitionable key conventions; int32 as under jax_enable_x64=False, int64 with x64=True); index i
+    of the reference's 47-entry table is entry i + 1 of the 48-entry one here.
+
+    Differences from the reference, by design: particles are wrapped into the box before imaging (the
+    reference images unwrapped positions); `layers` > 1 extends the offsets to [-layers, layers]^3 (and
+    the draw to (2 layers + 1)^3 rotations) so shells beyond 1.5 boxes are covered, and `rotate=False`
+    gives plain periodic replication. With drift_on_lightcone (the reference default) every image is
+    drifted to its own lightcone time inside the same paint kernel."""
 
     drift_on_lightcone: bool = True
     layers: int = 1
     rotate: bool = True
     seed: int = 42
+    x64: bool = False
 
     def init(self) -> InterpTilerState:
         if self.painting.target != "spherical":
@@ -339,7 +344,7 @@ class OnionTiler(AbstractInterp):
             raise ValueError("layers must be >= 1")
         n = (2 * self.layers + 1) ** 3
         if self.rotate:  # entries 1..47 of the symmetry table (0 is the identity)
-            rots = np.random.default_rng(self.seed).integers(0, 47, size=n) + 1
+            rots = prng.randint(prng.prng_key(self.seed), n, 0, 47, x64=self.x64).astype(np.int64) + 1
         else:
             rots = np.zeros(n, dtype=np.int64)
         return InterpTilerState(rotation_idx=-1, shell_idx=-1, rotations=rots.astype(np.int32))

@@ -37,6 +37,15 @@ Parity status per function:
                                         (lo, hi]; mass = selected weight / pixel volume)
   paint_flat_vjp                        pinned (float64 central differences of paint_flat along random
                                         directions; z gets nothing)
+  threefry2x32                          pinned (Random123's known-answer vectors for 20 rounds: zero
+                                        key and counter, all-ones, pi digits)
+  jax_split_original                    pinned (JAX's documented split(PRNGKey(0)) under the original
+                                        scheme: [4146024105, 967050713], [2718843009, 1272950319])
+  jax_randint                           parity unpinned (the partitionable split / random-bits
+                                        composition and randint's reduction restate JAX's published
+                                        source, jax/_src/prng.py and random.py; no JAX in this image and
+                                        no golden draw in the reference); its Threefry core is pinned,
+                                        and the values are checked to lie in range and to be uniform
   image_positions                       pinned (code 0, zero shift and no drift give the unwrapped
                                         grid positions; a code permutes and flips the centred axes as
                                         cube_symmetry's matrix does; the drift moves each image by
@@ -53,6 +62,73 @@ from . import pm_oracle as orc
 
 H0_OVER_C = 100.0 / 299792.458  # h / Mpc (jax_cosmo.constants H0 = 100 km/s/(Mpc/h), c in km/s)
 FLT_EPS = float(np.finfo(np.float32).eps)
+
+
+# ---------------------------------------------------------------------------- JAX's generator
+_MASK32 = 0xFFFFFFFF
+
+
+def threefry2x32(k0, k1, c0, c1):
+    """Threefry-2x32, 20 rounds (Salmon et al. 2011, Random123), on Python ints: rounds
+    x0 += x1, x1 = rotl(x1, R) ^ x0 with R cycling (13, 15, 26, 6), (17, 29, 16, 24); the key words
+    (k0, k1, k0 ^ k1 ^ 0x1BD11BDA) injected before the first round and after every fourth, the
+    j-th injection adding j to the second word."""
+    key = [k0 & _MASK32, k1 & _MASK32]
+    key.append(key[0] ^ key[1] ^ 0x1BD11BDA)
+    rots = [13, 15, 26, 6, 17, 29, 16, 24]
+    x = [(c0 + key[0]) & _MASK32, (c1 + key[1]) & _MASK32]
+    for rnd in range(20):
+        r = rots[rnd % 8]
+        x[0] = (x[0] + x[1]) & _MASK32
+        x[1] = (((x[1] << r) | (x[1] >> (32 - r))) & _MASK32) ^ x[0]
+        if rnd % 4 == 3:
+            j = rnd // 4 + 1
+            x[0] = (x[0] + key[j % 3]) & _MASK32
+            x[1] = (x[1] + key[(j + 1) % 3] + j) & _MASK32
+    return x[0], x[1]
+
+
+def jax_key(seed):
+    """jax.random.PRNGKey(seed): (seed >> 32, low 32 bits)."""
+    seed &= (1 << 64) - 1
+    return seed >> 32, seed & _MASK32
+
+
+def jax_split_original(key, n):
+    """split(key, n) under jax_threefry_partitionable=False: threefry over the counts 0..2n-1, the
+    first half as first words and the second half as second words, the 2n outputs reshaped (n, 2)."""
+    out0, out1 = [], []
+    for i in range(n):
+        y = threefry2x32(key[0], key[1], i, n + i)
+        out0.append(y[0])
+        out1.append(y[1])
+    flat = out0 + out1
+    return [(flat[2 * i], flat[2 * i + 1]) for i in range(n)]
+
+
+def jax_split(key, n):
+    """split(key, n) under jax_threefry_partitionable=True (JAX >= 0.5): key i = threefry(key, i as
+    the 64-bit count (hi, lo))."""
+    return [threefry2x32(key[0], key[1], i >> 32, i & _MASK32) for i in range(n)]
+
+
+def jax_bits(key, n, width):
+    """n random words of 32 (y0 ^ y1) or 64 (y0 << 32 | y1) bits, partitionable scheme."""
+    out = []
+    for i in range(n):
+        y0, y1 = threefry2x32(key[0], key[1], i >> 32, i & _MASK32)
+        out.append(y0 ^ y1 if width == 32 else (y0 << 32) | y1)
+    return out
+
+
+def jax_randint(seed, n, minval, maxval, x64=False):
+    """jax.random.randint(PRNGKey(seed), (n,), minval, maxval): two draws of the dtype's width w from
+    the halves of split(key, 2) read as one 2w-bit number, reduced modulo span = maxval - minval."""
+    w = 64 if x64 else 32
+    k1, k2 = jax_split(jax_key(seed), 2)
+    hi, lo = jax_bits(k1, n, w), jax_bits(k2, n, w)
+    span = maxval - minval
+    return np.array([minval + ((h << w) + l) % span for h, l in zip(hi, lo)], dtype=np.int64)
 
 
 # ---------------------------------------------------------------------------- images

@@ -275,3 +275,40 @@ def test_native_request_plan_equals_python(pkg, oc, s8, n):
     ref = np.array([[np.nan if o.pre is None else o.pre, o.kick, np.nan if o.post is None else o.post] for o in pops],
                    np.float32)
     assert nops.value == len(pops) and np.array_equal(ops[: nops.value], ref, equal_nan=True)
+
+
+@pytest.mark.parametrize("x64", [False, True])
+def test_jax_prng_matches_oracle(pkg, x64):
+    """jax_fli_amd.prng (vectorised uint32 NumPy, JAX's multiplier form of randint's reduction) against
+    the oracle's scalar restatement (the 2w-bit number reduced directly)."""
+    from oracle import sky_oracle as so
+
+    prng = pkg.prng
+    rng = np.random.default_rng(5)
+    k = rng.integers(0, 2**32, size=(4, 2), dtype=np.uint64)
+    c = rng.integers(0, 2**32, size=(4, 2), dtype=np.uint64)
+    for kk, cc in zip(k, c):
+        y0, y1 = prng.threefry2x32((kk[0], kk[1]), np.array([cc[0]]), np.array([cc[1]]))
+        assert (int(y0[0]), int(y1[0])) == so.threefry2x32(int(kk[0]), int(kk[1]), int(cc[0]), int(cc[1]))
+    for seed in (0, 42, (5 << 32) + 9):
+        key = prng.prng_key(seed)
+        assert tuple(int(w) for w in key) == so.jax_key(seed)
+        assert [tuple(int(w) for w in r) for r in prng.split(key, 3)] == so.jax_split(so.jax_key(seed), 3)
+        for n, lo, hi in ((27, 0, 47), (125, 0, 47), (40, -3, 1000)):
+            got = prng.randint(key, n, lo, hi, x64=x64)
+            assert got.dtype == (np.int64 if x64 else np.int32)
+            assert np.array_equal(got, so.jax_randint(seed, n, lo, hi, x64=x64))
+
+
+def test_onion_tiler_rotations_are_the_reference_draw(pkg):
+    """OnionTiler's 27 symmetries: jax.random.randint(PRNGKey(42), (27,), 0, 47) into the 47
+    non-identity entries (src/jax_fli/pm/interp.py:333-336); layers=2 draws 125."""
+    from oracle import sky_oracle as so
+
+    sph = pkg.PaintingOptions(target="spherical")
+    assert np.array_equal(pkg.OnionTiler(painting=sph).init().rotations, so.jax_randint(42, 27, 0, 47) + 1)
+    assert np.array_equal(pkg.OnionTiler(painting=sph, x64=True).init().rotations,
+                          so.jax_randint(42, 27, 0, 47, x64=True) + 1)
+    assert np.array_equal(pkg.OnionTiler(painting=sph, layers=2).init().rotations,
+                          so.jax_randint(42, 125, 0, 47) + 1)
+    assert not pkg.OnionTiler(painting=sph, rotate=False).init().rotations.any()

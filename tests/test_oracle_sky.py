@@ -16,6 +16,35 @@ from oracle import pm_oracle as orc
 from oracle import sky_oracle as so
 
 
+# ---------------------------------------------------------------- JAX's generator
+def test_threefry_known_answers():
+    """Random123's known-answer vectors for Threefry-2x32 with 20 rounds (key, counter) -> output."""
+    m = 0xFFFFFFFF
+    assert so.threefry2x32(0, 0, 0, 0) == (0x6B200159, 0x99BA4EFE)
+    assert so.threefry2x32(m, m, m, m) == (0x1CB996FC, 0xBB002BE7)
+    assert so.threefry2x32(0x13198A2E, 0x03707344, 0x243F6A88, 0x85A308D3) == (0xC4923A9C, 0x483DF7A0)
+
+
+def test_jax_split_original_matches_the_documented_keys():
+    """JAX's documentation prints split(PRNGKey(0)) under the original scheme as [4146024105,
+    967050713] and [2718843009, 1272950319]: pins the key words and the counter layout."""
+    assert so.jax_key(0) == (0, 0) and so.jax_key(42) == (0, 42) and so.jax_key((3 << 32) + 5) == (3, 5)
+    assert so.jax_split_original(so.jax_key(0), 2) == [(4146024105, 967050713), (2718843009, 1272950319)]
+
+
+@pytest.mark.parametrize("x64", [False, True])
+def test_jax_randint_range_and_uniformity(x64):
+    n, span = 47 * 300, 47
+    v = so.jax_randint(7, n, 0, span, x64=x64)
+    assert v.min() >= 0 and v.max() < span
+    counts = np.bincount(v, minlength=span)
+    chi2 = float(((counts - n / span) ** 2 / (n / span)).sum())
+    assert chi2 < 46 + 6 * np.sqrt(2 * 46)  # 46 degrees of freedom, about six sigma
+    assert np.array_equal(so.jax_randint(7, 5, 10, 12, x64=x64) >= 10, np.ones(5, bool))
+    # 32- and 64-bit draws differ (different words and widths)
+    assert not np.array_equal(so.jax_randint(42, 27, 0, 47), so.jax_randint(42, 27, 0, 47, x64=True))
+
+
 # ---------------------------------------------------------------- images
 def test_cube_symmetries_are_the_48_signed_permutations():
     mats = [so.cube_symmetry(c) for c in range(48)]
