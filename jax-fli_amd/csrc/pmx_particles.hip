@@ -794,7 +794,7 @@ __global__ void k_set_scale(unsigned* __restrict__ bits, float mul, float* __res
 }
 
 // out = S^T U = -sum_d S_d U_d, S_d the 4-point stencil 8 (u[+1] - u[-1]) - (u[+2] - u[-2]) along d;
-// component d of U at U[d] + cell * ustride. One workgroup per (x, y) row, as k_grad4.
+// component d of U at U[d] + cell * ustride. One workgroup per (x, y) row.
 __global__ void __launch_bounds__(256) k_div4(const float* __restrict__ U0, const float* __restrict__ U1,
                                               const float* __restrict__ U2, int ustride, float* __restrict__ out,
                                               int NX, int NY, int NZ) {
