@@ -466,6 +466,8 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 512^3-on-1024^3 forward + VJP measurement")
     ap.add_argument("--no-config1", action="store_true",
                     help="skip the 64^3 workload (keeps profiler runs to 256^3 dispatches)")
+    ap.add_argument("--vjp-only", action="store_true",
+                    help="print only the differentiable (128^3 forward + VJP) measurement and exit")
     ap.add_argument("--large-only", action="store_true",
                     help="run only the 512^3-on-1024^3 forward + VJP timing (no differences) and print its JSON")
     ap.add_argument("--large-sweep", action="store_true",
@@ -490,6 +492,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.vjp_only:
+        print(json.dumps({"vjp": bench_vjp(pkg, dev)}))
+        return
     if args.large_only:
         print(json.dumps({"large": bench_large(pkg, dev, fd_dirs=0)}))
         return

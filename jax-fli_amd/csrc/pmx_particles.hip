@@ -774,11 +774,25 @@ __global__ void __launch_bounds__(256) k_paint_faces(SortGeom g, const float* __
 // -sign(f - 1), with sign(0) = 0.
 __device__ __forceinline__ float sgnf(float v) { return v > 0.f ? 1.f : (v < 0.f ? -1.f : 0.f); }
 
-// Largest |a[i]| into *bits.
+__device__ __forceinline__ float absmax4(float m, float4 v) {
+  return fmaxf(fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y))), fmaxf(fabsf(v.z), fabsf(v.w)));
+}
+__device__ __forceinline__ bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+// Largest |a[i]| into *bits: 16-byte loads, four in flight per thread (a scalar tail, and a scalar
+// walk for unaligned arrays).
 __global__ void __launch_bounds__(256) k_absmax(const float* __restrict__ a, size_t n, unsigned* __restrict__ bits) {
   float m = 0.f;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    m = fmaxf(m, fabsf(a[i]));
+  const size_t T = (size_t)gridDim.x * blockDim.x, tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t n4 = aligned16(a) ? n / 4 : 0;
+  const float4* a4 = reinterpret_cast<const float4*>(a);
+  size_t i = tid;
+  for (; i + 3 * T < n4; i += 4 * T) {
+    const float4 v0 = a4[i], v1 = a4[i + T], v2 = a4[i + 2 * T], v3 = a4[i + 3 * T];
+    m = absmax4(absmax4(absmax4(absmax4(m, v0), v1), v2), v3);
+  }
+  for (; i < n4; i += T) m = absmax4(m, a4[i]);
+  for (size_t j = 4 * n4 + tid; j < n; j += T) m = fmaxf(m, fabsf(a[j]));
   block_max_to(m, bits);
 }
 
@@ -814,6 +828,70 @@ __global__ void __launch_bounds__(256) k_div4(const float* __restrict__ U0, cons
     const float gz = 8.0f * (U2[(r0 + wr(z + 1, NZ)) * st] - U2[(r0 + wr(z - 1, NZ)) * st]) -
                      (U2[(r0 + wr(z + 2, NZ)) * st] - U2[(r0 + wr(z - 2, NZ)) * st]);
     out[r0 + z] = -(gx + gy + gz);
+  }
+}
+
+// k_div4 for separate component meshes (ustride 1), marching along x like the gradient stencil: a
+// workgroup takes yb consecutive rows (x, y0 .. y0 + yb - 1), tpr = NZ / 4 threads per row with 4 cells
+// each, and walks xb x planes. U0[x - 2 .. x + 2] of a thread's cells stay in registers; U1's plane
+// rows y0 - 2 .. y0 + vrows + 1 and U2's own rows sit in LDS for the y and z neighbours. The same
+// expressions as k_div4 (bitwise the same output).
+constexpr int DIV4_ROWS_FLOATS = 5120;  // (yb + 4) * NZ with yb = 256 / tpr, for every NZ up to 1024
+__global__ void __launch_bounds__(256) k_div4m(const float* __restrict__ U0, const float* __restrict__ U1,
+                                               const float* __restrict__ U2, float* __restrict__ out, int NX, int NY,
+                                               int NZ, int tpr, int yb, int xb, int nyb) {
+  __shared__ float4 p1[DIV4_ROWS_FLOATS / 4];
+  __shared__ float4 p2[1024 / 4];  // yb * NZ = 4 * tpr * yb <= 1024
+  const float* r1 = reinterpret_cast<const float*>(p1);
+  const float* r2 = reinterpret_cast<const float*>(p2);
+  const unsigned Lb = xcd_logical(blockIdx.x, gridDim.x);
+  const int by = (int)(Lb % (unsigned)nyb), bx = (int)(Lb / (unsigned)nyb);
+  const int me = (int)threadIdx.x, nt = (int)blockDim.x;
+  const int lr = me / tpr, t = me - lr * tpr, z0 = 4 * t;
+  const int y0 = by * yb, x0 = bx * xb;
+  const int vrows = min(yb, NY - y0);
+  const bool valid = lr < vrows;
+  const int y = valid ? y0 + lr : y0;
+  const int xend = min(x0 + xb, NX);
+  auto wr = [](int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); };
+  auto ld = [&](const float* U, int xx, int yy) {
+    return *reinterpret_cast<const float4*>(U + ((size_t)xx * NY + yy) * NZ + z0);
+  };
+  float4 w0 = ld(U0, wr(x0 - 2, NX), y), w1 = ld(U0, wr(x0 - 1, NX), y), w2 = ld(U0, x0, y),
+         w3 = ld(U0, wr(x0 + 1, NX), y);
+  const int hq = 4 * tpr;
+  for (int x = x0; x < xend; ++x) {
+    const float4 w4 = ld(U0, wr(x + 2, NX), y);
+    if (valid) {
+      p1[((lr + 2) * NZ + z0) / 4] = ld(U1, x, y);
+      p2[(lr * NZ + z0) / 4] = ld(U2, x, y);
+    }
+    for (int q = me; q < hq; q += nt) {  // U1 halo rows y0 - 2, y0 - 1, y0 + vrows, y0 + vrows + 1
+      const int h = q / tpr, c = q - h * tpr;
+      const int ly = h < 2 ? h : vrows + h;
+      const int yy = wr(y0 - 2 + ly, NY);
+      p1[(ly * NZ) / 4 + c] = *reinterpret_cast<const float4*>(U1 + ((size_t)x * NY + yy) * NZ + 4 * c);
+    }
+    __syncthreads();
+    const float* a1 = r1 + (lr + 2) * NZ;
+    const float* a2 = r2 + lr * NZ;
+    float o[4];
+    const float ax1[4] = {w3.x, w3.y, w3.z, w3.w}, bx1[4] = {w1.x, w1.y, w1.z, w1.w};
+    const float ax2[4] = {w4.x, w4.y, w4.z, w4.w}, bx2[4] = {w0.x, w0.y, w0.z, w0.w};
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int z = z0 + m;
+      const float gx = 8.0f * (ax1[m] - bx1[m]) - (ax2[m] - bx2[m]);
+      const float gy = 8.0f * (a1[z + NZ] - a1[z - NZ]) - (a1[z + 2 * NZ] - a1[z - 2 * NZ]);
+      const float gz = 8.0f * (a2[wr(z + 1, NZ)] - a2[wr(z - 1, NZ)]) - (a2[wr(z + 2, NZ)] - a2[wr(z - 2, NZ)]);
+      o[m] = -(gx + gy + gz);
+    }
+    if (valid) *reinterpret_cast<float4*>(out + ((size_t)x * NY + y) * NZ + z0) = make_float4(o[0], o[1], o[2], o[3]);
+    __syncthreads();  // p1 and p2 are rewritten for the next plane
+    w0 = w1;
+    w1 = w2;
+    w2 = w3;
+    w3 = w4;
   }
 }
 
@@ -935,11 +1013,33 @@ __global__ void __launch_bounds__(256) k_adj_prep(const float* __restrict__ xbar
                                                   double* __restrict__ part, unsigned* __restrict__ bits) {
   double local = 0.0;
   float m = 0.f;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const float xb = xbar[i];
-    local += (double)xb * (double)v[i];
-    const float vb = vbar[i] + cd * xb;
-    vbar[i] = vb;
+  const size_t T = (size_t)gridDim.x * blockDim.x, tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  // 16-byte lanes, two in flight per array and thread, when all three arrays allow it; scalar tail
+  const size_t n4 = (aligned16(xbar) && aligned16(vbar) && aligned16(v)) ? n / 4 : 0;
+  const float4* x4 = reinterpret_cast<const float4*>(xbar);
+  const float4* v4 = reinterpret_cast<const float4*>(v);
+  float4* b4 = reinterpret_cast<float4*>(vbar);
+  auto one4 = [&](const float4 xb, const float4 vv, const float4 bb) {
+    local += (double)xb.x * (double)vv.x;
+    local += (double)xb.y * (double)vv.y;
+    local += (double)xb.z * (double)vv.z;
+    local += (double)xb.w * (double)vv.w;
+    const float4 nb = make_float4(bb.x + cd * xb.x, bb.y + cd * xb.y, bb.z + cd * xb.z, bb.w + cd * xb.w);
+    m = absmax4(m, nb);
+    return nb;
+  };
+  size_t i = tid;
+  for (; i + T < n4; i += 2 * T) {
+    const float4 xa = x4[i], xc = x4[i + T], va = v4[i], vc = v4[i + T], ba = b4[i], bc = b4[i + T];
+    b4[i] = one4(xa, va, ba);
+    b4[i + T] = one4(xc, vc, bc);
+  }
+  for (; i < n4; i += T) b4[i] = one4(x4[i], v4[i], b4[i]);
+  for (size_t j = 4 * n4 + tid; j < n; j += T) {
+    const float xb = xbar[j];
+    local += (double)xb * (double)v[j];
+    const float vb = vbar[j] + cd * xb;
+    vbar[j] = vb;
     m = fmaxf(m, fabsf(vb));
   }
   if (bits) block_max_to(m, bits);
@@ -1198,6 +1298,146 @@ __global__ void __launch_bounds__(256) k_readout_lds(PartArgs a, const float* __
       st3(a.x, p, axpy3(x, make_float3(f[0], f[1], f[2]), ck));
     }
   }
+}
+
+// k_vjp_gather on the readout's Lagrangian blocks (RB_I x RB_J x RB_K particles, one per thread): the
+// block's corner box of M (NC floats per cell) and rhob (one) is staged in LDS when it holds at most
+// RO_CAP cells, else the block gathers from global memory. Same weights, corner order and sums as
+// k_vjp_gather: bitwise the same results. Blocks are walked grid-stride so that part[] keeps one
+// partial per workgroup (at most kRedMax).
+template <int NC, bool HAS_U, bool HAS_RHO, bool WBAR>
+__global__ void __launch_bounds__(256) k_vjp_gather_lds(PartArgs a, const float* __restrict__ u, float umul,
+                                                        const float* __restrict__ M, const float* __restrict__ rhob,
+                                                        const float* __restrict__ w, float w0,
+                                                        float* __restrict__ xbar, int accumulate,
+                                                        float* __restrict__ wbar, double* __restrict__ part,
+                                                        float* __restrict__ vkick, float ckick, int NX, int NY,
+                                                        int NZ, int rx, int ry, int rz, int nbj, int nbk,
+                                                        unsigned nblocks, FDiv dj, FDiv dk) {
+  constexpr int S = (HAS_U ? NC : 0) + (HAS_RHO ? 1 : 0);  // floats staged per cell: M components, then rhob
+  __shared__ float sm[RO_CAP * S];
+  __shared__ int red[6][4];
+  const int t = threadIdx.x, wv = t >> 6;
+  double local = 0.0;
+  for (unsigned b = xcd_logical(blockIdx.x, gridDim.x); b < nblocks; b += gridDim.x) {
+    const unsigned bq = fdiv(b, dk), bi_ = fdiv(bq, dj);
+    const int bk = (int)(b - bq * (unsigned)nbk), bj = (int)(bq - bi_ * (unsigned)nbj), bi = (int)bi_;
+    const int i = bi * RB_I + t / (RB_J * RB_K), j = bj * RB_J + (t / RB_K) % RB_J, k = bk * RB_K + t % RB_K;
+    const unsigned p = ((unsigned)i * a.npy + j) * a.npz + k;
+    const float3 x = ld3(a.x, p);
+    const Axis1D ax = axis_of(a.mode, i * rx, x.x), ay = axis_of(a.mode, j * ry, x.y), az = axis_of(a.mode, k * rz, x.z);
+    {
+      const int r0 = wave_minmax<false>(ax.b), r1 = wave_minmax<false>(ay.b), r2 = wave_minmax<false>(az.b);
+      const int r3 = wave_minmax<true>(ax.b), r4 = wave_minmax<true>(ay.b), r5 = wave_minmax<true>(az.b);
+      if ((t & 63) == 0) {
+        red[0][wv] = r0;
+        red[1][wv] = r1;
+        red[2][wv] = r2;
+        red[3][wv] = r3;
+        red[4][wv] = r4;
+        red[5][wv] = r5;
+      }
+    }
+    __syncthreads();
+    int mn[3], mx[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      mn[c] = __builtin_amdgcn_readfirstlane(min(min(red[c][0], red[c][1]), min(red[c][2], red[c][3])));
+      mx[c] = __builtin_amdgcn_readfirstlane(
+                  max(max(red[3 + c][0], red[3 + c][1]), max(red[3 + c][2], red[3 + c][3]))) + 1;
+    }
+    const unsigned ex = (unsigned)(mx[0] - mn[0] + 1), ey = (unsigned)(mx[1] - mn[1] + 1),
+                   ez = (unsigned)(mx[2] - mn[2] + 1);
+    const bool fits = ex <= (unsigned)RO_CAP && ey <= (unsigned)RO_CAP && ez <= (unsigned)RO_CAP &&
+                      ex * ey <= (unsigned)RO_CAP && ex * ey * ez <= (unsigned)RO_CAP;
+    const int EY = (int)ey, EZ = (int)ez;
+    if (fits) {
+      const int ncell = (int)(ex * ey * ez);
+      const float iy = __builtin_amdgcn_rcpf((float)EY), iz = __builtin_amdgcn_rcpf((float)EZ);
+      const int gx0 = wrap_cell(mn[0], NX), gy0 = wrap_cell(mn[1], NY), gz0 = wrap_cell(mn[2], NZ);
+      const bool exact = !(ex <= (unsigned)NX && ey <= (unsigned)NY && ez <= (unsigned)NZ);
+      for (int q = t; q < ncell; q += 256) {
+        int qz, qy;
+        const int qx = div_small(div_small(q, iz, EZ, qz), iy, EY, qy);
+        int gx = gx0 + qx, gy = gy0 + qy, gz = gz0 + qz;
+        if (!exact) {
+          gx -= gx >= NX ? NX : 0;
+          gy -= gy >= NY ? NY : 0;
+          gz -= gz >= NZ ? NZ : 0;
+        } else {
+          gx %= NX;
+          gy %= NY;
+          gz %= NZ;
+        }
+        const size_t cell = ((size_t)gx * NY + gy) * NZ + gz;
+        if constexpr (HAS_U) {
+#pragma unroll
+          for (int d = 0; d < NC; ++d) sm[q * S + d] = M[cell * NC + d];
+        }
+        if constexpr (HAS_RHO) sm[q * S + S - 1] = rhob[cell];
+      }
+      __syncthreads();
+    }
+    const int lx = ax.b - mn[0], ly = ay.b - mn[1], lz = az.b - mn[2];
+    const int x0 = wrap_idx(ax.b, NX), x1 = wrap_idx(ax.b + 1, NX), y0 = wrap_idx(ay.b, NY),
+              y1 = wrap_idx(ay.b + 1, NY), z0 = wrap_idx(az.b, NZ), z1 = wrap_idx(az.b + 1, NZ);
+    const unsigned nyz = (unsigned)NY * NZ;
+    const float fx = x.x - floorf(x.x), fy = x.y - floorf(x.y), fz = x.z - floorf(x.z);
+    const float dwx[2] = {-sgnf(fx), -sgnf(fx - 1.0f)}, dwy[2] = {-sgnf(fy), -sgnf(fy - 1.0f)},
+                dwz[2] = {-sgnf(fz), -sgnf(fz - 1.0f)};
+    const float wxv[2] = {ax.w0, ax.w1}, wyv[2] = {ay.w0, ay.w1}, wzv[2] = {az.w0, az.w1};
+    float ur[NC > 0 ? NC : 1] = {}, uu[NC > 0 ? NC : 1] = {};
+    if (HAS_U) {
+#pragma unroll
+      for (int d = 0; d < NC; ++d) {
+        ur[d] = u[(size_t)p * NC + d];
+        uu[d] = ur[d] * umul;
+      }
+    }
+    const float wt = HAS_RHO ? (w ? w[p] : w0) : 0.f;
+    float gxs = 0.f, gys = 0.f, gzs = 0.f, ws = 0.f, Fd = 0.f;
+    float Fv[NC > 0 ? NC : 1] = {};
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+      const float wgt = ((ox ? ax.w1 : ax.w0) * (oy ? ay.w1 : ay.w0)) * (oz ? az.w1 : az.w0);
+      const int lc = ((lx + ox) * EY + (ly + oy)) * EZ + (lz + oz);
+      const size_t gc = (size_t)(ox ? x1 : x0) * nyz + (unsigned)(oy ? y1 : y0) * NZ + (unsigned)(oz ? z1 : z0);
+      float sv = 0.f;
+      if (HAS_U) {
+        float um = 0.f, uf = 0.f;
+#pragma unroll
+        for (int d = 0; d < NC; ++d) {
+          const float m = fits ? sm[lc * S + d] : M[gc * NC + d];
+          um += uu[d] * m;
+          uf += ur[d] * m;
+          Fv[d] = Fv[d] + m * wgt;
+        }
+        sv += um;
+        Fd += wgt * uf;
+      }
+      if (HAS_RHO) {
+        const float r = fits ? sm[lc * S + S - 1] : rhob[gc];
+        sv += wt * r;
+        if (WBAR) ws += wgt * r;
+      }
+      gxs += sv * ((dwx[ox] * wyv[oy]) * wzv[oz]);
+      gys += sv * ((wxv[ox] * dwy[oy]) * wzv[oz]);
+      gzs += sv * ((wxv[ox] * wyv[oy]) * dwz[oz]);
+    }
+    float3 g = make_float3(gxs, gys, gzs);
+    if (accumulate) {
+      const float3 o = ld3(xbar, p);
+      g = make_float3(o.x + g.x, o.y + g.y, o.z + g.z);
+    }
+    st3(xbar, p, g);
+    if (HAS_U && NC == 3 && vkick != nullptr)  // v += F ck, F summed in the readout's corner order
+      st3(vkick, p, axpy3(ld3(vkick, p), make_float3(Fv[0], Fv[1], Fv[2 % NC]), ckick));
+    if (WBAR) wbar[p] = ws;
+    if (HAS_U) local += (double)Fd;
+    __syncthreads();  // red and sm are rewritten by the next block
+  }
+  if (HAS_U && part != nullptr) block_sum_store(local, part);
 }
 
 __global__ void k_finalize_map(const float* __restrict__ acc, float* __restrict__ map, long n, double scale,
@@ -1760,6 +2000,18 @@ int pmx_launch_set_scale(unsigned* bits, float mul, float* scale, hipStream_t s)
 }
 int pmx_launch_div4(pmx_plan_t* p, const float* U0, const float* U1, const float* U2, int ustride, float* out,
                     hipStream_t s) {
+  const bool vec = ustride == 1 && p->nz % 4 == 0 && p->nz <= 1024 &&
+                   ((reinterpret_cast<uintptr_t>(U0) | reinterpret_cast<uintptr_t>(U1) | reinterpret_cast<uintptr_t>(U2) |
+                     reinterpret_cast<uintptr_t>(out)) & 15u) == 0;
+  if (vec) {
+    const int tpr = p->nz / 4, yb = 256 / tpr;
+    const int nyb = (p->ny + yb - 1) / yb;
+    const int xb = p->nx < 4 ? p->nx : 4;
+    const int nxb = (p->nx + xb - 1) / xb;
+    hipLaunchKernelGGL(k_div4m, dim3((unsigned)(nxb * nyb)), dim3(tpr * yb), 0, s, U0, U1, U2, out, p->nx, p->ny,
+                       p->nz, tpr, yb, xb, nyb);
+    return pmx_check_launch("div4");
+  }
   const unsigned rows = (unsigned)p->nx * p->ny;
   const int threads = p->nz < 256 ? p->nz : 256;
   hipLaunchKernelGGL(k_div4, dim3(rows), dim3(threads), 0, s, U0, U1, U2, ustride, out, p->nx, p->ny, p->nz);
@@ -1771,11 +2023,25 @@ int pmx_launch_vjp_gather(pmx_plan_t* p, const PartArgs& a, int ncomp, const flo
   int rx, ry, rz;
   ratios(p, a, rx, ry, rz);
   const size_t np = (size_t)a.npx * a.npy * a.npz;
-  const dim3 G(grid_particles(np)), B(256);
   double* part = dot ? p->red : nullptr;
+  // the readout's staged form when its blocks tile the particle grid and a block's box is small
+  const bool staged = a.npx % RB_I == 0 && a.npy % RB_J == 0 && a.npz % RB_K == 0 &&
+                      (RB_I * rx + 2) * (RB_J * ry + 2) * (RB_K * rz + 2) * 2 <= RO_CAP;
+  const unsigned nblocks = (unsigned)(np / (RB_I * RB_J * RB_K));
+  const int nbj = a.npy / RB_J, nbk = a.npz / RB_K;
+  const FDiv dj = make_fdiv((unsigned)(nbj > 0 ? nbj : 1)), dk = make_fdiv((unsigned)(nbk > 0 ? nbk : 1));
+  const dim3 G(staged ? (nblocks < (unsigned)kRedMax ? nblocks : (unsigned)kRedMax) : (unsigned)grid_particles(np)),
+      B(256);
 #define PMX_G(NC, HU, HR, WB)                                                                                         \
-  hipLaunchKernelGGL((k_vjp_gather<NC, HU, HR, WB>), G, B, 0, s, a, u, umul, M, rhob, w, w0, xbar, accumulate, wbar, \
-                     part, vkick, ckick, p->nx, p->ny, p->nz, rx, ry, rz)
+  do {                                                                                                                \
+    if (staged)                                                                                                       \
+      hipLaunchKernelGGL((k_vjp_gather_lds<NC, HU, HR, WB>), G, B, 0, s, a, u, umul, M, rhob, w, w0, xbar,          \
+                         accumulate, wbar, part, vkick, ckick, p->nx, p->ny, p->nz, rx, ry, rz, nbj, nbk, nblocks, dj, \
+                         dk);                                                                                         \
+    else                                                                                                              \
+      hipLaunchKernelGGL((k_vjp_gather<NC, HU, HR, WB>), G, B, 0, s, a, u, umul, M, rhob, w, w0, xbar, accumulate,  \
+                         wbar, part, vkick, ckick, p->nx, p->ny, p->nz, rx, ry, rz);                                  \
+  } while (0)
   if (u && rhob) {
     if (ncomp == 3) PMX_G(3, true, true, false);
     else PMX_G(1, true, true, false);

@@ -115,6 +115,29 @@ def test_forces_vjp_directional(pkg, cuda, mode):
     assert _rel(float(dot.cpu()), ref_dot) < 1e-5
 
 
+def test_forces_vjp_staged_and_global_blocks(pkg, cuda):
+    """The adjoint gather on the readout's 4x8x8 particle blocks: a smooth displacement field (blocks
+    whose corner box is staged in LDS) with one slab of widely scattered particles (blocks that gather
+    from global memory), against the float64 oracle's forces VJP and dot."""
+    n = 32
+    mesh = (n, n, n)
+    rng = np.random.default_rng(43)
+    ii = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).astype(np.float64)
+    dx = 1.5 * np.sin(2 * np.pi * ii[..., ::-1] / n + np.arange(3)) + 0.2 * rng.standard_normal(ii.shape)
+    dx[8:12] = rng.standard_normal((4, n, n, 3)) * 7.0
+    pos = dx.astype(np.float32)
+    fbar = rng.standard_normal(pos.shape).astype(np.float32)
+    dot = torch.zeros((), dtype=torch.float64, device=cuda)
+    pb = pkg.engine.forces_vjp(torch.from_numpy(pos).to(cuda), mesh, torch.from_numpy(fbar).to(cuda),
+                               mode="relative", scale=0.45, dot=dot)
+    ref = orc.pm_forces_vjp(pos, mesh, fbar, "relative", scale=0.45)
+    err = _rel_l2(pb.cpu().numpy(), ref)
+    print(f"forces vjp staged + global blocks: rel L2 {err:.2e}")
+    assert err < 1e-4
+    ref_dot = float((orc.pm_forces(pos, mesh, "relative", scale=0.45, dtype=np.float64) * fbar).sum())
+    assert _rel(float(dot.cpu()), ref_dot) < 1e-5
+
+
 @pytest.mark.parametrize("saved", [False, True])
 def test_forces_vjp_bar_scale_and_accumulate(pkg, cuda, saved):
     """bar_scale folds a kick coefficient into the pull-back (pos_bar += c J^T u) and leaves the dot
