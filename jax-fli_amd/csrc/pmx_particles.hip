@@ -646,8 +646,10 @@ __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, c
   char* lds = reinterpret_cast<char*>(tile);
   const int ncell = BX * BY * BZ;
   const int nface = PY * PZ + BX * PZ + BX * BY;
-  // one component after the other through the same LDS tile (the entries stay in L1/L2)
-  for (int comp = 0; comp < ncomp; ++comp) {
+  // one component per workgroup (blockIdx.y) when the grid has a y extent, else one after the
+  // other through the same LDS tile (the entries stay in L1/L2)
+  const int c_lo = gridDim.y > 1 ? (int)blockIdx.y : 0, c_hi = gridDim.y > 1 ? c_lo + 1 : ncomp;
+  for (int comp = c_lo; comp < c_hi; ++comp) {
     for (int i = threadIdx.x; i < npad; i += blockDim.x) tile[i] = 0ull;
     __syncthreads();
     constexpr int U = 4;
@@ -1830,12 +1832,16 @@ int pmx_paint_deposit(pmx_plan_t* p, const PartArgs& a_in, const float* w, int w
   const int face_stride = pmx_paint_face_cells(p);
   const size_t fcomp = (size_t)ntiles * face_stride;
   const unsigned S = (unsigned)p->nslots;
+  // components in parallel workgroups when the tiles alone are too few to fill the GPU (128^3: 256
+  // tiles; one workgroup per component cuts the 3-component deposit 69 -> 53 us); with enough tiles,
+  // one workgroup walks the components so their reads of the tile's particles stay in cache
+  const unsigned gy = (ncomp > 1 && (size_t)ntiles * S < 1024) ? (unsigned)ncomp : 1u;
   if (is_signed)
-    hipLaunchKernelGGL(k_paint_deposit<true>, dim3(ntiles, 1, S), dim3(512), 0, s, a, g, w, wstride, wmul, w0, ncomp,
+    hipLaunchKernelGGL(k_paint_deposit<true>, dim3(ntiles, gy, S), dim3(512), 0, s, a, g, w, wstride, wmul, w0, ncomp,
                        p->ps_tcount, p->ps_offset, p->ps_seg, p->ps_scale, mesh, mstride, mcomp, p->ps_faces,
                        face_stride, fcomp, accumulate, p->slots);
   else
-    hipLaunchKernelGGL(k_paint_deposit<false>, dim3(ntiles, 1, S), dim3(512), 0, s, a, g, w, wstride, wmul, w0, ncomp,
+    hipLaunchKernelGGL(k_paint_deposit<false>, dim3(ntiles, gy, S), dim3(512), 0, s, a, g, w, wstride, wmul, w0, ncomp,
                        p->ps_tcount, p->ps_offset, p->ps_seg, p->ps_scale, mesh, mstride, mcomp, p->ps_faces,
                        face_stride, fcomp, accumulate, p->slots);
   int rc = pmx_check_launch("paint_deposit");
