@@ -553,6 +553,10 @@ __device__ __forceinline__ double phi_of(const double Xt[3]) {
   return (phi < 2.0 * M_PI) ? phi : 0.0;  // -tiny + 2 pi can round up to 2 pi
 }
 
+// One instance per scheme (0 ngp, 1 bilinear, 2 rbf_neighbor): the rbf scheme's nine pixels per
+// image would otherwise set the register allocation of every scheme (256 VGPRs, one wave per SIMD,
+// against 155 and two for bilinear alone).
+template <int SCHEME>
 __global__ void __launch_bounds__(256) k_paint_healpix(PartArgs a, const float* __restrict__ w, float w0, ShellArgs sa,
                                                        HpGeom g, double* __restrict__ acc) {
   __shared__ unsigned keys[HP_HASH];
@@ -589,9 +593,9 @@ __global__ void __launch_bounds__(256) k_paint_healpix(PartArgs a, const float* 
       }
       if (!in_shell(sa, Xt, rr)) continue;
       const double phi = phi_of(Xt);
-      if (sa.scheme == 0) {
+      if constexpr (SCHEME == 0) {
         hp_table_add(keys, vals, (unsigned)hp_ang2pix(g, Xt[2] / rr, phi), wt, acc, &full);
-      } else if (sa.scheme == 2) {
+      } else if constexpr (SCHEME == 2) {
         const double n[3] = {Xt[0] / rr, Xt[1] / rr, Xt[2] / rr};
         long pix[9];
         double wg[9];
@@ -1418,8 +1422,13 @@ int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w
   double* acc = p->acc;
   hipLaunchKernelGGL(k_zero_d, dim3(grid_for((size_t)g.npix, 256)), dim3(256), 0, s, acc, g.npix);
   PMX_CHECK_LAUNCH("healpix_zero");
-  hipLaunchKernelGGL(k_paint_healpix, dim3((unsigned)sa.nbx * sa.nby * sa.nbz), dim3(256), 0, s, a, w, w0, sa, g,
-                     acc);
+  const dim3 grid((unsigned)sa.nbx * sa.nby * sa.nbz);
+  if (sa.scheme == 0)
+    hipLaunchKernelGGL(k_paint_healpix<0>, grid, dim3(256), 0, s, a, w, w0, sa, g, acc);
+  else if (sa.scheme == 2)
+    hipLaunchKernelGGL(k_paint_healpix<2>, grid, dim3(256), 0, s, a, w, w0, sa, g, acc);
+  else
+    hipLaunchKernelGGL(k_paint_healpix<1>, grid, dim3(256), 0, s, a, w, w0, sa, g, acc);
   PMX_CHECK_LAUNCH("paint_healpix");
   const double vol = (4.0 * M_PI / (double)g.npix) * (rmax * rmax * rmax - rmin * rmin * rmin) / 3.0;
   hipLaunchKernelGGL(k_finalize_map_d, dim3(grid_for((size_t)g.npix, 256)), dim3(256), 0, s, acc, map, g.npix,
