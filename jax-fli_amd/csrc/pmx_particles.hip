@@ -619,7 +619,7 @@ __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, c
                                                        const float* __restrict__ scale, float* __restrict__ mesh,
                                                        int mstride, size_t mcomp, float* __restrict__ faces,
                                                        int face_stride, size_t fcomp, int accumulate, Slots sl) {
-  __shared__ unsigned long long tile[TILE_PAD_MAX];
+  __shared__ __align__(16) unsigned long long tile[TILE_PAD_MAX];
   const unsigned zs = blockIdx.z;
   if (!slot_live(sl, zs)) return;
   a.x = at_slot(a.x, sl.stride, zs);
@@ -650,7 +650,11 @@ __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, c
   // other through the same LDS tile (the entries stay in L1/L2)
   const int c_lo = gridDim.y > 1 ? (int)blockIdx.y : 0, c_hi = gridDim.y > 1 ? c_lo + 1 : ncomp;
   for (int comp = c_lo; comp < c_hi; ++comp) {
-    for (int i = threadIdx.x; i < npad; i += blockDim.x) tile[i] = 0ull;
+    {  // zero the padded tile with 16-byte stores
+      ulonglong2* t2 = reinterpret_cast<ulonglong2*>(tile);
+      for (int i = threadIdx.x; i < npad / 2; i += blockDim.x) t2[i] = make_ulonglong2(0ull, 0ull);
+      if ((npad & 1) && threadIdx.x == 0) tile[npad - 1] = 0ull;
+    }
     __syncthreads();
     constexpr int U = 4;
     for (unsigned e0 = threadIdx.x; e0 < n; e0 += U * blockDim.x) {
@@ -688,11 +692,28 @@ __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, c
     }
     __syncthreads();
     float* mc = mesh + comp * mcomp;
-    for (int q = threadIdx.x; q < ncell; q += blockDim.x) {
-      const int iz = q & (BZ - 1), iy = (q >> g.sz) & (BY - 1), ix = q >> (g.sz + g.sy);
-      float* dst = mc + (((size_t)(T0x + ix) * g.NY + (T0y + iy)) * g.NZ + (T0z + iz)) * mstride;
-      const float v = (float)((double)(long long)tile[(ix * PY + iy) * PZ + iz] * inv);
-      *dst = accumulate ? *dst + v : v;
+    if (mstride == 1 && BZ >= 4 && (g.NZ & 3) == 0 && (reinterpret_cast<uintptr_t>(mc) & 15u) == 0) {
+      // four consecutive z cells per thread: one 16-byte store (105 -> 98 us at 256^3 with the zeroing)
+      for (int q = threadIdx.x; 4 * q < ncell; q += blockDim.x) {
+        const int q4 = 4 * q;
+        const int iz = q4 & (BZ - 1), iy = (q4 >> g.sz) & (BY - 1), ix = q4 >> (g.sz + g.sy);
+        float4* dst = reinterpret_cast<float4*>(mc + ((size_t)(T0x + ix) * g.NY + (T0y + iy)) * g.NZ + (T0z + iz));
+        const unsigned long long* t = tile + (ix * PY + iy) * PZ + iz;
+        float4 v = make_float4((float)((double)(long long)t[0] * inv), (float)((double)(long long)t[1] * inv),
+                               (float)((double)(long long)t[2] * inv), (float)((double)(long long)t[3] * inv));
+        if (accumulate) {
+          const float4 o = *dst;
+          v = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
+        }
+        *dst = v;
+      }
+    } else {
+      for (int q = threadIdx.x; q < ncell; q += blockDim.x) {
+        const int iz = q & (BZ - 1), iy = (q >> g.sz) & (BY - 1), ix = q >> (g.sz + g.sy);
+        float* dst = mc + (((size_t)(T0x + ix) * g.NY + (T0y + iy)) * g.NZ + (T0z + iz)) * mstride;
+        const float v = (float)((double)(long long)tile[(ix * PY + iy) * PZ + iz] * inv);
+        *dst = accumulate ? *dst + v : v;
+      }
     }
     float* f = faces + comp * fcomp + (size_t)L * face_stride;
     const float ipz = __builtin_amdgcn_rcpf((float)PZ);
