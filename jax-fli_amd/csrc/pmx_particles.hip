@@ -322,25 +322,6 @@ __device__ __forceinline__ unsigned slot_tile(const SortGeom& g, int3 c, int s) 
   return ((unsigned)tx * g.nty + ty) * g.ntz + tz;
 }
 
-// Wave-aggregated counter add: lanes with `on` add 1 to cnt[key]; each gets the counter's old value
-// plus its rank among the lanes sharing its key (one LDS atomic per distinct key in the wave).
-__device__ __forceinline__ unsigned wave_add(bool on, int key, unsigned* cnt) {
-  const int lane = __lane_id();
-  unsigned long long pending = __ballot(on);
-  unsigned res = 0;
-  while (pending) {
-    const int leader = __ffsll((long long)pending) - 1;
-    const int lk = __builtin_amdgcn_readlane(key, leader);
-    const unsigned long long mask = __ballot(on && key == lk) & pending;
-    unsigned old = 0;
-    if (lane == leader) old = atomicAdd(&cnt[lk], (unsigned)__popcll(mask));
-    old = __builtin_amdgcn_readlane(old, leader);
-    if ((mask >> lane) & 1ull) res = old + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
-    pending &= ~mask;
-  }
-  return res;
-}
-
 // Visit the particles of this thread in sub-block (i0, j0, k0) (4 loads in flight):
 // fn(valid, p, tile, slot) with slot = window slot of the tile or -1; called by every lane, so fn
 // may use wave-wide operations.
@@ -417,7 +398,9 @@ __device__ __forceinline__ void count_relative(const PartArgs& a, const SortGeom
       const bool inw = (unsigned)dx < (unsigned)SW_X && (unsigned)dy < (unsigned)SW_Y && (unsigned)dz < (unsigned)SW_Z;
       const int slot = (dx * SW_Y + dy) * SW_Z + dz;
       const bool on = vs[u];
-      const unsigned rk = wave_add(on && inw, slot, lcnt);  // rank within the slot's segment
+      // rank within the slot's segment: one LDS atomic per lane (lanes of a wave that share a slot
+      // are serialised by the LDS unit, which costs less than aggregating them per wave)
+      const unsigned rk = (on && inw) ? atomicAdd(&lcnt[slot], 1u) : 0u;
       if (on) pack[ps[u]] = inw ? ((unsigned)slot << 16) | rk : SORT_FAR;
       if (on && !inw) {
         const Base bb = base_of(a, g, is[u], js[u], k, xs[u]);
@@ -471,7 +454,7 @@ __global__ void __launch_bounds__(256) k_sort_count(PartArgs a, SortGeom g, floa
     count_relative(a, g, i0, j0, k0, c, lcnt, count, pack, nfar);
   } else {
     for_particles(a, g, i0, j0, k0, c, [&](bool on, unsigned p, unsigned tile, int slot) {
-      const unsigned r = wave_add(on && slot >= 0, slot, lcnt);  // rank within the slot's segment
+      const unsigned r = (on && slot >= 0) ? atomicAdd(&lcnt[slot], 1u) : 0u;  // rank within the slot's segment
       if (on) pack[p] = slot >= 0 ? ((unsigned)slot << 16) | r : SORT_FAR;
       if (on && slot < 0) {
         atomicAdd(count + tile, 1u);
