@@ -181,16 +181,19 @@ __device__ __forceinline__ void hp_ring_info(const HpGeom& g, long ring, long& s
   theta = g.rth[ring];
 }
 
-// pixels on one ring around phi and the phi weight of the second one; dphi = ring pixel width
+// pixels on one ring around phi and the phi weight of the second one; idphi = 1 / ring pixel width.
+// In units of the pixel width, t = phi / width - shift / 2 puts pixel i1 = floor(t) at t = i1 and the
+// weight of the second pixel is t - i1; both come from one product instead of three divisions
+// (the last-bit differences only move t across a pixel edge, where the two weights are 0 and 1).
 __device__ __forceinline__ void hp_ring_pair(const HpGeom& g, long ring, double phi, long& p1, long& p2, double& w1,
-                                             double& theta, double& dphi) {
+                                             double& theta, double& idphi) {
   long sp, nr;
   bool shift;
   hp_ring_info(g, ring, sp, nr, theta, shift);
-  dphi = 2.0 * M_PI / nr;
-  const double tmp = phi / dphi - 0.5 * shift;
+  idphi = (double)nr * (0.5 / M_PI);
+  const double tmp = phi * idphi - 0.5 * shift;
   long i1 = (tmp < 0) ? (long)tmp - 1 : (long)tmp;
-  w1 = (phi - (i1 + 0.5 * shift) * dphi) / dphi;
+  w1 = tmp - (double)i1;
   long i2 = i1 + 1;
   if (i1 < 0) i1 += nr;
   if (i2 >= nr) i2 -= nr;
@@ -207,24 +210,24 @@ __device__ __forceinline__ void hp_interpol(const HpGeom& g, double theta, doubl
   const long ir1 = hp_ring_above(g, z);
   const long ir2 = ir1 + 1;
   const long nring = 4 * (long)g.nside;
-  double theta1 = 0, theta2 = 0, dphi;
+  double theta1 = 0, theta2 = 0, idphi;
   double w1;
   if (ir1 > 0) {
-    hp_ring_pair(g, ir1, phi, pix[0], pix[1], w1, theta1, dphi);
+    hp_ring_pair(g, ir1, phi, pix[0], pix[1], w1, theta1, idphi);
     wgt[0] = 1 - w1;
     wgt[1] = w1;
     if (DER) {
-      dph[0] = -1.0 / dphi;
-      dph[1] = 1.0 / dphi;
+      dph[0] = -idphi;
+      dph[1] = idphi;
     }
   }
   if (ir2 < nring) {
-    hp_ring_pair(g, ir2, phi, pix[2], pix[3], w1, theta2, dphi);
+    hp_ring_pair(g, ir2, phi, pix[2], pix[3], w1, theta2, idphi);
     wgt[2] = 1 - w1;
     wgt[3] = w1;
     if (DER) {
-      dph[2] = -1.0 / dphi;
-      dph[3] = 1.0 / dphi;
+      dph[2] = -idphi;
+      dph[3] = idphi;
     }
   }
   if (ir1 == 0) {  // north polar cap: the missing ring is the 4 pixels around the pole
