@@ -650,6 +650,11 @@ __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, c
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) xs[u] = (e0 + u * blockDim.x < n) ? ld3(a.x, ps[u]) : make_float3(0.f, 0.f, 0.f);
+      // the particle weights in flight with the positions (loaded inside the corner loop they cost one
+      // memory round trip per particle: the 3-component adjoint deposit 49 -> 45 us at 128^3)
+      float wv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) wv[u] = (w && e0 + u * blockDim.x < n) ? w[(size_t)ps[u] * wstride + comp] : 0.f;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (e0 + u * blockDim.x >= n) continue;
@@ -658,7 +663,7 @@ __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, c
         if (a.mode == 0) lattice_of(a, g, p, i, j, k);
         const Base b = base_of(a, g, i, j, k, xs[u]);
         const int base = (((b.cx - T0x) * PY + (b.cy - T0y)) * PZ + (b.cz - T0z)) * 8;
-        const float c = w ? w[(size_t)p * wstride + comp] * mS : wS;
+        const float c = w ? wv[u] * mS : wS;
         const float wx[2] = {1.0f - fabsf(b.fx), 1.0f - fabsf(b.fx - 1.0f)};
         const float wy[2] = {1.0f - fabsf(b.fy), 1.0f - fabsf(b.fy - 1.0f)};
         // c is folded into the z weights: exact when c is a power of two (scalar weight 1)
