@@ -640,14 +640,17 @@ __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, c
     }
     __syncthreads();
     constexpr int U = 4;
+    unsigned pn[U];  // the particle indices of the thread's next group, loaded one group ahead
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned e = threadIdx.x + u * blockDim.x;
+      pn[u] = e < n ? seg[beg + e] : 0u;
+    }
     for (unsigned e0 = threadIdx.x; e0 < n; e0 += U * blockDim.x) {
       unsigned ps[U];
       float3 xs[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const unsigned e = e0 + u * blockDim.x;
-        ps[u] = e < n ? seg[beg + e] : 0u;
-      }
+      for (int u = 0; u < U; ++u) ps[u] = pn[u];
 #pragma unroll
       for (int u = 0; u < U; ++u) xs[u] = (e0 + u * blockDim.x < n) ? ld3(a.x, ps[u]) : make_float3(0.f, 0.f, 0.f);
       // the particle weights in flight with the positions (loaded inside the corner loop they cost one
@@ -655,6 +658,12 @@ __global__ void __launch_bounds__(512) k_paint_deposit(PartArgs a, SortGeom g, c
       float wv[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) wv[u] = (w && e0 + u * blockDim.x < n) ? w[(size_t)ps[u] * wstride + comp] : 0.f;
+      // the next group's particle indices in flight with this group's positions (99.9 -> 97.0 us at 256^3)
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const unsigned e = e0 + (U + u) * blockDim.x;
+        pn[u] = e < n ? seg[beg + e] : 0u;
+      }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (e0 + u * blockDim.x >= n) continue;
