@@ -338,6 +338,23 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
   const int kz = blockIdx.y, ky0 = (int)xcd_x() * L, tid = threadIdx.x;
   const int nl = min(L, NY - ky0);
   const float2* src = in + ((size_t)kz * NY + ky0) * N;
+  // XP_POT: the thread's R = L N / NT spectrum coefficients, from the tables while the lines load
+  constexpr int R = (L * N) / NT;
+  float pc[MODE == XP_POT ? R : 1];
+  if constexpr (MODE == XP_POT) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = tid + r * NT, ll = i / N, kx = i - ll * N;
+      const int ky = min(ky0 + ll, NY - 1);
+      const float k2 = tb.w2x[kx] + tb.w2y[ky] + tb.w2z[kz];
+      float coef = (k2 > 0.f) ? op.scale / k2 : 0.f;
+      if (op.flags & 1u) coef *= tb.dx[kx] * tb.dy[ky] * tb.dz[kz];
+      if ((op.flags & 2u) && k2 > 0.f) coef *= expf(-op.pgd_kl2 / k2) * expf(-(k2 * k2) / op.pgd_ks4);
+      if ((op.flags & 4u) && k2 > 0.f)
+        coef *= op.mix_a + op.mix_b * (expf(-op.pgd_kl2 / k2) * expf(-(k2 * k2) / op.pgd_ks4));
+      pc[r] = coef;
+    }
+  }
   for (int i = tid; i < N; i += NT) stw[i] = tw[i];
   for (int i = tid; i < nl * N; i += NT) {
     const int l = i / N, k = i - l * N;
@@ -412,17 +429,12 @@ __global__ void __launch_bounds__(256) k_xpass(const float2* __restrict__ in, fl
       __syncthreads();
     }
   } else if constexpr (MODE == XP_POT) {
-    // potential for the real-space FastPM gradient: phi_hat = delta_hat * scale / k^2 (0 at k = 0)
-    for (int i = tid; i < L * N; i += NT) {
-      const int ll = i / N, kx = i - ll * N;
-      const int ky = min(ky0 + ll, NY - 1);
-      const float k2 = tb.w2x[kx] + tb.w2y[ky] + tb.w2z[kz];
-      float coef = (k2 > 0.f) ? op.scale / k2 : 0.f;
-      if (op.flags & 1u) coef *= tb.dx[kx] * tb.dy[ky] * tb.dz[kz];
-      if ((op.flags & 2u) && k2 > 0.f) coef *= expf(-op.pgd_kl2 / k2) * expf(-(k2 * k2) / op.pgd_ks4);
-      if ((op.flags & 4u) && k2 > 0.f)
-        coef *= op.mix_a + op.mix_b * (expf(-op.pgd_kl2 / k2) * expf(-(k2 * k2) / op.pgd_ks4));
-      res[ll * LS + px<N>(kx)] = cscale(res[ll * LS + px<N>(kx)], coef);
+    // potential for the real-space FastPM gradient: phi_hat = delta_hat * scale / k^2 (0 at k = 0),
+    // the coefficients computed at the start
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = tid + r * NT, ll = i / N, kx = i - ll * N;
+      res[ll * LS + px<N>(kx)] = cscale(res[ll * LS + px<N>(kx)], pc[r]);
     }
     __syncthreads();
     fft_inplace<N, true>(res + l * LS, t, stw);
