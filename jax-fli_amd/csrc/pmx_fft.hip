@@ -608,7 +608,8 @@ __global__ void __launch_bounds__(384) k_c2r_z(const float2* __restrict__ inD, f
 constexpr int GRAD4_LDS_FLOATS = 5120;  // (yb + 4) * NZ <= 5120 for every supported NZ (yb = 256 / tpr)
 __global__ void __launch_bounds__(256) k_grad4(const float* __restrict__ phi, float* __restrict__ F, int NX, int NY,
                                                 int NZ, int tpr, int yb, int xb, int nyb, Slots sl) {
-  __shared__ float4 plane4[GRAD4_LDS_FLOATS / 4];
+  // (yb + 4) * NZ floats, sized at launch (8 KB at NZ = 256: eight workgroups per CU instead of five)
+  extern __shared__ __align__(16) float4 plane4[];
   __shared__ float4 stage[3 * 256];
   float* plane = reinterpret_cast<float*>(plane4);
   const unsigned zs = blockIdx.z;
@@ -835,7 +836,8 @@ int pmx_grad4(pmx_plan_t* p, const float* phi, float* F, hipStream_t s) {
   const int nyb = (p->ny + yb - 1) / yb;
   const int xb = p->nx < 4 ? p->nx : 4;      // x planes each workgroup walks
   const int nxb = (p->nx + xb - 1) / xb;
-  hipLaunchKernelGGL(k_grad4, dim3((unsigned)(nxb * nyb), 1, p->nslots), dim3(tpr * yb), 0, s, phi, F, p->nx, p->ny,
+  const size_t shm = (size_t)(yb + 4) * p->nz * sizeof(float);
+  hipLaunchKernelGGL(k_grad4, dim3((unsigned)(nxb * nyb), 1, p->nslots), dim3(tpr * yb), shm, s, phi, F, p->nx, p->ny,
                      p->nz, tpr, yb, xb, nyb, p->slots);
   return pmx_check_launch("grad4");
 }
