@@ -608,7 +608,8 @@ __global__ void __launch_bounds__(384) k_c2r_z(const float2* __restrict__ inD, f
 constexpr int GRAD4_LDS_FLOATS = 5120;  // (yb + 4) * NZ <= 5120 for every supported NZ (yb = 256 / tpr)
 __global__ void __launch_bounds__(256) k_grad4(const float* __restrict__ phi, float* __restrict__ F, int NX, int NY,
                                                 int NZ, int tpr, int yb, int xb, int nyb, Slots sl) {
-  // (yb + 4) * NZ floats, sized at launch (8 KB at NZ = 256: eight workgroups per CU instead of five)
+  // (yb + 4) * NZ floats, sized at launch (8 KB at NZ = 256: seven workgroups per CU, as the 68 VGPRs
+  // allow, instead of the five a fixed 20 KB array left room for)
   extern __shared__ __align__(16) float4 plane4[];
   __shared__ float4 stage[3 * 256];
   float* plane = reinterpret_cast<float*>(plane4);
